@@ -1,0 +1,215 @@
+/*
+ * maze_poisson.h -- C-ABI of libmaze_poisson.so, the MI355X (gfx950) HIP
+ * implementation of the MaZe electrostatics + MD hot path.
+ *
+ * Part 1 is the reference's ABI: same names, argument meaning and error
+ * behaviour as the reference library that maze_poisson/c_api binds through
+ * ctypes (maze_poisson/c_api/solver.py, get_functions.py, mpi_base.py).
+ * Each declaration cites the reference definition it replaces.  State is one
+ * process-global replica resident in GPU memory; array arguments are
+ * caller-owned host buffers (C-contiguous, float64 unless stated), copied in
+ * at initialisation and copied out by the getters.
+ *
+ * Part 2 lists this library's extensions (no reference counterpart, or a
+ * counterpart that exists only as commented-out code).
+ *
+ * Units are atomic units throughout (Bohr, Hartree, electron masses, a.u.
+ * time).  Grid arrays are n*n*n, C order [i][j][k], k fastest.
+ */
+#ifndef MAZE_POISSON_H
+#define MAZE_POISSON_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================= Part 1: the reference ABI ========================= */
+
+/* library/src/mpi_base.c:169-177 (serial build): returns 0. */
+int init_mpi(void);
+/* library/src/mpi_base.c (get_rank): returns 0 (one GPU, one rank). */
+int get_rank(void);
+
+/* Enum discovery, library/src/grid_general.c:8-22: grid LCG 0, FFT 1,
+ * MULTIGRID 2, MAZE-LCG 3, MAZE-MULTIGRID 4; precond NONE 0, JACOBI 1, MG 2,
+ * SSOR 3, BLOCKJACOBI 4. */
+int get_grid_type_num(void);
+char *get_grid_type_str(int n);
+int get_precond_type_num(void);
+char *get_precond_type_str(int n);
+/* library/src/particles.c:17-36: potential TF 0, LJ 1, SC 2; charge
+ * assignment CIC 0, SPL_QUADR 1, SPL_CUBIC 2. */
+int get_potential_type_num(void);
+char *get_potential_type_str(int n);
+int get_ca_scheme_type_num(void);
+char *get_ca_scheme_type_str(int n);
+/* library/src/integrator_general.c:7-15: OVRVO 0, VERLET 1. */
+int get_integrator_type_num(void);
+char *get_integrator_type_str(int n);
+
+/* library/src/solver_md.c:19-38: prints a banner, selects the GPU. */
+void solver_initialize(void);
+
+/* library/src/solver_md.c:40-44 -> grid_general.c:24-83.  Exits(1) with a
+ * message for FFT (not provided), multigrid on an odd n or n < 32, and
+ * MAZE-LCG preconditioners other than NONE/JACOBI. */
+void solver_initialize_grid(int n, double L, double h, double tol, double eps_s, double eps_int,
+                            int grid_type, int precond_type);
+
+/* library/src/solver_md.c:46-49 -> grid_general.c:85-100.  Enables the
+ * linear Poisson-Boltzmann operator; dielectric maps start at 1 and the
+ * screening map at 0 until solver_set_eps_k2 installs them. */
+void solver_initialize_grid_pois_boltz(double w, double kbar2, int nonpolar_enabled);
+
+/* library/src/solver_md.c:51-65 -> particles.c:68-109, :181-228.
+ * types int32[n_p]; pos, vel double[n_p][3]; mass, charges double[n_p];
+ * pot_params double[n_typ][n_typ][5] = (A, B, C, D, sigma).  Only TF and CIC
+ * are provided (others exit(1)); 1 <= n_typ <= 16. */
+void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, int pot_type,
+                                 int cas_type, int *types, double *pos, double *vel, double *mass,
+                                 double *charges, double *pot_params);
+
+/* library/src/solver_md.c:67-69 (accepted; radius-based PB forces are not
+ * provided). */
+void solver_initialize_particles_pois_boltz(double gamma_np, double beta_np, double *radii);
+
+/* library/src/solver_md.c:71-89, integrator_ovrvo.c:140-148. */
+void solver_initialize_integrator(int n_p, double dt, double T, double gamma, int itg_type,
+                                  int itg_enabled);
+
+/* library/src/solver_md.c:91-107: CIC spreading; returns 0, or prints the
+ * charge-conservation error and exit(1)s (see solver_set_charge_check). */
+int solver_update_charges(void);
+
+/* library/src/solver_md.c:109-111 -> grid_maze_lcg.c:100-124: cold solve. */
+void solver_init_field(void);
+
+/* library/src/solver_md.c:113-129. phi double[n][n][n]. */
+void solver_set_field(double *phi);
+void solver_set_field_prev(double *phi);
+
+/* library/src/solver_md.c:131-133 -> grid_*.c update_field: the CG
+ * iteration index at convergence (MAZE-LCG, LCG), the V-cycle count
+ * (MULTIGRID, MAZE-MULTIGRID), or -1 if not converged.  In the mixed
+ * precision mode (solver_set_precision(1)) it is the total number of FP32
+ * inner iterations. */
+int solver_update_field(void);
+
+/* library/src/solver_md.c:135-138 (no-op: the maps are installed with
+ * solver_set_eps_k2). */
+void solver_update_eps_k2(void);
+
+/* library/src/solver_md.c:140-154 -> forces.c:26-100, :112-186,
+ * particles.c:777-795.  solver_compute_forces_noel returns the Tosi-Fumi
+ * potential energy; solver_compute_forces_pb returns 0. */
+void solver_compute_forces_elec(void);
+double solver_compute_forces_noel(void);
+double solver_compute_forces_pb(void);
+void solver_compute_forces_tot(void);
+
+/* library/src/solver_md.c:162-172 -> integrator_verlet.c:12-71,
+ * integrator_ovrvo.c:45-138, particles.c:841-857. */
+void integrator_part_1(void);
+void integrator_part_2(void);
+void solver_rescale_velocities(void);
+
+/* library/src/solver_md.c:210-222. */
+int solver_check_thermostat(void);
+
+/* library/src/solver_md.c:230-245: frees the replica. */
+void solver_finalize(void);
+
+/* Getters, library/src/solver_md.c:247-354: copy device state into the
+ * caller's arrays.  (n_p,3): pos, vel, fcs_*; (n_p): types (int32), charges,
+ * masses, radii; (n,n,n): field, field_prev, q; (3): momentum.  fcs_db,
+ * fcs_ib, fcs_np and radii are zero (radius-based PB forces not provided). */
+void get_pos(double *recv);
+void get_vel(double *recv);
+void get_fcs_elec(double *recv);
+void get_fcs_noel(double *recv);
+void get_fcs_db(double *recv);
+void get_fcs_ib(double *recv);
+void get_fcs_np(double *recv);
+void get_fcs_tot(double *recv);
+void get_types(int *recv);
+void get_charges(double *recv);
+void get_masses(double *recv);
+void get_radii(double *recv);
+void get_field(double *recv);
+void get_field_prev(double *recv);
+void get_q(double *recv);
+double get_kinetic_energy(void);
+/* library/src/grid_general.c:304-316: half the sum of phi (no charge). */
+double get_energy_elec(void);
+void get_momentum(double *recv);
+double get_temperature(void);
+
+/* ========================= Part 2: extensions ============================= */
+
+/* One MD step (integrator part 1, charges, field, forces, part 2) without
+ * host round trips between the phases; the reference has it only as
+ * commented-out code, library/src/solver_md.c:202-208 and :224-228. */
+void solver_md_loop_iter(void);
+void solver_run_n_steps(int n_steps);
+
+/* Blocks until all work of the replica has finished. */
+void solver_sync(void);
+
+/* Seed of the counter-based (Philox4x32-10) Langevin noise that replaces
+ * the reference's rand() stream (integrator_ovrvo.c:12-36); resets the
+ * stream position. */
+void solver_set_rng_seed(uint64_t seed);
+
+/* Installs face-centred dielectric maps and the screening map of the linear
+ * PB operator (each double[n][n][n]); needs
+ * solver_initialize_grid_pois_boltz first. */
+void solver_set_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
+
+/* Field-solve arithmetic: 0 = FP64 CG as in the reference (default);
+ * 1 = FP32-storage CG inside FP64 iterative refinement, stopping on the
+ * reference's criterion max|A y - sigma| <= tol evaluated in FP64.  Returns
+ * 0, or -1 for an unknown mode. */
+int solver_set_precision(int mode);
+int solver_get_precision(void);
+/* Relative residual reduction of each FP32 inner solve (default 1e-4). */
+void solver_set_inner_rtol(double rtol);
+
+/* 1 (default): a charge-conservation failure exits like the reference;
+ * 0: solver_update_charges returns 1 instead. */
+void solver_set_charge_check(int strict);
+
+/* The MaZe constraint field y, double[n][n][n]. */
+void get_y(double *recv);
+/* CIC neighbour table as the reference stores it (particles.c:297-346):
+ * int64[n_p][8][3]. */
+void get_neighbors(int64_t *recv);
+/* Number of partners within the Tosi-Fumi cutoff per ion, int32[n_p]. */
+void get_tf_neighbor_counts(int *recv);
+
+/* Per-kernel timing with HIP events on the replica's stream (only solver
+ * iterations that did work are counted).  Names: "cg_dir", "cg_update",
+ * "cg_dir_f32", "cg_update_f32". */
+void maze_prof_enable(int on);
+void maze_prof_reset(void);
+int maze_prof_get(const char *name, double *total_ms, long *count);
+int maze_grid_blocks(void);
+
+/* Kernel-level entry points on host buffers, run on a private replica (for
+ * parity tests against the CPU oracle). */
+void maze_test_laplace(const double *u, double *out, int n);
+void maze_test_laplace_pb(const double *u, double *out, int n, const double *eps_x,
+                          const double *eps_y, const double *eps_z, const double *k2);
+int maze_test_conj_grad(const double *b, double *x, double tol, int n);
+int maze_test_conj_grad_pb(const double *b, double *x, double tol, int n, const double *eps_x,
+                           const double *eps_y, const double *eps_z, const double *k2);
+void maze_test_rbgs(const double *in, double *out, int n, int iters);
+int maze_test_restrict27(const double *in, double *out, int n);
+int maze_test_prolong_add(const double *in, double *out, int nc);
+int maze_test_mg_vcycle(const double *in, double *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAZE_POISSON_H */

@@ -1,0 +1,329 @@
+// abi.hip -- the C-ABI of libmaze_poisson.so.
+//
+// The reference symbols keep their names, signatures and semantics
+// (library/src/solver_md.c:19-354 and the enum tables of grid_general.c:8-22,
+// particles.c:17-36, integrator_general.c:7-15), over one process-global
+// GPU-resident replica, so maze_poisson/c_api binds this library unchanged.
+// Inputs are copied to HBM at initialisation; getters copy out; nothing else
+// crosses the PCIe bus during a step.  Extensions are declared in
+// include/maze_poisson.h.
+#include <cstring>
+#include <vector>
+
+#include "sim.hpp"
+
+using maze::Sim;
+
+static Sim* g_sim = nullptr;
+
+static Sim& S() {
+    if (!g_sim) g_sim = new Sim();
+    return *g_sim;
+}
+
+static void d2h(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    S().sync();
+    MAZE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+
+static void h2d(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    S().sync();
+    MAZE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+
+static char g_grid_str[5][16] = {"LCG", "FFT", "MULTIGRID", "MAZE-LCG", "MAZE-MULTIGRID"};
+static char g_precond_str[5][16] = {"NONE", "JACOBI", "MG", "SSOR", "BLOCKJACOBI"};
+static char g_pot_str[3][16] = {"TF", "LJ", "SC"};
+static char g_cas_str[3][16] = {"CIC", "SPL_QUADR", "SPL_CUBIC"};
+static char g_itg_str[2][16] = {"OVRVO", "VERLET"};
+
+extern "C" {
+
+// ---- MPI hooks (serial semantics, library/src/mpi_base.c:169-177) --------
+int init_mpi() { return 0; }
+int get_rank() { return 0; }
+
+// ---- enum discovery ----------------------------------------------------------
+int get_grid_type_num() { return 5; }
+char* get_grid_type_str(int i) { return g_grid_str[i]; }
+int get_precond_type_num() { return 5; }
+char* get_precond_type_str(int i) { return g_precond_str[i]; }
+int get_potential_type_num() { return 3; }
+char* get_potential_type_str(int i) { return g_pot_str[i]; }
+int get_ca_scheme_type_num() { return 3; }
+char* get_ca_scheme_type_str(int i) { return g_cas_str[i]; }
+int get_integrator_type_num() { return 2; }
+char* get_integrator_type_str(int i) { return g_itg_str[i]; }
+
+// ---- initialisation ------------------------------------------------------------
+void solver_initialize() {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    MAZE_HIP(hipGetDevice(&dev));
+    MAZE_HIP(hipGetDeviceProperties(&prop, dev));
+    std::printf("******************************************************\n");
+    std::printf("* MAZE_POISSON (HIP, %s)\n", prop.gcnArchName);
+    std::printf("******************************************************\n");
+    std::fflush(stdout);
+    S();
+}
+
+void solver_initialize_grid(int n, double L, double h, double tol, double eps, double eps_int,
+                            int grid_type, int precond_type) {
+    S().init_grid(n, L, h, tol, eps, eps_int, grid_type, precond_type);
+}
+
+void solver_initialize_grid_pois_boltz(double w, double kbar2, int nonpolar_enabled) {
+    (void)nonpolar_enabled;
+    S().init_pb(w, kbar2);
+}
+
+void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, int pot_type,
+                                 int cas_type, int* types, double* pos, double* vel, double* mass,
+                                 double* charges, double* pot_params) {
+    if (pot_type != 0) {
+        std::fprintf(stderr, "maze: only the TF pair potential is provided (got %d)\n", pot_type);
+        std::exit(1);
+    }
+    if (cas_type != 0) {
+        std::fprintf(stderr, "maze: only CIC charge assignment is provided (got %d)\n", cas_type);
+        std::exit(1);
+    }
+    S().init_particles(n, n_typ, L, h, n_p, types, pos, vel, mass, charges, pot_params);
+}
+
+void solver_initialize_particles_pois_boltz(double gamma_np, double beta_np, double* radii) {
+    (void)gamma_np; (void)beta_np; (void)radii;
+}
+
+void solver_initialize_integrator(int n_p, double dt, double T, double gamma, int itg_type,
+                                  int itg_enabled) {
+    (void)n_p;
+    S().init_integrator(dt, T, gamma, itg_type, itg_enabled);
+}
+
+// ---- step ------------------------------------------------------------------------
+int solver_update_charges() { return S().update_charges(); }
+void solver_init_field() { S().init_field(); }
+int solver_update_field() { return S().update_field(); }
+void solver_update_eps_k2() {}
+void solver_compute_forces_elec() { S().forces_elec(); }
+double solver_compute_forces_noel() { return S().forces_noel(true); }
+double solver_compute_forces_pb() { return 0.0; }
+void solver_compute_forces_tot() { S().forces_tot(); }
+void integrator_part_1() { S().part1(); }
+void integrator_part_2() { S().part2(); }
+void solver_rescale_velocities() { S().rescale_velocities(); }
+int solver_check_thermostat() { return S().check_thermostat(); }
+
+void solver_set_field(double* phi) { h2d(S().phi, phi, S().G * sizeof(double)); }
+void solver_set_field_prev(double* phi) { h2d(S().phip, phi, S().G * sizeof(double)); }
+
+void solver_finalize() {
+    if (g_sim) {
+        delete g_sim;
+        g_sim = nullptr;
+    }
+}
+
+// ---- getters -------------------------------------------------------------------
+void get_pos(double* o) { d2h(o, S().p.pos, 3L * S().p.np * sizeof(double)); }
+void get_vel(double* o) { d2h(o, S().p.vel, 3L * S().p.np * sizeof(double)); }
+void get_fcs_elec(double* o) { d2h(o, S().p.fe, 3L * S().p.np * sizeof(double)); }
+void get_fcs_noel(double* o) { d2h(o, S().p.fn, 3L * S().p.np * sizeof(double)); }
+void get_fcs_tot(double* o) { d2h(o, S().p.ft, 3L * S().p.np * sizeof(double)); }
+void get_fcs_db(double* o) { std::memset(o, 0, 3L * S().p.np * sizeof(double)); }
+void get_fcs_ib(double* o) { std::memset(o, 0, 3L * S().p.np * sizeof(double)); }
+void get_fcs_np(double* o) { std::memset(o, 0, 3L * S().p.np * sizeof(double)); }
+void get_types(int* o) { d2h(o, S().p.types, S().p.np * sizeof(int)); }
+void get_charges(double* o) { d2h(o, S().p.charges, S().p.np * sizeof(double)); }
+void get_masses(double* o) { d2h(o, S().p.mass, S().p.np * sizeof(double)); }
+void get_radii(double* o) { std::memset(o, 0, S().p.np * sizeof(double)); }
+void get_field(double* o) { d2h(o, S().phi, S().G * sizeof(double)); }
+void get_field_prev(double* o) { d2h(o, S().phip, S().G * sizeof(double)); }
+void get_q(double* o) { d2h(o, S().q, S().G * sizeof(double)); }
+double get_kinetic_energy() { return S().kinetic(); }
+double get_energy_elec() { return S().energy_elec(); }
+void get_momentum(double* o) { S().momentum(o); }
+double get_temperature() { return 2 * S().kinetic() / (3 * S().p.np * 3.1668e-6); }
+
+// ---- extensions -------------------------------------------------------------------
+void solver_md_loop_iter() { S().md_step(); }
+
+// Block until every launched kernel of the replica has finished.
+void solver_sync() { S().sync(); }
+
+void solver_run_n_steps(int n_steps) {
+    for (int i = 0; i < n_steps; i++) S().md_step();
+}
+
+void solver_set_rng_seed(uint64_t seed) {
+    S().seed = seed;
+    S().calls = 0;
+}
+
+void solver_set_eps_k2(double* ex, double* ey, double* ez, double* k2) {
+    S().set_eps_k2(ex, ey, ez, k2);
+}
+
+int solver_set_precision(int mode) {
+    if (mode < 0 || mode > 1) return -1;
+    S().precision = mode;
+    return 0;
+}
+
+int solver_get_precision() { return S().precision; }
+
+void solver_set_inner_rtol(double rtol) { S().inner_rtol = rtol; }
+
+void solver_set_charge_check(int strict) { S().strict_charge_check = strict; }
+
+void get_y(double* o) { d2h(o, S().y, S().G * sizeof(double)); }
+
+void get_neighbors(int64_t* o) {
+    Sim& s = S();
+    const int np = s.p.np;
+    std::vector<int> base(3L * np);
+    d2h(base.data(), s.p.base, 3L * np * sizeof(int));
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                  {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    for (int i = 0; i < np; i++)
+        for (int c = 0; c < 8; c++)
+            for (int d = 0; d < 3; d++)
+                o[(i * 8L + c) * 3 + d] = (base[3 * i + d] + off[c][d]) % s.p.n;
+}
+
+void get_tf_neighbor_counts(int* o) { d2h(o, S().p.counts, S().p.np * sizeof(int)); }
+
+// Per-kernel timing with HIP events on the solver stream (only iterations
+// that did work are counted).
+void maze_prof_enable(int on) { S().prof = on; }
+
+void maze_prof_reset() {
+    for (auto& t : S().timers) {
+        t.total_ms = 0.0;
+        t.count = 0;
+    }
+}
+
+int maze_prof_get(const char* name, double* total_ms, long* count) {
+    for (auto& t : S().timers)
+        if (std::strcmp(t.name, name) == 0) {
+            *total_ms = t.total_ms;
+            *count = t.count;
+            return 0;
+        }
+    return -1;
+}
+
+int maze_grid_blocks() { return S().dims.blocks; }
+
+// ---- kernel-level entry points on host buffers (for parity tests) ------------------
+// Each runs on a private replica and copies in and out; none touches the
+// process-global state.
+static Sim* scratch_sim(int n) {
+    if (n < 2) {
+        std::fprintf(stderr, "maze: test grids need n >= 2\n");
+        std::exit(1);
+    }
+    Sim* s = new Sim();
+    s->init_grid(n, 1.0, 1.0 / n, 0.0, 1.0, 1.0, n % 2 == 0 && n >= 32 ? 2 : 3, 0);
+    return s;
+}
+
+void maze_test_laplace(const double* u, double* out, int n) {
+    Sim* s = scratch_sim(n);
+    MAZE_HIP(hipMemcpy(s->tmp, u, s->G * sizeof(double), hipMemcpyHostToDevice));
+    maze::launch_laplace(s->stream, s->dims, maze::PBCoef{}, s->tmp, s->r);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->r, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+}
+
+void maze_test_laplace_pb(const double* u, double* out, int n, const double* ex, const double* ey,
+                          const double* ez, const double* k2) {
+    Sim* s = scratch_sim(n);
+    s->init_pb(0.0, 0.0);
+    s->set_eps_k2(ex, ey, ez, k2);
+    MAZE_HIP(hipMemcpy(s->tmp, u, s->G * sizeof(double), hipMemcpyHostToDevice));
+    maze::launch_laplace(s->stream, s->dims, s->pbc(), s->tmp, s->r);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->r, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+}
+
+int maze_test_conj_grad(const double* b, double* x, double tol, int n) {
+    Sim* s = scratch_sim(n);
+    MAZE_HIP(hipMemcpy(s->sigma, b, s->G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->y, x, s->G * sizeof(double), hipMemcpyHostToDevice));
+    int pred = 32;
+    const int res = s->cg_solve(s->dims, maze::PBCoef{}, s->sigma, s->y, tol, n * n, s->r, s->p0,
+                                s->p1, &pred);
+    MAZE_HIP(hipMemcpy(x, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+    return res;
+}
+
+int maze_test_conj_grad_pb(const double* b, double* x, double tol, int n, const double* ex,
+                           const double* ey, const double* ez, const double* k2) {
+    Sim* s = scratch_sim(n);
+    s->init_pb(0.0, 0.0);
+    s->set_eps_k2(ex, ey, ez, k2);
+    MAZE_HIP(hipMemcpy(s->sigma, b, s->G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->y, x, s->G * sizeof(double), hipMemcpyHostToDevice));
+    int pred = 32;
+    const int res = s->cg_solve(s->dims, s->pbc(), s->sigma, s->y, tol, n * n, s->r, s->p0, s->p1,
+                                &pred);
+    MAZE_HIP(hipMemcpy(x, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+    return res;
+}
+
+void maze_test_rbgs(const double* in, double* out, int n, int iters) {
+    Sim* s = scratch_sim(n);
+    MAZE_HIP(hipMemcpy(s->sigma, in, s->G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->y, out, s->G * sizeof(double), hipMemcpyHostToDevice));
+    maze::launch_rbgs(s->stream, s->dims, s->sigma, s->y, iters);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+}
+
+int maze_test_restrict27(const double* in, double* out, int n) {
+    if (n < 2 || n % 2 != 0) return -1;
+    Sim* s = scratch_sim(n);
+    MAZE_HIP(hipMemcpy(s->tmp, in, s->G * sizeof(double), hipMemcpyHostToDevice));
+    maze::launch_restrict27(s->stream, n, s->tmp, s->r);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->r, s->G / 8 * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+    return 0;
+}
+
+int maze_test_prolong_add(const double* in, double* out, int nc) {
+    if (nc < 1) return -1;
+    Sim* s = scratch_sim(2 * nc);
+    MAZE_HIP(hipMemcpy(s->tmp, in, s->G / 8 * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->r, out, s->G * sizeof(double), hipMemcpyHostToDevice));
+    maze::launch_prolong_add(s->stream, nc, s->tmp, s->r);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->r, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+    return 0;
+}
+
+int maze_test_mg_vcycle(const double* in, double* out, int n) {
+    if (n < 32 || n % 2 != 0) return -1;
+    Sim* s = scratch_sim(n);
+    MAZE_HIP(hipMemcpy(s->sigma, in, s->G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->y, out, s->G * sizeof(double), hipMemcpyHostToDevice));
+    s->mg_vcycle(s->sigma, s->y);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// common.hpp -- shared device/host helpers for the MaZe HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+namespace maze {
+
+#define MAZE_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            std::fprintf(stderr, "maze: HIP error %s at %s:%d: %s\n", hipGetErrorName(e_), \
+                         __FILE__, __LINE__, #call);                                     \
+            std::abort();                                                                \
+        }                                                                                \
+    } while (0)
+
+constexpr int kBlock = 256;       // threads per work-group (4 wave64)
+constexpr int kWave = 64;
+constexpr double kPi = 3.14159265358979323846;
+
+// Geometry of a periodic n^3 grid stored C-order [i][j][k], k fastest.
+struct GridDims {
+    int n;        // points per side
+    int n2;       // n*n (points per i-plane)
+    int chunks;   // ceil(n2 / kBlock): plane chunks, one per work-group column
+    int slab;     // i-planes marched by one work-group
+    int slabs;    // ceil(n / slab)
+    int blocks;   // chunks * slabs
+};
+
+inline GridDims make_dims(int n, int slab) {
+    GridDims d;
+    d.n = n;
+    d.n2 = n * n;
+    d.chunks = (d.n2 + kBlock - 1) / kBlock;
+    d.slab = slab < n ? slab : n;
+    d.slabs = (n + d.slab - 1) / d.slab;
+    d.blocks = d.chunks * d.slabs;
+    return d;
+}
+
+// Work-group -> (chunk, slab) map that keeps neighbouring slabs on one XCD.
+// Consecutive work-group ids are dispatched round-robin over the 8 XCDs, each
+// with its own L2; regrouping ids so that one XCD walks a contiguous range of
+// (slab, chunk) tiles keeps the i+-1 halo planes in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+    const int per = nblocks >> 3;
+    if ((nblocks & 7) != 0) return b;
+    return (b & 7) * per + (b >> 3);
+}
+
+// Neighbour offsets of a point inside its i-plane (periodic in j and k).
+struct PlaneNb {
+    int jm, jp, km, kp;
+};
+
+__device__ __forceinline__ PlaneNb plane_nb(int pl, int n, int n2) {
+    const int j = pl / n;
+    const int k = pl - j * n;
+    PlaneNb o;
+    o.jm = (j == 0) ? pl + n2 - n : pl - n;
+    o.jp = (j == n - 1) ? pl - n2 + n : pl + n;
+    o.km = (k == 0) ? pl + n - 1 : pl - 1;
+    o.kp = (k == n - 1) ? pl - n + 1 : pl + 1;
+    return o;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Fixed-shape work-group reductions (deterministic: butterfly within each
+// wave, then the 4 wave results in wave order).  Result valid in all threads.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < kBlock / kWave; i++) s += sh[i];
+    return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    v = wave_max(v);
+    const int w = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < kBlock / kWave; i++) s = fmax(s, sh[i]);
+    return s;
+}
+
+// Last-arriving work-group detection for the single-pass reductions: every
+// work-group publishes its partial, fences at device scope and bumps the
+// counter; the one that sees nblocks-1 reduces all partials in index order,
+// so the result does not depend on arrival order.
+__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int nblocks) {
+    __shared__ bool last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int prev = atomicAdd(counter, 1u);
+        last = (prev == nblocks - 1);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// Per-work-group partials are exchanged through device-scope atomic loads and
+// stores so that they are coherent across the XCDs' separate L2 caches.
+__device__ __forceinline__ void store_partial(double* p, int i, double v) {
+    __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_partial(const double* p, int i) {
+    return __hip_atomic_load(const_cast<double*>(p) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum / max of partials[0..cnt) in a fixed order; valid in all threads.
+__device__ __forceinline__ double reduce_partials_sum(const double* partials, int cnt, double* sh) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += kBlock) s += load_partial(partials, i);
+    return block_sum(s, sh);
+}
+
+__device__ __forceinline__ double reduce_partials_max(const double* partials, int cnt, double* sh) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += kBlock) s = fmax(s, load_partial(partials, i));
+    return block_max(s, sh);
+}
+
+}  // namespace maze

@@ -1,0 +1,82 @@
+// grid.hpp -- device state and launchers for the grid solvers.
+#pragma once
+
+#include "common.hpp"
+
+namespace maze {
+
+// Scalars of one Krylov solve, resident in device memory so that the
+// per-iteration kernels never wait for the host.  Written only by the last
+// work-group of each reduction pass.
+struct CGState {
+    double rdv;      // <r, v> of the current direction (v = -r/6)
+    double pAp;      // <p, A p>
+    double alpha, beta;
+    double rr, rmax; // <r, r> and max|r| after the last update
+    double tol;      // absolute stop on max|r|
+    double rtol;     // relative stop on max|r| / max|r0| (0: absolute mode)
+    double r0;       // max|r| of the initial residual
+    double resid;    // spare scalar (true-residual passes)
+    int iter;        // 0-based index of the iteration in flight
+    int limit;       // iteration cap
+    int done;        // 1 once stopped (converged, capped or broken down)
+    int res;         // result code (see cg_result)
+    int first;       // 1 until the first direction has been formed
+    int count_mode;  // 1: result is the number of iterations (coarse CG)
+    unsigned int counter;
+    int pad;
+};
+
+// Face-centred dielectric maps and screening term of the PB operator; the
+// Poisson operator passes nullptrs.
+struct PBCoef {
+    const double* ex;
+    const double* ey;
+    const double* ez;
+    const double* k2;
+};
+
+struct Reduce {
+    double* psum;
+    double* pmax;
+    unsigned int* counter;
+};
+
+// ---- launchers (grid_kernels.hip) ----------------------------------------
+void launch_laplace(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* u, double* out);
+void launch_pred_sigma(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* phi,
+                       const double* phip, const double* q, double coef, double* phistar,
+                       double* sigma);
+template <class Real>
+void launch_cg_init(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* x,
+                    const Real* b, Real* r, CGState* st, Reduce red);
+template <class Real>
+void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p_old,
+                   const Real* r, Real* p_new, CGState* st, Reduce red);
+template <class Real>
+void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
+                      Real* r, CGState* st, Reduce red);
+void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit, int count_mode);
+
+// Mixed precision: FP64 true residual b32 = sigma - A (y + d), y <- y + d.
+void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
+                   const float* dcorr, const double* sigma, double* y_out, float* b32,
+                   CGState* st, Reduce red, int add_d);
+
+void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u);
+void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out);
+void launch_fill(hipStream_t s, long n, double v, double* out);
+void launch_fill_f(hipStream_t s, long n, float v, float* out);
+void launch_copy(hipStream_t s, long n, const double* in, double* out);
+void launch_half_sum(hipStream_t s, long n, const double* phi, double* out, Reduce red);
+
+// Reference two-grid multigrid pieces.
+void launch_rbgs(hipStream_t s, const GridDims& d, const double* in, double* out, int iters);
+void launch_mg_residual(hipStream_t s, const GridDims& d, const double* in, const double* out,
+                        double* r);
+void launch_restrict27(hipStream_t s, int n, const double* in, double* out);
+void launch_prolong_add(hipStream_t s, int nc, const double* in, double* out);
+void launch_true_residual_max(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
+                              const double* sigma, CGState* st, Reduce red);
+
+}  // namespace maze

@@ -1,0 +1,614 @@
+// grid_kernels.hip -- 7-point Poisson / linear Poisson-Boltzmann kernels for gfx950.
+//
+// Every grid pass is one i-marching stencil sweep: a work-group owns 256
+// consecutive points of an i-plane (coalesced along the contiguous plane) and
+// walks `slab` planes, keeping the i-1 / i / i+1 values of its column in
+// registers.  The value being differenced is produced on the fly by a source
+// functor (e.g. 2 phi - phi_prev, or beta p + r/6), so fused passes never
+// materialise intermediates.  Reductions are single-pass: per-work-group
+// partials plus a last-arriving work-group that folds them in index order
+// (deterministic) and advances the solver scalars in device memory.
+//
+// Floating-point expressions follow the reference's operation order
+// (library/src/laplace.c:301-309, :366-379; library/src/plcg.c:41-83;
+// library/src/verlet.c:221-240), so every point-wise result matches the CPU
+// oracle bit for bit; only the order of the global sums differs.
+#include "grid.hpp"
+
+namespace maze {
+
+// ---- source functors -------------------------------------------------------
+template <class Real>
+struct SrcArr {
+    const Real* a;
+    __device__ __forceinline__ double operator()(int i) const { return (double)a[i]; }
+};
+
+struct SrcPred {  // phi* = 2 phi - phi_prev   (library/src/verlet.c:223-225)
+    const double* phi;
+    const double* phip;
+    __device__ __forceinline__ double operator()(int i) const { return 2 * phi[i] - phip[i]; }
+};
+
+template <class Real>
+struct SrcDir {  // p = beta p + r/6, or r/6 for the first direction (plcg.c:52-53, :79-80)
+    const Real* p;
+    const Real* r;
+    double beta;
+    int first;
+    __device__ __forceinline__ double operator()(int i) const {
+        const double c6 = 1.0 / 6.0;
+        double v = first ? (double)r[i] * c6 : (double)p[i] * beta + c6 * (double)r[i];
+        return (double)(Real)v;
+    }
+};
+
+struct SrcSum {  // y + d
+    const double* y;
+    const float* d;
+    int add;
+    __device__ __forceinline__ double operator()(int i) const {
+        return add ? y[i] + (double)d[i] : y[i];
+    }
+};
+
+struct SrcZero {
+    __device__ __forceinline__ double operator()(int) const { return 0.0; }
+};
+
+// ---- the i-marching stencil sweep -------------------------------------------
+// Calls body(c, value_at_c, (A value)_c) for every point of this work-group's
+// tile.  Threads past the end of the plane do nothing but stay for the
+// work-group reductions.
+template <bool PB, class Src, class Body>
+__device__ __forceinline__ void march(const GridDims& d, const PBCoef& pb, const Src& src,
+                                      Body& body) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int chunk = b % d.chunks;
+    const int s = b / d.chunks;
+    const int pl = chunk * kBlock + threadIdx.x;
+    if (pl >= d.n2) return;
+    const int n = d.n, n2 = d.n2;
+    const PlaneNb nb = plane_nb(pl, n, n2);
+    const int i0 = s * d.slab;
+    const int i1 = min(n, i0 + d.slab);
+    const int im0 = (i0 == 0) ? n - 1 : i0 - 1;
+    double vm = src(im0 * n2 + pl);
+    double vc = src(i0 * n2 + pl);
+    double exm = 0.0;
+    if (PB) exm = pb.ex[im0 * n2 + pl];
+    for (int i = i0; i < i1; i++) {
+        const int ip = (i + 1 == n) ? 0 : i + 1;
+        const int base = i * n2;
+        const int c = base + pl;
+        const double vp = src(ip * n2 + pl);
+        const double vjm = src(base + nb.jm), vjp = src(base + nb.jp);
+        const double vkm = src(base + nb.km), vkp = src(base + nb.kp);
+        double lap;
+        if (!PB) {
+            lap = vp + vm + vjp + vjm + vkp + vkm - vc * 6.0;
+        } else {
+            const double exc = pb.ex[c], eyc = pb.ey[c], ezc = pb.ez[c];
+            const double eym = pb.ey[base + nb.jm], ezm = pb.ez[base + nb.km];
+            lap = vp * exc + vm * exm + vjp * eyc + vjm * eym + vkp * ezc + vkm * ezm -
+                  vc * (exc + exm + eyc + eym + ezc + ezm + pb.k2[c]);
+            exm = exc;
+        }
+        body(c, vc, lap);
+        vm = vc;
+        vc = vp;
+    }
+}
+
+template <bool PB, class Src, class Body>
+__global__ __launch_bounds__(kBlock) void k_pass(GridDims d, PBCoef pb, Src src, Body body,
+                                                 Reduce red, CGState* st) {
+    if (Body::kSkipWhenDone && st->done) return;
+    Body bd = body;
+    march<PB>(d, pb, src, bd);
+    if constexpr (Body::kReduce) {
+        __shared__ double sh[kBlock / kWave];
+        const double s = block_sum(bd.s, sh);
+        const double m = block_max(bd.m, sh);
+        if (threadIdx.x == 0) {
+            store_partial(red.psum, blockIdx.x, s);
+            store_partial(red.pmax, blockIdx.x, m);
+        }
+        if (arrive_last(red.counter, gridDim.x)) {
+            const double S = reduce_partials_sum(red.psum, gridDim.x, sh);
+            const double M = reduce_partials_max(red.pmax, gridDim.x, sh);
+            if (threadIdx.x == 0) {
+                Body::finalize(st, S, M);
+                __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+}
+
+template <bool PB, class Src, class Body>
+static void run_pass(hipStream_t s, const GridDims& d, const PBCoef& pb, const Src& src,
+                     const Body& body, Reduce red, CGState* st) {
+    hipLaunchKernelGGL((k_pass<PB, Src, Body>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, src,
+                       body, red, st);
+}
+
+#define DISPATCH_PB(pb, ...)                \
+    do {                                    \
+        if ((pb).ex) {                      \
+            constexpr bool kPB = true;      \
+            __VA_ARGS__;                    \
+        } else {                            \
+            constexpr bool kPB = false;     \
+            __VA_ARGS__;                    \
+        }                                   \
+    } while (0)
+
+// ---- bodies -----------------------------------------------------------------
+struct BodyLaplace {
+    static constexpr bool kReduce = false, kSkipWhenDone = false;
+    double* out;
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double, double lap) { out[c] = lap; }
+    __device__ static void finalize(CGState*, double, double) {}
+};
+
+// phi* and sigma = A phi* + coef q (library/src/verlet.c:221-230).
+struct BodyPred {
+    static constexpr bool kReduce = false, kSkipWhenDone = false;
+    double* phistar;
+    double* sigma;
+    const double* q;
+    double coef;
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double v, double lap) {
+        phistar[c] = v;
+        sigma[c] = lap + coef * q[c];
+    }
+    __device__ static void finalize(CGState*, double, double) {}
+};
+
+// r = A x - b; <r, r/6>; max|r|   (library/src/plcg.c:41-58)
+template <class Real>
+struct BodyInit {
+    static constexpr bool kReduce = true, kSkipWhenDone = false;
+    const Real* b;
+    Real* r;
+    Real* x_zero;  // when set, x is taken as 0 and written back as 0
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double, double lap) {
+        const Real rc = (Real)(lap + -1.0 * (double)b[c]);
+        r[c] = rc;
+        if (x_zero) x_zero[c] = (Real)0;
+        const double rd = (double)rc;
+        s += rd * (rd * (1.0 / 6.0));
+        m = fmax(m, fabs(rd));
+    }
+    __device__ static void finalize(CGState* st, double S, double M) {
+        st->rdv = -S;
+        st->r0 = M;
+        st->iter = 0;
+        st->first = 1;
+        st->beta = 0.0;
+        st->alpha = 0.0;
+        // An identically zero residual has no direction to search; the
+        // reference would divide 0 by 0 here.  Stop with the input unchanged.
+        if (M == 0.0) {
+            st->done = 1;
+            st->res = 0;
+        } else {
+            st->done = 0;
+            st->res = -1;
+        }
+    }
+};
+
+// p_new = beta p + r/6 (written), <p_new, A p_new>   (library/src/plcg.c:61-63)
+template <class Real>
+struct BodyDir {
+    static constexpr bool kReduce = true, kSkipWhenDone = true;
+    Real* p_new;
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double v, double lap) {
+        p_new[c] = (Real)v;
+        s += v * lap;
+    }
+    __device__ static void finalize(CGState* st, double S, double) {
+        st->pAp = S;
+        st->first = 0;
+        if (fabs(S) < 1e-300) {  // breakdown guard (library/src/multigrid.c:59-63)
+            st->done = 1;
+            st->res = st->count_mode ? st->iter : st->iter;
+            st->alpha = 0.0;
+        } else {
+            st->alpha = st->rdv / S;
+        }
+    }
+};
+
+// x += alpha p; r += alpha A p; <r, r>, max|r|; stop test and beta
+// (library/src/plcg.c:64-80; relative stop of library/src/multigrid.c:74-78).
+template <class Real>
+struct BodyUpdate {
+    static constexpr bool kReduce = true, kSkipWhenDone = true;
+    Real* x;
+    Real* r;
+    double alpha;
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double v, double lap) {
+        x[c] = (Real)((double)x[c] + alpha * v);
+        const Real rn = (Real)((double)r[c] + alpha * lap);
+        r[c] = rn;
+        const double rd = (double)rn;
+        s += rd * rd;
+        m = fmax(m, fabs(rd));
+    }
+    __device__ static void finalize(CGState* st, double S, double M) {
+        st->rr = S;
+        st->rmax = M;
+        if (st->count_mode) {
+            if (M <= st->rtol * st->r0) {
+                st->done = 1;
+                st->res = st->iter + 1;
+                return;
+            }
+        } else if (M <= st->tol) {
+            st->done = 1;
+            st->res = st->iter;
+            return;
+        }
+        const double rn_dot_vn = -S / 6.0;
+        st->beta = rn_dot_vn / st->rdv;
+        st->rdv = rn_dot_vn;
+        st->iter += 1;
+        if (st->iter >= st->limit) {
+            st->done = 1;
+            st->res = st->count_mode ? st->limit : -1;
+        }
+    }
+};
+
+// y_out = y + d; b32 = sigma - A y_out (FP64); max|b32|
+struct BodyRefine {
+    static constexpr bool kReduce = true, kSkipWhenDone = false;
+    double* y_out;
+    const double* sigma;
+    float* b32;
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double v, double lap) {
+        if (y_out) y_out[c] = v;
+        const double rr = sigma[c] - lap;
+        b32[c] = (float)rr;
+        m = fmax(m, fabs(rr));
+    }
+    __device__ static void finalize(CGState* st, double, double M) { st->resid = M; }
+};
+
+// max|A y - sigma|  (library/src/verlet.c:300-302)
+struct BodyTrueRes {
+    static constexpr bool kReduce = true, kSkipWhenDone = false;
+    const double* sigma;
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double, double lap) {
+        const double t = lap + -1.0 * sigma[c];
+        m = fmax(m, fabs(t));
+    }
+    __device__ static void finalize(CGState* st, double, double M) { st->resid = M; }
+};
+
+// ---- launchers --------------------------------------------------------------
+void launch_laplace(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* u,
+                    double* out) {
+    BodyLaplace body{out};
+    DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcArr<double>{u}, body, Reduce{}, nullptr));
+}
+
+void launch_pred_sigma(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* phi,
+                       const double* phip, const double* q, double coef, double* phistar,
+                       double* sigma) {
+    BodyPred body{phistar, sigma, q, coef};
+    DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcPred{phi, phip}, body, Reduce{}, nullptr));
+}
+
+template <class Real>
+void launch_cg_init(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* x,
+                    const Real* b, Real* r, CGState* st, Reduce red) {
+    if (x) {
+        BodyInit<Real> body{b, r, nullptr};
+        DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcArr<Real>{x}, body, red, st));
+    } else {
+        // x == nullptr means "start from zero"; the caller zeroes x separately.
+        BodyInit<Real> body{b, r, nullptr};
+        DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcZero{}, body, red, st));
+    }
+}
+
+// The direction/update passes read beta/alpha from device memory inside the
+// source functor; they are loaded once per work-group through these wrappers.
+template <class Real>
+struct SrcDirDev {
+    const Real* p;
+    const Real* r;
+    const CGState* st;
+    __device__ __forceinline__ double operator()(int i) const {
+        SrcDir<Real> f{p, r, st->beta, st->first};
+        return f(i);
+    }
+};
+
+template <bool PB, class Real>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(GridDims d, PBCoef pb, const Real* p_old,
+                                                   const Real* r, Real* p_new, Reduce red,
+                                                   CGState* st) {
+    if (st->done) return;
+    const SrcDir<Real> src{p_old, r, st->beta, st->first};
+    BodyDir<Real> bd{p_new};
+    march<PB>(d, pb, src, bd);
+    __shared__ double sh[kBlock / kWave];
+    const double S = block_sum(bd.s, sh);
+    if (threadIdx.x == 0) store_partial(red.psum, blockIdx.x, S);
+    if (arrive_last(red.counter, gridDim.x)) {
+        const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            BodyDir<Real>::finalize(st, T, 0.0);
+            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <bool PB, class Real>
+__global__ __launch_bounds__(kBlock) void k_cg_update(GridDims d, PBCoef pb, const Real* p,
+                                                      Real* x, Real* r, Reduce red, CGState* st) {
+    if (st->done) return;
+    BodyUpdate<Real> bd{x, r, st->alpha};
+    march<PB>(d, pb, SrcArr<Real>{p}, bd);
+    __shared__ double sh[kBlock / kWave];
+    const double S = block_sum(bd.s, sh);
+    const double M = block_max(bd.m, sh);
+    if (threadIdx.x == 0) {
+        store_partial(red.psum, blockIdx.x, S);
+        store_partial(red.pmax, blockIdx.x, M);
+    }
+    if (arrive_last(red.counter, gridDim.x)) {
+        const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
+        const double U = reduce_partials_max(red.pmax, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            BodyUpdate<Real>::finalize(st, T, U);
+            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <class Real>
+void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p_old,
+                   const Real* r, Real* p_new, CGState* st, Reduce red) {
+    DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_dir<kPB, Real>), dim3(d.blocks), dim3(kBlock), 0, s,
+                                       d, pb, p_old, r, p_new, red, st));
+}
+
+template <class Real>
+void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
+                      Real* r, CGState* st, Reduce red) {
+    DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_update<kPB, Real>), dim3(d.blocks), dim3(kBlock), 0,
+                                       s, d, pb, p, x, r, red, st));
+}
+
+template void launch_cg_init<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
+                                     const double*, double*, CGState*, Reduce);
+template void launch_cg_init<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
+                                    const float*, float*, CGState*, Reduce);
+template void launch_cg_dir<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
+                                    const double*, double*, CGState*, Reduce);
+template void launch_cg_dir<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
+                                   const float*, float*, CGState*, Reduce);
+template void launch_cg_update<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
+                                       double*, double*, CGState*, Reduce);
+template void launch_cg_update<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
+                                      float*, float*, CGState*, Reduce);
+
+__global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int count_mode) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->tol = tol;
+        st->rtol = rtol;
+        st->limit = limit;
+        st->count_mode = count_mode;
+        st->done = 0;
+        st->res = -1;
+        st->iter = 0;
+        st->first = 1;
+    }
+}
+
+void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit,
+                     int count_mode) {
+    hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode);
+}
+
+void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
+                   const float* dcorr, const double* sigma, double* y_out, float* b32,
+                   CGState* st, Reduce red, int add_d) {
+    BodyRefine body{add_d ? y_out : nullptr, sigma, b32};
+    DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcSum{y, dcorr, add_d}, body, red, st));
+}
+
+void launch_true_residual_max(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
+                              const double* sigma, CGState* st, Reduce red) {
+    BodyTrueRes body{sigma};
+    DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcArr<double>{y}, body, red, st));
+}
+
+// ---- point-wise kernels ------------------------------------------------------
+__global__ void k_axpy(long n, double alpha, const double* __restrict__ v, double* __restrict__ u) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        u[i] += alpha * v[i];
+}
+__global__ void k_scale_copy(long n, double c, const double* __restrict__ in, double* __restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = in[i] * c;
+}
+__global__ void k_fill(long n, double v, double* out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = v;
+}
+__global__ void k_fill_f(long n, float v, float* out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = v;
+}
+__global__ void k_copy(long n, const double* __restrict__ in, double* __restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+static int pw_blocks(long n) {
+    long b = (n + kBlock - 1) / kBlock;
+    return (int)(b < 4096 ? (b < 1 ? 1 : b) : 4096);
+}
+
+void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u) {
+    hipLaunchKernelGGL(k_axpy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, alpha, v, u);
+}
+void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out) {
+    hipLaunchKernelGGL(k_scale_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, c, in, out);
+}
+void launch_fill(hipStream_t s, long n, double v, double* out) {
+    hipLaunchKernelGGL(k_fill, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, v, out);
+}
+void launch_fill_f(hipStream_t s, long n, float v, float* out) {
+    hipLaunchKernelGGL(k_fill_f, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, v, out);
+}
+void launch_copy(hipStream_t s, long n, const double* in, double* out) {
+    hipLaunchKernelGGL(k_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, in, out);
+}
+
+// 1/2 sum phi (library/src/grid_general.c:304-316) into st->resid.
+__global__ void k_half_sum(long n, const double* __restrict__ phi, Reduce red, double* out) {
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s += 0.5 * phi[i];
+    __shared__ double sh[kBlock / kWave];
+    const double S = block_sum(s, sh);
+    if (threadIdx.x == 0) store_partial(red.psum, blockIdx.x, S);
+    if (arrive_last(red.counter, gridDim.x)) {
+        const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            *out = T;
+            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+void launch_half_sum(hipStream_t s, long n, const double* phi, double* out, Reduce red) {
+    hipLaunchKernelGGL(k_half_sum, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, phi, red, out);
+}
+
+// ---- reference two-grid multigrid (library/src/multigrid.c) -----------------
+
+// One colour of red-black Gauss-Seidel in place: points with (i+j+k)%2 ==
+// color read only the other colour.  library/src/multigrid.c:663-727
+// (neighbour order i-1, i+1, j-1, j+1, k-1, k+1).
+__global__ __launch_bounds__(kBlock) void k_rbgs(GridDims d, const double* __restrict__ in,
+                                                 double* out, int color) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int chunk = b % d.chunks;
+    const int s = b / d.chunks;
+    const int pl = chunk * kBlock + threadIdx.x;
+    if (pl >= d.n2) return;
+    const int n = d.n, n2 = d.n2;
+    const int j = pl / n, k = pl - j * n;
+    const PlaneNb nb = plane_nb(pl, n, n2);
+    const int i0 = s * d.slab, i1 = min(n, i0 + d.slab);
+    const double inv_diag = -1.0 / 6.0;
+    for (int i = i0; i < i1; i++) {
+        if (((i + j + k) & 1) != color) continue;
+        const int base = i * n2, c = base + pl;
+        const int im = (i == 0 ? n - 1 : i - 1) * n2 + pl;
+        const int ip = (i == n - 1 ? 0 : i + 1) * n2 + pl;
+        const double sum_nb = out[im] + out[ip] + out[base + nb.jm] + out[base + nb.jp] +
+                              out[base + nb.km] + out[base + nb.kp];
+        out[c] = (in[c] - sum_nb) * inv_diag;
+    }
+}
+
+void launch_rbgs(hipStream_t s, const GridDims& d, const double* in, double* out, int iters) {
+    for (int it = 0; it < iters; it++) {
+        hipLaunchKernelGGL(k_rbgs, dim3(d.blocks), dim3(kBlock), 0, s, d, in, out, 0);
+        hipLaunchKernelGGL(k_rbgs, dim3(d.blocks), dim3(kBlock), 0, s, d, in, out, 1);
+    }
+}
+
+// r = in - A out, as (A out)(-1) + 1 in (library/src/multigrid.c:142-144).
+struct BodyMgRes {
+    static constexpr bool kReduce = false, kSkipWhenDone = false;
+    const double* in;
+    double* r;
+    double s = 0.0, m = 0.0;
+    __device__ void operator()(int c, double, double lap) { r[c] = lap * -1.0 + 1.0 * in[c]; }
+    __device__ static void finalize(CGState*, double, double) {}
+};
+
+void launch_mg_residual(hipStream_t s, const GridDims& d, const double* in, const double* out,
+                        double* r) {
+    BodyMgRes body{in, r};
+    run_pass<false>(s, d, PBCoef{}, SrcArr<double>{out}, body, Reduce{}, nullptr);
+}
+
+// 27-point full weighting, coarse (I,J,K) <- fine (2I,2J,2K).
+// library/src/multigrid.c:523-594 (sum order :571-590).
+__global__ void k_restrict27(int n, const double* __restrict__ in, double* __restrict__ out) {
+    const int M = n / 2;
+    const long total = (long)M * M * M;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int K = (int)(t % M), J = (int)((t / M) % M), I = (int)(t / ((long)M * M));
+        const int i = 2 * I, j = 2 * J, k = 2 * K;
+        const int ip = (i + 1) % n, im = (i - 1 + n) % n;
+        const int jp = (j + 1) % n, jm = (j - 1 + n) % n;
+        const int kp = (k + 1) % n, km = (k - 1 + n) % n;
+        const long N = n, N2 = N * N;
+#define F(a, b, c) in[(a) * N2 + (b) * N + (c)]
+        const double f = F(im, j, k) + F(ip, j, k) + F(i, jm, k) + F(i, jp, k) + F(i, j, km) + F(i, j, kp);
+        const double e = F(ip, j, km) + F(ip, j, kp) + F(im, j, km) + F(im, j, kp) + F(ip, jm, k) +
+                         F(ip, jp, k) + F(im, jm, k) + F(im, jp, k) + F(i, jm, km) + F(i, jm, kp) +
+                         F(i, jp, km) + F(i, jp, kp);
+        const double cc = F(ip, jp, km) + F(ip, jp, kp) + F(ip, jm, km) + F(ip, jm, kp) +
+                          F(im, jp, km) + F(im, jp, kp) + F(im, jm, km) + F(im, jm, kp);
+        out[t] = (8.0 * F(i, j, k) + 4.0 * f + 2.0 * e + cc) * (1.0 / 64.0);
+#undef F
+    }
+}
+
+void launch_restrict27(hipStream_t s, int n, const double* in, double* out) {
+    const long M = n / 2;
+    hipLaunchKernelGGL(k_restrict27, dim3(pw_blocks(M * M * M)), dim3(kBlock), 0, s, n, in, out);
+}
+
+// out += trilinear(in) from an nc^3 grid to (2 nc)^3 (library/src/multigrid.c:393-463,
+// fused with the correction daxpy of :158).
+__global__ void k_prolong_add(int nc, const double* __restrict__ in, double* __restrict__ out) {
+    const int N = 2 * nc;
+    const long total = (long)N * N * N;
+    const long M = nc, M2 = M * M;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % N), j = (int)((t / N) % N), i = (int)(t / ((long)N * N));
+        const int a0 = (i / 2) % nc, a1 = (a0 + 1) % nc;
+        const int b0 = (j / 2) % nc, b1 = (b0 + 1) % nc;
+        const int c0 = (k / 2) % nc, c1 = (c0 + 1) % nc;
+        const double wi0 = (i & 1) ? 0.5 : 1.0, wi1 = (i & 1) ? 0.5 : 0.0;
+        const double wj0 = (j & 1) ? 0.5 : 1.0, wj1 = (j & 1) ? 0.5 : 0.0;
+        const double wk0 = (k & 1) ? 0.5 : 1.0, wk1 = (k & 1) ? 0.5 : 0.0;
+#define C(a, b, c) in[(a) * M2 + (b) * M + (c)]
+        const double v = C(a0, b0, c0) * wi0 * wj0 * wk0 + C(a1, b0, c0) * wi1 * wj0 * wk0 +
+                         C(a0, b1, c0) * wi0 * wj1 * wk0 + C(a1, b1, c0) * wi1 * wj1 * wk0 +
+                         C(a0, b0, c1) * wi0 * wj0 * wk1 + C(a1, b0, c1) * wi1 * wj0 * wk1 +
+                         C(a0, b1, c1) * wi0 * wj1 * wk1 + C(a1, b1, c1) * wi1 * wj1 * wk1;
+#undef C
+        out[t] += 1.0 * v;
+    }
+}
+
+void launch_prolong_add(hipStream_t s, int nc, const double* in, double* out) {
+    const long N = 2L * nc;
+    hipLaunchKernelGGL(k_prolong_add, dim3(pw_blocks(N * N * N)), dim3(kBlock), 0, s, nc, in, out);
+}
+
+}  // namespace maze

@@ -1,0 +1,493 @@
+// particle_kernels.hip -- charge assignment, forces and integrators for gfx950.
+//
+// Point-wise expressions follow the reference's operation order
+// (library/src/charges.c:8-14, :123-141; library/src/forces.c:52-94,
+// :132-183; library/src/integrator_verlet.c:37-70;
+// library/src/integrator_ovrvo.c:45-123) so that per-ion results match the
+// CPU oracle bit for bit wherever no transcendental function is involved.
+#include <hipcub/hipcub.hpp>
+
+#include "particles.hpp"
+
+namespace maze {
+
+// CIC weight with the periodic distance (library/src/charges.c:8-14).
+__device__ __forceinline__ double cic_w(double x, double L, double h) {
+    x = fabs(x - round(x / L) * L);
+    if (x >= h) return 0.0;
+    return 1.0 - x / h;
+}
+
+__constant__ int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                  {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+
+static int blocks_for(long n) {
+    long b = (n + kBlock - 1) / kBlock;
+    return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
+}
+
+// Base cell floor(pos/h), wrapped into [0, n) (library/src/particles.c:315-321),
+// and its linear key for the cell sort.
+__global__ void k_cic_base(int np, int n, double h, const double* __restrict__ pos,
+                           int* __restrict__ base, unsigned int* __restrict__ keys,
+                           int* __restrict__ ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int c = (int)floor(pos[3 * i + d] / h);
+        c %= n;
+        if (c < 0) c += n;
+        b[d] = c;
+        base[3 * i + d] = c;
+    }
+    keys[i] = ((unsigned int)b[0] * n + b[1]) * n + b[2];
+    ids[i] = i;
+}
+
+void launch_cic_base(hipStream_t s, const PDev& p) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_cic_base, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h,
+                       p.pos, p.base, p.keys, p.ids);
+}
+
+size_t sort_tmp_bytes(int np, int key_bits) {
+    size_t bytes = 0;
+    MAZE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (unsigned int*)nullptr,
+                                                (unsigned int*)nullptr, (int*)nullptr,
+                                                (int*)nullptr, np, 0, key_bits));
+    return bytes;
+}
+
+// Stable radix sort of (cell key, ion id): ions of one cell stay in id order.
+void launch_cic_sort(hipStream_t s, const PDev& p, int key_bits) {
+    if (p.np == 0) return;
+    size_t bytes = p.sort_tmp_bytes;
+    MAZE_HIP(hipcub::DeviceRadixSort::SortPairs(p.sort_tmp, bytes, p.keys, p.keys_sorted, p.ids,
+                                                p.ids_sorted, p.np, 0, key_bits, s));
+}
+
+__device__ __forceinline__ int lower_bound_u32(const unsigned int* a, int n, unsigned int key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Contribution of ion j to grid node (gi, gj, gk): q_j * wx * wy * wz
+// (library/src/charges.c:136-137).
+__device__ __forceinline__ double cic_contrib(const double* pos, const double* charges, int j,
+                                             int gi, int gj, int gk, double L, double h) {
+    const double app = cic_w(pos[3 * j] - gi * h, L, h) * cic_w(pos[3 * j + 1] - gj * h, L, h) *
+                       cic_w(pos[3 * j + 2] - gk * h, L, h);
+    return charges[j] * app;
+}
+
+// Owner-computes gather: the thread of (ion i, corner c) finds every ion
+// whose CIC stencil covers node g = base_i + corner_c (ions in the 8 cells
+// g - {0,1}^3, looked up in the sorted cell keys).  Only the lowest ion id
+// writes q[g], summing the contributions in increasing ion id -- the order of
+// the reference's serial scatter -- so no float atomics are needed and the
+// result is bit-identical to it.
+__global__ void k_cic_gather(int np, int n, double h, const double* __restrict__ pos,
+                             const double* __restrict__ charges, const int* __restrict__ base,
+                             const unsigned int* __restrict__ keys_sorted,
+                             const int* __restrict__ ids_sorted, double* __restrict__ q,
+                             double* __restrict__ contrib) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= 8L * np) return;
+    const int i = (int)(t >> 3), c = (int)(t & 7);
+    const double L = n * h;
+    const int gi = (base[3 * i] + kCorner[c][0]) % n;
+    const int gj = (base[3 * i + 1] + kCorner[c][1]) % n;
+    const int gk = (base[3 * i + 2] + kCorner[c][2]) % n;
+    contrib[t] = cic_contrib(pos, charges, i, gi, gj, gk, L, h);
+
+    // Candidate cells and their ranges in the sorted key array.
+    int lo[8], hi[8];
+    int owner = i;
+    for (int cc = 0; cc < 8; cc++) {
+        const int ci = (gi - kCorner[cc][0] + n) % n;
+        const int cj = (gj - kCorner[cc][1] + n) % n;
+        const int ck = (gk - kCorner[cc][2] + n) % n;
+        const unsigned int key = ((unsigned int)ci * n + cj) * n + ck;
+        int a = lower_bound_u32(keys_sorted, np, key), b = a;
+        while (b < np && keys_sorted[b] == key) {
+            owner = min(owner, ids_sorted[b]);
+            b++;
+        }
+        lo[cc] = a;
+        hi[cc] = b;
+    }
+    if (owner != i) return;
+    // Sum contributions in increasing ion id (selection over the candidates).
+    double acc = 0.0;
+    int last = -1;
+    for (;;) {
+        int next = 0x7fffffff;
+        for (int cc = 0; cc < 8; cc++)
+            for (int s = lo[cc]; s < hi[cc]; s++) {
+                const int j = ids_sorted[s];
+                if (j > last && j < next) next = j;
+            }
+        if (next == 0x7fffffff) break;
+        acc += cic_contrib(pos, charges, next, gi, gj, gk, L, h);
+        last = next;
+    }
+    q[((long)gi * n + gj) * n + gk] = acc;
+}
+
+void launch_cic_gather(hipStream_t s, const PDev& p, double* q) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_cic_gather, dim3(blocks_for(8L * p.np)), dim3(kBlock), 0, s, p.np, p.n,
+                       p.h, p.pos, p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
+}
+
+// Single-work-group fixed-order sum of n values into *out.
+__device__ void block_sum_to(const double* v, long n, double* out) {
+    __shared__ double sh[kBlock / kWave];
+    double s = 0.0;
+    for (long i = threadIdx.x; i < n; i += kBlock) s += v[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_sum1(const double* v, long n, double* out) { block_sum_to(v, n, out); }
+
+void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce) {
+    hipLaunchKernelGGL(k_sum1, dim3(1), dim3(kBlock), 0, s, p.contrib, 8L * p.np, &ps->q_sum);
+}
+
+// Field force -q w grad(phi) by central differences at the 8 CIC corners
+// (library/src/forces.c:52-94).
+__global__ void k_force_fd(int np, int n, double h, const double* __restrict__ phi,
+                           const int* __restrict__ base, const double* __restrict__ charges,
+                           const double* __restrict__ pos, double* __restrict__ forces) {
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    const long N = n, N2 = N * N;
+    const double h2 = 2.0 * h, L = N * h;
+    const double px = pos[3 * ip], py = pos[3 * ip + 1], pz = pos[3 * ip + 2], chg = charges[ip];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int c = 0; c < 8; c++) {
+        const long i = (base[3 * ip] + kCorner[c][0]) % n;
+        const long j = (base[3 * ip + 1] + kCorner[c][1]) % n;
+        const long k = (base[3 * ip + 2] + kCorner[c][2]) % n;
+        const double qc = chg * cic_w(px - i * h, L, h) * cic_w(py - j * h, L, h) * cic_w(pz - k * h, L, h);
+        const long im = (i == 0 ? N - 1 : i - 1), ii = (i == N - 1 ? 0 : i + 1);
+        const long jm = (j == 0 ? N - 1 : j - 1), jj = (j == N - 1 ? 0 : j + 1);
+        const long km = (k == 0 ? N - 1 : k - 1), kk = (k == N - 1 ? 0 : k + 1);
+        double E = (phi[im * N2 + j * N + k] - phi[ii * N2 + j * N + k]) / h2;
+        f0 += qc * E;
+        E = (phi[i * N2 + jm * N + k] - phi[i * N2 + jj * N + k]) / h2;
+        f1 += qc * E;
+        E = (phi[i * N2 + j * N + km] - phi[i * N2 + j * N + kk]) / h2;
+        f2 += qc * E;
+    }
+    forces[3 * ip] = f0;
+    forces[3 * ip + 1] = f1;
+    forces[3 * ip + 2] = f2;
+}
+
+void launch_force_fd(hipStream_t s, const PDev& p, const double* phi) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_force_fd, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
+                       p.base, p.charges, p.pos, p.fe);
+}
+
+// Shifted-force Tosi-Fumi over all pairs within r_cut (library/src/forces.c:
+// 132-183).  Thread (i, segment s) visits j in its segment in increasing
+// order; the wave's lanes are consecutive i at the same j, so the j position
+// is a broadcast load.  Partials per segment are folded in segment order.
+__global__ __launch_bounds__(kBlock) void k_tf_pairs(int np, int n_typ, double L, double r_cut,
+                                                     const double* __restrict__ pos,
+                                                     const int* __restrict__ types,
+                                                     const double* __restrict__ table, int seg,
+                                                     double* __restrict__ fpart,
+                                                     double* __restrict__ epart,
+                                                     int* __restrict__ cpart) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (i >= np) return;
+    const int j0 = s * seg, j1 = min(np, j0 + seg);
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int ti = types[i];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, e = 0.0;
+    int cnt = 0;
+    for (int j = j0; j < j1; j++) {
+        if (j == i) continue;
+        double app = xi - pos[3 * j];
+        app -= L * round(app / L);
+        double r2 = app * app;
+        double d0 = app;
+        app = yi - pos[3 * j + 1];
+        app -= L * round(app / L);
+        double d1 = app;
+        r2 += app * app;
+        app = zi - pos[3 * j + 2];
+        app -= L * round(app / L);
+        double d2 = app;
+        r2 += app * app;
+        const double r = sqrt(r2);
+        if (r > r_cut) continue;
+        cnt++;
+        d0 /= r;
+        d1 /= r;
+        d2 /= r;
+        const double* t = table + (ti * n_typ + types[j]) * 7;
+        const double a = t[0], b = t[1], c = t[2], dd = t[3], sg = t[4], al = t[5], be = t[6];
+        const double ex = exp(b * (sg - r));
+        const double ir = 1.0 / r;
+        const double ir2 = ir * ir;
+        const double ir6 = ir2 * ir2 * ir2;
+        const double ir8 = ir6 * ir2;
+        const double fm = b * a * ex - 6 * c * (ir6 * ir) - 8 * dd * (ir8 * ir) - al;
+        const double vm = a * ex - c * ir6 - dd * ir8 + al * r + be;
+        f0 += fm * d0;
+        f1 += fm * d1;
+        f2 += fm * d2;
+        e += vm;
+    }
+    const long o = (long)s * np + i;
+    fpart[3 * o] = f0;
+    fpart[3 * o + 1] = f1;
+    fpart[3 * o + 2] = f2;
+    epart[o] = e;
+    cpart[o] = cnt;
+}
+
+__global__ void k_tf_fold(int np, int segs, const double* __restrict__ fpart,
+                          const double* __restrict__ epart, const int* __restrict__ cpart,
+                          double* __restrict__ fn, double* __restrict__ ei,
+                          int* __restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, e = 0.0;
+    int c = 0;
+    for (int s = 0; s < segs; s++) {
+        const long o = (long)s * np + i;
+        f0 += fpart[3 * o];
+        f1 += fpart[3 * o + 1];
+        f2 += fpart[3 * o + 2];
+        e += epart[o];
+        c += cpart[o];
+    }
+    fn[3 * i] = f0;
+    fn[3 * i + 1] = f1;
+    fn[3 * i + 2] = f2;
+    ei[i] = e;
+    counts[i] = c;
+}
+
+__global__ void k_half(const double* v, long n, double* out) {
+    __shared__ double sh[kBlock / kWave];
+    double s = 0.0;
+    for (long i = threadIdx.x; i < n; i += kBlock) s += v[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = s / 2;
+}
+
+void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce) {
+    if (p.np == 0) {
+        hipLaunchKernelGGL(k_half, dim3(1), dim3(kBlock), 0, s, p.epart, 0L, &ps->pe_tf);
+        return;
+    }
+    const int segs = p.segments;
+    const int seg = (p.np + segs - 1) / segs;
+    hipLaunchKernelGGL(k_tf_pairs, dim3((p.np + kBlock - 1) / kBlock, segs), dim3(kBlock), 0, s,
+                       p.np, p.n_typ, p.L, p.r_cut, p.pos, p.types, p.table, seg, p.fpart,
+                       p.epart, p.cpart);
+    // The per-ion energies reuse the first segment row of epart.
+    hipLaunchKernelGGL(k_tf_fold, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, segs, p.fpart,
+                       p.epart, p.cpart, p.fn, p.epart + (long)segs * p.np, p.counts);
+    hipLaunchKernelGGL(k_half, dim3(1), dim3(kBlock), 0, s, p.epart + (long)segs * p.np,
+                       (long)p.np, &ps->pe_tf);
+}
+
+// F_tot = 0 + 1 F_elec + 1 F_noel (library/src/particles.c:777-795).
+__global__ void k_ftot(long n, const double* fe, const double* fn, double* ft) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    v += 1.0 * fe[i];
+    v += 1.0 * fn[i];
+    ft[i] = v;
+}
+
+void launch_force_total(hipStream_t s, const PDev& p) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_ftot, dim3(blocks_for(3L * p.np)), dim3(kBlock), 0, s, 3L * p.np, p.fe,
+                       p.fn, p.ft);
+}
+
+// K = 1/2 sum m |v|^2 (library/src/particles.c:802-820); optionally the
+// Verlet thermostat factor sqrt(T / T_now) (integrator_verlet.c:26-35).
+__global__ void k_kinetic(int np, const double* vel, const double* mass, PScalars* ps,
+                          double T_target, int want_scale) {
+    __shared__ double sh[kBlock / kWave];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += kBlock) {
+        double app = 0.0;
+        for (int d = 0; d < 3; d++) app += vel[3 * i + d] * vel[3 * i + d];
+        s += mass[i] * app;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        const double K = 0.5 * s;
+        ps->kinetic = K;
+        if (want_scale) {
+            const double T_now = 2 * K / (3 * np * 3.1668e-6);
+            ps->scale = sqrt(T_target / T_now);
+        }
+    }
+}
+
+void launch_kinetic(hipStream_t s, const PDev& p, PScalars* ps, Reduce, double T_target,
+                    int want_scale) {
+    hipLaunchKernelGGL(k_kinetic, dim3(1), dim3(kBlock), 0, s, p.np, p.vel, p.mass, ps, T_target,
+                       want_scale);
+}
+
+__device__ __forceinline__ double wrap_pos(double a, double L) {
+    if (a < 0) return a + L;
+    if (a >= L) return a - L;
+    return a;
+}
+
+// Velocity Verlet part 1 (library/src/integrator_verlet.c:12-53).
+__global__ void k_verlet1(int np, double dt, double L, double* pos, double* vel, const double* ft,
+                          const double* mass, const PScalars* ps, int rescale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double scale = rescale ? ps->scale : 1.0;
+    for (int d = 0; d < 3; d++) {
+        const int k = 3 * i + d;
+        double v = vel[k];
+        if (rescale) v *= scale;
+        v += 0.5 * dt * ft[k] / mass[i];
+        vel[k] = v;
+        pos[k] = wrap_pos(pos[k] + dt * v, L);
+    }
+}
+
+void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_verlet1, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, dt, p.L, p.pos,
+                       p.vel, p.ft, p.mass, ps, rescale);
+}
+
+// v += cdt F / m with cdt = 0.5 dt (Verlet) or 0.5 dt c2 (OVRVO V block).
+__global__ void k_kick(int np, double cdt, double* vel, const double* ft, const double* mass) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    for (int d = 0; d < 3; d++) vel[3 * i + d] += cdt * ft[3 * i + d] / mass[i];
+}
+
+void launch_kick(hipStream_t s, const PDev& p, double cdt) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_kick, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, cdt, p.vel, p.ft,
+                       p.mass);
+}
+
+// x += c2 dt v with a single-image wrap (library/src/integrator_ovrvo.c:98-123).
+__global__ void k_drift(int np, double c2dt, double L, double* pos, const double* vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    for (int d = 0; d < 3; d++) pos[3 * i + d] = wrap_pos(pos[3 * i + d] + c2dt * vel[3 * i + d], L);
+}
+
+void launch_drift(hipStream_t s, const PDev& p, double c2dt) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_drift, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, c2dt, p.L, p.pos,
+                       p.vel);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11): 10 rounds of the published
+// multiply-xor network with Weyl key increments.
+__device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// O block: v <- sqrt(c1) v + sqrt((1 - c1) kB T / m) xi
+// (library/src/integrator_ovrvo.c:45-78) with xi from (seed, call, ion).
+__global__ void k_o_block(int np, uint64_t seed, uint64_t call, double c1s, double var2,
+                          const double* mass, double* vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    uint32_t c[4] = {(uint32_t)i, (uint32_t)call, (uint32_t)(call >> 32), 0x4D415A45u};
+    philox4x32(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double sc = 2.3283064365386963e-10;
+    const double u0 = ((double)c[0] + 0.5) * sc, u1 = ((double)c[1] + 0.5) * sc;
+    const double u2 = ((double)c[2] + 0.5) * sc, u3 = ((double)c[3] + 0.5) * sc;
+    const double ra = sqrt(-2.0 * log(u0)), rb = sqrt(-2.0 * log(u2));
+    const double ta = 6.283185307179586 * u1, tb = 6.283185307179586 * u3;
+    const double xi[3] = {ra * cos(ta), ra * sin(ta), rb * cos(tb)};
+    const double var1 = sqrt(var2 / mass[i]);
+    for (int d = 0; d < 3; d++) {
+        double v = vel[3 * i + d];
+        v *= c1s;
+        v += var1 * xi[d];
+        vel[3 * i + d] = v;
+    }
+}
+
+void launch_o_block(hipStream_t s, const PDev& p, uint64_t seed, uint64_t call, double c1s,
+                    double var2) {
+    if (p.np == 0) return;
+    hipLaunchKernelGGL(k_o_block, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, seed, call, c1s,
+                       var2, p.mass, p.vel);
+}
+
+// sum m v per component (library/src/particles.c:822-839).
+__global__ void k_momentum(int np, const double* vel, const double* mass, PScalars* ps) {
+    __shared__ double sh[kBlock / kWave];
+    for (int d = 0; d < 3; d++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < np; i += kBlock) s += mass[i] * vel[3 * i + d];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) ps->mom[d] = s;
+        __syncthreads();
+    }
+}
+
+void launch_momentum(hipStream_t s, const PDev& p, PScalars* ps) {
+    hipLaunchKernelGGL(k_momentum, dim3(1), dim3(kBlock), 0, s, p.np, p.vel, p.mass, ps);
+}
+
+// v -= 2 (sum of v over the ion's type) / n_p (library/src/particles.c:841-857).
+__global__ void k_rescale(int np, int n_typ, const int* types, double* vel, PScalars* ps) {
+    __shared__ double sh[kBlock / kWave];
+    for (int t = 0; t < n_typ; t++)
+        for (int d = 0; d < 3; d++) {
+            double s = 0.0;
+            for (int i = threadIdx.x; i < np; i += kBlock)
+                if (types[i] == t) s += vel[3 * i + d];
+            s = block_sum(s, sh);
+            if (threadIdx.x == 0) ps->typesum[t * 3 + d] = s;
+            __syncthreads();
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += kBlock)
+        for (int d = 0; d < 3; d++) vel[3 * i + d] -= 2 * ps->typesum[types[i] * 3 + d] / np;
+}
+
+void launch_rescale_velocities(hipStream_t s, const PDev& p, PScalars* ps) {
+    hipLaunchKernelGGL(k_rescale, dim3(1), dim3(kBlock), 0, s, p.np, p.n_typ, p.types, p.vel, ps);
+}
+
+}  // namespace maze

@@ -1,0 +1,64 @@
+// particles.hpp -- device particle state and launchers.
+#pragma once
+
+#include "common.hpp"
+#include "grid.hpp"
+
+namespace maze {
+
+struct PDev {
+    int np, n_typ, n;
+    double L, h, r_cut;
+    int* types;
+    double* pos;      // (np, 3) Bohr
+    double* vel;      // (np, 3)
+    double* mass;     // (np)
+    double* charges;  // (np)
+    double* fe;       // electrostatic force (np, 3)
+    double* fn;       // Tosi-Fumi force (np, 3)
+    double* ft;       // total force (np, 3)
+    int* base;        // CIC base cell (np, 3), wrapped into [0, n)
+    unsigned int* keys;
+    unsigned int* keys_sorted;
+    int* ids;
+    int* ids_sorted;
+    double* table;    // Tosi-Fumi constants per type pair, 7 each
+    double* fpart;    // pair-force partials (segments, np, 3)
+    double* epart;    // pair-energy partials (segments, np)
+    int* cpart;       // pair counts (segments, np)
+    int* counts;      // pairs within the cutoff per ion
+    double* contrib;  // per-(ion, corner) CIC contribution, (np, 8)
+    void* sort_tmp;
+    size_t sort_tmp_bytes;
+    int segments;
+};
+
+// Device-resident scalars of the particle side.
+struct PScalars {
+    double q_sum;      // sum of all CIC contributions of the last spread
+    double pe_tf;      // Tosi-Fumi potential energy
+    double kinetic;    // 1/2 sum m v^2
+    double scale;      // velocity-rescale factor of the Verlet thermostat
+    double mom[3];
+    double typesum[3 * 16];
+};
+
+size_t sort_tmp_bytes(int np, int key_bits);
+void launch_cic_base(hipStream_t s, const PDev& p);
+void launch_cic_sort(hipStream_t s, const PDev& p, int key_bits);
+void launch_cic_gather(hipStream_t s, const PDev& p, double* q);
+void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
+void launch_force_fd(hipStream_t s, const PDev& p, const double* phi);
+void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
+void launch_force_total(hipStream_t s, const PDev& p);
+void launch_kinetic(hipStream_t s, const PDev& p, PScalars* ps, Reduce red, double T_target,
+                    int want_scale);
+void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale);
+void launch_kick(hipStream_t s, const PDev& p, double cdt);
+void launch_drift(hipStream_t s, const PDev& p, double c2dt);
+void launch_o_block(hipStream_t s, const PDev& p, uint64_t seed, uint64_t call, double c1s,
+                    double var2);
+void launch_momentum(hipStream_t s, const PDev& p, PScalars* ps);
+void launch_rescale_velocities(hipStream_t s, const PDev& p, PScalars* ps);
+
+}  // namespace maze

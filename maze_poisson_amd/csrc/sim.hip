@@ -1,0 +1,605 @@
+// sim.hip -- host-side driver of one GPU-resident replica.
+//
+// All state lives in HBM; the host only launches kernels and, once per
+// solve (in chunks of iterations), reads the small CGState block to learn
+// whether the device-side stop test has fired.  Kernels launched after the
+// stop are no-ops (they test st->done first), so the state is exactly the
+// state at convergence.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "sim.hpp"
+
+namespace maze {
+
+static constexpr double kBoltz = 3.1668e-6;  // Hartree / K, library/include/constants.h:10
+static constexpr int kMaxPartials = 1 << 15;
+
+static void fatal(const char* msg) {
+    std::fprintf(stderr, "%s\n", msg);
+    std::fflush(stderr);
+    std::exit(1);
+}
+
+template <class T>
+static T* dalloc(long count) {
+    T* p = nullptr;
+    if (count <= 0) count = 1;
+    MAZE_HIP(hipMalloc(&p, sizeof(T) * (size_t)count));
+    MAZE_HIP(hipMemset(p, 0, sizeof(T) * (size_t)count));
+    return p;
+}
+
+template <class T>
+static void dfree(T*& p) {
+    if (p) MAZE_HIP(hipFree(p));
+    p = nullptr;
+}
+
+// Planes per work-group so that one sweep launches about 1024+ work-groups.
+static int choose_slab(int n) {
+    const long chunks = ((long)n * n + kBlock - 1) / kBlock;
+    long s = chunks * n / 1024;
+    if (s < 1) s = 1;
+    if (s > 16) s = 16;
+    return (int)s;
+}
+
+Sim::Sim() {
+    MAZE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    st = dalloc<CGState>(1);
+    MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
+    std::memset(st_host, 0, sizeof(CGState));
+    red.psum = dalloc<double>(kMaxPartials);
+    red.pmax = dalloc<double>(kMaxPartials);
+    red.counter = dalloc<unsigned int>(4);
+    dscalar = dalloc<double>(8);
+    MAZE_HIP(hipHostMalloc(&hscalar, 8 * sizeof(double)));
+    ps = dalloc<PScalars>(1);
+    MAZE_HIP(hipHostMalloc(&ps_host, sizeof(PScalars)));
+}
+
+Sim::~Sim() {
+    sync();
+    free_grid();
+    free_particles();
+    dfree(st);
+    dfree(red.psum);
+    dfree(red.pmax);
+    dfree(red.counter);
+    dfree(dscalar);
+    dfree(ps);
+    if (st_host) MAZE_HIP(hipHostFree(st_host));
+    if (hscalar) MAZE_HIP(hipHostFree(hscalar));
+    if (ps_host) MAZE_HIP(hipHostFree(ps_host));
+    for (auto e : ev_pool) MAZE_HIP(hipEventDestroy(e));
+    MAZE_HIP(hipStreamDestroy(stream));
+}
+
+void Sim::free_grid() {
+    for (double** a : {&q, &y, &phi, &phip, &tmp, &sigma, &r, &p0, &p1, &y_alt, &ex, &ey, &ez, &k2,
+                       &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1})
+        dfree(*a);
+    for (float** a : {&b32, &d32, &r32, &q32a, &q32b}) dfree(*a);
+    G = 0;
+}
+
+void Sim::free_particles() {
+    dfree(p.types);
+    for (double** a : {&p.pos, &p.vel, &p.mass, &p.charges, &p.fe, &p.fn, &p.ft, &p.table,
+                       &p.fpart, &p.epart, &p.contrib})
+        dfree(*a);
+    dfree(p.base);
+    dfree(p.keys);
+    dfree(p.keys_sorted);
+    dfree(p.ids);
+    dfree(p.ids_sorted);
+    dfree(p.cpart);
+    dfree(p.counts);
+    if (p.sort_tmp) MAZE_HIP(hipFree(p.sort_tmp));
+    p.sort_tmp = nullptr;
+    p.np = 0;
+}
+
+PBCoef Sim::pbc() const {
+    if (!pb) return PBCoef{nullptr, nullptr, nullptr, nullptr};
+    return PBCoef{ex, ey, ez, k2};
+}
+
+// library/src/grid_general.c:24-83 and the per-type init functions.
+void Sim::init_grid(int n_, double L_, double h_, double tol_, double eps, double eps_int_,
+                    int type, int precond_) {
+    if (type == kFFT)
+        fatal("maze: the FFT grid (continuous-symbol FFTW solve) is not provided by this library");
+    if (type < 0 || type > 4) fatal("maze: invalid grid type");
+    if ((type == kMG || type == kMazeMG) && (n_ % 2 != 0 || n_ < 32))
+        fatal("Multigrid: requires an even grid of at least 32 points per side");
+    if (type == kMazeLCG && precond_ > kPJacobi)
+        fatal("maze: MAZE-LCG provides the NONE and JACOBI preconditioners");
+    free_grid();
+    n = n_; L = L_; h = h_; tol = tol_; eps_s = eps; eps_int = eps_int_;
+    grid_type = type; precond = precond_;
+    pb = 0;
+    G = (long)n * n * n;
+    dims = make_dims(n, choose_slab(n));
+    if (dims.blocks > kMaxPartials) fatal("maze: grid too large for the reduction buffers");
+    q = dalloc<double>(G); y = dalloc<double>(G); phi = dalloc<double>(G);
+    phip = dalloc<double>(G); tmp = dalloc<double>(G); sigma = dalloc<double>(G);
+    r = dalloc<double>(G); p0 = dalloc<double>(G); p1 = dalloc<double>(G);
+    if (type == kMG || type == kMazeMG) {
+        nc = n / 2;
+        cdims = make_dims(nc, choose_slab(nc));
+        const long Gc = (long)nc * nc * nc;
+        mg_r = dalloc<double>(G);
+        c_rhs = dalloc<double>(Gc); c_eps = dalloc<double>(Gc); c_r = dalloc<double>(Gc);
+        c_p0 = dalloc<double>(Gc); c_p1 = dalloc<double>(Gc);
+    }
+    last_iters = 32;
+    sync();
+}
+
+// library/src/grid_general.c:85-100 (maps default to vacuum until set).
+void Sim::init_pb(double w_, double kbar2_) {
+    pb = 1; w = w_; kbar2 = kbar2_;
+    ex = dalloc<double>(G); ey = dalloc<double>(G); ez = dalloc<double>(G); k2 = dalloc<double>(G);
+    launch_fill(stream, G, 1.0, ex);
+    launch_fill(stream, G, 1.0, ey);
+    launch_fill(stream, G, 1.0, ez);
+    launch_fill(stream, G, 0.0, k2);
+    sync();
+}
+
+void Sim::set_eps_k2(const double* ex_, const double* ey_, const double* ez_, const double* k2_) {
+    if (!pb) fatal("maze: solver_set_eps_k2 needs solver_initialize_grid_pois_boltz first");
+    MAZE_HIP(hipMemcpy(ex, ex_, G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(ey, ey_, G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(ez, ez_, G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(k2, k2_, G * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// library/src/solver_md.c:51-65, particles.c:68-109, :181-228.
+void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, const int* types,
+                         const double* pos, const double* vel, const double* mass,
+                         const double* charges, const double* pp) {
+    if (n_typ < 1 || n_typ > 16) fatal("maze: between 1 and 16 particle types are supported");
+    if (np < 0) fatal("maze: negative particle count");
+    free_particles();
+    p.np = np; p.n_typ = n_typ; p.n = n_; p.L = L_; p.h = h_;
+    p.r_cut = L_ / 2.0;
+    p.types = dalloc<int>(np);
+    p.pos = dalloc<double>(3L * np); p.vel = dalloc<double>(3L * np);
+    p.mass = dalloc<double>(np); p.charges = dalloc<double>(np);
+    p.fe = dalloc<double>(3L * np); p.fn = dalloc<double>(3L * np); p.ft = dalloc<double>(3L * np);
+    p.base = dalloc<int>(3L * np);
+    p.keys = dalloc<unsigned int>(np); p.keys_sorted = dalloc<unsigned int>(np);
+    p.ids = dalloc<int>(np); p.ids_sorted = dalloc<int>(np);
+    p.contrib = dalloc<double>(8L * np);
+    // Enough j-segments that np * segments work-items fill the chip.
+    int segs = (int)std::ceil(65536.0 / std::max(np, 1));
+    p.segments = std::max(1, std::min(64, segs));
+    p.fpart = dalloc<double>(3L * np * p.segments);
+    p.epart = dalloc<double>((long)np * (p.segments + 1));
+    p.cpart = dalloc<int>((long)np * p.segments);
+    p.counts = dalloc<int>(np);
+    // Tosi-Fumi shifted-force constants per type pair, with the reference's
+    // expressions (library/src/particles.c:191-215), evaluated on the host.
+    std::vector<double> table(7 * n_typ * n_typ);
+    const double rc = p.r_cut;
+    const double rc6 = std::pow(rc, 6), rc7 = rc6 * rc, rc8 = rc7 * rc, rc9 = rc8 * rc;
+    for (int t = 0; t < n_typ * n_typ; t++) {
+        const double A = pp[t * 5], B = pp[t * 5 + 1], C = pp[t * 5 + 2], D = pp[t * 5 + 3],
+                     s = pp[t * 5 + 4];
+        const double v_shift = A * std::exp(B * (s - rc)) - C / rc6 - D / rc8;
+        const double alpha = A * B * std::exp(B * (s - rc)) - 6 * C / rc7 - 8 * D / rc9;
+        const double beta = -v_shift - alpha * rc;
+        double* o = &table[t * 7];
+        o[0] = A; o[1] = B; o[2] = C; o[3] = D; o[4] = s; o[5] = alpha; o[6] = beta;
+    }
+    p.table = dalloc<double>(7L * n_typ * n_typ);
+    MAZE_HIP(hipMemcpy(p.table, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (np > 0) {
+        MAZE_HIP(hipMemcpy(p.types, types, np * sizeof(int), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(p.pos, pos, 3L * np * sizeof(double), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(p.vel, vel, 3L * np * sizeof(double), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(p.mass, mass, np * sizeof(double), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(p.charges, charges, np * sizeof(double), hipMemcpyHostToDevice));
+    }
+    const unsigned long cells = (unsigned long)n_ * n_ * n_;
+    key_bits = 1;
+    while ((1ul << key_bits) < cells) key_bits++;
+    p.sort_tmp_bytes = sort_tmp_bytes(std::max(np, 1), key_bits);
+    MAZE_HIP(hipMalloc(&p.sort_tmp, std::max<size_t>(p.sort_tmp_bytes, 16)));
+    sync();
+}
+
+// library/src/solver_md.c:71-89, integrator_ovrvo.c:140-148.
+void Sim::init_integrator(double dt_, double T_, double gamma, int type, int en) {
+    itype = type; dt = dt_; T = T_; enabled = 0; c1 = 1.0; c2 = 1.0; calls = 0;
+    if (en == 1) {
+        enabled = 1;
+        if (type == kOVRVO) {
+            c1 = std::exp(-gamma * dt);
+            c2 = std::sqrt(2 / (gamma * dt) * std::tanh(0.5 * gamma * dt));
+        }
+    }
+}
+
+void Sim::read_state() {
+    MAZE_HIP(hipMemcpyAsync(st_host, st, sizeof(CGState), hipMemcpyDeviceToHost, stream));
+    sync();
+}
+
+hipEvent_t Sim::next_event() {
+    if (ev_next == ev_pool.size()) {
+        hipEvent_t e;
+        MAZE_HIP(hipEventCreate(&e));
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_next++];
+}
+
+// Accumulate the durations of the first `iters_done` iterations' kernels
+// (later launches were no-ops after the stop and are not counted).
+void Sim::prof_flush(int iters_done, int timer_base) {
+    for (auto& pe : pending) {
+        if (pe.iter >= iters_done) continue;
+        float ms = 0.f;
+        MAZE_HIP(hipEventElapsedTime(&ms, pe.a, pe.b));
+        KernelTimer& t = timers[timer_base + pe.timer];
+        t.total_ms += ms;
+        t.count += 1;
+    }
+    pending.clear();
+    ev_next = 0;
+}
+
+// Implicit-Jacobi CG, library/src/plcg.c:28-90: x holds the warm start.
+// Iterations are launched in chunks sized from the previous solve and the
+// device-side stop flag is polled between chunks.
+int Sim::cg_solve(const GridDims& d, const PBCoef& pbc_, const double* b, double* x, double tol_,
+                  int limit, double* rbuf, double* pa, double* pbuf, int* pred) {
+    launch_cg_reset(stream, st, tol_, 0.0, limit, 0);
+    launch_cg_init<double>(stream, d, pbc_, x, b, rbuf, st, red);
+    int launched = 0;
+    int chunk = std::max(4, *pred + 1);
+    double* pin = pa;
+    double* pout = pbuf;
+    const bool timing = prof != 0;
+    for (;;) {
+        for (int c = 0; c < chunk && launched < limit; c++, launched++) {
+            if (timing) {
+                hipEvent_t e0 = next_event(), e1 = next_event(), e2 = next_event();
+                MAZE_HIP(hipEventRecord(e0, stream));
+                launch_cg_dir<double>(stream, d, pbc_, pin, rbuf, pout, st, red);
+                MAZE_HIP(hipEventRecord(e1, stream));
+                launch_cg_update<double>(stream, d, pbc_, pout, x, rbuf, st, red);
+                MAZE_HIP(hipEventRecord(e2, stream));
+                pending.push_back({0, e0, e1, launched});
+                pending.push_back({1, e1, e2, launched});
+            } else {
+                launch_cg_dir<double>(stream, d, pbc_, pin, rbuf, pout, st, red);
+                launch_cg_update<double>(stream, d, pbc_, pout, x, rbuf, st, red);
+            }
+            std::swap(pin, pout);
+        }
+        read_state();
+        if (st_host->done || launched >= limit) break;
+        chunk = std::max(4, chunk / 4);
+    }
+    const int done_iters = st_host->res >= 0 ? st_host->res + 1 : launched;
+    if (timing) prof_flush(done_iters, 0);
+    *pred = done_iters;
+    return st_host->res;
+}
+
+// Coarse CG with a relative stop, returning the iteration count
+// (library/src/multigrid.c:13-96); x must hold the initial guess.
+int Sim::cg_count_solve(const GridDims& d, const double* b, double* x, double rtol, int maxit,
+                        double* rbuf, double* pa, double* pbuf, int* pred) {
+    launch_cg_reset(stream, st, 0.0, rtol, maxit, 1);
+    launch_cg_init<double>(stream, d, PBCoef{}, x, b, rbuf, st, red);
+    int launched = 0;
+    int chunk = std::max(4, *pred + 1);
+    double* pin = pa;
+    double* pout = pbuf;
+    for (;;) {
+        for (int c = 0; c < chunk && launched < maxit; c++, launched++) {
+            launch_cg_dir<double>(stream, d, PBCoef{}, pin, rbuf, pout, st, red);
+            launch_cg_update<double>(stream, d, PBCoef{}, pout, x, rbuf, st, red);
+            std::swap(pin, pout);
+        }
+        read_state();
+        if (st_host->done || launched >= maxit) break;
+        chunk = std::max(4, chunk / 4);
+    }
+    *pred = std::max(1, st_host->res);
+    return st_host->res;
+}
+
+// FP32-storage CG for the refinement correction A d = b (d starts at 0),
+// relative stop max|r| <= rtol max|b|; returns the iterations performed.
+int Sim::cg_inner_f32(const float* b, float* x, double rtol, int limit) {
+    launch_fill_f(stream, G, 0.0f, x);
+    launch_cg_reset(stream, st, 0.0, rtol, limit, 1);
+    launch_cg_init<float>(stream, dims, pbc(), nullptr, b, r32, st, red);
+    int launched = 0;
+    int chunk = std::max(4, last_inner + 1);
+    float* pin = q32a;
+    float* pout = q32b;
+    const bool timing = prof != 0;
+    for (;;) {
+        for (int c = 0; c < chunk && launched < limit; c++, launched++) {
+            if (timing) {
+                hipEvent_t e0 = next_event(), e1 = next_event(), e2 = next_event();
+                MAZE_HIP(hipEventRecord(e0, stream));
+                launch_cg_dir<float>(stream, dims, pbc(), pin, r32, pout, st, red);
+                MAZE_HIP(hipEventRecord(e1, stream));
+                launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red);
+                MAZE_HIP(hipEventRecord(e2, stream));
+                pending.push_back({0, e0, e1, launched});
+                pending.push_back({1, e1, e2, launched});
+            } else {
+                launch_cg_dir<float>(stream, dims, pbc(), pin, r32, pout, st, red);
+                launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red);
+            }
+            std::swap(pin, pout);
+        }
+        read_state();
+        if (st_host->done || launched >= limit) break;
+        chunk = std::max(4, chunk / 4);
+    }
+    const int it = st_host->res;
+    if (timing) prof_flush(it, 2);
+    last_inner = std::max(1, it);
+    return it;
+}
+
+// MaZe step in FP64, library/src/verlet.c:31-71.
+int Sim::maze_fp64() {
+    const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
+    launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma);
+    double* old_phip = phip;
+    phip = phi;
+    phi = tmp;
+    tmp = old_phip;
+    const int res = cg_solve(dims, pbc(), sigma, y, tol, n * n, r, p0, p1, &last_iters);
+    launch_axpy(stream, G, -1.0, y, phi);
+    return res;
+}
+
+// MaZe step with FP32-storage CG inside FP64 iterative refinement: the
+// outer loop recomputes the true residual sigma - A y in FP64 and stops on
+// the reference's criterion max|A y - sigma| <= tol; each inner solve is an
+// FP32 CG to a relative reduction inner_rtol.  Returns the total number of
+// inner iterations, or -1 if the outer loop did not converge.
+int Sim::maze_mixed() {
+    if (!b32) {
+        b32 = dalloc<float>(G); d32 = dalloc<float>(G); r32 = dalloc<float>(G);
+        q32a = dalloc<float>(G); q32b = dalloc<float>(G); y_alt = dalloc<double>(G);
+    }
+    const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
+    launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma);
+    double* old_phip = phip;
+    phip = phi;
+    phi = tmp;
+    tmp = old_phip;
+    launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 0);
+    read_state();
+    double resid = st_host->resid;
+    int total = 0, outer = 0, res = -1;
+    const int inner_limit = n * n;
+    while (true) {
+        if (resid <= tol) {
+            res = total;
+            break;
+        }
+        if (outer >= 50) break;
+        const int it = cg_inner_f32(b32, d32, inner_rtol, inner_limit);
+        total += std::max(it, 0);
+        launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 1);
+        std::swap(y, y_alt);
+        read_state();
+        resid = st_host->resid;
+        outer++;
+    }
+    launch_axpy(stream, G, -1.0, y, phi);
+    last_res = res;
+    return res;
+}
+
+// library/src/grid_lcg.c:126-149.
+int Sim::lcg_update() {
+    double c = -4 * kPi / h;
+    if (!pb) c /= eps_s;
+    launch_copy(stream, G, phi, phip);
+    launch_scale_copy(stream, G, c, q, y);
+    return cg_solve(dims, pbc(), y, phi, tol, n * n, r, p0, p1, &last_iters);
+}
+
+// Reference two-grid V-cycle, library/src/multigrid.c:99-168.
+void Sim::mg_vcycle(const double* in, double* out) {
+    launch_rbgs(stream, dims, in, out, 3);
+    launch_mg_residual(stream, dims, in, out, mg_r);
+    launch_restrict27(stream, n, mg_r, c_rhs);
+    launch_fill(stream, (long)nc * nc * nc, 0.0, c_eps);
+    cg_count_solve(cdims, c_rhs, c_eps, 1e-5, 50, c_r, c_p0, c_p1, &last_coarse);
+    launch_prolong_add(stream, nc, c_eps, out);
+    launch_rbgs(stream, dims, in, out, 3);
+}
+
+// library/src/grid_multigrid.c:110-212 (maze == false) and
+// library/src/verlet.c:86-149 (maze == true).
+int Sim::mg_update(bool maze) {
+    if (pb) fatal("maze: the Poisson-Boltzmann multigrid is not provided; use MAZE-LCG");
+    double* x;
+    if (maze) {
+        const double coef = (4 * kPi) / (h * eps_s);
+        launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma);
+        double* old_phip = phip;
+        phip = phi;
+        phi = tmp;
+        tmp = old_phip;
+        x = y;
+    } else {
+        launch_pred_sigma(stream, dims, pbc(), phi, phip, q, 0.0, tmp, sigma);
+        double* old_phip = phip;
+        phip = phi;
+        phi = tmp;
+        tmp = old_phip;
+        double c = -4 * kPi / h;
+        c /= eps_s;
+        launch_scale_copy(stream, G, c, q, sigma);
+        x = phi;
+    }
+    int res = -1, k = 0;
+    while (k < 1000) {
+        mg_vcycle(sigma, x);
+        launch_true_residual_max(stream, dims, PBCoef{}, x, sigma, st, red);
+        read_state();
+        k++;
+        if (st_host->resid <= tol) {
+            res = k;
+            break;
+        }
+    }
+    if (maze) launch_axpy(stream, G, -1.0, y, phi);
+    if (res == -1 && maze) std::fprintf(stderr, "Warning: Multigrid did not converge after 1000 iterations.\n");
+    return res;
+}
+
+// library/src/solver_md.c:91-107.
+int Sim::update_charges() {
+    launch_cic_base(stream, p);
+    launch_cic_sort(stream, p, key_bits);
+    launch_fill(stream, G, 0.0, q);
+    launch_cic_gather(stream, p, q);
+    launch_contrib_sum(stream, p, ps, red);
+    MAZE_HIP(hipMemcpyAsync(&ps_host->q_sum, &ps->q_sum, sizeof(double), hipMemcpyDeviceToHost,
+                            stream));
+    sync();
+    const double loc = ps_host->q_sum;
+    if (std::fabs(q_tot - loc) > 1e-6) {
+        std::printf("Charge conservation error: q_tot = %.6f, q_tot_loc = %.6f\n", q_tot, loc);
+        std::fflush(stdout);
+        if (strict_charge_check) std::exit(1);
+        return 1;
+    }
+    q_tot = loc;
+    return 0;
+}
+
+// library/src/grid_maze_lcg.c:100-124 and the matching init_field of the
+// other grid types: cold solve A phi = -4 pi q / (h eps) from zero.
+void Sim::init_field() {
+    double c = -4 * kPi / h;
+    if (!pb) c /= eps_s;
+    launch_fill(stream, G, 0.0, y);
+    launch_copy(stream, G, phi, phip);
+    launch_scale_copy(stream, G, c, q, sigma);
+    launch_fill(stream, G, 0.0, phi);
+    int pred = std::max(32, last_iters);
+    cg_solve(dims, pbc(), sigma, phi, tol, n * n, r, p0, p1, &pred);
+}
+
+int Sim::update_field() {
+    switch (grid_type) {
+        case kLCG:
+            return lcg_update();
+        case kMazeLCG:
+            return precision == kFP64 ? maze_fp64() : maze_mixed();
+        case kMG:
+            return mg_update(false);
+        case kMazeMG: {
+            const int res = mg_update(true);
+            if (precond != kPNone) fatal("Maze Multigrid with preconditioner not implemented yet.");
+            return res;
+        }
+    }
+    return -1;
+}
+
+void Sim::forces_elec() { launch_force_fd(stream, p, phi); }
+
+double Sim::forces_noel(bool read_back) {
+    launch_tf(stream, p, ps, red);
+    if (!read_back) return 0.0;
+    MAZE_HIP(hipMemcpyAsync(&ps_host->pe_tf, &ps->pe_tf, sizeof(double), hipMemcpyDeviceToHost,
+                            stream));
+    sync();
+    return ps_host->pe_tf;
+}
+
+void Sim::forces_tot() { launch_force_total(stream, p); }
+
+// library/src/integrator_verlet.c:12-71, integrator_ovrvo.c:45-138.
+void Sim::part1() {
+    if (itype == kVerlet) {
+        if (enabled) launch_kinetic(stream, p, ps, red, T, 1);
+        launch_verlet_part1(stream, p, dt, ps, enabled);
+    } else {
+        if (enabled) launch_o_block(stream, p, seed, calls++, std::sqrt(c1), (1 - c1) * kBoltz * T);
+        launch_kick(stream, p, 0.5 * dt * c2);
+        launch_drift(stream, p, c2 * dt);
+    }
+}
+
+void Sim::part2() {
+    if (itype == kVerlet) {
+        launch_kick(stream, p, 0.5 * dt);
+    } else {
+        launch_kick(stream, p, 0.5 * dt * c2);
+        if (enabled) launch_o_block(stream, p, seed, calls++, std::sqrt(c1), (1 - c1) * kBoltz * T);
+    }
+}
+
+// One MD step in the order of maze_poisson/solver.py:483-492.
+void Sim::md_step() {
+    part1();
+    update_charges();
+    update_field();
+    forces_elec();
+    forces_noel(false);
+    forces_tot();
+    part2();
+}
+
+void Sim::rescale_velocities() { launch_rescale_velocities(stream, p, ps); }
+
+double Sim::kinetic() {
+    launch_kinetic(stream, p, ps, red, 0.0, 0);
+    MAZE_HIP(hipMemcpyAsync(&ps_host->kinetic, &ps->kinetic, sizeof(double), hipMemcpyDeviceToHost,
+                            stream));
+    sync();
+    return ps_host->kinetic;
+}
+
+int Sim::check_thermostat() {
+    if (enabled) {
+        const double temp = 2 * kinetic() / (3 * p.np * kBoltz);
+        if (std::fabs(temp - T) < 100) {
+            enabled = 0;
+            if (itype == kOVRVO) { c1 = 1.0; c2 = 1.0; }
+            return 1;
+        }
+    }
+    return 0;
+}
+
+double Sim::energy_elec() {
+    launch_half_sum(stream, G, phi, dscalar, red);
+    MAZE_HIP(hipMemcpyAsync(hscalar, dscalar, sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    return hscalar[0];
+}
+
+void Sim::momentum(double* out) {
+    launch_momentum(stream, p, ps);
+    MAZE_HIP(hipMemcpyAsync(ps_host->mom, ps->mom, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    out[0] = ps_host->mom[0];
+    out[1] = ps_host->mom[1];
+    out[2] = ps_host->mom[2];
+}
+
+}  // namespace maze

@@ -1,0 +1,129 @@
+// sim.hpp -- one simulation replica resident on the GPU.
+#pragma once
+
+#include <vector>
+
+#include "grid.hpp"
+#include "particles.hpp"
+
+namespace maze {
+
+enum GridType { kLCG = 0, kFFT = 1, kMG = 2, kMazeLCG = 3, kMazeMG = 4 };
+enum PrecondType { kPNone = 0, kPJacobi = 1, kPMG = 2, kPSSOR = 3, kPBlockJacobi = 4 };
+enum ItgType { kOVRVO = 0, kVerlet = 1 };
+// Arithmetic of the field solve: the reference FP64 CG, or FP32-storage CG
+// inside FP64 iterative refinement.
+enum Precision { kFP64 = 0, kMixedCG = 1 };
+
+// Per-kernel timing collected with HIP events on the solver stream.
+struct KernelTimer {
+    const char* name;
+    double total_ms = 0.0;
+    long count = 0;
+    double bytes_per_launch = 0.0;
+};
+
+struct Sim {
+    hipStream_t stream = nullptr;
+
+    // ---- grid ----
+    int n = 0;
+    double L = 0, h = 0, tol = 0, eps_s = 1, eps_int = 1;
+    int grid_type = kMazeLCG, precond = kPNone;
+    int precision = kFP64;
+    double inner_rtol = 1e-4;
+    int pb = 0;
+    double w = 0, kbar2 = 0;
+    long G = 0;
+    GridDims dims{};
+    double *q = nullptr, *y = nullptr, *phi = nullptr, *phip = nullptr, *tmp = nullptr;
+    double *sigma = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *y_alt = nullptr;
+    double *ex = nullptr, *ey = nullptr, *ez = nullptr, *k2 = nullptr;
+    float *b32 = nullptr, *d32 = nullptr, *r32 = nullptr, *q32a = nullptr, *q32b = nullptr;
+    // reference multigrid coarse level
+    int nc = 0;
+    GridDims cdims{};
+    double *mg_r = nullptr, *c_rhs = nullptr, *c_eps = nullptr, *c_r = nullptr, *c_p0 = nullptr,
+           *c_p1 = nullptr;
+
+    CGState* st = nullptr;      // device
+    CGState* st_host = nullptr; // pinned
+    Reduce red{};
+    double* dscalar = nullptr;  // device scratch scalar
+    double* hscalar = nullptr;  // pinned
+
+    int last_iters = 32;        // prediction for the chunked convergence polling
+    int last_inner = 32;
+    int last_coarse = 16;
+    int last_res = 0;
+
+    // ---- particles ----
+    PDev p{};
+    PScalars* ps = nullptr;      // device
+    PScalars* ps_host = nullptr; // pinned
+    int key_bits = 1;
+    double q_tot = 0.0;          // library/src/solver_md.c:16
+    int strict_charge_check = 1;
+
+    // ---- integrator ----
+    int itype = kVerlet, enabled = 0;
+    double dt = 0, T = 0, c1 = 1, c2 = 1;
+    uint64_t seed = 42, calls = 0;
+
+    // ---- profiling ----
+    int prof = 0;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+    struct Pending { int timer; hipEvent_t a, b; int iter; };
+    std::vector<Pending> pending;
+    KernelTimer timers[4] = {{"cg_dir"}, {"cg_update"}, {"cg_dir_f32"}, {"cg_update_f32"}};
+
+    Sim();
+    ~Sim();
+
+    void init_grid(int n, double L, double h, double tol, double eps, double eps_int, int type,
+                   int precond);
+    void init_pb(double w, double kbar2);
+    void set_eps_k2(const double* ex, const double* ey, const double* ez, const double* k2);
+    void init_particles(int n, int n_typ, double L, double h, int np, const int* types,
+                        const double* pos, const double* vel, const double* mass,
+                        const double* charges, const double* pot_params);
+    void init_integrator(double dt, double T, double gamma, int type, int enabled);
+
+    int update_charges();
+    void init_field();
+    int update_field();
+    void forces_elec();
+    double forces_noel(bool read_back = true);
+    void forces_tot();
+    void part1();
+    void part2();
+    void md_step();
+    void rescale_velocities();
+    int check_thermostat();
+    double kinetic();
+    double energy_elec();
+    void momentum(double* out);
+
+    // solvers
+    int cg_solve(const GridDims& d, const PBCoef& pbc, const double* b, double* x, double tol,
+                 int limit, double* rbuf, double* pa, double* pbuf, int* pred);
+    int cg_count_solve(const GridDims& d, const double* b, double* x, double rtol, int maxit,
+                       double* rbuf, double* pa, double* pbuf, int* pred);
+    int cg_inner_f32(const float* b, float* x, double rtol, int limit);
+    int maze_fp64();
+    int maze_mixed();
+    int lcg_update();
+    int mg_update(bool maze);
+    void mg_vcycle(const double* in, double* out);
+    PBCoef pbc() const;
+
+    void sync() { MAZE_HIP(hipStreamSynchronize(stream)); }
+    void read_state();
+    hipEvent_t next_event();
+    void prof_flush(int iters_done, int timer_base);
+    void free_grid();
+    void free_particles();
+};
+
+}  // namespace maze

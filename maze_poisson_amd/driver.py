@@ -1,0 +1,97 @@
+"""MD driver over the MaZe C-ABI.
+
+Makes the reference driver's call sequence (``maze_poisson/solver.py:97-510``): the
+initialisation (``initialize`` :97-108), the two bootstrap steps with cold field
+solves (``initialize_md`` :371-409), and the MD loop (``md_loop_iter`` :483-492),
+with electrostatic and non-electrostatic forces enabled.  It works unchanged on any
+library exporting the ABI (:class:`maze_poisson_amd.capi.MazeLib`).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .capi import MazeLib
+from .workloads import System, dt_au
+
+
+@dataclass
+class RunConfig:
+    method: str = "MAZE-LCG"
+    precond: str = "NONE"
+    tol: float = 1e-7
+    eps_s: float = 1.0
+    eps_int: float = 1.0
+    dt_fs: float = 0.25
+    T: float = 1500.0
+    gamma: float = 1e-3
+    integrator: str = "VERLET"
+    thermostat: bool = False
+
+
+def initialize(lib: MazeLib, s: System, cfg: RunConfig) -> None:
+    """``maze_poisson/solver.py:97-106,129-144,276-346,358-369``"""
+    maps = lib.type_maps()
+    lib.solver_initialize()
+    lib.solver_initialize_grid(s.n_grid, s.L, s.h, cfg.tol, cfg.eps_s, cfg.eps_int,
+                               maps["grid"][cfg.method.upper()],
+                               maps["precond"][cfg.precond.upper()])
+    lib.solver_initialize_particles(s.n_grid, s.n_typ, s.L, s.h, s.n_p,
+                                    maps["potential"]["TF"], maps["cas"]["CIC"],
+                                    s.types, s.pos, s.vel, s.mass, s.charges, s.tf)
+    lib.solver_initialize_integrator(s.n_p, dt_au(cfg.dt_fs), cfg.T, cfg.gamma,
+                                     maps["integrator"][cfg.integrator.upper()],
+                                     1 if cfg.thermostat else 0)
+
+
+def compute_forces(lib: MazeLib) -> float:
+    """``maze_poisson/solver.py:431-441``; returns the TF potential energy."""
+    lib.solver_compute_forces_elec()
+    v = lib.solver_compute_forces_noel()
+    lib.solver_compute_forces_tot()
+    return v
+
+
+def bootstrap(lib: MazeLib) -> None:
+    """The two Verlet start-up steps with cold solves, ``maze_poisson/solver.py:376-398``."""
+    lib.solver_update_charges()
+    lib.solver_init_field()
+    compute_forces(lib)
+    lib.integrator_part_1()
+    lib.solver_update_charges()
+    lib.solver_init_field()
+    compute_forces(lib)
+    lib.integrator_part_2()
+
+
+def md_step(lib: MazeLib) -> tuple[int, float]:
+    """One MD step, ``maze_poisson/solver.py:483-492``; returns (CG result, V_TF)."""
+    lib.integrator_part_1()
+    lib.solver_update_charges()
+    it = lib.solver_update_field()
+    v = compute_forces(lib)
+    lib.integrator_part_2()
+    return it, v
+
+
+def snapshot(lib: MazeLib, s: System, fields: bool = True) -> dict:
+    """Copy the observable state out through the getters."""
+    n, N = s.n_grid, s.n_p
+    out = {}
+    for name in ("pos", "vel", "fcs_elec", "fcs_noel", "fcs_tot"):
+        a = np.zeros((N, 3))
+        getattr(lib, "get_" + name)(a)
+        out[name] = a
+    if fields:
+        for name in ("field", "field_prev", "q"):
+            a = np.zeros((n, n, n))
+            getattr(lib, "get_" + name)(a)
+            out[name] = a
+    out["kinetic"] = lib.get_kinetic_energy()
+    out["temperature"] = lib.get_temperature()
+    out["energy_elec"] = lib.get_energy_elec()
+    m = np.zeros(3)
+    lib.get_momentum(m)
+    out["momentum"] = m
+    return out

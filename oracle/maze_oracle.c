@@ -1,0 +1,578 @@
+/*
+ * maze_oracle.c -- CPU oracle (test infrastructure, FP64, single thread).
+ *
+ * Restates the reference hot path (Crivella/MaZe_Poisson library/src) with
+ * the same floating-point operation order so that it can be pinned bit for
+ * bit against the reference C.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product path never does.
+ *
+ * Citations are path:line in the reference tree.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "maze_oracle.h"
+
+#define OR_PI 3.14159265358979323846
+
+static inline long wrapi(long i, long n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+/* ------------------------------------------------------------------------ */
+/* 7-point periodic Laplacian, unscaled: sum of the 6 neighbours - 6 u.
+ * library/src/laplace.c:24-75.  Summation order: i+1, i-1, j+1, j-1, k+1,
+ * k-1, then minus 6u (laplace.c:301-309). */
+void or_laplace(const double *u, double *out, int n) {
+    const long N = n, N2 = N * N;
+    for (long i = 0; i < N; i++) {
+        const long ip = wrapi(i + 1, N) * N2, im = wrapi(i - 1, N) * N2, i0 = i * N2;
+        for (long j = 0; j < N; j++) {
+            const long jp = wrapi(j + 1, N) * N, jm = wrapi(j - 1, N) * N, j0 = j * N;
+            for (long k = 0; k < N; k++) {
+                const long kp = wrapi(k + 1, N), km = wrapi(k - 1, N);
+                out[i0 + j0 + k] = u[ip + j0 + k] + u[im + j0 + k] + u[i0 + jp + k] +
+                                   u[i0 + jm + k] + u[i0 + j0 + kp] + u[i0 + j0 + km] -
+                                   u[i0 + j0 + k] * 6.0;
+            }
+        }
+    }
+}
+
+/* Variable-coefficient (linear Poisson-Boltzmann) operator.  eps_x[c] is the
+ * dielectric value on the face between c and its +x neighbour.
+ * library/src/laplace.c:77-145 (term order :366-379). */
+void or_laplace_pb(const double *u, double *out, int n, const double *ex,
+                   const double *ey, const double *ez, const double *k2) {
+    const long N = n, N2 = N * N;
+    for (long i = 0; i < N; i++) {
+        const long ip = wrapi(i + 1, N) * N2, im = wrapi(i - 1, N) * N2, i0 = i * N2;
+        for (long j = 0; j < N; j++) {
+            const long jp = wrapi(j + 1, N) * N, jm = wrapi(j - 1, N) * N, j0 = j * N;
+            for (long k = 0; k < N; k++) {
+                const long kp = wrapi(k + 1, N), km = wrapi(k - 1, N);
+                const long c = i0 + j0 + k, cx = im + j0 + k, cy = i0 + jm + k, cz = i0 + j0 + km;
+                out[c] = u[ip + j0 + k] * ex[c] + u[cx] * ex[cx] + u[i0 + jp + k] * ey[c] +
+                         u[cy] * ey[cy] + u[i0 + j0 + kp] * ez[c] + u[cz] * ez[cz] -
+                         u[c] * (ex[c] + ex[cx] + ey[c] + ey[cy] + ez[c] + ez[cz] + k2[c]);
+            }
+        }
+    }
+}
+
+/* ---- BLAS-1 with the reference's serial order (library/src/linalg.c:105-212) */
+static double dot(const double *a, const double *b, long n) {
+    double s = 0.0;
+    for (long i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+static void axpy(const double *v, double *u, double alpha, long n) { /* u += alpha v */
+    for (long i = 0; i < n; i++) u[i] += alpha * v[i];
+}
+static void scal(double *x, double alpha, long n) {
+    for (long i = 0; i < n; i++) x[i] *= alpha;
+}
+static double amax(const double *u, long n) {
+    double m = 0.0;
+    for (long i = 0; i < n; i++) {
+        double a = fabs(u[i]);
+        if (a > m) m = a;
+    }
+    return m;
+}
+
+typedef void (*op_fn)(const double *, double *, int, const void *);
+struct pb_coef { const double *ex, *ey, *ez, *k2; };
+static void op_lap(const double *u, double *o, int n, const void *c) { (void)c; or_laplace(u, o, n); }
+static void op_pb(const double *u, double *o, int n, const void *c) {
+    const struct pb_coef *p = (const struct pb_coef *)c;
+    or_laplace_pb(u, o, n, p->ex, p->ey, p->ez, p->k2);
+}
+
+/* CG with the implicit Jacobi preconditioner P = -6 I, so v = -r/6 and
+ * p = -v = r/6.  x holds the initial guess on entry (the reference's
+ * in-place form x == x0).  Stops when max|r| <= tol, checked after each
+ * update; returns the 0-based iteration index, -1 after n*n iterations.
+ * library/src/plcg.c:28-90 (Poisson) and :169-238 (PB, same recurrence). */
+static int cg_implicit(op_fn op, const void *ctx, const double *b, double *x, double tol, int n) {
+    const long G = (long)n * n * n, limit = (long)n * n;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *Ap = (double *)malloc(G * sizeof(double));
+    double *p = (double *)malloc(G * sizeof(double));
+    int res = -1;
+    op(x, r, n, ctx);               /* r = A x */
+    axpy(b, r, -1.0, G);            /* r = A x - b */
+    memcpy(p, r, G * sizeof(double));
+    scal(p, 1.0 / 6.0, G);          /* p = r / 6 */
+    double r_dot_v = -dot(r, p, G);
+    for (long it = 0; it < limit; it++) {
+        op(p, Ap, n, ctx);
+        double alpha = r_dot_v / dot(p, Ap, G);
+        axpy(p, x, alpha, G);
+        axpy(Ap, r, alpha, G);
+        double rr = dot(r, r, G);
+        if (amax(r, G) <= tol) { res = (int)it; break; }
+        double rn_dot_vn = -rr / 6.0;
+        double beta = rn_dot_vn / r_dot_v;
+        r_dot_v = rn_dot_vn;
+        scal(p, beta, G);
+        axpy(r, p, 1.0 / 6.0, G);
+    }
+    free(r); free(Ap); free(p);
+    return res;
+}
+
+int or_conj_grad(const double *b, double *x, double tol, int n) {
+    return cg_implicit(op_lap, NULL, b, x, tol, n);
+}
+
+int or_conj_grad_pb(const double *b, double *x, double tol, int n, const double *ex,
+                    const double *ey, const double *ez, const double *k2) {
+    struct pb_coef c = {ex, ey, ez, k2};
+    return cg_implicit(op_pb, &c, b, x, tol, n);
+}
+
+/* CG with an explicit Jacobi preconditioner v = -r/6 (precond_jacobi.c:8-14),
+ * library/src/plcg.c:104-166. */
+int or_conj_grad_jacobi(const double *b, double *x, double tol, int n) {
+    const long G = (long)n * n * n, limit = (long)n * n;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *Ap = (double *)malloc(G * sizeof(double));
+    double *v = (double *)malloc(G * sizeof(double));
+    double *p = (double *)malloc(G * sizeof(double));
+    int res = -1;
+    or_laplace(x, r, n);
+    axpy(b, r, -1.0, G);
+    for (long i = 0; i < G; i++) v[i] = -r[i] / 6.0;
+    memcpy(p, v, G * sizeof(double));
+    scal(p, -1.0, G);
+    double r_dot_v = dot(r, v, G);
+    for (long it = 0; it < limit; it++) {
+        or_laplace(p, Ap, n);
+        double alpha = r_dot_v / dot(p, Ap, G);
+        axpy(p, x, alpha, G);
+        axpy(Ap, r, alpha, G);
+        (void)dot(r, r, G);
+        if (amax(r, G) <= tol) { res = (int)it; break; }
+        for (long i = 0; i < G; i++) v[i] = -r[i] / 6.0;
+        double rn_dot_vn = dot(r, v, G);
+        double beta = rn_dot_vn / r_dot_v;
+        r_dot_v = rn_dot_vn;
+        scal(p, beta, G);
+        axpy(v, p, -1.0, G);
+    }
+    free(r); free(Ap); free(v); free(p);
+    return res;
+}
+
+/* MaZe step: phi* = 2 phi - phi_prev, phi_prev <- phi; sigma = A phi* +
+ * (4 pi / h) q; solve A y = sigma warm-started from y; phi = phi* - y.
+ * library/src/verlet.c:31-71 (h already multiplied by eps_s by the caller,
+ * library/src/grid_maze_lcg.c:158-164). */
+int or_verlet_poisson(double tol, double h, double *phi, double *phi_prev, const double *q,
+                      double *y, int n, int jacobi_explicit) {
+    const long G = (long)n * n * n;
+    double *sigma = (double *)malloc(G * sizeof(double));
+    for (long i = 0; i < G; i++) {
+        double a = phi[i];
+        phi[i] = 2 * a - phi_prev[i];
+        phi_prev[i] = a;
+    }
+    or_laplace(phi, sigma, n);
+    axpy(q, sigma, (4 * OR_PI) / h, G);
+    int it = jacobi_explicit ? or_conj_grad_jacobi(sigma, y, tol, n) : or_conj_grad(sigma, y, tol, n);
+    axpy(y, phi, -1.0, G);
+    free(sigma);
+    return it;
+}
+
+/* library/src/verlet.c:152-194 */
+int or_verlet_poisson_pb(double tol, double h, double *phi, double *phi_prev, const double *q,
+                         double *y, int n, const double *ex, const double *ey, const double *ez,
+                         const double *k2) {
+    const long G = (long)n * n * n;
+    double *sigma = (double *)malloc(G * sizeof(double));
+    for (long i = 0; i < G; i++) {
+        double a = phi[i];
+        phi[i] = 2 * a - phi_prev[i];
+        phi_prev[i] = a;
+    }
+    or_laplace_pb(phi, sigma, n, ex, ey, ez, k2);
+    axpy(q, sigma, (4 * OR_PI) / h, G);
+    int it = or_conj_grad_pb(sigma, y, tol, n, ex, ey, ez, k2);
+    axpy(y, phi, -1.0, G);
+    free(sigma);
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference multigrid (library/src/multigrid.c), serial ranks, n even.     */
+
+/* Red-black Gauss-Seidel, `iters` red+black pairs; red = (i+j+k) even.
+ * library/src/multigrid.c:636-729.  Neighbour order i-1, i+1, j-1, j+1,
+ * k-1, k+1 (:686-689). */
+void or_mg_smooth_rbgs(const double *in, double *out, int n, int iters) {
+    const long N = n, N2 = N * N;
+    const double inv_diag = -1.0 / 6.0;
+    for (int it = 0; it < iters; it++) {
+        for (int color = 0; color < 2; color++) {
+            for (long i = 0; i < N; i++) {
+                const long i0 = i * N2, ip = wrapi(i + 1, N) * N2, im = wrapi(i - 1, N) * N2;
+                for (long j = 0; j < N; j++) {
+                    const long j0 = j * N, jp = wrapi(j + 1, N) * N, jm = wrapi(j - 1, N) * N;
+                    for (long k = (i + j + color) & 1; k < N; k += 2) {
+                        const long kp = wrapi(k + 1, N), km = wrapi(k - 1, N);
+                        const long c = i0 + j0 + k;
+                        double s = out[im + j0 + k] + out[ip + j0 + k] + out[i0 + jm + k] +
+                                   out[i0 + jp + k] + out[i0 + j0 + km] + out[i0 + j0 + kp];
+                        out[c] = (in[c] - s) * inv_diag;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* 27-point nodal full weighting onto the (n/2)^3 grid, coarse (I,J,K) at
+ * fine (2I,2J,2K); weights 8/4/2/1 over 64.  library/src/multigrid.c:523-594
+ * (sum order :571-590). */
+void or_mg_restrict27(const double *in, double *out, int n) {
+    const long N = n, N2 = N * N, M = N / 2;
+#define F(a, b, c) in[(a) * N2 + (b) * N + (c)]
+    for (long I = 0; I < M; I++) {
+        const long i = 2 * I, ip = wrapi(i + 1, N), im = wrapi(i - 1, N);
+        for (long J = 0; J < M; J++) {
+            const long j = 2 * J, jp = wrapi(j + 1, N), jm = wrapi(j - 1, N);
+            for (long K = 0; K < M; K++) {
+                const long k = 2 * K, kp = wrapi(k + 1, N), km = wrapi(k - 1, N);
+                double f = F(im, j, k) + F(ip, j, k) + F(i, jm, k) + F(i, jp, k) + F(i, j, km) + F(i, j, kp);
+                double e = F(ip, j, km) + F(ip, j, kp) + F(im, j, km) + F(im, j, kp) + F(ip, jm, k) +
+                           F(ip, jp, k) + F(im, jm, k) + F(im, jp, k) + F(i, jm, km) + F(i, jm, kp) +
+                           F(i, jp, km) + F(i, jp, kp);
+                double c = F(ip, jp, km) + F(ip, jp, kp) + F(ip, jm, km) + F(ip, jm, kp) + F(im, jp, km) +
+                           F(im, jp, kp) + F(im, jm, km) + F(im, jm, kp);
+                out[(I * M + J) * M + K] = (8.0 * F(i, j, k) + 4.0 * f + 2.0 * e + c) * (1.0 / 64.0);
+            }
+        }
+    }
+#undef F
+}
+
+/* Trilinear prolongation from an nc^3 grid to a (2 nc)^3 grid (overwrites
+ * out).  library/src/multigrid.c:393-463 (term order :439-447). */
+void or_mg_prolong_trilinear(const double *in, double *out, int nc) {
+    const long M = nc, M2 = M * M, N = 2 * M;
+    for (long i = 0; i < N; i++) {
+        const long a0 = (i / 2) % M, a1 = (a0 + 1) % M;
+        const double wi0 = (i & 1) ? 0.5 : 1.0, wi1 = (i & 1) ? 0.5 : 0.0;
+        for (long j = 0; j < N; j++) {
+            const long b0 = (j / 2) % M, b1 = (b0 + 1) % M;
+            const double wj0 = (j & 1) ? 0.5 : 1.0, wj1 = (j & 1) ? 0.5 : 0.0;
+            for (long k = 0; k < N; k++) {
+                const long c0 = (k / 2) % M, c1 = (c0 + 1) % M;
+                const double wk0 = (k & 1) ? 0.5 : 1.0, wk1 = (k & 1) ? 0.5 : 0.0;
+#define C(a, b, c) in[(a) * M2 + (b) * M + (c)]
+                out[(i * N + j) * N + k] =
+                    C(a0, b0, c0) * wi0 * wj0 * wk0 + C(a1, b0, c0) * wi1 * wj0 * wk0 +
+                    C(a0, b1, c0) * wi0 * wj1 * wk0 + C(a1, b1, c0) * wi1 * wj1 * wk0 +
+                    C(a0, b0, c1) * wi0 * wj0 * wk1 + C(a1, b0, c1) * wi1 * wj0 * wk1 +
+                    C(a0, b1, c1) * wi0 * wj1 * wk1 + C(a1, b1, c1) * wi1 * wj1 * wk1;
+#undef C
+            }
+        }
+    }
+}
+
+/* Coarse-grid CG with a relative infinity-norm stop and a breakdown guard.
+ * Returns the number of iterations performed.  library/src/multigrid.c:13-96 */
+int or_mg_cg_coarse(const double *b, double *x, int n, int maxit, double rtol) {
+    const long G = (long)n * n * n;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *p = (double *)malloc(G * sizeof(double));
+    double *Ap = (double *)malloc(G * sizeof(double));
+    int k;
+    or_laplace(x, r, n);
+    axpy(b, r, -1.0, G);
+    memcpy(p, r, G * sizeof(double));
+    scal(p, 1.0 / 6.0, G);
+    double r0 = amax(r, G);
+    if (r0 == 0.0) { free(r); free(p); free(Ap); return 0; }
+    double r_dot_v = -dot(r, p, G);
+    for (k = 0; k < maxit; ++k) {
+        or_laplace(p, Ap, n);
+        double denom = dot(p, Ap, G);
+        if (fabs(denom) < 1e-300) break;
+        double alpha = r_dot_v / denom;
+        axpy(p, x, alpha, G);
+        axpy(Ap, r, alpha, G);
+        if (amax(r, G) <= rtol * r0) { ++k; break; }
+        double rn = dot(r, r, G);
+        double rn_dot_vn = -rn / 6.0;
+        double beta = rn_dot_vn / r_dot_v;
+        r_dot_v = rn_dot_vn;
+        scal(p, beta, G);
+        axpy(r, p, 1.0 / 6.0, G);
+    }
+    free(r); free(p); free(Ap);
+    return k;
+}
+
+/* One reference V-cycle: the recursion stops at depth 1, so it is a
+ * two-grid cycle with `sm` RBGS pre/post sweeps and a coarse CG (<= 50
+ * iterations, rtol 1e-5) on the unscaled coarse stencil.
+ * library/src/multigrid.c:99-168, selectors :743-765. */
+void or_mg_vcycle(const double *in, double *out, int n, int sm) {
+    const long G = (long)n * n * n;
+    const int nc = n / 2;
+    const long Gc = (long)nc * nc * nc;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *rhs = (double *)malloc(Gc * sizeof(double));
+    double *eps = (double *)calloc(Gc, sizeof(double));
+    or_mg_smooth_rbgs(in, out, n, sm);
+    or_laplace(out, r, n);
+    scal(r, -1.0, G);
+    axpy(in, r, 1.0, G);
+    or_mg_restrict27(r, rhs, n);
+    or_mg_cg_coarse(rhs, eps, nc, 50, 1e-5);
+    or_mg_prolong_trilinear(eps, r, nc);
+    axpy(r, out, 1.0, G);
+    or_mg_smooth_rbgs(in, out, n, sm);
+    free(r); free(rhs); free(eps);
+}
+
+/* MaZe step solved with repeated V-cycles, stop on the true residual
+ * max|A y - sigma| <= tol; returns the 1-based cycle count or -1 after 1000.
+ * library/src/verlet.c:86-149. */
+int or_verlet_poisson_mg(double tol, double h, double *phi, double *phi_prev, const double *q,
+                         double *y, int n) {
+    const long G = (long)n * n * n;
+    double *tmp = (double *)malloc(G * sizeof(double));
+    double *sigma = (double *)malloc(G * sizeof(double));
+    int res = -1, it = 0;
+    for (long i = 0; i < G; i++) {
+        double a = phi[i];
+        phi[i] = 2 * a - phi_prev[i];
+        phi_prev[i] = a;
+    }
+    or_laplace(phi, sigma, n);
+    axpy(q, sigma, (4 * OR_PI) / h, G);
+    while (it < 1000) {
+        or_mg_vcycle(sigma, y, n, 3);
+        or_laplace(y, tmp, n);
+        axpy(sigma, tmp, -1.0, G);
+        double a = amax(tmp, G);
+        it++;
+        if (a <= tol) { res = it; break; }
+    }
+    axpy(y, phi, -1.0, G);
+    free(tmp); free(sigma);
+    return res;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Particles                                                                */
+
+/* CIC weight with the periodic distance, Im et al. (1998) eq. 24.
+ * library/src/charges.c:8-14 */
+double or_cic_weight(double x, double L, double h) {
+    x = fabs(x - round(x / L) * L);
+    if (x >= h) return 0.0;
+    return 1.0 - x / h;
+}
+
+/* Base cell floor(pos/h) and the +1 corners, stored (n_p, 8, 3) in the
+ * reference's corner order 000,100,010,001,110,101,011,111.
+ * library/src/particles.c:297-346.  The base index is wrapped into [0,n):
+ * the reference leaves it unwrapped (:315-317), which only matters for a
+ * position that rounds to pos/h >= n, where the reference would index past
+ * the grid. */
+void or_cic_neighbors(const double *pos, int np, int n, double h, int64_t *nb) {
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                  {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    for (int i = 0; i < np; i++) {
+        int base[3];
+        for (int d = 0; d < 3; d++) {
+            int c = (int)floor(pos[3 * i + d] / h);
+            c %= n;
+            if (c < 0) c += n;
+            base[d] = c;
+        }
+        for (int c = 0; c < 8; c++)
+            for (int d = 0; d < 3; d++) nb[(i * 8 + c) * 3 + d] = (base[d] + off[c][d]) % n;
+    }
+}
+
+/* Scatter q_ion * wx * wy * wz to the 8 corners, serial ion order; returns
+ * the sum of all contributions.  The period used inside the weight is n*h
+ * (library/src/charges.c:100-144, :113). */
+double or_spread_cic(int n, int np, double h, const double *pos, const int64_t *nb,
+                     const double *charges, double *q) {
+    const long N = n, N2 = N * N, G = N2 * N;
+    const double L = n * h;
+    double q_tot = 0.0;
+    for (long i = 0; i < G; i++) q[i] = 0.0;
+    for (long i = 0; i < np; i++) {
+        const double chg = charges[i], px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        for (int c = 0; c < 8; c++) {
+            const int64_t *t = nb + (i * 8 + c) * 3;
+            double app = or_cic_weight(px - t[0] * h, L, h) * or_cic_weight(py - t[1] * h, L, h) *
+                         or_cic_weight(pz - t[2] * h, L, h);
+            double upd = chg * app;
+            q_tot += upd;
+            q[t[0] * N2 + t[1] * N + t[2]] += upd;
+        }
+    }
+    return q_tot;
+}
+
+/* Field force by CIC-weighted central differences at the 8 corners;
+ * returns the sum of the weighted charges.  library/src/forces.c:26-100. */
+double or_force_fd(int n, int np, double h, const double *phi, const int64_t *nb,
+                   const double *charges, const double *pos, double *forces) {
+    const long N = n, N2 = N * N;
+    const double h2 = 2.0 * h, L = N * h;
+    double sum_q = 0.0;
+    for (long ip = 0; ip < np; ip++) {
+        double *f = forces + 3 * ip;
+        const double px = pos[3 * ip], py = pos[3 * ip + 1], pz = pos[3 * ip + 2], chg = charges[ip];
+        f[0] = f[1] = f[2] = 0.0;
+        for (int c = 0; c < 8; c++) {
+            const int64_t *t = nb + (ip * 8 + c) * 3;
+            const long i = t[0], j = t[1], k = t[2];
+            double qc = chg * or_cic_weight(px - i * h, L, h) * or_cic_weight(py - j * h, L, h) *
+                        or_cic_weight(pz - k * h, L, h);
+            sum_q += qc;
+            double E = (phi[wrapi(i - 1, N) * N2 + j * N + k] - phi[wrapi(i + 1, N) * N2 + j * N + k]) / h2;
+            f[0] += qc * E;
+            E = (phi[i * N2 + wrapi(j - 1, N) * N + k] - phi[i * N2 + wrapi(j + 1, N) * N + k]) / h2;
+            f[1] += qc * E;
+            E = (phi[i * N2 + j * N + wrapi(k - 1, N)] - phi[i * N2 + j * N + wrapi(k + 1, N)]) / h2;
+            f[2] += qc * E;
+        }
+    }
+    return sum_q;
+}
+
+/* Shifted-force Tosi-Fumi constants per type pair: table[(t1*n_typ+t2)*7 +
+ * {A,B,C,D,sigma,alpha,beta}], alpha = -V'(rc), beta = -V(rc) - alpha rc.
+ * library/src/particles.c:181-228 (expressions :191-215). */
+void or_tf_pair_table(int n_typ, const double *pp, double r_cut, double *table) {
+    double rc6 = pow(r_cut, 6), rc7 = rc6 * r_cut, rc8 = rc7 * r_cut, rc9 = rc8 * r_cut;
+    for (int t = 0; t < n_typ * n_typ; t++) {
+        const double A = pp[t * 5], B = pp[t * 5 + 1], C = pp[t * 5 + 2], D = pp[t * 5 + 3],
+                     s = pp[t * 5 + 4];
+        double v_shift = A * exp(B * (s - r_cut)) - C / rc6 - D / rc8;
+        double alpha = A * B * exp(B * (s - r_cut)) - 6 * C / rc7 - 8 * D / rc9;
+        double beta = -v_shift - alpha * r_cut;
+        double *o = table + t * 7;
+        o[0] = A; o[1] = B; o[2] = C; o[3] = D; o[4] = s; o[5] = alpha; o[6] = beta;
+    }
+}
+
+/* Minimum-image separation, accumulated x, y, z as in forces.c:145-157. */
+static inline double min_image_r(const double *a, const double *b, double L, double d[3]) {
+    double app = a[0] - b[0];
+    app -= L * round(app / L);
+    double r2 = app * app;
+    d[0] = app;
+    app = a[1] - b[1];
+    app -= L * round(app / L);
+    d[1] = app;
+    r2 += app * app;
+    app = a[2] - b[2];
+    app -= L * round(app / L);
+    d[2] = app;
+    r2 += app * app;
+    return sqrt(r2);
+}
+
+/* All-pairs shifted-force Tosi-Fumi, r <= r_cut, full j loop per i; returns
+ * sum V / 2.  library/src/forces.c:112-186. */
+double or_tf_forces(int np, double L, const double *pos, const int *types, int n_typ,
+                    const double *table, double r_cut, double *forces) {
+    double pe = 0.0;
+    for (int i = 0; i < np; i++) {
+        double *f = forces + 3 * i;
+        f[0] = f[1] = f[2] = 0.0;
+        for (int j = 0; j < np; j++) {
+            if (i == j) continue;
+            double d[3];
+            double r = min_image_r(pos + 3 * i, pos + 3 * j, L, d);
+            if (r > r_cut) continue;
+            d[0] /= r; d[1] /= r; d[2] /= r;
+            const double *t = table + (types[i] * n_typ + types[j]) * 7;
+            const double a = t[0], b = t[1], c = t[2], dd = t[3], s = t[4], al = t[5], be = t[6];
+            double fm = b * a * exp(b * (s - r)) - 6 * c / pow(r, 7) - 8 * dd / pow(r, 9) - al;
+            double vm = a * exp(b * (s - r)) - c / pow(r, 6) - dd / pow(r, 8) + al * r + be;
+            f[0] += fm * d[0];
+            f[1] += fm * d[1];
+            f[2] += fm * d[2];
+            pe += vm;
+        }
+    }
+    return pe / 2;
+}
+
+/* Number of j != i with r <= r_cut under the same predicate as
+ * or_tf_forces (the pair list the reference evaluates, forces.c:158). */
+void or_tf_neighbor_counts(int np, double L, const double *pos, double r_cut, int *counts) {
+    for (int i = 0; i < np; i++) {
+        int c = 0;
+        for (int j = 0; j < np; j++) {
+            if (i == j) continue;
+            double d[3];
+            if (min_image_r(pos + 3 * i, pos + 3 * j, L, d) <= r_cut) c++;
+        }
+        counts[i] = c;
+    }
+}
+
+/* K = 1/2 sum m |v|^2, serial order.  library/src/particles.c:802-820 */
+double or_kinetic(int np, const double *vel, const double *mass) {
+    double kin = 0.0;
+    for (int i = 0; i < np; i++) {
+        double app = 0.0;
+        for (int d = 0; d < 3; d++) app += pow(vel[3 * i + d], 2);
+        kin += mass[i] * app;
+    }
+    return 0.5 * kin;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and Box-Muller.  The
+ * reference draws its Langevin noise from rand() with a racy static cache
+ * (library/src/integrator_ovrvo.c:12-36), which no two runs reproduce; this
+ * build replaces it by a counter-based stream keyed by (seed, call, ion) so
+ * that CPU and GPU draw the same deviates. */
+void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Three standard normal deviates for (seed, call, ion).  Uniforms are
+ * (x + 0.5) 2^-32 in (0,1); pairs (u0,u1) and (u2,u3) go through
+ * Box-Muller; the fourth deviate is unused. */
+void or_normal3(uint64_t seed, uint64_t call, uint32_t ion, double out[3]) {
+    uint32_t ctr[4] = {ion, (uint32_t)call, (uint32_t)(call >> 32), 0x4D415A45u};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    or_philox4x32(ctr, key, o);
+    const double s = 2.3283064365386963e-10; /* 2^-32 */
+    double u0 = ((double)o[0] + 0.5) * s, u1 = ((double)o[1] + 0.5) * s;
+    double u2 = ((double)o[2] + 0.5) * s, u3 = ((double)o[3] + 0.5) * s;
+    double ra = sqrt(-2.0 * log(u0)), rb = sqrt(-2.0 * log(u2));
+    double ta = 6.283185307179586 * u1, tb = 6.283185307179586 * u3;
+    out[0] = ra * cos(ta);
+    out[1] = ra * sin(ta);
+    out[2] = rb * cos(tb);
+}

@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Writes the golden vectors in this directory from the reference C library.
+
+The reference (Crivella/MaZe_Poisson, library/src) is compiled from its own sources by
+``make -C oracle ref`` (strict IEEE flags) into oracle/_ref/libmaze_ref.so; this script
+drives it through its C-ABI and its exported kernel functions on seeded inputs and
+stores inputs and outputs as .npz fixtures.  Run with OMP_NUM_THREADS=1: the reference's
+charge scatter races under OpenMP (library/src/charges.c:123-140).
+
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+"""
+import ctypes
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from maze_poisson_amd import driver, workloads as W  # noqa: E402
+from maze_poisson_amd.capi import MazeLib  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "libmaze_ref.so")
+REF_PY = "/root/reference/maze_poisson/c_api"
+
+
+def ghosted(a):
+    """A copy of an (n,n,n) array inside an (n+2,n,n) buffer, as the reference's
+    mpi_grid_allocate lays it out (library/src/mpi_base.c:223-233); returns
+    (buffer, pointer to plane 0)."""
+    n = a.shape[0]
+    buf = np.zeros((n + 2, n, n))
+    buf[1:-1] = a
+    return buf, ctypes.c_void_p(buf.ctypes.data + n * n * 8)
+
+
+def kernel_vectors(lib):
+    L = lib.lib
+    vp = ctypes.c_void_p
+    out = {}
+    # laplace_filter (library/src/laplace.c:24-75)
+    n = 8
+    u = np.random.default_rng(101).standard_normal((n, n, n))
+    ub, up = ghosted(u)
+    o = np.zeros((n, n, n))
+    L.laplace_filter.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    L.laplace_filter(up, o.ctypes.data_as(vp), n, n)
+    out["lap_in"], out["lap_out"] = u, o
+    # laplace_filter_pb (library/src/laplace.c:77-145)
+    eps = [np.random.default_rng(s).uniform(2, 80, (n, n, n)) for s in (102, 103, 104)]
+    k2 = np.random.default_rng(105).uniform(0, 0.1, (n, n, n))
+    bufs = [ghosted(e) for e in eps]
+    ub, up = ghosted(u)
+    o = np.zeros((n, n, n))
+    L.laplace_filter_pb.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+    L.laplace_filter_pb(up, o.ctypes.data_as(vp), n, n, bufs[0][1], bufs[1][1], bufs[2][1],
+                        k2.ctypes.data_as(vp))
+    out["lappb_eps"], out["lappb_k2"], out["lappb_out"] = np.stack(eps), k2, o
+    # conj_grad (library/src/plcg.c:28-90)
+    n = 12
+    b = np.random.default_rng(106).standard_normal((n, n, n))
+    b -= b.mean()
+    x0 = np.random.default_rng(107).standard_normal((n, n, n)) * 0.1
+    xb, xp = ghosted(x0)
+    L.conj_grad.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.conj_grad.restype = ctypes.c_int
+    it = L.conj_grad(b.ctypes.data_as(vp), xp, xp, 1e-10, n, n)
+    out["cg_b"], out["cg_x0"], out["cg_x"], out["cg_iter"] = b, x0, xb[1:-1].copy(), np.array(it)
+    # red-black Gauss-Seidel, 27-point restriction, trilinear prolongation
+    # (library/src/multigrid.c:636-729, :523-594, :393-463)
+    n = 16
+    rhs = np.random.default_rng(108).standard_normal((n, n, n))
+    x = np.random.default_rng(109).standard_normal((n, n, n))
+    xb, xp = ghosted(x)
+    L.smooth_rbgs.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.init_mpi()
+    L.smooth_rbgs(rhs.ctypes.data_as(vp), xp, n, n, 3.0)
+    out["rbgs_rhs"], out["rbgs_x0"], out["rbgs_out"] = rhs, x, xb[1:-1].copy()
+    rb, rp = ghosted(rhs)
+    c = np.zeros((n // 2,) * 3)
+    L.restriction_27pt.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.restriction_27pt(rp, c.ctypes.data_as(vp), n, n, 0)
+    out["restr_out"] = c
+    coarse = np.random.default_rng(110).standard_normal((n // 2,) * 3)
+    cb, cp = ghosted(coarse)
+    f = np.zeros((n, n, n))
+    L.prolong_trilinear.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int]
+    L.prolong_trilinear(cp, f.ctypes.data_as(vp), n // 2, n // 2, n, n, 0)
+    out["prol_in"], out["prol_out"] = coarse, f
+    return out
+
+
+def md_vectors(lib, n_ions, n_grid, seed, method, steps, **kw):
+    s = W.nacl_system(n_ions, n_grid, seed=seed)
+    cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s), **kw)
+    driver.initialize(lib, s, cfg)
+    driver.bootstrap(lib)
+    its, vs = [], []
+    for _ in range(steps):
+        it, v = driver.md_step(lib)
+        its.append(it)
+        vs.append(v)
+    snap = driver.snapshot(lib, s)
+    lib.solver_finalize()
+    out = {"iters": np.array(its), "v_tf": np.array(vs)}
+    for k in ("pos", "vel", "fcs_elec", "fcs_noel", "fcs_tot", "momentum"):
+        out[k] = snap[k]
+    for k in ("kinetic", "temperature", "energy_elec"):
+        out[k] = np.array(snap[k])
+    for k in ("field", "field_prev", "q"):
+        a = snap[k]
+        out[k + "_plane0"] = a[0].copy()
+        out[k + "_stats"] = np.array([a.sum(), (a * a).sum(), np.abs(a).max()])
+    return out
+
+
+def split_top(s):
+    """Split on commas outside parentheses."""
+    parts, depth, cur = [], 0, ""
+    for ch in s:
+        if ch == "(":
+            depth += 1
+        elif ch == ")":
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    parts.append(cur)
+    return [" ".join(p.split()) for p in parts]
+
+
+def abi_fixture():
+    """Names, restypes and argtypes registered by the reference's ctypes layer, parsed
+    as text (the reference Python is not imported)."""
+    entries = []
+    for fname in ("solver.py", "get_functions.py", "mpi_base.py"):
+        with open(os.path.join(REF_PY, fname)) as fh:
+            text = fh.read()
+        for m in re.finditer(r"^capi\.register_function\(\s*'([a-z_0-9]+)'\s*,\s*([^,\[]+?)\s*,\s*\[(.*?)\]",
+                             text, re.S | re.M):
+            args = [a.strip() for a in split_top(m.group(3)) if a.strip()]
+            entries.append({"name": m.group(1), "restype": m.group(2).strip(), "argtypes": args,
+                            "file": f"maze_poisson/c_api/{fname}"})
+    return entries
+
+
+def main():
+    if os.environ.get("OMP_NUM_THREADS") != "1":
+        sys.exit("run with OMP_NUM_THREADS=1 (the reference's scatter races under OpenMP)")
+    lib = MazeLib(REF, extensions=False)
+    np.savez_compressed(os.path.join(HERE, "ref_kernels.npz"), **kernel_vectors(lib))
+    cases = {
+        "maze_lcg_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=5),
+        "maze_lcg_jacobi_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=3,
+                                       precond="JACOBI"),
+        "lcg_512_32": dict(n_ions=512, n_grid=32, seed=4, method="LCG", steps=3),
+        "mg_512_32": dict(n_ions=512, n_grid=32, seed=2, method="MULTIGRID", steps=2),
+        "maze_mg_512_32": dict(n_ions=512, n_grid=32, seed=2, method="MAZE-MULTIGRID", steps=2),
+        "verlet_thermo_216_32": dict(n_ions=216, n_grid=32, seed=3, method="MAZE-LCG", steps=3,
+                                     thermostat=True),
+        "ovrvo_nve_216_32": dict(n_ions=216, n_grid=32, seed=3, method="MAZE-LCG", steps=3,
+                                 integrator="OVRVO"),
+        "maze_lcg_512_64": dict(n_ions=512, n_grid=64, seed=1, method="MAZE-LCG", steps=1),
+    }
+    meta = {}
+    for name, kw in cases.items():
+        kw = dict(kw)
+        args = {k: kw.pop(k) for k in ("n_ions", "n_grid", "seed", "method", "steps")}
+        vec = md_vectors(lib, **args, **kw)
+        np.savez_compressed(os.path.join(HERE, f"ref_md_{name}.npz"), **vec)
+        meta[name] = {**args, **kw}
+        print(name, vec["iters"].tolist())
+    with open(os.path.join(HERE, "ref_md_cases.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+    if os.path.isdir(REF_PY):
+        with open(os.path.join(HERE, "reference_abi.json"), "w") as fh:
+            json.dump(abi_fixture(), fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+"""HIP library vs the CPU oracle on the same seeded inputs (sizes the oracle finishes
+in seconds).  Point-wise kernels must agree bit for bit (they follow the reference's
+operation order); anything that goes through a global sum (CG scalars) agrees to
+the stated tolerances.  Every test calls the product through its C-ABI."""
+import numpy as np
+import pytest
+
+from maze_poisson_amd import driver, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(1234)
+
+
+def rand_grid(n, seed):
+    return np.ascontiguousarray(np.random.default_rng(seed).standard_normal((n, n, n)))
+
+
+@pytest.mark.parametrize("n", [8, 17, 48, 64])
+def test_laplace_bitexact(hip_raw, oracle_raw, n):
+    u = rand_grid(n, n)
+    a, b = np.zeros_like(u), np.zeros_like(u)
+    hip_raw.maze_test_laplace(u, a, n)
+    oracle_raw.or_laplace(u, b, n)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [16, 40])
+def test_laplace_pb_bitexact(hip_raw, oracle_raw, n):
+    u = rand_grid(n, 1)
+    ex, ey, ez = (np.ascontiguousarray(2 + 78 * np.random.default_rng(s).random((n, n, n)))
+                  for s in (2, 3, 4))
+    k2 = np.ascontiguousarray(0.01 * np.random.default_rng(5).random((n, n, n)))
+    a, b = np.zeros_like(u), np.zeros_like(u)
+    hip_raw.maze_test_laplace_pb(u, a, n, ex, ey, ez, k2)
+    oracle_raw.or_laplace_pb(u, b, n, ex, ey, ez, k2)
+    assert np.array_equal(a, b)
+
+
+def zero_mean(a):
+    return np.ascontiguousarray(a - a.mean())
+
+
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_conj_grad_matches_oracle(hip_raw, oracle_raw, n):
+    b = zero_mean(rand_grid(n, 7))
+    tol = 1e-9 * np.abs(b).max()
+    xg, xo = np.zeros_like(b), np.zeros_like(b)
+    ig = hip_raw.maze_test_conj_grad(b, xg, tol, n)
+    io = oracle_raw.or_conj_grad(b, xo, tol, n)
+    # Same recurrence; only the summation order of the dot products differs.
+    assert io >= 0 and abs(ig - io) <= 1, (ig, io)
+    r = np.zeros_like(b)
+    oracle_raw.or_laplace(xg, r, n)
+    assert np.abs(r - b).max() <= 2 * tol
+    # Two CG solutions of one singular system differ by a constant plus O(tol) error.
+    d = (xg - xg.mean()) - (xo - xo.mean())
+    assert np.abs(d).max() <= 1e-6 * np.abs(xo).max()
+
+
+def test_rbgs_restrict_prolong_bitexact(hip_raw, oracle_raw):
+    n = 32
+    rhs, x0 = rand_grid(n, 11), rand_grid(n, 12)
+    a, b = x0.copy(), x0.copy()
+    hip_raw.maze_test_rbgs(rhs, a, n, 3)
+    oracle_raw.or_mg_smooth_rbgs(rhs, b, n, 3)
+    assert np.array_equal(a, b)
+    c1, c2 = np.zeros((16, 16, 16)), np.zeros((16, 16, 16))
+    assert hip_raw.maze_test_restrict27(rhs, c1, n) == 0
+    oracle_raw.or_mg_restrict27(rhs, c2, n)
+    assert np.array_equal(c1, c2)
+    coarse = rand_grid(16, 13)
+    f1 = x0.copy()
+    assert hip_raw.maze_test_prolong_add(coarse, f1, 16) == 0
+    f2 = np.zeros_like(x0)
+    oracle_raw.or_mg_prolong_trilinear(coarse, f2, 16)
+    assert np.array_equal(f1, x0 + f2)
+
+
+def test_vcycle_matches_oracle(hip_raw, oracle_raw):
+    n = 32
+    rhs = zero_mean(rand_grid(n, 21))
+    a, b = np.zeros_like(rhs), np.zeros_like(rhs)
+    assert hip_raw.maze_test_mg_vcycle(rhs, a, n) == 0
+    oracle_raw.or_mg_vcycle(rhs, b, n, 3)
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+
+
+# ---- full MD sequences through the reference ABI ---------------------------------
+
+def run(lib, s, cfg, steps, precision=None):
+    driver.initialize(lib, s, cfg)
+    if precision is not None:
+        lib.lib.solver_set_precision(precision)
+    driver.bootstrap(lib)
+    its, vs = [], []
+    for _ in range(steps):
+        it, v = driver.md_step(lib)
+        its.append(it)
+        vs.append(v)
+    snap = driver.snapshot(lib, s)
+    nb = np.zeros((s.n_p, 8, 3), dtype=np.int64)
+    lib.get_neighbors(nb)
+    cnt = np.zeros(s.n_p, dtype=np.int32)
+    lib.get_tf_neighbor_counts(cnt)
+    snap["neighbors"], snap["counts"] = nb, cnt
+    lib.solver_finalize()
+    return its, np.array(vs), snap
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+@pytest.mark.parametrize("method", ["MAZE-LCG", "LCG"])
+def test_md_sequence_cg(hip, oracle, method):
+    s = W.nacl_system(512, 32, seed=1)
+    cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s))
+    its_g, v_g, g = run(hip, s, cfg, 4)
+    its_o, v_o, o = run(oracle, s, cfg, 4)
+    assert all(abs(a - b) <= 1 for a, b in zip(its_g, its_o)), (its_g, its_o)
+    assert np.array_equal(g["neighbors"], o["neighbors"])
+    assert np.array_equal(g["counts"], o["counts"])
+    for k in ("pos", "vel"):
+        assert rel(g[k], o[k]) <= 1e-11, k
+    for k in ("fcs_elec", "fcs_noel", "fcs_tot", "field", "field_prev"):
+        assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+    assert rel(g["q"], o["q"]) <= 1e-12
+    assert rel(v_g, v_o) <= 1e-12
+
+
+@pytest.mark.parametrize("method", ["MULTIGRID", "MAZE-MULTIGRID"])
+def test_md_sequence_mg(hip, oracle, method):
+    s = W.nacl_system(512, 32, seed=2)
+    cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s))
+    its_g, v_g, g = run(hip, s, cfg, 2)
+    its_o, v_o, o = run(oracle, s, cfg, 2)
+    assert all(abs(a - b) <= 1 for a, b in zip(its_g, its_o)), (its_g, its_o)
+    for k in ("fcs_elec", "fcs_tot", "field"):
+        assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+
+
+def test_cic_bitexact_at_64(hip, oracle):
+    """Charge grid and neighbour tables bit-exact on the 512-ion / 64^3 workload."""
+    s = W.nacl_system(512, 64, seed=1)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    out = []
+    for lib in (hip, oracle):
+        driver.initialize(lib, s, cfg)
+        lib.solver_update_charges()
+        q = np.zeros((64, 64, 64))
+        lib.get_q(q)
+        nb = np.zeros((s.n_p, 8, 3), dtype=np.int64)
+        lib.get_neighbors(nb)
+        out.append((q, nb))
+        lib.solver_finalize()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_ovrvo_thermostat_matches_oracle(hip, oracle):
+    s = W.nacl_system(216, 32, seed=3)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s), integrator="OVRVO", thermostat=True, gamma=2.4189e-5)
+    for lib in (hip, oracle):
+        lib.solver_set_rng_seed(7)
+    _, _, g = run(hip, s, cfg, 3)
+    _, _, o = run(oracle, s, cfg, 3)
+    assert rel(g["vel"], o["vel"]) <= 1e-9
+    assert rel(g["pos"], o["pos"]) <= 1e-11
+
+
+def test_mixed_precision_forces(hip):
+    """FP32 CG + FP64 refinement vs the FP64 path: force RMS error <= 1e-5 relative."""
+    s = W.nacl_system(512, 64, seed=1)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    _, _, f64 = run(hip, s, cfg, 3, precision=0)
+    _, _, f32 = run(hip, s, cfg, 3, precision=1)
+    d = f32["fcs_tot"] - f64["fcs_tot"]
+    err = np.sqrt((d ** 2).mean()) / np.sqrt((f64["fcs_tot"] ** 2).mean())
+    assert err <= 1e-5, err
+
+
+def test_determinism(hip):
+    s = W.nacl_system(512, 32, seed=5)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    a = run(hip, s, cfg, 3)[2]
+    b = run(hip, s, cfg, 3)[2]
+    for k in ("pos", "vel", "fcs_tot", "field", "q"):
+        assert np.array_equal(a[k], b[k]), k
