@@ -169,12 +169,20 @@ void solver_set_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
 
 /* Field-solve arithmetic: 0 = FP64 CG as in the reference (default);
  * 1 = FP32-storage CG inside FP64 iterative refinement, stopping on the
- * reference's criterion max|A y - sigma| <= tol evaluated in FP64.  Returns
- * 0, or -1 for an unknown mode. */
+ * reference's criterion max|A y - sigma| <= tol evaluated in FP64;
+ * 2 = as 1 with the multigrid-preconditioned FP32 CG below.  Returns 0, or
+ * -1 for an unknown mode. */
 int solver_set_precision(int mode);
 int solver_get_precision(void);
 /* Relative residual reduction of each FP32 inner solve (default 1e-4). */
 void solver_set_inner_rtol(double rtol);
+/* Mode 2 = FP32 multigrid-preconditioned CG inside FP64 iterative
+ * refinement (V(2,2) damped Jacobi, exact FP64 coarsest solve); falls back
+ * to mode 1 when the grid has no multigrid hierarchy (halving reaches an odd
+ * size above 12) or the PB operator is enabled.  Jacobi damping (default
+ * 6/7) and the number of levels: */
+void solver_set_mg_omega(double omega);
+int solver_mg_levels(void);
 
 /* 1 (default): a charge-conservation failure exits like the reference;
  * 0: solver_update_charges returns 1 instead. */
@@ -208,6 +216,7 @@ void maze_test_rbgs(const double *in, double *out, int n, int iters);
 int maze_test_restrict27(const double *in, double *out, int n);
 int maze_test_prolong_add(const double *in, double *out, int nc);
 int maze_test_mg_vcycle(const double *in, double *out, int n);
+int maze_test_mg_pcg(const double *b, double *x, double rtol, int n);
 
 #ifdef __cplusplus
 }

@@ -95,6 +95,13 @@ EXTENSION_ABI = [
     ("get_y", None, [_D3]),
     ("get_neighbors", None, [_L3]),
     ("get_tf_neighbor_counts", None, [_I1]),
+    ("solver_sync", None, []),
+    ("solver_set_precision", _c_int, [_c_int]),
+    ("solver_get_precision", _c_int, []),
+    ("solver_set_inner_rtol", None, [_c_double]),
+    ("solver_set_mg_omega", None, [_c_double]),
+    ("solver_mg_levels", _c_int, []),
+    ("solver_set_charge_check", None, [_c_int]),
 ]
 
 

@@ -169,7 +169,7 @@ void solver_set_eps_k2(double* ex, double* ey, double* ez, double* k2) {
 }
 
 int solver_set_precision(int mode) {
-    if (mode < 0 || mode > 1) return -1;
+    if (mode < 0 || mode > 2) return -1;
     S().precision = mode;
     return 0;
 }
@@ -177,6 +177,12 @@ int solver_set_precision(int mode) {
 int solver_get_precision() { return S().precision; }
 
 void solver_set_inner_rtol(double rtol) { S().inner_rtol = rtol; }
+
+// Damping of the multigrid Jacobi smoother (default 6/7).
+void solver_set_mg_omega(double omega) { S().mg_omega = omega; }
+
+// Number of multigrid levels for the current grid (0: unavailable).
+int solver_mg_levels() { return S().setup_mg() ? (int)S().mg.size() : 0; }
 
 void solver_set_charge_check(int strict) { S().strict_charge_check = strict; }
 
@@ -289,6 +295,38 @@ void maze_test_rbgs(const double* in, double* out, int n, int iters) {
     s->sync();
     MAZE_HIP(hipMemcpy(out, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
     delete s;
+}
+
+// Multigrid-preconditioned FP32 CG for A x = b from x = 0 (relative stop);
+// returns the iteration count, or -1 when no multigrid hierarchy exists for n.
+int maze_test_mg_pcg(const double* b, double* x, double rtol, int n) {
+    Sim* s = scratch_sim(n);
+    s->b32 = nullptr;
+    s->precision = maze::kMixedMG;
+    if (!s->setup_mg()) {
+        delete s;
+        return -1;
+    }
+    const long G = s->G;
+    std::vector<float> bf(G), xf(G);
+    for (long i = 0; i < G; i++) bf[i] = (float)b[i];
+    float *db = nullptr, *dx = nullptr, *dr = nullptr, *pa = nullptr, *pb = nullptr;
+    MAZE_HIP(hipMalloc(&db, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dx, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dr, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&pa, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&pb, G * sizeof(float)));
+    MAZE_HIP(hipMemcpy(db, bf.data(), G * sizeof(float), hipMemcpyHostToDevice));
+    s->r32 = dr;
+    s->q32a = pa;
+    s->q32b = pb;
+    const int it = s->cg_inner_mg(db, dx, rtol, n * n);
+    MAZE_HIP(hipMemcpy(xf.data(), dx, G * sizeof(float), hipMemcpyDeviceToHost));
+    for (long i = 0; i < G; i++) x[i] = (double)xf[i];
+    s->r32 = s->q32a = s->q32b = nullptr;
+    for (float* a : {db, dx, dr, pa, pb}) MAZE_HIP(hipFree(a));
+    delete s;
+    return it;
 }
 
 int maze_test_restrict27(const double* in, double* out, int n) {

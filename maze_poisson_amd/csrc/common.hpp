@@ -108,30 +108,39 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return s;
 }
 
-// Last-arriving work-group detection for the single-pass reductions: every
-// work-group publishes its partial, fences at device scope and bumps the
-// counter; the one that sees nblocks-1 reduces all partials in index order,
-// so the result does not depend on arrival order.
-__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int nblocks) {
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int prev = atomicAdd(counter, 1u);
-        last = (prev == nblocks - 1);
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
-
 // Per-work-group partials are exchanged through device-scope atomic loads and
-// stores so that they are coherent across the XCDs' separate L2 caches.
+// stores, which are performed at the level shared by all XCDs (their L2
+// caches are not coherent with each other).
 __device__ __forceinline__ void store_partial(double* p, int i, double v) {
     __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double load_partial(const double* p, int i) {
     return __hip_atomic_load(const_cast<double*>(p) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Single-pass reduction hand-off: thread 0 publishes the work-group's
+// partial(s), waits until those device-scope stores have completed, then
+// bumps the arrival counter; the work-group that sees nblocks-1 reduces all
+// partials in index order (so the result does not depend on arrival order).
+// Only the partials need to be visible to the last work-group, so no
+// device-wide release fence (an L2 write-back on this chip) is issued.
+__device__ __forceinline__ bool publish_arrive(double* psum, double s, double* pmax, double m,
+                                               unsigned int* counter, unsigned int nblocks) {
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        store_partial(psum, blockIdx.x, s);
+        if (pmax) store_partial(pmax, blockIdx.x, m);
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        const unsigned int prev =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == nblocks - 1);
+    }
+    __syncthreads();
+    return last;
+}
+
+__device__ __forceinline__ void reset_counter(unsigned int* counter) {
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Sum / max of partials[0..cnt) in a fixed order; valid in all threads.

@@ -23,7 +23,7 @@ struct CGState {
     int res;         // result code (see cg_result)
     int first;       // 1 until the first direction has been formed
     int count_mode;  // 1: result is the number of iterations (coarse CG)
-    unsigned int counter;
+    int pcg;         // 1: beta comes from the preconditioner pass, not from <r, r>
     int pad;
 };
 
@@ -56,7 +56,8 @@ void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Rea
 template <class Real>
 void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
                       Real* r, CGState* st, Reduce red);
-void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit, int count_mode);
+void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit, int count_mode,
+                     int pcg = 0);
 
 // Mixed precision: FP64 true residual b32 = sigma - A (y + d), y <- y + d.
 void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
@@ -69,6 +70,8 @@ void launch_fill(hipStream_t s, long n, double v, double* out);
 void launch_fill_f(hipStream_t s, long n, float v, float* out);
 void launch_copy(hipStream_t s, long n, const double* in, double* out);
 void launch_half_sum(hipStream_t s, long n, const double* phi, double* out, Reduce red);
+void launch_sum_scaled(hipStream_t s, long n, double scale, const double* v, double* out,
+                       Reduce red);
 
 // Reference two-grid multigrid pieces.
 void launch_rbgs(hipStream_t s, const GridDims& d, const double* in, double* out, int iters);
@@ -78,5 +81,19 @@ void launch_restrict27(hipStream_t s, int n, const double* in, double* out);
 void launch_prolong_add(hipStream_t s, int nc, const double* in, double* out);
 void launch_true_residual_max(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
                               const double* sigma, CGState* st, Reduce red);
+
+// FP32 multigrid preconditioner (mg_kernels.hip).
+void launch_mg_pre(hipStream_t s, const GridDims& d, const float* b, float* dout, double c,
+                   CGState* st);
+void launch_mg_resid(hipStream_t s, const GridDims& d, const float* b, const float* din, float* r,
+                     CGState* st);
+void launch_mg_post(hipStream_t s, const GridDims& d, const float* b, const float* din,
+                    float* dout, double c, CGState* st, int dot, Reduce red);
+void launch_mg_restrict(hipStream_t s, int n, const float* r, float* bc, CGState* st);
+void launch_mg_prolong_add(hipStream_t s, int nc, const float* ec, float* d, CGState* st);
+void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
+                      const double* ilam, CGState* st);
+void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
+                    float* p_new, CGState* st, Reduce red);
 
 }  // namespace maze

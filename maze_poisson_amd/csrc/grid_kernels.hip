@@ -13,135 +13,9 @@
 // (library/src/laplace.c:301-309, :366-379; library/src/plcg.c:41-83;
 // library/src/verlet.c:221-240), so every point-wise result matches the CPU
 // oracle bit for bit; only the order of the global sums differs.
-#include "grid.hpp"
+#include "stencil.hpp"
 
 namespace maze {
-
-// ---- source functors -------------------------------------------------------
-template <class Real>
-struct SrcArr {
-    const Real* a;
-    __device__ __forceinline__ double operator()(int i) const { return (double)a[i]; }
-};
-
-struct SrcPred {  // phi* = 2 phi - phi_prev   (library/src/verlet.c:223-225)
-    const double* phi;
-    const double* phip;
-    __device__ __forceinline__ double operator()(int i) const { return 2 * phi[i] - phip[i]; }
-};
-
-template <class Real>
-struct SrcDir {  // p = beta p + r/6, or r/6 for the first direction (plcg.c:52-53, :79-80)
-    const Real* p;
-    const Real* r;
-    double beta;
-    int first;
-    __device__ __forceinline__ double operator()(int i) const {
-        const double c6 = 1.0 / 6.0;
-        double v = first ? (double)r[i] * c6 : (double)p[i] * beta + c6 * (double)r[i];
-        return (double)(Real)v;
-    }
-};
-
-struct SrcSum {  // y + d
-    const double* y;
-    const float* d;
-    int add;
-    __device__ __forceinline__ double operator()(int i) const {
-        return add ? y[i] + (double)d[i] : y[i];
-    }
-};
-
-struct SrcZero {
-    __device__ __forceinline__ double operator()(int) const { return 0.0; }
-};
-
-// ---- the i-marching stencil sweep -------------------------------------------
-// Calls body(c, value_at_c, (A value)_c) for every point of this work-group's
-// tile.  Threads past the end of the plane do nothing but stay for the
-// work-group reductions.
-template <bool PB, class Src, class Body>
-__device__ __forceinline__ void march(const GridDims& d, const PBCoef& pb, const Src& src,
-                                      Body& body) {
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
-    const int chunk = b % d.chunks;
-    const int s = b / d.chunks;
-    const int pl = chunk * kBlock + threadIdx.x;
-    if (pl >= d.n2) return;
-    const int n = d.n, n2 = d.n2;
-    const PlaneNb nb = plane_nb(pl, n, n2);
-    const int i0 = s * d.slab;
-    const int i1 = min(n, i0 + d.slab);
-    const int im0 = (i0 == 0) ? n - 1 : i0 - 1;
-    double vm = src(im0 * n2 + pl);
-    double vc = src(i0 * n2 + pl);
-    double exm = 0.0;
-    if (PB) exm = pb.ex[im0 * n2 + pl];
-    for (int i = i0; i < i1; i++) {
-        const int ip = (i + 1 == n) ? 0 : i + 1;
-        const int base = i * n2;
-        const int c = base + pl;
-        const double vp = src(ip * n2 + pl);
-        const double vjm = src(base + nb.jm), vjp = src(base + nb.jp);
-        const double vkm = src(base + nb.km), vkp = src(base + nb.kp);
-        double lap;
-        if (!PB) {
-            lap = vp + vm + vjp + vjm + vkp + vkm - vc * 6.0;
-        } else {
-            const double exc = pb.ex[c], eyc = pb.ey[c], ezc = pb.ez[c];
-            const double eym = pb.ey[base + nb.jm], ezm = pb.ez[base + nb.km];
-            lap = vp * exc + vm * exm + vjp * eyc + vjm * eym + vkp * ezc + vkm * ezm -
-                  vc * (exc + exm + eyc + eym + ezc + ezm + pb.k2[c]);
-            exm = exc;
-        }
-        body(c, vc, lap);
-        vm = vc;
-        vc = vp;
-    }
-}
-
-template <bool PB, class Src, class Body>
-__global__ __launch_bounds__(kBlock) void k_pass(GridDims d, PBCoef pb, Src src, Body body,
-                                                 Reduce red, CGState* st) {
-    if (Body::kSkipWhenDone && st->done) return;
-    Body bd = body;
-    march<PB>(d, pb, src, bd);
-    if constexpr (Body::kReduce) {
-        __shared__ double sh[kBlock / kWave];
-        const double s = block_sum(bd.s, sh);
-        const double m = block_max(bd.m, sh);
-        if (threadIdx.x == 0) {
-            store_partial(red.psum, blockIdx.x, s);
-            store_partial(red.pmax, blockIdx.x, m);
-        }
-        if (arrive_last(red.counter, gridDim.x)) {
-            const double S = reduce_partials_sum(red.psum, gridDim.x, sh);
-            const double M = reduce_partials_max(red.pmax, gridDim.x, sh);
-            if (threadIdx.x == 0) {
-                Body::finalize(st, S, M);
-                __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-    }
-}
-
-template <bool PB, class Src, class Body>
-static void run_pass(hipStream_t s, const GridDims& d, const PBCoef& pb, const Src& src,
-                     const Body& body, Reduce red, CGState* st) {
-    hipLaunchKernelGGL((k_pass<PB, Src, Body>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, src,
-                       body, red, st);
-}
-
-#define DISPATCH_PB(pb, ...)                \
-    do {                                    \
-        if ((pb).ex) {                      \
-            constexpr bool kPB = true;      \
-            __VA_ARGS__;                    \
-        } else {                            \
-            constexpr bool kPB = false;     \
-            __VA_ARGS__;                    \
-        }                                   \
-    } while (0)
 
 // ---- bodies -----------------------------------------------------------------
 struct BodyLaplace {
@@ -256,9 +130,11 @@ struct BodyUpdate {
             st->res = st->iter;
             return;
         }
-        const double rn_dot_vn = -S / 6.0;
-        st->beta = rn_dot_vn / st->rdv;
-        st->rdv = rn_dot_vn;
+        if (!st->pcg) {
+            const double rn_dot_vn = -S / 6.0;
+            st->beta = rn_dot_vn / st->rdv;
+            st->rdv = rn_dot_vn;
+        }
         st->iter += 1;
         if (st->iter >= st->limit) {
             st->done = 1;
@@ -345,12 +221,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(GridDims d, PBCoef pb, const 
     march<PB>(d, pb, src, bd);
     __shared__ double sh[kBlock / kWave];
     const double S = block_sum(bd.s, sh);
-    if (threadIdx.x == 0) store_partial(red.psum, blockIdx.x, S);
-    if (arrive_last(red.counter, gridDim.x)) {
+    if (publish_arrive(red.psum, S, nullptr, 0.0, red.counter, gridDim.x)) {
         const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
         if (threadIdx.x == 0) {
             BodyDir<Real>::finalize(st, T, 0.0);
-            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            reset_counter(red.counter);
         }
     }
 }
@@ -364,16 +239,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(GridDims d, PBCoef pb, con
     __shared__ double sh[kBlock / kWave];
     const double S = block_sum(bd.s, sh);
     const double M = block_max(bd.m, sh);
-    if (threadIdx.x == 0) {
-        store_partial(red.psum, blockIdx.x, S);
-        store_partial(red.pmax, blockIdx.x, M);
-    }
-    if (arrive_last(red.counter, gridDim.x)) {
+    if (publish_arrive(red.psum, S, red.pmax, M, red.counter, gridDim.x)) {
         const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
         const double U = reduce_partials_max(red.pmax, gridDim.x, sh);
         if (threadIdx.x == 0) {
             BodyUpdate<Real>::finalize(st, T, U);
-            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            reset_counter(red.counter);
         }
     }
 }
@@ -405,8 +276,10 @@ template void launch_cg_update<double>(hipStream_t, const GridDims&, const PBCoe
 template void launch_cg_update<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
                                       float*, float*, CGState*, Reduce);
 
-__global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int count_mode) {
+__global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int count_mode,
+                           int pcg) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->pcg = pcg;
         st->tol = tol;
         st->rtol = rtol;
         st->limit = limit;
@@ -419,8 +292,8 @@ __global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int 
 }
 
 void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit,
-                     int count_mode) {
-    hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode);
+                     int count_mode, int pcg) {
+    hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode, pcg);
 }
 
 void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
@@ -479,25 +352,34 @@ void launch_copy(hipStream_t s, long n, const double* in, double* out) {
     hipLaunchKernelGGL(k_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, in, out);
 }
 
-// 1/2 sum phi (library/src/grid_general.c:304-316) into st->resid.
-__global__ void k_half_sum(long n, const double* __restrict__ phi, Reduce red, double* out) {
+// out = sum(scale * v[i]) with the single-pass deterministic reduction; with
+// scale 1/2 over phi this is grid_get_energy_elec (library/src/grid_general.c:304-316).
+__global__ void k_sum_scaled(long n, double scale, const double* __restrict__ v, Reduce red,
+                             double* out) {
     double s = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        s += 0.5 * phi[i];
+        s += scale * v[i];
     __shared__ double sh[kBlock / kWave];
     const double S = block_sum(s, sh);
-    if (threadIdx.x == 0) store_partial(red.psum, blockIdx.x, S);
-    if (arrive_last(red.counter, gridDim.x)) {
+    if (publish_arrive(red.psum, S, nullptr, 0.0, red.counter, gridDim.x)) {
         const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
         if (threadIdx.x == 0) {
             *out = T;
-            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            reset_counter(red.counter);
         }
     }
 }
 
+void launch_sum_scaled(hipStream_t s, long n, double scale, const double* v, double* out,
+                       Reduce red) {
+    long b = (n + 4L * kBlock - 1) / (4L * kBlock);
+    if (b < 1) b = 1;
+    if (b > 2048) b = 2048;
+    hipLaunchKernelGGL(k_sum_scaled, dim3((int)b), dim3(kBlock), 0, s, n, scale, v, red, out);
+}
+
 void launch_half_sum(hipStream_t s, long n, const double* phi, double* out, Reduce red) {
-    hipLaunchKernelGGL(k_half_sum, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, phi, red, out);
+    launch_sum_scaled(s, n, 0.5, phi, out, red);
 }
 
 // ---- reference two-grid multigrid (library/src/multigrid.c) -----------------

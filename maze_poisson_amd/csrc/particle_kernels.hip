@@ -147,19 +147,8 @@ void launch_cic_gather(hipStream_t s, const PDev& p, double* q) {
                        p.h, p.pos, p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
 }
 
-// Single-work-group fixed-order sum of n values into *out.
-__device__ void block_sum_to(const double* v, long n, double* out) {
-    __shared__ double sh[kBlock / kWave];
-    double s = 0.0;
-    for (long i = threadIdx.x; i < n; i += kBlock) s += v[i];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) *out = s;
-}
-
-__global__ void k_sum1(const double* v, long n, double* out) { block_sum_to(v, n, out); }
-
-void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce) {
-    hipLaunchKernelGGL(k_sum1, dim3(1), dim3(kBlock), 0, s, p.contrib, 8L * p.np, &ps->q_sum);
+void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
+    launch_sum_scaled(s, 8L * p.np, 1.0, p.contrib, &ps->q_sum, red);
 }
 
 // Field force -q w grad(phi) by central differences at the 8 CIC corners
@@ -283,17 +272,9 @@ __global__ void k_tf_fold(int np, int segs, const double* __restrict__ fpart,
     counts[i] = c;
 }
 
-__global__ void k_half(const double* v, long n, double* out) {
-    __shared__ double sh[kBlock / kWave];
-    double s = 0.0;
-    for (long i = threadIdx.x; i < n; i += kBlock) s += v[i];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) *out = s / 2;
-}
-
-void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce) {
+void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
     if (p.np == 0) {
-        hipLaunchKernelGGL(k_half, dim3(1), dim3(kBlock), 0, s, p.epart, 0L, &ps->pe_tf);
+        launch_sum_scaled(s, 0, 0.5, p.epart, &ps->pe_tf, red);
         return;
     }
     const int segs = p.segments;
@@ -304,8 +285,7 @@ void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce) {
     // The per-ion energies reuse the first segment row of epart.
     hipLaunchKernelGGL(k_tf_fold, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, segs, p.fpart,
                        p.epart, p.cpart, p.fn, p.epart + (long)segs * p.np, p.counts);
-    hipLaunchKernelGGL(k_half, dim3(1), dim3(kBlock), 0, s, p.epart + (long)segs * p.np,
-                       (long)p.np, &ps->pe_tf);
+    launch_sum_scaled(s, p.np, 0.5, p.epart + (long)segs * p.np, &ps->pe_tf, red);
 }
 
 // F_tot = 0 + 1 F_elec + 1 F_noel (library/src/particles.c:777-795).

@@ -82,6 +82,7 @@ void Sim::free_grid() {
                        &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1})
         dfree(*a);
     for (float** a : {&b32, &d32, &r32, &q32a, &q32b}) dfree(*a);
+    free_mg();
     G = 0;
 }
 
@@ -355,6 +356,132 @@ int Sim::cg_inner_f32(const float* b, float* x, double rtol, int limit) {
     return it;
 }
 
+void Sim::free_mg() {
+    for (size_t l = 0; l < mg.size(); l++) {
+        if (l > 0) dfree(mg[l].b);
+        dfree(mg[l].d);
+        dfree(mg[l].t);
+    }
+    mg.clear();
+    dfree(mg_Qf);
+    dfree(mg_ilam);
+    mg_ok = -1;
+}
+
+// Levels n, n/2, ... while the size is even and above 12; the last level
+// (<= 12 points per side) is solved exactly.  Unavailable when halving stops
+// at an odd size above 12.
+bool Sim::setup_mg() {
+    if (mg_ok >= 0) return mg_ok == 1;
+    std::vector<int> sizes{n};
+    while (sizes.back() % 2 == 0 && sizes.back() > 12) sizes.push_back(sizes.back() / 2);
+    if (sizes.back() > 12 || sizes.size() < 2) {
+        mg_ok = 0;
+        return false;
+    }
+    for (size_t l = 0; l < sizes.size(); l++) {
+        MGLevel L;
+        L.n = sizes[l];
+        L.dims = make_dims(L.n, choose_slab(L.n));
+        const long g = (long)L.n * L.n * L.n;
+        L.b = l == 0 ? nullptr : dalloc<float>(g);
+        L.d = dalloc<float>(g);
+        L.t = dalloc<float>(g);
+        mg.push_back(L);
+    }
+    // Real orthonormal eigenbasis of the periodic 1-D second difference on c
+    // points: constant, cos/sin pairs, and the alternating mode for even c;
+    // eigenvalue 2 cos(2 pi k / c) - 2 per row.
+    const int c = sizes.back();
+    mg_c = c;
+    std::vector<double> Q(c * c), mu(c);
+    int row = 0;
+    for (int m = 0; m < c; m++) Q[row * c + m] = 1.0 / std::sqrt((double)c);
+    mu[row++] = 0.0;
+    for (int k = 1; 2 * k < c; k++) {
+        const double lam = 2 * std::cos(2 * kPi * k / c) - 2;
+        for (int m = 0; m < c; m++) Q[row * c + m] = std::sqrt(2.0 / c) * std::cos(2 * kPi * k * m / c);
+        mu[row++] = lam;
+        for (int m = 0; m < c; m++) Q[row * c + m] = std::sqrt(2.0 / c) * std::sin(2 * kPi * k * m / c);
+        mu[row++] = lam;
+    }
+    if (c % 2 == 0) {
+        for (int m = 0; m < c; m++) Q[row * c + m] = (m % 2 ? -1.0 : 1.0) / std::sqrt((double)c);
+        mu[row++] = -4.0;
+    }
+    std::vector<double> il(c * c * c);
+    for (int a = 0; a < c; a++)
+        for (int b = 0; b < c; b++)
+            for (int d = 0; d < c; d++) {
+                const double l3 = mu[a] + mu[b] + mu[d];
+                il[(a * c + b) * c + d] = (a == 0 && b == 0 && d == 0) ? 0.0 : 1.0 / l3;
+            }
+    mg_Qf = dalloc<double>(c * c);
+    mg_ilam = dalloc<double>(c * c * c);
+    MAZE_HIP(hipMemcpy(mg_Qf, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(mg_ilam, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice));
+    mg_ok = 1;
+    return true;
+}
+
+// V(2,2): pre-smooth, residual, restrict, recurse, prolong-correct, post-smooth.
+void Sim::vcycle(int l) {
+    MGLevel& L = mg[l];
+    const double cw = mg_omega / 6.0;
+    if (l == (int)mg.size() - 1) {
+        launch_mg_coarse(stream, mg_c, L.b, L.d, mg_Qf, mg_ilam, st);
+        return;
+    }
+    launch_mg_pre(stream, L.dims, L.b, L.d, cw, st);
+    launch_mg_resid(stream, L.dims, L.b, L.d, L.t, st);
+    launch_mg_restrict(stream, L.n, L.t, mg[l + 1].b, st);
+    vcycle(l + 1);
+    launch_mg_prolong_add(stream, mg[l + 1].n, mg[l + 1].d, L.d, st);
+    launch_mg_post(stream, L.dims, L.b, L.d, L.t, cw, st, 0, red);
+    launch_mg_post(stream, L.dims, L.b, L.t, L.d, cw, st, l == 0, red);
+}
+
+// Multigrid-preconditioned CG in FP32 storage for A x = b from x = 0, with the
+// explicit-preconditioner recurrence of library/src/plcg.c:104-166 and a
+// relative stop max|r| <= rtol max|b|; returns the iterations performed.
+int Sim::cg_inner_mg(const float* b, float* x, double rtol, int limit) {
+    launch_fill_f(stream, G, 0.0f, x);
+    launch_cg_reset(stream, st, 0.0, rtol, limit, 1, 1);
+    launch_cg_init<float>(stream, dims, PBCoef{}, nullptr, b, r32, st, red);
+    mg[0].b = r32;
+    int launched = 0;
+    int chunk = std::max(2, last_inner + 1);
+    float* pin = q32a;
+    float* pout = q32b;
+    const bool timing = prof != 0;
+    for (;;) {
+        for (int c = 0; c < chunk && launched < limit; c++, launched++) {
+            vcycle(0);
+            if (timing) {
+                hipEvent_t e0 = next_event(), e1 = next_event(), e2 = next_event();
+                MAZE_HIP(hipEventRecord(e0, stream));
+                launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red);
+                MAZE_HIP(hipEventRecord(e1, stream));
+                launch_cg_update<float>(stream, dims, PBCoef{}, pout, x, r32, st, red);
+                MAZE_HIP(hipEventRecord(e2, stream));
+                pending.push_back({0, e0, e1, launched});
+                pending.push_back({1, e1, e2, launched});
+            } else {
+                launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red);
+                launch_cg_update<float>(stream, dims, PBCoef{}, pout, x, r32, st, red);
+            }
+            std::swap(pin, pout);
+        }
+        read_state();
+        if (st_host->done || launched >= limit) break;
+        chunk = std::max(2, chunk / 2);
+    }
+    const int it = st_host->res;
+    if (timing) prof_flush(it, 2);
+    last_inner = std::max(1, it);
+    return it;
+}
+
 // MaZe step in FP64, library/src/verlet.c:31-71.
 int Sim::maze_fp64() {
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
@@ -395,7 +522,9 @@ int Sim::maze_mixed() {
             break;
         }
         if (outer >= 50) break;
-        const int it = cg_inner_f32(b32, d32, inner_rtol, inner_limit);
+        const bool use_mg = precision == kMixedMG && !pb && setup_mg();
+        const int it = use_mg ? cg_inner_mg(b32, d32, inner_rtol, inner_limit)
+                              : cg_inner_f32(b32, d32, inner_rtol, inner_limit);
         total += std::max(it, 0);
         launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 1);
         std::swap(y, y_alt);
@@ -507,7 +636,7 @@ int Sim::update_field() {
         case kLCG:
             return lcg_update();
         case kMazeLCG:
-            return precision == kFP64 ? maze_fp64() : maze_mixed();
+            return precision == kFP64 ? maze_fp64() : maze_mixed();  // kMixedCG / kMixedMG
         case kMG:
             return mg_update(false);
         case kMazeMG: {

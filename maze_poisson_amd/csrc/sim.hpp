@@ -13,7 +13,16 @@ enum PrecondType { kPNone = 0, kPJacobi = 1, kPMG = 2, kPSSOR = 3, kPBlockJacobi
 enum ItgType { kOVRVO = 0, kVerlet = 1 };
 // Arithmetic of the field solve: the reference FP64 CG, or FP32-storage CG
 // inside FP64 iterative refinement.
-enum Precision { kFP64 = 0, kMixedCG = 1 };
+enum Precision { kFP64 = 0, kMixedCG = 1, kMixedMG = 2 };
+
+// One level of the FP32 multigrid preconditioner.
+struct MGLevel {
+    int n;
+    GridDims dims;
+    float* b;  // right-hand side (level 0: the CG residual)
+    float* d;  // correction / cycle output
+    float* t;  // residual and Jacobi ping-pong buffer
+};
 
 // Per-kernel timing collected with HIP events on the solver stream.
 struct KernelTimer {
@@ -45,6 +54,14 @@ struct Sim {
     GridDims cdims{};
     double *mg_r = nullptr, *c_rhs = nullptr, *c_eps = nullptr, *c_r = nullptr, *c_p0 = nullptr,
            *c_p1 = nullptr;
+
+    // FP32 multigrid preconditioner
+    std::vector<MGLevel> mg;
+    double* mg_Qf = nullptr;
+    double* mg_ilam = nullptr;
+    int mg_c = 0;
+    double mg_omega = 6.0 / 7.0;
+    int mg_ok = -1;             // -1 not set up, 0 unavailable for this n, 1 ready
 
     CGState* st = nullptr;      // device
     CGState* st_host = nullptr; // pinned
@@ -111,6 +128,10 @@ struct Sim {
     int cg_count_solve(const GridDims& d, const double* b, double* x, double rtol, int maxit,
                        double* rbuf, double* pa, double* pbuf, int* pred);
     int cg_inner_f32(const float* b, float* x, double rtol, int limit);
+    int cg_inner_mg(const float* b, float* x, double rtol, int limit);
+    bool setup_mg();
+    void free_mg();
+    void vcycle(int level);
     int maze_fp64();
     int maze_mixed();
     int lcg_update();

@@ -99,6 +99,9 @@ def hip_raw(hip):
         "maze_test_restrict27": (ci, [D, D, ci]),
         "maze_test_prolong_add": (ci, [D, D, ci]),
         "maze_test_mg_vcycle": (ci, [D, D, ci]),
+        "maze_test_mg_pcg": (ci, [D, D, cd, ci]),
+        "solver_set_mg_omega": (None, [cd]),
+        "solver_mg_levels": (ci, []),
         "solver_set_precision": (ci, [ci]),
         "solver_set_inner_rtol": (None, [cd]),
         "maze_prof_enable": (None, [ci]),

@@ -169,13 +169,30 @@ def test_ovrvo_thermostat_matches_oracle(hip, oracle):
     assert rel(g["pos"], o["pos"]) <= 1e-11
 
 
-def test_mixed_precision_forces(hip):
-    """FP32 CG + FP64 refinement vs the FP64 path: force RMS error <= 1e-5 relative."""
+@pytest.mark.parametrize("n", [32, 64])
+def test_mg_pcg_solves(hip_raw, oracle_raw, n):
+    """FP32 multigrid-preconditioned CG: few iterations, FP32-level agreement with the
+    FP64 solution of the same system."""
+    b = zero_mean(rand_grid(n, 31))
+    x = np.zeros_like(b)
+    it = hip_raw.maze_test_mg_pcg(b, x, 1e-5, n)
+    assert 0 < it <= 20, it
+    xo = np.zeros_like(b)
+    oracle_raw.or_conj_grad(b, xo, 1e-10 * np.abs(b).max(), n)
+    d = (x - x.mean()) - (xo - xo.mean())
+    assert np.abs(d).max() <= 1e-4 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_precision_forces(hip, precision):
+    """FP32 CG (1) or FP32 MG-PCG (2) inside FP64 refinement: force RMS error <= 1e-5
+    relative to the FP64 solution (FP64 CG converged 1000x tighter)."""
     s = W.nacl_system(512, 64, seed=1)
+    tight = driver.RunConfig(tol=W.poisson_tol(s) * 1e-3)
     cfg = driver.RunConfig(tol=W.poisson_tol(s))
-    _, _, f64 = run(hip, s, cfg, 3, precision=0)
-    _, _, f32 = run(hip, s, cfg, 3, precision=1)
-    d = f32["fcs_tot"] - f64["fcs_tot"]
+    _, _, f64 = run(hip, s, tight, 3, precision=0)
+    _, _, fmx = run(hip, s, cfg, 3, precision=precision)
+    d = fmx["fcs_tot"] - f64["fcs_tot"]
     err = np.sqrt((d ** 2).mean()) / np.sqrt((f64["fcs_tot"] ** 2).mean())
     assert err <= 1e-5, err
 
