@@ -203,6 +203,9 @@ void maze_prof_enable(int on);
 void maze_prof_reset(void);
 int maze_prof_get(const char *name, double *total_ms, long *count);
 int maze_grid_blocks(void);
+/* One launch of the plain copy kernel over n doubles (reads and writes 8n
+ * bytes): the calibration reference for hardware byte counters. */
+void maze_calib_copy(long n_doubles);
 
 /* Kernel-level entry points on host buffers, run on a private replica (for
  * parity tests against the CPU oracle). */

@@ -215,6 +215,7 @@ void maze_prof_reset() {
 }
 
 int maze_prof_get(const char* name, double* total_ms, long* count) {
+    S().prof_flush(1 << 30);
     for (auto& t : S().timers)
         if (std::strcmp(t.name, name) == 0) {
             *total_ms = t.total_ms;
@@ -225,6 +226,20 @@ int maze_prof_get(const char* name, double* total_ms, long* count) {
 }
 
 int maze_grid_blocks() { return S().dims.blocks; }
+
+// One launch of the plain copy kernel over n_doubles doubles (read n*8 bytes,
+// write n*8 bytes): the calibration reference for hardware byte counters.
+void maze_calib_copy(long n_doubles) {
+    double *a = nullptr, *b = nullptr;
+    MAZE_HIP(hipMalloc(&a, n_doubles * sizeof(double)));
+    MAZE_HIP(hipMalloc(&b, n_doubles * sizeof(double)));
+    maze::launch_fill(S().stream, n_doubles, 1.0, a);
+    S().sync();
+    maze::launch_copy(S().stream, n_doubles, a, b);
+    S().sync();
+    MAZE_HIP(hipFree(a));
+    MAZE_HIP(hipFree(b));
+}
 
 // ---- kernel-level entry points on host buffers (for parity tests) ------------------
 // Each runs on a private replica and copies in and out; none touches the
@@ -320,7 +335,7 @@ int maze_test_mg_pcg(const double* b, double* x, double rtol, int n) {
     s->r32 = dr;
     s->q32a = pa;
     s->q32b = pb;
-    const int it = s->cg_inner_mg(db, dx, rtol, n * n);
+    const int it = s->cg_inner_mg(db, dx, rtol, n * n, 0);
     MAZE_HIP(hipMemcpy(xf.data(), dx, G * sizeof(float), hipMemcpyDeviceToHost));
     for (long i = 0; i < G; i++) x[i] = (double)xf[i];
     s->r32 = s->q32a = s->q32b = nullptr;

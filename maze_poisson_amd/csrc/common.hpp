@@ -108,6 +108,26 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return s;
 }
 
+// Sum and max of one value pair in a single LDS exchange.  sh holds 8 doubles.
+__device__ __forceinline__ void block_sum_max(double& s, double& m, double* sh) {
+    s = wave_sum(s);
+    m = wave_max(m);
+    const int w = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sh[w] = s;
+        sh[4 + w] = m;
+    }
+    __syncthreads();
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < kBlock / kWave; i++) {
+        a += sh[i];
+        b = fmax(b, sh[4 + i]);
+    }
+    s = a;
+    m = b;
+}
+
 // Per-work-group partials are exchanged through device-scope atomic loads and
 // stores, which are performed at the level shared by all XCDs (their L2
 // caches are not coherent with each other).
@@ -144,16 +164,59 @@ __device__ __forceinline__ void reset_counter(unsigned int* counter) {
 }
 
 // Sum / max of partials[0..cnt) in a fixed order; valid in all threads.
+// All loads of a thread are issued before they are combined (cnt <= 8 kBlock
+// in one round trip).
 __device__ __forceinline__ double reduce_partials_sum(const double* partials, int cnt, double* sh) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int i = threadIdx.x + u * kBlock;
+        v[u] = i < cnt ? load_partial(partials, i) : 0.0;
+    }
     double s = 0.0;
-    for (int i = threadIdx.x; i < cnt; i += kBlock) s += load_partial(partials, i);
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += v[u];
+    for (int i = threadIdx.x + 8 * kBlock; i < cnt; i += kBlock) s += load_partial(partials, i);
     return block_sum(s, sh);
 }
 
 __device__ __forceinline__ double reduce_partials_max(const double* partials, int cnt, double* sh) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int i = threadIdx.x + u * kBlock;
+        v[u] = i < cnt ? load_partial(partials, i) : 0.0;
+    }
     double s = 0.0;
-    for (int i = threadIdx.x; i < cnt; i += kBlock) s = fmax(s, load_partial(partials, i));
+#pragma unroll
+    for (int u = 0; u < 8; u++) s = fmax(s, v[u]);
+    for (int i = threadIdx.x + 8 * kBlock; i < cnt; i += kBlock) s = fmax(s, load_partial(partials, i));
     return block_max(s, sh);
+}
+
+// Both folds with the loads of the two arrays issued together.
+__device__ __forceinline__ void reduce_partials_sum_max(const double* ps, const double* pm, int cnt,
+                                                        double* sh, double& S, double& M) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int i = threadIdx.x + u * kBlock;
+        a[u] = i < cnt ? load_partial(ps, i) : 0.0;
+        b[u] = i < cnt ? load_partial(pm, i) : 0.0;
+    }
+    double s = 0.0, m = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        s += a[u];
+        m = fmax(m, b[u]);
+    }
+    for (int i = threadIdx.x + 4 * kBlock; i < cnt; i += kBlock) {
+        s += load_partial(ps, i);
+        m = fmax(m, load_partial(pm, i));
+    }
+    block_sum_max(s, m, sh);
+    S = s;
+    M = m;
 }
 
 }  // namespace maze

@@ -49,10 +49,10 @@ void launch_pred_sigma(hipStream_t s, const GridDims& d, const PBCoef& pb, const
                        double* sigma);
 template <class Real>
 void launch_cg_init(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* x,
-                    const Real* b, Real* r, CGState* st, Reduce red);
+                    const Real* b, Real* r, CGState* st, Reduce red, Real* zero_x = nullptr);
 template <class Real>
 void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p_old,
-                   const Real* r, Real* p_new, CGState* st, Reduce red);
+                   const Real* r, Real* p_new, CGState* st, Reduce red, int first);
 template <class Real>
 void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
                       Real* r, CGState* st, Reduce red);
@@ -94,6 +94,12 @@ void launch_mg_prolong_add(hipStream_t s, int nc, const float* ec, float* d, CGS
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st);
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
-                    float* p_new, CGState* st, Reduce red);
+                    float* p_new, CGState* st, Reduce red, int first);
+// Rest of the V-cycle from a small level (sizes n[0..nlev), finest first) down
+// to the exact solve and back, in one work-group with the data in LDS:
+// reads that level's right-hand side b, writes its correction d.
+bool mg_tail_fits(int nlev, const int* n);
+void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float* d, double c,
+                    const double* Qf, const double* ilam, CGState* st);
 
 }  // namespace maze

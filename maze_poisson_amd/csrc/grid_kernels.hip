@@ -1,13 +1,9 @@
 // grid_kernels.hip -- 7-point Poisson / linear Poisson-Boltzmann kernels for gfx950.
 //
-// Every grid pass is one i-marching stencil sweep: a work-group owns 256
-// consecutive points of an i-plane (coalesced along the contiguous plane) and
-// walks `slab` planes, keeping the i-1 / i / i+1 values of its column in
-// registers.  The value being differenced is produced on the fly by a source
-// functor (e.g. 2 phi - phi_prev, or beta p + r/6), so fused passes never
-// materialise intermediates.  Reductions are single-pass: per-work-group
-// partials plus a last-arriving work-group that folds them in index order
-// (deterministic) and advances the solver scalars in device memory.
+// Every grid pass is one i-marching stencil sweep (stencil.hpp) with a fused
+// body.  Reductions are single-pass: per-work-group partials plus a
+// last-arriving work-group that folds them in index order (deterministic) and
+// advances the solver scalars in device memory.
 //
 // Floating-point expressions follow the reference's operation order
 // (library/src/laplace.c:301-309, :366-379; library/src/plcg.c:41-83;
@@ -18,25 +14,32 @@
 namespace maze {
 
 // ---- bodies -----------------------------------------------------------------
+// A body declares the centre operands it needs (Raw, load) and combines them
+// with the source value and its stencil in apply; kReduce/kMax select the
+// (sum, max) reduction handed to finalize.
 struct BodyLaplace {
-    static constexpr bool kReduce = false, kSkipWhenDone = false;
+    static constexpr bool kReduce = false, kMax = false, kSkipWhenDone = false;
     double* out;
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double, double lap) { out[c] = lap; }
+    using Raw = NoRaw;
+    __device__ Raw load(int) const { return Raw{}; }
+    __device__ void apply(int c, const Raw&, double, double lap) { out[c] = lap; }
     __device__ static void finalize(CGState*, double, double) {}
 };
 
 // phi* and sigma = A phi* + coef q (library/src/verlet.c:221-230).
 struct BodyPred {
-    static constexpr bool kReduce = false, kSkipWhenDone = false;
+    static constexpr bool kReduce = false, kMax = false, kSkipWhenDone = false;
     double* phistar;
     double* sigma;
     const double* q;
     double coef;
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double v, double lap) {
+    using Raw = double;
+    __device__ Raw load(int c) const { return q[c]; }
+    __device__ void apply(int c, const Raw& qc, double v, double lap) {
         phistar[c] = v;
-        sigma[c] = lap + coef * q[c];
+        sigma[c] = lap + coef * qc;
     }
     __device__ static void finalize(CGState*, double, double) {}
 };
@@ -44,13 +47,15 @@ struct BodyPred {
 // r = A x - b; <r, r/6>; max|r|   (library/src/plcg.c:41-58)
 template <class Real>
 struct BodyInit {
-    static constexpr bool kReduce = true, kSkipWhenDone = false;
+    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = false;
     const Real* b;
     Real* r;
     Real* x_zero;  // when set, x is taken as 0 and written back as 0
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double, double lap) {
-        const Real rc = (Real)(lap + -1.0 * (double)b[c]);
+    using Raw = Real;
+    __device__ Raw load(int c) const { return b[c]; }
+    __device__ void apply(int c, const Raw& bc, double, double lap) {
+        const Real rc = (Real)(lap + -1.0 * (double)bc);
         r[c] = rc;
         if (x_zero) x_zero[c] = (Real)0;
         const double rd = (double)rc;
@@ -79,10 +84,12 @@ struct BodyInit {
 // p_new = beta p + r/6 (written), <p_new, A p_new>   (library/src/plcg.c:61-63)
 template <class Real>
 struct BodyDir {
-    static constexpr bool kReduce = true, kSkipWhenDone = true;
+    static constexpr bool kReduce = true, kMax = false, kSkipWhenDone = true;
     Real* p_new;
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double v, double lap) {
+    using Raw = NoRaw;
+    __device__ Raw load(int) const { return Raw{}; }
+    __device__ void apply(int c, const Raw&, double v, double lap) {
         p_new[c] = (Real)v;
         s += v * lap;
     }
@@ -91,7 +98,7 @@ struct BodyDir {
         st->first = 0;
         if (fabs(S) < 1e-300) {  // breakdown guard (library/src/multigrid.c:59-63)
             st->done = 1;
-            st->res = st->count_mode ? st->iter : st->iter;
+            st->res = st->iter;
             st->alpha = 0.0;
         } else {
             st->alpha = st->rdv / S;
@@ -103,14 +110,16 @@ struct BodyDir {
 // (library/src/plcg.c:64-80; relative stop of library/src/multigrid.c:74-78).
 template <class Real>
 struct BodyUpdate {
-    static constexpr bool kReduce = true, kSkipWhenDone = true;
+    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = true;
     Real* x;
     Real* r;
     double alpha;
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double v, double lap) {
-        x[c] = (Real)((double)x[c] + alpha * v);
-        const Real rn = (Real)((double)r[c] + alpha * lap);
+    struct Raw { Real x, r; };
+    __device__ Raw load(int c) const { return Raw{x[c], r[c]}; }
+    __device__ void apply(int c, const Raw& o, double v, double lap) {
+        x[c] = (Real)((double)o.x + alpha * v);
+        const Real rn = (Real)((double)o.r + alpha * lap);
         r[c] = rn;
         const double rd = (double)rn;
         s += rd * rd;
@@ -144,15 +153,18 @@ struct BodyUpdate {
 };
 
 // y_out = y + d; b32 = sigma - A y_out (FP64); max|b32|
+template <bool Add>
 struct BodyRefine {
-    static constexpr bool kReduce = true, kSkipWhenDone = false;
+    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = false;
     double* y_out;
     const double* sigma;
     float* b32;
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double v, double lap) {
-        if (y_out) y_out[c] = v;
-        const double rr = sigma[c] - lap;
+    using Raw = double;
+    __device__ Raw load(int c) const { return sigma[c]; }
+    __device__ void apply(int c, const Raw& sg, double v, double lap) {
+        if (Add) y_out[c] = v;
+        const double rr = sg - lap;
         b32[c] = (float)rr;
         m = fmax(m, fabs(rr));
     }
@@ -161,11 +173,13 @@ struct BodyRefine {
 
 // max|A y - sigma|  (library/src/verlet.c:300-302)
 struct BodyTrueRes {
-    static constexpr bool kReduce = true, kSkipWhenDone = false;
+    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = false;
     const double* sigma;
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double, double lap) {
-        const double t = lap + -1.0 * sigma[c];
+    using Raw = double;
+    __device__ Raw load(int c) const { return sigma[c]; }
+    __device__ void apply(int, const Raw& sg, double, double lap) {
+        const double t = lap + -1.0 * sg;
         m = fmax(m, fabs(t));
     }
     __device__ static void finalize(CGState* st, double, double M) { st->resid = M; }
@@ -187,36 +201,23 @@ void launch_pred_sigma(hipStream_t s, const GridDims& d, const PBCoef& pb, const
 
 template <class Real>
 void launch_cg_init(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* x,
-                    const Real* b, Real* r, CGState* st, Reduce red) {
+                    const Real* b, Real* r, CGState* st, Reduce red, Real* zero_x) {
     if (x) {
         BodyInit<Real> body{b, r, nullptr};
         DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcArr<Real>{x}, body, red, st));
     } else {
-        // x == nullptr means "start from zero"; the caller zeroes x separately.
-        BodyInit<Real> body{b, r, nullptr};
+        // x == nullptr means "start from zero": zero_x (if given) is cleared here.
+        BodyInit<Real> body{b, r, zero_x};
         DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcZero{}, body, red, st));
     }
 }
 
-// The direction/update passes read beta/alpha from device memory inside the
-// source functor; they are loaded once per work-group through these wrappers.
-template <class Real>
-struct SrcDirDev {
-    const Real* p;
-    const Real* r;
-    const CGState* st;
-    __device__ __forceinline__ double operator()(int i) const {
-        SrcDir<Real> f{p, r, st->beta, st->first};
-        return f(i);
-    }
-};
-
-template <bool PB, class Real>
+template <bool PB, class Real, bool First>
 __global__ __launch_bounds__(kBlock) void k_cg_dir(GridDims d, PBCoef pb, const Real* p_old,
                                                    const Real* r, Real* p_new, Reduce red,
                                                    CGState* st) {
     if (st->done) return;
-    const SrcDir<Real> src{p_old, r, st->beta, st->first};
+    const SrcDir<Real, First> src{p_old, r, First ? 0.0 : st->beta};
     BodyDir<Real> bd{p_new};
     march<PB>(d, pb, src, bd);
     __shared__ double sh[kBlock / kWave];
@@ -236,12 +237,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(GridDims d, PBCoef pb, con
     if (st->done) return;
     BodyUpdate<Real> bd{x, r, st->alpha};
     march<PB>(d, pb, SrcArr<Real>{p}, bd);
-    __shared__ double sh[kBlock / kWave];
-    const double S = block_sum(bd.s, sh);
-    const double M = block_max(bd.m, sh);
+    __shared__ double sh[2 * kBlock / kWave];
+    double S = bd.s, M = bd.m;
+    block_sum_max(S, M, sh);
     if (publish_arrive(red.psum, S, red.pmax, M, red.counter, gridDim.x)) {
-        const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
-        const double U = reduce_partials_max(red.pmax, gridDim.x, sh);
+        double T, U;
+        reduce_partials_sum_max(red.psum, red.pmax, gridDim.x, sh, T, U);
         if (threadIdx.x == 0) {
             BodyUpdate<Real>::finalize(st, T, U);
             reset_counter(red.counter);
@@ -249,11 +250,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(GridDims d, PBCoef pb, con
     }
 }
 
+// `first` is known to the host (the first direction of a solve is launched
+// first) and selects the kernel, so the direction formula has no branch.
 template <class Real>
 void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p_old,
-                   const Real* r, Real* p_new, CGState* st, Reduce red) {
-    DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_dir<kPB, Real>), dim3(d.blocks), dim3(kBlock), 0, s,
-                                       d, pb, p_old, r, p_new, red, st));
+                   const Real* r, Real* p_new, CGState* st, Reduce red, int first) {
+    if (first) {
+        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_dir<kPB, Real, true>), dim3(d.blocks),
+                                           dim3(kBlock), 0, s, d, pb, p_old, r, p_new, red, st));
+    } else {
+        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_dir<kPB, Real, false>), dim3(d.blocks),
+                                           dim3(kBlock), 0, s, d, pb, p_old, r, p_new, red, st));
+    }
 }
 
 template <class Real>
@@ -264,13 +272,13 @@ void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const 
 }
 
 template void launch_cg_init<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
-                                     const double*, double*, CGState*, Reduce);
+                                     const double*, double*, CGState*, Reduce, double*);
 template void launch_cg_init<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
-                                    const float*, float*, CGState*, Reduce);
+                                    const float*, float*, CGState*, Reduce, float*);
 template void launch_cg_dir<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
-                                    const double*, double*, CGState*, Reduce);
+                                    const double*, double*, CGState*, Reduce, int);
 template void launch_cg_dir<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
-                                   const float*, float*, CGState*, Reduce);
+                                   const float*, float*, CGState*, Reduce, int);
 template void launch_cg_update<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
                                        double*, double*, CGState*, Reduce);
 template void launch_cg_update<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
@@ -299,8 +307,13 @@ void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int li
 void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
                    const float* dcorr, const double* sigma, double* y_out, float* b32,
                    CGState* st, Reduce red, int add_d) {
-    BodyRefine body{add_d ? y_out : nullptr, sigma, b32};
-    DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcSum{y, dcorr, add_d}, body, red, st));
+    if (add_d) {
+        BodyRefine<true> body{y_out, sigma, b32};
+        DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcSum<true>{y, dcorr}, body, red, st));
+    } else {
+        BodyRefine<false> body{nullptr, sigma, b32};
+        DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcSum<false>{y, dcorr}, body, red, st));
+    }
 }
 
 void launch_true_residual_max(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
@@ -419,11 +432,13 @@ void launch_rbgs(hipStream_t s, const GridDims& d, const double* in, double* out
 
 // r = in - A out, as (A out)(-1) + 1 in (library/src/multigrid.c:142-144).
 struct BodyMgRes {
-    static constexpr bool kReduce = false, kSkipWhenDone = false;
+    static constexpr bool kReduce = false, kMax = false, kSkipWhenDone = false;
     const double* in;
     double* r;
     double s = 0.0, m = 0.0;
-    __device__ void operator()(int c, double, double lap) { r[c] = lap * -1.0 + 1.0 * in[c]; }
+    using Raw = double;
+    __device__ Raw load(int c) const { return in[c]; }
+    __device__ void apply(int c, const Raw& ic, double, double lap) { r[c] = lap * -1.0 + 1.0 * ic; }
     __device__ static void finalize(CGState*, double, double) {}
 };
 

@@ -242,12 +242,13 @@ hipEvent_t Sim::next_event() {
 
 // Accumulate the durations of the first `iters_done` iterations' kernels
 // (later launches were no-ops after the stop and are not counted).
-void Sim::prof_flush(int iters_done, int timer_base) {
+void Sim::prof_flush(int iters_done) {
+    if (!pending.empty()) sync();
     for (auto& pe : pending) {
         if (pe.iter >= iters_done) continue;
         float ms = 0.f;
         MAZE_HIP(hipEventElapsedTime(&ms, pe.a, pe.b));
-        KernelTimer& t = timers[timer_base + pe.timer];
+        KernelTimer& t = timers[pe.timer];
         t.total_ms += ms;
         t.count += 1;
     }
@@ -266,22 +267,12 @@ int Sim::cg_solve(const GridDims& d, const PBCoef& pbc_, const double* b, double
     int chunk = std::max(4, *pred + 1);
     double* pin = pa;
     double* pout = pbuf;
-    const bool timing = prof != 0;
     for (;;) {
         for (int c = 0; c < chunk && launched < limit; c++, launched++) {
-            if (timing) {
-                hipEvent_t e0 = next_event(), e1 = next_event(), e2 = next_event();
-                MAZE_HIP(hipEventRecord(e0, stream));
-                launch_cg_dir<double>(stream, d, pbc_, pin, rbuf, pout, st, red);
-                MAZE_HIP(hipEventRecord(e1, stream));
-                launch_cg_update<double>(stream, d, pbc_, pout, x, rbuf, st, red);
-                MAZE_HIP(hipEventRecord(e2, stream));
-                pending.push_back({0, e0, e1, launched});
-                pending.push_back({1, e1, e2, launched});
-            } else {
-                launch_cg_dir<double>(stream, d, pbc_, pin, rbuf, pout, st, red);
-                launch_cg_update<double>(stream, d, pbc_, pout, x, rbuf, st, red);
-            }
+            timed(kTCgDir, launched,
+                  [&] { launch_cg_dir<double>(stream, d, pbc_, pin, rbuf, pout, st, red, launched == 0); });
+            timed(kTCgUpdate, launched,
+                  [&] { launch_cg_update<double>(stream, d, pbc_, pout, x, rbuf, st, red); });
             std::swap(pin, pout);
         }
         read_state();
@@ -289,7 +280,7 @@ int Sim::cg_solve(const GridDims& d, const PBCoef& pbc_, const double* b, double
         chunk = std::max(4, chunk / 4);
     }
     const int done_iters = st_host->res >= 0 ? st_host->res + 1 : launched;
-    if (timing) prof_flush(done_iters, 0);
+    prof_flush(done_iters);
     *pred = done_iters;
     return st_host->res;
 }
@@ -306,7 +297,7 @@ int Sim::cg_count_solve(const GridDims& d, const double* b, double* x, double rt
     double* pout = pbuf;
     for (;;) {
         for (int c = 0; c < chunk && launched < maxit; c++, launched++) {
-            launch_cg_dir<double>(stream, d, PBCoef{}, pin, rbuf, pout, st, red);
+            launch_cg_dir<double>(stream, d, PBCoef{}, pin, rbuf, pout, st, red, launched == 0);
             launch_cg_update<double>(stream, d, PBCoef{}, pout, x, rbuf, st, red);
             std::swap(pin, pout);
         }
@@ -320,30 +311,20 @@ int Sim::cg_count_solve(const GridDims& d, const double* b, double* x, double rt
 
 // FP32-storage CG for the refinement correction A d = b (d starts at 0),
 // relative stop max|r| <= rtol max|b|; returns the iterations performed.
-int Sim::cg_inner_f32(const float* b, float* x, double rtol, int limit) {
-    launch_fill_f(stream, G, 0.0f, x);
+int Sim::cg_inner_f32(const float* b, float* x, double rtol, int limit, int slot) {
     launch_cg_reset(stream, st, 0.0, rtol, limit, 1);
-    launch_cg_init<float>(stream, dims, pbc(), nullptr, b, r32, st, red);
+    launch_cg_init<float>(stream, dims, pbc(), nullptr, b, r32, st, red, x);
     int launched = 0;
-    int chunk = std::max(4, last_inner + 1);
+    int& pred = inner_pred[std::min(slot, 7)];
+    int chunk = std::max(2, pred);
     float* pin = q32a;
     float* pout = q32b;
-    const bool timing = prof != 0;
     for (;;) {
         for (int c = 0; c < chunk && launched < limit; c++, launched++) {
-            if (timing) {
-                hipEvent_t e0 = next_event(), e1 = next_event(), e2 = next_event();
-                MAZE_HIP(hipEventRecord(e0, stream));
-                launch_cg_dir<float>(stream, dims, pbc(), pin, r32, pout, st, red);
-                MAZE_HIP(hipEventRecord(e1, stream));
-                launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red);
-                MAZE_HIP(hipEventRecord(e2, stream));
-                pending.push_back({0, e0, e1, launched});
-                pending.push_back({1, e1, e2, launched});
-            } else {
-                launch_cg_dir<float>(stream, dims, pbc(), pin, r32, pout, st, red);
-                launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red);
-            }
+            timed(kTCgDirF32, launched,
+                  [&] { launch_cg_dir<float>(stream, dims, pbc(), pin, r32, pout, st, red, launched == 0); });
+            timed(kTCgUpdateF32, launched,
+                  [&] { launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red); });
             std::swap(pin, pout);
         }
         read_state();
@@ -351,8 +332,8 @@ int Sim::cg_inner_f32(const float* b, float* x, double rtol, int limit) {
         chunk = std::max(4, chunk / 4);
     }
     const int it = st_host->res;
-    if (timing) prof_flush(it, 2);
-    last_inner = std::max(1, it);
+    prof_flush(it);
+    pred = std::max(1, it);
     return it;
 }
 
@@ -420,72 +401,92 @@ bool Sim::setup_mg() {
     mg_ilam = dalloc<double>(c * c * c);
     MAZE_HIP(hipMemcpy(mg_Qf, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(mg_ilam, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice));
+    // The one-work-group tail takes over from the first level below the
+    // finest whose remaining levels fit its LDS (<= 16^3); without such a
+    // level the exact coarsest solve runs as its own kernel.
+    mg_tail = (int)mg.size();
+    for (int l = 1; l < (int)mg.size(); l++) {
+        int ns[8];
+        const int nl = (int)mg.size() - l;
+        if (nl > 8) continue;
+        for (int k = 0; k < nl; k++) ns[k] = mg[l + k].n;
+        if (mg_tail_fits(nl, ns)) {
+            mg_tail = l;
+            break;
+        }
+    }
     mg_ok = 1;
     return true;
 }
 
 // V(2,2): pre-smooth, residual, restrict, recurse, prolong-correct, post-smooth.
-void Sim::vcycle(int l) {
+void Sim::vcycle(int l, int iter) {
     MGLevel& L = mg[l];
     const double cw = mg_omega / 6.0;
+    if (l >= mg_tail) {
+        int ns[8];
+        const int nl = (int)mg.size() - l;
+        for (int k = 0; k < nl; k++) ns[k] = mg[l + k].n;
+        launch_mg_tail(stream, nl, ns, L.b, L.d, cw, mg_Qf, mg_ilam, st);
+        return;
+    }
     if (l == (int)mg.size() - 1) {
         launch_mg_coarse(stream, mg_c, L.b, L.d, mg_Qf, mg_ilam, st);
         return;
     }
-    launch_mg_pre(stream, L.dims, L.b, L.d, cw, st);
-    launch_mg_resid(stream, L.dims, L.b, L.d, L.t, st);
+    const bool top = l == 0;
+    auto t = [&](int timer, auto&& f) {
+        if (top) timed(timer, iter, f);
+        else f();
+    };
+    t(kTMgPre, [&] { launch_mg_pre(stream, L.dims, L.b, L.d, cw, st); });
+    t(kTMgResid, [&] { launch_mg_resid(stream, L.dims, L.b, L.d, L.t, st); });
     launch_mg_restrict(stream, L.n, L.t, mg[l + 1].b, st);
-    vcycle(l + 1);
+    vcycle(l + 1, iter);
     launch_mg_prolong_add(stream, mg[l + 1].n, mg[l + 1].d, L.d, st);
-    launch_mg_post(stream, L.dims, L.b, L.d, L.t, cw, st, 0, red);
-    launch_mg_post(stream, L.dims, L.b, L.t, L.d, cw, st, l == 0, red);
+    t(kTMgPost, [&] { launch_mg_post(stream, L.dims, L.b, L.d, L.t, cw, st, 0, red); });
+    t(kTMgPostDot, [&] { launch_mg_post(stream, L.dims, L.b, L.t, L.d, cw, st, top, red); });
 }
 
 // Multigrid-preconditioned CG in FP32 storage for A x = b from x = 0, with the
 // explicit-preconditioner recurrence of library/src/plcg.c:104-166 and a
 // relative stop max|r| <= rtol max|b|; returns the iterations performed.
-int Sim::cg_inner_mg(const float* b, float* x, double rtol, int limit) {
-    launch_fill_f(stream, G, 0.0f, x);
+int Sim::cg_inner_mg(const float* b, float* x, double rtol, int limit, int slot) {
     launch_cg_reset(stream, st, 0.0, rtol, limit, 1, 1);
-    launch_cg_init<float>(stream, dims, PBCoef{}, nullptr, b, r32, st, red);
+    launch_cg_init<float>(stream, dims, PBCoef{}, nullptr, b, r32, st, red, x);
     mg[0].b = r32;
     int launched = 0;
-    int chunk = std::max(2, last_inner + 1);
+    // Launch as many iterations as this refinement pass needed last step, then
+    // poll; later chunks are small (a no-op iteration costs a dozen launches).
+    int& pred = inner_pred[std::min(slot, 7)];
+    int chunk = std::max(1, pred);
     float* pin = q32a;
     float* pout = q32b;
-    const bool timing = prof != 0;
     for (;;) {
         for (int c = 0; c < chunk && launched < limit; c++, launched++) {
-            vcycle(0);
-            if (timing) {
-                hipEvent_t e0 = next_event(), e1 = next_event(), e2 = next_event();
-                MAZE_HIP(hipEventRecord(e0, stream));
-                launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red);
-                MAZE_HIP(hipEventRecord(e1, stream));
+            vcycle(0, launched);
+            timed(kTCgDirF32, launched,
+                  [&] { launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red, launched == 0); });
+            timed(kTCgUpdateF32, launched, [&] {
                 launch_cg_update<float>(stream, dims, PBCoef{}, pout, x, r32, st, red);
-                MAZE_HIP(hipEventRecord(e2, stream));
-                pending.push_back({0, e0, e1, launched});
-                pending.push_back({1, e1, e2, launched});
-            } else {
-                launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red);
-                launch_cg_update<float>(stream, dims, PBCoef{}, pout, x, r32, st, red);
-            }
+            });
             std::swap(pin, pout);
         }
         read_state();
         if (st_host->done || launched >= limit) break;
-        chunk = std::max(2, chunk / 2);
+        chunk = 1;
     }
     const int it = st_host->res;
-    if (timing) prof_flush(it, 2);
-    last_inner = std::max(1, it);
+    prof_flush(it);
+    pred = std::max(1, it);
     return it;
 }
 
 // MaZe step in FP64, library/src/verlet.c:31-71.
 int Sim::maze_fp64() {
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
-    launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma);
+    timed(kTPred, 0,
+          [&] { launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma); });
     double* old_phip = phip;
     phip = phi;
     phi = tmp;
@@ -506,12 +507,14 @@ int Sim::maze_mixed() {
         q32a = dalloc<float>(G); q32b = dalloc<float>(G); y_alt = dalloc<double>(G);
     }
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
-    launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma);
+    timed(kTPred, 0,
+          [&] { launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma); });
     double* old_phip = phip;
     phip = phi;
     phi = tmp;
     tmp = old_phip;
-    launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 0);
+    timed(kTRefine, 0,
+          [&] { launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 0); });
     read_state();
     double resid = st_host->resid;
     int total = 0, outer = 0, res = -1;
@@ -523,10 +526,11 @@ int Sim::maze_mixed() {
         }
         if (outer >= 50) break;
         const bool use_mg = precision == kMixedMG && !pb && setup_mg();
-        const int it = use_mg ? cg_inner_mg(b32, d32, inner_rtol, inner_limit)
-                              : cg_inner_f32(b32, d32, inner_rtol, inner_limit);
+        const int it = use_mg ? cg_inner_mg(b32, d32, inner_rtol, inner_limit, outer)
+                              : cg_inner_f32(b32, d32, inner_rtol, inner_limit, outer);
         total += std::max(it, 0);
-        launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 1);
+        timed(kTRefine, 0,
+              [&] { launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 1); });
         std::swap(y, y_alt);
         read_state();
         resid = st_host->resid;
@@ -651,7 +655,7 @@ int Sim::update_field() {
 void Sim::forces_elec() { launch_force_fd(stream, p, phi); }
 
 double Sim::forces_noel(bool read_back) {
-    launch_tf(stream, p, ps, red);
+    timed(kTTf, 0, [&] { launch_tf(stream, p, ps, red); });
     if (!read_back) return 0.0;
     MAZE_HIP(hipMemcpyAsync(&ps_host->pe_tf, &ps->pe_tf, sizeof(double), hipMemcpyDeviceToHost,
                             stream));

@@ -25,6 +25,10 @@ struct MGLevel {
 };
 
 // Per-kernel timing collected with HIP events on the solver stream.
+enum TimerId {
+    kTCgDir = 0, kTCgUpdate, kTCgDirF32, kTCgUpdateF32, kTMgPre, kTMgResid, kTMgPost,
+    kTMgPostDot, kTRefine, kTPred, kTTf, kTCic
+};
 struct KernelTimer {
     const char* name;
     double total_ms = 0.0;
@@ -62,6 +66,7 @@ struct Sim {
     int mg_c = 0;
     double mg_omega = 6.0 / 7.0;
     int mg_ok = -1;             // -1 not set up, 0 unavailable for this n, 1 ready
+    int mg_tail = 0;            // first level handled by the one-work-group tail kernel
 
     CGState* st = nullptr;      // device
     CGState* st_host = nullptr; // pinned
@@ -71,6 +76,7 @@ struct Sim {
 
     int last_iters = 32;        // prediction for the chunked convergence polling
     int last_inner = 32;
+    int inner_pred[8] = {4, 4, 4, 4, 4, 4, 4, 4};  // iterations per refinement pass, last step
     int last_coarse = 16;
     int last_res = 0;
 
@@ -93,7 +99,25 @@ struct Sim {
     size_t ev_next = 0;
     struct Pending { int timer; hipEvent_t a, b; int iter; };
     std::vector<Pending> pending;
-    KernelTimer timers[4] = {{"cg_dir"}, {"cg_update"}, {"cg_dir_f32"}, {"cg_update_f32"}};
+    std::vector<KernelTimer> timers = {{"cg_dir"}, {"cg_update"}, {"cg_dir_f32"},
+                                       {"cg_update_f32"}, {"mg_pre_l0"}, {"mg_resid_l0"},
+                                       {"mg_post_l0"}, {"mg_post_dot_l0"}, {"refine"},
+                                       {"pred_sigma"}, {"tf_pairs"}, {"cic"}};
+    // Records HIP events around `launch` on the solver stream when profiling
+    // is on; `iter` tags solver iterations so that launches after the stop
+    // (no-ops) can be left out.
+    template <class F>
+    void timed(int timer, int iter, F&& launch) {
+        if (!prof) {
+            launch();
+            return;
+        }
+        hipEvent_t a = next_event(), b = next_event();
+        MAZE_HIP(hipEventRecord(a, stream));
+        launch();
+        MAZE_HIP(hipEventRecord(b, stream));
+        pending.push_back({timer, a, b, iter});
+    }
 
     Sim();
     ~Sim();
@@ -127,11 +151,11 @@ struct Sim {
                  int limit, double* rbuf, double* pa, double* pbuf, int* pred);
     int cg_count_solve(const GridDims& d, const double* b, double* x, double rtol, int maxit,
                        double* rbuf, double* pa, double* pbuf, int* pred);
-    int cg_inner_f32(const float* b, float* x, double rtol, int limit);
-    int cg_inner_mg(const float* b, float* x, double rtol, int limit);
+    int cg_inner_f32(const float* b, float* x, double rtol, int limit, int slot);
+    int cg_inner_mg(const float* b, float* x, double rtol, int limit, int slot);
     bool setup_mg();
     void free_mg();
-    void vcycle(int level);
+    void vcycle(int level, int iter);
     int maze_fp64();
     int maze_mixed();
     int lcg_update();
@@ -142,7 +166,7 @@ struct Sim {
     void sync() { MAZE_HIP(hipStreamSynchronize(stream)); }
     void read_state();
     hipEvent_t next_event();
-    void prof_flush(int iters_done, int timer_base);
+    void prof_flush(int iters_done);
     void free_grid();
     void free_particles();
 };
