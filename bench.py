@@ -40,8 +40,8 @@ PATHS = {"fp64": 0, "mixed_cg": 1, "mixed_mg": 2}
 BYTES_PER_POINT = {
     "cg_dir": 3 * 8, "cg_update": 5 * 8,                  # FP64 Jacobi CG
     "cg_dir_f32": 3 * 4, "cg_update_f32": 5 * 4,          # FP32 CG / PCG direction, update
-    "mg_pre_l0": 2 * 4, "mg_resid_l0": 3 * 4,             # finest multigrid level
-    "mg_post_l0": 3 * 4, "mg_post_dot_l0": 3 * 4,
+    "mg_down_l0": 4 + 4 + 0.5,                            # finest multigrid level: b -> d2, b_H
+    "mg_up_l0": 4 + 4 + 0.5 + 4,                          # b, d2, e -> d
     "refine": 8 + 4 + 8 + 8 + 4,                          # y, d, sigma -> y', b32
     "pred_sigma": 5 * 8,                                  # phi, phi_p, q -> phi*, sigma
 }
@@ -72,28 +72,37 @@ def start(lib, precision, tol_factor=1.0):
     return s
 
 
-def run_path(lib, precision, steps, warmup, profile):
-    """Bootstrap, warm up, then time `steps` MD steps one by one, each followed by a
-    device synchronisation (for the per-step latency distribution)."""
+def run_path(lib, precision, steps, warmup, lat_steps, prof_steps):
+    """Bootstrap and warm up, then three phases on the same trajectory:
+    1. `steps` MD steps back to back between two device synchronisations (throughput);
+    2. `lat_steps` steps each followed by a synchronisation (step latency p50/p99);
+    3. `prof_steps` steps with HIP events around the timed kernels (per-kernel
+       average durations; the events add gaps, so this phase is not timed)."""
     L = lib.lib
-    s = start(lib, precision)
+    start(lib, precision)
     for _ in range(warmup):
         lib.solver_md_loop_iter()
     lib.solver_sync()
-    L.maze_prof_reset()
-    L.maze_prof_enable(1 if profile else 0)
-    lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
+        lib.solver_md_loop_iter()
+    lib.solver_sync()
+    elapsed = time.perf_counter() - t0
+    lat = []
+    for _ in range(lat_steps):
         a = time.perf_counter()
         lib.solver_md_loop_iter()
         lib.solver_sync()
         lat.append(time.perf_counter() - a)
-    elapsed = time.perf_counter() - t0
+    L.maze_prof_reset()
+    L.maze_prof_enable(1 if prof_steps else 0)
+    for _ in range(prof_steps):
+        lib.solver_md_loop_iter()
+    lib.solver_sync()
     L.maze_prof_enable(0)
     prof = {k: prof_get(L, k) for k in BYTES_PER_POINT}
     lib.solver_finalize()
-    lat = np.array(lat) * 1e3
+    lat = np.array(lat) * 1e3 if lat else np.zeros(1)
     return {"steps_per_s": steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "prof": prof}
@@ -151,7 +160,10 @@ def cpu_baseline(steps, cores):
 def roofline(prof):
     """Dominant grid kernel (largest total event time) of the headline path."""
     G = N_GRID ** 3
-    best = max(((k, v) for k, v in prof.items() if v[1] > 0), key=lambda kv: kv[1][0])
+    timed = [(k, v) for k, v in prof.items() if v[1] > 0]
+    if not timed:
+        return None
+    best = max(timed, key=lambda kv: kv[1][0])
     name, (tot_ms, cnt) = best
     avg_s = tot_ms / cnt * 1e-3
     bpl = BYTES_PER_POINT[name] * G
@@ -168,6 +180,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline path only")
+    ap.add_argument("--prof-steps", type=int, default=30,
+                    help="steps run with per-kernel HIP events after the timed steps")
     ap.add_argument("--cpu-steps", type=int, default=15)
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--cpu-child", action="store_true")
@@ -188,11 +202,12 @@ def main():
         lib.lib.maze_calib_copy.argtypes = [__import__("ctypes").c_long]
         lib.lib.maze_calib_copy(32 << 20)
 
-    head = run_path(lib, PATHS["mixed_mg"], args.steps, args.warmup, True)
+    head = run_path(lib, PATHS["mixed_mg"], args.steps, args.warmup, args.steps, args.prof_steps)
     line_extra = {}
     if not args.no_extra:
-        fp64 = run_path(lib, PATHS["fp64"], max(10, args.steps // 2), args.warmup, True)
-        mcg = run_path(lib, PATHS["mixed_cg"], max(5, args.steps // 10), 2, False)
+        k = max(10, args.steps // 2)
+        fp64 = run_path(lib, PATHS["fp64"], k, args.warmup, k, args.prof_steps)
+        mcg = run_path(lib, PATHS["mixed_cg"], max(5, args.steps // 10), 2, 0, 0)
         # Force error of each path after 5 steps, against the FP64 solve converged
         # 1000x tighter from the same start.
         f_ref, _ = forces_after(lib, PATHS["fp64"], 1e-3, 5)

@@ -174,7 +174,8 @@ void solver_set_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
  * -1 for an unknown mode. */
 int solver_set_precision(int mode);
 int solver_get_precision(void);
-/* Relative residual reduction of each FP32 inner solve (default 1e-4). */
+/* Relative residual reduction of each FP32 inner solve; <= 0 restores the
+ * default of the mode (1e-4 for mode 1, 1e-2 for mode 2). */
 void solver_set_inner_rtol(double rtol);
 /* Mode 2 = FP32 multigrid-preconditioned CG inside FP64 iterative
  * refinement (V(2,2) damped Jacobi, exact FP64 coarsest solve); falls back
@@ -220,6 +221,15 @@ int maze_test_restrict27(const double *in, double *out, int n);
 int maze_test_prolong_add(const double *in, double *out, int nc);
 int maze_test_mg_vcycle(const double *in, double *out, int n);
 int maze_test_mg_pcg(const double *b, double *x, double rtol, int n);
+/* The fused FP32 multigrid level kernels (n even, c = damping / 6):
+ * down: d2 = two damped-Jacobi sweeps from zero on A d = b, bc = 4 x
+ *       full-weighting restriction of b - A d2 ((n/2)^3 values);
+ * up:   d = two damped-Jacobi sweeps on A d = b from d2 + trilinear(e). */
+int maze_test_mg_down(const float *b, float *d2, float *bc, int n, double c);
+int maze_test_mg_up(const float *b, const float *d2, const float *e, float *d, int n, double c);
+/* The one-work-group coarse tail of the V-cycle set up for an n^3 grid, on the
+ * first level it covers (its size is returned; -1 if the grid has no tail). */
+int maze_test_mg_tail(const float *b, float *d, int n);
 
 #ifdef __cplusplus
 }

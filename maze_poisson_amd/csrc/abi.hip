@@ -171,12 +171,16 @@ void solver_set_eps_k2(double* ex, double* ey, double* ez, double* k2) {
 int solver_set_precision(int mode) {
     if (mode < 0 || mode > 2) return -1;
     S().precision = mode;
+    S().sched.clear();
     return 0;
 }
 
 int solver_get_precision() { return S().precision; }
 
-void solver_set_inner_rtol(double rtol) { S().inner_rtol = rtol; }
+void solver_set_inner_rtol(double rtol) {
+    S().inner_rtol = rtol;
+    S().sched.clear();
+}
 
 // Damping of the multigrid Jacobi smoother (default 6/7).
 void solver_set_mg_omega(double omega) { S().mg_omega = omega; }
@@ -316,7 +320,6 @@ void maze_test_rbgs(const double* in, double* out, int n, int iters) {
 // returns the iteration count, or -1 when no multigrid hierarchy exists for n.
 int maze_test_mg_pcg(const double* b, double* x, double rtol, int n) {
     Sim* s = scratch_sim(n);
-    s->b32 = nullptr;
     s->precision = maze::kMixedMG;
     if (!s->setup_mg()) {
         delete s;
@@ -335,7 +338,7 @@ int maze_test_mg_pcg(const double* b, double* x, double rtol, int n) {
     s->r32 = dr;
     s->q32a = pa;
     s->q32b = pb;
-    const int it = s->cg_inner_mg(db, dx, rtol, n * n, 0);
+    const int it = s->inner_solve(true, db, dx, rtol, n * n, 1);
     MAZE_HIP(hipMemcpy(xf.data(), dx, G * sizeof(float), hipMemcpyDeviceToHost));
     for (long i = 0; i < G; i++) x[i] = (double)xf[i];
     s->r32 = s->q32a = s->q32b = nullptr;
@@ -365,6 +368,68 @@ int maze_test_prolong_add(const double* in, double* out, int nc) {
     MAZE_HIP(hipMemcpy(out, s->r, s->G * sizeof(double), hipMemcpyDeviceToHost));
     delete s;
     return 0;
+}
+
+// Fused multigrid level kernels on FP32 host buffers (n even).
+int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c) {
+    if (n < 2 || n % 2 != 0) return -1;
+    Sim* s = scratch_sim(n);
+    const long G = s->G, Gc = G / 8;
+    float *db = nullptr, *dd = nullptr, *dc = nullptr;
+    MAZE_HIP(hipMalloc(&db, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dd, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dc, Gc * sizeof(float)));
+    MAZE_HIP(hipMemcpy(db, b, G * sizeof(float), hipMemcpyHostToDevice));
+    maze::launch_mg_down(s->stream, n, db, dd, dc, c, s->st);
+    s->sync();
+    MAZE_HIP(hipMemcpy(d2, dd, G * sizeof(float), hipMemcpyDeviceToHost));
+    MAZE_HIP(hipMemcpy(bc, dc, Gc * sizeof(float), hipMemcpyDeviceToHost));
+    for (float* a : {db, dd, dc}) MAZE_HIP(hipFree(a));
+    delete s;
+    return 0;
+}
+
+int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, int n, double c) {
+    if (n < 2 || n % 2 != 0) return -1;
+    Sim* s = scratch_sim(n);
+    const long G = s->G, Gc = G / 8;
+    float *db = nullptr, *dd2 = nullptr, *de = nullptr, *dd = nullptr;
+    MAZE_HIP(hipMalloc(&db, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dd2, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&de, Gc * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dd, G * sizeof(float)));
+    MAZE_HIP(hipMemcpy(db, b, G * sizeof(float), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(dd2, d2, G * sizeof(float), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(de, e, Gc * sizeof(float), hipMemcpyHostToDevice));
+    maze::launch_mg_up(s->stream, n, db, dd2, de, dd, c, s->st, 0, s->red);
+    s->sync();
+    MAZE_HIP(hipMemcpy(d, dd, G * sizeof(float), hipMemcpyDeviceToHost));
+    for (float* a : {db, dd2, de, dd}) MAZE_HIP(hipFree(a));
+    delete s;
+    return 0;
+}
+
+// The one-work-group tail of the V-cycle as set up for an n^3 fine grid: from
+// the first level it takes over (returned, or -1 if there is none), on FP32
+// host buffers of that level's size.
+int maze_test_mg_tail(const float* b, float* d, int n) {
+    Sim* s = scratch_sim(n);
+    s->precision = maze::kMixedMG;
+    if (!s->setup_mg() || s->mg_tail >= (int)s->mg.size()) {
+        delete s;
+        return -1;
+    }
+    const int l = s->mg_tail, nl = (int)s->mg.size() - l;
+    int ns[8];
+    for (int k = 0; k < nl; k++) ns[k] = s->mg[l + k].n;
+    const long g = (long)ns[0] * ns[0] * ns[0];
+    MAZE_HIP(hipMemcpy(s->mg[l].b, b, g * sizeof(float), hipMemcpyHostToDevice));
+    maze::launch_mg_tail(s->stream, nl, ns, s->mg[l].b, s->mg[l].d, s->mg_omega / 6.0, s->mg_Qf,
+                         s->mg_ilam, s->st);
+    s->sync();
+    MAZE_HIP(hipMemcpy(d, s->mg[l].d, g * sizeof(float), hipMemcpyDeviceToHost));
+    delete s;
+    return ns[0];
 }
 
 int maze_test_mg_vcycle(const double* in, double* out, int n) {

@@ -85,7 +85,9 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Fixed-shape work-group reductions (deterministic: butterfly within each
-// wave, then the 4 wave results in wave order).  Result valid in all threads.
+// wave, then the NT/64 wave results in wave order).  Result valid in all
+// threads; sh holds NT/64 doubles.
+template <int NT = kBlock>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     v = wave_sum(v);
     const int w = threadIdx.x / kWave;
@@ -93,7 +95,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     if ((threadIdx.x & (kWave - 1)) == 0) sh[w] = v;
     __syncthreads();
     double s = 0.0;
-    for (int i = 0; i < kBlock / kWave; i++) s += sh[i];
+    for (int i = 0; i < NT / kWave; i++) s += sh[i];
     return s;
 }
 
@@ -164,20 +166,21 @@ __device__ __forceinline__ void reset_counter(unsigned int* counter) {
 }
 
 // Sum / max of partials[0..cnt) in a fixed order; valid in all threads.
-// All loads of a thread are issued before they are combined (cnt <= 8 kBlock
+// All loads of a thread are issued before they are combined (cnt <= 8 NT
 // in one round trip).
+template <int NT = kBlock>
 __device__ __forceinline__ double reduce_partials_sum(const double* partials, int cnt, double* sh) {
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-        const int i = threadIdx.x + u * kBlock;
+        const int i = threadIdx.x + u * NT;
         v[u] = i < cnt ? load_partial(partials, i) : 0.0;
     }
     double s = 0.0;
 #pragma unroll
     for (int u = 0; u < 8; u++) s += v[u];
-    for (int i = threadIdx.x + 8 * kBlock; i < cnt; i += kBlock) s += load_partial(partials, i);
-    return block_sum(s, sh);
+    for (int i = threadIdx.x + 8 * NT; i < cnt; i += NT) s += load_partial(partials, i);
+    return block_sum<NT>(s, sh);
 }
 
 __device__ __forceinline__ double reduce_partials_max(const double* partials, int cnt, double* sh) {

@@ -5,6 +5,8 @@
 
 namespace maze {
 
+constexpr int kMaxPasses = 16;  // refinement passes recorded per solve
+
 // Scalars of one Krylov solve, resident in device memory so that the
 // per-iteration kernels never wait for the host.  Written only by the last
 // work-group of each reduction pass.
@@ -24,7 +26,15 @@ struct CGState {
     int first;       // 1 until the first direction has been formed
     int count_mode;  // 1: result is the number of iterations (coarse CG)
     int pcg;         // 1: beta comes from the preconditioner pass, not from <r, r>
-    int pad;
+    // Outer FP64 refinement loop of the mixed-precision solve, advanced by the
+    // refinement pass so that a whole predicted schedule of passes can be
+    // queued without host round trips (passes after convergence are no-ops).
+    int outer_done;  // 1 once max|sigma - A y| <= otol
+    int ysel;        // which of the two y buffers holds the current iterate
+    int passes;      // refinement passes applied
+    int inner_total; // inner iterations summed over the applied passes
+    int pass_iters[kMaxPasses];  // per pass: iterations if converged, else -(iterations)
+    double otol;     // absolute stop of the outer loop
 };
 
 // Face-centred dielectric maps and screening term of the PB operator; the
@@ -53,16 +63,28 @@ void launch_cg_init(hipStream_t s, const GridDims& d, const PBCoef& pb, const Re
 template <class Real>
 void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p_old,
                    const Real* r, Real* p_new, CGState* st, Reduce red, int first);
+// zero_x: x is taken as 0 and only written (first update from a zero start).
 template <class Real>
 void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
-                      Real* r, CGState* st, Reduce red);
+                      Real* r, CGState* st, Reduce red, int zero_x = 0);
+// gated: the solve is started only while the outer refinement loop runs.
 void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit, int count_mode,
-                     int pcg = 0);
+                     int pcg = 0, int gated = 0);
 
-// Mixed precision: FP64 true residual b32 = sigma - A (y + d), y <- y + d.
-void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
-                   const float* dcorr, const double* sigma, double* y_out, float* b32,
-                   CGState* st, Reduce red, int add_d);
+// Mixed precision, FP64 true residual of the refinement loop, b = sigma - A y.
+// First pass (add_d == 0): y = y0, starts the loop (ysel = 0).
+// Later passes: y_new = y + d with y/y_new the two buffers y0/y1 selected by
+// st->ysel (flipped); skipped once the loop is done.  Each pass stops the
+// loop when max|b| <= otol, else starts the next inner solve A x = b from
+// x = 0 (r32 = -b in FP32, relative stop rtol, iteration cap limit, pcg as
+// in CGState).
+struct RefineArgs {
+    double otol, rtol;
+    int limit, pcg;
+};
+void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, double* y0, double* y1,
+                   const float* dcorr, const double* sigma, float* r32, RefineArgs a, CGState* st,
+                   Reduce red, int add_d);
 
 void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u);
 void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out);
@@ -95,6 +117,14 @@ void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const doub
                       const double* ilam, CGState* st);
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                     float* p_new, CGState* st, Reduce red, int first);
+// Fused level kernels of the V-cycle on an n^3 level (n even):
+// down: d2 = two pre-smoothing sweeps from zero, b_coarse = restricted residual;
+// up:   d = two post-smoothing sweeps of d2 + prolong(e); dot: also <b, d> and
+//       the PCG beta update (finest level).
+void launch_mg_down(hipStream_t s, int n, const float* b, float* d2, float* bc, double c,
+                    CGState* st);
+void launch_mg_up(hipStream_t s, int n, const float* b, const float* d2, const float* e,
+                  float* d, double c, CGState* st, int dot, Reduce red);
 // Rest of the V-cycle from a small level (sizes n[0..nlev), finest first) down
 // to the exact solve and back, in one work-group with the data in LDS:
 // reads that level's right-hand side b, writes its correction d.

@@ -47,7 +47,9 @@ struct BodyPred {
 // r = A x - b; <r, r/6>; max|r|   (library/src/plcg.c:41-58)
 template <class Real>
 struct BodyInit {
-    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = false;
+    // Skipped when the reset before it left the solve stopped (a gated solve
+    // after the outer loop has converged).
+    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = true;
     const Real* b;
     Real* r;
     Real* x_zero;  // when set, x is taken as 0 and written back as 0
@@ -108,7 +110,9 @@ struct BodyDir {
 
 // x += alpha p; r += alpha A p; <r, r>, max|r|; stop test and beta
 // (library/src/plcg.c:64-80; relative stop of library/src/multigrid.c:74-78).
-template <class Real>
+// ZeroX: x is taken as 0 (first update of a solve started from zero), so it
+// is written without being read.
+template <class Real, bool ZeroX = false>
 struct BodyUpdate {
     static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = true;
     Real* x;
@@ -116,7 +120,7 @@ struct BodyUpdate {
     double alpha;
     double s = 0.0, m = 0.0;
     struct Raw { Real x, r; };
-    __device__ Raw load(int c) const { return Raw{x[c], r[c]}; }
+    __device__ Raw load(int c) const { return Raw{ZeroX ? (Real)0 : x[c], r[c]}; }
     __device__ void apply(int c, const Raw& o, double v, double lap) {
         x[c] = (Real)((double)o.x + alpha * v);
         const Real rn = (Real)((double)o.r + alpha * lap);
@@ -152,23 +156,65 @@ struct BodyUpdate {
     }
 };
 
-// y_out = y + d; b32 = sigma - A y_out (FP64); max|b32|
+// y_out = y + d; the FP64 residual b = sigma - A y_out; max|b|.  Advances the
+// outer refinement loop and starts the next inner solve A x = b from x = 0:
+// r32 = A 0 - b = -b rounded to FP32 and <r, r/6>, as the init pass of
+// library/src/plcg.c:41-58 would compute them.
 template <bool Add>
 struct BodyRefine {
-    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = false;
     double* y_out;
     const double* sigma;
-    float* b32;
+    float* r32;
+    double otol, rtol;
+    int limit, pcg;
     double s = 0.0, m = 0.0;
     using Raw = double;
     __device__ Raw load(int c) const { return sigma[c]; }
     __device__ void apply(int c, const Raw& sg, double v, double lap) {
         if (Add) y_out[c] = v;
         const double rr = sg - lap;
-        b32[c] = (float)rr;
+        const float rf = (float)(-rr);
+        r32[c] = rf;
+        const double rd = (double)rf;
+        s += rd * (rd * (1.0 / 6.0));
         m = fmax(m, fabs(rr));
     }
-    __device__ static void finalize(CGState* st, double, double M) { st->resid = M; }
+    __device__ void finalize(CGState* st, double S, double M) const {
+        st->resid = M;
+        st->otol = otol;
+        if (Add) {
+            // The inner solve of this pass: its iteration count if it met its
+            // relative stop, else minus the iterations it was given.
+            const int it = st->done ? max(st->res, 0) : -st->iter;
+            const int k = st->passes;
+            if (k < kMaxPasses) st->pass_iters[k] = it;
+            st->inner_total += it >= 0 ? it : -it;
+            st->passes = k + 1;
+            st->ysel ^= 1;
+        } else {
+            st->ysel = 0;
+            st->passes = 0;
+            st->inner_total = 0;
+        }
+        st->outer_done = M <= otol ? 1 : 0;
+        if (st->outer_done) {
+            st->done = 1;
+            return;
+        }
+        st->tol = 0.0;
+        st->rtol = rtol;
+        st->limit = limit;
+        st->count_mode = 1;
+        st->pcg = pcg;
+        st->rdv = -S;
+        st->r0 = M;
+        st->iter = 0;
+        st->first = 1;
+        st->alpha = 0.0;
+        st->beta = 0.0;
+        st->done = 0;
+        st->res = -1;
+    }
 };
 
 // max|A y - sigma|  (library/src/verlet.c:300-302)
@@ -220,34 +266,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(GridDims d, PBCoef pb, const 
     const SrcDir<Real, First> src{p_old, r, First ? 0.0 : st->beta};
     BodyDir<Real> bd{p_new};
     march<PB>(d, pb, src, bd);
-    __shared__ double sh[kBlock / kWave];
-    const double S = block_sum(bd.s, sh);
-    if (publish_arrive(red.psum, S, nullptr, 0.0, red.counter, gridDim.x)) {
-        const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            BodyDir<Real>::finalize(st, T, 0.0);
-            reset_counter(red.counter);
-        }
-    }
+    reduce_and_finalize<false>(bd, red, st);
 }
 
-template <bool PB, class Real>
+template <bool PB, class Real, bool ZeroX>
 __global__ __launch_bounds__(kBlock) void k_cg_update(GridDims d, PBCoef pb, const Real* p,
                                                       Real* x, Real* r, Reduce red, CGState* st) {
     if (st->done) return;
-    BodyUpdate<Real> bd{x, r, st->alpha};
+    BodyUpdate<Real, ZeroX> bd{x, r, st->alpha};
     march<PB>(d, pb, SrcArr<Real>{p}, bd);
-    __shared__ double sh[2 * kBlock / kWave];
-    double S = bd.s, M = bd.m;
-    block_sum_max(S, M, sh);
-    if (publish_arrive(red.psum, S, red.pmax, M, red.counter, gridDim.x)) {
-        double T, U;
-        reduce_partials_sum_max(red.psum, red.pmax, gridDim.x, sh, T, U);
-        if (threadIdx.x == 0) {
-            BodyUpdate<Real>::finalize(st, T, U);
-            reset_counter(red.counter);
-        }
-    }
+    reduce_and_finalize<true>(bd, red, st);
 }
 
 // `first` is known to the host (the first direction of a solve is launched
@@ -266,9 +294,14 @@ void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Rea
 
 template <class Real>
 void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
-                      Real* r, CGState* st, Reduce red) {
-    DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_update<kPB, Real>), dim3(d.blocks), dim3(kBlock), 0,
-                                       s, d, pb, p, x, r, red, st));
+                      Real* r, CGState* st, Reduce red, int zero_x) {
+    if (zero_x) {
+        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_update<kPB, Real, true>), dim3(d.blocks),
+                                           dim3(kBlock), 0, s, d, pb, p, x, r, red, st));
+    } else {
+        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_update<kPB, Real, false>), dim3(d.blocks),
+                                           dim3(kBlock), 0, s, d, pb, p, x, r, red, st));
+    }
 }
 
 template void launch_cg_init<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
@@ -280,13 +313,17 @@ template void launch_cg_dir<double>(hipStream_t, const GridDims&, const PBCoef&,
 template void launch_cg_dir<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
                                    const float*, float*, CGState*, Reduce, int);
 template void launch_cg_update<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
-                                       double*, double*, CGState*, Reduce);
+                                       double*, double*, CGState*, Reduce, int);
 template void launch_cg_update<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
-                                      float*, float*, CGState*, Reduce);
+                                      float*, float*, CGState*, Reduce, int);
 
 __global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int count_mode,
-                           int pcg) {
+                           int pcg, int gated) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (gated && st->outer_done) {
+            st->done = 1;
+            return;
+        }
         st->pcg = pcg;
         st->tol = tol;
         st->rtol = rtol;
@@ -300,19 +337,41 @@ __global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int 
 }
 
 void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit,
-                     int count_mode, int pcg) {
-    hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode, pcg);
+                     int count_mode, int pcg, int gated) {
+    hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode, pcg,
+                       gated);
 }
 
-void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
-                   const float* dcorr, const double* sigma, double* y_out, float* b32,
-                   CGState* st, Reduce red, int add_d) {
+// The y buffers are chosen on the device (st->ysel), so a queued pass needs
+// no host round trip; a pass after the outer loop converged returns at once.
+// An inner solve that applied no update (stopped before its first one) left
+// d untouched, so d then counts as zero.
+template <bool PB, bool Add>
+__global__ __launch_bounds__(kBlock) void k_refine(GridDims d, PBCoef pb, double* y0, double* y1,
+                                                   const float* dc, const double* sigma,
+                                                   float* r32, RefineArgs a, Reduce red,
+                                                   CGState* st) {
+    int sel = 0;
+    double dscale = 0.0;
+    if (Add) {
+        if (st->outer_done) return;
+        sel = st->ysel;
+        dscale = (st->done ? st->res : st->iter) > 0 ? 1.0 : 0.0;
+    }
+    BodyRefine<Add> bd{sel ? y0 : y1, sigma, r32, a.otol, a.rtol, a.limit, a.pcg};
+    march<PB>(d, pb, SrcSum<Add>{sel ? y1 : y0, dc, dscale}, bd);
+    reduce_and_finalize<true>(bd, red, st);
+}
+
+void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, double* y0, double* y1,
+                   const float* dcorr, const double* sigma, float* r32, RefineArgs a, CGState* st,
+                   Reduce red, int add_d) {
     if (add_d) {
-        BodyRefine<true> body{y_out, sigma, b32};
-        DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcSum<true>{y, dcorr}, body, red, st));
+        DISPATCH_PB(pb, hipLaunchKernelGGL((k_refine<kPB, true>), dim3(d.blocks), dim3(kBlock), 0,
+                                           s, d, pb, y0, y1, dcorr, sigma, r32, a, red, st));
     } else {
-        BodyRefine<false> body{nullptr, sigma, b32};
-        DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcSum<false>{y, dcorr}, body, red, st));
+        DISPATCH_PB(pb, hipLaunchKernelGGL((k_refine<kPB, false>), dim3(d.blocks), dim3(kBlock), 0,
+                                           s, d, pb, y0, y1, dcorr, sigma, r32, a, red, st));
     }
 }
 

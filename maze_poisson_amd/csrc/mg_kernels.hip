@@ -185,6 +185,256 @@ void launch_mg_prolong_add(hipStream_t s, int nc, const float* ec, float* d, CGS
                        d, st);
 }
 
+// ---- fused, LDS-tiled level kernels ------------------------------------------
+// Each work-group owns a (TI, TJ, TK) tile of a fine level (even sizes) and
+// loads its operands once, with the halo the fused steps need, into LDS; the
+// steps then run out of LDS with redundant halo arithmetic (neighbouring tiles
+// compute the same halo values from the same inputs).  Arithmetic is FP32,
+// as is the storage: the cycle only preconditions the FP32 inner solve.
+//
+// down: d2 = two damped-Jacobi sweeps from zero on A d = b, which is the
+//       stencil d2 = -2c b - c^2 A b; written for the owned points; then
+//       b_H = 4 * full-weighting(b - A d2) for the tile's coarse points.
+//       Halo: b on [-3, T+2), d2 on [-2, T+1), residual on [-1, T).
+// up:   d = d2 + trilinear(e), then two sweeps d <- d - c (b - A d); the
+//       finest level also forms <b, d> (PCG) and the beta update.
+//       Halo: d on [-2, T+2), b and the first sweep on [-1, T+1), e on the
+//       coarse points [T0/2 - 1, T0/2 + T/2 + 2).
+
+__device__ __forceinline__ int wrap_mod(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// Calls f(x) for x = threadIdx.x + u NT < N (NT threads), every iteration
+// unrolled (N is a compile-time constant) so that the loads of all
+// iterations are in flight together.
+template <int N, int NT, class F>
+__device__ __forceinline__ void tile_for(F&& f) {
+#pragma unroll
+    for (int u = 0; u < (N + NT - 1) / NT; u++) {
+        const int x = (int)threadIdx.x + u * NT;
+        if (x < N) f(x);
+    }
+}
+
+template <int TI, int TJ, int TK>
+struct MgTileMap {
+    int i0, j0, k0;
+    __device__ MgTileMap(int n) {
+        const int ntj = (n + TJ - 1) / TJ, ntk = (n + TK - 1) / TK;
+        const int t = xcd_tile(blockIdx.x, gridDim.x);
+        const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
+        i0 = ti * TI;
+        j0 = tj * TJ;
+        k0 = tk * TK;
+    }
+};
+
+template <int TI, int TJ, int TK>
+int mg_tiles(int n) {
+    return ((n + TI - 1) / TI) * ((n + TJ - 1) / TJ) * ((n + TK - 1) / TK);
+}
+
+template <int TI, int TJ, int TK, int NT>
+__global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__ b,
+                                                    float* __restrict__ d2out,
+                                                    float* __restrict__ bcoarse, float c,
+                                                    const CGState* st) {
+    if (st->done) return;
+    constexpr int BI = TI + 5, BJ = TJ + 5, BK = TK + 5;
+    constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
+    constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
+    constexpr int CI = TI / 2, CJ = TJ / 2, CK = TK / 2;
+    __shared__ float sb[BI * BJ * BK];
+    __shared__ float sd[DI * DJ * DK];
+    __shared__ float sr[RI * RJ * RK];
+    __shared__ int gi[BI], gj[BJ], gk[BK];
+    const MgTileMap<TI, TJ, TK> T(n);
+    const int tid = threadIdx.x;
+    const int n2 = n * n;
+    if (tid < BI) gi[tid] = wrap_mod(T.i0 - 3 + tid, n) * n2;
+    if (tid >= 64 && tid < 64 + BJ) gj[tid - 64] = wrap_mod(T.j0 - 3 + tid - 64, n) * n;
+    if (tid >= 128 && tid < 128 + BK) gk[tid - 128] = wrap_mod(T.k0 - 3 + tid - 128, n);
+    __syncthreads();
+    tile_for<BI * BJ * BK, NT>([&](int x) {
+        const int k = x % BK, j = (x / BK) % BJ, i = x / (BK * BJ);
+        sb[x] = b[gi[i] + gj[j] + gk[k]];
+    });
+    __syncthreads();
+    const float c2 = c * c, tc = 2.0f * c;
+    tile_for<DI * DJ * DK, NT>([&](int x) {
+        const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
+        const int o = ((i + 1) * BJ + (j + 1)) * BK + (k + 1);
+        const float bc = sb[o];
+        const float lap = sb[o - BJ * BK] + sb[o + BJ * BK] + sb[o - BK] + sb[o + BK] + sb[o - 1] +
+                          sb[o + 1] - 6.0f * bc;
+        sd[x] = -tc * bc - c2 * lap;
+    });
+    __syncthreads();
+    tile_for<RI * RJ * RK, NT>([&](int x) {
+        const int k = x % RK, j = (x / RK) % RJ, i = x / (RK * RJ);
+        const int o = ((i + 1) * DJ + (j + 1)) * DK + (k + 1);
+        const float dc = sd[o];
+        const float lap = sd[o - DJ * DK] + sd[o + DJ * DK] + sd[o - DK] + sd[o + DK] + sd[o - 1] +
+                          sd[o + 1] - 6.0f * dc;
+        sr[x] = sb[((i + 2) * BJ + (j + 2)) * BK + (k + 2)] - lap;
+    });
+    // Owned d2 (this loop and the residual pass only read sd).
+    tile_for<TI * TJ * TK, NT>([&](int x) {
+        const int k = x % TK, j = (x / TK) % TJ, i = x / (TK * TJ);
+        if (T.i0 + i < n && T.j0 + j < n && T.k0 + k < n)
+            d2out[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] =
+                sd[((i + 2) * DJ + (j + 2)) * DK + (k + 2)];
+    });
+    __syncthreads();
+    const int nc = n / 2;
+    tile_for<CI * CJ * CK, NT>([&](int x) {
+        const int K = x % CK, J = (x / CK) % CJ, I = x / (CK * CJ);
+        const int gI = T.i0 / 2 + I, gJ = T.j0 / 2 + J, gK = T.k0 / 2 + K;
+        if (gI >= nc || gJ >= nc || gK >= nc) return;
+        const int o = ((2 * I + 1) * RJ + (2 * J + 1)) * RK + (2 * K + 1);
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = -1; a <= 1; a++)
+#pragma unroll
+            for (int bb = -1; bb <= 1; bb++)
+#pragma unroll
+                for (int cc = -1; cc <= 1; cc++) {
+                    const float w = (float)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
+                    acc += w * sr[o + (a * RJ + bb) * RK + cc];
+                }
+        bcoarse[(gI * nc + gJ) * nc + gK] = acc * (4.0f / 64.0f);
+    });
+}
+
+template <int TI, int TJ, int TK, int NT, bool Dot>
+__global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b,
+                                                  const float* __restrict__ d2,
+                                                  const float* __restrict__ e,
+                                                  float* __restrict__ dout, float c, Reduce red,
+                                                  CGState* st) {
+    if (st->done) return;
+    constexpr int DI = TI + 4, DJ = TJ + 4, DK = TK + 4;
+    constexpr int PI = TI + 2, PJ = TJ + 2, PK = TK + 2;
+    constexpr int EI = TI / 2 + 3, EJ = TJ / 2 + 3, EK = TK / 2 + 3;
+    __shared__ float sd[DI * DJ * DK];
+    __shared__ float sp[PI * PJ * PK];
+    __shared__ float sb[PI * PJ * PK];
+    __shared__ float se[EI * EJ * EK];
+    __shared__ int gi[DI], gj[DJ], gk[DK], ci[EI], cj[EJ], ck[EK];
+    const MgTileMap<TI, TJ, TK> T(n);
+    const int tid = threadIdx.x;
+    const int n2 = n * n, nc = n / 2;
+    if (tid < DI) gi[tid] = wrap_mod(T.i0 - 2 + tid, n) * n2;
+    if (tid >= 32 && tid < 32 + DJ) gj[tid - 32] = wrap_mod(T.j0 - 2 + tid - 32, n) * n;
+    if (tid >= 64 && tid < 64 + DK) gk[tid - 64] = wrap_mod(T.k0 - 2 + tid - 64, n);
+    if (tid >= 128 && tid < 128 + EI) ci[tid - 128] = wrap_mod(T.i0 / 2 - 1 + tid - 128, nc) * nc * nc;
+    if (tid >= 160 && tid < 160 + EJ) cj[tid - 160] = wrap_mod(T.j0 / 2 - 1 + tid - 160, nc) * nc;
+    if (tid >= 192 && tid < 192 + EK) ck[tid - 192] = wrap_mod(T.k0 / 2 - 1 + tid - 192, nc);
+    __syncthreads();
+    tile_for<DI * DJ * DK, NT>([&](int x) {
+        const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
+        sd[x] = d2[gi[i] + gj[j] + gk[k]];
+    });
+    tile_for<PI * PJ * PK, NT>([&](int x) {
+        const int k = x % PK, j = (x / PK) % PJ, i = x / (PK * PJ);
+        sb[x] = b[gi[i + 1] + gj[j + 1] + gk[k + 1]];
+    });
+    tile_for<EI * EJ * EK, NT>([&](int x) {
+        const int k = x % EK, j = (x / EK) % EJ, i = x / (EK * EJ);
+        se[x] = e[ci[i] + cj[j] + ck[k]];
+    });
+    __syncthreads();
+    // d = d2 + trilinear(e): fine local index l has coarse neighbours l>>1 and
+    // (l>>1)+1 and weight 1/2 on odd l (tile origins are even).
+    tile_for<DI * DJ * DK, NT>([&](int x) {
+        const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
+        const int a = i >> 1, bb = j >> 1, cc = k >> 1;
+        const float wi = (i & 1) ? 0.5f : 0.0f, wj = (j & 1) ? 0.5f : 0.0f,
+                    wk = (k & 1) ? 0.5f : 0.0f;
+        const int o = (a * EJ + bb) * EK + cc;
+        const float e000 = se[o], e001 = se[o + 1], e010 = se[o + EK], e011 = se[o + EK + 1];
+        const float e100 = se[o + EJ * EK], e101 = se[o + EJ * EK + 1];
+        const float e110 = se[o + EJ * EK + EK], e111 = se[o + EJ * EK + EK + 1];
+        const float v00 = e000 + wk * (e001 - e000);
+        const float v01 = e010 + wk * (e011 - e010);
+        const float v10 = e100 + wk * (e101 - e100);
+        const float v11 = e110 + wk * (e111 - e110);
+        const float v0 = v00 + wj * (v01 - v00);
+        const float v1 = v10 + wj * (v11 - v10);
+        sd[x] = sd[x] + (v0 + wi * (v1 - v0));
+    });
+    __syncthreads();
+    tile_for<PI * PJ * PK, NT>([&](int x) {
+        const int k = x % PK, j = (x / PK) % PJ, i = x / (PK * PJ);
+        const int o = ((i + 1) * DJ + (j + 1)) * DK + (k + 1);
+        const float dc = sd[o];
+        const float lap = sd[o - DJ * DK] + sd[o + DJ * DK] + sd[o - DK] + sd[o + DK] + sd[o - 1] +
+                          sd[o + 1] - 6.0f * dc;
+        sp[x] = dc - c * (sb[x] - lap);
+    });
+    __syncthreads();
+    double s = 0.0;
+    tile_for<TI * TJ * TK, NT>([&](int x) {
+        const int k = x % TK, j = (x / TK) % TJ, i = x / (TK * TJ);
+        const int o = ((i + 1) * PJ + (j + 1)) * PK + (k + 1);
+        const float pc = sp[o];
+        const float lap = sp[o - PJ * PK] + sp[o + PJ * PK] + sp[o - PK] + sp[o + PK] + sp[o - 1] +
+                          sp[o + 1] - 6.0f * pc;
+        const float bc = sb[o];
+        const float z = pc - c * (bc - lap);
+        if (T.i0 + i < n && T.j0 + j < n && T.k0 + k < n) {
+            dout[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] = z;
+            if (Dot) s += (double)bc * (double)z;
+        }
+    });
+    if constexpr (Dot) {
+        struct {
+            double s, m;
+            __device__ void finalize(CGState* q, double S, double) const {
+                q->beta = q->first ? 0.0 : S / q->rdv;
+                q->rdv = S;
+            }
+        } bd{s, 0.0};
+        reduce_and_finalize<false, decltype(bd), NT>(bd, red, st);
+    }
+}
+
+// Tiles of 8x16x16 fine points with 1024 threads (a tile's steps are short
+// dependent chains, so a work-group needs many waves); small levels get
+// 4x8x8 tiles with 512 threads (more work-groups for few points).
+constexpr int kBigNT = 1024, kSmallNT = 512;
+
+void launch_mg_down(hipStream_t s, int n, const float* b, float* d2, float* bc, double c,
+                    CGState* st) {
+    if (n >= 64)
+        hipLaunchKernelGGL((k_mg_down<8, 16, 16, kBigNT>), dim3(mg_tiles<8, 16, 16>(n)),
+                           dim3(kBigNT), 0, s, n, b, d2, bc, (float)c, st);
+    else
+        hipLaunchKernelGGL((k_mg_down<4, 8, 8, kSmallNT>), dim3(mg_tiles<4, 8, 8>(n)),
+                           dim3(kSmallNT), 0, s, n, b, d2, bc, (float)c, st);
+}
+
+void launch_mg_up(hipStream_t s, int n, const float* b, const float* d2, const float* e,
+                  float* d, double c, CGState* st, int dot, Reduce red) {
+    if (n >= 64) {
+        if (dot)
+            hipLaunchKernelGGL((k_mg_up<8, 16, 16, kBigNT, true>), dim3(mg_tiles<8, 16, 16>(n)),
+                               dim3(kBigNT), 0, s, n, b, d2, e, d, (float)c, red, st);
+        else
+            hipLaunchKernelGGL((k_mg_up<8, 16, 16, kBigNT, false>), dim3(mg_tiles<8, 16, 16>(n)),
+                               dim3(kBigNT), 0, s, n, b, d2, e, d, (float)c, red, st);
+    } else {
+        if (dot)
+            hipLaunchKernelGGL((k_mg_up<4, 8, 8, kSmallNT, true>), dim3(mg_tiles<4, 8, 8>(n)),
+                               dim3(kSmallNT), 0, s, n, b, d2, e, d, (float)c, red, st);
+        else
+            hipLaunchKernelGGL((k_mg_up<4, 8, 8, kSmallNT, false>), dim3(mg_tiles<4, 8, 8>(n)),
+                               dim3(kSmallNT), 0, s, n, b, d2, e, d, (float)c, red, st);
+    }
+}
+
 // ---- exact FP64 solve on the coarsest grid -----------------------------------
 // A e = b on a c^3 grid (c <= 12): forward transform along k, j, i with the
 // real orthonormal eigenbasis Qf (row t = eigenvector t), divide by the
@@ -264,11 +514,16 @@ void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const doub
 // ---- the coarse tail of the V-cycle in one work-group --------------------------
 // Levels of at most 16^3 points are launch-latency bound when each pass is its
 // own kernel.  k_mg_tail copies the right-hand side of the first such level
-// into LDS and runs the rest of the cycle there -- pre-smooth, residual and
-// restriction down to the exact FP64 solve, prolongation and post-smoothing
-// back up -- then writes that level's correction back; barriers separate the
-// passes.  LDS holds b, d, t of every level (at most 3 * (16^3 + 8^3 + ...)
-// floats); the exact solve's two double arrays reuse the first level's t.
+// into LDS and runs the rest of the cycle there -- the same pre-smoothing,
+// residual and restriction down to the exact FP64 solve, prolongation and
+// post-smoothing back up (FP32, as the fused level kernels) -- then writes
+// that level's correction back; barriers separate the passes.  LDS holds b, d,
+// t of every level (at most 3 * (16^3 + 8^3 + ...) floats); the exact solve's
+// two double arrays reuse the first level's t.
+//
+// Thread (ti, tj, tk) = (tid >> 8, (tid >> 4) & 15, tid & 15) owns the (j, k)
+// column (tj, tk) of every level (when tj, tk < m) at planes ti, ti + 4, ...,
+// so neighbour offsets are set once per level and no index is divided.
 constexpr int kTailThreads = 1024;
 constexpr int kTailMaxLevels = 6;
 constexpr int kTailMaxPts = 4096;  // 16^3
@@ -279,19 +534,27 @@ struct TailLevels {
     int n[kTailMaxLevels];
 };
 
-__device__ __forceinline__ void nb6(int x, int m, int* o) {
-    const int m2 = m * m;
-    const int k = x % m, j = (x / m) % m, i = x / m2;
-    o[0] = (i == 0 ? x + (m - 1) * m2 : x - m2);
-    o[1] = (i == m - 1 ? x - (m - 1) * m2 : x + m2);
-    o[2] = (j == 0 ? x + (m - 1) * m : x - m);
-    o[3] = (j == m - 1 ? x - (m - 1) * m : x + m);
-    o[4] = (k == 0 ? x + m - 1 : x - 1);
-    o[5] = (k == m - 1 ? x - m + 1 : x + 1);
-}
+// Periodic neighbours of column (j, k) on an m^3 level: row offsets jm, j0, jp
+// (times m) and k indices km, k0, kp.
+struct TailCol {
+    int jm, j0, jp, km, k0, kp;
+    __device__ TailCol(int j, int k, int m) {
+        jm = (j == 0 ? m - 1 : j - 1) * m;
+        j0 = j * m;
+        jp = (j == m - 1 ? 0 : j + 1) * m;
+        km = k == 0 ? m - 1 : k - 1;
+        k0 = k;
+        kp = k == m - 1 ? 0 : k + 1;
+    }
+    // A u at plane offsets (im, i0, ip) (times m^2) for this column.
+    __device__ float lap(const float* u, int im, int i0, int ip) const {
+        return u[im + j0 + k0] + u[ip + j0 + k0] + u[i0 + jm + k0] + u[i0 + jp + k0] +
+               u[i0 + j0 + km] + u[i0 + j0 + kp] - 6.0f * u[i0 + j0 + k0];
+    }
+};
 
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const float* __restrict__ bin,
-                                                         float* __restrict__ dout, double c,
+                                                         float* __restrict__ dout, float c,
                                                          const double* __restrict__ Qf,
                                                          const double* __restrict__ ilam,
                                                          const CGState* st) {
@@ -309,51 +572,62 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
         off += 3 * g;
     }
     const int tid = threadIdx.x;
+    const int tk = tid & 15, tj = (tid >> 4) & 15, ti = tid >> 8;
     {
-        const int g0 = T.n[0] * T.n[0] * T.n[0];
-        for (int x = tid; x < g0; x += kTailThreads) B[0][x] = bin[x];
+        const int m = T.n[0], m2 = m * m;
+        if (tj < m && tk < m)
+            for (int i = ti; i < m; i += 4) {
+                const int x = i * m2 + tj * m + tk;
+                B[0][x] = bin[x];
+            }
     }
     __syncthreads();
-    int o[6];
-    // ---- down: pre-smooth, residual, restrict ----
+    const float c2 = c * c, tc = 2.0f * c;
+    // ---- down: d2 = -2c b - c^2 A b, t = b - A d2, restrict t ----
     for (int l = 0; l < T.nlev - 1; l++) {
-        const int m = T.n[l], G = m * m * m;
+        const int m = T.n[l], m2 = m * m;
         const float* b = B[l];
         float* d = D[l];
         float* t = R[l];
-        for (int x = tid; x < G; x += kTailThreads) {
-            nb6(x, m, o);
-            const double d1 = (double)(float)(-c * (double)b[x]);
-            double lap = -6.0 * d1;
-            for (int q = 0; q < 6; q++) lap += (double)(float)(-c * (double)b[o[q]]);
-            d[x] = (float)(d1 - c * ((double)b[x] - lap));
-        }
-        __syncthreads();
-        for (int x = tid; x < G; x += kTailThreads) {
-            nb6(x, m, o);
-            double lap = -6.0 * (double)d[x];
-            for (int q = 0; q < 6; q++) lap += (double)d[o[q]];
-            t[x] = (float)((double)b[x] - lap);
-        }
-        __syncthreads();
-        const int M = m / 2, Gc = M * M * M;
-        float* bc = B[l + 1];
-        for (int x = tid; x < Gc; x += kTailThreads) {
-            const int K = x % M, J = (x / M) % M, I = x / (M * M);
-            const int i = 2 * I, j = 2 * J, k = 2 * K;
-            double acc = 0.0;
-            for (int a = -1; a <= 1; a++) {
-                const int ii = (i + a + m) % m;
-                for (int bb = -1; bb <= 1; bb++) {
-                    const int jj = (j + bb + m) % m;
-                    for (int cc = -1; cc <= 1; cc++) {
-                        const int kk = (k + cc + m) % m;
-                        const double w = (double)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
-                        acc += w * (double)t[(ii * m + jj) * m + kk];
-                    }
-                }
+        const bool on = tj < m && tk < m;
+        const TailCol col(on ? tj : 0, on ? tk : 0, m);
+        if (on)
+            for (int i = ti; i < m; i += 4) {
+                const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
+                          ip = (i == m - 1 ? 0 : i + 1) * m2;
+                const int x = i0 + col.j0 + col.k0;
+                d[x] = -tc * b[x] - c2 * col.lap(b, im, i0, ip);
             }
-            bc[x] = (float)(acc * (4.0 / 64.0));
+        __syncthreads();
+        if (on)
+            for (int i = ti; i < m; i += 4) {
+                const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
+                          ip = (i == m - 1 ? 0 : i + 1) * m2;
+                const int x = i0 + col.j0 + col.k0;
+                t[x] = b[x] - col.lap(d, im, i0, ip);
+            }
+        __syncthreads();
+        const int M = m / 2;
+        float* bc = B[l + 1];
+        if (tj < M && tk < M) {
+            const int j = 2 * tj, k = 2 * tk;
+            const int jj[3] = {(j == 0 ? m - 1 : j - 1) * m, j * m, (j + 1) * m};
+            const int kk[3] = {k == 0 ? m - 1 : k - 1, k, k + 1};
+            for (int I = ti; I < M; I += 4) {
+                const int i = 2 * I;
+                const int ii[3] = {(i == 0 ? m - 1 : i - 1) * m2, i * m2, (i + 1) * m2};
+                float acc = 0.0f;
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+                        for (int cc = 0; cc < 3; cc++) {
+                            const float w = (float)((a == 1 ? 2 : 1) * (bb == 1 ? 2 : 1) * (cc == 1 ? 2 : 1));
+                            acc += w * t[ii[a] + jj[bb] + kk[cc]];
+                        }
+                bc[(I * M + tj) * M + tk] = acc * (4.0f / 64.0f);
+            }
         }
         __syncthreads();
     }
@@ -361,13 +635,12 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
     {
         const int L = T.nlev - 1, cn = T.n[L], N = cn * cn * cn;
         double* A;
-        double* Bd;
         if (T.nlev >= 2) {
             A = reinterpret_cast<double*>(R[0]);  // 4 m0^3 bytes >= 16 (m0/2)^3
         } else {
             A = pool + N;  // after b, d of the only level (2N floats = N doubles)
         }
-        Bd = A + N;
+        double* Bd = A + N;
         for (int x = tid; x < cn * cn; x += kTailThreads) Q[x] = Qf[x];
         for (int x = tid; x < N; x += kTailThreads) A[x] = (double)B[L][x];
         __syncthreads();
@@ -375,49 +648,59 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
         for (int x = tid; x < N; x += kTailThreads) D[L][x] = (float)A[x];
         __syncthreads();
     }
-    // ---- up: prolong-correct, two post-smoothing sweeps ----
+    // ---- up: d += trilinear(e), two sweeps d <- d - c (b - A d) ----
     for (int l = T.nlev - 2; l >= 0; l--) {
-        const int m = T.n[l], G = m * m * m, nc = T.n[l + 1];
+        const int m = T.n[l], m2 = m * m, nc = T.n[l + 1];
         const float* b = B[l];
         float* d = D[l];
         float* t = R[l];
         const float* e = D[l + 1];
-        const int M2 = nc * nc;
-        for (int x = tid; x < G; x += kTailThreads) {
-            const int k = x % m, j = (x / m) % m, i = x / (m * m);
-            const int a0 = i >> 1, a1 = (a0 + 1) % nc;
-            const int b0 = j >> 1, b1 = (b0 + 1) % nc;
-            const int c0 = k >> 1, c1 = (c0 + 1) % nc;
-            const double wi = (i & 1) ? 0.5 : 0.0, wj = (j & 1) ? 0.5 : 0.0, wk = (k & 1) ? 0.5 : 0.0;
-#define C(a, bb, cc) (double)e[(a) * M2 + (bb) * nc + (cc)]
-            const double v00 = C(a0, b0, c0) + wk * (C(a0, b0, c1) - C(a0, b0, c0));
-            const double v01 = C(a0, b1, c0) + wk * (C(a0, b1, c1) - C(a0, b1, c0));
-            const double v10 = C(a1, b0, c0) + wk * (C(a1, b0, c1) - C(a1, b0, c0));
-            const double v11 = C(a1, b1, c0) + wk * (C(a1, b1, c1) - C(a1, b1, c0));
-#undef C
-            const double v0 = v00 + wj * (v01 - v00);
-            const double v1 = v10 + wj * (v11 - v10);
-            d[x] = (float)((double)d[x] + v0 + wi * (v1 - v0));
+        const bool on = tj < m && tk < m;
+        const TailCol col(on ? tj : 0, on ? tk : 0, m);
+        if (on) {
+            const int b0 = tj >> 1, b1 = (b0 + 1 == nc) ? 0 : b0 + 1;
+            const int c0 = tk >> 1, c1 = (c0 + 1 == nc) ? 0 : c0 + 1;
+            const float wj = (tj & 1) ? 0.5f : 0.0f, wk = (tk & 1) ? 0.5f : 0.0f;
+            for (int i = ti; i < m; i += 4) {
+                const int a0 = i >> 1, a1 = (a0 + 1 == nc) ? 0 : a0 + 1;
+                const float wi = (i & 1) ? 0.5f : 0.0f;
+                const float* e0 = e + a0 * nc * nc;
+                const float* e1 = e + a1 * nc * nc;
+                const float v00 = e0[b0 * nc + c0] + wk * (e0[b0 * nc + c1] - e0[b0 * nc + c0]);
+                const float v01 = e0[b1 * nc + c0] + wk * (e0[b1 * nc + c1] - e0[b1 * nc + c0]);
+                const float v10 = e1[b0 * nc + c0] + wk * (e1[b0 * nc + c1] - e1[b0 * nc + c0]);
+                const float v11 = e1[b1 * nc + c0] + wk * (e1[b1 * nc + c1] - e1[b1 * nc + c0]);
+                const float v0 = v00 + wj * (v01 - v00);
+                const float v1 = v10 + wj * (v11 - v10);
+                const int x = i * m2 + col.j0 + col.k0;
+                d[x] = d[x] + (v0 + wi * (v1 - v0));
+            }
         }
         __syncthreads();
-        for (int x = tid; x < G; x += kTailThreads) {
-            nb6(x, m, o);
-            double lap = -6.0 * (double)d[x];
-            for (int q = 0; q < 6; q++) lap += (double)d[o[q]];
-            t[x] = (float)((double)d[x] - c * ((double)b[x] - lap));
-        }
+        if (on)
+            for (int i = ti; i < m; i += 4) {
+                const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
+                          ip = (i == m - 1 ? 0 : i + 1) * m2;
+                const int x = i0 + col.j0 + col.k0;
+                t[x] = d[x] - c * (b[x] - col.lap(d, im, i0, ip));
+            }
         __syncthreads();
-        for (int x = tid; x < G; x += kTailThreads) {
-            nb6(x, m, o);
-            double lap = -6.0 * (double)t[x];
-            for (int q = 0; q < 6; q++) lap += (double)t[o[q]];
-            d[x] = (float)((double)t[x] - c * ((double)b[x] - lap));
-        }
+        if (on)
+            for (int i = ti; i < m; i += 4) {
+                const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
+                          ip = (i == m - 1 ? 0 : i + 1) * m2;
+                const int x = i0 + col.j0 + col.k0;
+                d[x] = t[x] - c * (b[x] - col.lap(t, im, i0, ip));
+            }
         __syncthreads();
     }
     {
-        const int g0 = T.n[0] * T.n[0] * T.n[0];
-        for (int x = tid; x < g0; x += kTailThreads) dout[x] = D[0][x];
+        const int m = T.n[0], m2 = m * m;
+        if (tj < m && tk < m)
+            for (int i = ti; i < m; i += 4) {
+                const int x = i * m2 + tj * m + tk;
+                dout[x] = D[0][x];
+            }
     }
 }
 
@@ -436,7 +719,8 @@ void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float
     TailLevels T;
     T.nlev = nlev;
     for (int l = 0; l < nlev; l++) T.n[l] = n[l];
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s, T, b, d, c, Qf, ilam, st);
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s, T, b, d, (float)c, Qf, ilam,
+                       st);
 }
 
 // PCG direction p = -v (first) or beta p - v, with <p, A p>

@@ -81,8 +81,9 @@ void Sim::free_grid() {
     for (double** a : {&q, &y, &phi, &phip, &tmp, &sigma, &r, &p0, &p1, &y_alt, &ex, &ey, &ez, &k2,
                        &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1})
         dfree(*a);
-    for (float** a : {&b32, &d32, &r32, &q32a, &q32b}) dfree(*a);
+    for (float** a : {&d32, &r32, &q32a, &q32b}) dfree(*a);
     free_mg();
+    sched.clear();
     G = 0;
 }
 
@@ -256,6 +257,27 @@ void Sim::prof_flush(int iters_done) {
     ev_next = 0;
 }
 
+// As prof_flush for a refinement loop: launches tagged with pass k count if
+// that pass was applied and, for inner iterations, if the iteration ran
+// (iters[k] iterations if converged, -iters[k] if stopped by its budget).
+void Sim::prof_flush_passes(const int* iters, int npass) {
+    if (!pending.empty()) sync();
+    for (auto& pe : pending) {
+        if (pe.pass >= 0) {
+            if (pe.pass >= npass) continue;
+            const int done = iters[pe.pass] >= 0 ? iters[pe.pass] : -iters[pe.pass];
+            if (pe.iter >= std::max(done, 1)) continue;
+        }
+        float ms = 0.f;
+        MAZE_HIP(hipEventElapsedTime(&ms, pe.a, pe.b));
+        KernelTimer& t = timers[pe.timer];
+        t.total_ms += ms;
+        t.count += 1;
+    }
+    pending.clear();
+    ev_next = 0;
+}
+
 // Implicit-Jacobi CG, library/src/plcg.c:28-90: x holds the warm start.
 // Iterations are launched in chunks sized from the previous solve and the
 // device-side stop flag is polled between chunks.
@@ -309,31 +331,50 @@ int Sim::cg_count_solve(const GridDims& d, const double* b, double* x, double rt
     return st_host->res;
 }
 
-// FP32-storage CG for the refinement correction A d = b (d starts at 0),
-// relative stop max|r| <= rtol max|b|; returns the iterations performed.
-int Sim::cg_inner_f32(const float* b, float* x, double rtol, int limit, int slot) {
-    launch_cg_reset(stream, st, 0.0, rtol, limit, 1);
-    launch_cg_init<float>(stream, dims, pbc(), nullptr, b, r32, st, red, x);
+// One iteration of the FP32-storage inner solve (Jacobi-CG, or CG with the
+// multigrid preconditioner of Sim::vcycle); `it` is its 0-based index.
+void Sim::inner_iter(bool use_mg, float* x, int it) {
+    float* pin = (it & 1) ? q32b : q32a;
+    float* pout = (it & 1) ? q32a : q32b;
+    if (use_mg) {
+        vcycle(0, it);
+        timed(kTCgDirF32, it,
+              [&] { launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red, it == 0); });
+        timed(kTCgUpdateF32, it, [&] {
+            launch_cg_update<float>(stream, dims, PBCoef{}, pout, x, r32, st, red, it == 0);
+        });
+    } else {
+        timed(kTCgDirF32, it,
+              [&] { launch_cg_dir<float>(stream, dims, pbc(), pin, r32, pout, st, red, it == 0); });
+        timed(kTCgUpdateF32, it,
+              [&] { launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red, it == 0); });
+    }
+}
+
+// Start of an inner solve A x = b from x = 0 with the relative stop
+// max|r| <= rtol max|b| (library/src/multigrid.c:74-78); the multigrid
+// variant uses the explicit-preconditioner recurrence of
+// library/src/plcg.c:104-166.  gated: skipped once the outer loop is done.
+void Sim::inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated) {
+    launch_cg_reset(stream, st, 0.0, rtol, limit, 1, use_mg ? 1 : 0, gated);
+    launch_cg_init<float>(stream, dims, use_mg ? PBCoef{} : pbc(), nullptr, b, r32, st, red, x);
+    if (use_mg) mg[0].b = r32;
+}
+
+// Complete inner solve, polling the device after every iteration past the
+// first `first_chunk`; returns the iterations performed.
+int Sim::inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit,
+                     int first_chunk) {
+    inner_start(use_mg, b, x, rtol, limit, 0);
     int launched = 0;
-    int& pred = inner_pred[std::min(slot, 7)];
-    int chunk = std::max(2, pred);
-    float* pin = q32a;
-    float* pout = q32b;
-    for (;;) {
-        for (int c = 0; c < chunk && launched < limit; c++, launched++) {
-            timed(kTCgDirF32, launched,
-                  [&] { launch_cg_dir<float>(stream, dims, pbc(), pin, r32, pout, st, red, launched == 0); });
-            timed(kTCgUpdateF32, launched,
-                  [&] { launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red); });
-            std::swap(pin, pout);
-        }
+    for (; launched < std::min(first_chunk, limit); launched++) inner_iter(use_mg, x, launched);
+    read_state();
+    while (!st_host->done && launched < limit) {
+        inner_iter(use_mg, x, launched++);
         read_state();
-        if (st_host->done || launched >= limit) break;
-        chunk = std::max(4, chunk / 4);
     }
     const int it = st_host->res;
     prof_flush(it);
-    pred = std::max(1, it);
     return it;
 }
 
@@ -419,7 +460,8 @@ bool Sim::setup_mg() {
     return true;
 }
 
-// V(2,2): pre-smooth, residual, restrict, recurse, prolong-correct, post-smooth.
+// V(2,2): pre-smooth, residual, restrict, recurse, prolong-correct, post-smooth,
+// as one fused down and one fused up kernel per level above the tail.
 void Sim::vcycle(int l, int iter) {
     MGLevel& L = mg[l];
     const double cw = mg_omega / 6.0;
@@ -439,47 +481,9 @@ void Sim::vcycle(int l, int iter) {
         if (top) timed(timer, iter, f);
         else f();
     };
-    t(kTMgPre, [&] { launch_mg_pre(stream, L.dims, L.b, L.d, cw, st); });
-    t(kTMgResid, [&] { launch_mg_resid(stream, L.dims, L.b, L.d, L.t, st); });
-    launch_mg_restrict(stream, L.n, L.t, mg[l + 1].b, st);
+    t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.b, L.t, mg[l + 1].b, cw, st); });
     vcycle(l + 1, iter);
-    launch_mg_prolong_add(stream, mg[l + 1].n, mg[l + 1].d, L.d, st);
-    t(kTMgPost, [&] { launch_mg_post(stream, L.dims, L.b, L.d, L.t, cw, st, 0, red); });
-    t(kTMgPostDot, [&] { launch_mg_post(stream, L.dims, L.b, L.t, L.d, cw, st, top, red); });
-}
-
-// Multigrid-preconditioned CG in FP32 storage for A x = b from x = 0, with the
-// explicit-preconditioner recurrence of library/src/plcg.c:104-166 and a
-// relative stop max|r| <= rtol max|b|; returns the iterations performed.
-int Sim::cg_inner_mg(const float* b, float* x, double rtol, int limit, int slot) {
-    launch_cg_reset(stream, st, 0.0, rtol, limit, 1, 1);
-    launch_cg_init<float>(stream, dims, PBCoef{}, nullptr, b, r32, st, red, x);
-    mg[0].b = r32;
-    int launched = 0;
-    // Launch as many iterations as this refinement pass needed last step, then
-    // poll; later chunks are small (a no-op iteration costs a dozen launches).
-    int& pred = inner_pred[std::min(slot, 7)];
-    int chunk = std::max(1, pred);
-    float* pin = q32a;
-    float* pout = q32b;
-    for (;;) {
-        for (int c = 0; c < chunk && launched < limit; c++, launched++) {
-            vcycle(0, launched);
-            timed(kTCgDirF32, launched,
-                  [&] { launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red, launched == 0); });
-            timed(kTCgUpdateF32, launched, [&] {
-                launch_cg_update<float>(stream, dims, PBCoef{}, pout, x, r32, st, red);
-            });
-            std::swap(pin, pout);
-        }
-        read_state();
-        if (st_host->done || launched >= limit) break;
-        chunk = 1;
-    }
-    const int it = st_host->res;
-    prof_flush(it);
-    pred = std::max(1, it);
-    return it;
+    t(kTMgUp, [&] { launch_mg_up(stream, L.n, L.b, L.t, mg[l + 1].d, L.d, cw, st, top, red); });
 }
 
 // MaZe step in FP64, library/src/verlet.c:31-71.
@@ -496,16 +500,22 @@ int Sim::maze_fp64() {
     return res;
 }
 
-// MaZe step with FP32-storage CG inside FP64 iterative refinement: the
-// outer loop recomputes the true residual sigma - A y in FP64 and stops on
-// the reference's criterion max|A y - sigma| <= tol; each inner solve is an
-// FP32 CG to a relative reduction inner_rtol.  Returns the total number of
-// inner iterations, or -1 if the outer loop did not converge.
+// MaZe step with FP32-storage inner solves inside FP64 iterative refinement:
+// each refinement pass recomputes the true residual sigma - A y in FP64, and
+// the loop stops on the reference's criterion max|A y - sigma| <= tol; each
+// inner solve reduces its residual by inner_rtol.  The passes of the previous
+// step (with their inner iteration counts) are queued without waiting: the
+// device advances the loop and turns passes after convergence into no-ops,
+// so a step normally costs one host round trip.  Further passes, if needed,
+// are driven from the host.  Returns the total number of inner iterations, or
+// -1 if the outer loop did not converge in kMaxOuter passes.
 int Sim::maze_mixed() {
-    if (!b32) {
-        b32 = dalloc<float>(G); d32 = dalloc<float>(G); r32 = dalloc<float>(G);
+    if (!d32) {
+        d32 = dalloc<float>(G); r32 = dalloc<float>(G);
         q32a = dalloc<float>(G); q32b = dalloc<float>(G); y_alt = dalloc<double>(G);
     }
+    const bool use_mg = precision == kMixedMG && !pb && setup_mg();
+    if (use_mg) mg[0].b = r32;
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
     timed(kTPred, 0,
           [&] { launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma); });
@@ -513,30 +523,46 @@ int Sim::maze_mixed() {
     phip = phi;
     phi = tmp;
     tmp = old_phip;
-    timed(kTRefine, 0,
-          [&] { launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 0); });
-    read_state();
-    double resid = st_host->resid;
-    int total = 0, outer = 0, res = -1;
     const int inner_limit = n * n;
-    while (true) {
-        if (resid <= tol) {
-            res = total;
-            break;
+    const RefineArgs ra{tol, inner_rtol > 0 ? inner_rtol : (use_mg ? 1e-2 : 1e-4), inner_limit,
+                        use_mg ? 1 : 0};
+    // Each refinement pass also starts the next pass's inner solve.
+    auto refine = [&](int add) {
+        timed(kTRefine, 0, [&] {
+            launch_refine(stream, dims, pbc(), y, y_alt, d32, sigma, r32, ra, st, red, add);
+        });
+    };
+    refine(0);
+    auto queue_pass = [&](int k, int m, bool poll) {
+        cur_pass = k;
+        int launched = 0;
+        for (; launched < std::min(m, inner_limit); launched++) inner_iter(use_mg, d32, launched);
+        if (poll) {
+            read_state();
+            while (!st_host->done && launched < inner_limit) {
+                inner_iter(use_mg, d32, launched++);
+                read_state();
+            }
         }
-        if (outer >= 50) break;
-        const bool use_mg = precision == kMixedMG && !pb && setup_mg();
-        const int it = use_mg ? cg_inner_mg(b32, d32, inner_rtol, inner_limit, outer)
-                              : cg_inner_f32(b32, d32, inner_rtol, inner_limit, outer);
-        total += std::max(it, 0);
-        timed(kTRefine, 0,
-              [&] { launch_refine(stream, dims, pbc(), y, d32, sigma, y_alt, b32, st, red, 1); });
-        std::swap(y, y_alt);
-        read_state();
-        resid = st_host->resid;
-        outer++;
+        refine(1);
+        cur_pass = -1;
+    };
+    int queued = 0;
+    for (; queued < (int)sched.size(); queued++) {
+        const int m = sched[queued] >= 0 ? std::max(sched[queued], 1) : 1 - sched[queued];
+        queue_pass(queued, m, false);
     }
+    read_state();
+    while (!st_host->outer_done && st_host->passes < kMaxOuter) {
+        queue_pass(queued++, 1, true);
+        read_state();
+    }
+    const int passes = std::min(st_host->passes, kMaxPasses);
+    sched.assign(st_host->pass_iters, st_host->pass_iters + passes);
+    prof_flush_passes(st_host->pass_iters, passes);
+    if (st_host->ysel) std::swap(y, y_alt);
     launch_axpy(stream, G, -1.0, y, phi);
+    const int res = st_host->outer_done ? st_host->inner_total : -1;
     last_res = res;
     return res;
 }
@@ -601,8 +627,9 @@ int Sim::mg_update(bool maze) {
     return res;
 }
 
-// library/src/solver_md.c:91-107.
-int Sim::update_charges() {
+// library/src/solver_md.c:91-107.  defer: the charge-conservation check
+// is made by check_charge() after the next host round trip instead of here.
+int Sim::update_charges(bool defer) {
     launch_cic_base(stream, p);
     launch_cic_sort(stream, p, key_bits);
     launch_fill(stream, G, 0.0, q);
@@ -610,7 +637,12 @@ int Sim::update_charges() {
     launch_contrib_sum(stream, p, ps, red);
     MAZE_HIP(hipMemcpyAsync(&ps_host->q_sum, &ps->q_sum, sizeof(double), hipMemcpyDeviceToHost,
                             stream));
+    if (defer) return 0;
     sync();
+    return check_charge();
+}
+
+int Sim::check_charge() {
     const double loc = ps_host->q_sum;
     if (std::fabs(q_tot - loc) > 1e-6) {
         std::printf("Charge conservation error: q_tot = %.6f, q_tot_loc = %.6f\n", q_tot, loc);
@@ -689,8 +721,10 @@ void Sim::part2() {
 // One MD step in the order of maze_poisson/solver.py:483-492.
 void Sim::md_step() {
     part1();
-    update_charges();
+    update_charges(true);
     update_field();
+    sync();
+    check_charge();
     forces_elec();
     forces_noel(false);
     forces_tot();

@@ -26,8 +26,8 @@ struct MGLevel {
 
 // Per-kernel timing collected with HIP events on the solver stream.
 enum TimerId {
-    kTCgDir = 0, kTCgUpdate, kTCgDirF32, kTCgUpdateF32, kTMgPre, kTMgResid, kTMgPost,
-    kTMgPostDot, kTRefine, kTPred, kTTf, kTCic
+    kTCgDir = 0, kTCgUpdate, kTCgDirF32, kTCgUpdateF32, kTMgDown, kTMgUp, kTRefine, kTPred, kTTf,
+    kTCic
 };
 struct KernelTimer {
     const char* name;
@@ -44,7 +44,11 @@ struct Sim {
     double L = 0, h = 0, tol = 0, eps_s = 1, eps_int = 1;
     int grid_type = kMazeLCG, precond = kPNone;
     int precision = kFP64;
-    double inner_rtol = 1e-4;
+    // Relative residual reduction of each inner solve; <= 0 selects the
+    // default of the mode (1e-4 for Jacobi-CG, 1e-2 for the multigrid-
+    // preconditioned CG, which converges fast enough that short inner solves
+    // and more FP64 passes cost less).
+    double inner_rtol = 0.0;
     int pb = 0;
     double w = 0, kbar2 = 0;
     long G = 0;
@@ -52,7 +56,7 @@ struct Sim {
     double *q = nullptr, *y = nullptr, *phi = nullptr, *phip = nullptr, *tmp = nullptr;
     double *sigma = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *y_alt = nullptr;
     double *ex = nullptr, *ey = nullptr, *ez = nullptr, *k2 = nullptr;
-    float *b32 = nullptr, *d32 = nullptr, *r32 = nullptr, *q32a = nullptr, *q32b = nullptr;
+    float *d32 = nullptr, *r32 = nullptr, *q32a = nullptr, *q32b = nullptr;
     // reference multigrid coarse level
     int nc = 0;
     GridDims cdims{};
@@ -76,7 +80,10 @@ struct Sim {
 
     int last_iters = 32;        // prediction for the chunked convergence polling
     int last_inner = 32;
-    int inner_pred[8] = {4, 4, 4, 4, 4, 4, 4, 4};  // iterations per refinement pass, last step
+    // Inner iterations of each refinement pass of the last mixed solve
+    // (negative: the pass stopped on its iteration budget, not its tolerance).
+    std::vector<int> sched;
+    static constexpr int kMaxOuter = 50;  // refinement passes per solve
     int last_coarse = 16;
     int last_res = 0;
 
@@ -97,12 +104,13 @@ struct Sim {
     int prof = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_next = 0;
-    struct Pending { int timer; hipEvent_t a, b; int iter; };
+    struct Pending { int timer; hipEvent_t a, b; int iter, pass; };
+    int cur_pass = -1;  // refinement pass being queued (-1: none)
     std::vector<Pending> pending;
-    std::vector<KernelTimer> timers = {{"cg_dir"}, {"cg_update"}, {"cg_dir_f32"},
-                                       {"cg_update_f32"}, {"mg_pre_l0"}, {"mg_resid_l0"},
-                                       {"mg_post_l0"}, {"mg_post_dot_l0"}, {"refine"},
-                                       {"pred_sigma"}, {"tf_pairs"}, {"cic"}};
+    std::vector<KernelTimer> timers = {{"cg_dir"},     {"cg_update"},  {"cg_dir_f32"},
+                                       {"cg_update_f32"}, {"mg_down_l0"}, {"mg_up_l0"},
+                                       {"refine"},     {"pred_sigma"}, {"tf_pairs"},
+                                       {"cic"}};
     // Records HIP events around `launch` on the solver stream when profiling
     // is on; `iter` tags solver iterations so that launches after the stop
     // (no-ops) can be left out.
@@ -116,7 +124,7 @@ struct Sim {
         MAZE_HIP(hipEventRecord(a, stream));
         launch();
         MAZE_HIP(hipEventRecord(b, stream));
-        pending.push_back({timer, a, b, iter});
+        pending.push_back({timer, a, b, iter, cur_pass});
     }
 
     Sim();
@@ -131,7 +139,8 @@ struct Sim {
                         const double* charges, const double* pot_params);
     void init_integrator(double dt, double T, double gamma, int type, int enabled);
 
-    int update_charges();
+    int update_charges(bool defer = false);
+    int check_charge();
     void init_field();
     int update_field();
     void forces_elec();
@@ -151,8 +160,9 @@ struct Sim {
                  int limit, double* rbuf, double* pa, double* pbuf, int* pred);
     int cg_count_solve(const GridDims& d, const double* b, double* x, double rtol, int maxit,
                        double* rbuf, double* pa, double* pbuf, int* pred);
-    int cg_inner_f32(const float* b, float* x, double rtol, int limit, int slot);
-    int cg_inner_mg(const float* b, float* x, double rtol, int limit, int slot);
+    void inner_iter(bool use_mg, float* x, int it);
+    void inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated);
+    int inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit, int first_chunk);
     bool setup_mg();
     void free_mg();
     void vcycle(int level, int iter);
@@ -167,6 +177,7 @@ struct Sim {
     void read_state();
     hipEvent_t next_event();
     void prof_flush(int iters_done);
+    void prof_flush_passes(const int* iters, int npass);
     void free_grid();
     void free_particles();
 };

@@ -57,16 +57,17 @@ struct SrcDir {
 };
 
 template <bool Add>
-struct SrcSum {  // y + d
+struct SrcSum {  // y + dscale d  (dscale 1, or 0 to ignore d)
     const double* y;
     const float* d;
+    double dscale;
     struct Raw { double y; float d; };
     __device__ __forceinline__ Raw load(int i) const {
         if (Add) return Raw{y[i], d[i]};
         return Raw{y[i], 0.0f};
     }
     __device__ __forceinline__ double val(const Raw& r) const {
-        return Add ? r.y + (double)r.d : r.y;
+        return Add ? r.y + dscale * (double)r.d : r.y;
     }
 };
 
@@ -80,9 +81,38 @@ struct SrcZero {
 // Calls body.apply(c, body.load(c), value_at_c, (A value)_c) for every point of
 // this work-group's tile.  Threads past the end of the plane do nothing but
 // stay for the work-group reductions.
+//
+// Software-pipelined by one plane: the operands of plane i+1 are requested
+// before plane i is computed, so each thread has two planes of loads in
+// flight and the march is not one memory latency per plane.  (The extra
+// plane fetched after the last one is in bounds -- indices wrap -- and
+// unused.)  Every pass writes arrays other than the ones its source reads,
+// so requesting the next plane before this plane's stores is safe.
+struct PBPlane {
+    double exc, eyc, ezc, eym, ezm, k2c;
+};
+
+template <bool PB>
+__device__ __forceinline__ PBPlane pb_load(const PBCoef& pb, int base, int pl,
+                                                 const PlaneNb& nb) {
+    PBPlane r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (PB) {
+        const int c = base + pl;
+        r.exc = pb.ex[c];
+        r.eyc = pb.ey[c];
+        r.ezc = pb.ez[c];
+        r.eym = pb.ey[base + nb.jm];
+        r.ezm = pb.ez[base + nb.km];
+        r.k2c = pb.k2[c];
+    }
+    return r;
+}
+
 template <bool PB, class Src, class Body>
 __device__ __forceinline__ void march(const GridDims& d, const PBCoef& pb, const Src& src,
                                       Body& body) {
+    using SR = typename Src::Raw;
+    using BR = typename Body::Raw;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int chunk = b % d.chunks;
     const int s = b / d.chunks;
@@ -93,43 +123,81 @@ __device__ __forceinline__ void march(const GridDims& d, const PBCoef& pb, const
     const int i0 = s * d.slab;
     const int i1 = min(n, i0 + d.slab);
     const int im0 = (i0 == 0) ? n - 1 : i0 - 1;
+    auto wrap = [n](int i) { return i >= n ? i - n : i; };
+    // The operands of one plane: the i+1 neighbour, the four in-plane
+    // neighbours, the body's centre operands and the PB coefficients.
+    struct Ops {
+        SR p, jm, jp, km, kp;
+        BR c;
+        PBPlane cf;
+    };
+    auto fetch = [&](int i) {
+        const int base = wrap(i) * n2;
+        Ops o;
+        o.p = src.load(wrap(i + 1) * n2 + pl);
+        o.jm = src.load(base + nb.jm);
+        o.jp = src.load(base + nb.jp);
+        o.km = src.load(base + nb.km);
+        o.kp = src.load(base + nb.kp);
+        o.c = body.load(base + pl);
+        o.cf = pb_load<PB>(pb, base, pl, nb);
+        return o;
+    };
     double vm = src.val(src.load(im0 * n2 + pl));
     double vc = src.val(src.load(i0 * n2 + pl));
     double exm = 0.0;
     if (PB) exm = pb.ex[im0 * n2 + pl];
-#pragma unroll 2
-    for (int i = i0; i < i1; i++) {
-        const int ip = (i + 1 == n) ? 0 : i + 1;
-        const int base = i * n2;
-        const int c = base + pl;
-        const typename Src::Raw rp = src.load(ip * n2 + pl);
-        const typename Src::Raw rjm = src.load(base + nb.jm);
-        const typename Src::Raw rjp = src.load(base + nb.jp);
-        const typename Src::Raw rkm = src.load(base + nb.km);
-        const typename Src::Raw rkp = src.load(base + nb.kp);
-        const typename Body::Raw bc = body.load(c);
-        double exc = 0.0, eyc = 0.0, ezc = 0.0, eym = 0.0, ezm = 0.0, k2c = 0.0;
-        if (PB) {
-            exc = pb.ex[c];
-            eyc = pb.ey[c];
-            ezc = pb.ez[c];
-            eym = pb.ey[base + nb.jm];
-            ezm = pb.ez[base + nb.km];
-            k2c = pb.k2[c];
-        }
-        const double vp = src.val(rp), vjm = src.val(rjm), vjp = src.val(rjp);
-        const double vkm = src.val(rkm), vkp = src.val(rkp);
+    auto compute = [&](int i, const Ops& o) {
+        const double vp = src.val(o.p), vjm = src.val(o.jm), vjp = src.val(o.jp);
+        const double vkm = src.val(o.km), vkp = src.val(o.kp);
         double lap;
         if (!PB) {
             lap = vp + vm + vjp + vjm + vkp + vkm - vc * 6.0;
         } else {
-            lap = vp * exc + vm * exm + vjp * eyc + vjm * eym + vkp * ezc + vkm * ezm -
-                  vc * (exc + exm + eyc + eym + ezc + ezm + k2c);
-            exm = exc;
+            const PBPlane& f = o.cf;
+            lap = vp * f.exc + vm * exm + vjp * f.eyc + vjm * f.eym + vkp * f.ezc + vkm * f.ezm -
+                  vc * (f.exc + exm + f.eyc + f.eym + f.ezc + f.ezm + f.k2c);
+            exm = f.exc;
         }
-        body.apply(c, bc, vc, lap);
+        body.apply(i * n2 + pl, o.c, vc, lap);
         vm = vc;
         vc = vp;
+    };
+    // Two operand sets alternate (no register copies of pending loads).
+    Ops A = fetch(i0);
+    for (int i = i0; i < i1; i += 2) {
+        const Ops B = fetch(i + 1);
+        compute(i, A);
+        if (i + 1 >= i1) break;
+        A = fetch(i + 2);
+        compute(i + 1, B);
+    }
+}
+
+// Single-pass reduction of the (sum, max) a body accumulated: the last
+// work-group calls bd.finalize(st, S, M) with the totals.
+// NT: threads per work-group (the (sum, max) form needs NT == kBlock).
+template <bool kMax, class Body, int NT = kBlock>
+__device__ __forceinline__ void reduce_and_finalize(Body& bd, Reduce red, CGState* st) {
+    static_assert(!kMax || NT == kBlock, "sum+max reduction assumes kBlock threads");
+    __shared__ double sh[2 * NT / kWave];
+    double s = bd.s, m = 0.0;
+    if constexpr (kMax) {
+        m = bd.m;
+        block_sum_max(s, m, sh);
+    } else {
+        s = block_sum<NT>(s, sh);
+    }
+    if (publish_arrive(red.psum, s, kMax ? red.pmax : nullptr, m, red.counter, gridDim.x)) {
+        double S, M = 0.0;
+        if constexpr (kMax)
+            reduce_partials_sum_max(red.psum, red.pmax, gridDim.x, sh, S, M);
+        else
+            S = reduce_partials_sum<NT>(red.psum, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            bd.finalize(st, S, M);
+            reset_counter(red.counter);
+        }
     }
 }
 
@@ -141,28 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(GridDims d, PBCoef pb, Src src,
     if (Body::kSkipWhenDone && st->done) return;
     Body bd = body;
     march<PB>(d, pb, src, bd);
-    if constexpr (Body::kReduce) {
-        __shared__ double sh[2 * kBlock / kWave];
-        double s = bd.s, m = 0.0;
-        if constexpr (Body::kMax) {
-            m = bd.m;
-            block_sum_max(s, m, sh);
-        } else {
-            s = block_sum(s, sh);
-        }
-        if (publish_arrive(red.psum, s, Body::kMax ? red.pmax : nullptr, m, red.counter,
-                           gridDim.x)) {
-            double S, M = 0.0;
-            if constexpr (Body::kMax)
-                reduce_partials_sum_max(red.psum, red.pmax, gridDim.x, sh, S, M);
-            else
-                S = reduce_partials_sum(red.psum, gridDim.x, sh);
-            if (threadIdx.x == 0) {
-                Body::finalize(st, S, M);
-                reset_counter(red.counter);
-            }
-        }
-    }
+    if constexpr (Body::kReduce) reduce_and_finalize<Body::kMax>(bd, red, st);
 }
 
 template <bool PB, class Src, class Body>

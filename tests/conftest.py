@@ -89,6 +89,7 @@ def hip_raw(hip):
     import ctypes
     lib = hip.lib
     D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     ci, cd = ctypes.c_int, ctypes.c_double
     sig = {
         "maze_test_laplace": (None, [D, D, ci]),
@@ -100,6 +101,9 @@ def hip_raw(hip):
         "maze_test_prolong_add": (ci, [D, D, ci]),
         "maze_test_mg_vcycle": (ci, [D, D, ci]),
         "maze_test_mg_pcg": (ci, [D, D, cd, ci]),
+        "maze_test_mg_down": (ci, [F, F, F, ci, cd]),
+        "maze_test_mg_up": (ci, [F, F, F, F, ci, cd]),
+        "maze_test_mg_tail": (ci, [F, F, ci]),
         "solver_set_mg_omega": (None, [cd]),
         "solver_mg_levels": (ci, []),
         "solver_set_precision": (ci, [ci]),

@@ -183,6 +183,109 @@ def test_mg_pcg_solves(hip_raw, oracle_raw, n):
     assert np.abs(d).max() <= 1e-4 * np.abs(xo).max()
 
 
+def _lap(u):
+    return sum(np.roll(u, s, axis=a) for a in range(3) for s in (1, -1)) - 6 * u
+
+
+def _restrict_fw(r):
+    """4 x 27-point full weighting onto the even points (weights 1-2-1 per axis / 64)."""
+    acc = np.zeros_like(r)
+    for a in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            for c in (-1, 0, 1):
+                w = (2 - abs(a)) * (2 - abs(b)) * (2 - abs(c))
+                acc += w * np.roll(r, (-a, -b, -c), axis=(0, 1, 2))
+    return acc[::2, ::2, ::2] * (4.0 / 64.0)
+
+
+def _prolong_trilinear(e):
+    """Coarse -> fine: copies on even points, averages of the two neighbours on odd."""
+    for ax in range(3):
+        n = e.shape[ax]
+        f = np.zeros(e.shape[:ax] + (2 * n,) + e.shape[ax + 1:])
+        idx = [slice(None)] * 3
+        idx[ax] = slice(0, None, 2)
+        f[tuple(idx)] = e
+        idx[ax] = slice(1, None, 2)
+        f[tuple(idx)] = 0.5 * (e + np.roll(e, -1, axis=ax))
+        e = f
+    return e
+
+
+@pytest.mark.parametrize("n", [8, 16, 20, 64, 72])
+def test_mg_fused_down(hip_raw, n):
+    """Fused FP32 down kernel (two Jacobi sweeps from zero, residual, restriction) vs the
+    same steps in FP64 NumPy; n = 20 and 72 leave partial tiles, n = 8 wraps the halo."""
+    c = (6.0 / 7.0) / 6.0
+    b = np.random.default_rng(n).standard_normal((n, n, n)).astype(np.float32)
+    d2 = np.zeros_like(b)
+    bc = np.zeros((n // 2,) * 3, np.float32)
+    assert hip_raw.maze_test_mg_down(b, d2, bc, n, c) == 0
+    b64 = b.astype(np.float64)
+    d1 = -c * b64
+    d2_ref = d1 - c * (b64 - _lap(d1))
+    bc_ref = _restrict_fw(b64 - _lap(d2_ref))
+    assert np.abs(d2 - d2_ref).max() <= 2e-6 * np.abs(d2_ref).max()
+    assert np.abs(bc - bc_ref).max() <= 2e-6 * np.abs(bc_ref).max()
+
+
+@pytest.mark.parametrize("n", [8, 16, 20, 64, 72])
+def test_mg_fused_up(hip_raw, n):
+    """Fused FP32 up kernel (trilinear prolongation-correction, two Jacobi sweeps)."""
+    c = (6.0 / 7.0) / 6.0
+    g = np.random.default_rng(100 + n)
+    b = g.standard_normal((n, n, n)).astype(np.float32)
+    d2 = g.standard_normal((n, n, n)).astype(np.float32)
+    e = g.standard_normal((n // 2,) * 3).astype(np.float32)
+    d = np.zeros_like(b)
+    assert hip_raw.maze_test_mg_up(b, d2, e, d, n, c) == 0
+    b64 = b.astype(np.float64)
+    x = d2.astype(np.float64) + _prolong_trilinear(e.astype(np.float64))
+    for _ in range(2):
+        x = x - c * (b64 - _lap(x))
+    assert np.abs(d - x).max() <= 2e-6 * np.abs(x).max()
+
+
+def _exact_periodic(b):
+    """Solve A e = b (periodic 7-point, unscaled) with the constant mode dropped."""
+    n = b.shape[0]
+    lam1 = 2 * np.cos(2 * np.pi * np.arange(n) / n) - 2
+    lam = lam1[:, None, None] + lam1[None, :, None] + lam1[None, None, :]
+    lam[0, 0, 0] = 1.0
+    eh = np.fft.fftn(b) / lam
+    eh[0, 0, 0] = 0.0
+    return np.fft.ifftn(eh).real
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_mg_tail(hip_raw, n):
+    """The one-work-group tail (16^3 -> exact 8^3 -> 16^3) vs the same cycle in FP64."""
+    m = 16
+    c = (6.0 / 7.0) / 6.0
+    b = np.random.default_rng(n).standard_normal((m, m, m)).astype(np.float32)
+    d = np.zeros_like(b)
+    assert hip_raw.maze_test_mg_tail(b, d, n) == m
+    b64 = b.astype(np.float64)
+    d2 = -c * b64
+    d2 = d2 - c * (b64 - _lap(d2))
+    e = _exact_periodic(_restrict_fw(b64 - _lap(d2)))
+    x = d2 + _prolong_trilinear(e)
+    for _ in range(2):
+        x = x - c * (b64 - _lap(x))
+    assert np.abs(d - x).max() <= 1e-5 * np.abs(x).max()
+
+
+def test_prolong_restrict_closed_forms():
+    """The NumPy restatements above: restriction of a constant is 4x it, prolongation of
+    a constant is that constant, and the restriction is 1/2 x (1/8) x 4 x the adjoint of
+    the prolongation (<R r, e> = (4/8) <r, P e>)."""
+    g = np.random.default_rng(7)
+    r, e = g.standard_normal((8, 8, 8)), g.standard_normal((4, 4, 4))
+    assert np.allclose(_restrict_fw(np.full((8, 8, 8), 3.0)), 12.0)
+    assert np.allclose(_prolong_trilinear(np.full((4, 4, 4), 2.0)), 2.0)
+    assert np.isclose((_restrict_fw(r) * e).sum(), 0.5 * (r * _prolong_trilinear(e)).sum())
+
+
 @pytest.mark.parametrize("precision", [1, 2])
 def test_mixed_precision_forces(hip, precision):
     """FP32 CG (1) or FP32 MG-PCG (2) inside FP64 refinement: force RMS error <= 1e-5
