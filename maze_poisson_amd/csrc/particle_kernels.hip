@@ -189,103 +189,187 @@ void launch_force_fd(hipStream_t s, const PDev& p, const double* phi) {
 }
 
 // Shifted-force Tosi-Fumi over all pairs within r_cut (library/src/forces.c:
-// 132-183).  Thread (i, segment s) visits j in its segment in increasing
-// order; the wave's lanes are consecutive i at the same j, so the j position
-// is a broadcast load.  Partials per segment are folded in segment order.
-__global__ __launch_bounds__(kBlock) void k_tf_pairs(int np, int n_typ, double L, double r_cut,
-                                                     const double* __restrict__ pos,
-                                                     const int* __restrict__ types,
-                                                     const double* __restrict__ table, int seg,
-                                                     double* __restrict__ fpart,
-                                                     double* __restrict__ epart,
-                                                     int* __restrict__ cpart) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    if (i >= np) return;
-    const int j0 = s * seg, j1 = min(np, j0 + seg);
-    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    const int ti = types[i];
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0, e = 0.0;
-    int cnt = 0;
-    for (int j = j0; j < j1; j++) {
-        if (j == i) continue;
-        double app = xi - pos[3 * j];
-        app -= L * round(app / L);
-        double r2 = app * app;
-        double d0 = app;
-        app = yi - pos[3 * j + 1];
-        app -= L * round(app / L);
-        double d1 = app;
-        r2 += app * app;
-        app = zi - pos[3 * j + 2];
-        app -= L * round(app / L);
-        double d2 = app;
-        r2 += app * app;
-        const double r = sqrt(r2);
-        if (r > r_cut) continue;
-        cnt++;
-        d0 /= r;
-        d1 /= r;
-        d2 /= r;
-        const double* t = table + (ti * n_typ + types[j]) * 7;
-        const double a = t[0], b = t[1], c = t[2], dd = t[3], sg = t[4], al = t[5], be = t[6];
-        const double ex = exp(b * (sg - r));
-        const double ir = 1.0 / r;
-        const double ir2 = ir * ir;
-        const double ir6 = ir2 * ir2 * ir2;
-        const double ir8 = ir6 * ir2;
-        const double fm = b * a * ex - 6 * c * (ir6 * ir) - 8 * dd * (ir8 * ir) - al;
-        const double vm = a * ex - c * ir6 - dd * ir8 + al * r + be;
-        f0 += fm * d0;
-        f1 += fm * d1;
-        f2 += fm * d2;
-        e += vm;
+// 132-183), each unordered pair evaluated once (Newton's third law).
+//
+// Ions are split into blocks of 64; one work-group of 4 waves handles a block
+// pair (I, J), I <= J.  Lane l owns ion i = 64 I + l and at step s meets
+// j = 64 J + (l + s) mod 64, so the 64 lanes of a step meet 64 distinct j;
+// wave w takes the steps s = w, w + 4, ...  The force on i stays in
+// registers; the reaction on j goes to the wave's own LDS accumulator of j
+// (no two lanes of a wave touch the same j in a step; a barrier separates
+// steps).  The diagonal pair (I, I) takes steps 1..32 (step 32 on lanes < 32
+// only) so that every pair inside the block is met once.  The waves' partials
+// are summed in wave order and the work-group writes the i-role and j-role
+// partials (force, energy, count) of its block pair; k_tf_fold sums each
+// ion's partials in a fixed order (deterministic).  The pair energy goes to
+// both ions, so the potential energy is half the sum over ions, as in the
+// reference.
+constexpr int kTfB = 64;
+constexpr int kTfWaves = 4;
+
+__device__ __forceinline__ int tf_item(int I, int J, int nb) { return I * nb - I * (I - 1) / 2 + (J - I); }
+
+// The type-pair table sits in LDS for up to kTfLdsTypes types, else it is
+// read from global memory.
+constexpr int kTfLdsTypes = 4;
+
+template <bool LdsTab>
+__global__ __launch_bounds__(kTfB * kTfWaves) void k_tf_tiles(
+    int np, int n_typ, double L, double invL, double r_cut, const double* __restrict__ pos,
+    const int* __restrict__ types, const double* __restrict__ table, double* __restrict__ part) {
+    const int J = blockIdx.x, I = blockIdx.y, nb = gridDim.x;
+    if (I > J) return;
+    __shared__ double sx[kTfB], sy[kTfB], sz[kTfB];
+    __shared__ int styp[kTfB];
+    __shared__ double acc[kTfWaves][5][kTfB];  // j-role, per wave
+    __shared__ double own[kTfWaves][5][kTfB];  // i-role, per wave
+    __shared__ double tab_lds[kTfLdsTypes * kTfLdsTypes * 7];
+    const int tid = threadIdx.x, l = tid & (kTfB - 1), w = tid / kTfB;
+    if (LdsTab)
+        for (int t = tid; t < n_typ * n_typ * 7; t += kTfB * kTfWaves) tab_lds[t] = table[t];
+    const double* tab = LdsTab ? tab_lds : table;
+    if (w == 0) {
+        const int jl = J * kTfB + l;
+        sx[l] = jl < np ? pos[3 * jl] : 0.0;
+        sy[l] = jl < np ? pos[3 * jl + 1] : 0.0;
+        sz[l] = jl < np ? pos[3 * jl + 2] : 0.0;
+        styp[l] = jl < np ? types[jl] : 0;
     }
-    const long o = (long)s * np + i;
-    fpart[3 * o] = f0;
-    fpart[3 * o + 1] = f1;
-    fpart[3 * o + 2] = f2;
-    epart[o] = e;
-    cpart[o] = cnt;
+    for (int c = 0; c < 5; c++) acc[w][c][l] = 0.0;
+    const int i = I * kTfB + l;
+    const bool iv = i < np;
+    const double xi = iv ? pos[3 * i] : 0.0, yi = iv ? pos[3 * i + 1] : 0.0,
+                 zi = iv ? pos[3 * i + 2] : 0.0;
+    const int ti = iv ? types[i] : 0;
+    __syncthreads();
+    const bool diag = I == J;
+    const int s0 = diag ? 1 : 0, s1 = diag ? 32 : 63;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, e = 0.0, cnt = 0.0;
+    // Every wave runs the same number of steps (the barrier is work-group wide).
+    const int nsteps = (s1 - s0 + kTfWaves) / kTfWaves;
+    for (int k = 0; k < nsteps; k++) {
+        const int s = s0 + w + k * kTfWaves;
+        const int jj = (l + s) & (kTfB - 1);
+        const int j = J * kTfB + jj;
+        const bool ok = s <= s1 && iv && j < np && !(diag && s == 32 && l >= 32);
+        if (ok) {
+            double app = xi - sx[jj];
+            app -= L * round(app * invL);
+            double r2 = app * app;
+            const double d0 = app;
+            app = yi - sy[jj];
+            app -= L * round(app * invL);
+            const double d1 = app;
+            r2 += app * app;
+            app = zi - sz[jj];
+            app -= L * round(app * invL);
+            const double d2 = app;
+            r2 += app * app;
+            const double r = sqrt(r2);
+            if (r <= r_cut) {
+                const double ir = 1.0 / r;
+                const double* t = tab + (ti * n_typ + styp[jj]) * 7;
+                const double a = t[0], b = t[1], c = t[2], dd = t[3], sg = t[4], al = t[5],
+                             be = t[6];
+                const double ex = exp(b * (sg - r));
+                const double ir2 = ir * ir;
+                const double ir6 = ir2 * ir2 * ir2;
+                const double ir8 = ir6 * ir2;
+                const double fm = b * a * ex - 6 * c * (ir6 * ir) - 8 * dd * (ir8 * ir) - al;
+                const double vm = a * ex - c * ir6 - dd * ir8 + al * r + be;
+                const double g0 = fm * (d0 * ir), g1 = fm * (d1 * ir), g2 = fm * (d2 * ir);
+                f0 += g0;
+                f1 += g1;
+                f2 += g2;
+                e += vm;
+                cnt += 1.0;
+                acc[w][0][jj] -= g0;
+                acc[w][1][jj] -= g1;
+                acc[w][2][jj] -= g2;
+                acc[w][3][jj] += vm;
+                acc[w][4][jj] += 1.0;
+            }
+        }
+        __syncthreads();
+    }
+    own[w][0][l] = f0;
+    own[w][1][l] = f1;
+    own[w][2][l] = f2;
+    own[w][3][l] = e;
+    own[w][4][l] = cnt;
+    __syncthreads();
+    // part[item][component][slot]: slots 0..63 the i-role, 64..127 the j-role.
+    double* o = part + (long)tf_item(I, J, nb) * 5 * 2 * kTfB;
+    for (int x = tid; x < 5 * 2 * kTfB; x += kTfB * kTfWaves) {
+        const int c = x / (2 * kTfB), slot = x % (2 * kTfB);
+        const bool jr = slot >= kTfB;
+        const int q = jr ? slot - kTfB : slot;
+        double v = 0.0;
+        for (int u = 0; u < kTfWaves; u++) v += jr ? acc[u][c][q] : own[u][c][q];
+        o[x] = v;
+    }
 }
 
-__global__ void k_tf_fold(int np, int segs, const double* __restrict__ fpart,
-                          const double* __restrict__ epart, const int* __restrict__ cpart,
-                          double* __restrict__ fn, double* __restrict__ ei,
-                          int* __restrict__ counts) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0, e = 0.0;
-    int c = 0;
-    for (int s = 0; s < segs; s++) {
-        const long o = (long)s * np + i;
-        f0 += fpart[3 * o];
-        f1 += fpart[3 * o + 1];
-        f2 += fpart[3 * o + 2];
-        e += epart[o];
-        c += cpart[o];
+// Ion p = 64 B + l: its i-role partials of the pairs (B, J), J = B..nb-1, then
+// its j-role partials of the pairs (I, B), I = 0..B -- nb + 1 items, split
+// over kFoldGroups thread groups (group g takes items g, g + G, ...) and
+// summed in group order.
+constexpr int kFoldGroups = 8;
+
+__global__ __launch_bounds__(kTfB * kFoldGroups) void k_tf_fold(
+    int np, int nb, const double* __restrict__ part, double* __restrict__ fn,
+    double* __restrict__ eion, int* __restrict__ counts) {
+    __shared__ double sum[kFoldGroups][5][kTfB];
+    const int B = blockIdx.x;
+    const int l = threadIdx.x & (kTfB - 1), g = threadIdx.x / kTfB;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = g; k <= nb; k += kFoldGroups) {
+        // k < nb - B: i-role of (B, B + k); else j-role of (k - (nb - B), B).
+        const int ni = nb - B;
+        const double* o = k < ni ? part + (long)tf_item(B, B + k, nb) * 5 * 2 * kTfB + l
+                                 : part + (long)tf_item(k - ni, B, nb) * 5 * 2 * kTfB + kTfB + l;
+        double t[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) t[c] = o[c * 2 * kTfB];
+#pragma unroll
+        for (int c = 0; c < 5; c++) v[c] += t[c];
     }
-    fn[3 * i] = f0;
-    fn[3 * i + 1] = f1;
-    fn[3 * i + 2] = f2;
-    ei[i] = e;
-    counts[i] = c;
+#pragma unroll
+    for (int c = 0; c < 5; c++) sum[g][c][l] = v[c];
+    __syncthreads();
+    const int p = B * kTfB + l;
+    if (g != 0 || p >= np) return;
+    double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int u = 0; u < kFoldGroups; u++)
+        for (int c = 0; c < 5; c++) t[c] += sum[u][c][l];
+    fn[3 * p] = t[0];
+    fn[3 * p + 1] = t[1];
+    fn[3 * p + 2] = t[2];
+    eion[p] = t[3];
+    counts[p] = (int)t[4];
+}
+
+int tf_blocks(int np) { return (np + kTfB - 1) / kTfB; }
+long tf_part_doubles(int np) {
+    const long nb = tf_blocks(np);
+    return nb * (nb + 1) / 2 * 5 * 2 * kTfB;
 }
 
 void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
     if (p.np == 0) {
-        launch_sum_scaled(s, 0, 0.5, p.epart, &ps->pe_tf, red);
+        launch_sum_scaled(s, 0, 0.5, p.eion, &ps->pe_tf, red);
         return;
     }
-    const int segs = p.segments;
-    const int seg = (p.np + segs - 1) / segs;
-    hipLaunchKernelGGL(k_tf_pairs, dim3((p.np + kBlock - 1) / kBlock, segs), dim3(kBlock), 0, s,
-                       p.np, p.n_typ, p.L, p.r_cut, p.pos, p.types, p.table, seg, p.fpart,
-                       p.epart, p.cpart);
-    // The per-ion energies reuse the first segment row of epart.
-    hipLaunchKernelGGL(k_tf_fold, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, segs, p.fpart,
-                       p.epart, p.cpart, p.fn, p.epart + (long)segs * p.np, p.counts);
-    launch_sum_scaled(s, p.np, 0.5, p.epart + (long)segs * p.np, &ps->pe_tf, red);
+    const int nb = tf_blocks(p.np);
+    if (p.n_typ <= kTfLdsTypes)
+        hipLaunchKernelGGL(k_tf_tiles<true>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
+                           p.n_typ, p.L, 1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart);
+    else
+        hipLaunchKernelGGL(k_tf_tiles<false>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
+                           p.n_typ, p.L, 1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart);
+    hipLaunchKernelGGL(k_tf_fold, dim3(nb), dim3(kTfB * kFoldGroups), 0, s, p.np, nb, p.tpart,
+                       p.fn, p.eion, p.counts);
+    launch_sum_scaled(s, p.np, 0.5, p.eion, &ps->pe_tf, red);
 }
 
 // F_tot = 0 + 1 F_elec + 1 F_noel (library/src/particles.c:777-795).

@@ -23,14 +23,12 @@ struct PDev {
     int* ids;
     int* ids_sorted;
     double* table;    // Tosi-Fumi constants per type pair, 7 each
-    double* fpart;    // pair-force partials (segments, np, 3)
-    double* epart;    // pair-energy partials (segments, np)
-    int* cpart;       // pair counts (segments, np)
+    double* tpart;    // Tosi-Fumi partials per 64-ion block pair (tf_part_doubles)
+    double* eion;     // Tosi-Fumi energy per ion (each pair's energy on both ions)
     int* counts;      // pairs within the cutoff per ion
     double* contrib;  // per-(ion, corner) CIC contribution, (np, 8)
     void* sort_tmp;
     size_t sort_tmp_bytes;
-    int segments;
 };
 
 // Device-resident scalars of the particle side.
@@ -49,6 +47,7 @@ void launch_cic_sort(hipStream_t s, const PDev& p, int key_bits);
 void launch_cic_gather(hipStream_t s, const PDev& p, double* q);
 void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi);
+long tf_part_doubles(int np);
 void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 void launch_force_total(hipStream_t s, const PDev& p);
 void launch_kinetic(hipStream_t s, const PDev& p, PScalars* ps, Reduce red, double T_target,

@@ -37,7 +37,8 @@ static void dfree(T*& p) {
     p = nullptr;
 }
 
-// Planes per work-group so that one sweep launches about 1024+ work-groups.
+// Planes per work-group so that one sweep launches about 1024 work-groups
+// (measured: twice as many work-groups with half the planes each is slower).
 static int choose_slab(int n) {
     const long chunks = ((long)n * n + kBlock - 1) / kBlock;
     long s = chunks * n / 1024;
@@ -90,14 +91,13 @@ void Sim::free_grid() {
 void Sim::free_particles() {
     dfree(p.types);
     for (double** a : {&p.pos, &p.vel, &p.mass, &p.charges, &p.fe, &p.fn, &p.ft, &p.table,
-                       &p.fpart, &p.epart, &p.contrib})
+                       &p.tpart, &p.eion, &p.contrib})
         dfree(*a);
     dfree(p.base);
     dfree(p.keys);
     dfree(p.keys_sorted);
     dfree(p.ids);
     dfree(p.ids_sorted);
-    dfree(p.cpart);
     dfree(p.counts);
     if (p.sort_tmp) MAZE_HIP(hipFree(p.sort_tmp));
     p.sort_tmp = nullptr;
@@ -177,12 +177,8 @@ void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, const 
     p.keys = dalloc<unsigned int>(np); p.keys_sorted = dalloc<unsigned int>(np);
     p.ids = dalloc<int>(np); p.ids_sorted = dalloc<int>(np);
     p.contrib = dalloc<double>(8L * np);
-    // Enough j-segments that np * segments work-items fill the chip.
-    int segs = (int)std::ceil(65536.0 / std::max(np, 1));
-    p.segments = std::max(1, std::min(64, segs));
-    p.fpart = dalloc<double>(3L * np * p.segments);
-    p.epart = dalloc<double>((long)np * (p.segments + 1));
-    p.cpart = dalloc<int>((long)np * p.segments);
+    p.tpart = dalloc<double>(tf_part_doubles(np));
+    p.eion = dalloc<double>(np);
     p.counts = dalloc<int>(np);
     // Tosi-Fumi shifted-force constants per type pair, with the reference's
     // expressions (library/src/particles.c:191-215), evaluated on the host.
