@@ -157,6 +157,19 @@ def cpu_baseline(steps, cores):
                       f"after the 2 bootstrap steps and 1 untimed step"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_4096ions_128grid.json")
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a timed kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh).get(name, {}).get("traffic_bytes")
+    except (OSError, ValueError):
+        return None
+
+
 def roofline(prof):
     """Dominant grid kernel (largest total event time) of the headline path."""
     G = N_GRID ** 3
@@ -169,7 +182,7 @@ def roofline(prof):
     bpl = BYTES_PER_POINT[name] * G
     ach = bpl / avg_s / 1e9
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
             "avg_us": round(avg_s * 1e6, 2), "launches": cnt, "bytes_per_launch": bpl}
 
 

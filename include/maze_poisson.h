@@ -175,8 +175,12 @@ void solver_set_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
 int solver_set_precision(int mode);
 int solver_get_precision(void);
 /* Relative residual reduction of each FP32 inner solve; <= 0 restores the
- * default of the mode (1e-4 for mode 1, 1e-2 for mode 2). */
+ * default of the mode (1e-4 for mode 1, 1e-3 for mode 2). */
 void solver_set_inner_rtol(double rtol);
+/* Mixed-precision MaZe solve: start from the linear extrapolation
+ * 2 y_n-1 - y_n-2 of the correction field (1) instead of y_n-1 (0, the
+ * reference's start).  The stop test is unchanged. */
+void solver_set_y_extrapolation(int on);
 /* Mode 2 = FP32 multigrid-preconditioned CG inside FP64 iterative
  * refinement (V(2,2) damped Jacobi, exact FP64 coarsest solve); falls back
  * to mode 1 when the grid has no multigrid hierarchy (halving reaches an odd

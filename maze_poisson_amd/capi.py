@@ -99,6 +99,7 @@ EXTENSION_ABI = [
     ("solver_set_precision", _c_int, [_c_int]),
     ("solver_get_precision", _c_int, []),
     ("solver_set_inner_rtol", None, [_c_double]),
+    ("solver_set_y_extrapolation", None, [_c_int]),
     ("solver_set_mg_omega", None, [_c_double]),
     ("solver_mg_levels", _c_int, []),
     ("solver_set_charge_check", None, [_c_int]),

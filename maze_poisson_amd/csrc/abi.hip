@@ -182,6 +182,14 @@ void solver_set_inner_rtol(double rtol) {
     S().sched.clear();
 }
 
+// Start each mixed-precision MaZe solve from 2 y_n-1 - y_n-2 (1) or from
+// y_n-1 as the reference does (0, default).
+void solver_set_y_extrapolation(int on) {
+    S().y_extrap = on ? 1 : 0;
+    S().hist_ok = 0;
+    S().sched.clear();
+}
+
 // Damping of the multigrid Jacobi smoother (default 6/7).
 void solver_set_mg_omega(double omega) { S().mg_omega = omega; }
 

@@ -1,6 +1,7 @@
 // common.hpp -- shared device/host helpers for the MaZe HIP library (gfx950).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -19,6 +20,27 @@ namespace maze {
                          __FILE__, __LINE__, #call);                                     \
             std::abort();                                                                \
         }                                                                                \
+    } while (0)
+
+// Kernel timing: when `start` is set, the next MAZE_LAUNCH records the
+// kernel's own start and end in (start, stop) -- the dispatch's timestamps,
+// the same interval a kernel trace reports -- and clears `start`, so a
+// launcher that issues several kernels times only its first.
+struct LaunchTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTimer g_launch_timer;
+
+#define MAZE_LAUNCH(kernel, grid, block, shmem, stream, ...)                                   \
+    do {                                                                                       \
+        if (::maze::g_launch_timer.start) {                                                    \
+            const hipEvent_t a_ = ::maze::g_launch_timer.start;                                \
+            ::maze::g_launch_timer.start = nullptr;                                            \
+            hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, a_,                      \
+                                  ::maze::g_launch_timer.stop, 0, __VA_ARGS__);                \
+        } else {                                                                               \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);               \
+        }                                                                                      \
     } while (0)
 
 constexpr int kBlock = 256;       // threads per work-group (4 wave64)

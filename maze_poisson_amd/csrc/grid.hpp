@@ -85,6 +85,11 @@ struct RefineArgs {
 void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, double* y0, double* y1,
                    const float* dcorr, const double* sigma, float* r32, RefineArgs a, CGState* st,
                    Reduce red, int add_d);
+// First pass from the extrapolated iterate y_out = 2 y - y_prev (then y_out is
+// the loop's y0).
+void launch_refine_extrap(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
+                          const double* y_prev, double* y_out, const double* sigma, float* r32,
+                          RefineArgs a, CGState* st, Reduce red);
 
 void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u);
 void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out);

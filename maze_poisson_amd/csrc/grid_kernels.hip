@@ -156,11 +156,13 @@ struct BodyUpdate {
     }
 };
 
-// y_out = y + d; the FP64 residual b = sigma - A y_out; max|b|.  Advances the
-// outer refinement loop and starts the next inner solve A x = b from x = 0:
+// y_out = the new iterate (y + d, or the extrapolated start 2 y - y_prev);
+// the FP64 residual b = sigma - A y_out; max|b|.  Advances the outer
+// refinement loop and starts the next inner solve A x = b from x = 0:
 // r32 = A 0 - b = -b rounded to FP32 and <r, r/6>, as the init pass of
 // library/src/plcg.c:41-58 would compute them.
-template <bool Add>
+// Write: y_out is written; Add: a refinement pass (else the loop's start).
+template <bool Write, bool Add>
 struct BodyRefine {
     double* y_out;
     const double* sigma;
@@ -171,7 +173,7 @@ struct BodyRefine {
     using Raw = double;
     __device__ Raw load(int c) const { return sigma[c]; }
     __device__ void apply(int c, const Raw& sg, double v, double lap) {
-        if (Add) y_out[c] = v;
+        if (Write) y_out[c] = v;
         const double rr = sg - lap;
         const float rf = (float)(-rr);
         r32[c] = rf;
@@ -284,10 +286,10 @@ template <class Real>
 void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p_old,
                    const Real* r, Real* p_new, CGState* st, Reduce red, int first) {
     if (first) {
-        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_dir<kPB, Real, true>), dim3(d.blocks),
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_cg_dir<kPB, Real, true>), dim3(d.blocks),
                                            dim3(kBlock), 0, s, d, pb, p_old, r, p_new, red, st));
     } else {
-        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_dir<kPB, Real, false>), dim3(d.blocks),
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_cg_dir<kPB, Real, false>), dim3(d.blocks),
                                            dim3(kBlock), 0, s, d, pb, p_old, r, p_new, red, st));
     }
 }
@@ -296,10 +298,10 @@ template <class Real>
 void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
                       Real* r, CGState* st, Reduce red, int zero_x) {
     if (zero_x) {
-        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_update<kPB, Real, true>), dim3(d.blocks),
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_cg_update<kPB, Real, true>), dim3(d.blocks),
                                            dim3(kBlock), 0, s, d, pb, p, x, r, red, st));
     } else {
-        DISPATCH_PB(pb, hipLaunchKernelGGL((k_cg_update<kPB, Real, false>), dim3(d.blocks),
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_cg_update<kPB, Real, false>), dim3(d.blocks),
                                            dim3(kBlock), 0, s, d, pb, p, x, r, red, st));
     }
 }
@@ -338,7 +340,7 @@ __global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int 
 
 void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit,
                      int count_mode, int pcg, int gated) {
-    hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode, pcg,
+    MAZE_LAUNCH(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode, pcg,
                        gated);
 }
 
@@ -358,19 +360,37 @@ __global__ __launch_bounds__(kBlock) void k_refine(GridDims d, PBCoef pb, double
         sel = st->ysel;
         dscale = (st->done ? st->res : st->iter) > 0 ? 1.0 : 0.0;
     }
-    BodyRefine<Add> bd{sel ? y0 : y1, sigma, r32, a.otol, a.rtol, a.limit, a.pcg};
+    BodyRefine<Add, Add> bd{sel ? y0 : y1, sigma, r32, a.otol, a.rtol, a.limit, a.pcg};
     march<PB>(d, pb, SrcSum<Add>{sel ? y1 : y0, dc, dscale}, bd);
     reduce_and_finalize<true>(bd, red, st);
+}
+
+// Start of the loop from the extrapolated iterate y_out = 2 y - y_prev.
+template <bool PB>
+__global__ __launch_bounds__(kBlock) void k_refine_extrap(GridDims d, PBCoef pb, const double* y,
+                                                          const double* y_prev, double* y_out,
+                                                          const double* sigma, float* r32,
+                                                          RefineArgs a, Reduce red, CGState* st) {
+    BodyRefine<true, false> bd{y_out, sigma, r32, a.otol, a.rtol, a.limit, a.pcg};
+    march<PB>(d, pb, SrcPred{y, y_prev}, bd);
+    reduce_and_finalize<true>(bd, red, st);
+}
+
+void launch_refine_extrap(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
+                          const double* y_prev, double* y_out, const double* sigma, float* r32,
+                          RefineArgs a, CGState* st, Reduce red) {
+    DISPATCH_PB(pb, MAZE_LAUNCH((k_refine_extrap<kPB>), dim3(d.blocks), dim3(kBlock), 0, s,
+                                       d, pb, y, y_prev, y_out, sigma, r32, a, red, st));
 }
 
 void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, double* y0, double* y1,
                    const float* dcorr, const double* sigma, float* r32, RefineArgs a, CGState* st,
                    Reduce red, int add_d) {
     if (add_d) {
-        DISPATCH_PB(pb, hipLaunchKernelGGL((k_refine<kPB, true>), dim3(d.blocks), dim3(kBlock), 0,
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_refine<kPB, true>), dim3(d.blocks), dim3(kBlock), 0,
                                            s, d, pb, y0, y1, dcorr, sigma, r32, a, red, st));
     } else {
-        DISPATCH_PB(pb, hipLaunchKernelGGL((k_refine<kPB, false>), dim3(d.blocks), dim3(kBlock), 0,
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_refine<kPB, false>), dim3(d.blocks), dim3(kBlock), 0,
                                            s, d, pb, y0, y1, dcorr, sigma, r32, a, red, st));
     }
 }
@@ -409,19 +429,19 @@ static int pw_blocks(long n) {
 }
 
 void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u) {
-    hipLaunchKernelGGL(k_axpy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, alpha, v, u);
+    MAZE_LAUNCH(k_axpy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, alpha, v, u);
 }
 void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out) {
-    hipLaunchKernelGGL(k_scale_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, c, in, out);
+    MAZE_LAUNCH(k_scale_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, c, in, out);
 }
 void launch_fill(hipStream_t s, long n, double v, double* out) {
-    hipLaunchKernelGGL(k_fill, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, v, out);
+    MAZE_LAUNCH(k_fill, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, v, out);
 }
 void launch_fill_f(hipStream_t s, long n, float v, float* out) {
-    hipLaunchKernelGGL(k_fill_f, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, v, out);
+    MAZE_LAUNCH(k_fill_f, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, v, out);
 }
 void launch_copy(hipStream_t s, long n, const double* in, double* out) {
-    hipLaunchKernelGGL(k_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, in, out);
+    MAZE_LAUNCH(k_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, in, out);
 }
 
 // out = sum(scale * v[i]) with the single-pass deterministic reduction; with
@@ -447,7 +467,7 @@ void launch_sum_scaled(hipStream_t s, long n, double scale, const double* v, dou
     long b = (n + 4L * kBlock - 1) / (4L * kBlock);
     if (b < 1) b = 1;
     if (b > 2048) b = 2048;
-    hipLaunchKernelGGL(k_sum_scaled, dim3((int)b), dim3(kBlock), 0, s, n, scale, v, red, out);
+    MAZE_LAUNCH(k_sum_scaled, dim3((int)b), dim3(kBlock), 0, s, n, scale, v, red, out);
 }
 
 void launch_half_sum(hipStream_t s, long n, const double* phi, double* out, Reduce red) {
@@ -484,8 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_rbgs(GridDims d, const double* __res
 
 void launch_rbgs(hipStream_t s, const GridDims& d, const double* in, double* out, int iters) {
     for (int it = 0; it < iters; it++) {
-        hipLaunchKernelGGL(k_rbgs, dim3(d.blocks), dim3(kBlock), 0, s, d, in, out, 0);
-        hipLaunchKernelGGL(k_rbgs, dim3(d.blocks), dim3(kBlock), 0, s, d, in, out, 1);
+        MAZE_LAUNCH(k_rbgs, dim3(d.blocks), dim3(kBlock), 0, s, d, in, out, 0);
+        MAZE_LAUNCH(k_rbgs, dim3(d.blocks), dim3(kBlock), 0, s, d, in, out, 1);
     }
 }
 
@@ -534,7 +554,7 @@ __global__ void k_restrict27(int n, const double* __restrict__ in, double* __res
 
 void launch_restrict27(hipStream_t s, int n, const double* in, double* out) {
     const long M = n / 2;
-    hipLaunchKernelGGL(k_restrict27, dim3(pw_blocks(M * M * M)), dim3(kBlock), 0, s, n, in, out);
+    MAZE_LAUNCH(k_restrict27, dim3(pw_blocks(M * M * M)), dim3(kBlock), 0, s, n, in, out);
 }
 
 // out += trilinear(in) from an nc^3 grid to (2 nc)^3 (library/src/multigrid.c:393-463,
@@ -564,7 +584,7 @@ __global__ void k_prolong_add(int nc, const double* __restrict__ in, double* __r
 
 void launch_prolong_add(hipStream_t s, int nc, const double* in, double* out) {
     const long N = 2L * nc;
-    hipLaunchKernelGGL(k_prolong_add, dim3(pw_blocks(N * N * N)), dim3(kBlock), 0, s, nc, in, out);
+    MAZE_LAUNCH(k_prolong_add, dim3(pw_blocks(N * N * N)), dim3(kBlock), 0, s, nc, in, out);
 }
 
 }  // namespace maze

@@ -147,7 +147,7 @@ __global__ void k_mg_restrict(int n, const float* __restrict__ in, float* __rest
 
 void launch_mg_restrict(hipStream_t s, int n, const float* r, float* bc, CGState* st) {
     const long M = n / 2;
-    hipLaunchKernelGGL(k_mg_restrict, dim3(pw_blocks_mg(M * M * M)), dim3(kBlock), 0, s, n, r, bc, st);
+    MAZE_LAUNCH(k_mg_restrict, dim3(pw_blocks_mg(M * M * M)), dim3(kBlock), 0, s, n, r, bc, st);
 }
 
 // d += trilinear(e) from an nc^3 grid to (2 nc)^3.
@@ -181,7 +181,7 @@ __global__ void k_mg_prolong_add(int nc, const float* __restrict__ e, float* __r
 
 void launch_mg_prolong_add(hipStream_t s, int nc, const float* ec, float* d, CGState* st) {
     const long N = 2L * nc;
-    hipLaunchKernelGGL(k_mg_prolong_add, dim3(pw_blocks_mg(N * N * N)), dim3(kBlock), 0, s, nc, ec,
+    MAZE_LAUNCH(k_mg_prolong_add, dim3(pw_blocks_mg(N * N * N)), dim3(kBlock), 0, s, nc, ec,
                        d, st);
 }
 
@@ -409,10 +409,10 @@ constexpr int kBigNT = 1024, kSmallNT = 512;
 void launch_mg_down(hipStream_t s, int n, const float* b, float* d2, float* bc, double c,
                     CGState* st) {
     if (n >= 64)
-        hipLaunchKernelGGL((k_mg_down<8, 16, 16, kBigNT>), dim3(mg_tiles<8, 16, 16>(n)),
+        MAZE_LAUNCH((k_mg_down<8, 16, 16, kBigNT>), dim3(mg_tiles<8, 16, 16>(n)),
                            dim3(kBigNT), 0, s, n, b, d2, bc, (float)c, st);
     else
-        hipLaunchKernelGGL((k_mg_down<4, 8, 8, kSmallNT>), dim3(mg_tiles<4, 8, 8>(n)),
+        MAZE_LAUNCH((k_mg_down<4, 8, 8, kSmallNT>), dim3(mg_tiles<4, 8, 8>(n)),
                            dim3(kSmallNT), 0, s, n, b, d2, bc, (float)c, st);
 }
 
@@ -420,17 +420,17 @@ void launch_mg_up(hipStream_t s, int n, const float* b, const float* d2, const f
                   float* d, double c, CGState* st, int dot, Reduce red) {
     if (n >= 64) {
         if (dot)
-            hipLaunchKernelGGL((k_mg_up<8, 16, 16, kBigNT, true>), dim3(mg_tiles<8, 16, 16>(n)),
+            MAZE_LAUNCH((k_mg_up<8, 16, 16, kBigNT, true>), dim3(mg_tiles<8, 16, 16>(n)),
                                dim3(kBigNT), 0, s, n, b, d2, e, d, (float)c, red, st);
         else
-            hipLaunchKernelGGL((k_mg_up<8, 16, 16, kBigNT, false>), dim3(mg_tiles<8, 16, 16>(n)),
+            MAZE_LAUNCH((k_mg_up<8, 16, 16, kBigNT, false>), dim3(mg_tiles<8, 16, 16>(n)),
                                dim3(kBigNT), 0, s, n, b, d2, e, d, (float)c, red, st);
     } else {
         if (dot)
-            hipLaunchKernelGGL((k_mg_up<4, 8, 8, kSmallNT, true>), dim3(mg_tiles<4, 8, 8>(n)),
+            MAZE_LAUNCH((k_mg_up<4, 8, 8, kSmallNT, true>), dim3(mg_tiles<4, 8, 8>(n)),
                                dim3(kSmallNT), 0, s, n, b, d2, e, d, (float)c, red, st);
         else
-            hipLaunchKernelGGL((k_mg_up<4, 8, 8, kSmallNT, false>), dim3(mg_tiles<4, 8, 8>(n)),
+            MAZE_LAUNCH((k_mg_up<4, 8, 8, kSmallNT, false>), dim3(mg_tiles<4, 8, 8>(n)),
                                dim3(kSmallNT), 0, s, n, b, d2, e, d, (float)c, red, st);
     }
 }
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse(int c, const float* __rest
 
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st) {
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kBlock), 0, s, c, b, e, Qf, ilam, st);
+    MAZE_LAUNCH(k_mg_coarse, dim3(1), dim3(kBlock), 0, s, c, b, e, Qf, ilam, st);
 }
 
 // ---- the coarse tail of the V-cycle in one work-group --------------------------
@@ -719,7 +719,7 @@ void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float
     TailLevels T;
     T.nlev = nlev;
     for (int l = 0; l < nlev; l++) T.n[l] = n[l];
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s, T, b, d, (float)c, Qf, ilam,
+    MAZE_LAUNCH(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s, T, b, d, (float)c, Qf, ilam,
                        st);
 }
 
@@ -784,10 +784,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, const float* p_o
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                     float* p_new, CGState* st, Reduce red, int first) {
     if (first)
-        hipLaunchKernelGGL(k_pcg_dir<true>, dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
+        MAZE_LAUNCH(k_pcg_dir<true>, dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
                            red, st);
     else
-        hipLaunchKernelGGL(k_pcg_dir<false>, dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v,
+        MAZE_LAUNCH(k_pcg_dir<false>, dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v,
                            p_new, red, st);
 }
 

@@ -48,7 +48,7 @@ __global__ void k_cic_base(int np, int n, double h, const double* __restrict__ p
 
 void launch_cic_base(hipStream_t s, const PDev& p) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_cic_base, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h,
+    MAZE_LAUNCH(k_cic_base, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h,
                        p.pos, p.base, p.keys, p.ids);
 }
 
@@ -143,7 +143,7 @@ __global__ void k_cic_gather(int np, int n, double h, const double* __restrict__
 
 void launch_cic_gather(hipStream_t s, const PDev& p, double* q) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_cic_gather, dim3(blocks_for(8L * p.np)), dim3(kBlock), 0, s, p.np, p.n,
+    MAZE_LAUNCH(k_cic_gather, dim3(blocks_for(8L * p.np)), dim3(kBlock), 0, s, p.np, p.n,
                        p.h, p.pos, p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
 }
 
@@ -184,7 +184,7 @@ __global__ void k_force_fd(int np, int n, double h, const double* __restrict__ p
 
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_force_fd, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
+    MAZE_LAUNCH(k_force_fd, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
                        p.base, p.charges, p.pos, p.fe);
 }
 
@@ -362,12 +362,12 @@ void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
     }
     const int nb = tf_blocks(p.np);
     if (p.n_typ <= kTfLdsTypes)
-        hipLaunchKernelGGL(k_tf_tiles<true>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
+        MAZE_LAUNCH(k_tf_tiles<true>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
                            p.n_typ, p.L, 1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart);
     else
-        hipLaunchKernelGGL(k_tf_tiles<false>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
+        MAZE_LAUNCH(k_tf_tiles<false>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
                            p.n_typ, p.L, 1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart);
-    hipLaunchKernelGGL(k_tf_fold, dim3(nb), dim3(kTfB * kFoldGroups), 0, s, p.np, nb, p.tpart,
+    MAZE_LAUNCH(k_tf_fold, dim3(nb), dim3(kTfB * kFoldGroups), 0, s, p.np, nb, p.tpart,
                        p.fn, p.eion, p.counts);
     launch_sum_scaled(s, p.np, 0.5, p.eion, &ps->pe_tf, red);
 }
@@ -384,7 +384,7 @@ __global__ void k_ftot(long n, const double* fe, const double* fn, double* ft) {
 
 void launch_force_total(hipStream_t s, const PDev& p) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_ftot, dim3(blocks_for(3L * p.np)), dim3(kBlock), 0, s, 3L * p.np, p.fe,
+    MAZE_LAUNCH(k_ftot, dim3(blocks_for(3L * p.np)), dim3(kBlock), 0, s, 3L * p.np, p.fe,
                        p.fn, p.ft);
 }
 
@@ -412,7 +412,7 @@ __global__ void k_kinetic(int np, const double* vel, const double* mass, PScalar
 
 void launch_kinetic(hipStream_t s, const PDev& p, PScalars* ps, Reduce, double T_target,
                     int want_scale) {
-    hipLaunchKernelGGL(k_kinetic, dim3(1), dim3(kBlock), 0, s, p.np, p.vel, p.mass, ps, T_target,
+    MAZE_LAUNCH(k_kinetic, dim3(1), dim3(kBlock), 0, s, p.np, p.vel, p.mass, ps, T_target,
                        want_scale);
 }
 
@@ -440,7 +440,7 @@ __global__ void k_verlet1(int np, double dt, double L, double* pos, double* vel,
 
 void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_verlet1, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, dt, p.L, p.pos,
+    MAZE_LAUNCH(k_verlet1, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, dt, p.L, p.pos,
                        p.vel, p.ft, p.mass, ps, rescale);
 }
 
@@ -453,7 +453,7 @@ __global__ void k_kick(int np, double cdt, double* vel, const double* ft, const 
 
 void launch_kick(hipStream_t s, const PDev& p, double cdt) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_kick, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, cdt, p.vel, p.ft,
+    MAZE_LAUNCH(k_kick, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, cdt, p.vel, p.ft,
                        p.mass);
 }
 
@@ -466,7 +466,7 @@ __global__ void k_drift(int np, double c2dt, double L, double* pos, const double
 
 void launch_drift(hipStream_t s, const PDev& p, double c2dt) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_drift, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, c2dt, p.L, p.pos,
+    MAZE_LAUNCH(k_drift, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, c2dt, p.L, p.pos,
                        p.vel);
 }
 
@@ -513,7 +513,7 @@ __global__ void k_o_block(int np, uint64_t seed, uint64_t call, double c1s, doub
 void launch_o_block(hipStream_t s, const PDev& p, uint64_t seed, uint64_t call, double c1s,
                     double var2) {
     if (p.np == 0) return;
-    hipLaunchKernelGGL(k_o_block, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, seed, call, c1s,
+    MAZE_LAUNCH(k_o_block, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, seed, call, c1s,
                        var2, p.mass, p.vel);
 }
 
@@ -530,7 +530,7 @@ __global__ void k_momentum(int np, const double* vel, const double* mass, PScala
 }
 
 void launch_momentum(hipStream_t s, const PDev& p, PScalars* ps) {
-    hipLaunchKernelGGL(k_momentum, dim3(1), dim3(kBlock), 0, s, p.np, p.vel, p.mass, ps);
+    MAZE_LAUNCH(k_momentum, dim3(1), dim3(kBlock), 0, s, p.np, p.vel, p.mass, ps);
 }
 
 // v -= 2 (sum of v over the ion's type) / n_p (library/src/particles.c:841-857).
@@ -551,7 +551,7 @@ __global__ void k_rescale(int np, int n_typ, const int* types, double* vel, PSca
 }
 
 void launch_rescale_velocities(hipStream_t s, const PDev& p, PScalars* ps) {
-    hipLaunchKernelGGL(k_rescale, dim3(1), dim3(kBlock), 0, s, p.np, p.n_typ, p.types, p.vel, ps);
+    MAZE_LAUNCH(k_rescale, dim3(1), dim3(kBlock), 0, s, p.np, p.n_typ, p.types, p.vel, ps);
 }
 
 }  // namespace maze

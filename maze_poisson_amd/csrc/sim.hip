@@ -13,6 +13,8 @@
 
 namespace maze {
 
+thread_local LaunchTimer g_launch_timer;
+
 static constexpr double kBoltz = 3.1668e-6;  // Hartree / K, library/include/constants.h:10
 static constexpr int kMaxPartials = 1 << 15;
 
@@ -85,6 +87,8 @@ void Sim::free_grid() {
     for (float** a : {&d32, &r32, &q32a, &q32b}) dfree(*a);
     free_mg();
     sched.clear();
+    dfree(y_hist);
+    hist_ok = 0;
     G = 0;
 }
 
@@ -505,11 +509,16 @@ int Sim::maze_fp64() {
 // so a step normally costs one host round trip.  Further passes, if needed,
 // are driven from the host.  Returns the total number of inner iterations, or
 // -1 if the outer loop did not converge in kMaxOuter passes.
+//
+// y_extrap: the loop starts from 2 y_n-1 - y_n-2 instead of y_n-1 (the
+// converged y satisfies the same stop test; only the start differs).  The
+// three y buffers then rotate: y (y_n-1), y_hist (y_n-2) and y_alt (spare).
 int Sim::maze_mixed() {
     if (!d32) {
         d32 = dalloc<float>(G); r32 = dalloc<float>(G);
         q32a = dalloc<float>(G); q32b = dalloc<float>(G); y_alt = dalloc<double>(G);
     }
+    if (y_extrap && !y_hist) y_hist = dalloc<double>(G);
     const bool use_mg = precision == kMixedMG && !pb && setup_mg();
     if (use_mg) mg[0].b = r32;
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
@@ -520,15 +529,25 @@ int Sim::maze_mixed() {
     phi = tmp;
     tmp = old_phip;
     const int inner_limit = n * n;
-    const RefineArgs ra{tol, inner_rtol > 0 ? inner_rtol : (use_mg ? 1e-2 : 1e-4), inner_limit,
+    const RefineArgs ra{tol, inner_rtol > 0 ? inner_rtol : (use_mg ? 1e-3 : 1e-4), inner_limit,
                         use_mg ? 1 : 0};
+    // The two buffers the refinement passes alternate between (st->ysel).
+    const bool extrap = y_extrap && hist_ok;
+    double* ya = extrap ? y_alt : y;
+    double* yb = extrap ? y_hist : y_alt;
+    if (y_extrap && !hist_ok) launch_copy(stream, G, y, y_hist);
     // Each refinement pass also starts the next pass's inner solve.
     auto refine = [&](int add) {
         timed(kTRefine, 0, [&] {
-            launch_refine(stream, dims, pbc(), y, y_alt, d32, sigma, r32, ra, st, red, add);
+            launch_refine(stream, dims, pbc(), ya, yb, d32, sigma, r32, ra, st, red, add);
         });
     };
-    refine(0);
+    if (extrap)
+        timed(kTRefine, 0, [&] {
+            launch_refine_extrap(stream, dims, pbc(), y, y_hist, y_alt, sigma, r32, ra, st, red);
+        });
+    else
+        refine(0);
     auto queue_pass = [&](int k, int m, bool poll) {
         cur_pass = k;
         int launched = 0;
@@ -556,7 +575,17 @@ int Sim::maze_mixed() {
     const int passes = std::min(st_host->passes, kMaxPasses);
     sched.assign(st_host->pass_iters, st_host->pass_iters + passes);
     prof_flush_passes(st_host->pass_iters, passes);
-    if (st_host->ysel) std::swap(y, y_alt);
+    double* fin = st_host->ysel ? yb : ya;
+    if (!extrap) {
+        if (fin != y) std::swap(y, y_alt);
+        if (y_extrap) hist_ok = 1;  // y_hist holds y_n-1 (copied above)
+    } else {
+        // y_n-1 becomes the history, the result the current y.
+        double* other = fin == ya ? yb : ya;
+        y_hist = y;
+        y = fin;
+        y_alt = other;
+    }
     launch_axpy(stream, G, -1.0, y, phi);
     const int res = st_host->outer_done ? st_host->inner_total : -1;
     last_res = res;
@@ -653,6 +682,7 @@ int Sim::check_charge() {
 // library/src/grid_maze_lcg.c:100-124 and the matching init_field of the
 // other grid types: cold solve A phi = -4 pi q / (h eps) from zero.
 void Sim::init_field() {
+    hist_ok = 0;
     double c = -4 * kPi / h;
     if (!pb) c /= eps_s;
     launch_fill(stream, G, 0.0, y);

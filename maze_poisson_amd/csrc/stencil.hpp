@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(GridDims d, PBCoef pb, Src src,
 template <bool PB, class Src, class Body>
 static void run_pass(hipStream_t s, const GridDims& d, const PBCoef& pb, const Src& src,
                      const Body& body, Reduce red, CGState* st) {
-    hipLaunchKernelGGL((k_pass<PB, Src, Body>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, src,
+    MAZE_LAUNCH((k_pass<PB, Src, Body>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, src,
                        body, red, st);
 }
 

@@ -369,6 +369,201 @@ int or_verlet_poisson_mg(double tol, double h, double *phi, double *phi_prev, co
     return res;
 }
 
+/* ---- Poisson-Boltzmann multigrid ---------------------------------------- */
+
+/* RBGS for A_pb u = in: each colour sets u = (sum of eps_f u_nb - in) / diag,
+ * diag = sum of the six face coefficients + k2 (floored at 1e-14), red
+ * ((i+j+k) even) then black, `iters` times, in place.
+ * library/src/multigrid_pb.c:596-690 (term order :637-645). */
+void or_mg_smooth_pb_rbgs(const double *in, double *out, int n, int iters, const double *ex,
+                          const double *ey, const double *ez, const double *k2) {
+    const long N = n, N2 = N * N;
+    for (int it = 0; it < iters; it++) {
+        for (int color = 0; color < 2; color++) {
+            for (long i = 0; i < N; i++) {
+                const long i0 = i * N2, ip = wrapi(i + 1, N) * N2, im = wrapi(i - 1, N) * N2;
+                for (long j = 0; j < N; j++) {
+                    const long j0 = j * N, jp = wrapi(j + 1, N) * N, jm = wrapi(j - 1, N) * N;
+                    for (long k = (i + j + color) & 1; k < N; k += 2) {
+                        const long kp = wrapi(k + 1, N), km = wrapi(k - 1, N);
+                        const long c = i0 + j0 + k, cx = im + j0 + k, cy = i0 + jm + k,
+                                   cz = i0 + j0 + km;
+                        const double ex_p = ex[c], ex_m = ex[cx], ey_p = ey[c], ey_m = ey[cy];
+                        const double ez_p = ez[c], ez_m = ez[cz];
+                        double diag = ex_p + ex_m + ey_p + ey_m + ez_p + ez_m + k2[c];
+                        if (diag < 1e-14) diag = 1e-14;
+                        const double rhs = ex_p * out[ip + j0 + k] + ex_m * out[cx] +
+                                           ey_p * out[i0 + jp + k] + ey_m * out[cy] +
+                                           ez_p * out[i0 + j0 + kp] + ez_m * out[cz] - in[c];
+                        out[c] = rhs / diag;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* Face-centred coefficient onto the (n/2)^3 grid: 2x2 average over the plane
+ * normal to `axis` (0: x faces on fine plane 2I; 1, 2: fine planes 2I and
+ * 2I+1).  library/src/multigrid_pb.c:375-500 (sum order per axis). */
+void or_mg_restrict_eps(const double *in, double *out, int n, int axis) {
+    const long N = n, N2 = N * N, M = N / 2;
+    for (long I = 0; I < M; I++) {
+        const long i0 = 2 * I * N2, i1 = i0 + N2;
+        for (long J = 0; J < M; J++) {
+            const long j0 = 2 * J * N, j1 = wrapi(2 * J + 1, N) * N;
+            for (long K = 0; K < M; K++) {
+                const long k0 = 2 * K, k1 = wrapi(2 * K + 1, N);
+                double s;
+                if (axis == 0)
+                    s = in[i0 + j0 + k0] + in[i0 + j0 + k1] + in[i0 + j1 + k0] + in[i0 + j1 + k1];
+                else if (axis == 1)
+                    s = in[i0 + j0 + k0] + in[i0 + j0 + k1] + in[i1 + j0 + k0] + in[i1 + j0 + k1];
+                else
+                    s = in[i0 + j0 + k0] + in[i1 + j0 + k0] + in[i0 + j1 + k0] + in[i1 + j1 + k0];
+                out[(I * M + J) * M + K] = s * 0.25;
+            }
+        }
+    }
+}
+
+/* Cell-centred screening term onto the (n/2)^3 grid: 8-point average of the
+ * fine cells (2I+1, 2I+2) x (2J, 2J+1) x (2K, 2K+1) (the serial i offset is
+ * 1).  library/src/multigrid_pb.c:503-568. */
+void or_mg_restrict_k2(const double *in, double *out, int n) {
+    const long N = n, N2 = N * N, M = N / 2;
+    for (long i = 1; i < N; i += 2) {
+        const long i0 = i * N2, i1 = wrapi(i + 1, N) * N2, a = (i / 2) * M * M;
+        for (long j = 0; j < N; j += 2) {
+            const long j0 = j * N, j1 = wrapi(j + 1, N) * N, b = (j / 2) * M;
+            for (long k = 0; k < N; k += 2) {
+                const long k1 = wrapi(k + 1, N);
+                const double sum = in[i0 + j0 + k] + in[i0 + j0 + k1] + in[i0 + j1 + k] +
+                                   in[i0 + j1 + k1] + in[i1 + j0 + k] + in[i1 + j0 + k1] +
+                                   in[i1 + j1 + k] + in[i1 + j1 + k1];
+                out[a + b + k / 2] = (1.0 / 8.0) * sum;
+            }
+        }
+    }
+}
+
+/* Unpreconditioned coarse CG for A_pb x = b (p = -r, relative
+ * infinity-norm stop, breakdown guard); returns the iterations performed.
+ * library/src/multigrid_pb.c:16-98 */
+int or_mg_cg_coarse_pb(const double *b, double *x, int n, int maxit, double rtol,
+                       const double *ex, const double *ey, const double *ez, const double *k2) {
+    const long G = (long)n * n * n;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *p = (double *)malloc(G * sizeof(double));
+    double *Ap = (double *)malloc(G * sizeof(double));
+    int k;
+    or_laplace_pb(x, r, n, ex, ey, ez, k2);
+    axpy(b, r, -1.0, G);
+    memcpy(p, r, G * sizeof(double));
+    scal(p, -1.0, G);
+    double r0 = amax(r, G);
+    if (r0 == 0.0) { free(r); free(p); free(Ap); return 0; }
+    double rr = dot(r, r, G);
+    for (k = 0; k < maxit; ++k) {
+        or_laplace_pb(p, Ap, n, ex, ey, ez, k2);
+        double denom = dot(p, Ap, G);
+        if (fabs(denom) < 1e-300) break;
+        double alpha = rr / denom;
+        axpy(p, x, alpha, G);
+        axpy(Ap, r, alpha, G);
+        if (amax(r, G) <= rtol * r0) { ++k; break; }
+        double rn = dot(r, r, G);
+        double beta = rn / rr;
+        rr = rn;
+        scal(p, beta, G);
+        axpy(r, p, -1.0, G);
+    }
+    free(r); free(p); free(Ap);
+    return k;
+}
+
+/* One reference PB V-cycle: two-grid, `sm` RBGS-PB pre/post sweeps, 27-point
+ * restriction of the residual, coarse coefficients by restriction_eps /
+ * restriction_k2screen, coarse CG-PB (<= 50 iterations, rtol 1e-5), trilinear
+ * prolongation.  library/src/multigrid_pb.c:275-355. */
+void or_mg_vcycle_pb(const double *in, double *out, int n, int sm, const double *ex,
+                     const double *ey, const double *ez, const double *k2) {
+    const long G = (long)n * n * n;
+    const int nc = n / 2;
+    const long Gc = (long)nc * nc * nc;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *rhs = (double *)malloc(Gc * sizeof(double));
+    double *e = (double *)calloc(Gc, sizeof(double));
+    double *cx = (double *)malloc(Gc * sizeof(double));
+    double *cy = (double *)malloc(Gc * sizeof(double));
+    double *cz = (double *)malloc(Gc * sizeof(double));
+    double *ck = (double *)malloc(Gc * sizeof(double));
+    or_mg_smooth_pb_rbgs(in, out, n, sm, ex, ey, ez, k2);
+    or_laplace_pb(out, r, n, ex, ey, ez, k2);
+    scal(r, -1.0, G);
+    axpy(in, r, 1.0, G);
+    or_mg_restrict27(r, rhs, n);
+    or_mg_restrict_eps(ex, cx, n, 0);
+    or_mg_restrict_eps(ey, cy, n, 1);
+    or_mg_restrict_eps(ez, cz, n, 2);
+    or_mg_restrict_k2(k2, ck, n);
+    or_mg_cg_coarse_pb(rhs, e, nc, 50, 1e-5, cx, cy, cz, ck);
+    or_mg_prolong_trilinear(e, r, nc);
+    axpy(r, out, 1.0, G);
+    or_mg_smooth_pb_rbgs(in, out, n, sm, ex, ey, ez, k2);
+    free(r); free(rhs); free(e); free(cx); free(cy); free(cz); free(ck);
+}
+
+/* MaZe PB step with repeated PB V-cycles (4 sweeps), stop on max|A_pb y -
+ * sigma| <= tol; returns the 1-based cycle count or -1 after 1000.
+ * library/src/verlet.c:211-271. */
+int or_verlet_pb_multigrid(double tol, double h, double *phi, double *phi_prev, const double *q,
+                           double *y, int n, const double *ex, const double *ey,
+                           const double *ez, const double *k2) {
+    const long G = (long)n * n * n;
+    double *tmp = (double *)malloc(G * sizeof(double));
+    double *sigma = (double *)malloc(G * sizeof(double));
+    int res = -1, it = 0;
+    for (long i = 0; i < G; i++) {
+        double a = phi[i];
+        phi[i] = 2 * a - phi_prev[i];
+        phi_prev[i] = a;
+    }
+    or_laplace_pb(phi, sigma, n, ex, ey, ez, k2);
+    axpy(q, sigma, (4 * OR_PI) / h, G);
+    while (it < 1000) {
+        or_mg_vcycle_pb(sigma, y, n, 4, ex, ey, ez, k2);
+        or_laplace_pb(y, tmp, n, ex, ey, ez, k2);
+        axpy(sigma, tmp, -1.0, G);
+        double a = amax(tmp, G);
+        it++;
+        if (a <= tol) { res = it; break; }
+    }
+    axpy(y, phi, -1.0, G);
+    free(tmp); free(sigma);
+    return res;
+}
+
+/* Non-MaZe PB multigrid update: repeated PB V-cycles on A_pb phi = b from the
+ * previous phi, stop on max|A_pb phi - b| <= tol.
+ * library/src/grid_multigrid.c:149-175. */
+int or_mg_pb_update(double tol, const double *b, double *phi, int n, const double *ex,
+                    const double *ey, const double *ez, const double *k2) {
+    const long G = (long)n * n * n;
+    double *tmp = (double *)malloc(G * sizeof(double));
+    int res = -1, it = 0;
+    while (it < 1000) {
+        or_mg_vcycle_pb(b, phi, n, 4, ex, ey, ez, k2);
+        or_laplace_pb(phi, tmp, n, ex, ey, ez, k2);
+        axpy(b, tmp, -1.0, G);
+        double a = amax(tmp, G);
+        it++;
+        if (a <= tol) { res = it; break; }
+    }
+    free(tmp);
+    return res;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Particles                                                                */
 

@@ -56,6 +56,21 @@ void or_mg_prolong_trilinear(const double *in, double *out, int nc);
 int or_mg_cg_coarse(const double *b, double *x, int n, int maxit, double rtol);
 void or_mg_vcycle(const double *in, double *out, int n, int sm);
 
+/* ---- reference Poisson-Boltzmann multigrid ------------------------------ */
+void or_mg_smooth_pb_rbgs(const double *in, double *out, int n, int iters, const double *ex,
+                          const double *ey, const double *ez, const double *k2);
+void or_mg_restrict_eps(const double *in, double *out, int n, int axis);
+void or_mg_restrict_k2(const double *in, double *out, int n);
+int or_mg_cg_coarse_pb(const double *b, double *x, int n, int maxit, double rtol,
+                       const double *ex, const double *ey, const double *ez, const double *k2);
+void or_mg_vcycle_pb(const double *in, double *out, int n, int sm, const double *ex,
+                     const double *ey, const double *ez, const double *k2);
+int or_verlet_pb_multigrid(double tol, double h, double *phi, double *phi_prev, const double *q,
+                           double *y, int n, const double *ex, const double *ey,
+                           const double *ez, const double *k2);
+int or_mg_pb_update(double tol, const double *b, double *phi, int n, const double *ex,
+                    const double *ey, const double *ez, const double *k2);
+
 /* ---- particles --------------------------------------------------------- */
 double or_cic_weight(double x, double L, double h);
 void or_cic_neighbors(const double *pos, int np, int n, double h, int64_t *nb);

@@ -225,11 +225,11 @@ static int lcg_update(void) {
     return or_conj_grad(g.y, g.phi_n, g.tol, g.n); /* precond ignored here, as in grid_lcg.c */
 }
 
-/* library/src/grid_multigrid.c:110-212 (Poisson branch) */
+/* library/src/grid_multigrid.c:110-212 (Poisson branch; PB branch :149-175) */
 static int mg_update(void) {
     const long G = g.G;
     double c = -4 * OR_PI / g.h;
-    c /= g.eps_s;
+    if (!g.pb) c /= g.eps_s;
     for (long i = 0; i < G; i++) {
         double a = g.phi_n[i];
         g.phi_n[i] = 2 * a - g.phi_p[i];
@@ -239,6 +239,11 @@ static int mg_update(void) {
     double *t = (double *)malloc(G * sizeof(double));
     for (long i = 0; i < G; i++) b[i] = g.q[i];
     for (long i = 0; i < G; i++) b[i] *= c;
+    if (g.pb) {
+        int r = or_mg_pb_update(g.tol, b, g.phi_n, g.n, g.ex, g.ey, g.ez, g.k2);
+        free(b); free(t);
+        return r;
+    }
     int res = -1, k = 0;
     while (k < 1000) {
         or_mg_vcycle(b, g.phi_n, g.n, 3);
@@ -265,11 +270,11 @@ int solver_update_field(void) {
         return or_verlet_poisson(g.tol, g.h * g.eps_s, g.phi_n, g.phi_p, g.q, g.y, g.n,
                                  g.precond == P_JACOBI);
     case G_MG:
-        if (g.pb) die("oracle: PB multigrid not provided");
         return mg_update();
     case G_MAZE_MG: {
-        if (g.pb) die("oracle: PB multigrid not provided");
-        int r = or_verlet_poisson_mg(g.tol, g.h * g.eps_s, g.phi_n, g.phi_p, g.q, g.y, g.n);
+        int r = g.pb ? or_verlet_pb_multigrid(g.tol, g.h, g.phi_n, g.phi_p, g.q, g.y, g.n, g.ex,
+                                              g.ey, g.ez, g.k2)
+                     : or_verlet_poisson_mg(g.tol, g.h * g.eps_s, g.phi_n, g.phi_p, g.q, g.y, g.n);
         if (g.precond != P_NONE) die("Maze Multigrid with preconditioner not implemented yet.");
         return r;
     }
