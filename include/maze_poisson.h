@@ -224,6 +224,16 @@ void maze_test_rbgs(const double *in, double *out, int n, int iters);
 int maze_test_restrict27(const double *in, double *out, int n);
 int maze_test_prolong_add(const double *in, double *out, int nc);
 int maze_test_mg_vcycle(const double *in, double *out, int n);
+/* Reference Poisson-Boltzmann multigrid pieces (library/src/multigrid_pb.c):
+ * RBGS-PB sweeps in place, the coarse coefficient restrictions
+ * ((n/2)^3 outputs), one two-grid PB V-cycle (n even, >= 32). */
+void maze_test_rbgs_pb(const double *in, double *out, int n, int iters, const double *eps_x,
+                       const double *eps_y, const double *eps_z, const double *k2);
+int maze_test_restrict_pb_coef(const double *eps_x, const double *eps_y, const double *eps_z,
+                               const double *k2, double *c_eps_x, double *c_eps_y,
+                               double *c_eps_z, double *c_k2, int n);
+int maze_test_mg_vcycle_pb(const double *in, double *out, int n, const double *eps_x,
+                           const double *eps_y, const double *eps_z, const double *k2);
 int maze_test_mg_pcg(const double *b, double *x, double rtol, int n);
 /* The fused FP32 multigrid level kernels (n even, c = damping / 6):
  * down: d2 = two damped-Jacobi sweeps from zero on A d = b, bc = 4 x

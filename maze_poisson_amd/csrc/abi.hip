@@ -440,6 +440,54 @@ int maze_test_mg_tail(const float* b, float* d, int n) {
     return ns[0];
 }
 
+void maze_test_rbgs_pb(const double* in, double* out, int n, int iters, const double* ex,
+                       const double* ey, const double* ez, const double* k2) {
+    Sim* s = scratch_sim(n);
+    s->init_pb(0.0, 0.0);
+    s->set_eps_k2(ex, ey, ez, k2);
+    MAZE_HIP(hipMemcpy(s->sigma, in, s->G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->y, out, s->G * sizeof(double), hipMemcpyHostToDevice));
+    maze::launch_rbgs_pb(s->stream, s->dims, s->pbc(), s->sigma, s->y, iters);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+}
+
+int maze_test_restrict_pb_coef(const double* ex, const double* ey, const double* ez,
+                               const double* k2, double* cex, double* cey, double* cez,
+                               double* ck2, int n) {
+    if (n < 2 || n % 2 != 0) return -1;
+    Sim* s = scratch_sim(n);
+    s->init_pb(0.0, 0.0);
+    s->set_eps_k2(ex, ey, ez, k2);
+    const long Gc = s->G / 8;
+    double* d = nullptr;
+    MAZE_HIP(hipMalloc(&d, 4 * Gc * sizeof(double)));
+    maze::launch_restrict_pb_coef(s->stream, n, s->pbc(), d, d + Gc, d + 2 * Gc, d + 3 * Gc);
+    s->sync();
+    double* outs[4] = {cex, cey, cez, ck2};
+    for (int a = 0; a < 4; a++)
+        MAZE_HIP(hipMemcpy(outs[a], d + a * Gc, Gc * sizeof(double), hipMemcpyDeviceToHost));
+    MAZE_HIP(hipFree(d));
+    delete s;
+    return 0;
+}
+
+int maze_test_mg_vcycle_pb(const double* in, double* out, int n, const double* ex,
+                           const double* ey, const double* ez, const double* k2) {
+    if (n < 32 || n % 2 != 0) return -1;
+    Sim* s = scratch_sim(n);
+    s->init_pb(0.0, 0.0);
+    s->set_eps_k2(ex, ey, ez, k2);
+    MAZE_HIP(hipMemcpy(s->sigma, in, s->G * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->y, out, s->G * sizeof(double), hipMemcpyHostToDevice));
+    s->mg_vcycle(s->sigma, s->y);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    delete s;
+    return 0;
+}
+
 int maze_test_mg_vcycle(const double* in, double* out, int n) {
     if (n < 32 || n % 2 != 0) return -1;
     Sim* s = scratch_sim(n);

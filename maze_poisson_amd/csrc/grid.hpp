@@ -26,6 +26,7 @@ struct CGState {
     int first;       // 1 until the first direction has been formed
     int count_mode;  // 1: result is the number of iterations (coarse CG)
     int pcg;         // 1: beta comes from the preconditioner pass, not from <r, r>
+    int plain;       // 1: plain CG (p = -r), else the implicit-Jacobi recurrence (p = r/6)
     // Outer FP64 refinement loop of the mixed-precision solve, advanced by the
     // refinement pass so that a whole predicted schedule of passes can be
     // queued without host round trips (passes after convergence are no-ops).
@@ -59,7 +60,8 @@ void launch_pred_sigma(hipStream_t s, const GridDims& d, const PBCoef& pb, const
                        double* sigma);
 template <class Real>
 void launch_cg_init(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* x,
-                    const Real* b, Real* r, CGState* st, Reduce red, Real* zero_x = nullptr);
+                    const Real* b, Real* r, CGState* st, Reduce red, Real* zero_x = nullptr,
+                    int plain = 0);
 template <class Real>
 void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p_old,
                    const Real* r, Real* p_new, CGState* st, Reduce red, int first);
@@ -69,7 +71,7 @@ void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const 
                       Real* r, CGState* st, Reduce red, int zero_x = 0);
 // gated: the solve is started only while the outer refinement loop runs.
 void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit, int count_mode,
-                     int pcg = 0, int gated = 0);
+                     int pcg = 0, int gated = 0, int plain = 0);
 
 // Mixed precision, FP64 true residual of the refinement loop, b = sigma - A y.
 // First pass (add_d == 0): y = y0, starts the loop (ysel = 0).
@@ -102,8 +104,13 @@ void launch_sum_scaled(hipStream_t s, long n, double scale, const double* v, dou
 
 // Reference two-grid multigrid pieces.
 void launch_rbgs(hipStream_t s, const GridDims& d, const double* in, double* out, int iters);
-void launch_mg_residual(hipStream_t s, const GridDims& d, const double* in, const double* out,
-                        double* r);
+void launch_mg_residual(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* in,
+                        const double* out, double* r);
+// Reference Poisson-Boltzmann multigrid pieces (library/src/multigrid_pb.c).
+void launch_rbgs_pb(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* in,
+                    double* out, int iters);
+void launch_restrict_pb_coef(hipStream_t s, int n, const PBCoef& fine, double* ex, double* ey,
+                             double* ez, double* k2);
 void launch_restrict27(hipStream_t s, int n, const double* in, double* out);
 void launch_prolong_add(hipStream_t s, int nc, const double* in, double* out);
 void launch_true_residual_max(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,

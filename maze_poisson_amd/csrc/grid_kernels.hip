@@ -45,6 +45,8 @@ struct BodyPred {
 };
 
 // r = A x - b; <r, r/6>; max|r|   (library/src/plcg.c:41-58)
+// cr: 1/6 for the implicit-Jacobi recurrence, 1 for plain CG (st->plain):
+// <r, v> = -<r, r/6> with v = -r/6, or <r, r> (library/src/multigrid_pb.c:55).
 template <class Real>
 struct BodyInit {
     // Skipped when the reset before it left the solve stopped (a gated solve
@@ -53,6 +55,7 @@ struct BodyInit {
     const Real* b;
     Real* r;
     Real* x_zero;  // when set, x is taken as 0 and written back as 0
+    double cr;
     double s = 0.0, m = 0.0;
     using Raw = Real;
     __device__ Raw load(int c) const { return b[c]; }
@@ -61,11 +64,11 @@ struct BodyInit {
         r[c] = rc;
         if (x_zero) x_zero[c] = (Real)0;
         const double rd = (double)rc;
-        s += rd * (rd * (1.0 / 6.0));
+        s += rd * (rd * cr);
         m = fmax(m, fabs(rd));
     }
     __device__ static void finalize(CGState* st, double S, double M) {
-        st->rdv = -S;
+        st->rdv = st->plain ? S : -S;
         st->r0 = M;
         st->iter = 0;
         st->first = 1;
@@ -144,7 +147,9 @@ struct BodyUpdate {
             return;
         }
         if (!st->pcg) {
-            const double rn_dot_vn = -S / 6.0;
+            // <r_new, v_new> with v_new = -r_new/6, or <r_new, r_new> for plain CG
+            // (library/src/plcg.c:74-78, library/src/multigrid_pb.c:83-86).
+            const double rn_dot_vn = st->plain ? S : -S / 6.0;
             st->beta = rn_dot_vn / st->rdv;
             st->rdv = rn_dot_vn;
         }
@@ -208,6 +213,7 @@ struct BodyRefine {
         st->limit = limit;
         st->count_mode = 1;
         st->pcg = pcg;
+        st->plain = 0;
         st->rdv = -S;
         st->r0 = M;
         st->iter = 0;
@@ -249,13 +255,14 @@ void launch_pred_sigma(hipStream_t s, const GridDims& d, const PBCoef& pb, const
 
 template <class Real>
 void launch_cg_init(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* x,
-                    const Real* b, Real* r, CGState* st, Reduce red, Real* zero_x) {
+                    const Real* b, Real* r, CGState* st, Reduce red, Real* zero_x, int plain) {
+    const double cr = plain ? 1.0 : 1.0 / 6.0;
     if (x) {
-        BodyInit<Real> body{b, r, nullptr};
+        BodyInit<Real> body{b, r, nullptr, cr};
         DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcArr<Real>{x}, body, red, st));
     } else {
         // x == nullptr means "start from zero": zero_x (if given) is cleared here.
-        BodyInit<Real> body{b, r, zero_x};
+        BodyInit<Real> body{b, r, zero_x, cr};
         DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcZero{}, body, red, st));
     }
 }
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(GridDims d, PBCoef pb, const 
                                                    const Real* r, Real* p_new, Reduce red,
                                                    CGState* st) {
     if (st->done) return;
-    const SrcDir<Real, First> src{p_old, r, First ? 0.0 : st->beta};
+    const SrcDir<Real, First> src{p_old, r, First ? 0.0 : st->beta, st->plain ? -1.0 : 1.0 / 6.0};
     BodyDir<Real> bd{p_new};
     march<PB>(d, pb, src, bd);
     reduce_and_finalize<false>(bd, red, st);
@@ -307,9 +314,9 @@ void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const 
 }
 
 template void launch_cg_init<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
-                                     const double*, double*, CGState*, Reduce, double*);
+                                     const double*, double*, CGState*, Reduce, double*, int);
 template void launch_cg_init<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
-                                    const float*, float*, CGState*, Reduce, float*);
+                                    const float*, float*, CGState*, Reduce, float*, int);
 template void launch_cg_dir<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,
                                     const double*, double*, CGState*, Reduce, int);
 template void launch_cg_dir<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
@@ -320,12 +327,13 @@ template void launch_cg_update<float>(hipStream_t, const GridDims&, const PBCoef
                                       float*, float*, CGState*, Reduce, int);
 
 __global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int count_mode,
-                           int pcg, int gated) {
+                           int pcg, int gated, int plain) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         if (gated && st->outer_done) {
             st->done = 1;
             return;
         }
+        st->plain = plain;
         st->pcg = pcg;
         st->tol = tol;
         st->rtol = rtol;
@@ -339,9 +347,9 @@ __global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int 
 }
 
 void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit,
-                     int count_mode, int pcg, int gated) {
-    MAZE_LAUNCH(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode, pcg,
-                       gated);
+                     int count_mode, int pcg, int gated, int plain) {
+    MAZE_LAUNCH(k_cg_reset, dim3(1), dim3(64), 0, s, st, tol, rtol, limit, count_mode, pcg, gated,
+                plain);
 }
 
 // The y buffers are chosen on the device (st->ysel), so a queued pass needs
@@ -509,6 +517,100 @@ void launch_rbgs(hipStream_t s, const GridDims& d, const double* in, double* out
     }
 }
 
+// One colour of the Poisson-Boltzmann RBGS in place: out = (sum of eps_f out_nb
+// - in) / diag, diag = the six face coefficients + k2 floored at 1e-14.
+// library/src/multigrid_pb.c:596-690 (term order :628-645).
+__global__ __launch_bounds__(kBlock) void k_rbgs_pb(GridDims d, PBCoef pb,
+                                                    const double* __restrict__ in, double* out,
+                                                    int color) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int chunk = b % d.chunks;
+    const int s = b / d.chunks;
+    const int pl = chunk * kBlock + threadIdx.x;
+    if (pl >= d.n2) return;
+    const int n = d.n, n2 = d.n2;
+    const int j = pl / n, k = pl - j * n;
+    const PlaneNb nb = plane_nb(pl, n, n2);
+    const int i0 = s * d.slab, i1 = min(n, i0 + d.slab);
+    for (int i = i0; i < i1; i++) {
+        if (((i + j + k) & 1) != color) continue;
+        const int base = i * n2, c = base + pl;
+        const int im = (i == 0 ? n - 1 : i - 1) * n2 + pl;
+        const int ip = (i == n - 1 ? 0 : i + 1) * n2 + pl;
+        const int cy = base + nb.jm, cz = base + nb.km;
+        const double ex_p = pb.ex[c], ex_m = pb.ex[im], ey_p = pb.ey[c], ey_m = pb.ey[cy];
+        const double ez_p = pb.ez[c], ez_m = pb.ez[cz];
+        double diag = ex_p + ex_m + ey_p + ey_m + ez_p + ez_m + pb.k2[c];
+        if (diag < 1e-14) diag = 1e-14;
+        const double rhs = ex_p * out[ip] + ex_m * out[im] + ey_p * out[base + nb.jp] +
+                           ey_m * out[cy] + ez_p * out[base + nb.kp] + ez_m * out[cz] - in[c];
+        out[c] = rhs / diag;
+    }
+}
+
+void launch_rbgs_pb(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* in,
+                    double* out, int iters) {
+    for (int it = 0; it < iters; it++) {
+        MAZE_LAUNCH(k_rbgs_pb, dim3(d.blocks), dim3(kBlock), 0, s, d, pb, in, out, 0);
+        MAZE_LAUNCH(k_rbgs_pb, dim3(d.blocks), dim3(kBlock), 0, s, d, pb, in, out, 1);
+    }
+}
+
+// Face coefficient of the coarse cell (I,J,K): 2x2 average over the plane
+// normal to `axis` (x: fine plane 2I; y, z: fine planes 2I and 2I+1).
+// library/src/multigrid_pb.c:375-500 (sum order per axis).
+__global__ void k_restrict_eps(int n, int axis, const double* __restrict__ in,
+                               double* __restrict__ out) {
+    const int M = n / 2;
+    const long total = (long)M * M * M;
+    const long N = n, N2 = N * N;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int K = (int)(t % M), J = (int)((t / M) % M), I = (int)(t / ((long)M * M));
+        const long i0 = 2L * I * N2, i1 = i0 + N2;
+        const long j0 = 2L * J * N, j1 = (long)((2 * J + 1) % n) * N;
+        const long k0 = 2 * K, k1 = (2 * K + 1) % n;
+        double sm;
+        if (axis == 0)
+            sm = in[i0 + j0 + k0] + in[i0 + j0 + k1] + in[i0 + j1 + k0] + in[i0 + j1 + k1];
+        else if (axis == 1)
+            sm = in[i0 + j0 + k0] + in[i0 + j0 + k1] + in[i1 + j0 + k0] + in[i1 + j0 + k1];
+        else
+            sm = in[i0 + j0 + k0] + in[i1 + j0 + k0] + in[i0 + j1 + k0] + in[i1 + j1 + k0];
+        out[t] = sm * 0.25;
+    }
+}
+
+// Screening term of the coarse cell (I,J,K): 8-point average of the fine
+// cells (2I+1, 2I+2) x (2J, 2J+1) x (2K, 2K+1) (the serial i offset is 1).
+// library/src/multigrid_pb.c:503-568.
+__global__ void k_restrict_k2(int n, const double* __restrict__ in, double* __restrict__ out) {
+    const int M = n / 2;
+    const long total = (long)M * M * M;
+    const long N = n, N2 = N * N;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int K = (int)(t % M), J = (int)((t / M) % M), I = (int)(t / ((long)M * M));
+        const long i0 = (2L * I + 1) * N2, i1 = (long)((2 * I + 2) % n) * N2;
+        const long j0 = 2L * J * N, j1 = (long)((2 * J + 1) % n) * N;
+        const long k0 = 2 * K, k1 = (2 * K + 1) % n;
+        const double sum = in[i0 + j0 + k0] + in[i0 + j0 + k1] + in[i0 + j1 + k0] +
+                           in[i0 + j1 + k1] + in[i1 + j0 + k0] + in[i1 + j0 + k1] +
+                           in[i1 + j1 + k0] + in[i1 + j1 + k1];
+        out[t] = (1.0 / 8.0) * sum;
+    }
+}
+
+void launch_restrict_pb_coef(hipStream_t s, int n, const PBCoef& fine, double* ex, double* ey,
+                             double* ez, double* k2) {
+    const long M = n / 2;
+    const int nb = pw_blocks(M * M * M);
+    MAZE_LAUNCH(k_restrict_eps, dim3(nb), dim3(kBlock), 0, s, n, 0, fine.ex, ex);
+    MAZE_LAUNCH(k_restrict_eps, dim3(nb), dim3(kBlock), 0, s, n, 1, fine.ey, ey);
+    MAZE_LAUNCH(k_restrict_eps, dim3(nb), dim3(kBlock), 0, s, n, 2, fine.ez, ez);
+    MAZE_LAUNCH(k_restrict_k2, dim3(nb), dim3(kBlock), 0, s, n, fine.k2, k2);
+}
+
 // r = in - A out, as (A out)(-1) + 1 in (library/src/multigrid.c:142-144).
 struct BodyMgRes {
     static constexpr bool kReduce = false, kMax = false, kSkipWhenDone = false;
@@ -521,10 +623,10 @@ struct BodyMgRes {
     __device__ static void finalize(CGState*, double, double) {}
 };
 
-void launch_mg_residual(hipStream_t s, const GridDims& d, const double* in, const double* out,
-                        double* r) {
+void launch_mg_residual(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* in,
+                        const double* out, double* r) {
     BodyMgRes body{in, r};
-    run_pass<false>(s, d, PBCoef{}, SrcArr<double>{out}, body, Reduce{}, nullptr);
+    DISPATCH_PB(pb, run_pass<kPB>(s, d, pb, SrcArr<double>{out}, body, Reduce{}, nullptr));
 }
 
 // 27-point full weighting, coarse (I,J,K) <- fine (2I,2J,2K).

@@ -85,6 +85,7 @@ void Sim::free_grid() {
                        &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1})
         dfree(*a);
     for (float** a : {&d32, &r32, &q32a, &q32b}) dfree(*a);
+    for (double** a : {&cex, &cey, &cez, &ck2}) dfree(*a);
     free_mg();
     sched.clear();
     dfree(y_hist);
@@ -153,6 +154,7 @@ void Sim::init_pb(double w_, double kbar2_) {
     launch_fill(stream, G, 1.0, ey);
     launch_fill(stream, G, 1.0, ez);
     launch_fill(stream, G, 0.0, k2);
+    restrict_pb_coef();
     sync();
 }
 
@@ -162,6 +164,8 @@ void Sim::set_eps_k2(const double* ex_, const double* ey_, const double* ez_, co
     MAZE_HIP(hipMemcpy(ey, ey_, G * sizeof(double), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(ez, ez_, G * sizeof(double), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(k2, k2_, G * sizeof(double), hipMemcpyHostToDevice));
+    restrict_pb_coef();
+    sync();
 }
 
 // library/src/solver_md.c:51-65, particles.c:68-109, :181-228.
@@ -307,20 +311,23 @@ int Sim::cg_solve(const GridDims& d, const PBCoef& pbc_, const double* b, double
     return st_host->res;
 }
 
-// Coarse CG with a relative stop, returning the iteration count
-// (library/src/multigrid.c:13-96); x must hold the initial guess.
-int Sim::cg_count_solve(const GridDims& d, const double* b, double* x, double rtol, int maxit,
-                        double* rbuf, double* pa, double* pbuf, int* pred) {
-    launch_cg_reset(stream, st, 0.0, rtol, maxit, 1);
-    launch_cg_init<double>(stream, d, PBCoef{}, x, b, rbuf, st, red);
+// Coarse CG with a relative stop, returning the iteration count; x must hold
+// the initial guess.  Poisson: implicit-Jacobi recurrence (library/src/
+// multigrid.c:13-96); PB (coarse coefficients cpb): plain CG (library/src/
+// multigrid_pb.c:16-98).
+int Sim::cg_count_solve(const GridDims& d, const PBCoef& cpb, const double* b, double* x,
+                        double rtol, int maxit, double* rbuf, double* pa, double* pbuf, int* pred) {
+    const int plain = cpb.ex ? 1 : 0;
+    launch_cg_reset(stream, st, 0.0, rtol, maxit, 1, 0, 0, plain);
+    launch_cg_init<double>(stream, d, cpb, x, b, rbuf, st, red, nullptr, plain);
     int launched = 0;
     int chunk = std::max(4, *pred + 1);
     double* pin = pa;
     double* pout = pbuf;
     for (;;) {
         for (int c = 0; c < chunk && launched < maxit; c++, launched++) {
-            launch_cg_dir<double>(stream, d, PBCoef{}, pin, rbuf, pout, st, red, launched == 0);
-            launch_cg_update<double>(stream, d, PBCoef{}, pout, x, rbuf, st, red);
+            launch_cg_dir<double>(stream, d, cpb, pin, rbuf, pout, st, red, launched == 0);
+            launch_cg_update<double>(stream, d, cpb, pout, x, rbuf, st, red);
             std::swap(pin, pout);
         }
         read_state();
@@ -601,24 +608,44 @@ int Sim::lcg_update() {
     return cg_solve(dims, pbc(), y, phi, tol, n * n, r, p0, p1, &last_iters);
 }
 
-// Reference two-grid V-cycle, library/src/multigrid.c:99-168.
+// Reference two-grid V-cycle, library/src/multigrid.c:99-168 (3 RBGS sweeps),
+// or its Poisson-Boltzmann form, library/src/multigrid_pb.c:275-355 (4 RBGS-PB
+// sweeps, coarse coefficients restricted once per coefficient update).
 void Sim::mg_vcycle(const double* in, double* out) {
-    launch_rbgs(stream, dims, in, out, 3);
-    launch_mg_residual(stream, dims, in, out, mg_r);
+    if (pb)
+        launch_rbgs_pb(stream, dims, pbc(), in, out, 4);
+    else
+        launch_rbgs(stream, dims, in, out, 3);
+    launch_mg_residual(stream, dims, pbc(), in, out, mg_r);
     launch_restrict27(stream, n, mg_r, c_rhs);
     launch_fill(stream, (long)nc * nc * nc, 0.0, c_eps);
-    cg_count_solve(cdims, c_rhs, c_eps, 1e-5, 50, c_r, c_p0, c_p1, &last_coarse);
+    const PBCoef cpb = pb ? PBCoef{cex, cey, cez, ck2} : PBCoef{};
+    cg_count_solve(cdims, cpb, c_rhs, c_eps, 1e-5, 50, c_r, c_p0, c_p1, &last_coarse);
     launch_prolong_add(stream, nc, c_eps, out);
-    launch_rbgs(stream, dims, in, out, 3);
+    if (pb)
+        launch_rbgs_pb(stream, dims, pbc(), in, out, 4);
+    else
+        launch_rbgs(stream, dims, in, out, 3);
+}
+
+// Coarse PB coefficients of the reference two-grid cycle.
+void Sim::restrict_pb_coef() {
+    if (!pb || !(grid_type == kMG || grid_type == kMazeMG)) return;
+    const long Gc = (long)nc * nc * nc;
+    if (!cex) {
+        cex = dalloc<double>(Gc); cey = dalloc<double>(Gc);
+        cez = dalloc<double>(Gc); ck2 = dalloc<double>(Gc);
+    }
+    launch_restrict_pb_coef(stream, n, pbc(), cex, cey, cez, ck2);
 }
 
 // library/src/grid_multigrid.c:110-212 (maze == false) and
-// library/src/verlet.c:86-149 (maze == true).
+// library/src/verlet.c:86-149 (maze == true); with PB, grid_multigrid.c:149-175
+// and verlet.c:211-271.
 int Sim::mg_update(bool maze) {
-    if (pb) fatal("maze: the Poisson-Boltzmann multigrid is not provided; use MAZE-LCG");
     double* x;
     if (maze) {
-        const double coef = (4 * kPi) / (h * eps_s);
+        const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
         launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma);
         double* old_phip = phip;
         phip = phi;
@@ -632,14 +659,14 @@ int Sim::mg_update(bool maze) {
         phi = tmp;
         tmp = old_phip;
         double c = -4 * kPi / h;
-        c /= eps_s;
+        if (!pb) c /= eps_s;
         launch_scale_copy(stream, G, c, q, sigma);
         x = phi;
     }
     int res = -1, k = 0;
     while (k < 1000) {
         mg_vcycle(sigma, x);
-        launch_true_residual_max(stream, dims, PBCoef{}, x, sigma, st, red);
+        launch_true_residual_max(stream, dims, pbc(), x, sigma, st, red);
         read_state();
         k++;
         if (st_host->resid <= tol) {

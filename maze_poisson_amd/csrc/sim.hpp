@@ -56,6 +56,8 @@ struct Sim {
     double *q = nullptr, *y = nullptr, *phi = nullptr, *phip = nullptr, *tmp = nullptr;
     double *sigma = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *y_alt = nullptr;
     double* y_hist = nullptr;  // y of the step before last (extrapolated start)
+    // Coarse-grid PB coefficients of the reference two-grid cycle.
+    double *cex = nullptr, *cey = nullptr, *cez = nullptr, *ck2 = nullptr;
     int y_extrap = 0;          // start the mixed solve from 2 y_n-1 - y_n-2
     int hist_ok = 0;           // y_hist holds the previous step's y
     double *ex = nullptr, *ey = nullptr, *ez = nullptr, *k2 = nullptr;
@@ -166,8 +168,9 @@ struct Sim {
     // solvers
     int cg_solve(const GridDims& d, const PBCoef& pbc, const double* b, double* x, double tol,
                  int limit, double* rbuf, double* pa, double* pbuf, int* pred);
-    int cg_count_solve(const GridDims& d, const double* b, double* x, double rtol, int maxit,
-                       double* rbuf, double* pa, double* pbuf, int* pred);
+    int cg_count_solve(const GridDims& d, const PBCoef& cpb, const double* b, double* x,
+                       double rtol, int maxit, double* rbuf, double* pa, double* pbuf, int* pred);
+    void restrict_pb_coef();
     void inner_iter(bool use_mg, float* x, int it);
     void inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated);
     int inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit, int first_chunk);

@@ -37,21 +37,23 @@ struct SrcPred {  // phi* = 2 phi - phi_prev   (library/src/verlet.c:223-225)
     __device__ __forceinline__ double val(const Raw& r) const { return 2 * r.a - r.b; }
 };
 
-// p = beta p + r/6, or r/6 for the first direction (library/src/plcg.c:52-53, :79-80),
-// rounded to the storage type so that every thread sees the stored value.
+// p = beta p + cr r, or cr r for the first direction, with cr = 1/6 for the
+// implicit-Jacobi recurrence (library/src/plcg.c:52-53, :79-80) or -1 for plain
+// CG (library/src/multigrid_pb.c:43-44, :89-90); rounded to the storage type so
+// that every thread sees the stored value.
 template <class Real, bool First>
 struct SrcDir {
     const Real* p;
     const Real* r;
     double beta;
+    double cr;
     struct Raw { Real p, r; };
     __device__ __forceinline__ Raw load(int i) const {
         if (First) return Raw{(Real)0, r[i]};
         return Raw{p[i], r[i]};
     }
     __device__ __forceinline__ double val(const Raw& x) const {
-        const double c6 = 1.0 / 6.0;
-        const double v = First ? (double)x.r * c6 : (double)x.p * beta + c6 * (double)x.r;
+        const double v = First ? (double)x.r * cr : (double)x.p * beta + cr * (double)x.r;
         return (double)(Real)v;
     }
 };

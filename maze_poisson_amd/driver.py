@@ -13,7 +13,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from .capi import MazeLib
-from .workloads import System, dt_au
+from .workloads import System, dt_au, kbar2, pb_eps_field
 
 
 @dataclass
@@ -28,6 +28,12 @@ class RunConfig:
     gamma: float = 1e-3
     integrator: str = "VERLET"
     thermostat: bool = False
+    # Linearized Poisson-Boltzmann: synthetic face dielectric in [2, 80] (seeded) and a
+    # uniform screening term for `ionic_molar` at eps_s (= 80 in water).
+    pb: bool = False
+    ionic_molar: float = 0.1
+    pb_eps_s: float = 80.0
+    eps_seed: int = 0
 
 
 def initialize(lib: MazeLib, s: System, cfg: RunConfig) -> None:
@@ -37,6 +43,11 @@ def initialize(lib: MazeLib, s: System, cfg: RunConfig) -> None:
     lib.solver_initialize_grid(s.n_grid, s.L, s.h, cfg.tol, cfg.eps_s, cfg.eps_int,
                                maps["grid"][cfg.method.upper()],
                                maps["precond"][cfg.precond.upper()])
+    if cfg.pb:
+        kb2 = kbar2(cfg.ionic_molar, cfg.pb_eps_s, cfg.T, s.h)
+        lib.solver_initialize_grid_pois_boltz(0.0, kb2, 0)
+        ex, ey, ez = pb_eps_field(s.n_grid, s.L, cfg.eps_seed)
+        lib.solver_set_eps_k2(ex, ey, ez, np.full((s.n_grid,) * 3, kb2))
     lib.solver_initialize_particles(s.n_grid, s.n_typ, s.L, s.h, s.n_p,
                                     maps["potential"]["TF"], maps["cas"]["CIC"],
                                     s.types, s.pos, s.vel, s.mass, s.charges, s.tf)

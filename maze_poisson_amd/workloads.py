@@ -139,3 +139,50 @@ def poisson_tol(sys_: System, rtol: float = 1e-8, eps_s: float = 1.0) -> float:
     largest single-ion grid charge (|q| <= 1 on one node)."""
     qmax = float(np.max(np.abs(sys_.charges))) if sys_.n_p else 1.0
     return rtol * 4.0 * math.pi * qmax / (sys_.h * eps_s)
+
+
+# ---- linearized Poisson-Boltzmann inputs -----------------------------------------
+# SI constants of the screening term, maze_poisson/constants.py:14-18.
+EPS0_SI = 8.854187817e-12      # F/m
+KB_SI = 1.380649e-23           # J/K
+NA_SI = 6.02214076e23          # 1/mol
+EC_SI = 1.602176634e-19        # C
+BR_SI = 5.29177210903e-11      # m
+
+
+def kbar2(ionic_molar: float, eps_s: float, T: float, h_bohr: float) -> float:
+    """Grid-scaled screening constant kbar^2 = kappa^2 h^2 for an ionic strength in
+    mol/L, ``maze_poisson/solver.py:149-153``."""
+    return ((8 * np.pi * NA_SI * EC_SI ** 2 * ionic_molar * 1e3)
+            / (eps_s * EPS0_SI * KB_SI * T) * BR_SI ** 2 * h_bohr ** 2)
+
+
+def pb_eps_field(n: int, L: float, seed: int, lo: float = 2.0, hi: float = 80.0,
+                 n_gauss: int = 8) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Smooth synthetic dielectric on the faces of an n^3 periodic grid of side L:
+    eps = lo + (hi - lo) S / max S, with S a sum of `n_gauss` Gaussians (centres
+    uniform in the box, widths uniform in [0.05, 0.2] L, minimum-image distance),
+    evaluated at the +x, +y and +z face centres (eps_x[i,j,k] sits at
+    ((i + 1/2) h, j h, k h), and so on)."""
+    g = np.random.default_rng(seed)
+    centres = g.uniform(0.0, L, (n_gauss, 3))
+    widths = g.uniform(0.05, 0.2, n_gauss) * L
+    h = L / n
+    idx = np.arange(n) * h
+    S = []
+    for axis in range(3):
+        coords = [idx, idx, idx]
+        coords[axis] = idx + 0.5 * h
+        acc = np.zeros((n, n, n))
+        for c, w in zip(centres, widths):
+            d2 = 0.0
+            for a in range(3):
+                d = coords[a] - c[a]
+                d -= L * np.round(d / L)
+                shape = [1, 1, 1]
+                shape[a] = n
+                d2 = d2 + (d * d).reshape(shape)
+            acc += np.exp(-0.5 * d2 / (w * w))
+        S.append(acc)
+    smax = max(s.max() for s in S)
+    return tuple(np.ascontiguousarray(lo + (hi - lo) * s / smax) for s in S)

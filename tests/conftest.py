@@ -50,6 +50,13 @@ def oracle_raw(oracle):
         "or_mg_prolong_trilinear": (None, [D, D, ci]),
         "or_mg_cg_coarse": (ci, [D, D, ci, ci, cd]),
         "or_mg_vcycle": (None, [D, D, ci, ci]),
+        "or_mg_smooth_pb_rbgs": (None, [D, D, ci, ci, D, D, D, D]),
+        "or_mg_restrict_eps": (None, [D, D, ci, ci]),
+        "or_mg_restrict_k2": (None, [D, D, ci]),
+        "or_mg_cg_coarse_pb": (ci, [D, D, ci, ci, cd, D, D, D, D]),
+        "or_mg_vcycle_pb": (None, [D, D, ci, ci, D, D, D, D]),
+        "or_verlet_pb_multigrid": (ci, [cd, cd, D, D, D, D, ci, D, D, D, D]),
+        "or_mg_pb_update": (ci, [cd, D, D, ci, D, D, D, D]),
         "or_cic_weight": (cd, [cd, cd, cd]),
         "or_cic_neighbors": (None, [D, ci, ci, cd, L]),
         "or_spread_cic": (cd, [ci, ci, cd, D, L, D, D]),
@@ -100,6 +107,9 @@ def hip_raw(hip):
         "maze_test_restrict27": (ci, [D, D, ci]),
         "maze_test_prolong_add": (ci, [D, D, ci]),
         "maze_test_mg_vcycle": (ci, [D, D, ci]),
+        "maze_test_rbgs_pb": (None, [D, D, ci, ci, D, D, D, D]),
+        "maze_test_restrict_pb_coef": (ci, [D, D, D, D, D, D, D, D, ci]),
+        "maze_test_mg_vcycle_pb": (ci, [D, D, ci, D, D, D, D]),
         "maze_test_mg_pcg": (ci, [D, D, cd, ci]),
         "maze_test_mg_down": (ci, [F, F, F, ci, cd]),
         "maze_test_mg_up": (ci, [F, F, F, F, ci, cd]),

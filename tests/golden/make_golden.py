@@ -95,6 +95,74 @@ def kernel_vectors(lib):
     return out
 
 
+def pb_coefficients(n, seed):
+    """Face dielectric maps in [2, 80] and a screening term in [0, 0.1], seeded."""
+    g = np.random.default_rng(seed)
+    return [g.uniform(2, 80, (n, n, n)) for _ in range(3)], g.uniform(0, 0.1, (n, n, n))
+
+
+def pb_mg_vectors(lib):
+    """Reference Poisson-Boltzmann multigrid pieces (library/src/multigrid_pb.c) and the
+    MaZe PB multigrid step (library/src/verlet.c:211-271).  Inputs are regenerated from
+    the seeds by the tests; only outputs are stored."""
+    L = lib.lib
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    out = {}
+    L.init_mpi()
+    # RBGS-PB, 2 iterations, and the coefficient restrictions on 16^3 (seed 201)
+    n = 16
+    eps, k2 = pb_coefficients(n, 201)
+    rhs = np.random.default_rng(202).standard_normal((n, n, n))
+    x = np.random.default_rng(203).standard_normal((n, n, n))
+    eb = [ghosted(e) for e in eps]
+    xb, xp = ghosted(x)
+    L.smooth_pb_rbgs.argtypes = [vp, vp, ci, ci, cd, vp, vp, vp, vp]
+    L.smooth_pb_rbgs(rhs.ctypes.data_as(vp), xp, n, n, 2.0, eb[0][1], eb[1][1], eb[2][1],
+                     k2.ctypes.data_as(vp))
+    out["rbgspb_out"] = xb[1:-1].copy()
+    L.restriction_eps.argtypes = [vp, vp, ci, ci, ci]
+    for axis in range(3):
+        c = np.zeros((n // 2,) * 3)
+        L.restriction_eps(eb[axis][1], c.ctypes.data_as(vp), n, n, axis)
+        out[f"reps{axis}_out"] = c
+    c = np.zeros((n // 2,) * 3)
+    L.restriction_k2screen.argtypes = [vp, vp, ci, ci, ci]
+    L.restriction_k2screen(k2.ctypes.data_as(vp), c.ctypes.data_as(vp), n, n, 0)
+    out["rk2_out"] = c
+    # conj_grad_pb on 16^3 (library/src/plcg.c:169-238)
+    b = np.random.default_rng(204).standard_normal((n, n, n))
+    xb, xp = ghosted(np.zeros((n, n, n)))
+    L.conj_grad_pb.argtypes = [vp, vp, vp, cd, ci, ci, vp, vp, vp, vp]
+    L.conj_grad_pb.restype = ci
+    it = L.conj_grad_pb(b.ctypes.data_as(vp), xp, xp, 1e-9, n, n, eb[0][1], eb[1][1], eb[2][1],
+                        k2.ctypes.data_as(vp))
+    out["cgpb_iter"], out["cgpb_x"] = np.array(it), xb[1:-1].copy()
+    # One PB V-cycle on 32^3 from zero (seed 205, 4 sweeps as multigrid_pb_apply uses)
+    n = 32
+    eps, k2 = pb_coefficients(n, 205)
+    rhs = np.random.default_rng(206).standard_normal((n, n, n))
+    eb = [ghosted(e) for e in eps]
+    xb, xp = ghosted(np.zeros((n, n, n)))
+    L.v_cycle_pb.argtypes = [vp, vp, ci, ci, ci, ci, ci, vp, vp, vp, vp]
+    L.v_cycle_pb(rhs.ctypes.data_as(vp), xp, n, n, 0, 4, 0, eb[0][1], eb[1][1], eb[2][1],
+                 k2.ctypes.data_as(vp))
+    out["vcpb_out"] = xb[1:-1].copy()
+    # MaZe PB multigrid step on 32^3 (seed 207): phi, phi_prev, q, y warm start
+    g = np.random.default_rng(207)
+    phi, phip = g.standard_normal((n, n, n)), g.standard_normal((n, n, n))
+    q = g.standard_normal((n, n, n)) * 1e-3
+    y = g.standard_normal((n, n, n)) * 1e-2
+    pb_, ppb_, yb_ = ghosted(phi), ghosted(phip), ghosted(y)
+    L.verlet_pb_multigrid.argtypes = [cd, cd, vp, vp, vp, vp, ci, ci, vp, vp, vp, vp]
+    L.verlet_pb_multigrid.restype = ci
+    it = L.verlet_pb_multigrid(1e-6, 0.5, pb_[1], ppb_[1], q.ctypes.data_as(vp), yb_[1], n, n,
+                               eb[0][1], eb[1][1], eb[2][1], k2.ctypes.data_as(vp))
+    out["vpbmg_iter"] = np.array(it)
+    # (phi_prev becomes the input phi, so only phi and y are stored.)
+    out["vpbmg_phi"], out["vpbmg_y"] = pb_[0][1:-1].copy(), yb_[0][1:-1].copy()
+    return out
+
+
 def md_vectors(lib, n_ions, n_grid, seed, method, steps, **kw):
     s = W.nacl_system(n_ions, n_grid, seed=seed)
     cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s), **kw)
@@ -156,6 +224,7 @@ def main():
         sys.exit("run with OMP_NUM_THREADS=1 (the reference's scatter races under OpenMP)")
     lib = MazeLib(REF, extensions=False)
     np.savez_compressed(os.path.join(HERE, "ref_kernels.npz"), **kernel_vectors(lib))
+    np.savez_compressed(os.path.join(HERE, "ref_pb_mg.npz"), **pb_mg_vectors(lib))
     cases = {
         "maze_lcg_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=5),
         "maze_lcg_jacobi_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=3,

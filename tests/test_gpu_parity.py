@@ -77,6 +77,43 @@ def test_rbgs_restrict_prolong_bitexact(hip_raw, oracle_raw):
     assert np.array_equal(f1, x0 + f2)
 
 
+def pb_coef(n, seed):
+    g = np.random.default_rng(seed)
+    eps = [np.ascontiguousarray(g.uniform(2, 80, (n, n, n))) for _ in range(3)]
+    return eps, np.ascontiguousarray(g.uniform(0, 0.1, (n, n, n)))
+
+
+@pytest.mark.parametrize("n", [16, 32, 40])
+def test_rbgs_pb_and_coefficient_restriction_bitexact(hip_raw, oracle_raw, n):
+    eps, k2 = pb_coef(n, 300 + n)
+    rhs, x0 = rand_grid(n, 301), rand_grid(n, 302)
+    a, b = x0.copy(), x0.copy()
+    hip_raw.maze_test_rbgs_pb(rhs, a, n, 3, *eps, k2)
+    oracle_raw.or_mg_smooth_pb_rbgs(rhs, b, n, 3, *eps, k2)
+    assert np.array_equal(a, b)
+    m = n // 2
+    g = [np.zeros((m, m, m)) for _ in range(4)]
+    assert hip_raw.maze_test_restrict_pb_coef(*eps, k2, *g, n) == 0
+    for axis in range(3):
+        o = np.zeros((m, m, m))
+        oracle_raw.or_mg_restrict_eps(eps[axis], o, n, axis)
+        assert np.array_equal(g[axis], o), axis
+    o = np.zeros((m, m, m))
+    oracle_raw.or_mg_restrict_k2(k2, o, n)
+    assert np.array_equal(g[3], o)
+
+
+def test_vcycle_pb_matches_oracle(hip_raw, oracle_raw):
+    n = 32
+    eps, k2 = pb_coef(n, 310)
+    rhs = rand_grid(n, 311)
+    a, b = np.zeros_like(rhs), np.zeros_like(rhs)
+    assert hip_raw.maze_test_mg_vcycle_pb(rhs, a, n, *eps, k2) == 0
+    oracle_raw.or_mg_vcycle_pb(rhs, b, n, 4, *eps, k2)
+    # Same cycle; the coarse CG's global sums are ordered differently.
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+
+
 def test_vcycle_matches_oracle(hip_raw, oracle_raw):
     n = 32
     rhs = zero_mean(rand_grid(n, 21))
@@ -138,6 +175,26 @@ def test_md_sequence_mg(hip, oracle, method):
     assert all(abs(a - b) <= 1 for a, b in zip(its_g, its_o)), (its_g, its_o)
     for k in ("fcs_elec", "fcs_tot", "field"):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+
+
+@pytest.mark.parametrize("method", ["MAZE-LCG", "LCG", "MULTIGRID", "MAZE-MULTIGRID"])
+def test_md_sequence_pb(hip, oracle, method):
+    """Linearized Poisson-Boltzmann (synthetic dielectric in [2, 80], 0.1 M screening)
+    through every grid type: the GPU against the oracle on the same inputs."""
+    s = W.nacl_system(216, 32, seed=3)
+    cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s), pb=True, eps_seed=4)
+    its_g, v_g, g = run(hip, s, cfg, 2)
+    its_o, v_o, o = run(oracle, s, cfg, 2)
+    # The implicit-Jacobi CG on the variable-coefficient operator (p = r/6 with a
+    # diagonal that is not 6, library/src/plcg.c:197-204) converges irregularly, so
+    # the order of its global sums moves its stop by a few of ~100 iterations; the
+    # multigrid cycle counts agree to one.
+    slack = [max(1, int(0.06 * b)) for b in its_o] if "LCG" in method else [1] * len(its_o)
+    assert all(abs(a - b) <= t for a, b, t in zip(its_g, its_o, slack)), (its_g, its_o)
+    for k in ("fcs_elec", "fcs_tot", "field", "field_prev"):
+        assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+    assert rel(g["pos"], o["pos"]) <= 1e-11
+    assert rel(v_g, v_o) <= 1e-12
 
 
 def test_cic_bitexact_at_64(hip, oracle):

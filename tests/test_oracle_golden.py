@@ -88,3 +88,67 @@ def test_multigrid_pieces_golden(oracle_raw, kv):
     f = np.zeros((n,) * 3)
     oracle_raw.or_mg_prolong_trilinear(np.ascontiguousarray(kv["prol_in"]), f, n // 2)
     assert np.array_equal(f, kv["prol_out"])
+
+
+# ---- reference Poisson-Boltzmann multigrid (library/src/multigrid_pb.c) ----------
+
+def pb_coefficients(n, seed):
+    """The seeded coefficient fields make_golden.py fed the reference."""
+    g = np.random.default_rng(seed)
+    eps = [g.uniform(2, 80, (n, n, n)) for _ in range(3)]
+    return eps, g.uniform(0, 0.1, (n, n, n))
+
+
+@pytest.fixture(scope="module")
+def pbv():
+    return np.load(os.path.join(GOLDEN, "ref_pb_mg.npz"))
+
+
+def test_pb_rbgs_and_restrictions_golden(oracle_raw, pbv):
+    n = 16
+    eps, k2 = pb_coefficients(n, 201)
+    rhs = np.random.default_rng(202).standard_normal((n, n, n))
+    x = np.random.default_rng(203).standard_normal((n, n, n))
+    oracle_raw.or_mg_smooth_pb_rbgs(rhs, x, n, 2, *eps, k2)
+    assert np.array_equal(x, pbv["rbgspb_out"])
+    for axis in range(3):
+        c = np.zeros((n // 2,) * 3)
+        oracle_raw.or_mg_restrict_eps(eps[axis], c, n, axis)
+        assert np.array_equal(c, pbv[f"reps{axis}_out"]), axis
+    c = np.zeros((n // 2,) * 3)
+    oracle_raw.or_mg_restrict_k2(k2, c, n)
+    assert np.array_equal(c, pbv["rk2_out"])
+
+
+def test_conj_grad_pb_golden(oracle_raw, pbv):
+    n = 16
+    eps, k2 = pb_coefficients(n, 201)
+    b = np.random.default_rng(204).standard_normal((n, n, n))
+    x = np.zeros((n, n, n))
+    it = oracle_raw.or_conj_grad_pb(b, x, 1e-9, n, *eps, k2)
+    assert it == int(pbv["cgpb_iter"])
+    assert np.array_equal(x, pbv["cgpb_x"])
+
+
+def test_vcycle_pb_golden(oracle_raw, pbv):
+    n = 32
+    eps, k2 = pb_coefficients(n, 205)
+    rhs = np.random.default_rng(206).standard_normal((n, n, n))
+    x = np.zeros((n, n, n))
+    oracle_raw.or_mg_vcycle_pb(rhs, x, n, 4, *eps, k2)
+    assert np.array_equal(x, pbv["vcpb_out"])
+
+
+def test_verlet_pb_multigrid_golden(oracle_raw, pbv):
+    n = 32
+    eps, k2 = pb_coefficients(n, 205)
+    g = np.random.default_rng(207)
+    phi, phip = g.standard_normal((n, n, n)), g.standard_normal((n, n, n))
+    q = g.standard_normal((n, n, n)) * 1e-3
+    y = g.standard_normal((n, n, n)) * 1e-2
+    phi_in = phi.copy()
+    it = oracle_raw.or_verlet_pb_multigrid(1e-6, 0.5, phi, phip, q, y, n, *eps, k2)
+    assert it == int(pbv["vpbmg_iter"])
+    assert np.array_equal(y, pbv["vpbmg_y"])
+    assert np.array_equal(phi, pbv["vpbmg_phi"])
+    assert np.array_equal(phip, phi_in)
