@@ -108,6 +108,29 @@ def run_path(lib, precision, steps, warmup, lat_steps, prof_steps):
             "prof": prof}
 
 
+def driver_rate(lib, precision, steps, getters):
+    """Steps/s through the reference driver's own per-step call sequence (about 8 C-ABI
+    calls, maze_poisson/solver.py:483-492), optionally copying positions, velocities and
+    total forces to the host after every step (PCIe-inclusive, as a trajectory writer
+    would)."""
+    s = start(lib, precision)
+    buf = np.zeros((s.n_p, 3))
+    for _ in range(5):
+        driver.md_step(lib)
+    lib.solver_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        driver.md_step(lib)
+        if getters:
+            lib.get_pos(buf)
+            lib.get_vel(buf)
+            lib.get_fcs_tot(buf)
+    lib.solver_sync()
+    el = time.perf_counter() - t0
+    lib.solver_finalize()
+    return steps / el
+
+
 def forces_after(lib, precision, tol_factor, nsteps):
     s = start(lib, precision, tol_factor)
     its = [driver.md_step(lib)[0] for _ in range(nsteps)]
@@ -230,6 +253,9 @@ def main():
             errs[name] = rms_rel(f, f_ref)
             iters[name] = float(np.mean(its))
         line_extra = {
+            "python_driver_steps_per_s": round(driver_rate(lib, PATHS["mixed_mg"], 100, False), 2),
+            "python_driver_with_getters_steps_per_s": round(
+                driver_rate(lib, PATHS["mixed_mg"], 100, True), 2),
             "fp64_steps_per_s": round(fp64["steps_per_s"], 3),
             "fp64_p50_ms": round(fp64["p50_ms"], 4), "fp64_p99_ms": round(fp64["p99_ms"], 4),
             "mixed_cg_steps_per_s": round(mcg["steps_per_s"], 3),

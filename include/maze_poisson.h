@@ -181,6 +181,9 @@ void solver_set_inner_rtol(double rtol);
  * 2 y_n-1 - y_n-2 of the correction field (1) instead of y_n-1 (0, the
  * reference's start).  The stop test is unchanged. */
 void solver_set_y_extrapolation(int on);
+/* 1/2 sum_g q_g phi_g (Hartree): the electrostatic energy of the grid charges.
+ * get_energy_elec keeps the reference's 1/2 sum_g phi_g. */
+double get_energy_elec_qphi(void);
 /* Mode 2 = FP32 multigrid-preconditioned CG inside FP64 iterative
  * refinement (V(2,2) damped Jacobi, exact FP64 coarsest solve); falls back
  * to mode 1 when the grid has no multigrid hierarchy (halving reaches an odd

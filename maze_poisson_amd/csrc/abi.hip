@@ -146,6 +146,7 @@ void get_field_prev(double* o) { d2h(o, S().phip, S().G * sizeof(double)); }
 void get_q(double* o) { d2h(o, S().q, S().G * sizeof(double)); }
 double get_kinetic_energy() { return S().kinetic(); }
 double get_energy_elec() { return S().energy_elec(); }
+double get_energy_elec_qphi() { return S().energy_qphi(); }
 void get_momentum(double* o) { S().momentum(o); }
 double get_temperature() { return 2 * S().kinetic() / (3 * S().p.np * 3.1668e-6); }
 

@@ -99,6 +99,8 @@ void launch_fill(hipStream_t s, long n, double v, double* out);
 void launch_fill_f(hipStream_t s, long n, float v, float* out);
 void launch_copy(hipStream_t s, long n, const double* in, double* out);
 void launch_half_sum(hipStream_t s, long n, const double* phi, double* out, Reduce red);
+void launch_dot_scaled(hipStream_t s, long n, double scale, const double* a, const double* b,
+                       double* out, Reduce red);
 void launch_sum_scaled(hipStream_t s, long n, double scale, const double* v, double* out,
                        Reduce red);
 

@@ -482,6 +482,31 @@ void launch_half_sum(hipStream_t s, long n, const double* phi, double* out, Redu
     launch_sum_scaled(s, n, 0.5, phi, out, red);
 }
 
+// out = scale * sum(a[i] * b[i]), single-pass deterministic reduction.
+__global__ void k_dot_scaled(long n, double scale, const double* __restrict__ a,
+                             const double* __restrict__ b, Reduce red, double* out) {
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s += a[i] * b[i];
+    __shared__ double sh[kBlock / kWave];
+    const double S = block_sum(s, sh);
+    if (publish_arrive(red.psum, S, nullptr, 0.0, red.counter, gridDim.x)) {
+        const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            *out = scale * T;
+            reset_counter(red.counter);
+        }
+    }
+}
+
+void launch_dot_scaled(hipStream_t s, long n, double scale, const double* a, const double* b,
+                       double* out, Reduce red) {
+    long nb = (n + 4L * kBlock - 1) / (4L * kBlock);
+    if (nb < 1) nb = 1;
+    if (nb > 2048) nb = 2048;
+    MAZE_LAUNCH(k_dot_scaled, dim3((int)nb), dim3(kBlock), 0, s, n, scale, a, b, red, out);
+}
+
 // ---- reference two-grid multigrid (library/src/multigrid.c) -----------------
 
 // One colour of red-black Gauss-Seidel in place: points with (i+j+k)%2 ==

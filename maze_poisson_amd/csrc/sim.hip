@@ -806,6 +806,16 @@ int Sim::check_thermostat() {
     return 0;
 }
 
+// 1/2 sum_g q_g phi_g: the electrostatic energy of the grid charges in the
+// field (Hartree; phi solves A phi = -4 pi q / (h eps), i.e. the potential in
+// atomic units).
+double Sim::energy_qphi() {
+    launch_dot_scaled(stream, G, 0.5, q, phi, dscalar, red);
+    MAZE_HIP(hipMemcpyAsync(hscalar, dscalar, sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    return hscalar[0];
+}
+
 double Sim::energy_elec() {
     launch_half_sum(stream, G, phi, dscalar, red);
     MAZE_HIP(hipMemcpyAsync(hscalar, dscalar, sizeof(double), hipMemcpyDeviceToHost, stream));

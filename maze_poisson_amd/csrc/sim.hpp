@@ -163,6 +163,7 @@ struct Sim {
     int check_thermostat();
     double kinetic();
     double energy_elec();
+    double energy_qphi();
     void momentum(double* out);
 
     // solvers

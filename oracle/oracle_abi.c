@@ -435,6 +435,13 @@ double get_energy_elec(void) {
     return e;
 }
 
+/* Extension: 1/2 sum_g q_g phi_g, the electrostatic energy of the grid charges. */
+double get_energy_elec_qphi(void) {
+    double e = 0.0;
+    for (long i = 0; i < g.G; i++) e += g.q[i] * g.phi_n[i];
+    return 0.5 * e;
+}
+
 /* library/src/particles.c:822-839 */
 void get_momentum(double *o) {
     double px = 0.0, py = 0.0, pz = 0.0;

@@ -364,3 +364,33 @@ def test_determinism(hip):
     b = run(hip, s, cfg, 3)[2]
     for k in ("pos", "vel", "fcs_tot", "field", "q"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_energy_drift_no_worse_than_oracle(hip, oracle):
+    """NVE energy conservation (reference work bookkeeping) of the mixed-precision GPU
+    path against the FP64 oracle on the same 216-ion / 32^3 inputs over 300 steps."""
+    from maze_poisson_amd import energy
+    s = W.nacl_system(216, 32, seed=3)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    g = energy.summarize(energy.run_energy(hip, s, cfg, 300, precision=2), cfg.dt_fs)["work"]
+    o = energy.summarize(energy.run_energy(oracle, s, cfg, 300, qphi=False), cfg.dt_fs)["work"]
+    assert g["std_over_mean"] <= 2 * o["std_over_mean"] + 1e-9, (g, o)
+    assert abs(g["slope_per_ps_over_mean"]) <= 2 * abs(o["slope_per_ps_over_mean"]) + 1e-8, (g, o)
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_field_matches_fft_poisson_at_128(hip, precision):
+    """The 4096-ion / 128^3 field after 3 MD steps against an FP64 FFT solve of the same
+    discrete equation A phi = -4 pi q / h (eps = 1), constant mode removed."""
+    s = W.nacl_system(4096, 128, seed=2)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    _, _, g = run(hip, s, cfg, 3, precision=precision)
+    n, q = 128, g["q"]
+    lam1 = 2 * np.cos(2 * np.pi * np.arange(n) / n) - 2
+    lam = lam1[:, None, None] + lam1[None, :, None] + lam1[None, None, :]
+    lam[0, 0, 0] = 1.0
+    ph = np.fft.fftn(-4 * np.pi * q / s.h) / lam
+    ph[0, 0, 0] = 0.0
+    phi = np.fft.ifftn(ph).real
+    d = (g["field"] - g["field"].mean()) - phi
+    assert np.abs(d).max() <= 1e-6 * np.abs(phi).max(), np.abs(d).max() / np.abs(phi).max()
