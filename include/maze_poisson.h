@@ -190,6 +190,9 @@ double get_energy_elec_qphi(void);
  * size above 12) or the PB operator is enabled.  Jacobi damping (default
  * 6/7) and the number of levels: */
 void solver_set_mg_omega(double omega);
+/* Different weights for the two smoothing sweeps of each level (pre-smoothing
+ * omega1 then omega2, post-smoothing omega2 then omega1). */
+void solver_set_mg_omegas(double omega1, double omega2);
 int solver_mg_levels(void);
 
 /* 1 (default): a charge-conservation failure exits like the reference;
@@ -238,12 +241,14 @@ int maze_test_restrict_pb_coef(const double *eps_x, const double *eps_y, const d
 int maze_test_mg_vcycle_pb(const double *in, double *out, int n, const double *eps_x,
                            const double *eps_y, const double *eps_z, const double *k2);
 int maze_test_mg_pcg(const double *b, double *x, double rtol, int n);
-/* The fused FP32 multigrid level kernels (n even, c = damping / 6):
+/* The fused FP32 multigrid level kernels (n even, c = damping / 6, tile = the
+ * work-group tile shape 0..4, or -1 for the level's default):
  * down: d2 = two damped-Jacobi sweeps from zero on A d = b, bc = 4 x
  *       full-weighting restriction of b - A d2 ((n/2)^3 values);
  * up:   d = two damped-Jacobi sweeps on A d = b from d2 + trilinear(e). */
-int maze_test_mg_down(const float *b, float *d2, float *bc, int n, double c);
-int maze_test_mg_up(const float *b, const float *d2, const float *e, float *d, int n, double c);
+int maze_test_mg_down(const float *b, float *d2, float *bc, int n, double c, int tile);
+int maze_test_mg_up(const float *b, const float *d2, const float *e, float *d, int n, double c,
+                    int tile);
 /* The one-work-group coarse tail of the V-cycle set up for an n^3 grid, on the
  * first level it covers (its size is returned; -1 if the grid has no tail). */
 int maze_test_mg_tail(const float *b, float *d, int n);

@@ -100,6 +100,7 @@ EXTENSION_ABI = [
     ("solver_get_precision", _c_int, []),
     ("solver_set_inner_rtol", None, [_c_double]),
     ("solver_set_y_extrapolation", None, [_c_int]),
+    ("solver_set_mg_omegas", None, [_c_double, _c_double]),
     ("get_energy_elec_qphi", _c_double, []),
     ("solver_set_mg_omega", None, [_c_double]),
     ("solver_mg_levels", _c_int, []),

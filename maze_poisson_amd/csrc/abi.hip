@@ -192,7 +192,14 @@ void solver_set_y_extrapolation(int on) {
 }
 
 // Damping of the multigrid Jacobi smoother (default 6/7).
-void solver_set_mg_omega(double omega) { S().mg_omega = omega; }
+void solver_set_mg_omega(double omega) { S().mg_omega = S().mg_omega2 = omega; }
+
+// Two different sweep weights (a degree-2 polynomial smoother).
+void solver_set_mg_omegas(double omega1, double omega2) {
+    S().mg_omega = omega1;
+    S().mg_omega2 = omega2;
+    S().sched.clear();
+}
 
 // Number of multigrid levels for the current grid (0: unavailable).
 int solver_mg_levels() { return S().setup_mg() ? (int)S().mg.size() : 0; }
@@ -380,8 +387,9 @@ int maze_test_prolong_add(const double* in, double* out, int nc) {
 }
 
 // Fused multigrid level kernels on FP32 host buffers (n even).
-int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c) {
-    if (n < 2 || n % 2 != 0) return -1;
+int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c, int tile) {
+    if (n < 2 || n % 2 != 0 || tile >= maze::kMgTiles) return -1;
+    if (tile < 0) tile = maze::mg_default_tile(n);
     Sim* s = scratch_sim(n);
     const long G = s->G, Gc = G / 8;
     float *db = nullptr, *dd = nullptr, *dc = nullptr;
@@ -389,7 +397,7 @@ int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c) {
     MAZE_HIP(hipMalloc(&dd, G * sizeof(float)));
     MAZE_HIP(hipMalloc(&dc, Gc * sizeof(float)));
     MAZE_HIP(hipMemcpy(db, b, G * sizeof(float), hipMemcpyHostToDevice));
-    maze::launch_mg_down(s->stream, n, db, dd, dc, c, s->st);
+    maze::launch_mg_down(s->stream, n, tile, db, dd, dc, c, c, s->st);
     s->sync();
     MAZE_HIP(hipMemcpy(d2, dd, G * sizeof(float), hipMemcpyDeviceToHost));
     MAZE_HIP(hipMemcpy(bc, dc, Gc * sizeof(float), hipMemcpyDeviceToHost));
@@ -398,8 +406,10 @@ int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c) {
     return 0;
 }
 
-int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, int n, double c) {
-    if (n < 2 || n % 2 != 0) return -1;
+int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, int n, double c,
+                    int tile) {
+    if (n < 2 || n % 2 != 0 || tile >= maze::kMgTiles) return -1;
+    if (tile < 0) tile = maze::mg_default_tile(n);
     Sim* s = scratch_sim(n);
     const long G = s->G, Gc = G / 8;
     float *db = nullptr, *dd2 = nullptr, *de = nullptr, *dd = nullptr;
@@ -410,7 +420,7 @@ int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, i
     MAZE_HIP(hipMemcpy(db, b, G * sizeof(float), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(dd2, d2, G * sizeof(float), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(de, e, Gc * sizeof(float), hipMemcpyHostToDevice));
-    maze::launch_mg_up(s->stream, n, db, dd2, de, dd, c, s->st, 0, s->red);
+    maze::launch_mg_up(s->stream, n, tile, db, dd2, de, dd, c, c, s->st, 0, s->red);
     s->sync();
     MAZE_HIP(hipMemcpy(d, dd, G * sizeof(float), hipMemcpyDeviceToHost));
     for (float* a : {db, dd2, de, dd}) MAZE_HIP(hipFree(a));
@@ -433,7 +443,8 @@ int maze_test_mg_tail(const float* b, float* d, int n) {
     for (int k = 0; k < nl; k++) ns[k] = s->mg[l + k].n;
     const long g = (long)ns[0] * ns[0] * ns[0];
     MAZE_HIP(hipMemcpy(s->mg[l].b, b, g * sizeof(float), hipMemcpyHostToDevice));
-    maze::launch_mg_tail(s->stream, nl, ns, s->mg[l].b, s->mg[l].d, s->mg_omega / 6.0, s->mg_Qf,
+    maze::launch_mg_tail(s->stream, nl, ns, s->mg[l].b, s->mg[l].d, s->mg_omega / 6.0,
+                         s->mg_omega2 / 6.0, s->mg_Qf,
                          s->mg_ilam, s->st);
     s->sync();
     MAZE_HIP(hipMemcpy(d, s->mg[l].d, g * sizeof(float), hipMemcpyDeviceToHost));

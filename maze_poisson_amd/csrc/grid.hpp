@@ -135,15 +135,28 @@ void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const 
 // down: d2 = two pre-smoothing sweeps from zero, b_coarse = restricted residual;
 // up:   d = two post-smoothing sweeps of d2 + prolong(e); dot: also <b, d> and
 //       the PCG beta update (finest level).
-void launch_mg_down(hipStream_t s, int n, const float* b, float* d2, float* bc, double c,
-                    CGState* st);
-void launch_mg_up(hipStream_t s, int n, const float* b, const float* d2, const float* e,
-                  float* d, double c, CGState* st, int dot, Reduce red);
+// `tile` selects the work-group tile shape (MgTile).
+enum MgTile {
+    kMgT8x16x16_1024 = 0,
+    kMgT8x16x16_512,
+    kMgT8x8x16_512,
+    kMgT4x8x8_512,
+    kMgT4x8x8_256,
+    kMgTiles
+};
+int mg_default_tile(int n);
+const char* mg_tile_name(int tile);
+// Smoother weights c1, c2 (damping / 6): pre-smoothing sweeps c1 then c2,
+// post-smoothing c2 then c1 (polynomials in A, so the cycle stays symmetric).
+void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
+                    double c1, double c2, CGState* st);
+void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d2, const float* e,
+                  float* d, double c1, double c2, CGState* st, int dot, Reduce red);
 // Rest of the V-cycle from a small level (sizes n[0..nlev), finest first) down
 // to the exact solve and back, in one work-group with the data in LDS:
 // reads that level's right-hand side b, writes its correction d.
 bool mg_tail_fits(int nlev, const int* n);
-void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float* d, double c,
-                    const double* Qf, const double* ilam, CGState* st);
+void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float* d, double c1,
+                    double c2, const double* Qf, const double* ilam, CGState* st);
 
 }  // namespace maze

@@ -239,8 +239,8 @@ int mg_tiles(int n) {
 template <int TI, int TJ, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__ b,
                                                     float* __restrict__ d2out,
-                                                    float* __restrict__ bcoarse, float c,
-                                                    const CGState* st) {
+                                                    float* __restrict__ bcoarse, float c1,
+                                                    float c2, const CGState* st) {
     if (st->done) return;
     constexpr int BI = TI + 5, BJ = TJ + 5, BK = TK + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
@@ -262,14 +262,15 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
         sb[x] = b[gi[i] + gj[j] + gk[k]];
     });
     __syncthreads();
-    const float c2 = c * c, tc = 2.0f * c;
+    // Sweeps from zero with weights c1 then c2: d2 = -(c1 + c2) b - c1 c2 A b.
+    const float cs = c1 + c2, cp = c1 * c2;
     tile_for<DI * DJ * DK, NT>([&](int x) {
         const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
         const int o = ((i + 1) * BJ + (j + 1)) * BK + (k + 1);
         const float bc = sb[o];
         const float lap = sb[o - BJ * BK] + sb[o + BJ * BK] + sb[o - BK] + sb[o + BK] + sb[o - 1] +
                           sb[o + 1] - 6.0f * bc;
-        sd[x] = -tc * bc - c2 * lap;
+        sd[x] = -cs * bc - cp * lap;
     });
     __syncthreads();
     tile_for<RI * RJ * RK, NT>([&](int x) {
@@ -312,7 +313,8 @@ template <int TI, int TJ, int TK, int NT, bool Dot>
 __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b,
                                                   const float* __restrict__ d2,
                                                   const float* __restrict__ e,
-                                                  float* __restrict__ dout, float c, Reduce red,
+                                                  float* __restrict__ dout, float c1, float c2,
+                                                  Reduce red,
                                                   CGState* st) {
     if (st->done) return;
     constexpr int DI = TI + 4, DJ = TJ + 4, DK = TK + 4;
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
         const float dc = sd[o];
         const float lap = sd[o - DJ * DK] + sd[o + DJ * DK] + sd[o - DK] + sd[o + DK] + sd[o - 1] +
                           sd[o + 1] - 6.0f * dc;
-        sp[x] = dc - c * (sb[x] - lap);
+        sp[x] = dc - c2 * (sb[x] - lap);
     });
     __syncthreads();
     double s = 0.0;
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
         const float lap = sp[o - PJ * PK] + sp[o + PJ * PK] + sp[o - PK] + sp[o + PK] + sp[o - 1] +
                           sp[o + 1] - 6.0f * pc;
         const float bc = sb[o];
-        const float z = pc - c * (bc - lap);
+        const float z = pc - c1 * (bc - lap);
         if (T.i0 + i < n && T.j0 + j < n && T.k0 + k < n) {
             dout[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] = z;
             if (Dot) s += (double)bc * (double)z;
@@ -401,37 +403,56 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
     }
 }
 
-// Tiles of 8x16x16 fine points with 1024 threads (a tile's steps are short
-// dependent chains, so a work-group needs many waves); small levels get
-// 4x8x8 tiles with 512 threads (more work-groups for few points).
-constexpr int kBigNT = 1024, kSmallNT = 512;
-
-void launch_mg_down(hipStream_t s, int n, const float* b, float* d2, float* bc, double c,
-                    CGState* st) {
-    if (n >= 64)
-        MAZE_LAUNCH((k_mg_down<8, 16, 16, kBigNT>), dim3(mg_tiles<8, 16, 16>(n)),
-                           dim3(kBigNT), 0, s, n, b, d2, bc, (float)c, st);
-    else
-        MAZE_LAUNCH((k_mg_down<4, 8, 8, kSmallNT>), dim3(mg_tiles<4, 8, 8>(n)),
-                           dim3(kSmallNT), 0, s, n, b, d2, bc, (float)c, st);
+// Tile shapes (fine points per work-group, threads) of the fused level kernels;
+// Sim picks one per level by timing them (mg_autotune) or takes the default.
+template <int TI, int TJ, int TK, int NT>
+static void down_t(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
+                   double c2, CGState* st) {
+    MAZE_LAUNCH((k_mg_down<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s, n, b,
+                d2, bc, (float)c1, (float)c2, st);
 }
 
-void launch_mg_up(hipStream_t s, int n, const float* b, const float* d2, const float* e,
-                  float* d, double c, CGState* st, int dot, Reduce red) {
-    if (n >= 64) {
-        if (dot)
-            MAZE_LAUNCH((k_mg_up<8, 16, 16, kBigNT, true>), dim3(mg_tiles<8, 16, 16>(n)),
-                               dim3(kBigNT), 0, s, n, b, d2, e, d, (float)c, red, st);
-        else
-            MAZE_LAUNCH((k_mg_up<8, 16, 16, kBigNT, false>), dim3(mg_tiles<8, 16, 16>(n)),
-                               dim3(kBigNT), 0, s, n, b, d2, e, d, (float)c, red, st);
-    } else {
-        if (dot)
-            MAZE_LAUNCH((k_mg_up<4, 8, 8, kSmallNT, true>), dim3(mg_tiles<4, 8, 8>(n)),
-                               dim3(kSmallNT), 0, s, n, b, d2, e, d, (float)c, red, st);
-        else
-            MAZE_LAUNCH((k_mg_up<4, 8, 8, kSmallNT, false>), dim3(mg_tiles<4, 8, 8>(n)),
-                               dim3(kSmallNT), 0, s, n, b, d2, e, d, (float)c, red, st);
+template <int TI, int TJ, int TK, int NT>
+static void up_t(hipStream_t s, int n, const float* b, const float* d2, const float* e, float* d,
+                 double c1, double c2, CGState* st, int dot, Reduce red) {
+    if (dot)
+        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, true>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s,
+                    n, b, d2, e, d, (float)c1, (float)c2, red, st);
+    else
+        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, false>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0,
+                    s, n, b, d2, e, d, (float)c1, (float)c2, red, st);
+}
+
+const char* mg_tile_name(int tile) {
+    static const char* names[kMgTiles] = {"8x16x16/1024", "8x16x16/512", "8x8x16/512",
+                                          "4x8x8/512", "4x8x8/256"};
+    return tile >= 0 && tile < kMgTiles ? names[tile] : "?";
+}
+
+// Large levels: 8x16x16 tiles with 1024 threads (a tile's steps are short
+// dependent chains, so a work-group needs many waves); small levels: 4x8x8
+// tiles with 512 threads (more work-groups for few points).
+int mg_default_tile(int n) { return n >= 64 ? kMgT8x16x16_1024 : kMgT4x8x8_512; }
+
+void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
+                    double c1, double c2, CGState* st) {
+    switch (tile) {
+        case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT4x8x8_512: down_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st); break;
+        default: down_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st); break;
+    }
+}
+
+void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d2, const float* e,
+                  float* d, double c1, double c2, CGState* st, int dot, Reduce red) {
+    switch (tile) {
+        case kMgT8x16x16_1024: up_t<8, 16, 16, 1024>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
+        case kMgT8x16x16_512: up_t<8, 16, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
+        case kMgT8x8x16_512: up_t<8, 8, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
+        case kMgT4x8x8_512: up_t<4, 8, 8, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
+        default: up_t<4, 8, 8, 256>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
     }
 }
 
@@ -554,7 +575,8 @@ struct TailCol {
 };
 
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const float* __restrict__ bin,
-                                                         float* __restrict__ dout, float c,
+                                                         float* __restrict__ dout, float c1,
+                                                         float c2,
                                                          const double* __restrict__ Qf,
                                                          const double* __restrict__ ilam,
                                                          const CGState* st) {
@@ -582,8 +604,8 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
             }
     }
     __syncthreads();
-    const float c2 = c * c, tc = 2.0f * c;
-    // ---- down: d2 = -2c b - c^2 A b, t = b - A d2, restrict t ----
+    const float cs = c1 + c2, cp = c1 * c2;
+    // ---- down: d2 = -(c1 + c2) b - c1 c2 A b, t = b - A d2, restrict t ----
     for (int l = 0; l < T.nlev - 1; l++) {
         const int m = T.n[l], m2 = m * m;
         const float* b = B[l];
@@ -596,7 +618,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
                           ip = (i == m - 1 ? 0 : i + 1) * m2;
                 const int x = i0 + col.j0 + col.k0;
-                d[x] = -tc * b[x] - c2 * col.lap(b, im, i0, ip);
+                d[x] = -cs * b[x] - cp * col.lap(b, im, i0, ip);
             }
         __syncthreads();
         if (on)
@@ -648,7 +670,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
         for (int x = tid; x < N; x += kTailThreads) D[L][x] = (float)A[x];
         __syncthreads();
     }
-    // ---- up: d += trilinear(e), two sweeps d <- d - c (b - A d) ----
+    // ---- up: d += trilinear(e), sweeps d <- d - c (b - A d) with c2 then c1 ----
     for (int l = T.nlev - 2; l >= 0; l--) {
         const int m = T.n[l], m2 = m * m, nc = T.n[l + 1];
         const float* b = B[l];
@@ -682,7 +704,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
                           ip = (i == m - 1 ? 0 : i + 1) * m2;
                 const int x = i0 + col.j0 + col.k0;
-                t[x] = d[x] - c * (b[x] - col.lap(d, im, i0, ip));
+                t[x] = d[x] - c2 * (b[x] - col.lap(d, im, i0, ip));
             }
         __syncthreads();
         if (on)
@@ -690,7 +712,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
                           ip = (i == m - 1 ? 0 : i + 1) * m2;
                 const int x = i0 + col.j0 + col.k0;
-                d[x] = t[x] - c * (b[x] - col.lap(t, im, i0, ip));
+                d[x] = t[x] - c1 * (b[x] - col.lap(t, im, i0, ip));
             }
         __syncthreads();
     }
@@ -714,13 +736,13 @@ bool mg_tail_fits(int nlev, const int* n) {
     return floats <= 2L * kTailPool && n[nlev - 1] <= kMaxCoarse;
 }
 
-void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float* d, double c,
-                    const double* Qf, const double* ilam, CGState* st) {
+void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float* d, double c1,
+                    double c2, const double* Qf, const double* ilam, CGState* st) {
     TailLevels T;
     T.nlev = nlev;
     for (int l = 0; l < nlev; l++) T.n[l] = n[l];
-    MAZE_LAUNCH(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s, T, b, d, (float)c, Qf, ilam,
-                       st);
+    MAZE_LAUNCH(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s, T, b, d, (float)c1, (float)c2, Qf,
+                ilam, st);
 }
 
 // PCG direction p = -v (first) or beta p - v, with <p, A p>

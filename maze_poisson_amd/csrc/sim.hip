@@ -39,13 +39,21 @@ static void dfree(T*& p) {
     p = nullptr;
 }
 
-// Planes per work-group so that one sweep launches about 1024 work-groups
-// (measured: twice as many work-groups with half the planes each is slower).
+// Planes per work-group so that one sweep launches about 512 work-groups.
+// Measured at 128^3: 16 planes per work-group (512 work-groups) is fastest;
+// 8 planes is 1.3x slower and 4 planes 1.8x slower (each work-group reloads
+// its two halo planes and pays the reduction hand-off), 64 planes is 2.6x
+// slower (too few waves).
 static int choose_slab(int n) {
     const long chunks = ((long)n * n + kBlock - 1) / kBlock;
-    long s = chunks * n / 1024;
+    static const long target = [] {
+        const char* e = std::getenv("MAZE_SLAB_WG");  // tuning override
+        const long v = e ? std::atol(e) : 0;
+        return v > 0 ? v : 512L;
+    }();
+    long s = chunks * n / target;
     if (s < 1) s = 1;
-    if (s > 16) s = 16;
+    if (s > 128) s = 128;
     return (int)s;
 }
 
@@ -414,6 +422,15 @@ bool Sim::setup_mg() {
         L.dims = make_dims(L.n, choose_slab(L.n));
         const long g = (long)L.n * L.n * L.n;
         L.b = l == 0 ? nullptr : dalloc<float>(g);
+        L.tile = mg_default_tile(L.n);
+        if (const char* e = std::getenv("MAZE_MG_TILES")) {  // "t0,t1,...": tuning override
+            const char* q = e;
+            for (size_t k = 0; k < l && q; k++) {
+                q = std::strchr(q, ',');
+                if (q) q++;
+            }
+            if (q && *q >= '0' && *q < '0' + kMgTiles) L.tile = *q - '0';
+        }
         L.d = dalloc<float>(g);
         L.t = dalloc<float>(g);
         mg.push_back(L);
@@ -471,12 +488,12 @@ bool Sim::setup_mg() {
 // as one fused down and one fused up kernel per level above the tail.
 void Sim::vcycle(int l, int iter) {
     MGLevel& L = mg[l];
-    const double cw = mg_omega / 6.0;
+    const double c1 = mg_omega / 6.0, c2 = mg_omega2 / 6.0;
     if (l >= mg_tail) {
         int ns[8];
         const int nl = (int)mg.size() - l;
         for (int k = 0; k < nl; k++) ns[k] = mg[l + k].n;
-        launch_mg_tail(stream, nl, ns, L.b, L.d, cw, mg_Qf, mg_ilam, st);
+        launch_mg_tail(stream, nl, ns, L.b, L.d, c1, c2, mg_Qf, mg_ilam, st);
         return;
     }
     if (l == (int)mg.size() - 1) {
@@ -488,9 +505,11 @@ void Sim::vcycle(int l, int iter) {
         if (top) timed(timer, iter, f);
         else f();
     };
-    t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.b, L.t, mg[l + 1].b, cw, st); });
+    t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st); });
     vcycle(l + 1, iter);
-    t(kTMgUp, [&] { launch_mg_up(stream, L.n, L.b, L.t, mg[l + 1].d, L.d, cw, st, top, red); });
+    t(kTMgUp, [&] {
+        launch_mg_up(stream, L.n, L.tile, L.b, L.t, mg[l + 1].d, L.d, c1, c2, st, top, red);
+    });
 }
 
 // MaZe step in FP64, library/src/verlet.c:31-71.

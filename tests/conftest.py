@@ -111,8 +111,8 @@ def hip_raw(hip):
         "maze_test_restrict_pb_coef": (ci, [D, D, D, D, D, D, D, D, ci]),
         "maze_test_mg_vcycle_pb": (ci, [D, D, ci, D, D, D, D]),
         "maze_test_mg_pcg": (ci, [D, D, cd, ci]),
-        "maze_test_mg_down": (ci, [F, F, F, ci, cd]),
-        "maze_test_mg_up": (ci, [F, F, F, F, ci, cd]),
+        "maze_test_mg_down": (ci, [F, F, F, ci, cd, ci]),
+        "maze_test_mg_up": (ci, [F, F, F, F, ci, cd, ci]),
         "maze_test_mg_tail": (ci, [F, F, ci]),
         "solver_set_mg_omega": (None, [cd]),
         "solver_mg_levels": (ci, []),

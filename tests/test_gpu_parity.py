@@ -269,15 +269,19 @@ def _prolong_trilinear(e):
     return e
 
 
-@pytest.mark.parametrize("n", [8, 16, 20, 64, 72])
-def test_mg_fused_down(hip_raw, n):
+MG_TILE_CASES = [(n, -1) for n in (8, 16, 20, 64, 72)] + [(n, t) for n in (20, 64) for t in range(5)]
+
+
+@pytest.mark.parametrize("n,tile", MG_TILE_CASES)
+def test_mg_fused_down(hip_raw, n, tile):
     """Fused FP32 down kernel (two Jacobi sweeps from zero, residual, restriction) vs the
-    same steps in FP64 NumPy; n = 20 and 72 leave partial tiles, n = 8 wraps the halo."""
+    same steps in FP64 NumPy, for every tile shape; n = 20 and 72 leave partial tiles,
+    n = 8 wraps the halo."""
     c = (6.0 / 7.0) / 6.0
     b = np.random.default_rng(n).standard_normal((n, n, n)).astype(np.float32)
     d2 = np.zeros_like(b)
     bc = np.zeros((n // 2,) * 3, np.float32)
-    assert hip_raw.maze_test_mg_down(b, d2, bc, n, c) == 0
+    assert hip_raw.maze_test_mg_down(b, d2, bc, n, c, tile) == 0
     b64 = b.astype(np.float64)
     d1 = -c * b64
     d2_ref = d1 - c * (b64 - _lap(d1))
@@ -286,8 +290,8 @@ def test_mg_fused_down(hip_raw, n):
     assert np.abs(bc - bc_ref).max() <= 2e-6 * np.abs(bc_ref).max()
 
 
-@pytest.mark.parametrize("n", [8, 16, 20, 64, 72])
-def test_mg_fused_up(hip_raw, n):
+@pytest.mark.parametrize("n,tile", MG_TILE_CASES)
+def test_mg_fused_up(hip_raw, n, tile):
     """Fused FP32 up kernel (trilinear prolongation-correction, two Jacobi sweeps)."""
     c = (6.0 / 7.0) / 6.0
     g = np.random.default_rng(100 + n)
@@ -295,7 +299,7 @@ def test_mg_fused_up(hip_raw, n):
     d2 = g.standard_normal((n, n, n)).astype(np.float32)
     e = g.standard_normal((n // 2,) * 3).astype(np.float32)
     d = np.zeros_like(b)
-    assert hip_raw.maze_test_mg_up(b, d2, e, d, n, c) == 0
+    assert hip_raw.maze_test_mg_up(b, d2, e, d, n, c, tile) == 0
     b64 = b.astype(np.float64)
     x = d2.astype(np.float64) + _prolong_trilinear(e.astype(np.float64))
     for _ in range(2):
