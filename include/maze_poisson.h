@@ -201,8 +201,8 @@ void solver_set_charge_check(int strict);
 
 /* The MaZe constraint field y, double[n][n][n]. */
 void get_y(double *recv);
-/* CIC neighbour table as the reference stores it (particles.c:297-346):
- * int64[n_p][8][3]. */
+/* Assignment-node table as the reference stores it: int64[n_p][8][3] for CIC
+ * (particles.c:297-346), int64[n_p][64][3] for the splines (:348-385). */
 void get_neighbors(int64_t *recv);
 /* Number of partners within the Tosi-Fumi cutoff per ion, int32[n_p]. */
 void get_tf_neighbor_counts(int *recv);

@@ -87,11 +87,7 @@ void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, 
         std::fprintf(stderr, "maze: only the TF pair potential is provided (got %d)\n", pot_type);
         std::exit(1);
     }
-    if (cas_type != 0) {
-        std::fprintf(stderr, "maze: only CIC charge assignment is provided (got %d)\n", cas_type);
-        std::exit(1);
-    }
-    S().init_particles(n, n_typ, L, h, n_p, types, pos, vel, mass, charges, pot_params);
+    S().init_particles(n, n_typ, L, h, n_p, cas_type, types, pos, vel, mass, charges, pot_params);
 }
 
 void solver_initialize_particles_pois_boltz(double gamma_np, double beta_np, double* radii) {
@@ -215,10 +211,14 @@ void get_neighbors(int64_t* o) {
     d2h(base.data(), s.p.base, 3L * np * sizeof(int));
     static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
                                   {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    const int n = s.p.n, nn = s.p.nn;
     for (int i = 0; i < np; i++)
-        for (int c = 0; c < 8; c++)
-            for (int d = 0; d < 3; d++)
-                o[(i * 8L + c) * 3 + d] = (base[3 * i + d] + off[c][d]) % s.p.n;
+        for (int c = 0; c < nn; c++)
+            for (int d = 0; d < 3; d++) {
+                const int o_cd = nn == 8 ? off[c][d]
+                                         : (d == 0 ? c >> 4 : (d == 1 ? (c >> 2) & 3 : c & 3)) - 1;
+                o[((long)i * nn + c) * 3 + d] = (base[3 * i + d] + o_cd + n) % n;
+            }
 }
 
 void get_tf_neighbor_counts(int* o) { d2h(o, S().p.counts, S().p.np * sizeof(int)); }

@@ -78,64 +78,127 @@ __device__ __forceinline__ int lower_bound_u32(const unsigned int* a, int n, uns
     return lo;
 }
 
+// x^3 rounded once (the reference's pow(x, 3)): x*x split exactly into
+// hi + lo by an FMA, then hi*x + lo*x with one final rounding.
+__device__ __forceinline__ double cube(double x) {
+    const double hi = x * x;
+    const double lo = fma(x, x, -hi);
+    return fma(hi, x, lo * x);
+}
+
+// Quadratic and cubic B-spline weights (library/src/charges.c:16-38).
+__device__ __forceinline__ double spline_quadr_w(double x, double L, double h) {
+    x = fabs(x - round(x / L) * L) / h;
+    if (x < 0.5) return 0.75 - x * x;
+    if (x < 1.5) {
+        const double y = 1.5 - x;
+        return 0.5 * (y * y);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double spline_cubic_w(double x, double L, double h) {
+    x = fabs(x - round(x / L) * L) / h;
+    if (x < 1) return (4.0 - 6.0 * (x * x) + 3.0 * cube(x)) / 6.0;
+    if (x < 2) return cube(2 - x) / 6.0;
+    return 0.0;
+}
+
+// Charge-assignment schemes (library/src/particles.c:38-66): CIC on the 8
+// corners of the base cell in the reference's order (:297-346), splines on
+// the 4x4x4 block base + {-1..2}^3 with the last offset fastest (:348-385).
+template <int CAS>
+struct Assign;
+
+template <>
+struct Assign<0> {
+    static constexpr int kNodes = 8, kLo = 0, kSpan = 2;  // node - base in [kLo, kLo + kSpan)
+    __device__ static int off(int c, int d) { return kCorner[c][d]; }
+    __device__ static double w(double x, double L, double h) { return cic_w(x, L, h); }
+};
+
+template <int CAS>
+struct AssignSpline {
+    static constexpr int kNodes = 64, kLo = -1, kSpan = 4;
+    __device__ static int off(int c, int d) {
+        return (d == 0 ? c >> 4 : (d == 1 ? (c >> 2) & 3 : c & 3)) - 1;
+    }
+    __device__ static double w(double x, double L, double h) {
+        return CAS == 1 ? spline_quadr_w(x, L, h) : spline_cubic_w(x, L, h);
+    }
+};
+template <>
+struct Assign<1> : AssignSpline<1> {};
+template <>
+struct Assign<2> : AssignSpline<2> {};
+
 // Contribution of ion j to grid node (gi, gj, gk): q_j * wx * wy * wz
 // (library/src/charges.c:136-137).
-__device__ __forceinline__ double cic_contrib(const double* pos, const double* charges, int j,
-                                             int gi, int gj, int gk, double L, double h) {
-    const double app = cic_w(pos[3 * j] - gi * h, L, h) * cic_w(pos[3 * j + 1] - gj * h, L, h) *
-                       cic_w(pos[3 * j + 2] - gk * h, L, h);
+template <class A>
+__device__ __forceinline__ double assign_contrib(const double* pos, const double* charges, int j,
+                                                 int gi, int gj, int gk, double L, double h) {
+    const double app = A::w(pos[3 * j] - gi * h, L, h) * A::w(pos[3 * j + 1] - gj * h, L, h) *
+                       A::w(pos[3 * j + 2] - gk * h, L, h);
     return charges[j] * app;
 }
 
-// Owner-computes gather: the thread of (ion i, corner c) finds every ion
-// whose CIC stencil covers node g = base_i + corner_c (ions in the 8 cells
-// g - {0,1}^3, looked up in the sorted cell keys).  Only the lowest ion id
-// writes q[g], summing the contributions in increasing ion id -- the order of
-// the reference's serial scatter -- so no float atomics are needed and the
-// result is bit-identical to it.
-__global__ void k_cic_gather(int np, int n, double h, const double* __restrict__ pos,
-                             const double* __restrict__ charges, const int* __restrict__ base,
-                             const unsigned int* __restrict__ keys_sorted,
-                             const int* __restrict__ ids_sorted, double* __restrict__ q,
-                             double* __restrict__ contrib) {
+// Owner-computes gather: the thread of (ion i, node c) finds every ion whose
+// stencil covers node g = base_i + offset_c: ions whose base cell lies in the
+// kSpan^3 block g - offsets, looked up row by row ((i, j) fixed, k along the
+// row) in the sorted cell keys.  Only the lowest ion id writes q[g], summing
+// the contributions in increasing ion id -- the order of the reference's
+// serial scatter -- so no float atomics are needed and q matches the
+// reference's serial result.
+template <int CAS>
+__global__ void k_assign_gather(int np, int n, double h, const double* __restrict__ pos,
+                                const double* __restrict__ charges, const int* __restrict__ base,
+                                const unsigned int* __restrict__ keys_sorted,
+                                const int* __restrict__ ids_sorted, double* __restrict__ q,
+                                double* __restrict__ contrib) {
+    using A = Assign<CAS>;
+    constexpr int NN = A::kNodes, S = A::kSpan;
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= 8L * np) return;
-    const int i = (int)(t >> 3), c = (int)(t & 7);
+    if (t >= (long)NN * np) return;
+    const int i = (int)(t / NN), c = (int)(t % NN);
     const double L = n * h;
-    const int gi = (base[3 * i] + kCorner[c][0]) % n;
-    const int gj = (base[3 * i + 1] + kCorner[c][1]) % n;
-    const int gk = (base[3 * i + 2] + kCorner[c][2]) % n;
-    contrib[t] = cic_contrib(pos, charges, i, gi, gj, gk, L, h);
+    const int gi = (base[3 * i] + A::off(c, 0) + n) % n;
+    const int gj = (base[3 * i + 1] + A::off(c, 1) + n) % n;
+    const int gk = (base[3 * i + 2] + A::off(c, 2) + n) % n;
+    contrib[t] = assign_contrib<A>(pos, charges, i, gi, gj, gk, L, h);
 
-    // Candidate cells and their ranges in the sorted key array.
-    int lo[8], hi[8];
+    // Rows (ci, cj) of candidate base cells and their ranges in the sorted keys.
+    int lo[S * S], hi[S * S];
     int owner = i;
-    for (int cc = 0; cc < 8; cc++) {
-        const int ci = (gi - kCorner[cc][0] + n) % n;
-        const int cj = (gj - kCorner[cc][1] + n) % n;
-        const int ck = (gk - kCorner[cc][2] + n) % n;
-        const unsigned int key = ((unsigned int)ci * n + cj) * n + ck;
-        int a = lower_bound_u32(keys_sorted, np, key), b = a;
-        while (b < np && keys_sorted[b] == key) {
-            owner = min(owner, ids_sorted[b]);
-            b++;
+    for (int a = 0; a < S; a++)
+        for (int b = 0; b < S; b++) {
+            const int ci = (gi - (A::kLo + a) + 2 * n) % n;
+            const int cj = (gj - (A::kLo + b) + 2 * n) % n;
+            const unsigned int row = ((unsigned int)ci * n + cj) * n;
+            const int r0 = lower_bound_u32(keys_sorted, np, row);
+            const int r1 = lower_bound_u32(keys_sorted, np, row + n);
+            lo[a * S + b] = r0;
+            hi[a * S + b] = r1;
+            for (int s = r0; s < r1; s++) {
+                const int dk = (gk - (int)(keys_sorted[s] - row) + n) % n;  // node - base along k
+                if (((dk - A::kLo + n) % n) < S) owner = min(owner, ids_sorted[s]);
+            }
         }
-        lo[cc] = a;
-        hi[cc] = b;
-    }
     if (owner != i) return;
     // Sum contributions in increasing ion id (selection over the candidates).
     double acc = 0.0;
     int last = -1;
     for (;;) {
         int next = 0x7fffffff;
-        for (int cc = 0; cc < 8; cc++)
-            for (int s = lo[cc]; s < hi[cc]; s++) {
+        for (int r = 0; r < S * S; r++)
+            for (int s = lo[r]; s < hi[r]; s++) {
+                const unsigned int row = keys_sorted[s] - keys_sorted[s] % n;
+                const int dk = (gk - (int)(keys_sorted[s] - row) + n) % n;
+                if (((dk - A::kLo + n) % n) >= S) continue;
                 const int j = ids_sorted[s];
                 if (j > last && j < next) next = j;
             }
         if (next == 0x7fffffff) break;
-        acc += cic_contrib(pos, charges, next, gi, gj, gk, L, h);
+        acc += assign_contrib<A>(pos, charges, next, gi, gj, gk, L, h);
         last = next;
     }
     q[((long)gi * n + gj) * n + gk] = acc;
@@ -143,30 +206,40 @@ __global__ void k_cic_gather(int np, int n, double h, const double* __restrict__
 
 void launch_cic_gather(hipStream_t s, const PDev& p, double* q) {
     if (p.np == 0) return;
-    MAZE_LAUNCH(k_cic_gather, dim3(blocks_for(8L * p.np)), dim3(kBlock), 0, s, p.np, p.n,
-                       p.h, p.pos, p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
+    const int nb = blocks_for((long)p.nn * p.np);
+    if (p.cas == 0)
+        MAZE_LAUNCH(k_assign_gather<0>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
+                    p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
+    else if (p.cas == 1)
+        MAZE_LAUNCH(k_assign_gather<1>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
+                    p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
+    else
+        MAZE_LAUNCH(k_assign_gather<2>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
+                    p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
 }
 
 void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
-    launch_sum_scaled(s, 8L * p.np, 1.0, p.contrib, &ps->q_sum, red);
+    launch_sum_scaled(s, (long)p.nn * p.np, 1.0, p.contrib, &ps->q_sum, red);
 }
 
-// Field force -q w grad(phi) by central differences at the 8 CIC corners
+// Field force -q w grad(phi) by central differences at the assignment nodes
 // (library/src/forces.c:52-94).
+template <int CAS>
 __global__ void k_force_fd(int np, int n, double h, const double* __restrict__ phi,
                            const int* __restrict__ base, const double* __restrict__ charges,
                            const double* __restrict__ pos, double* __restrict__ forces) {
+    using A = Assign<CAS>;
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= np) return;
     const long N = n, N2 = N * N;
     const double h2 = 2.0 * h, L = N * h;
     const double px = pos[3 * ip], py = pos[3 * ip + 1], pz = pos[3 * ip + 2], chg = charges[ip];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int c = 0; c < 8; c++) {
-        const long i = (base[3 * ip] + kCorner[c][0]) % n;
-        const long j = (base[3 * ip + 1] + kCorner[c][1]) % n;
-        const long k = (base[3 * ip + 2] + kCorner[c][2]) % n;
-        const double qc = chg * cic_w(px - i * h, L, h) * cic_w(py - j * h, L, h) * cic_w(pz - k * h, L, h);
+    for (int c = 0; c < A::kNodes; c++) {
+        const long i = (base[3 * ip] + A::off(c, 0) + n) % n;
+        const long j = (base[3 * ip + 1] + A::off(c, 1) + n) % n;
+        const long k = (base[3 * ip + 2] + A::off(c, 2) + n) % n;
+        const double qc = chg * A::w(px - i * h, L, h) * A::w(py - j * h, L, h) * A::w(pz - k * h, L, h);
         const long im = (i == 0 ? N - 1 : i - 1), ii = (i == N - 1 ? 0 : i + 1);
         const long jm = (j == 0 ? N - 1 : j - 1), jj = (j == N - 1 ? 0 : j + 1);
         const long km = (k == 0 ? N - 1 : k - 1), kk = (k == N - 1 ? 0 : k + 1);
@@ -184,8 +257,16 @@ __global__ void k_force_fd(int np, int n, double h, const double* __restrict__ p
 
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi) {
     if (p.np == 0) return;
-    MAZE_LAUNCH(k_force_fd, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
-                       p.base, p.charges, p.pos, p.fe);
+    const int nb = blocks_for(p.np);
+    if (p.cas == 0)
+        MAZE_LAUNCH(k_force_fd<0>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi, p.base,
+                    p.charges, p.pos, p.fe);
+    else if (p.cas == 1)
+        MAZE_LAUNCH(k_force_fd<1>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi, p.base,
+                    p.charges, p.pos, p.fe);
+    else
+        MAZE_LAUNCH(k_force_fd<2>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi, p.base,
+                    p.charges, p.pos, p.fe);
 }
 
 // Shifted-force Tosi-Fumi over all pairs within r_cut (library/src/forces.c:

@@ -8,6 +8,7 @@ namespace maze {
 
 struct PDev {
     int np, n_typ, n;
+    int cas, nn;      // charge-assignment scheme (0 CIC, 1/2 quadratic/cubic spline), nodes per ion
     double L, h, r_cut;
     int* types;
     double* pos;      // (np, 3) Bohr
@@ -26,7 +27,7 @@ struct PDev {
     double* tpart;    // Tosi-Fumi partials per 64-ion block pair (tf_part_doubles)
     double* eion;     // Tosi-Fumi energy per ion (each pair's energy on both ions)
     int* counts;      // pairs within the cutoff per ion
-    double* contrib;  // per-(ion, corner) CIC contribution, (np, 8)
+    double* contrib;  // per-(ion, node) assignment contribution, (np, nn)
     void* sort_tmp;
     size_t sort_tmp_bytes;
 };

@@ -177,13 +177,17 @@ void Sim::set_eps_k2(const double* ex_, const double* ey_, const double* ez_, co
 }
 
 // library/src/solver_md.c:51-65, particles.c:68-109, :181-228.
-void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, const int* types,
-                         const double* pos, const double* vel, const double* mass,
-                         const double* charges, const double* pp) {
+void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, int cas,
+                         const int* types, const double* pos, const double* vel,
+                         const double* mass, const double* charges, const double* pp) {
+    if (cas < 0 || cas > 2) fatal("Invalid charge assignment scheme type");
+    if (cas != 0 && n_ < 4) fatal("maze: spline charge assignment needs n >= 4");
     if (n_typ < 1 || n_typ > 16) fatal("maze: between 1 and 16 particle types are supported");
     if (np < 0) fatal("maze: negative particle count");
     free_particles();
     p.np = np; p.n_typ = n_typ; p.n = n_; p.L = L_; p.h = h_;
+    p.cas = cas;
+    p.nn = cas == 0 ? 8 : 64;
     p.r_cut = L_ / 2.0;
     p.types = dalloc<int>(np);
     p.pos = dalloc<double>(3L * np); p.vel = dalloc<double>(3L * np);
@@ -192,7 +196,7 @@ void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, const 
     p.base = dalloc<int>(3L * np);
     p.keys = dalloc<unsigned int>(np); p.keys_sorted = dalloc<unsigned int>(np);
     p.ids = dalloc<int>(np); p.ids_sorted = dalloc<int>(np);
-    p.contrib = dalloc<double>(8L * np);
+    p.contrib = dalloc<double>((long)p.nn * np);
     p.tpart = dalloc<double>(tf_part_doubles(np));
     p.eion = dalloc<double>(np);
     p.counts = dalloc<int>(np);

@@ -155,7 +155,7 @@ struct Sim {
                    int precond);
     void init_pb(double w, double kbar2);
     void set_eps_k2(const double* ex, const double* ey, const double* ez, const double* k2);
-    void init_particles(int n, int n_typ, double L, double h, int np, const int* types,
+    void init_particles(int n, int n_typ, double L, double h, int np, int cas, const int* types,
                         const double* pos, const double* vel, const double* mass,
                         const double* charges, const double* pot_params);
     void init_integrator(double dt, double T, double gamma, int type, int enabled);

@@ -28,6 +28,7 @@ class RunConfig:
     gamma: float = 1e-3
     integrator: str = "VERLET"
     thermostat: bool = False
+    cas: str = "CIC"          # charge assignment: CIC, SPL_QUADR or SPL_CUBIC
     # Linearized Poisson-Boltzmann: synthetic face dielectric in [2, 80] (seeded) and a
     # uniform screening term for `ionic_molar` at eps_s (= 80 in water).
     pb: bool = False
@@ -49,7 +50,7 @@ def initialize(lib: MazeLib, s: System, cfg: RunConfig) -> None:
         ex, ey, ez = pb_eps_field(s.n_grid, s.L, cfg.eps_seed)
         lib.solver_set_eps_k2(ex, ey, ez, np.full((s.n_grid,) * 3, kb2))
     lib.solver_initialize_particles(s.n_grid, s.n_typ, s.L, s.h, s.n_p,
-                                    maps["potential"]["TF"], maps["cas"]["CIC"],
+                                    maps["potential"]["TF"], maps["cas"][cfg.cas.upper()],
                                     s.types, s.pos, s.vel, s.mass, s.charges, s.tf)
     lib.solver_initialize_integrator(s.n_p, dt_au(cfg.dt_fs), cfg.T, cfg.gamma,
                                      maps["integrator"][cfg.integrator.upper()],

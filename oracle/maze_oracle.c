@@ -575,15 +575,42 @@ double or_cic_weight(double x, double L, double h) {
     return 1.0 - x / h;
 }
 
-/* Base cell floor(pos/h) and the +1 corners, stored (n_p, 8, 3) in the
- * reference's corner order 000,100,010,001,110,101,011,111.
- * library/src/particles.c:297-346.  The base index is wrapped into [0,n):
- * the reference leaves it unwrapped (:315-317), which only matters for a
+/* Quadratic and cubic B-spline weights.  library/src/charges.c:16-38 */
+double or_spline_quadr_weight(double x, double L, double h) {
+    x = fabs(x - round(x / L) * L) / h;
+    if (x < 0.5) return 0.75 - x * x;
+    if (x < 1.5) return 0.5 * pow((1.5 - x), 2);
+    return 0.0;
+}
+
+double or_spline_cubic_weight(double x, double L, double h) {
+    x = fabs(x - round(x / L) * L) / h;
+    if (x < 1) return (4.0 - 6.0 * pow(x, 2) + 3.0 * pow(x, 3)) / 6.0;
+    if (x < 2) return pow(2 - x, 3) / 6.0;
+    return 0.0;
+}
+
+/* Charge-assignment scheme: 0 CIC (8 nodes), 1 quadratic, 2 cubic spline (64
+ * nodes).  library/src/particles.c:38-66 */
+int or_cas_nodes(int cas) { return cas == 0 ? 8 : 64; }
+
+static double cas_weight(int cas, double x, double L, double h) {
+    if (cas == 1) return or_spline_quadr_weight(x, L, h);
+    if (cas == 2) return or_spline_cubic_weight(x, L, h);
+    return or_cic_weight(x, L, h);
+}
+
+/* Base cell floor(pos/h) and the assignment nodes, stored (n_p, nodes, 3):
+ * CIC in the reference's corner order 000,100,010,001,110,101,011,111
+ * (library/src/particles.c:297-346); splines the 4x4x4 block base + {-1..2}^3,
+ * last offset fastest (:348-385).  The base index is wrapped into [0,n): the
+ * reference leaves it unwrapped (:315-317, :367-369), which only matters for a
  * position that rounds to pos/h >= n, where the reference would index past
  * the grid. */
-void or_cic_neighbors(const double *pos, int np, int n, double h, int64_t *nb) {
+void or_neighbors(int cas, const double *pos, int np, int n, double h, int64_t *nb) {
     static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
                                   {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    const int nn = or_cas_nodes(cas);
     for (int i = 0; i < np; i++) {
         int base[3];
         for (int d = 0; d < 3; d++) {
@@ -592,26 +619,44 @@ void or_cic_neighbors(const double *pos, int np, int n, double h, int64_t *nb) {
             if (c < 0) c += n;
             base[d] = c;
         }
-        for (int c = 0; c < 8; c++)
-            for (int d = 0; d < 3; d++) nb[(i * 8 + c) * 3 + d] = (base[d] + off[c][d]) % n;
+        int64_t *o = nb + (long)i * nn * 3;
+        if (cas == 0) {
+            for (int c = 0; c < 8; c++)
+                for (int d = 0; d < 3; d++) o[c * 3 + d] = (base[d] + off[c][d]) % n;
+        } else {
+            for (int a = -1; a <= 2; a++)
+                for (int b = -1; b <= 2; b++)
+                    for (int c = -1; c <= 2; c++) {
+                        o[0] = (base[0] + a + n) % n;
+                        o[1] = (base[1] + b + n) % n;
+                        o[2] = (base[2] + c + n) % n;
+                        o += 3;
+                    }
+        }
     }
 }
 
-/* Scatter q_ion * wx * wy * wz to the 8 corners, serial ion order; returns
- * the sum of all contributions.  The period used inside the weight is n*h
- * (library/src/charges.c:100-144, :113). */
-double or_spread_cic(int n, int np, double h, const double *pos, const int64_t *nb,
-                     const double *charges, double *q) {
+void or_cic_neighbors(const double *pos, int np, int n, double h, int64_t *nb) {
+    or_neighbors(0, pos, np, n, h, nb);
+}
+
+/* Scatter q_ion * wx * wy * wz to the assignment nodes, serial ion order;
+ * returns the sum of all contributions.  The period used inside the weight is
+ * n*h.  library/src/charges.c:100-144 (:113, :130-140). */
+double or_spread(int cas, int n, int np, double h, const double *pos, const int64_t *nb,
+                 const double *charges, double *q) {
     const long N = n, N2 = N * N, G = N2 * N;
     const double L = n * h;
+    const int nn = or_cas_nodes(cas);
     double q_tot = 0.0;
     for (long i = 0; i < G; i++) q[i] = 0.0;
     for (long i = 0; i < np; i++) {
         const double chg = charges[i], px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-        for (int c = 0; c < 8; c++) {
-            const int64_t *t = nb + (i * 8 + c) * 3;
-            double app = or_cic_weight(px - t[0] * h, L, h) * or_cic_weight(py - t[1] * h, L, h) *
-                         or_cic_weight(pz - t[2] * h, L, h);
+        for (int c = 0; c < nn; c++) {
+            const int64_t *t = nb + (i * nn + c) * 3;
+            double app = cas_weight(cas, px - t[0] * h, L, h) *
+                         cas_weight(cas, py - t[1] * h, L, h) *
+                         cas_weight(cas, pz - t[2] * h, L, h);
             double upd = chg * app;
             q_tot += upd;
             q[t[0] * N2 + t[1] * N + t[2]] += upd;
@@ -620,22 +665,28 @@ double or_spread_cic(int n, int np, double h, const double *pos, const int64_t *
     return q_tot;
 }
 
-/* Field force by CIC-weighted central differences at the 8 corners;
+double or_spread_cic(int n, int np, double h, const double *pos, const int64_t *nb,
+                     const double *charges, double *q) {
+    return or_spread(0, n, np, h, pos, nb, charges, q);
+}
+
+/* Field force by weighted central differences at the assignment nodes;
  * returns the sum of the weighted charges.  library/src/forces.c:26-100. */
-double or_force_fd(int n, int np, double h, const double *phi, const int64_t *nb,
-                   const double *charges, const double *pos, double *forces) {
+double or_force_fd_cas(int cas, int n, int np, double h, const double *phi, const int64_t *nb,
+                       const double *charges, const double *pos, double *forces) {
     const long N = n, N2 = N * N;
     const double h2 = 2.0 * h, L = N * h;
+    const int nn = or_cas_nodes(cas);
     double sum_q = 0.0;
     for (long ip = 0; ip < np; ip++) {
         double *f = forces + 3 * ip;
         const double px = pos[3 * ip], py = pos[3 * ip + 1], pz = pos[3 * ip + 2], chg = charges[ip];
         f[0] = f[1] = f[2] = 0.0;
-        for (int c = 0; c < 8; c++) {
-            const int64_t *t = nb + (ip * 8 + c) * 3;
+        for (int c = 0; c < nn; c++) {
+            const int64_t *t = nb + (ip * nn + c) * 3;
             const long i = t[0], j = t[1], k = t[2];
-            double qc = chg * or_cic_weight(px - i * h, L, h) * or_cic_weight(py - j * h, L, h) *
-                        or_cic_weight(pz - k * h, L, h);
+            double qc = chg * cas_weight(cas, px - i * h, L, h) * cas_weight(cas, py - j * h, L, h) *
+                        cas_weight(cas, pz - k * h, L, h);
             sum_q += qc;
             double E = (phi[wrapi(i - 1, N) * N2 + j * N + k] - phi[wrapi(i + 1, N) * N2 + j * N + k]) / h2;
             f[0] += qc * E;
@@ -646,6 +697,11 @@ double or_force_fd(int n, int np, double h, const double *phi, const int64_t *nb
         }
     }
     return sum_q;
+}
+
+double or_force_fd(int n, int np, double h, const double *phi, const int64_t *nb,
+                   const double *charges, const double *pos, double *forces) {
+    return or_force_fd_cas(0, n, np, h, phi, nb, charges, pos, forces);
 }
 
 /* Shifted-force Tosi-Fumi constants per type pair: table[(t1*n_typ+t2)*7 +

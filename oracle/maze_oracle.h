@@ -73,6 +73,14 @@ int or_mg_pb_update(double tol, const double *b, double *phi, int n, const doubl
 
 /* ---- particles --------------------------------------------------------- */
 double or_cic_weight(double x, double L, double h);
+double or_spline_quadr_weight(double x, double L, double h);
+double or_spline_cubic_weight(double x, double L, double h);
+int or_cas_nodes(int cas);
+void or_neighbors(int cas, const double *pos, int np, int n, double h, int64_t *nb);
+double or_spread(int cas, int n, int np, double h, const double *pos, const int64_t *nb,
+                 const double *charges, double *q);
+double or_force_fd_cas(int cas, int n, int np, double h, const double *phi, const int64_t *nb,
+                       const double *charges, const double *pos, double *forces);
 void or_cic_neighbors(const double *pos, int np, int n, double h, int64_t *nb);
 double or_spread_cic(int n, int np, double h, const double *pos,
                      const int64_t *nb, const double *charges, double *q);

@@ -13,7 +13,7 @@
  *  - OVRVO noise comes from or_normal3 (Philox) instead of rand();
  *  - the Tosi-Fumi constants are held per type pair, not per particle pair
  *    (the values are identical: they only depend on the two types);
- *  - only TF pair potentials and CIC assignment are provided.
+ *  - only TF pair potentials are provided (CIC and both spline assignments are).
  */
 #include <math.h>
 #include <stdint.h>
@@ -54,6 +54,7 @@ static struct {
     int *types;
     double *pos, *vel, *mass, *charges, *fe, *fn, *ft, *radii;
     int64_t *nb;
+    int cas, nn;  /* charge-assignment scheme and its nodes per ion */
     double *table;
 } p;
 
@@ -133,8 +134,10 @@ void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, 
                                  int cas_type, int *types, double *pos, double *vel, double *mass,
                                  double *charges, double *pot_params) {
     if (pot_type != 0) die("oracle: only the TF potential is provided");
-    if (cas_type != 0) die("oracle: only CIC charge assignment is provided");
+    if (cas_type < 0 || cas_type > 2) die("Invalid charge assignment scheme type");
     memset(&p, 0, sizeof(p));
+    p.cas = cas_type;
+    p.nn = or_cas_nodes(cas_type);
     p.n = n; p.np = n_p; p.n_typ = n_typ; p.L = L; p.h = h;
     p.types = (int *)malloc(n_p * sizeof(int));
     p.pos = (double *)malloc(n_p * 3 * sizeof(double));
@@ -144,7 +147,7 @@ void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, 
     p.fe = (double *)calloc(n_p * 3, sizeof(double));
     p.fn = (double *)calloc(n_p * 3, sizeof(double));
     p.ft = (double *)calloc(n_p * 3, sizeof(double));
-    p.nb = (int64_t *)malloc((size_t)n_p * 24 * sizeof(int64_t));
+    p.nb = (int64_t *)malloc((size_t)n_p * p.nn * 3 * sizeof(int64_t));
     memcpy(p.types, types, n_p * sizeof(int));
     memcpy(p.pos, pos, n_p * 3 * sizeof(double));
     memcpy(p.vel, vel, n_p * 3 * sizeof(double));
@@ -183,8 +186,8 @@ void solver_set_rng_seed(uint64_t seed) { it.seed = seed; it.calls = 0; }
 
 /* library/src/solver_md.c:91-107 */
 int solver_update_charges(void) {
-    or_cic_neighbors(p.pos, p.np, p.n, p.h, p.nb);
-    double loc = or_spread_cic(g.n, p.np, g.h, p.pos, p.nb, p.charges, g.q);
+    or_neighbors(p.cas, p.pos, p.np, p.n, p.h, p.nb);
+    double loc = or_spread(p.cas, g.n, p.np, g.h, p.pos, p.nb, p.charges, g.q);
     if (fabs(q_tot - loc) > 1e-6) {
         printf("Charge conservation error: q_tot = %.6f, q_tot_loc = %.6f\n", q_tot, loc);
         exit(1);
@@ -283,7 +286,7 @@ int solver_update_field(void) {
 }
 
 void solver_compute_forces_elec(void) {
-    or_force_fd(p.n, p.np, p.h, g.phi_n, p.nb, p.charges, p.pos, p.fe);
+    or_force_fd_cas(p.cas, p.n, p.np, p.h, g.phi_n, p.nb, p.charges, p.pos, p.fe);
 }
 
 double solver_compute_forces_noel(void) {
@@ -425,7 +428,7 @@ void get_field(double *o) { memcpy(o, g.phi_n, g.G * sizeof(double)); }
 void get_field_prev(double *o) { memcpy(o, g.phi_p, g.G * sizeof(double)); }
 void get_q(double *o) { memcpy(o, g.q, g.G * sizeof(double)); }
 void get_y(double *o) { memcpy(o, g.y, g.G * sizeof(double)); }
-void get_neighbors(int64_t *o) { memcpy(o, p.nb, (size_t)p.np * 24 * sizeof(int64_t)); }
+void get_neighbors(int64_t *o) { memcpy(o, p.nb, (size_t)p.np * p.nn * 3 * sizeof(int64_t)); }
 void get_tf_neighbor_counts(int *o) { or_tf_neighbor_counts(p.np, p.L, p.pos, p.r_cut, o); }
 
 /* library/src/grid_general.c:304-316 (half the sum of phi, no charge) */

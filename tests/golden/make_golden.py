@@ -237,6 +237,10 @@ def main():
         "ovrvo_nve_216_32": dict(n_ions=216, n_grid=32, seed=3, method="MAZE-LCG", steps=3,
                                  integrator="OVRVO"),
         "maze_lcg_512_64": dict(n_ions=512, n_grid=64, seed=1, method="MAZE-LCG", steps=1),
+        "maze_lcg_quadr_216_32": dict(n_ions=216, n_grid=32, seed=3, method="MAZE-LCG", steps=3,
+                                      cas="SPL_QUADR"),
+        "maze_lcg_cubic_216_32": dict(n_ions=216, n_grid=32, seed=3, method="MAZE-LCG", steps=3,
+                                      cas="SPL_CUBIC"),
     }
     meta = {}
     for name, kw in cases.items():

@@ -136,7 +136,7 @@ def run(lib, s, cfg, steps, precision=None):
         its.append(it)
         vs.append(v)
     snap = driver.snapshot(lib, s)
-    nb = np.zeros((s.n_p, 8, 3), dtype=np.int64)
+    nb = np.zeros((s.n_p, 8 if cfg.cas.upper() == "CIC" else 64, 3), dtype=np.int64)
     lib.get_neighbors(nb)
     cnt = np.zeros(s.n_p, dtype=np.int32)
     lib.get_tf_neighbor_counts(cnt)
@@ -195,6 +195,23 @@ def test_md_sequence_pb(hip, oracle, method):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
     assert rel(g["pos"], o["pos"]) <= 1e-11
     assert rel(v_g, v_o) <= 1e-12
+
+
+@pytest.mark.parametrize("cas", ["SPL_QUADR", "SPL_CUBIC"])
+def test_md_sequence_spline(hip, oracle, cas):
+    """Quadratic / cubic B-spline charge assignment (64 nodes per ion) through MD steps:
+    node tables exact, charge grid to the last bits (the cubic weight's x^3 is rounded
+    once on both sides, but the CPU pow may differ from it by one unit in rare cases)."""
+    s = W.nacl_system(216, 32, seed=3)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s), cas=cas)
+    its_g, v_g, g = run(hip, s, cfg, 3)
+    its_o, v_o, o = run(oracle, s, cfg, 3)
+    assert all(abs(a - b) <= 1 for a, b in zip(its_g, its_o)), (its_g, its_o)
+    assert np.array_equal(g["neighbors"], o["neighbors"])
+    assert rel(g["q"], o["q"]) <= 1e-14
+    for k in ("fcs_elec", "fcs_tot", "field"):
+        assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+    assert rel(g["pos"], o["pos"]) <= 1e-11
 
 
 def test_cic_bitexact_at_64(hip, oracle):
@@ -318,20 +335,40 @@ def _exact_periodic(b):
     return np.fft.ifftn(eh).real
 
 
+def chebyshev_weights(lo=1.0 / 3.0):
+    """The library's default smoother weights: reciprocal degree-2 Chebyshev nodes of the
+    Jacobi-scaled spectrum [lo, 2]."""
+    mid, half = (2 + lo) / 2, (2 - lo) / 2 * np.cos(np.pi / 4)
+    return 1 / (mid + half), 1 / (mid - half)
+
+
+def test_chebyshev_smoother_damps_upper_spectrum():
+    """Two sweeps with the default weights damp every Jacobi-scaled mode mu in [1/3, 2]
+    by at least 1/T2(7/5) (about 0.34) and never amplify one in [0, 2]."""
+    w1, w2 = chebyshev_weights()
+    mu = np.linspace(0, 2, 2001)
+    p = (1 - w1 * mu) * (1 - w2 * mu)
+    assert np.abs(p).max() <= 1.0 + 1e-12
+    t2 = 2 * (7 / 5) ** 2 - 1
+    assert np.abs(p[mu >= 1 / 3]).max() <= 1 / t2 + 1e-9
+
+
 @pytest.mark.parametrize("n", [32, 128])
 def test_mg_tail(hip_raw, n):
-    """The one-work-group tail (16^3 -> exact 8^3 -> 16^3) vs the same cycle in FP64."""
+    """The one-work-group tail (16^3 -> exact 8^3 -> 16^3) vs the same cycle in FP64
+    (pre-smoothing weights c1 then c2, post-smoothing c2 then c1)."""
     m = 16
-    c = (6.0 / 7.0) / 6.0
+    w1, w2 = chebyshev_weights()
+    c1, c2 = w1 / 6.0, w2 / 6.0
     b = np.random.default_rng(n).standard_normal((m, m, m)).astype(np.float32)
     d = np.zeros_like(b)
     assert hip_raw.maze_test_mg_tail(b, d, n) == m
     b64 = b.astype(np.float64)
-    d2 = -c * b64
-    d2 = d2 - c * (b64 - _lap(d2))
+    d2 = -c1 * b64
+    d2 = d2 - c2 * (b64 - _lap(d2))
     e = _exact_periodic(_restrict_fw(b64 - _lap(d2)))
     x = d2 + _prolong_trilinear(e)
-    for _ in range(2):
+    for c in (c2, c1):
         x = x - c * (b64 - _lap(x))
     assert np.abs(d - x).max() <= 1e-5 * np.abs(x).max()
 

@@ -343,3 +343,24 @@ def test_workload_geometry():
     assert (s.types == 0).sum() == 1024 and s.charges.sum() == 0
     assert np.abs((s.mass[:, None] * s.vel).sum(0)).max() < 1e-9
     assert s.pos.min() >= 0 and s.pos.max() < s.L
+
+
+@pytest.mark.parametrize("which", ["or_spline_quadr_weight", "or_spline_cubic_weight"])
+def test_spline_weights_partition_of_unity(oracle_raw, which):
+    """B-spline assignment weights: over the 4 nodes base-1..base+2 they sum to 1 and
+    reproduce the position (first moment), are even, and take the textbook values at
+    the node (3/4 quadratic, 2/3 cubic)."""
+    import ctypes
+    f = getattr(oracle_raw, which)
+    f.restype, f.argtypes = ctypes.c_double, [ctypes.c_double] * 3
+    L, h = 10.0, 0.5
+    g = np.random.default_rng(9)
+    for x in g.uniform(0, L, 50):
+        base = np.floor(x / h)
+        nodes = (base + np.arange(-1, 3)) * h
+        w = np.array([f(x - xn, L, h) for xn in nodes])
+        assert abs(w.sum() - 1.0) <= 1e-14
+        assert abs((w * nodes).sum() - x) <= 1e-12
+        assert abs(f(x - nodes[1], L, h) - f(nodes[1] - x, L, h)) <= 1e-16
+    assert f(0.0, L, h) == (0.75 if "quadr" in which else 2.0 / 3.0)
+    assert f(2.1 * h, L, h) == 0.0
