@@ -59,20 +59,26 @@ void solver_initialize_grid(int n, double L, double h, double tol, double eps_s,
                             int grid_type, int precond_type);
 
 /* library/src/solver_md.c:46-49 -> grid_general.c:85-100.  Enables the
- * linear Poisson-Boltzmann operator; dielectric maps start at 1 and the
- * screening map at 0 until solver_set_eps_k2 installs them. */
+ * linear Poisson-Boltzmann operator with transition width w (Bohr) and grid
+ * screening kbar2; nonpolar_enabled switches on the non-polar boundary
+ * force.  Dielectric maps start at 1 and the screening map at 0 until
+ * solver_update_eps_k2 (with radii) or solver_set_eps_k2 sets them. */
 void solver_initialize_grid_pois_boltz(double w, double kbar2, int nonpolar_enabled);
 
 /* library/src/solver_md.c:51-65 -> particles.c:68-109, :181-228.
  * types int32[n_p]; pos, vel double[n_p][3]; mass, charges double[n_p];
- * pot_params double[n_typ][n_typ][5] = (A, B, C, D, sigma).  Only TF and CIC
- * are provided (others exit(1)); 1 <= n_typ <= 16. */
+ * pot_params double[n_typ][n_typ][5] = (A, B, C, D, sigma).  TF with CIC,
+ * SPL_QUADR or SPL_CUBIC assignment (other potentials exit(1));
+ * 1 <= n_typ <= 16. */
 void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, int pot_type,
                                  int cas_type, int *types, double *pos, double *vel, double *mass,
                                  double *charges, double *pot_params);
 
-/* library/src/solver_md.c:67-69 (accepted; radius-based PB forces are not
- * provided). */
+/* library/src/solver_md.c:67-69 -> particles.c:111-125.  Per-ion PB radius
+ * radii double[n_p] (solvation + probe radius, Bohr), the non-polar surface
+ * tension gamma_np and offset beta_np (a.u.).  Needs
+ * solver_initialize_grid_pois_boltz first; exits(1) when 2 (floor((r + w) /
+ * h) + 2) + 1 exceeds n (the reference would then visit points twice). */
 void solver_initialize_particles_pois_boltz(double gamma_np, double beta_np, double *radii);
 
 /* library/src/solver_md.c:71-89, integrator_ovrvo.c:140-148. */
@@ -97,13 +103,18 @@ void solver_set_field_prev(double *phi);
  * inner iterations. */
 int solver_update_field(void);
 
-/* library/src/solver_md.c:135-138 (no-op: the maps are installed with
- * solver_set_eps_k2). */
+/* library/src/solver_md.c:135-138 -> grid_general.c:138-301: rebuilds the
+ * dielectric and screening maps from the ion positions and radii (exits(1)
+ * if more than 2048 ions can reach one 8^3 grid tile).  Without radii it
+ * keeps the maps of solver_set_eps_k2. */
 void solver_update_eps_k2(void);
 
 /* library/src/solver_md.c:140-154 -> forces.c:26-100, :112-186,
- * particles.c:777-795.  solver_compute_forces_noel returns the Tosi-Fumi
- * potential energy; solver_compute_forces_pb returns 0. */
+ * particles.c:420-795.  solver_compute_forces_noel returns the Tosi-Fumi
+ * potential energy; solver_compute_forces_pb computes the dielectric-
+ * boundary, ionic-boundary and non-polar forces (needs the radii) and
+ * returns the non-polar energy, 0 as in the reference; the total adds them
+ * once the radii are set. */
 void solver_compute_forces_elec(void);
 double solver_compute_forces_noel(void);
 double solver_compute_forces_pb(void);
@@ -166,6 +177,11 @@ void solver_set_rng_seed(uint64_t seed);
  * PB operator (each double[n][n][n]); needs
  * solver_initialize_grid_pois_boltz first. */
 void solver_set_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
+
+/* Copies the current dielectric and screening maps out (each
+ * double[n][n][n]); no counterpart in the reference, where the maps are
+ * internal. */
+void get_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
 
 /* Field-solve arithmetic: 0 = FP64 CG as in the reference (default);
  * 1 = FP32-storage CG inside FP64 iterative refinement, stopping on the

@@ -15,7 +15,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libmaze_poisson.so")
-SOURCES = ["grid_kernels.hip", "mg_kernels.hip", "particle_kernels.hip", "sim.hip", "abi.hip"]
+SOURCES = ["grid_kernels.hip", "mg_kernels.hip", "particle_kernels.hip", "pb_kernels.hip", "sim.hip",
+           "abi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

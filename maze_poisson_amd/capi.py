@@ -92,6 +92,7 @@ EXTENSION_ABI = [
     ("solver_run_n_steps", None, [_c_int]),
     ("solver_set_rng_seed", None, [ctypes.c_uint64]),
     ("solver_set_eps_k2", None, [_D3, _D3, _D3, _D3]),
+    ("get_eps_k2", None, [_D3, _D3, _D3, _D3]),
     ("get_y", None, [_D3]),
     ("get_neighbors", None, [_L3]),
     ("get_tf_neighbor_counts", None, [_I1]),

@@ -76,8 +76,7 @@ void solver_initialize_grid(int n, double L, double h, double tol, double eps, d
 }
 
 void solver_initialize_grid_pois_boltz(double w, double kbar2, int nonpolar_enabled) {
-    (void)nonpolar_enabled;
-    S().init_pb(w, kbar2);
+    S().init_pb(w, kbar2, nonpolar_enabled);
 }
 
 void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, int pot_type,
@@ -91,7 +90,7 @@ void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, 
 }
 
 void solver_initialize_particles_pois_boltz(double gamma_np, double beta_np, double* radii) {
-    (void)gamma_np; (void)beta_np; (void)radii;
+    S().init_pb_particles(gamma_np, beta_np, radii);
 }
 
 void solver_initialize_integrator(int n_p, double dt, double T, double gamma, int itg_type,
@@ -103,11 +102,15 @@ void solver_initialize_integrator(int n_p, double dt, double T, double gamma, in
 // ---- step ------------------------------------------------------------------------
 int solver_update_charges() { return S().update_charges(); }
 void solver_init_field() { S().init_field(); }
-int solver_update_field() { return S().update_field(); }
-void solver_update_eps_k2() {}
+int solver_update_field() {
+    const int r = S().update_field();
+    S().check_pb_flag();
+    return r;
+}
+void solver_update_eps_k2() { S().update_eps_k2(); }
 void solver_compute_forces_elec() { S().forces_elec(); }
 double solver_compute_forces_noel() { return S().forces_noel(true); }
-double solver_compute_forces_pb() { return 0.0; }
+double solver_compute_forces_pb() { return S().forces_pb(); }
 void solver_compute_forces_tot() { S().forces_tot(); }
 void integrator_part_1() { S().part1(); }
 void integrator_part_2() { S().part2(); }
@@ -130,13 +133,21 @@ void get_vel(double* o) { d2h(o, S().p.vel, 3L * S().p.np * sizeof(double)); }
 void get_fcs_elec(double* o) { d2h(o, S().p.fe, 3L * S().p.np * sizeof(double)); }
 void get_fcs_noel(double* o) { d2h(o, S().p.fn, 3L * S().p.np * sizeof(double)); }
 void get_fcs_tot(double* o) { d2h(o, S().p.ft, 3L * S().p.np * sizeof(double)); }
-void get_fcs_db(double* o) { std::memset(o, 0, 3L * S().p.np * sizeof(double)); }
-void get_fcs_ib(double* o) { std::memset(o, 0, 3L * S().p.np * sizeof(double)); }
-void get_fcs_np(double* o) { std::memset(o, 0, 3L * S().p.np * sizeof(double)); }
+// PB boundary forces: zeros until the radii are given (library/src/solver_md.c:263-285).
+static void get_pb_force(double* o, const double* src) {
+    if (S().pbd.active) d2h(o, src, 3L * S().p.np * sizeof(double));
+    else std::memset(o, 0, 3L * S().p.np * sizeof(double));
+}
+void get_fcs_db(double* o) { get_pb_force(o, S().pbd.fdb); }
+void get_fcs_ib(double* o) { get_pb_force(o, S().pbd.fib); }
+void get_fcs_np(double* o) { get_pb_force(o, S().pbd.fnp); }
 void get_types(int* o) { d2h(o, S().p.types, S().p.np * sizeof(int)); }
 void get_charges(double* o) { d2h(o, S().p.charges, S().p.np * sizeof(double)); }
 void get_masses(double* o) { d2h(o, S().p.mass, S().p.np * sizeof(double)); }
-void get_radii(double* o) { std::memset(o, 0, S().p.np * sizeof(double)); }
+void get_radii(double* o) {
+    if (S().pbd.active) d2h(o, S().pbd.radii, S().p.np * sizeof(double));
+    else std::memset(o, 0, S().p.np * sizeof(double));
+}
 void get_field(double* o) { d2h(o, S().phi, S().G * sizeof(double)); }
 void get_field_prev(double* o) { d2h(o, S().phip, S().G * sizeof(double)); }
 void get_q(double* o) { d2h(o, S().q, S().G * sizeof(double)); }
@@ -163,6 +174,18 @@ void solver_set_rng_seed(uint64_t seed) {
 
 void solver_set_eps_k2(double* ex, double* ey, double* ez, double* k2) {
     S().set_eps_k2(ex, ey, ez, k2);
+}
+
+void get_eps_k2(double* ex, double* ey, double* ez, double* k2) {
+    Sim& s = S();
+    if (!s.pb) {
+        std::fprintf(stderr, "maze: get_eps_k2 needs solver_initialize_grid_pois_boltz first\n");
+        std::exit(1);
+    }
+    d2h(ex, s.ex, s.G * sizeof(double));
+    d2h(ey, s.ey, s.G * sizeof(double));
+    d2h(ez, s.ez, s.G * sizeof(double));
+    d2h(k2, s.k2, s.G * sizeof(double));
 }
 
 int solver_set_precision(int mode) {

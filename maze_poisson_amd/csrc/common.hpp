@@ -78,6 +78,25 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
+// x^3 rounded once (the reference's pow(x, 3)): x*x split exactly into
+// hi + lo by an FMA, then hi*x + lo*x with one final rounding.
+__device__ __forceinline__ double cube(double x) {
+    const double hi = x * x;
+    const double lo = fma(x, x, -hi);
+    return fma(hi, x, lo * x);
+}
+
+// First index in the ascending a[0..n) whose value is >= key.
+__device__ __forceinline__ int lower_bound_u32(const unsigned int* a, int n, unsigned int key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Neighbour offsets of a point inside its i-plane (periodic in j and k).
 struct PlaneNb {
     int jm, jp, km, kp;

@@ -68,23 +68,7 @@ void launch_cic_sort(hipStream_t s, const PDev& p, int key_bits) {
                                                 p.ids_sorted, p.np, 0, key_bits, s));
 }
 
-__device__ __forceinline__ int lower_bound_u32(const unsigned int* a, int n, unsigned int key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
 
-// x^3 rounded once (the reference's pow(x, 3)): x*x split exactly into
-// hi + lo by an FMA, then hi*x + lo*x with one final rounding.
-__device__ __forceinline__ double cube(double x) {
-    const double hi = x * x;
-    const double lo = fma(x, x, -hi);
-    return fma(hi, x, lo * x);
-}
 
 // Quadratic and cubic B-spline weights (library/src/charges.c:16-38).
 __device__ __forceinline__ double spline_quadr_w(double x, double L, double h) {
@@ -453,20 +437,33 @@ void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
     launch_sum_scaled(s, p.np, 0.5, p.eion, &ps->pe_tf, red);
 }
 
-// F_tot = 0 + 1 F_elec + 1 F_noel (library/src/particles.c:777-795).
-__global__ void k_ftot(long n, const double* fe, const double* fn, double* ft) {
+// F_tot = 0 + 1 F_elec + 1 F_noel [+ 1 F_db + 1 F_ib + 1 F_np]
+// (library/src/particles.c:777-795).
+template <bool PB>
+__global__ void k_ftot(long n, const double* fe, const double* fn, const double* fdb,
+                       const double* fib, const double* fnp, double* ft) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     double v = 0.0;
     v += 1.0 * fe[i];
     v += 1.0 * fn[i];
+    if (PB) {
+        v += 1.0 * fdb[i];
+        v += 1.0 * fib[i];
+        v += 1.0 * fnp[i];
+    }
     ft[i] = v;
 }
 
-void launch_force_total(hipStream_t s, const PDev& p) {
+void launch_force_total(hipStream_t s, const PDev& p, const double* fdb, const double* fib,
+                        const double* fnp) {
     if (p.np == 0) return;
-    MAZE_LAUNCH(k_ftot, dim3(blocks_for(3L * p.np)), dim3(kBlock), 0, s, 3L * p.np, p.fe,
-                       p.fn, p.ft);
+    if (fdb)
+        MAZE_LAUNCH(k_ftot<true>, dim3(blocks_for(3L * p.np)), dim3(kBlock), 0, s, 3L * p.np,
+                    p.fe, p.fn, fdb, fib, fnp, p.ft);
+    else
+        MAZE_LAUNCH(k_ftot<false>, dim3(blocks_for(3L * p.np)), dim3(kBlock), 0, s, 3L * p.np,
+                    p.fe, p.fn, fdb, fib, fnp, p.ft);
 }
 
 // K = 1/2 sum m |v|^2 (library/src/particles.c:802-820); optionally the

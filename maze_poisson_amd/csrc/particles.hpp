@@ -40,6 +40,7 @@ struct PScalars {
     double scale;      // velocity-rescale factor of the Verlet thermostat
     double mom[3];
     double typesum[3 * 16];
+    int pb_flag;       // set by the PB map kernel when a tile overflows
 };
 
 size_t sort_tmp_bytes(int np, int key_bits);
@@ -50,7 +51,9 @@ void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi);
 long tf_part_doubles(int np);
 void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
-void launch_force_total(hipStream_t s, const PDev& p);
+// F_tot = F_elec + F_noel (+ the PB boundary forces fdb, fib, fnp when given).
+void launch_force_total(hipStream_t s, const PDev& p, const double* fdb = nullptr,
+                        const double* fib = nullptr, const double* fnp = nullptr);
 void launch_kinetic(hipStream_t s, const PDev& p, PScalars* ps, Reduce red, double T_target,
                     int want_scale);
 void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale);

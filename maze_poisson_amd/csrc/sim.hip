@@ -115,6 +115,17 @@ void Sim::free_particles() {
     if (p.sort_tmp) MAZE_HIP(hipFree(p.sort_tmp));
     p.sort_tmp = nullptr;
     p.np = 0;
+    free_pb();
+}
+
+void Sim::free_pb() {
+    for (double** a : {&pbd.radii, &pbd.rp2, &pbd.rm2, &pbd.fdb, &pbd.fib, &pbd.fnp}) dfree(*a);
+    for (int** a : {&pbd.rmap, &pbd.rfrc, &pbd.ids, &pbd.ids_sorted}) dfree(*a);
+    dfree(pbd.keys);
+    dfree(pbd.keys_sorted);
+    if (pbd.sort_tmp) MAZE_HIP(hipFree(pbd.sort_tmp));
+    pbd = PBDev{};
+    pb_flag_pending = 0;
 }
 
 PBCoef Sim::pbc() const {
@@ -155,8 +166,8 @@ void Sim::init_grid(int n_, double L_, double h_, double tol_, double eps, doubl
 }
 
 // library/src/grid_general.c:85-100 (maps default to vacuum until set).
-void Sim::init_pb(double w_, double kbar2_) {
-    pb = 1; w = w_; kbar2 = kbar2_;
+void Sim::init_pb(double w_, double kbar2_, int nonpolar_) {
+    pb = 1; w = w_; kbar2 = kbar2_; nonpolar = nonpolar_;
     ex = dalloc<double>(G); ey = dalloc<double>(G); ez = dalloc<double>(G); k2 = dalloc<double>(G);
     launch_fill(stream, G, 1.0, ex);
     launch_fill(stream, G, 1.0, ey);
@@ -229,6 +240,86 @@ void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, int ca
     p.sort_tmp_bytes = sort_tmp_bytes(std::max(np, 1), key_bits);
     MAZE_HIP(hipMalloc(&p.sort_tmp, std::max<size_t>(p.sort_tmp_bytes, 16)));
     sync();
+}
+
+// library/src/particles.c:111-125: the ion radii (solvation + probe, Bohr)
+// of the radius-based PB boundary, with the per-ion constants of the map and
+// force kernels: (r + w)^2 and (r - w)^2 (as the reference evaluates them)
+// and the half-widths of the two boxes.
+void Sim::init_pb_particles(double gamma_np, double beta_np, const double* radii) {
+    if (!pb) fatal("maze: solver_initialize_grid_pois_boltz must come before the PB radii");
+    free_pb();
+    const int np = p.np;
+    pbd.np = np; pbd.n = n; pbd.h = h; pbd.w = w;
+    pbd.eps_s = eps_s; pbd.eps_int = eps_int; pbd.kbar2 = kbar2;
+    pbd.gamma_np = gamma_np; pbd.beta_np = beta_np; pbd.nonpolar = nonpolar;
+    std::vector<double> rp2(np), rm2(np);
+    std::vector<int> rmap(np), rfrc(np);
+    int rmax = 0;
+    for (int i = 0; i < np; i++) {
+        const double r = radii[i];
+        rp2[i] = std::pow(r + w, 2);
+        rm2[i] = std::pow(r - w, 2);
+        rmap[i] = (int)std::floor((r + w) / h) + 1;
+        rfrc[i] = (int)std::floor((r + w) / h) + 2;
+        if (!(r >= 0.0) || 2 * rfrc[i] + 1 > n)
+            fatal("maze: PB radius + transition width must be >= 0 and span less than the grid");
+        rmax = std::max(rmax, rmap[i]);
+    }
+    pbd.rmax = rmax;
+    pbd.nb = (n + kPbTile - 1) / kPbTile;
+    const unsigned long bins = (unsigned long)pbd.nb * pbd.nb * pbd.nb;
+    pbd.key_bits = 1;
+    while ((1ul << pbd.key_bits) < bins) pbd.key_bits++;
+    pbd.radii = dalloc<double>(np); pbd.rp2 = dalloc<double>(np); pbd.rm2 = dalloc<double>(np);
+    pbd.rmap = dalloc<int>(np); pbd.rfrc = dalloc<int>(np);
+    pbd.keys = dalloc<unsigned int>(np); pbd.keys_sorted = dalloc<unsigned int>(np);
+    pbd.ids = dalloc<int>(np); pbd.ids_sorted = dalloc<int>(np);
+    pbd.fdb = dalloc<double>(3L * np); pbd.fib = dalloc<double>(3L * np);
+    pbd.fnp = dalloc<double>(3L * np);
+    pbd.sort_bytes = pb_sort_bytes(std::max(np, 1), pbd.key_bits);
+    MAZE_HIP(hipMalloc(&pbd.sort_tmp, std::max<size_t>(pbd.sort_bytes, 16)));
+    pbd.flag = &ps->pb_flag;
+    if (np > 0) {
+        MAZE_HIP(hipMemcpy(pbd.radii, radii, np * sizeof(double), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(pbd.rp2, rp2.data(), np * sizeof(double), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(pbd.rm2, rm2.data(), np * sizeof(double), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(pbd.rmap, rmap.data(), np * sizeof(int), hipMemcpyHostToDevice));
+        MAZE_HIP(hipMemcpy(pbd.rfrc, rfrc.data(), np * sizeof(int), hipMemcpyHostToDevice));
+    }
+    pbd.active = 1;
+    sync();
+}
+
+// library/src/grid_general.c:138-301 (the maps of the current positions);
+// without radii the maps set by set_eps_k2 are kept.
+void Sim::update_eps_k2() {
+    if (!pb || !pbd.active) return;
+    launch_pb_bin(stream, pbd, p.pos);
+    launch_pb_maps(stream, pbd, p.pos, ex, ey, ez, k2);
+    restrict_pb_coef();
+    MAZE_HIP(hipMemcpyAsync(&ps_host->pb_flag, &ps->pb_flag, sizeof(int), hipMemcpyDeviceToHost,
+                            stream));
+    pb_flag_pending = 1;
+}
+
+void Sim::check_pb_flag() {
+    if (!pb_flag_pending) return;
+    sync();
+    pb_flag_pending = 0;
+    if (ps_host->pb_flag)
+        fatal("maze: more than 2048 ions within reach of one 8^3 tile of the PB maps");
+}
+
+// library/src/particles.c:420-775; returns the non-polar energy, which the
+// reference leaves at 0.
+double Sim::forces_pb() {
+    if (!pb) fatal("Poisson-Boltzmann forces are not enabled in the grid.");
+    if (!pbd.active)
+        fatal("maze: PB boundary forces need the ion radii (solver_initialize_particles_pois_boltz)");
+    check_pb_flag();
+    launch_pb_forces(stream, pbd, p.pos, phi, ex, ey, ez, k2);
+    return 0.0;
 }
 
 // library/src/solver_md.c:71-89, integrator_ovrvo.c:140-148.
@@ -771,7 +862,10 @@ double Sim::forces_noel(bool read_back) {
     return ps_host->pe_tf;
 }
 
-void Sim::forces_tot() { launch_force_total(stream, p); }
+void Sim::forces_tot() {
+    if (pbd.active) launch_force_total(stream, p, pbd.fdb, pbd.fib, pbd.fnp);
+    else launch_force_total(stream, p);
+}
 
 // library/src/integrator_verlet.c:12-71, integrator_ovrvo.c:45-138.
 void Sim::part1() {
@@ -798,11 +892,14 @@ void Sim::part2() {
 void Sim::md_step() {
     part1();
     update_charges(true);
+    update_eps_k2();
     update_field();
     sync();
     check_charge();
+    check_pb_flag();
     forces_elec();
     forces_noel(false);
+    if (pbd.active) forces_pb();
     forces_tot();
     part2();
 }

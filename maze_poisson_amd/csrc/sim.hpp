@@ -5,6 +5,7 @@
 
 #include "grid.hpp"
 #include "particles.hpp"
+#include "pb.hpp"
 
 namespace maze {
 
@@ -51,6 +52,7 @@ struct Sim {
     // refinement passes of about three cycles each reach the FP64 stop).
     double inner_rtol = 0.0;
     int pb = 0;
+    int nonpolar = 0;
     double w = 0, kbar2 = 0;
     long G = 0;
     GridDims dims{};
@@ -110,6 +112,9 @@ struct Sim {
     int key_bits = 1;
     double q_tot = 0.0;          // library/src/solver_md.c:16
     int strict_charge_check = 1;
+    // Poisson-Boltzmann boundary from ion radii (maps and boundary forces).
+    PBDev pbd;
+    int pb_flag_pending = 0;     // a map build whose overflow flag is still to be read
 
     // ---- integrator ----
     int itype = kVerlet, enabled = 0;
@@ -153,12 +158,17 @@ struct Sim {
 
     void init_grid(int n, double L, double h, double tol, double eps, double eps_int, int type,
                    int precond);
-    void init_pb(double w, double kbar2);
+    void init_pb(double w, double kbar2, int nonpolar = 0);
     void set_eps_k2(const double* ex, const double* ey, const double* ez, const double* k2);
     void init_particles(int n, int n_typ, double L, double h, int np, int cas, const int* types,
                         const double* pos, const double* vel, const double* mass,
                         const double* charges, const double* pot_params);
     void init_integrator(double dt, double T, double gamma, int type, int enabled);
+    void init_pb_particles(double gamma_np, double beta_np, const double* radii);
+    void update_eps_k2();
+    double forces_pb();
+    void check_pb_flag();
+    void free_pb();
 
     int update_charges(bool defer = false);
     int check_charge();

@@ -827,3 +827,185 @@ void or_normal3(uint64_t seed, uint64_t call, uint32_t ion, double out[3]) {
     out[1] = ra * sin(ta);
     out[2] = rb * cos(tb);
 }
+
+/* ---- Poisson-Boltzmann boundary from ion radii ----------------------------
+ * Each ion carries a solvation radius r (plus the probe radius, added by the
+ * driver) and a transition half-width w.  The smooth step
+ *   H(s) = -(1/(4 w^3)) s^3 + (3/(4 w^2)) s^2,   s = |x - x_ion| - r + w,
+ * rises from 0 at distance r - w to 1 at r + w; inside r - w the solvent is
+ * excluded (factor 0) and beyond r + w the ion leaves the point unchanged.
+ */
+static inline double pb_step(double s, double w2, double w3) {
+    return -(1 / (4 * w3)) * pow(s, 3) + (3 / (4 * w2)) * pow(s, 2);
+}
+
+/* d ln H / ds, library/src/particles.c:407-417 */
+static inline double pb_step_dlog(double s, double w2, double w3) {
+    return (-(3 / (4 * w3)) * pow(s, 2) + (3 / (2 * w2)) * s) /
+           (-(1 / (4 * w3)) * pow(s, 3) + (3 / (4 * w2)) * pow(s, 2));
+}
+
+/* Multiply v by the ion's mask at squared distance r2 (a point outside the
+ * shell keeps v, one inside the core becomes 0). */
+static inline void pb_mask(double *v, double r2, double rp2, double rm2, double r, double w,
+                           double w2, double w3) {
+    if (r2 >= rp2) return;
+    if (r2 > rm2) *v *= pb_step(sqrt(r2) - r + w, w2, w3);
+    else *v = 0.0;
+}
+
+/* library/src/grid_general.c:138-301: the face dielectric eps_x/y/z (face
+ * i+1/2 of point i along each axis) and the screening map k2 (at the point)
+ * are eps_int + (eps_s - eps_int) * prod_ions H and kbar2 * prod_ions H, the
+ * products taken in ion order over each ion's box of +-(floor((r+w)/h)+1)
+ * points around its cell. */
+void or_pb_maps(int n, double h, double w, double eps_s, double eps_int, double kbar2, int np,
+                const double *pos, const double *radii, double *ex, double *ey, double *ez,
+                double *k2) {
+    const long N = n, N2 = N * N, G = N2 * N;
+    const double w2 = w * w, w3 = w2 * w, hd2 = h / 2.0;
+    for (long i = 0; i < G; i++) {
+        ex[i] = eps_s - eps_int;
+        ey[i] = eps_s - eps_int;
+        ez[i] = eps_s - eps_int;
+        k2[i] = kbar2;
+    }
+    for (int a = 0; a < np; a++) {
+        const double r = radii[a];
+        const double px = pos[3 * a], py = pos[3 * a + 1], pz = pos[3 * a + 2];
+        const double rp2 = pow(r + w, 2), rm2 = pow(r - w, 2);
+        const int R = (int)floor((r + w) / h) + 1;
+        const int cx = (int)floor(px / h), cy = (int)floor(py / h), cz = (int)floor(pz / h);
+        for (int di = -R; di <= R; di++) {
+            const int ii = cx + di;
+            const double dx = px - ii * h, dx2 = dx * dx;
+            const long bi = ((ii + n) % n) * N2;
+            for (int dj = -R; dj <= R; dj++) {
+                const int jj = cy + dj;
+                const double dy = py - jj * h, dy2 = dy * dy;
+                const long bj = bi + ((jj + n) % n) * N;
+                for (int dk = -R; dk <= R; dk++) {
+                    const int kk = cz + dk;
+                    const double dz = pz - kk * h, dz2 = dz * dz;
+                    const long c = bj + (kk + n) % n;
+                    double t;
+                    pb_mask(&k2[c], dx2 + dy2 + dz2, rp2, rm2, r, w, w2, w3);
+                    t = dx - hd2;
+                    pb_mask(&ex[c], t * t + dy2 + dz2, rp2, rm2, r, w, w2, w3);
+                    t = dy - hd2;
+                    pb_mask(&ey[c], dx2 + t * t + dz2, rp2, rm2, r, w, w2, w3);
+                    t = dz - hd2;
+                    pb_mask(&ez[c], dx2 + dy2 + t * t, rp2, rm2, r, w, w2, w3);
+                }
+            }
+        }
+    }
+    for (long i = 0; i < G; i++) {
+        ex[i] += eps_int;
+        ey[i] += eps_int;
+        ez[i] += eps_int;
+    }
+}
+
+/* One face term of the boundary forces: the face at offset (fx, fy, fz) from
+ * the ion, coefficient eps_f - eps_int of that face, the potential step
+ * across it and, for the non-polar term, the normalised dielectric gradient
+ * component g; `sgn` is -1 for the face below the point, +1 above. */
+static inline void pb_face(double fx, double fy, double fz, double rp2, double rm2, double r,
+                           double w, double w2, double w3, double eps_f, double eps_int,
+                           double dphi, double phic, double g, int sgn, int nonpolar,
+                           double *db, double *npf) {
+    const double r2 = fx * fx + fy * fy + fz * fz;
+    if (r2 >= rp2 || r2 <= rm2) return;
+    const double r1 = sqrt(r2) + 1E-12;
+    const double ux = fx / r1, uy = fy / r1, uz = fz / r1;
+    const double c = (eps_f - eps_int) * pb_step_dlog(r1 - r + w, w2, w3);
+    db[0] -= c * dphi * phic * ux;
+    db[1] -= c * dphi * phic * uy;
+    db[2] -= c * dphi * phic * uz;
+    if (nonpolar) {
+        if (sgn < 0) {
+            npf[0] -= c * g * ux;
+            npf[1] -= c * g * uy;
+            npf[2] -= c * g * uz;
+        } else {
+            npf[0] += c * g * ux;
+            npf[1] += c * g * uy;
+            npf[2] += c * g * uz;
+        }
+    }
+}
+
+/* library/src/particles.c:420-775: dielectric-boundary (db), ionic-boundary
+ * (ib) and non-polar (np) forces from the field phi and the maps, summed per
+ * ion over its box of +-(floor((r+w)/h)+2) points in (i, j, k) order, then
+ * scaled by h/(8 pi), h/(8 pi) and -gamma_np h/(eps_s - eps_int).  Returns
+ * the non-polar energy, which the reference leaves at 0. */
+double or_pb_forces(int n, double h, double w, double eps_s, double eps_int, int nonpolar,
+                    double gamma_np, int np, const double *pos, const double *radii,
+                    const double *phi, const double *ex, const double *ey, const double *ez,
+                    const double *k2, double *fdb, double *fib, double *fnp) {
+    const long N = n, N2 = N * N;
+    const double w2 = w * w, w3 = w2 * w, hd2 = h / 2.0;
+    for (long i = 0; i < 3L * np; i++) fdb[i] = fib[i] = fnp[i] = 0.0;
+    for (int a = 0; a < np; a++) {
+        const double r = radii[a];
+        const double px = pos[3 * a], py = pos[3 * a + 1], pz = pos[3 * a + 2];
+        const double rp2 = pow(r + w, 2), rm2 = pow(r - w, 2);
+        const int R = (int)floor((r + w) / h) + 2;
+        const int cx = (int)floor(px / h), cy = (int)floor(py / h), cz = (int)floor(pz / h);
+        double *db = fdb + 3 * a, *ib = fib + 3 * a, *nf = fnp + 3 * a;
+        for (int di = -R; di <= R; di++) {
+            const int ii = cx + di;
+            const double dx = px - ii * h, dx2 = dx * dx;
+            const long i0 = ((ii + n) % n) * N2;
+            const long ip = wrapi(i0 / N2 + 1, N) * N2, im = wrapi(i0 / N2 - 1, N) * N2;
+            for (int dj = -R; dj <= R; dj++) {
+                const int jj = cy + dj;
+                const double dy = py - jj * h, dy2 = dy * dy;
+                const long j = (jj + n) % n;
+                const long j0 = j * N, jp = wrapi(j + 1, N) * N, jm = wrapi(j - 1, N) * N;
+                for (int dk = -R; dk <= R; dk++) {
+                    const int kk = cz + dk;
+                    const double dz = pz - kk * h, dz2 = dz * dz;
+                    const long k = (kk + n) % n, kp = wrapi(k + 1, N), km = wrapi(k - 1, N);
+                    const long c = i0 + j0 + k;
+                    const double exc = ex[c], eyc = ey[c], ezc = ez[c];
+                    const double exb = ex[im + j0 + k], eyb = ey[i0 + jm + k], ezb = ez[i0 + j0 + km];
+                    const double gx = exc - exb, gy = eyc - eyb, gz = ezc - ezb;
+                    const double gn = sqrt(gx * gx + gy * gy + gz * gz) / h;
+                    const double ig = (gn > 0.0) ? 1.0 / gn : 0.0;
+                    const double phic = phi[c];
+                    const double r2 = dx2 + dy2 + dz2;
+                    if (r2 < rp2 && r2 > rm2) {
+                        const double r1 = sqrt(r2) + 1E-12;
+                        const double f = k2[c] * phic * phic * pb_step_dlog(r1 - r + w, w2, w3);
+                        ib[0] += f * (dx / r1);
+                        ib[1] += f * (dy / r1);
+                        ib[2] += f * (dz / r1);
+                    }
+                    const double gxn = ig * gx, gyn = ig * gy, gzn = ig * gz;
+                    pb_face(dx + hd2, dy, dz, rp2, rm2, r, w, w2, w3, exb, eps_int,
+                            phi[im + j0 + k] - phic, phic, gxn, -1, nonpolar, db, nf);
+                    pb_face(dx - hd2, dy, dz, rp2, rm2, r, w, w2, w3, exc, eps_int,
+                            phi[ip + j0 + k] - phic, phic, gxn, +1, nonpolar, db, nf);
+                    pb_face(dx, dy + hd2, dz, rp2, rm2, r, w, w2, w3, eyb, eps_int,
+                            phi[i0 + jm + k] - phic, phic, gyn, -1, nonpolar, db, nf);
+                    pb_face(dx, dy - hd2, dz, rp2, rm2, r, w, w2, w3, eyc, eps_int,
+                            phi[i0 + jp + k] - phic, phic, gyn, +1, nonpolar, db, nf);
+                    pb_face(dx, dy, dz + hd2, rp2, rm2, r, w, w2, w3, ezb, eps_int,
+                            phi[i0 + j0 + km] - phic, phic, gzn, -1, nonpolar, db, nf);
+                    pb_face(dx, dy, dz - hd2, rp2, rm2, r, w, w2, w3, ezc, eps_int,
+                            phi[i0 + j0 + kp] - phic, phic, gzn, +1, nonpolar, db, nf);
+                }
+            }
+        }
+    }
+    const double s = h / (8.0 * M_PI), snp = -gamma_np * h / (eps_s - eps_int);
+    for (long i = 0; i < 3L * np; i++) {
+        fdb[i] *= s;
+        fib[i] *= s;
+        fnp[i] *= snp;
+    }
+    return 0.0;
+}

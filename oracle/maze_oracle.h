@@ -11,7 +11,7 @@
  * (left-to-right sums, the same multiply/divide grouping, serial reduction
  * order), so that, compiled with -ffp-contract=off and no fast-math, it
  * reproduces the reference C compiled the same way bit for bit.  That is
- * how it is pinned (tests/test_oracle_vs_ref.py); the remaining checks are
+ * how it is pinned (tests/test_oracle_golden.py); the remaining checks are
  * closed forms (tests/test_oracle_closed_forms.py).
  *
  * Grid arrays are n*n*n doubles, C order [i][j][k] with k fastest, and
@@ -94,6 +94,15 @@ double or_tf_forces(int np, double L, const double *pos, const int *types,
                     double *forces);
 void or_tf_neighbor_counts(int np, double L, const double *pos, double r_cut,
                            int *counts);
+
+/* ---- Poisson-Boltzmann boundary from ion radii -------------------------- */
+void or_pb_maps(int n, double h, double w, double eps_s, double eps_int, double kbar2, int np,
+                const double *pos, const double *radii, double *ex, double *ey, double *ez,
+                double *k2);
+double or_pb_forces(int n, double h, double w, double eps_s, double eps_int, int nonpolar,
+                    double gamma_np, int np, const double *pos, const double *radii,
+                    const double *phi, const double *ex, const double *ey, const double *ez,
+                    const double *k2, double *fdb, double *fib, double *fnp);
 
 /* ---- counter-based normal deviates (Philox4x32-10 + Box-Muller) --------- */
 void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

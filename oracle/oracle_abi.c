@@ -42,7 +42,7 @@ static struct {
     double L, h, tol, eps_s, eps_int;
     long G;
     double *q, *y, *phi_p, *phi_n;
-    int pb;
+    int pb, nonpolar;
     double w, kbar2;
     double *ex, *ey, *ez, *k2;
 } g;
@@ -53,6 +53,8 @@ static struct {
     double L, h, r_cut;
     int *types;
     double *pos, *vel, *mass, *charges, *fe, *fn, *ft, *radii;
+    double *fdb, *fib, *fnp;  /* PB boundary forces (with radii) */
+    double gamma_np, beta_np;
     int64_t *nb;
     int cas, nn;  /* charge-assignment scheme and its nodes per ion */
     double *table;
@@ -109,8 +111,7 @@ void solver_initialize_grid(int n, double L, double h, double tol, double eps, d
 
 /* library/src/grid_general.c:85-100 */
 void solver_initialize_grid_pois_boltz(double w, double kbar2, int nonpolar_enabled) {
-    (void)nonpolar_enabled;
-    g.pb = 1; g.w = w; g.kbar2 = kbar2;
+    g.pb = 1; g.w = w; g.kbar2 = kbar2; g.nonpolar = nonpolar_enabled;
     g.ex = (double *)malloc(g.G * sizeof(double));
     g.ey = (double *)malloc(g.G * sizeof(double));
     g.ez = (double *)malloc(g.G * sizeof(double));
@@ -127,7 +128,21 @@ void solver_set_eps_k2(const double *ex, const double *ey, const double *ez, con
     memcpy(g.k2, k2, g.G * sizeof(double));
 }
 
-void solver_update_eps_k2(void) {}
+/* library/src/solver_md.c:135-138; a no-op until radii are given (the
+ * reference needs them), so that maps set directly are kept. */
+void get_eps_k2(double *ex, double *ey, double *ez, double *k2) {
+    if (!g.pb) die("oracle: get_eps_k2 needs solver_initialize_grid_pois_boltz first");
+    memcpy(ex, g.ex, g.G * sizeof(double));
+    memcpy(ey, g.ey, g.G * sizeof(double));
+    memcpy(ez, g.ez, g.G * sizeof(double));
+    memcpy(k2, g.k2, g.G * sizeof(double));
+}
+
+void solver_update_eps_k2(void) {
+    if (!g.pb || !p.radii) return;
+    or_pb_maps(g.n, g.h, g.w, g.eps_s, g.eps_int, g.kbar2, p.np, p.pos, p.radii, g.ex, g.ey,
+               g.ez, g.k2);
+}
 
 /* library/src/solver_md.c:51-65, particles.c:68-109, :181-228 */
 void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, int pot_type,
@@ -159,10 +174,14 @@ void solver_initialize_particles(int n, int n_typ, double L, double h, int n_p, 
     p.ready = 1;
 }
 
+/* library/src/particles.c:111-125 */
 void solver_initialize_particles_pois_boltz(double gamma_np, double beta_np, double *radii) {
-    (void)gamma_np; (void)beta_np;
+    p.gamma_np = gamma_np; p.beta_np = beta_np;
     p.radii = (double *)malloc(p.np * sizeof(double));
     memcpy(p.radii, radii, p.np * sizeof(double));
+    p.fdb = (double *)calloc(p.np * 3, sizeof(double));
+    p.fib = (double *)calloc(p.np * 3, sizeof(double));
+    p.fnp = (double *)calloc(p.np * 3, sizeof(double));
 }
 
 /* library/src/solver_md.c:71-89, integrator_general.c:19-41,
@@ -293,7 +312,14 @@ double solver_compute_forces_noel(void) {
     return or_tf_forces(p.np, p.L, p.pos, p.types, p.n_typ, p.table, p.r_cut, p.fn);
 }
 
-double solver_compute_forces_pb(void) { return 0.0; }
+/* library/src/solver_md.c:148-150, particles.c:420-775 */
+double solver_compute_forces_pb(void) {
+    if (!g.pb) die("Poisson-Boltzmann forces are not enabled in the grid.");
+    if (!p.radii) die("Poisson-Boltzmann forces need the ion radii "
+                      "(solver_initialize_particles_pois_boltz).");
+    return or_pb_forces(g.n, g.h, g.w, g.eps_s, g.eps_int, g.nonpolar, p.gamma_np, p.np, p.pos,
+                        p.radii, g.phi_n, g.ex, g.ey, g.ez, g.k2, p.fdb, p.fib, p.fnp);
+}
 
 /* library/src/particles.c:777-795 */
 void solver_compute_forces_tot(void) {
@@ -301,6 +327,11 @@ void solver_compute_forces_tot(void) {
     for (int i = 0; i < s; i++) p.ft[i] = 0.0;
     for (int i = 0; i < s; i++) p.ft[i] += 1.0 * p.fe[i];
     for (int i = 0; i < s; i++) p.ft[i] += 1.0 * p.fn[i];
+    if (p.fdb) {
+        for (int i = 0; i < s; i++) p.ft[i] += 1.0 * p.fdb[i];
+        for (int i = 0; i < s; i++) p.ft[i] += 1.0 * p.fib[i];
+        for (int i = 0; i < s; i++) p.ft[i] += 1.0 * p.fnp[i];
+    }
 }
 
 double get_kinetic_energy(void) { return or_kinetic(p.np, p.vel, p.mass); }
@@ -405,6 +436,7 @@ void solver_finalize(void) {
     memset(&g, 0, sizeof(g));
     free(p.types); free(p.pos); free(p.vel); free(p.mass); free(p.charges);
     free(p.fe); free(p.fn); free(p.ft); free(p.nb); free(p.table); free(p.radii);
+    free(p.fdb); free(p.fib); free(p.fnp);
     memset(&p, 0, sizeof(p));
     q_tot = 0.0;
 }
@@ -414,9 +446,13 @@ void get_vel(double *o) { memcpy(o, p.vel, p.np * 3 * sizeof(double)); }
 void get_fcs_elec(double *o) { memcpy(o, p.fe, p.np * 3 * sizeof(double)); }
 void get_fcs_noel(double *o) { memcpy(o, p.fn, p.np * 3 * sizeof(double)); }
 void get_fcs_tot(double *o) { memcpy(o, p.ft, p.np * 3 * sizeof(double)); }
-void get_fcs_db(double *o) { memset(o, 0, p.np * 3 * sizeof(double)); }
-void get_fcs_ib(double *o) { memset(o, 0, p.np * 3 * sizeof(double)); }
-void get_fcs_np(double *o) { memset(o, 0, p.np * 3 * sizeof(double)); }
+static void get_or_zero(double *o, const double *src) {
+    if (src) memcpy(o, src, p.np * 3 * sizeof(double));
+    else memset(o, 0, p.np * 3 * sizeof(double));
+}
+void get_fcs_db(double *o) { get_or_zero(o, p.fdb); }
+void get_fcs_ib(double *o) { get_or_zero(o, p.fib); }
+void get_fcs_np(double *o) { get_or_zero(o, p.fnp); }
 void get_types(int *o) { memcpy(o, p.types, p.np * sizeof(int)); }
 void get_charges(double *o) { memcpy(o, p.charges, p.np * sizeof(double)); }
 void get_masses(double *o) { memcpy(o, p.mass, p.np * sizeof(double)); }

@@ -65,6 +65,8 @@ def oracle_raw(oracle):
         "or_tf_forces": (cd, [ci, cd, D, I, ci, D, cd, D]),
         "or_tf_neighbor_counts": (None, [ci, cd, D, cd, I]),
         "or_kinetic": (cd, [ci, D, D]),
+        "or_pb_maps": (None, [ci, cd, cd, cd, cd, cd, ci, D, D, D, D, D, D]),
+        "or_pb_forces": (cd, [ci, cd, cd, cd, cd, ci, cd, ci, D, D, D, D, D, D, D, D, D, D]),
         "or_normal3": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, D]),
     }
     for name, (res, args) in sig.items():

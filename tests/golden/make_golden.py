@@ -163,20 +163,23 @@ def pb_mg_vectors(lib):
     return out
 
 
-def md_vectors(lib, n_ions, n_grid, seed, method, steps, **kw):
-    s = W.nacl_system(n_ions, n_grid, seed=seed)
+def md_vectors(lib, n_ions, n_grid, seed, method, steps, density=1.379, **kw):
+    s = W.nacl_system(n_ions, n_grid, seed=seed, density=density)
     cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s), **kw)
     driver.initialize(lib, s, cfg)
-    driver.bootstrap(lib)
+    driver.bootstrap(lib, cfg)
     its, vs = [], []
     for _ in range(steps):
-        it, v = driver.md_step(lib)
+        it, v = driver.md_step(lib, cfg)
         its.append(it)
         vs.append(v)
     snap = driver.snapshot(lib, s)
     lib.solver_finalize()
     out = {"iters": np.array(its), "v_tf": np.array(vs)}
-    for k in ("pos", "vel", "fcs_elec", "fcs_noel", "fcs_tot", "momentum"):
+    keys = ["pos", "vel", "fcs_elec", "fcs_noel", "fcs_tot", "momentum"]
+    if cfg.pb_radii:
+        keys += ["fcs_db", "fcs_ib", "fcs_np"]
+    for k in keys:
         out[k] = snap[k]
     for k in ("kinetic", "temperature", "energy_elec"):
         out[k] = np.array(snap[k])
@@ -220,11 +223,17 @@ def abi_fixture():
 
 
 def main():
+    """All fixtures, or with ``--only name,...`` just those MD cases (merged into
+    ref_md_cases.json)."""
     if os.environ.get("OMP_NUM_THREADS") != "1":
         sys.exit("run with OMP_NUM_THREADS=1 (the reference's scatter races under OpenMP)")
+    only = None
+    if len(sys.argv) > 2 and sys.argv[1] == "--only":
+        only = set(sys.argv[2].split(","))
     lib = MazeLib(REF, extensions=False)
-    np.savez_compressed(os.path.join(HERE, "ref_kernels.npz"), **kernel_vectors(lib))
-    np.savez_compressed(os.path.join(HERE, "ref_pb_mg.npz"), **pb_mg_vectors(lib))
+    if only is None:
+        np.savez_compressed(os.path.join(HERE, "ref_kernels.npz"), **kernel_vectors(lib))
+        np.savez_compressed(os.path.join(HERE, "ref_pb_mg.npz"), **pb_mg_vectors(lib))
     cases = {
         "maze_lcg_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=5),
         "maze_lcg_jacobi_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=3,
@@ -241,8 +250,24 @@ def main():
                                       cas="SPL_QUADR"),
         "maze_lcg_cubic_216_32": dict(n_ions=216, n_grid=32, seed=3, method="MAZE-LCG", steps=3,
                                       cas="SPL_CUBIC"),
+        # Radius-based PB boundary (maps rebuilt every step, DB / IB / NP forces) on a
+        # dilute melt (density 0.1 g/cm^3), eps 80 / 1, 0.1 M, w = 1 Bohr.
+        "pb_radii_maze_lcg_64_32": dict(n_ions=64, n_grid=32, seed=21, density=0.1,
+                                        method="MAZE-LCG", steps=3, eps_s=80.0, pb_radii=True,
+                                        pb_w=1.0),
+        "pb_radii_np_maze_mg_64_32": dict(n_ions=64, n_grid=32, seed=22, density=0.1,
+                                          method="MAZE-MULTIGRID", steps=3, eps_s=80.0,
+                                          pb_radii=True, pb_w=1.0, nonpolar=True,
+                                          gamma_np=0.05, cas="SPL_CUBIC"),
+        "pb_radii_mg_64_32": dict(n_ions=64, n_grid=32, seed=23, density=0.1,
+                                  method="MULTIGRID", steps=2, eps_s=80.0, pb_radii=True,
+                                  pb_w=1.5, radii_ang=(1.2, 1.6)),
     }
     meta = {}
+    if only is not None:
+        with open(os.path.join(HERE, "ref_md_cases.json")) as fh:
+            meta = json.load(fh)
+        cases = {k: v for k, v in cases.items() if k in only}
     for name, kw in cases.items():
         kw = dict(kw)
         args = {k: kw.pop(k) for k in ("n_ions", "n_grid", "seed", "method", "steps")}

@@ -129,10 +129,10 @@ def run(lib, s, cfg, steps, precision=None):
     driver.initialize(lib, s, cfg)
     if precision is not None:
         lib.lib.solver_set_precision(precision)
-    driver.bootstrap(lib)
+    driver.bootstrap(lib, cfg)
     its, vs = [], []
     for _ in range(steps):
-        it, v = driver.md_step(lib)
+        it, v = driver.md_step(lib, cfg)
         its.append(it)
         vs.append(v)
     snap = driver.snapshot(lib, s)
@@ -435,3 +435,103 @@ def test_field_matches_fft_poisson_at_128(hip, precision):
     phi = np.fft.ifftn(ph).real
     d = (g["field"] - g["field"].mean()) - phi
     assert np.abs(d).max() <= 1e-6 * np.abs(phi).max(), np.abs(d).max() / np.abs(phi).max()
+
+
+# ---- Poisson-Boltzmann boundary from ion radii -------------------------------------
+
+PB_RADII_SYSTEMS = {
+    # name: (ions, grid, seed, density g/cm^3, w Bohr, (r_Na, r_Cl) A, method)
+    "dilute_64_32": (64, 32, 31, 0.1, 1.0, (0.95, 1.81), "MAZE-LCG"),
+    "dense_216_36": (216, 36, 32, 1.379, 1.0, (0.95, 1.81), "MAZE-LCG"),
+    "dense_4096_128": (4096, 128, 33, 1.379, 1.0, (0.95, 1.81), "MAZE-MULTIGRID"),
+    "dense_8000_160": (8000, 160, 34, 1.379, 0.7, (1.2, 1.6), "MAZE-MULTIGRID"),
+}
+
+
+def pb_radii_setup(lib, name, nonpolar=False):
+    n_ions, n, seed, dens, w, radii, method = PB_RADII_SYSTEMS[name]
+    s = W.nacl_system(n_ions, n, seed=seed, density=dens)
+    cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s), eps_s=80.0, pb_radii=True,
+                           pb_w=w, radii_ang=radii, nonpolar=nonpolar,
+                           gamma_np=0.05 if nonpolar else 0.0)
+    driver.initialize(lib, s, cfg)
+    return s, cfg
+
+
+def get_maps(lib, n):
+    out = [np.zeros((n, n, n)) for _ in range(4)]
+    lib.get_eps_k2(*out)
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(PB_RADII_SYSTEMS))
+def test_pb_maps_match_oracle(hip, oracle, name):
+    """Dielectric and screening maps built from the ion radii: every product is formed in
+    ion order on both sides; the cubic step's s^3 is rounded once (the oracle's pow may
+    differ from that by one unit in rare cases)."""
+    maps = []
+    for lib in (hip, oracle):
+        s, cfg = pb_radii_setup(lib, name)
+        lib.solver_update_charges()
+        lib.solver_update_eps_k2()
+        maps.append(get_maps(lib, s.n_grid))
+        lib.solver_finalize()
+    for g, o, what in zip(maps[0], maps[1], ("eps_x", "eps_y", "eps_z", "k2")):
+        scale = np.abs(o).max()
+        assert scale > 0, what
+        assert np.abs(g - o).max() <= 4e-16 * scale, (what, np.abs(g - o).max() / scale)
+        assert np.mean(g == o) > 0.999, what
+    k2 = maps[1][3]
+    assert (k2 == 0).any() or name.startswith("dilute")
+    assert ((k2 > 0) & (k2 < k2.max())).any()
+
+
+@pytest.mark.parametrize("name", ["dilute_64_32", "dense_216_36", "dense_4096_128"])
+@pytest.mark.parametrize("nonpolar", [False, True])
+def test_pb_forces_match_oracle(hip, oracle, name, nonpolar):
+    """Dielectric-boundary, ionic-boundary and non-polar forces from the same field and
+    maps: per-ion sums over the ion's box, reduced in a different order."""
+    res = []
+    n = PB_RADII_SYSTEMS[name][1]
+    x = 2 * np.pi * np.arange(n) / n
+    # A smooth periodic field with structure on every axis (the forces only read phi).
+    phi = np.ascontiguousarray(0.3 * np.sin(x)[:, None, None] * np.cos(2 * x)[None, :, None]
+                               + 0.2 * np.cos(3 * x)[None, None, :] + 0.1 * np.sin(x + 1.0)
+                               [None, :, None] * np.sin(2 * x)[None, None, :])
+    for lib in (oracle, hip):
+        s, cfg = pb_radii_setup(lib, name, nonpolar)
+        lib.solver_update_charges()
+        lib.solver_update_eps_k2()
+        lib.solver_set_field(phi)
+        assert lib.solver_compute_forces_pb() == 0.0
+        f = {}
+        for k in ("db", "ib", "np"):
+            a = np.zeros((s.n_p, 3))
+            getattr(lib, "get_fcs_" + k)(a)
+            f[k] = a
+        res.append(f)
+        lib.solver_finalize()
+    o, g = res
+    for k in ("db", "ib", "np"):
+        scale = np.abs(o[k]).max()
+        if k == "np" and not nonpolar:
+            assert scale == 0 and np.abs(g[k]).max() == 0
+            continue
+        assert scale > 0, k
+        assert np.abs(g[k] - o[k]).max() <= 1e-12 * scale, (k, np.abs(g[k] - o[k]).max() / scale)
+
+
+@pytest.mark.parametrize("method", ["MAZE-LCG", "LCG", "MULTIGRID", "MAZE-MULTIGRID"])
+def test_md_sequence_pb_radii(hip, oracle, method):
+    """MD steps with the radius-based PB boundary rebuilt every step and its forces in
+    the total, through every grid type."""
+    s = W.nacl_system(64, 32, seed=35, density=0.1)
+    cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s), eps_s=80.0, pb_radii=True,
+                           pb_w=1.0, nonpolar=True, gamma_np=0.05)
+    its_g, v_g, g = run(hip, s, cfg, 3)
+    its_o, v_o, o = run(oracle, s, cfg, 3)
+    slack = [max(1, int(0.06 * b)) for b in its_o] if "LCG" in method else [1] * len(its_o)
+    assert all(abs(a - b) <= t for a, b, t in zip(its_g, its_o, slack)), (its_g, its_o)
+    for k in ("fcs_elec", "fcs_db", "fcs_ib", "fcs_np", "fcs_tot", "field"):
+        assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+    assert rel(g["pos"], o["pos"]) <= 1e-11

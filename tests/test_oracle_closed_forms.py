@@ -364,3 +364,93 @@ def test_spline_weights_partition_of_unity(oracle_raw, which):
         assert abs(f(x - nodes[1], L, h) - f(nodes[1] - x, L, h)) <= 1e-16
     assert f(0.0, L, h) == (0.75 if "quadr" in which else 2.0 / 3.0)
     assert f(2.1 * h, L, h) == 0.0
+
+
+def pb_step_expected(dist, r, w):
+    """Solvent fraction of one ion at distance `dist`: 0 inside r - w, 1 beyond r + w,
+    and between them the cubic s^2 (3 w - s) / (4 w^3), s = dist - r + w, which has
+    zero slope at both ends and equals 1/2 at the radius."""
+    s = dist - r + w
+    mid = s * s * (3 * w - s) / (4 * w ** 3)
+    return np.where(dist <= r - w, 0.0, np.where(dist >= r + w, 1.0, mid))
+
+
+def test_pb_step_shape():
+    r, w = 2.0, 0.5
+    d = np.array([1.0, 1.5, 2.0, 2.5, 3.0])
+    assert np.allclose(pb_step_expected(d, r, w), [0.0, 0.0, 0.5, 1.0, 1.0], rtol=0, atol=1e-15)
+
+
+def test_pb_maps_geometry(oracle_raw):
+    """Radius-based PB maps against their geometric definition: eps_x/y/z live on the
+    +x/+y/+z face of each point ((i+1/2)h, jh, kh etc.), k2 on the point, each the
+    solvent fraction (product over ions, nearest periodic image) scaled into
+    [eps_int, eps_s] and [0, kbar2]; two overlapping ions multiply."""
+    n, h = 16, 0.5
+    L = n * h
+    eps_s, eps_int, kbar2, w = 80.0, 2.0, 0.03, 0.4
+    pos = np.array([[0.3, 7.9, 4.1], [1.4, 0.2, 4.6], [5.0, 5.0, 1.0]])
+    radii = np.array([1.2, 0.9, 1.5])
+    ex, ey, ez, k2 = (np.zeros((n, n, n)) for _ in range(4))
+    oracle_raw.or_pb_maps(n, h, w, eps_s, eps_int, kbar2, len(pos), pos.reshape(-1), radii,
+                          ex, ey, ez, k2)
+    g = np.arange(n) * h
+    for arr, off, lo, hi in ((ex, (0.5, 0, 0), eps_int, eps_s), (ey, (0, 0.5, 0), eps_int, eps_s),
+                             (ez, (0, 0, 0.5), eps_int, eps_s), (k2, (0, 0, 0), 0.0, kbar2)):
+        X = np.meshgrid(g + off[0] * h, g + off[1] * h, g + off[2] * h, indexing="ij")
+        frac = np.ones((n, n, n))
+        for p, r in zip(pos, radii):
+            d2 = 0.0
+            for a in range(3):
+                d = p[a] - X[a]
+                d -= L * np.round(d / L)
+                d2 = d2 + d * d
+            frac *= pb_step_expected(np.sqrt(d2), r, w)
+        want = lo + (hi - lo) * frac
+        assert np.abs(arr - want).max() <= 1e-13 * hi
+    assert (k2 == 0).any() and (k2 == kbar2).any() and ((k2 > 0) & (k2 < kbar2)).any()
+
+
+def test_pb_forces_symmetry_and_scaling(oracle_raw):
+    """A lone ion on a grid point in a field and maps with the cubic symmetry of the
+    grid feels no boundary force; a shifted copy of the whole system (ion, field, maps
+    by whole cells) gives the same forces; the non-polar term scales with gamma_np
+    and vanishes with it."""
+    n, h = 24, 0.5
+    L = n * h
+    eps_s, eps_int, kbar2, w, r = 80.0, 1.0, 0.05, 0.5, 2.0
+    g = np.arange(n) * h
+    X = np.meshgrid(g, g, g, indexing="ij")
+
+    def system(c):
+        pos = np.array([c])
+        ex, ey, ez, k2 = (np.zeros((n, n, n)) for _ in range(4))
+        oracle_raw.or_pb_maps(n, h, w, eps_s, eps_int, kbar2, 1, pos.reshape(-1),
+                              np.array([r]), ex, ey, ez, k2)
+        d2 = 0.0
+        for a in range(3):
+            d = X[a] - c[a]
+            d -= L * np.round(d / L)
+            d2 = d2 + d * d
+        phi = np.ascontiguousarray(1.0 / np.sqrt(d2 + 1.0))
+        return pos, phi, ex, ey, ez, k2
+
+    def forces(c, gamma=0.0, nonpolar=0):
+        pos, phi, ex, ey, ez, k2 = system(c)
+        out = [np.zeros(3) for _ in range(3)]
+        oracle_raw.or_pb_forces(n, h, w, eps_s, eps_int, nonpolar, gamma, 1, pos.reshape(-1),
+                                np.array([r]), phi, ex, ey, ez, k2, *out)
+        return out
+
+    db, ib, npf = forces([6.0, 6.0, 6.0], 0.3, 1)
+    scale = np.abs(forces([6.2, 5.9, 6.1], 0.3, 1)[0]).max()
+    assert scale > 0
+    # Face terms cancel pairwise only up to the face offsets' rounding.
+    assert np.abs(db).max() <= 1e-12 * scale and np.abs(ib).max() <= 1e-12 * scale
+    a = forces([6.2, 5.9, 6.1], 0.3, 1)
+    b = forces([6.2 + 3 * h, 5.9 - 2 * h, 6.1 + h], 0.3, 1)
+    for u, v in zip(a, b):
+        assert np.allclose(u, v, rtol=1e-9, atol=1e-15 * scale)
+    c = forces([6.2, 5.9, 6.1], 0.6, 1)
+    assert np.allclose(c[2], 2 * a[2], rtol=1e-12) and np.abs(a[2]).max() > 0
+    assert np.array_equal(forces([6.2, 5.9, 6.1], 0.3, 0)[2], np.zeros(3))

@@ -22,20 +22,25 @@ def test_md_golden(oracle, name):
     n_ions, n_grid, seed = kw.pop("n_ions"), kw.pop("n_grid"), kw.pop("seed")
     method, steps = kw.pop("method"), kw.pop("steps")
     ref = np.load(os.path.join(GOLDEN, f"ref_md_{name}.npz"))
-    s = W.nacl_system(n_ions, n_grid, seed=seed)
+    s = W.nacl_system(n_ions, n_grid, seed=seed, density=kw.pop("density", 1.379))
     cfg = driver.RunConfig(method=method, tol=W.poisson_tol(s), **kw)
     driver.initialize(oracle, s, cfg)
-    driver.bootstrap(oracle)
+    driver.bootstrap(oracle, cfg)
     its, vs = [], []
     for _ in range(steps):
-        it, v = driver.md_step(oracle)
+        it, v = driver.md_step(oracle, cfg)
         its.append(it)
         vs.append(v)
     snap = driver.snapshot(oracle, s)
     oracle.solver_finalize()
     assert its == ref["iters"].tolist()
     assert np.array_equal(np.array(vs), ref["v_tf"])
-    for k in ("pos", "vel", "fcs_elec", "fcs_noel", "fcs_tot", "momentum"):
+    keys = ["pos", "vel", "fcs_elec", "fcs_noel", "fcs_tot", "momentum"]
+    if cfg.pb_radii:  # radius-based PB boundary forces
+        keys += ["fcs_db", "fcs_ib", "fcs_np"]
+        assert np.abs(ref["fcs_db"]).max() > 0 and np.abs(ref["fcs_ib"]).max() > 0
+        assert (np.abs(ref["fcs_np"]).max() > 0) == cfg.nonpolar
+    for k in keys:
         assert np.array_equal(snap[k], ref[k]), k
     for k in ("kinetic", "temperature", "energy_elec"):
         assert snap[k] == float(ref[k]), k
