@@ -24,10 +24,15 @@ File formats (all CSV with a header row):
   kJ/mol A^8, A), one row per unordered pair of types;
 * restart field file: ``phi_prev,phi`` with n^3 rows in C order.
 
+With ``poisson_boltzmann: true`` the grid gets the linear PB operator with the
+screening of ionic strength ``I`` (mol/L) and transition width ``w`` (Bohr), each
+ion the radius of its species (particles-file column ``radius``, A) plus
+``probe_radius``, the dielectric and screening maps are rebuilt from the radii every
+step and the boundary forces join the total (``solver.py:146-162``, :348-356,
+:411-415, :421-431).
+
 Old-schema inputs (``charge,mass[,radius],x,y,z[,...]``) are converted by
-:mod:`.convert`.  Only the Tosi-Fumi potential is provided by the library; the
-reference's radius-based PB dielectric (``grid_update_eps_and_k2``) is not, so a run
-with ``poisson_boltzmann: true`` is rejected here.
+:mod:`.convert`.  Only the Tosi-Fumi potential is provided by the library.
 """
 from __future__ import annotations
 
@@ -43,7 +48,7 @@ import numpy as np
 from .capi import MazeLib
 from .config import GridSetting, MDVariables, OutputSettings, load_file
 from .outputs import OutputFiles
-from .workloads import A0, CONV_MASS, KJMOL_TO_HARTREE
+from .workloads import A0, CONV_MASS, KJMOL_TO_HARTREE, kbar2
 
 log = logging.getLogger("maze_poisson_amd")
 
@@ -124,13 +129,16 @@ class SolverMD:
         precond = self.gset.precond.upper()
         if precond not in self.maps["precond"]:
             raise ValueError(f"Preconditioner {precond} not recognized.")
-        if self.mdv.poisson_boltzmann:
-            raise ValueError("poisson_boltzmann: the dielectric and ionic maps built from "
-                             "ion radii (grid_update_eps_and_k2) are not provided by this "
-                             "library; set the maps through solver_set_eps_k2 instead")
         self.lib.solver_initialize_grid(self.N, self.L, self.h, self.mdv.tol, self.gset.eps_s,
                                         self.gset.eps_int, self.maps["grid"][method],
                                         self.maps["precond"][precond])
+        if self.mdv.poisson_boltzmann:
+            if self.gset.I is None or self.gset.w is None:
+                raise ValueError("poisson_boltzmann needs the ionic strength I and the "
+                                 "transition width w in grid_setting.")
+            kb2 = kbar2(self.gset.I, self.gset.eps_s, self.mdv.T, self.h)
+            self.lib.solver_initialize_grid_pois_boltz(self.gset.w, kb2,
+                                                       1 if self.mdv.nonpolar_forces else 0)
         prec = self.mdv.precision.upper()
         if prec not in PRECISION_MODES:
             raise ValueError(f"precision {prec} not recognized among "
@@ -246,6 +254,14 @@ class SolverMD:
                                              self.maps["potential"][potential],
                                              self.maps["cas"][cas], types, pos, vel, mass,
                                              charges, tf)
+        if self.mdv.poisson_boltzmann:
+            if any("radius" not in s for s in sp):
+                raise ValueError("Probe radius must be provided in the input file for "
+                                 "Poisson-Boltzmann.")
+            radius = np.array([s["radius"] for s in sp], dtype=np.float64) / A0
+            radius = np.ascontiguousarray(radius + self.mdv.probe_radius)
+            self.lib.solver_initialize_particles_pois_boltz(self.mdv.gamma_np_au,
+                                                            self.mdv.beta_np, radius)
 
     def initialize_integrator(self) -> None:
         """``solver.py:358-369``"""
@@ -263,10 +279,12 @@ class SolverMD:
         ffile = self.gset.restart_field_file
         if ffile is None or not self.mdv.invert_time:
             self.update_charges()
+            self.update_eps_k2()
             self.lib.solver_init_field()
             self.compute_forces()
             self.lib.integrator_part_1()
             self.update_charges()
+            self.update_eps_k2()
             self.lib.solver_init_field()
             self.compute_forces()
             self.lib.integrator_part_2()
@@ -289,6 +307,11 @@ class SolverMD:
         if self.lib.solver_update_charges() != 0:
             raise RuntimeError("Error: change initial position, charge is not preserved.")
 
+    def update_eps_k2(self) -> None:
+        """``solver.py:411-415``"""
+        if self.mdv.poisson_boltzmann:
+            self.lib.solver_update_eps_k2()
+
     def update_field(self) -> int:
         res = self.lib.solver_update_field()
         if res == -1:
@@ -301,6 +324,8 @@ class SolverMD:
             self.lib.solver_compute_forces_elec()
         if self.mdv.not_elec:
             self.potential_notelec = self.lib.solver_compute_forces_noel()
+        if self.mdv.poisson_boltzmann:
+            self.energy_nonpolar = self.lib.solver_compute_forces_pb()
         self.lib.solver_compute_forces_tot()
 
     def md_loop_iter(self) -> None:
@@ -308,6 +333,7 @@ class SolverMD:
         self.lib.integrator_part_1()
         if self.mdv.elec:
             self.update_charges()
+            self.update_eps_k2()
             t0 = time.perf_counter()
             self.n_iters = self.update_field()
             self.t_iters = time.perf_counter() - t0

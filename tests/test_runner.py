@@ -23,10 +23,14 @@ from maze_poisson_amd.solver import SolverMD, run_file
 A0 = W.A0
 
 
+PB_RADII_ANG = (0.95, 1.81)
+
+
 def write_inputs(d, s, with_vel=True, names=("Na", "Cl")):
-    """species.csv, tosi_fumi.csv and input.csv for System `s` under folder d."""
+    """species.csv (with PB radii), tosi_fumi.csv and input.csv for System `s` under
+    folder d."""
     write_species(os.path.join(d, "species.csv"),
-                  [("Na", 1.0, W.M_NA), ("Cl", -1.0, W.M_CL)])
+                  [("Na", 1.0, W.M_NA, PB_RADII_ANG[0]), ("Cl", -1.0, W.M_CL, PB_RADII_ANG[1])])
     write_tosi_fumi(os.path.join(d, "tosi_fumi.csv"))
     with open(os.path.join(d, "input.csv"), "w") as f:
         f.write("type,x,y,z" + (",vx,vy,vz" if with_vel else "") + "\n")
@@ -155,7 +159,7 @@ def test_rejected_inputs(tmp_path, oracle):
     write_inputs(str(tmp_path), s)
     with pytest.raises(ValueError, match="only TF"):
         run_yaml(tmp_path, yaml_text(s, "o1/").replace("potential: TF", "potential: LJ"), oracle)
-    with pytest.raises(ValueError, match="poisson_boltzmann"):
+    with pytest.raises(ValueError, match="ionic strength I"):
         run_yaml(tmp_path, yaml_text(s, "o2/", extra_md="  poisson_boltzmann: true"), oracle)
     with pytest.raises(ValueError, match="not recognized"):
         run_yaml(tmp_path, yaml_text(s, "o3/", method="SOR"), oracle)
@@ -257,13 +261,20 @@ def test_generated_velocities(tmp_path, oracle):
     assert np.allclose(v / scale, ref, rtol=1e-13, atol=0)
 
 
-def test_reference_library_through_runner(tmp_path, oracle, ref):
+@pytest.mark.parametrize("pb", [False, True])
+def test_reference_library_through_runner(tmp_path, oracle, ref, pb):
     """The reference C (built from its sources) and the oracle, both driven by the
-    runner from the same files, write the same trajectory."""
-    s = W.nacl_system(216, 24, seed=8)
+    runner from the same files, write the same trajectory (plain and with the
+    radius-based PB boundary)."""
+    if pb:
+        s = W.nacl_system(64, 32, seed=25, density=0.1)
+        kw = dict(extra_grid=PB_GRID, extra_md=PB_MD)
+    else:
+        s = W.nacl_system(216, 24, seed=8)
+        kw = {}
     write_inputs(str(tmp_path), s)
-    run_yaml(tmp_path, yaml_text(s, "o/", n_steps=4), oracle)
-    run_yaml(tmp_path, yaml_text(s, "r/", n_steps=4), ref)
+    run_yaml(tmp_path, yaml_text(s, "o/", n_steps=4, **kw), oracle)
+    run_yaml(tmp_path, yaml_text(s, "r/", n_steps=4, **kw), ref)
     _, xo, vo = restart_arrays(tmp_path / "o" / "restart.csv")
     _, xr, vr = restart_arrays(tmp_path / "r" / "restart.csv")
     assert np.abs(xo - xr).max() <= 1e-11 * np.abs(xr).max()
@@ -274,6 +285,42 @@ def test_reference_library_through_runner(tmp_path, oracle, ref):
     po = [int(r[2]) for r in read_rows(tmp_path / "o" / "performance.csv")[1]]
     pr = [int(r[2]) for r in read_rows(tmp_path / "r" / "performance.csv")[1]]
     assert all(abs(a - b) <= 1 for a, b in zip(po, pr)), (po, pr)
+
+
+PB_GRID = "  eps_s: 80.0\n  I: 0.1\n  w: 1.0\n"
+PB_MD = "  poisson_boltzmann: true\n  nonpolar_forces: true\n  gamma_np: 0.05\n"
+
+
+def test_pb_run_matches_driver(tmp_path, oracle):
+    """A PB run (radius-based maps every step, boundary forces, forces_pb.csv) equals
+    the scripted driver sequence with the same radii, screening and surface tension."""
+    s = W.nacl_system(64, 32, seed=24, density=0.1)
+    write_inputs(str(tmp_path), s)
+    run_yaml(tmp_path, yaml_text(s, "pb/", n_steps=3, extra_grid=PB_GRID, extra_md=PB_MD,
+                                 extra_out="  print_forces_pb: true"), oracle)
+    s2 = W.System(n_grid=s.n_grid, L=s.L, types=s.types,
+                  pos=np.ascontiguousarray(np.array([[float(repr(float(v))) for v in p]
+                                                     for p in s.pos * A0]) / A0),
+                  vel=s.vel, mass=s.mass, charges=s.charges)
+    cfg = driver.RunConfig(method="MAZE-LCG", tol=W.poisson_tol(s), T=1550, integrator="OVRVO",
+                           eps_s=80.0, pb_radii=True, pb_w=1.0, radii_ang=PB_RADII_ANG,
+                           nonpolar=True, gamma_np=0.05)
+    driver.initialize(oracle, s2, cfg)
+    driver.bootstrap(oracle, cfg)
+    for _ in range(3):
+        driver.md_step(oracle, cfg)
+    snap = driver.snapshot(oracle, s2, fields=False)
+    oracle.solver_finalize()
+    _, x, v = restart_arrays(tmp_path / "pb" / "restart.csv")
+    assert np.array_equal(x, snap["pos"] * A0) and np.array_equal(v, snap["vel"])
+    h, rows = read_rows(tmp_path / "pb" / "forces_pb.csv")
+    assert h == ["iter", "particle", "Fx_RF", "Fy_RF", "Fz_RF", "Fx_DB", "Fy_DB", "Fz_DB",
+                 "Fx_IB", "Fy_IB", "Fz_IB", "Fx_NP", "Fy_NP", "Fz_NP"]
+    last = np.array([[float(c) for c in r[2:]] for r in rows if r[0] == "3"])
+    assert last.shape == (s.n_p, 12)
+    for col, key in ((0, "fcs_elec"), (3, "fcs_db"), (6, "fcs_ib"), (9, "fcs_np")):
+        assert np.array_equal(last[:, col:col + 3], snap[key]), key
+        assert np.abs(snap[key]).max() > 0, key
 
 
 # ---- restart and time reversal -----------------------------------------------------------
@@ -406,6 +453,27 @@ def test_runner_gpu_matches_oracle(tmp_path, hip, oracle):
     gp = [int(r[2]) for r in read_rows(tmp_path / "g" / "performance.csv")[1]]
     op = [int(r[2]) for r in read_rows(tmp_path / "o" / "performance.csv")[1]]
     assert all(abs(a - b) <= 1 for a, b in zip(gp, op)), (gp, op)
+
+
+@pytest.mark.gpu
+def test_runner_gpu_pb_matches_oracle(tmp_path, hip, oracle):
+    """A PB run (maps from the radii every step, boundary forces) through the HIP library
+    and the oracle: trajectories and every per-ion force column of forces_pb.csv."""
+    s = W.nacl_system(64, 32, seed=26, density=0.1)
+    write_inputs(str(tmp_path), s)
+    for d, lib in (("g/", hip), ("o/", oracle)):
+        run_yaml(tmp_path, yaml_text(s, d, n_steps=4, method="MAZE-MULTIGRID", extra_grid=PB_GRID,
+                                     extra_md=PB_MD, extra_out="  print_forces_pb: true"), lib)
+    _, xg, _ = restart_arrays(tmp_path / "g" / "restart.csv")
+    _, xo, _ = restart_arrays(tmp_path / "o" / "restart.csv")
+    assert np.abs(xg - xo).max() <= 1e-11 * np.abs(xo).max()
+    g = np.array([[float(c) for c in r] for r in read_rows(tmp_path / "g" / "forces_pb.csv")[1]])
+    o = np.array([[float(c) for c in r] for r in read_rows(tmp_path / "o" / "forces_pb.csv")[1]])
+    assert np.array_equal(g[:, :2], o[:, :2])
+    for c in range(2, 14, 3):
+        scale = np.abs(o[:, c:c + 3]).max()
+        assert scale > 0
+        assert np.abs(g[:, c:c + 3] - o[:, c:c + 3]).max() <= 1e-7 * scale, c
 
 
 @pytest.mark.gpu
