@@ -131,6 +131,101 @@ def driver_rate(lib, precision, steps, getters):
     return steps / el
 
 
+# Radius-based PB boundary on the 8000-ion / 160^3 geometry (dense melt, radii
+# Na 0.95 A / Cl 1.81 A + 1.4 A probe, w = 1 Bohr, eps 80 / 1, 0.1 M).
+PB_IONS, PB_GRID, PB_SEED = 8000, 160, 3
+
+
+def pb_radii_rate(lib, steps, prof_steps):
+    """MD steps/s with the PB maps rebuilt from the radii every step and the boundary
+    forces in the total (field solve: FP32 Jacobi CG inside FP64 refinement), and the
+    average durations of the map and force kernels."""
+    L = lib.lib
+    s = W.nacl_system(PB_IONS, PB_GRID, seed=PB_SEED)
+    cfg = driver.RunConfig(method="MAZE-LCG", tol=W.poisson_tol(s), integrator="VERLET",
+                           eps_s=80.0, pb_radii=True, pb_w=1.0)
+    driver.initialize(lib, s, cfg)
+    lib.solver_set_precision(PATHS["mixed_cg"])
+    driver.bootstrap(lib, cfg)
+    for _ in range(3):
+        lib.solver_md_loop_iter()
+    lib.solver_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.solver_md_loop_iter()
+    lib.solver_sync()
+    el = time.perf_counter() - t0
+    L.maze_prof_reset()
+    L.maze_prof_enable(1)
+    for _ in range(prof_steps):
+        lib.solver_md_loop_iter()
+    lib.solver_sync()
+    L.maze_prof_enable(0)
+    prof = {k: prof_get(L, k) for k in ("pb_maps", "pb_forces")}
+    lib.solver_finalize()
+    out = {"workload": f"{PB_IONS}-ion NaCl melt, {PB_GRID}^3 grid, PB maps from radii every "
+                       "step + DB/IB/NP forces, MAZE-LCG FP32 CG in FP64 refinement",
+           "steps_per_s": round(steps / el, 3)}
+    G = PB_GRID ** 3
+    for k, (tot, cnt) in prof.items():
+        if cnt:
+            avg = tot / cnt * 1e-3
+            out[k + "_avg_us"] = round(avg * 1e6, 2)
+            if k == "pb_maps":  # writes eps_x, eps_y, eps_z, k2 (FP64) once per point
+                gbs = 32.0 * G / avg / 1e9
+                out["pb_maps_gbs"] = round(gbs, 1)
+                out["pb_maps_frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
+    return out
+
+
+def runner_rate(lib, steps):
+    """Steps/s of a whole run through the input-file runner (YAML + CSV inputs, outputs
+    energy / momentum / temperature / performance every 50 steps), headline workload."""
+    import tempfile
+    from maze_poisson_amd.convert import write_species, write_tosi_fumi
+    from maze_poisson_amd.solver import run_file
+    s, tol = make_system()
+    with tempfile.TemporaryDirectory() as d:
+        write_species(os.path.join(d, "species.csv"), [("Na", 1.0, W.M_NA), ("Cl", -1.0, W.M_CL)])
+        write_tosi_fumi(os.path.join(d, "tosi_fumi.csv"))
+        with open(os.path.join(d, "input.csv"), "w") as f:
+            f.write("type,x,y,z,vx,vy,vz\n")
+            for i in range(s.n_p):
+                f.write(",".join([("Na", "Cl")[s.types[i]]] + [repr(float(v)) for v in s.pos[i] * W.A0]
+                                 + [repr(float(v)) for v in s.vel[i]]) + "\n")
+        yml = os.path.join(d, "run.yml")
+        with open(yml, "w") as f:
+            f.write(f"""grid_setting:
+  N: {s.n_grid}
+  N_p: {s.n_p}
+  N_typs: 2
+  L: {s.L!r}
+  input_file: "{d}/input.csv"
+  particles_file: "{d}/species.csv"
+md_variables:
+  N_steps: {steps}
+  T: 1500
+  dt_fs: 0.25
+  tol: {tol!r}
+  method: MAZE-LCG
+  integrator: VERLET
+  potential: TF
+  potential_params_file: "{d}/tosi_fumi.csv"
+  precision: MIXED-MG
+output_settings:
+  path: "{d}/out"
+  print_energy: true
+  print_momentum: true
+  print_temperature: true
+  print_performance: true
+  stride: 50
+  flushstride: 5
+""")
+        solver = run_file(yml, lib=lib)
+        lib.solver_finalize()
+    return solver.timers["steps_per_s"]
+
+
 def forces_after(lib, precision, tol_factor, nsteps):
     s = start(lib, precision, tol_factor)
     its = [driver.md_step(lib)[0] for _ in range(nsteps)]
@@ -253,6 +348,8 @@ def main():
             errs[name] = rms_rel(f, f_ref)
             iters[name] = float(np.mean(its))
         line_extra = {
+            "runner_steps_per_s": round(runner_rate(lib, 200), 2),
+            "pb_radii": pb_radii_rate(lib, max(10, args.steps // 10), 5),
             "python_driver_steps_per_s": round(driver_rate(lib, PATHS["mixed_mg"], 100, False), 2),
             "python_driver_with_getters_steps_per_s": round(
                 driver_rate(lib, PATHS["mixed_mg"], 100, True), 2),

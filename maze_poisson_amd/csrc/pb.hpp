@@ -10,6 +10,7 @@ namespace maze {
 constexpr int kPbTile = 8;     // edge of the grid tiles and ion bins of the map kernel
 constexpr int kPbCap = 2048;   // candidate ions one map tile can hold
 constexpr int kPbMaxSeg = 512; // bin-row segments one map tile can scan
+constexpr int kPbChunk = 256;  // candidates staged in LDS at a time (= threads)
 
 struct PBDev {
     int np = 0, n = 0;

@@ -199,7 +199,8 @@ __global__ __launch_bounds__(kBlock) void k_pb_maps(PBMapArgs a) {
     }
     const int total = s_total;
     for (int sidx = threadIdx.x; sidx < nseg; sidx += kBlock)
-        for (int q = 0; q < seg_cnt[sidx]; q++) list[seg_off[sidx] + q] = a.ids_sorted[seg_lo[sidx] + q];
+        for (int q = 0; q < seg_cnt[sidx]; q++)
+            list[seg_off[sidx] + q] = a.ids_sorted[seg_lo[sidx] + q];
     int P = 1;
     while (P < total) P <<= 1;
     for (int q = total + threadIdx.x; q < P; q += kBlock) list[q] = 0x7fffffff;
@@ -228,29 +229,88 @@ __global__ __launch_bounds__(kBlock) void k_pb_maps(PBMapArgs a) {
     const double h = a.h, hd2 = h / 2.0, w = a.w;
     const double e0 = a.eps_s - a.eps_int;
     double k2v[2] = {a.kbar2, a.kbar2}, exv[2] = {e0, e0}, eyv[2] = {e0, e0}, ezv[2] = {e0, e0};
-    for (int c = 0; c < total; c++) {
-        const int id = list[c];
-        const double px = a.pos[3 * id], py = a.pos[3 * id + 1], pz = a.pos[3 * id + 2];
-        const int R = a.rmap[id];
-        const int cx = cell_of(px, h), cy = cell_of(py, h), cz = cell_of(pz, h);
-        const int oj = box_offset(gj, cy, n, R), ok = box_offset(gk, cz, n, R);
-        if (oj > R || ok > R) continue;
-        const double r = a.radii[id], rp2 = a.rp2[id], rm2 = a.rm2[id];
-        const double dy = py - (cy + oj) * h, dy2 = dy * dy;
-        const double dz = pz - (cz + ok) * h, dz2 = dz * dz;
-        const double ty = dy - hd2, tz = dz - hd2;
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int gi = u == 0 ? gi0 : gi1;
-            const int oi = box_offset(gi, cx, n, R);
-            if (oi > R) continue;
-            const double dx = px - (cx + oi) * h, dx2 = dx * dx;
-            const double tx = dx - hd2;
-            apply_mask(k2v[u], dx2 + dy2 + dz2, rp2, rm2, r, w, a.c);
-            apply_mask(exv[u], tx * tx + dy2 + dz2, rp2, rm2, r, w, a.c);
-            apply_mask(eyv[u], dx2 + ty * ty + dz2, rp2, rm2, r, w, a.c);
-            apply_mask(ezv[u], dx2 + dy2 + tz * tz, rp2, rm2, r, w, a.c);
+    // Candidates in id order, a chunk at a time: those whose sphere r + w can reach a
+    // point or face of the tile are kept (order preserved) with their data in LDS.
+    __shared__ double c_p[3][kPbChunk], c_r[kPbChunk], c_rp2[kPbChunk], c_rm2[kPbChunk];
+    __shared__ int c_cell[3][kPbChunk], c_R[kPbChunk];
+    __shared__ int c_wsum[kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    bool done = false;  // this wave's points are all 0 in every map (later factors keep 0)
+    for (int c0 = 0; c0 < total; c0 += kBlock) {
+        const int c = c0 + threadIdx.x;
+        bool keep = false;
+        double px = 0, py = 0, pz = 0, r = 0, rp2 = 0, rm2 = 0;
+        int R = 0, cell[3] = {0, 0, 0};
+        if (c < total) {
+            const int id = list[c];
+            px = a.pos[3 * id]; py = a.pos[3 * id + 1]; pz = a.pos[3 * id + 2];
+            r = a.radii[id]; rp2 = a.rp2[id]; rm2 = a.rm2[id]; R = a.rmap[id];
+            const double pp[3] = {px, py, pz};
+            double d2 = 0.0;
+            keep = true;
+            for (int d = 0; d < 3 && keep; d++) {
+                cell[d] = cell_of(pp[d], h);
+                // Nearest point or +face of the tile along this axis inside the box.
+                double best = -1.0;
+                for (int q = 0; q < kPbTile; q++) {
+                    const int g = tb[d] * kPbTile + q;
+                    if (g >= n) break;
+                    const int o = box_offset(g, cell[d], n, R);
+                    if (o > R) continue;
+                    double dd = fabs(pp[d] - (cell[d] + o) * h) - hd2;
+                    dd = dd > 0.0 ? dd : 0.0;
+                    if (best < 0.0 || dd < best) best = dd;
+                }
+                if (best < 0.0) keep = false;
+                else d2 += best * best;
+            }
+            keep = keep && d2 < rp2 * (1.0 + 1e-9) + 1e-300;
         }
+        // Stable compaction of the kept candidates of this chunk.
+        const unsigned long long bal = __ballot(keep);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) c_wsum[wave] = __popcll(bal);
+        __syncthreads();
+        int off = 0, kept = 0;
+        for (int q = 0; q < kBlock / kWave; q++) {
+            if (q < wave) off += c_wsum[q];
+            kept += c_wsum[q];
+        }
+        if (keep) {
+            const int slot = off + before;
+            c_p[0][slot] = px; c_p[1][slot] = py; c_p[2][slot] = pz;
+            c_r[slot] = r; c_rp2[slot] = rp2; c_rm2[slot] = rm2; c_R[slot] = R;
+            c_cell[0][slot] = cell[0]; c_cell[1][slot] = cell[1]; c_cell[2][slot] = cell[2];
+        }
+        __syncthreads();
+        for (int q = 0; q < kept && !done; q++) {
+            const int Rq = c_R[q];
+            const int oj = box_offset(gj, c_cell[1][q], n, Rq);
+            const int ok = box_offset(gk, c_cell[2][q], n, Rq);
+            if (oj <= Rq && ok <= Rq) {
+                const double qx = c_p[0][q], qy = c_p[1][q], qz = c_p[2][q];
+                const double rq = c_r[q], qp2 = c_rp2[q], qm2 = c_rm2[q];
+                const double dy = qy - (c_cell[1][q] + oj) * h, dy2 = dy * dy;
+                const double dz = qz - (c_cell[2][q] + ok) * h, dz2 = dz * dz;
+                const double ty = dy - hd2, tz = dz - hd2;
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int gi = u == 0 ? gi0 : gi1;
+                    const int oi = box_offset(gi, c_cell[0][q], n, Rq);
+                    if (oi > Rq) continue;
+                    const double dx = qx - (c_cell[0][q] + oi) * h, dx2 = dx * dx;
+                    const double tx = dx - hd2;
+                    apply_mask(k2v[u], dx2 + dy2 + dz2, qp2, qm2, rq, w, a.c);
+                    apply_mask(exv[u], tx * tx + dy2 + dz2, qp2, qm2, rq, w, a.c);
+                    apply_mask(eyv[u], dx2 + ty * ty + dz2, qp2, qm2, rq, w, a.c);
+                    apply_mask(ezv[u], dx2 + dy2 + tz * tz, qp2, qm2, rq, w, a.c);
+                }
+            }
+            const bool zero = k2v[0] == 0.0 && exv[0] == 0.0 && eyv[0] == 0.0 && ezv[0] == 0.0 &&
+                              k2v[1] == 0.0 && exv[1] == 0.0 && eyv[1] == 0.0 && ezv[1] == 0.0;
+            done = __all(zero);
+        }
+        __syncthreads();
     }
     const long n2 = (long)n * n;
 #pragma unroll
@@ -326,58 +386,128 @@ __global__ __launch_bounds__(kBlock) void k_pb_forces(PBForceArgs a) {
     const int R = a.rfrc[ion], S = 2 * R + 1;
     const int cx = cell_of(px, h), cy = cell_of(py, h), cz = cell_of(pz, h);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // db xyz, ib xyz, np xyz
-    const int total = S * S * S;
-    for (int q = threadIdx.x; q < total; q += kBlock) {
-        const int qi = q / (S * S), qj = (q / S) % S, qk = q % S;
-        const int ii = cx + qi - R, jj = cy + qj - R, kk = cz + qk - R;
-        const double dx = px - ii * h, dy = py - jj * h, dz = pz - kk * h;
-        const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
-        const double xm = dx + hd2, xp = dx - hd2, ym = dy + hd2, yp = dy - hd2;
-        const double zm = dz + hd2, zp = dz - hd2;
-        const double r2c = dx2 + dy2 + dz2;
-        const double r2xm = xm * xm + dy2 + dz2, r2xp = xp * xp + dy2 + dz2;
-        const double r2ym = dx2 + ym * ym + dz2, r2yp = dx2 + yp * yp + dz2;
-        const double r2zm = dx2 + dy2 + zm * zm, r2zp = dx2 + dy2 + zp * zp;
-        const bool bc = in_shell(r2c, rp2, rm2);
-        const bool bxm = in_shell(r2xm, rp2, rm2), bxp = in_shell(r2xp, rp2, rm2);
-        const bool bym = in_shell(r2ym, rp2, rm2), byp = in_shell(r2yp, rp2, rm2);
-        const bool bzm = in_shell(r2zm, rp2, rm2), bzp = in_shell(r2zp, rp2, rm2);
-        if (!(bc | bxm | bxp | bym | byp | bzm | bzp)) continue;
-        const int i = pmod(ii, n), j = pmod(jj, n), k = pmod(kk, n);
-        const long ci = i * n2, cj = (long)j * n;
-        const long im = wrap1(i - 1, n) * n2, ip = wrap1(i + 1, n) * n2;
-        const long jm = (long)wrap1(j - 1, n) * n, jp = (long)wrap1(j + 1, n) * n;
-        const int km = wrap1(k - 1, n), kp = wrap1(k + 1, n);
-        const long c = ci + cj + k;
-        const double phic = a.phi[c];
-        if (bc) {
-            const double r1 = sqrt(r2c) + 1E-12;
-            const double f = a.k2[c] * phic * phic * step_dlog(a.c, r1 - r + w);
-            acc[3] += f * (dx / r1);
-            acc[4] += f * (dy / r1);
-            acc[5] += f * (dz / r1);
+    // Only points whose centre lies within h/2 of the shell (rm2, rp2) can have a
+    // shell point or face: per box row (qi, qj) the k ranges of centre distances in
+    // (sqrt(rm2) - h/2, sqrt(rp2) + h/2), widened by a point on each side, are listed
+    // (two per row) and their points spread over the threads.
+    __shared__ int e_start[2 * kBlock], e_off[2 * kBlock + 1];
+    __shared__ int w_tot[kBlock / kWave];
+    const double band_hi = (sqrt(rp2) + hd2) * (1.0 + 1e-12);
+    double band_lo = (sqrt(rm2) - hd2) * (1.0 - 1e-12);
+    band_lo = band_lo > 0.0 ? band_lo : 0.0;
+    const double hi2 = band_hi * band_hi, lo2 = band_lo * band_lo;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int rows = S * S;
+    for (int row0 = 0; row0 < rows; row0 += kBlock) {
+        int st[2] = {0, 0}, cnt[2] = {0, 0};
+        const int row = row0 + threadIdx.x;
+        if (row < rows) {
+            const int qi = row / S, qj = row % S;
+            const double dx = px - (cx + qi - R) * h, dy = py - (cy + qj - R) * h;
+            const double rho2 = dx * dx + dy * dy;
+            if (rho2 < hi2) {
+                const double b = sqrt(hi2 - rho2);
+                const double c = rho2 < lo2 ? sqrt(lo2 - rho2) : 0.0;
+                // dz = pz - (cz + qk - R) h in [c, b] or [-b, -c].
+                const double base = pz / h - cz + R;
+                int a0 = (int)floor(base - b / h) - 1, a1 = (int)ceil(base - c / h) + 1;
+                int b0 = (int)floor(base + c / h) - 1, b1 = (int)ceil(base + b / h) + 1;
+                a0 = max(a0, 0); a1 = min(a1, S - 1);
+                b0 = max(b0, 0); b1 = min(b1, S - 1);
+                if (a1 >= b0 - 1) {  // the two ranges touch: one range
+                    a1 = max(a1, b1);
+                    b0 = 1; b1 = 0;
+                }
+                if (a1 >= a0) { st[0] = row * S + a0; cnt[0] = a1 - a0 + 1; }
+                if (b1 >= b0) { st[1] = row * S + b0; cnt[1] = b1 - b0 + 1; }
+            }
         }
-        if (bxm | bxp | bym | byp | bzm | bzp) {
-            const double exc = a.ex[c], eyc = a.ey[c], ezc = a.ez[c];
-            const double exb = a.ex[im + cj + k], eyb = a.ey[ci + jm + k], ezb = a.ez[ci + cj + km];
-            const double gx = exc - exb, gy = eyc - eyb, gz = ezc - ezb;
-            const double gn = sqrt(gx * gx + gy * gy + gz * gz) / h;
-            const double ig = (gn > 0.0) ? 1.0 / gn : 0.0;
-            const double gxn = ig * gx, gyn = ig * gy, gzn = ig * gz;
-            const double ei = a.eps_int;
-            if (bxm) face_term(xm, dy, dz, r2xm, r, w, a.c, exb, ei, a.phi[im + cj + k] - phic,
-                               phic, gxn, -1, a.nonpolar, acc);
-            if (bxp) face_term(xp, dy, dz, r2xp, r, w, a.c, exc, ei, a.phi[ip + cj + k] - phic,
-                               phic, gxn, +1, a.nonpolar, acc);
-            if (bym) face_term(dx, ym, dz, r2ym, r, w, a.c, eyb, ei, a.phi[ci + jm + k] - phic,
-                               phic, gyn, -1, a.nonpolar, acc);
-            if (byp) face_term(dx, yp, dz, r2yp, r, w, a.c, eyc, ei, a.phi[ci + jp + k] - phic,
-                               phic, gyn, +1, a.nonpolar, acc);
-            if (bzm) face_term(dx, dy, zm, r2zm, r, w, a.c, ezb, ei, a.phi[ci + cj + km] - phic,
-                               phic, gzn, -1, a.nonpolar, acc);
-            if (bzp) face_term(dx, dy, zp, r2zp, r, w, a.c, ezc, ei, a.phi[ci + cj + kp] - phic,
-                               phic, gzn, +1, a.nonpolar, acc);
+        // Exclusive scan of the 2 * kBlock range lengths.
+        int mine = cnt[0] + cnt[1], incl = mine;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
         }
+        if (lane == kWave - 1) w_tot[wave] = incl;
+        __syncthreads();
+        int wbase = 0, total = 0;
+        for (int q = 0; q < kBlock / kWave; q++) {
+            if (q < wave) wbase += w_tot[q];
+            total += w_tot[q];
+        }
+        const int ex0 = wbase + incl - mine;
+        e_start[2 * threadIdx.x] = st[0];
+        e_start[2 * threadIdx.x + 1] = st[1];
+        e_off[2 * threadIdx.x] = ex0;
+        e_off[2 * threadIdx.x + 1] = ex0 + cnt[0];
+        if (threadIdx.x == 0) e_off[2 * kBlock] = total;
+        __syncthreads();
+        for (int f = threadIdx.x; f < total; f += kBlock) {
+            // Last range with e_off <= f (empty ranges share their successor's offset).
+            int lo = 0, hi = 2 * kBlock - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (e_off[mid] <= f) lo = mid;
+                else hi = mid - 1;
+            }
+            const int q = e_start[lo] + (f - e_off[lo]);
+            const int qi = q / (S * S), qj = (q / S) % S, qk = q % S;
+            const int ii = cx + qi - R, jj = cy + qj - R, kk = cz + qk - R;
+            const double dx = px - ii * h, dy = py - jj * h, dz = pz - kk * h;
+            const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+            const double xm = dx + hd2, xp = dx - hd2, ym = dy + hd2, yp = dy - hd2;
+            const double zm = dz + hd2, zp = dz - hd2;
+            const double r2c = dx2 + dy2 + dz2;
+            const double r2xm = xm * xm + dy2 + dz2, r2xp = xp * xp + dy2 + dz2;
+            const double r2ym = dx2 + ym * ym + dz2, r2yp = dx2 + yp * yp + dz2;
+            const double r2zm = dx2 + dy2 + zm * zm, r2zp = dx2 + dy2 + zp * zp;
+            const bool bc = in_shell(r2c, rp2, rm2);
+            const bool bxm = in_shell(r2xm, rp2, rm2), bxp = in_shell(r2xp, rp2, rm2);
+            const bool bym = in_shell(r2ym, rp2, rm2), byp = in_shell(r2yp, rp2, rm2);
+            const bool bzm = in_shell(r2zm, rp2, rm2), bzp = in_shell(r2zp, rp2, rm2);
+            if (!(bc | bxm | bxp | bym | byp | bzm | bzp)) continue;
+            const int i = pmod(ii, n), j = pmod(jj, n), k = pmod(kk, n);
+            const long ci = i * n2, cj = (long)j * n;
+            const long im = wrap1(i - 1, n) * n2, ip = wrap1(i + 1, n) * n2;
+            const long jm = (long)wrap1(j - 1, n) * n, jp = (long)wrap1(j + 1, n) * n;
+            const int km = wrap1(k - 1, n), kp = wrap1(k + 1, n);
+            const long c = ci + cj + k;
+            const double phic = a.phi[c];
+            // A term whose coefficient is exactly 0 (k2 = 0, or eps_f = eps_int inside
+            // another ion's core) adds +-0 (H > 0 in the shell, so its log-derivative
+            // is finite) and is skipped.
+            const double k2c = bc ? a.k2[c] : 0.0;
+            if (k2c != 0.0) {
+                const double r1 = sqrt(r2c) + 1E-12;
+                const double f = k2c * phic * phic * step_dlog(a.c, r1 - r + w);
+                acc[3] += f * (dx / r1);
+                acc[4] += f * (dy / r1);
+                acc[5] += f * (dz / r1);
+            }
+            if (bxm | bxp | bym | byp | bzm | bzp) {
+                const double exc = a.ex[c], eyc = a.ey[c], ezc = a.ez[c];
+                const double exb = a.ex[im + cj + k], eyb = a.ey[ci + jm + k];
+                const double ezb = a.ez[ci + cj + km];
+                const double gx = exc - exb, gy = eyc - eyb, gz = ezc - ezb;
+                const double gn = sqrt(gx * gx + gy * gy + gz * gz) / h;
+                const double ig = (gn > 0.0) ? 1.0 / gn : 0.0;
+                const double gxn = ig * gx, gyn = ig * gy, gzn = ig * gz;
+                const double ei = a.eps_int;
+                auto face = [&](bool in, double fx, double fy, double fz, double r2f,
+                                double eps_f, long nbr, double g, int sgn) {
+                    if (in && eps_f != ei)
+                        face_term(fx, fy, fz, r2f, r, w, a.c, eps_f, ei, a.phi[nbr] - phic, phic,
+                                  g, sgn, a.nonpolar, acc);
+                };
+                face(bxm, xm, dy, dz, r2xm, exb, im + cj + k, gxn, -1);
+                face(bxp, xp, dy, dz, r2xp, exc, ip + cj + k, gxn, +1);
+                face(bym, dx, ym, dz, r2ym, eyb, ci + jm + k, gyn, -1);
+                face(byp, dx, yp, dz, r2yp, eyc, ci + jp + k, gyn, +1);
+                face(bzm, dx, dy, zm, r2zm, ezb, ci + cj + km, gzn, -1);
+                face(bzp, dx, dy, zp, r2zp, ezc, ci + cj + kp, gzn, +1);
+            }
+        }
+        __syncthreads();
     }
     double tot[9];
 #pragma unroll

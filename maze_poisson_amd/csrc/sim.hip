@@ -296,7 +296,7 @@ void Sim::init_pb_particles(double gamma_np, double beta_np, const double* radii
 void Sim::update_eps_k2() {
     if (!pb || !pbd.active) return;
     launch_pb_bin(stream, pbd, p.pos);
-    launch_pb_maps(stream, pbd, p.pos, ex, ey, ez, k2);
+    timed(kTPbMaps, 0, [&] { launch_pb_maps(stream, pbd, p.pos, ex, ey, ez, k2); });
     restrict_pb_coef();
     MAZE_HIP(hipMemcpyAsync(&ps_host->pb_flag, &ps->pb_flag, sizeof(int), hipMemcpyDeviceToHost,
                             stream));
@@ -318,7 +318,7 @@ double Sim::forces_pb() {
     if (!pbd.active)
         fatal("maze: PB boundary forces need the ion radii (solver_initialize_particles_pois_boltz)");
     check_pb_flag();
-    launch_pb_forces(stream, pbd, p.pos, phi, ex, ey, ez, k2);
+    timed(kTPbForces, 0, [&] { launch_pb_forces(stream, pbd, p.pos, phi, ex, ey, ez, k2); });
     return 0.0;
 }
 

@@ -29,7 +29,7 @@ struct MGLevel {
 // Per-kernel timing collected with HIP events on the solver stream.
 enum TimerId {
     kTCgDir = 0, kTCgUpdate, kTCgDirF32, kTCgUpdateF32, kTMgDown, kTMgUp, kTRefine, kTPred, kTTf,
-    kTCic
+    kTCic, kTPbMaps, kTPbForces
 };
 struct KernelTimer {
     const char* name;
@@ -131,7 +131,7 @@ struct Sim {
     std::vector<KernelTimer> timers = {{"cg_dir"},     {"cg_update"},  {"cg_dir_f32"},
                                        {"cg_update_f32"}, {"mg_down_l0"}, {"mg_up_l0"},
                                        {"refine"},     {"pred_sigma"}, {"tf_pairs"},
-                                       {"cic"}};
+                                       {"cic"},        {"pb_maps"},    {"pb_forces"}};
     // When profiling is on, the first kernel `launch` issues records its own
     // start and end in a pair of HIP events (MAZE_LAUNCH); `iter` tags solver
     // iterations so that launches after the stop (no-ops) can be left out.
