@@ -201,13 +201,15 @@ void solver_set_y_extrapolation(int on);
  * get_energy_elec keeps the reference's 1/2 sum_g phi_g. */
 double get_energy_elec_qphi(void);
 /* Mode 2 = FP32 multigrid-preconditioned CG inside FP64 iterative
- * refinement (V(2,2) damped Jacobi, exact FP64 coarsest solve); falls back
- * to mode 1 when the grid has no multigrid hierarchy (halving reaches an odd
- * size above 12) or the PB operator is enabled.  Jacobi damping (default
- * 6/7) and the number of levels: */
+ * refinement (V(2,2) with two Jacobi-type sweeps per side, exact FP64
+ * coarsest solve); falls back to mode 1 when the grid has no multigrid
+ * hierarchy (halving reaches an odd size above 12) or the PB operator is
+ * enabled.  One damping for both sweeps (plain damped Jacobi): */
 void solver_set_mg_omega(double omega);
 /* Different weights for the two smoothing sweeps of each level (pre-smoothing
- * omega1 then omega2, post-smoothing omega2 then omega1). */
+ * omega1 then omega2, post-smoothing omega2 then omega1).  Default: the
+ * degree-2 Chebyshev weights 0.5695 and 1.7319 for the Jacobi-scaled spectrum
+ * [1/3, 2]. */
 void solver_set_mg_omegas(double omega1, double omega2);
 int solver_mg_levels(void);
 

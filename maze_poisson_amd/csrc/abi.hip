@@ -210,8 +210,11 @@ void solver_set_y_extrapolation(int on) {
     S().sched.clear();
 }
 
-// Damping of the multigrid Jacobi smoother (default 6/7).
-void solver_set_mg_omega(double omega) { S().mg_omega = S().mg_omega2 = omega; }
+// One damping for both sweeps of the multigrid smoother (plain damped Jacobi).
+void solver_set_mg_omega(double omega) {
+    S().mg_omega = S().mg_omega2 = omega;
+    S().sched.clear();
+}
 
 // Two different sweep weights (a degree-2 polynomial smoother).
 void solver_set_mg_omegas(double omega1, double omega2) {

@@ -38,6 +38,18 @@ struct CGState {
     double otol;     // absolute stop of the outer loop
 };
 
+// What the host reads after a solve: the solver state and the charge sum of
+// the last spread, written by launch_publish_state into host-coherent memory
+// with `seq` last, so that the host can wait by polling seq instead of
+// synchronising the stream.
+struct StatePub {
+    CGState st;
+    double q_sum;
+    unsigned int seq;
+};
+void launch_publish_state(hipStream_t s, const CGState* st, const double* q_sum, StatePub* pub,
+                          unsigned int seq);
+
 // Face-centred dielectric maps and screening term of the PB operator; the
 // Poisson operator passes nullptrs.
 struct PBCoef {

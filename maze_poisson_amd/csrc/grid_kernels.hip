@@ -418,6 +418,27 @@ __global__ void k_scale_copy(long n, double c, const double* __restrict__ in, do
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         out[i] = in[i] * c;
 }
+// Copies the solver state and the charge sum to host-coherent memory, each
+// thread fencing its stores at system scope before the work-group barrier, then
+// publishes seq with a system-scope release store.
+__global__ __launch_bounds__(kWave) void k_publish_state(const CGState* st, const double* q_sum,
+                                                         StatePub* pub, unsigned int seq) {
+    static_assert(sizeof(CGState) % 4 == 0, "CGState is copied as 32-bit words");
+    const unsigned int* src = reinterpret_cast<const unsigned int*>(st);
+    unsigned int* dst = reinterpret_cast<unsigned int*>(&pub->st);
+    for (int i = threadIdx.x; i < (int)(sizeof(CGState) / 4); i += kWave) dst[i] = src[i];
+    if (threadIdx.x == 0) pub->q_sum = *q_sum;
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_publish_state(hipStream_t s, const CGState* st, const double* q_sum, StatePub* pub,
+                          unsigned int seq) {
+    MAZE_LAUNCH(k_publish_state, dim3(1), dim3(kWave), 0, s, st, q_sum, pub, seq);
+}
+
 __global__ void k_fill(long n, double v, double* out) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         out[i] = v;

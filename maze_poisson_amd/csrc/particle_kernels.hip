@@ -60,12 +60,24 @@ size_t sort_tmp_bytes(int np, int key_bits) {
     return bytes;
 }
 
-// Stable radix sort of (cell key, ion id): ions of one cell stay in id order.
+// row_start[r] = first sorted ion whose cell lies in row r = i n + j (keys of
+// row r are [r n, (r + 1) n)); row_start[n^2] = np.
+__global__ void k_row_start(int np, int n, const unsigned int* __restrict__ keys_sorted,
+                            int* __restrict__ row_start) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n * n) return;
+    row_start[r] = lower_bound_u32(keys_sorted, np, (unsigned int)r * (unsigned int)n);
+}
+
+// Stable radix sort of (cell key, ion id): ions of one cell stay in id order;
+// then the start of every cell row in the sorted order.
 void launch_cic_sort(hipStream_t s, const PDev& p, int key_bits) {
     if (p.np == 0) return;
     size_t bytes = p.sort_tmp_bytes;
     MAZE_HIP(hipcub::DeviceRadixSort::SortPairs(p.sort_tmp, bytes, p.keys, p.keys_sorted, p.ids,
                                                 p.ids_sorted, p.np, 0, key_bits, s));
+    MAZE_LAUNCH(k_row_start, dim3(blocks_for((long)p.n * p.n + 1)), dim3(kBlock), 0, s, p.np,
+                p.n, p.keys_sorted, p.row_start);
 }
 
 
@@ -137,7 +149,8 @@ template <int CAS>
 __global__ void k_assign_gather(int np, int n, double h, const double* __restrict__ pos,
                                 const double* __restrict__ charges, const int* __restrict__ base,
                                 const unsigned int* __restrict__ keys_sorted,
-                                const int* __restrict__ ids_sorted, double* __restrict__ q,
+                                const int* __restrict__ ids_sorted,
+                                const int* __restrict__ row_start, double* __restrict__ q,
                                 double* __restrict__ contrib) {
     using A = Assign<CAS>;
     constexpr int NN = A::kNodes, S = A::kSpan;
@@ -157,9 +170,10 @@ __global__ void k_assign_gather(int np, int n, double h, const double* __restric
         for (int b = 0; b < S; b++) {
             const int ci = (gi - (A::kLo + a) + 2 * n) % n;
             const int cj = (gj - (A::kLo + b) + 2 * n) % n;
-            const unsigned int row = ((unsigned int)ci * n + cj) * n;
-            const int r0 = lower_bound_u32(keys_sorted, np, row);
-            const int r1 = lower_bound_u32(keys_sorted, np, row + n);
+            const int ri = ci * n + cj;
+            const unsigned int row = (unsigned int)ri * n;
+            const int r0 = row_start[ri];
+            const int r1 = row_start[ri + 1];
             lo[a * S + b] = r0;
             hi[a * S + b] = r1;
             for (int s = r0; s < r1; s++) {
@@ -193,13 +207,16 @@ void launch_cic_gather(hipStream_t s, const PDev& p, double* q) {
     const int nb = blocks_for((long)p.nn * p.np);
     if (p.cas == 0)
         MAZE_LAUNCH(k_assign_gather<0>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
-                    p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
+                    p.charges, p.base, p.keys_sorted, p.ids_sorted, p.row_start, q,
+                    p.contrib);
     else if (p.cas == 1)
         MAZE_LAUNCH(k_assign_gather<1>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
-                    p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
+                    p.charges, p.base, p.keys_sorted, p.ids_sorted, p.row_start, q,
+                    p.contrib);
     else
         MAZE_LAUNCH(k_assign_gather<2>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
-                    p.charges, p.base, p.keys_sorted, p.ids_sorted, q, p.contrib);
+                    p.charges, p.base, p.keys_sorted, p.ids_sorted, p.row_start, q,
+                    p.contrib);
 }
 
 void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {

@@ -23,6 +23,7 @@ struct PDev {
     unsigned int* keys_sorted;
     int* ids;
     int* ids_sorted;
+    int* row_start;   // (n^2 + 1) first sorted ion of each cell row (i, j)
     double* table;    // Tosi-Fumi constants per type pair, 7 each
     double* tpart;    // Tosi-Fumi partials per 64-ion block pair (tf_part_doubles)
     double* eion;     // Tosi-Fumi energy per ion (each pair's energy on both ions)

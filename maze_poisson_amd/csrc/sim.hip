@@ -6,6 +6,7 @@
 // stop are no-ops (they test st->done first), so the state is exactly the
 // state at convergence.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -69,6 +70,9 @@ Sim::Sim() {
     MAZE_HIP(hipHostMalloc(&hscalar, 8 * sizeof(double)));
     ps = dalloc<PScalars>(1);
     MAZE_HIP(hipHostMalloc(&ps_host, sizeof(PScalars)));
+    MAZE_HIP(hipHostMalloc(&pub_host, sizeof(StatePub), hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(pub_host, 0, sizeof(StatePub));
+    MAZE_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&pub_dev), pub_host, 0));
 }
 
 Sim::~Sim() {
@@ -84,6 +88,7 @@ Sim::~Sim() {
     if (st_host) MAZE_HIP(hipHostFree(st_host));
     if (hscalar) MAZE_HIP(hipHostFree(hscalar));
     if (ps_host) MAZE_HIP(hipHostFree(ps_host));
+    if (pub_host) MAZE_HIP(hipHostFree(pub_host));
     for (auto e : ev_pool) MAZE_HIP(hipEventDestroy(e));
     MAZE_HIP(hipStreamDestroy(stream));
 }
@@ -111,6 +116,7 @@ void Sim::free_particles() {
     dfree(p.keys_sorted);
     dfree(p.ids);
     dfree(p.ids_sorted);
+    dfree(p.row_start);
     dfree(p.counts);
     if (p.sort_tmp) MAZE_HIP(hipFree(p.sort_tmp));
     p.sort_tmp = nullptr;
@@ -207,6 +213,7 @@ void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, int ca
     p.base = dalloc<int>(3L * np);
     p.keys = dalloc<unsigned int>(np); p.keys_sorted = dalloc<unsigned int>(np);
     p.ids = dalloc<int>(np); p.ids_sorted = dalloc<int>(np);
+    p.row_start = dalloc<int>((long)n_ * n_ + 1);
     p.contrib = dalloc<double>((long)p.nn * np);
     p.tpart = dalloc<double>(tf_part_doubles(np));
     p.eion = dalloc<double>(np);
@@ -334,9 +341,30 @@ void Sim::init_integrator(double dt_, double T_, double gamma, int type, int en)
     }
 }
 
+// The solver state (and the last charge sum) on the host.  A one-wave kernel
+// writes them to host-coherent memory with a sequence number last; the host
+// polls the number instead of synchronising the stream, which saves the
+// wake-up of a blocking wait on the critical path of every solve.  Should the
+// number not arrive within a second, the stream is synchronised and the
+// number checked once more.
 void Sim::read_state() {
-    MAZE_HIP(hipMemcpyAsync(st_host, st, sizeof(CGState), hipMemcpyDeviceToHost, stream));
-    sync();
+    const unsigned int want = ++pub_seq;
+    launch_publish_state(stream, st, &ps->q_sum, pub_dev, want);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long spin = 0;; spin++) {
+        if (__atomic_load_n(&pub_host->seq, __ATOMIC_ACQUIRE) == want) break;
+        if ((spin & 4095) == 4095 &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+            sync();
+            if (__atomic_load_n(&pub_host->seq, __ATOMIC_ACQUIRE) != want)
+                fatal("maze: the solver state was not published");
+            break;
+        }
+        __builtin_ia32_pause();
+    }
+    std::memcpy(st_host, &pub_host->st, sizeof(CGState));
+    ps_host->q_sum = pub_host->q_sum;
+    pub_count++;
 }
 
 hipEvent_t Sim::next_event() {
@@ -794,16 +822,16 @@ int Sim::mg_update(bool maze) {
 }
 
 // library/src/solver_md.c:91-107.  defer: the charge-conservation check
-// is made by check_charge() after the next host round trip instead of here.
+// is made by check_charge() after the next read_state() instead of here.
 int Sim::update_charges(bool defer) {
     launch_cic_base(stream, p);
     launch_cic_sort(stream, p, key_bits);
     launch_fill(stream, G, 0.0, q);
     launch_cic_gather(stream, p, q);
     launch_contrib_sum(stream, p, ps, red);
+    if (defer) return 0;  // read_state() brings q_sum with the solver state
     MAZE_HIP(hipMemcpyAsync(&ps_host->q_sum, &ps->q_sum, sizeof(double), hipMemcpyDeviceToHost,
                             stream));
-    if (defer) return 0;
     sync();
     return check_charge();
 }
@@ -893,8 +921,9 @@ void Sim::md_step() {
     part1();
     update_charges(true);
     update_eps_k2();
+    const unsigned long reads = pub_count;
     update_field();
-    sync();
+    if (pub_count == reads) read_state();  // every solve reads its state; kept for safety
     check_charge();
     check_pb_flag();
     forces_elec();

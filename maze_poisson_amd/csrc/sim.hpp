@@ -92,6 +92,10 @@ struct Sim {
 
     CGState* st = nullptr;      // device
     CGState* st_host = nullptr; // pinned
+    StatePub* pub_host = nullptr;  // host-coherent, written by the GPU (read_state)
+    StatePub* pub_dev = nullptr;   // its device address
+    unsigned int pub_seq = 0;
+    unsigned long pub_count = 0;   // read_state calls so far
     Reduce red{};
     double* dscalar = nullptr;  // device scratch scalar
     double* hscalar = nullptr;  // pinned
