@@ -265,6 +265,10 @@ int maze_test_mg_pcg(const double *b, double *x, double rtol, int n);
  *       full-weighting restriction of b - A d2 ((n/2)^3 values);
  * up:   d = two damped-Jacobi sweeps on A d = b from d2 + trilinear(e). */
 int maze_test_mg_down(const float *b, float *d2, float *bc, int n, double c, int tile);
+int maze_test_mg_down2(const float *b, float *d2, float *bc, int n, double c1, double c2,
+                       int tile);
+int maze_test_mg_up2(const float *b, const float *d2, const float *e, float *d, int n, double c1,
+                     double c2, int tile);
 int maze_test_mg_up(const float *b, const float *d2, const float *e, float *d, int n, double c,
                     int tile);
 /* The one-work-group coarse tail of the V-cycle set up for an n^3 grid, on the

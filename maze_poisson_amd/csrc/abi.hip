@@ -412,8 +412,10 @@ int maze_test_prolong_add(const double* in, double* out, int nc) {
     return 0;
 }
 
-// Fused multigrid level kernels on FP32 host buffers (n even).
-int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c, int tile) {
+// Fused multigrid level kernels on FP32 host buffers (n even), sweep weights
+// c1 then c2 (the *2 forms) or c for both.
+int maze_test_mg_down2(const float* b, float* d2, float* bc, int n, double c1, double c2,
+                       int tile) {
     if (n < 2 || n % 2 != 0 || tile >= maze::kMgTiles) return -1;
     if (tile < 0) tile = maze::mg_default_tile(n);
     Sim* s = scratch_sim(n);
@@ -423,7 +425,7 @@ int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c, int
     MAZE_HIP(hipMalloc(&dd, G * sizeof(float)));
     MAZE_HIP(hipMalloc(&dc, Gc * sizeof(float)));
     MAZE_HIP(hipMemcpy(db, b, G * sizeof(float), hipMemcpyHostToDevice));
-    maze::launch_mg_down(s->stream, n, tile, db, dd, dc, c, c, s->st);
+    maze::launch_mg_down(s->stream, n, tile, db, dd, dc, c1, c2, s->st);
     s->sync();
     MAZE_HIP(hipMemcpy(d2, dd, G * sizeof(float), hipMemcpyDeviceToHost));
     MAZE_HIP(hipMemcpy(bc, dc, Gc * sizeof(float), hipMemcpyDeviceToHost));
@@ -432,8 +434,12 @@ int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c, int
     return 0;
 }
 
-int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, int n, double c,
-                    int tile) {
+int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c, int tile) {
+    return maze_test_mg_down2(b, d2, bc, n, c, c, tile);
+}
+
+int maze_test_mg_up2(const float* b, const float* d2, const float* e, float* d, int n, double c1,
+                     double c2, int tile) {
     if (n < 2 || n % 2 != 0 || tile >= maze::kMgTiles) return -1;
     if (tile < 0) tile = maze::mg_default_tile(n);
     Sim* s = scratch_sim(n);
@@ -446,12 +452,17 @@ int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, i
     MAZE_HIP(hipMemcpy(db, b, G * sizeof(float), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(dd2, d2, G * sizeof(float), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(de, e, Gc * sizeof(float), hipMemcpyHostToDevice));
-    maze::launch_mg_up(s->stream, n, tile, db, dd2, de, dd, c, c, s->st, 0, s->red);
+    maze::launch_mg_up(s->stream, n, tile, db, dd2, de, dd, c1, c2, s->st, 0, s->red);
     s->sync();
     MAZE_HIP(hipMemcpy(d, dd, G * sizeof(float), hipMemcpyDeviceToHost));
     for (float* a : {db, dd2, de, dd}) MAZE_HIP(hipFree(a));
     delete s;
     return 0;
+}
+
+int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, int n, double c,
+                    int tile) {
+    return maze_test_mg_up2(b, d2, e, d, n, c, c, tile);
 }
 
 // The one-work-group tail of the V-cycle as set up for an n^3 fine grid: from

@@ -78,6 +78,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
+// Work-group barrier that orders LDS only: the fences are restricted to the
+// local address space, so the barrier waits for this wave's LDS operations
+// (lgkmcnt) but not for its outstanding global loads and stores, which a
+// __syncthreads() (workgroup fences over all address spaces) drains.  For
+// kernels whose barriers only protect LDS and whose global loads are issued
+// ahead of their use.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // x^3 rounded once (the reference's pow(x, 3)): x*x split exactly into
 // hi + lo by an FMA, then hi*x + lo*x with one final rounding.
 __device__ __forceinline__ double cube(double x) {

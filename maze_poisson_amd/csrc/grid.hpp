@@ -154,9 +154,14 @@ enum MgTile {
     kMgT8x8x16_512,
     kMgT4x8x8_512,
     kMgT4x8x8_256,
+    kMgMarch16,  // plane march, 16x16 columns, 16 planes per work-group (n % 16 == 0)
+    kMgMarch8,   // plane march, 16x16 columns, 8 planes per work-group (n % 16 == 0)
     kMgTiles
 };
 int mg_default_tile(int n);
+// Whether `tile` can run a level of n^3 points (the launchers fall back to the
+// default tile otherwise).
+bool mg_tile_ok(int tile, int n);
 const char* mg_tile_name(int tile);
 // Smoother weights c1, c2 (damping / 6): pre-smoothing sweeps c1 then c2,
 // post-smoothing c2 then c1 (polynomials in A, so the cycle stays symmetric).

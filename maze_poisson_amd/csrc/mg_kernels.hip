@@ -256,12 +256,12 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
     if (tid < BI) gi[tid] = wrap_mod(T.i0 - 3 + tid, n) * n2;
     if (tid >= 64 && tid < 64 + BJ) gj[tid - 64] = wrap_mod(T.j0 - 3 + tid - 64, n) * n;
     if (tid >= 128 && tid < 128 + BK) gk[tid - 128] = wrap_mod(T.k0 - 3 + tid - 128, n);
-    __syncthreads();
+    lds_barrier();
     tile_for<BI * BJ * BK, NT>([&](int x) {
         const int k = x % BK, j = (x / BK) % BJ, i = x / (BK * BJ);
         sb[x] = b[gi[i] + gj[j] + gk[k]];
     });
-    __syncthreads();
+    lds_barrier();
     // Sweeps from zero with weights c1 then c2: d2 = -(c1 + c2) b - c1 c2 A b.
     const float cs = c1 + c2, cp = c1 * c2;
     tile_for<DI * DJ * DK, NT>([&](int x) {
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
                           sb[o + 1] - 6.0f * bc;
         sd[x] = -cs * bc - cp * lap;
     });
-    __syncthreads();
+    lds_barrier();
     tile_for<RI * RJ * RK, NT>([&](int x) {
         const int k = x % RK, j = (x / RK) % RJ, i = x / (RK * RJ);
         const int o = ((i + 1) * DJ + (j + 1)) * DK + (k + 1);
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
             d2out[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] =
                 sd[((i + 2) * DJ + (j + 2)) * DK + (k + 2)];
     });
-    __syncthreads();
+    lds_barrier();
     const int nc = n / 2;
     tile_for<CI * CJ * CK, NT>([&](int x) {
         const int K = x % CK, J = (x / CK) % CJ, I = x / (CK * CJ);
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
     if (tid >= 128 && tid < 128 + EI) ci[tid - 128] = wrap_mod(T.i0 / 2 - 1 + tid - 128, nc) * nc * nc;
     if (tid >= 160 && tid < 160 + EJ) cj[tid - 160] = wrap_mod(T.j0 / 2 - 1 + tid - 160, nc) * nc;
     if (tid >= 192 && tid < 192 + EK) ck[tid - 192] = wrap_mod(T.k0 / 2 - 1 + tid - 192, nc);
-    __syncthreads();
+    lds_barrier();
     tile_for<DI * DJ * DK, NT>([&](int x) {
         const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
         sd[x] = d2[gi[i] + gj[j] + gk[k]];
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
         const int k = x % EK, j = (x / EK) % EJ, i = x / (EK * EJ);
         se[x] = e[ci[i] + cj[j] + ck[k]];
     });
-    __syncthreads();
+    lds_barrier();
     // d = d2 + trilinear(e): fine local index l has coarse neighbours l>>1 and
     // (l>>1)+1 and weight 1/2 on odd l (tile origins are even).
     tile_for<DI * DJ * DK, NT>([&](int x) {
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
         const float v1 = v10 + wj * (v11 - v10);
         sd[x] = sd[x] + (v0 + wi * (v1 - v0));
     });
-    __syncthreads();
+    lds_barrier();
     tile_for<PI * PJ * PK, NT>([&](int x) {
         const int k = x % PK, j = (x / PK) % PJ, i = x / (PK * PJ);
         const int o = ((i + 1) * DJ + (j + 1)) * DK + (k + 1);
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
                           sd[o + 1] - 6.0f * dc;
         sp[x] = dc - c2 * (sb[x] - lap);
     });
-    __syncthreads();
+    lds_barrier();
     double s = 0.0;
     tile_for<TI * TJ * TK, NT>([&](int x) {
         const int k = x % TK, j = (x / TK) % TJ, i = x / (TK * TJ);
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
 }
 
 // Tile shapes (fine points per work-group, threads) of the fused level kernels;
-// Sim picks one per level by timing them (mg_autotune) or takes the default.
+// Sim takes the default per level (mg_default_tile) unless MAZE_MG_TILES names one.
 template <int TI, int TJ, int TK, int NT>
 static void down_t(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
                    double c2, CGState* st) {
@@ -423,9 +423,311 @@ static void up_t(hipStream_t s, int n, const float* b, const float* d2, const fl
                     s, n, b, d2, e, d, (float)c1, (float)c2, red, st);
 }
 
+// ---- plane-marching level kernels ---------------------------------------------
+// The same down / up steps as the tiles above, with the same arithmetic in the
+// same order (so the outputs are bit-identical), organised as a march along i:
+// a 256-thread work-group owns a 16x16 column block of the (j, k) plane over S
+// planes and keeps the planes the stencils need in LDS rings of three, each
+// plane with its (j, k) halo.  Each plane's global loads are issued one plane
+// ahead into registers, so loads stay in flight during the plane's LDS steps,
+// and the halo is only in (j, k): 1.6x the points of the column block instead
+// of 2.3x for an 8x16x16 tile.  Needs n a multiple of 16 and of S.
+constexpr int kMarchC = 16;
+
+__device__ __forceinline__ int ring3(int q) { return ((q % 3) + 3) % 3; }
+
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_mg_down_march(int n, const float* __restrict__ b,
+                                                          float* __restrict__ d2out,
+                                                          float* __restrict__ bcoarse, float c1,
+                                                          float c2, const CGState* st) {
+    if (st->done) return;
+    constexpr int WB = kMarchC + 6, WD = kMarchC + 4, WR = kMarchC + 2;
+    constexpr int PB = WB * WB, PD = WD * WD, PR = WR * WR;
+    static_assert(PB <= 2 * kBlock, "two B-plane points per thread");
+    __shared__ float sb[3][PB], sd[3][PD], sr[3][PR];
+    const int tid = threadIdx.x;
+    const int nblk = n / kMarchC, n2 = n * n, nc = n / 2;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int cb = blk % (nblk * nblk), i0 = (blk / (nblk * nblk)) * S;
+    const int j0 = (cb / nblk) * kMarchC, k0 = (cb % nblk) * kMarchC;
+    int boff[2];
+    bool bok[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int r = tid + u * kBlock;
+        bok[u] = r < PB;
+        const int jj = r / WB, kk = r % WB;
+        boff[u] = bok[u] ? wrap_mod(j0 - 3 + jj, n) * n + wrap_mod(k0 - 3 + kk, n) : 0;
+    }
+    // Global loads run three planes ahead of the plane being stored: register
+    // sets ra, rb, rc rotate (loop unrolled by three), so each plane's loads
+    // have three planes' worth of LDS work to arrive.
+    auto load_b = [&](int q, float (&v)[2]) {
+        const int pl = wrap_mod(q, n) * n2;
+#pragma unroll
+        for (int u = 0; u < 2; u++) v[u] = bok[u] ? b[pl + boff[u]] : 0.0f;
+    };
+    const float cs = c1 + c2, cp = c1 * c2;
+    constexpr int U = S + 5;  // B planes i0-3 .. i0+S+1
+    // B planes i0-3 .. i0+S+1, d2 planes i0-2 .. i0+S, residual planes i0-1 .. i0+S-1.
+    auto step = [&](int u, float (&nb)[2]) {
+        if (u >= U) return;
+        const int qb = i0 - 3 + u;
+        float* B = sb[ring3(qb)];
+#pragma unroll
+        for (int v = 0; v < 2; v++)
+            if (bok[v]) B[tid + v * kBlock] = nb[v];
+        if (u + 3 < U) load_b(qb + 3, nb);
+        lds_barrier();
+        if (u >= 2) {
+            const int qd = qb - 1;
+            const float* Bm = sb[ring3(qd - 1)];
+            const float* B0 = sb[ring3(qd)];
+            const float* Bp = sb[ring3(qd + 1)];
+            float* D = sd[ring3(qd)];
+            const bool own = qd >= i0 && qd < i0 + S;
+#pragma unroll 2
+            for (int r = tid; r < PD; r += kBlock) {
+                const int jj = r / WD, kk = r % WD;
+                const int o = (jj + 1) * WB + (kk + 1);
+                const float bc = B0[o];
+                const float lap = Bm[o] + Bp[o] + B0[o - WB] + B0[o + WB] + B0[o - 1] + B0[o + 1] -
+                                  6.0f * bc;
+                const float v = -cs * bc - cp * lap;
+                D[r] = v;
+                if (own && jj >= 2 && jj < WD - 2 && kk >= 2 && kk < WD - 2)
+                    d2out[qd * n2 + (j0 + jj - 2) * n + k0 + kk - 2] = v;
+            }
+        }
+        lds_barrier();
+        if (u >= 4) {
+            const int qr = qb - 2;
+            const float* Dm = sd[ring3(qr - 1)];
+            const float* D0 = sd[ring3(qr)];
+            const float* Dp = sd[ring3(qr + 1)];
+            const float* B0 = sb[ring3(qr)];
+            float* R = sr[ring3(qr)];
+#pragma unroll 2
+            for (int r = tid; r < PR; r += kBlock) {
+                const int jj = r / WR, kk = r % WR;
+                const int o = (jj + 1) * WD + (kk + 1);
+                const float dc = D0[o];
+                const float lap = Dm[o] + Dp[o] + D0[o - WD] + D0[o + WD] + D0[o - 1] + D0[o + 1] -
+                                  6.0f * dc;
+                R[r] = B0[(jj + 2) * WB + (kk + 2)] - lap;
+            }
+            lds_barrier();
+            // Coarse plane Q of this slab from residual planes 2Q-1, 2Q, 2Q+1 =
+            // qr-2, qr-1, qr (qr odd relative to the even i0, and qr > i0 so that
+            // all three planes were computed here).
+            if (qr > i0 && ((qr - i0) & 1) && tid < (kMarchC / 2) * (kMarchC / 2)) {
+                const int Q = (qr - 1) / 2;
+                const int J = tid / (kMarchC / 2), K = tid % (kMarchC / 2);
+                const int o = (2 * J + 1) * WR + (2 * K + 1);
+                const float* Rp[3] = {sr[ring3(qr - 2)], sr[ring3(qr - 1)], sr[ring3(qr)]};
+                float acc = 0.0f;
+#pragma unroll
+                for (int a = -1; a <= 1; a++)
+#pragma unroll
+                    for (int bb = -1; bb <= 1; bb++)
+#pragma unroll
+                        for (int cc = -1; cc <= 1; cc++) {
+                            const float w =
+                                (float)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
+                            acc += w * Rp[a + 1][o + bb * WR + cc];
+                        }
+                bcoarse[(Q * nc + j0 / 2 + J) * nc + k0 / 2 + K] = acc * (4.0f / 64.0f);
+            }
+        }
+    };
+    float ra[2], rb[2], rc[2];
+    load_b(i0 - 3, ra);
+    load_b(i0 - 2, rb);
+    load_b(i0 - 1, rc);
+    for (int u = 0; u < U; u += 3) {
+        step(u, ra);
+        step(u + 1, rb);
+        step(u + 2, rc);
+    }
+}
+
+template <int S, bool Dot>
+__global__ __launch_bounds__(kBlock) void k_mg_up_march(int n, const float* __restrict__ b,
+                                                        const float* __restrict__ d2,
+                                                        const float* __restrict__ e,
+                                                        float* __restrict__ dout, float c1,
+                                                        float c2, Reduce red, CGState* st) {
+    if (st->done) return;
+    constexpr int WD = kMarchC + 4, WT = kMarchC + 2, WE = kMarchC / 2 + 3;
+    constexpr int PD = WD * WD, PT = WT * WT, PE = WE * WE;
+    static_assert(PD <= 2 * kBlock && PE <= kBlock, "plane points per thread");
+    __shared__ float sd[3][PD], st_[3][PT], sbp[3][PT], se[3][PE];
+    const int tid = threadIdx.x;
+    const int nblk = n / kMarchC, n2 = n * n, nc = n / 2;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int cb = blk % (nblk * nblk), i0 = (blk / (nblk * nblk)) * S;
+    const int j0 = (cb / nblk) * kMarchC, k0 = (cb % nblk) * kMarchC;
+    int doff[2], toff[2];
+    bool dok[2], tok[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int r = tid + u * kBlock;
+        dok[u] = r < PD;
+        doff[u] = dok[u] ? wrap_mod(j0 - 2 + r / WD, n) * n + wrap_mod(k0 - 2 + r % WD, n) : 0;
+        tok[u] = r < PT;
+        toff[u] = tok[u] ? wrap_mod(j0 - 1 + r / WT, n) * n + wrap_mod(k0 - 1 + r % WT, n) : 0;
+    }
+    const bool eok = tid < PE;
+    const int eoff = eok ? wrap_mod((j0 - 2) / 2 + tid / WE, nc) * nc +
+                               wrap_mod((k0 - 2) / 2 + tid % WE, nc)
+                         : 0;
+    // Registers of a plane three planes ahead (sets ra, rb, rc rotate, loop
+    // unrolled by three): d2 plane q, b plane q - 1 and, when q is even, coarse
+    // plane q / 2 + 1.
+    struct Regs {
+        float d[2], b[2], e;
+    };
+    auto load = [&](int q, Regs& g) {
+        const int pd = wrap_mod(q, n) * n2, pb = wrap_mod(q - 1, n) * n2;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            g.d[u] = dok[u] ? d2[pd + doff[u]] : 0.0f;
+            g.b[u] = tok[u] ? b[pb + toff[u]] : 0.0f;
+        }
+        g.e = ((q & 1) == 0 && eok) ? e[wrap_mod((q >> 1) + 1, nc) * nc * nc + eoff] : 0.0f;
+    };
+    // The first D plane (even) also needs coarse plane q >> 1.
+    if (eok) se[ring3(((i0 - 2) >> 1))][tid] = e[wrap_mod((i0 - 2) >> 1, nc) * nc * nc + eoff];
+    double s = 0.0;
+    constexpr int U = S + 4;
+    // D planes i0-2 .. i0+S+1, T planes i0-1 .. i0+S, Z planes i0 .. i0+S-1.
+    auto step = [&](int u, Regs& g) {
+        if (u >= U) return;
+        const int q = i0 - 2 + u;
+        const float dreg[2] = {g.d[0], g.d[1]};
+        float* Bq = sbp[ring3(q - 1)];
+#pragma unroll
+        for (int v = 0; v < 2; v++)
+            if (tok[v]) Bq[tid + v * kBlock] = g.b[v];
+        if ((q & 1) == 0 && eok) se[ring3((q >> 1) + 1)][tid] = g.e;
+        if (u + 3 < U) load(q + 3, g);
+        lds_barrier();
+        {  // D plane q = d2 + trilinear(e)
+            const int a = q >> 1;
+            const float wi = (q & 1) ? 0.5f : 0.0f;
+            const float* e0 = se[ring3(a)];
+            const float* e1 = se[ring3(a + 1)];
+            float* D = sd[ring3(q)];
+#pragma unroll
+            for (int v = 0; v < 2; v++) {
+                if (!dok[v]) continue;
+                const int r = tid + v * kBlock;
+                const int j = r / WD, k = r % WD;
+                const int bb = j >> 1, cc = k >> 1;
+                const float wj = (j & 1) ? 0.5f : 0.0f, wk = (k & 1) ? 0.5f : 0.0f;
+                const int o = bb * WE + cc;
+                const float e000 = e0[o], e001 = e0[o + 1], e010 = e0[o + WE], e011 = e0[o + WE + 1];
+                const float e100 = e1[o], e101 = e1[o + 1];
+                const float e110 = e1[o + WE], e111 = e1[o + WE + 1];
+                const float v00 = e000 + wk * (e001 - e000);
+                const float v01 = e010 + wk * (e011 - e010);
+                const float v10 = e100 + wk * (e101 - e100);
+                const float v11 = e110 + wk * (e111 - e110);
+                const float v0 = v00 + wj * (v01 - v00);
+                const float v1 = v10 + wj * (v11 - v10);
+                D[r] = dreg[v] + (v0 + wi * (v1 - v0));
+            }
+        }
+        lds_barrier();
+        if (u >= 2) {  // T plane q - 1: first sweep (weight c2)
+            const int qt = q - 1;
+            const float* Dm = sd[ring3(qt - 1)];
+            const float* D0 = sd[ring3(qt)];
+            const float* Dp = sd[ring3(qt + 1)];
+            const float* B0 = sbp[ring3(qt)];
+            float* T = st_[ring3(qt)];
+#pragma unroll 2
+            for (int r = tid; r < PT; r += kBlock) {
+                const int jj = r / WT, kk = r % WT;
+                const int o = (jj + 1) * WD + (kk + 1);
+                const float dc = D0[o];
+                const float lap = Dm[o] + Dp[o] + D0[o - WD] + D0[o + WD] + D0[o - 1] + D0[o + 1] -
+                                  6.0f * dc;
+                T[r] = dc - c2 * (B0[r] - lap);
+            }
+        }
+        lds_barrier();
+        if (u >= 4) {  // Z plane q - 2: second sweep (weight c1), written out
+            const int qz = q - 2;
+            const float* Tm = st_[ring3(qz - 1)];
+            const float* T0 = st_[ring3(qz)];
+            const float* Tp = st_[ring3(qz + 1)];
+            const float* B0 = sbp[ring3(qz)];
+            const int jj = tid / kMarchC, kk = tid % kMarchC;
+            const int o = (jj + 1) * WT + (kk + 1);
+            const float pc = T0[o];
+            const float lap = Tm[o] + Tp[o] + T0[o - WT] + T0[o + WT] + T0[o - 1] + T0[o + 1] -
+                              6.0f * pc;
+            const float bc = B0[o];
+            const float z = pc - c1 * (bc - lap);
+            dout[qz * n2 + (j0 + jj) * n + k0 + kk] = z;
+            if (Dot) s += (double)bc * (double)z;
+        }
+    };
+    Regs ra, rb, rc;
+    load(i0 - 2, ra);
+    load(i0 - 1, rb);
+    load(i0, rc);
+    for (int u = 0; u < U; u += 3) {
+        step(u, ra);
+        step(u + 1, rb);
+        step(u + 2, rc);
+    }
+    if constexpr (Dot) {
+        struct {
+            double s, m;
+            __device__ void finalize(CGState* q, double S_, double) const {
+                q->beta = q->first ? 0.0 : S_ / q->rdv;
+                q->rdv = S_;
+            }
+        } bd{s, 0.0};
+        reduce_and_finalize<false, decltype(bd), kBlock>(bd, red, st);
+    }
+}
+
+bool mg_tile_ok(int tile, int n) {
+    if (tile < 0 || tile >= kMgTiles) return false;
+    if (tile == kMgMarch16) return n >= 32 && n % 16 == 0;
+    if (tile == kMgMarch8) return n >= 32 && n % 16 == 0;
+    return true;
+}
+
+template <int S>
+static void down_m(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
+                   double c2, CGState* st) {
+    const int nblk = n / kMarchC;
+    MAZE_LAUNCH((k_mg_down_march<S>), dim3(nblk * nblk * (n / S)), dim3(kBlock), 0, s, n, b, d2,
+                bc, (float)c1, (float)c2, st);
+}
+
+template <int S>
+static void up_m(hipStream_t s, int n, const float* b, const float* d2, const float* e, float* d,
+                 double c1, double c2, CGState* st, int dot, Reduce red) {
+    const int nblk = n / kMarchC;
+    const dim3 g(nblk * nblk * (n / S));
+    if (dot)
+        MAZE_LAUNCH((k_mg_up_march<S, true>), g, dim3(kBlock), 0, s, n, b, d2, e, d, (float)c1,
+                    (float)c2, red, st);
+    else
+        MAZE_LAUNCH((k_mg_up_march<S, false>), g, dim3(kBlock), 0, s, n, b, d2, e, d, (float)c1,
+                    (float)c2, red, st);
+}
+
 const char* mg_tile_name(int tile) {
     static const char* names[kMgTiles] = {"8x16x16/1024", "8x16x16/512", "8x8x16/512",
-                                          "4x8x8/512", "4x8x8/256"};
+                                          "4x8x8/512",    "4x8x8/256",   "march16x16/S16",
+                                          "march16x16/S8"};
     return tile >= 0 && tile < kMgTiles ? names[tile] : "?";
 }
 
@@ -436,7 +738,10 @@ int mg_default_tile(int n) { return n >= 64 ? kMgT8x16x16_1024 : kMgT4x8x8_512; 
 
 void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
                     double c1, double c2, CGState* st) {
+    if (!mg_tile_ok(tile, n)) tile = mg_default_tile(n);
     switch (tile) {
+        case kMgMarch16: down_m<16>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgMarch8: down_m<8>(s, n, b, d2, bc, c1, c2, st); break;
         case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st); break;
         case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
         case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
@@ -447,7 +752,10 @@ void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, f
 
 void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d2, const float* e,
                   float* d, double c1, double c2, CGState* st, int dot, Reduce red) {
+    if (!mg_tile_ok(tile, n)) tile = mg_default_tile(n);
     switch (tile) {
+        case kMgMarch16: up_m<16>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
+        case kMgMarch8: up_m<8>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
         case kMgT8x16x16_1024: up_t<8, 16, 16, 1024>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
         case kMgT8x16x16_512: up_t<8, 16, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
         case kMgT8x8x16_512: up_t<8, 8, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
@@ -472,42 +780,42 @@ __device__ void coarse_exact(int c, const double* Q, const double* __restrict__ 
         for (int m = 0; m < c; m++) s += Q[t * c + m] * A[line + m];
         B[x] = s;
     }
-    __syncthreads();
+    lds_barrier();
     for (int x = tid; x < N; x += nt) {  // forward j
         const int k = x % c, t = (x / c) % c, i = x / c2;
         double s = 0.0;
         for (int m = 0; m < c; m++) s += Q[t * c + m] * B[i * c2 + m * c + k];
         A[x] = s;
     }
-    __syncthreads();
+    lds_barrier();
     for (int x = tid; x < N; x += nt) {  // forward i, scale by the inverse eigenvalue
         const int jk = x % c2, t = x / c2;
         double s = 0.0;
         for (int m = 0; m < c; m++) s += Q[t * c + m] * A[m * c2 + jk];
         B[x] = s * ilam[x];
     }
-    __syncthreads();
+    lds_barrier();
     for (int x = tid; x < N; x += nt) {  // inverse i
         const int jk = x % c2, i = x / c2;
         double s = 0.0;
         for (int m = 0; m < c; m++) s += Q[m * c + i] * B[m * c2 + jk];
         A[x] = s;
     }
-    __syncthreads();
+    lds_barrier();
     for (int x = tid; x < N; x += nt) {  // inverse j
         const int k = x % c, j = (x / c) % c, i = x / c2;
         double s = 0.0;
         for (int m = 0; m < c; m++) s += Q[m * c + j] * A[i * c2 + m * c + k];
         B[x] = s;
     }
-    __syncthreads();
+    lds_barrier();
     for (int x = tid; x < N; x += nt) {  // inverse k
         const int k = x % c, line = x - k;
         double s = 0.0;
         for (int m = 0; m < c; m++) s += Q[m * c + k] * B[line + m];
         A[x] = s;
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 __global__ __launch_bounds__(kBlock) void k_mg_coarse(int c, const float* __restrict__ b,
@@ -603,7 +911,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 B[0][x] = bin[x];
             }
     }
-    __syncthreads();
+    lds_barrier();
     const float cs = c1 + c2, cp = c1 * c2;
     // ---- down: d2 = -(c1 + c2) b - c1 c2 A b, t = b - A d2, restrict t ----
     for (int l = 0; l < T.nlev - 1; l++) {
@@ -620,7 +928,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 const int x = i0 + col.j0 + col.k0;
                 d[x] = -cs * b[x] - cp * col.lap(b, im, i0, ip);
             }
-        __syncthreads();
+        lds_barrier();
         if (on)
             for (int i = ti; i < m; i += 4) {
                 const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
@@ -628,7 +936,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 const int x = i0 + col.j0 + col.k0;
                 t[x] = b[x] - col.lap(d, im, i0, ip);
             }
-        __syncthreads();
+        lds_barrier();
         const int M = m / 2;
         float* bc = B[l + 1];
         if (tj < M && tk < M) {
@@ -651,7 +959,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 bc[(I * M + tj) * M + tk] = acc * (4.0f / 64.0f);
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
     // ---- exact FP64 solve on the coarsest level ----
     {
@@ -665,10 +973,10 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
         double* Bd = A + N;
         for (int x = tid; x < cn * cn; x += kTailThreads) Q[x] = Qf[x];
         for (int x = tid; x < N; x += kTailThreads) A[x] = (double)B[L][x];
-        __syncthreads();
+        lds_barrier();
         coarse_exact(cn, Q, ilam, A, Bd, tid, kTailThreads);
         for (int x = tid; x < N; x += kTailThreads) D[L][x] = (float)A[x];
-        __syncthreads();
+        lds_barrier();
     }
     // ---- up: d += trilinear(e), sweeps d <- d - c (b - A d) with c2 then c1 ----
     for (int l = T.nlev - 2; l >= 0; l--) {
@@ -698,7 +1006,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 d[x] = d[x] + (v0 + wi * (v1 - v0));
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (on)
             for (int i = ti; i < m; i += 4) {
                 const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
@@ -706,7 +1014,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 const int x = i0 + col.j0 + col.k0;
                 t[x] = d[x] - c2 * (b[x] - col.lap(d, im, i0, ip));
             }
-        __syncthreads();
+        lds_barrier();
         if (on)
             for (int i = ti; i < m; i += 4) {
                 const int i0 = i * m2, im = (i == 0 ? m - 1 : i - 1) * m2,
@@ -714,7 +1022,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailLevels T, const fl
                 const int x = i0 + col.j0 + col.k0;
                 d[x] = t[x] - c1 * (b[x] - col.lap(t, im, i0, ip));
             }
-        __syncthreads();
+        lds_barrier();
     }
     {
         const int m = T.n[0], m2 = m * m;

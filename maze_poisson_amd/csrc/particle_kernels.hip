@@ -158,28 +158,34 @@ __global__ void k_assign_gather(int np, int n, double h, const double* __restric
     if (t >= (long)NN * np) return;
     const int i = (int)(t / NN), c = (int)(t % NN);
     const double L = n * h;
-    const int gi = (base[3 * i] + A::off(c, 0) + n) % n;
-    const int gj = (base[3 * i + 1] + A::off(c, 1) + n) % n;
-    const int gk = (base[3 * i + 2] + A::off(c, 2) + n) % n;
-    contrib[t] = assign_contrib<A>(pos, charges, i, gi, gj, gk, L, h);
+    // Indices stay within one period of [0, n): wrap by one conditional step.
+    auto wrap = [n](int v) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+    const int gi = wrap(base[3 * i] + A::off(c, 0));
+    const int gj = wrap(base[3 * i + 1] + A::off(c, 1));
+    const int gk = wrap(base[3 * i + 2] + A::off(c, 2));
+    const double mine = assign_contrib<A>(pos, charges, i, gi, gj, gk, L, h);
+    contrib[t] = mine;
 
     // Rows (ci, cj) of candidate base cells and their ranges in the sorted keys.
     int lo[S * S], hi[S * S];
+    unsigned int rows[S * S];
+    // Whether a base cell at k index ck covers node gk along k (node - base in
+    // [kLo, kLo + S), periodic).
+    auto covers_k = [&](int ck) { return wrap(wrap(gk - ck) - A::kLo) < S; };
     int owner = i;
     for (int a = 0; a < S; a++)
         for (int b = 0; b < S; b++) {
-            const int ci = (gi - (A::kLo + a) + 2 * n) % n;
-            const int cj = (gj - (A::kLo + b) + 2 * n) % n;
+            const int ci = wrap(gi - (A::kLo + a));
+            const int cj = wrap(gj - (A::kLo + b));
             const int ri = ci * n + cj;
             const unsigned int row = (unsigned int)ri * n;
             const int r0 = row_start[ri];
             const int r1 = row_start[ri + 1];
             lo[a * S + b] = r0;
             hi[a * S + b] = r1;
-            for (int s = r0; s < r1; s++) {
-                const int dk = (gk - (int)(keys_sorted[s] - row) + n) % n;  // node - base along k
-                if (((dk - A::kLo + n) % n) < S) owner = min(owner, ids_sorted[s]);
-            }
+            rows[a * S + b] = row;
+            for (int s = r0; s < r1; s++)
+                if (covers_k((int)(keys_sorted[s] - row))) owner = min(owner, ids_sorted[s]);
         }
     if (owner != i) return;
     // Sum contributions in increasing ion id (selection over the candidates).
@@ -189,14 +195,12 @@ __global__ void k_assign_gather(int np, int n, double h, const double* __restric
         int next = 0x7fffffff;
         for (int r = 0; r < S * S; r++)
             for (int s = lo[r]; s < hi[r]; s++) {
-                const unsigned int row = keys_sorted[s] - keys_sorted[s] % n;
-                const int dk = (gk - (int)(keys_sorted[s] - row) + n) % n;
-                if (((dk - A::kLo + n) % n) >= S) continue;
+                if (!covers_k((int)(keys_sorted[s] - rows[r]))) continue;
                 const int j = ids_sorted[s];
                 if (j > last && j < next) next = j;
             }
         if (next == 0x7fffffff) break;
-        acc += assign_contrib<A>(pos, charges, next, gi, gj, gk, L, h);
+        acc += next == i ? mine : assign_contrib<A>(pos, charges, next, gi, gj, gk, L, h);
         last = next;
     }
     q[((long)gi * n + gj) * n + gk] = acc;
@@ -327,7 +331,9 @@ __global__ __launch_bounds__(kTfB * kTfWaves) void k_tf_tiles(
     const bool diag = I == J;
     const int s0 = diag ? 1 : 0, s1 = diag ? 32 : 63;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, e = 0.0, cnt = 0.0;
-    // Every wave runs the same number of steps (the barrier is work-group wide).
+    // Waves share only the read-only j block; each accumulates reactions in its
+    // own LDS rows, and within a wave the lanes of one step touch distinct j and
+    // successive steps are in program order, so the steps need no barrier.
     const int nsteps = (s1 - s0 + kTfWaves) / kTfWaves;
     for (int k = 0; k < nsteps; k++) {
         const int s = s0 + w + k * kTfWaves;
@@ -372,7 +378,6 @@ __global__ __launch_bounds__(kTfB * kTfWaves) void k_tf_tiles(
                 acc[w][4][jj] += 1.0;
             }
         }
-        __syncthreads();
     }
     own[w][0][l] = f0;
     own[w][1][l] = f1;

@@ -115,6 +115,8 @@ def hip_raw(hip):
         "maze_test_mg_pcg": (ci, [D, D, cd, ci]),
         "maze_test_mg_down": (ci, [F, F, F, ci, cd, ci]),
         "maze_test_mg_up": (ci, [F, F, F, F, ci, cd, ci]),
+        "maze_test_mg_down2": (ci, [F, F, F, ci, cd, cd, ci]),
+        "maze_test_mg_up2": (ci, [F, F, F, F, ci, cd, cd, ci]),
         "maze_test_mg_tail": (ci, [F, F, ci]),
         "solver_set_mg_omega": (None, [cd]),
         "solver_mg_levels": (ci, []),

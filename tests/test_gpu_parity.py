@@ -286,7 +286,8 @@ def _prolong_trilinear(e):
     return e
 
 
-MG_TILE_CASES = [(n, -1) for n in (8, 16, 20, 64, 72)] + [(n, t) for n in (20, 64) for t in range(5)]
+MG_TILE_CASES = ([(n, -1) for n in (8, 16, 20, 64, 72)] + [(n, t) for n in (20, 64) for t in range(5)]
+                 + [(n, t) for n in (32, 64) for t in (5, 6)])  # 5, 6: plane-marching kernels
 
 
 @pytest.mark.parametrize("n,tile", MG_TILE_CASES)
@@ -535,3 +536,30 @@ def test_md_sequence_pb_radii(hip, oracle, method):
     for k in ("fcs_elec", "fcs_db", "fcs_ib", "fcs_np", "fcs_tot", "field"):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
     assert rel(g["pos"], o["pos"]) <= 1e-11
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+@pytest.mark.parametrize("march", [5, 6])
+def test_mg_march_bitwise_equals_tiles(hip_raw, n, march):
+    """The plane-marching level kernels compute every value with the tile kernels'
+    arithmetic in the same order: down (d2 and the coarse right-hand side) and up agree
+    bit for bit with the 8x16x16 tiles, with the Chebyshev sweep weights."""
+    c1, c2 = (w / 6.0 for w in chebyshev_weights())
+    g = np.random.default_rng(500 + n)
+    b = g.standard_normal((n, n, n)).astype(np.float32)
+    out = {}
+    for tile in (0, march):
+        d2 = np.zeros_like(b)
+        bc = np.zeros((n // 2,) * 3, np.float32)
+        assert hip_raw.maze_test_mg_down2(b, d2, bc, n, c1, c2, tile) == 0
+        out[tile] = (d2, bc)
+    assert np.array_equal(out[0][0], out[march][0])
+    assert np.array_equal(out[0][1], out[march][1])
+    d2 = g.standard_normal((n, n, n)).astype(np.float32)
+    e = g.standard_normal((n // 2,) * 3).astype(np.float32)
+    up = {}
+    for tile in (0, march):
+        d = np.zeros_like(b)
+        assert hip_raw.maze_test_mg_up2(b, d2, e, d, n, c1, c2, tile) == 0
+        up[tile] = d
+    assert np.array_equal(up[0], up[march])
