@@ -302,8 +302,9 @@ constexpr int kTfLdsTypes = 4;
 template <bool LdsTab>
 __global__ __launch_bounds__(kTfB * kTfWaves) void k_tf_tiles(
     int np, int n_typ, double L, double invL, double r_cut, const double* __restrict__ pos,
-    const int* __restrict__ types, const double* __restrict__ table, double* __restrict__ part) {
-    const int J = blockIdx.x, I = blockIdx.y, nb = gridDim.x;
+    const int* __restrict__ types, const double* __restrict__ table, double* __restrict__ part,
+    int i_off) {
+    const int J = blockIdx.x, I = blockIdx.y + i_off, nb = gridDim.x;
     if (I > J) return;
     __shared__ double sx[kTfB], sy[kTfB], sz[kTfB];
     __shared__ int styp[kTfB];
@@ -442,21 +443,32 @@ long tf_part_doubles(int np) {
     return nb * (nb + 1) / 2 * 5 * 2 * kTfB;
 }
 
-void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
+void launch_tf_rows(hipStream_t s, const PDev& p, int i_lo, int i_hi) {
+    const int nb = tf_blocks(p.np);
+    if (p.np == 0 || i_hi <= i_lo) return;
+    const dim3 g(nb, i_hi - i_lo);
+    if (p.n_typ <= kTfLdsTypes)
+        MAZE_LAUNCH(k_tf_tiles<true>, g, dim3(kTfB * kTfWaves), 0, s, p.np, p.n_typ, p.L,
+                    1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart, i_lo);
+    else
+        MAZE_LAUNCH(k_tf_tiles<false>, g, dim3(kTfB * kTfWaves), 0, s, p.np, p.n_typ, p.L,
+                    1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart, i_lo);
+}
+
+void launch_tf_finish(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
     if (p.np == 0) {
         launch_sum_scaled(s, 0, 0.5, p.eion, &ps->pe_tf, red);
         return;
     }
     const int nb = tf_blocks(p.np);
-    if (p.n_typ <= kTfLdsTypes)
-        MAZE_LAUNCH(k_tf_tiles<true>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
-                           p.n_typ, p.L, 1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart);
-    else
-        MAZE_LAUNCH(k_tf_tiles<false>, dim3(nb, nb), dim3(kTfB * kTfWaves), 0, s, p.np,
-                           p.n_typ, p.L, 1.0 / p.L, p.r_cut, p.pos, p.types, p.table, p.tpart);
     MAZE_LAUNCH(k_tf_fold, dim3(nb), dim3(kTfB * kFoldGroups), 0, s, p.np, nb, p.tpart,
                        p.fn, p.eion, p.counts);
     launch_sum_scaled(s, p.np, 0.5, p.eion, &ps->pe_tf, red);
+}
+
+void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
+    launch_tf_rows(s, p, 0, tf_blocks(p.np));
+    launch_tf_finish(s, p, ps, red);
 }
 
 // F_tot = 0 + 1 F_elec + 1 F_noel [+ 1 F_db + 1 F_ib + 1 F_np]

@@ -52,6 +52,11 @@ void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi);
 long tf_part_doubles(int np);
 void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
+// The same in pieces: the 64-ion block pairs (I, J >= I) of rows I in [i_lo, i_hi),
+// then the per-ion fold and the energy (after every row has run).
+int tf_blocks(int np);
+void launch_tf_rows(hipStream_t s, const PDev& p, int i_lo, int i_hi);
+void launch_tf_finish(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 // F_tot = F_elec + F_noel (+ the PB boundary forces fdb, fib, fnp when given).
 void launch_force_total(hipStream_t s, const PDev& p, const double* fdb = nullptr,
                         const double* fib = nullptr, const double* fnp = nullptr);

@@ -60,6 +60,13 @@ static int choose_slab(int n) {
 
 Sim::Sim() {
     MAZE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    MAZE_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    MAZE_HIP(hipEventCreateWithFlags(&ev_pos, hipEventDisableTiming));
+    MAZE_HIP(hipEventCreateWithFlags(&ev_tf, hipEventDisableTiming));
+    red_side.psum = dalloc<double>(kMaxPartials);
+    red_side.pmax = dalloc<double>(kMaxPartials);
+    red_side.counter = dalloc<unsigned int>(4);
+    if (const char* e = std::getenv("MAZE_TF_OVERLAP")) tf_overlap = std::atoi(e) != 0;
     st = dalloc<CGState>(1);
     MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
     std::memset(st_host, 0, sizeof(CGState));
@@ -83,6 +90,9 @@ Sim::~Sim() {
     dfree(red.psum);
     dfree(red.pmax);
     dfree(red.counter);
+    dfree(red_side.psum);
+    dfree(red_side.pmax);
+    dfree(red_side.counter);
     dfree(dscalar);
     dfree(ps);
     if (st_host) MAZE_HIP(hipHostFree(st_host));
@@ -90,6 +100,10 @@ Sim::~Sim() {
     if (ps_host) MAZE_HIP(hipHostFree(ps_host));
     if (pub_host) MAZE_HIP(hipHostFree(pub_host));
     for (auto e : ev_pool) MAZE_HIP(hipEventDestroy(e));
+    MAZE_HIP(hipEventDestroy(ev_pos));
+    MAZE_HIP(hipEventDestroy(ev_tf));
+    MAZE_HIP(hipStreamSynchronize(side));
+    MAZE_HIP(hipStreamDestroy(side));
     MAZE_HIP(hipStreamDestroy(stream));
 }
 
@@ -379,7 +393,10 @@ hipEvent_t Sim::next_event() {
 // Accumulate the durations of the first `iters_done` iterations' kernels
 // (later launches were no-ops after the stop and are not counted).
 void Sim::prof_flush(int iters_done) {
-    if (!pending.empty()) sync();
+    if (!pending.empty()) {  // timed launches may sit on either stream
+        sync();
+        MAZE_HIP(hipStreamSynchronize(side));
+    }
     for (auto& pe : pending) {
         if (pe.iter >= iters_done) continue;
         float ms = 0.f;
@@ -396,7 +413,10 @@ void Sim::prof_flush(int iters_done) {
 // that pass was applied and, for inner iterations, if the iteration ran
 // (iters[k] iterations if converged, -iters[k] if stopped by its budget).
 void Sim::prof_flush_passes(const int* iters, int npass) {
-    if (!pending.empty()) sync();
+    if (!pending.empty()) {  // timed launches may sit on either stream
+        sync();
+        MAZE_HIP(hipStreamSynchronize(side));
+    }
     for (auto& pe : pending) {
         if (pe.pass >= 0) {
             if (pe.pass >= npass) continue;
@@ -916,9 +936,17 @@ void Sim::part2() {
     }
 }
 
-// One MD step in the order of maze_poisson/solver.py:483-492.
+// One MD step in the order of maze_poisson/solver.py:483-492; the Tosi-Fumi
+// forces overlap the charge spread and the field solve on the side stream.
 void Sim::md_step() {
     part1();
+    const bool overlap = tf_overlap && p.np > 0;
+    if (overlap) {
+        MAZE_HIP(hipEventRecord(ev_pos, stream));
+        MAZE_HIP(hipStreamWaitEvent(side, ev_pos, 0));
+        timed(kTTf, 0, [&] { launch_tf(side, p, ps, red_side); });
+        MAZE_HIP(hipEventRecord(ev_tf, side));
+    }
     update_charges(true);
     update_eps_k2();
     const unsigned long reads = pub_count;
@@ -927,8 +955,9 @@ void Sim::md_step() {
     check_charge();
     check_pb_flag();
     forces_elec();
-    forces_noel(false);
+    if (!overlap) forces_noel(false);
     if (pbd.active) forces_pb();
+    if (overlap) MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
     forces_tot();
     part2();
 }

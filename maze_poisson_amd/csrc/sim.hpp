@@ -97,6 +97,15 @@ struct Sim {
     unsigned int pub_seq = 0;
     unsigned long pub_count = 0;   // read_state calls so far
     Reduce red{};
+    // The Tosi-Fumi forces depend on the positions only, so one MD step
+    // (md_step) runs them on a second stream (default priority) concurrently
+    // with the charge spread and the field solve, with their own reduction
+    // buffers; the total force waits for them.  MAZE_TF_OVERLAP=0 turns it off.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_pos = nullptr, ev_tf = nullptr;
+    Reduce red_side{};
+    int tf_overlap = 1;
+
     double* dscalar = nullptr;  // device scratch scalar
     double* hscalar = nullptr;  // pinned
 
