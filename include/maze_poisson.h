@@ -190,6 +190,10 @@ void get_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
  * -1 for an unknown mode. */
 int solver_set_precision(int mode);
 int solver_get_precision(void);
+/* The refinement passes of the last mixed-precision solve: returns their
+ * number and writes each pass's inner iterations (negative when the pass
+ * stopped on its iteration budget) to iters[0 .. min(count, max)). */
+int solver_get_last_passes(int *iters, int max);
 /* Relative residual reduction of each FP32 inner solve; <= 0 restores the
  * default of the mode (1e-4 for mode 1, 1e-3 for mode 2). */
 void solver_set_inner_rtol(double rtol);

@@ -98,6 +98,7 @@ EXTENSION_ABI = [
     ("get_tf_neighbor_counts", None, [_I1]),
     ("solver_sync", None, []),
     ("solver_set_precision", _c_int, [_c_int]),
+    ("solver_get_last_passes", _c_int, [_I1, _c_int]),
     ("solver_get_precision", _c_int, []),
     ("solver_set_inner_rtol", None, [_c_double]),
     ("solver_set_y_extrapolation", None, [_c_int]),

@@ -197,6 +197,15 @@ int solver_set_precision(int mode) {
 
 int solver_get_precision() { return S().precision; }
 
+// Refinement passes of the last mixed-precision solve and each pass's inner
+// iterations (negative: stopped by the iteration budget), at most `max`.
+int solver_get_last_passes(int* iters, int max) {
+    const auto& v = S().sched;
+    const int k = std::min((int)v.size(), max);
+    for (int i = 0; i < k; i++) iters[i] = v[i];
+    return (int)v.size();
+}
+
 void solver_set_inner_rtol(double rtol) {
     S().inner_rtol = rtol;
     S().sched.clear();

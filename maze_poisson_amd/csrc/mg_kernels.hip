@@ -246,9 +246,14 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
     constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
     constexpr int CI = TI / 2, CJ = TJ / 2, CK = TK / 2;
+    // All three arrays use b's row length BK, so that the threads of one wave
+    // (consecutive positions along a row, rows in order) read and write
+    // consecutive LDS words in every pass: d2 at (i, j, q) = D point (i, j,
+    // q - 1), the residual at (i, j, q) = R point (i, j, q - 2); positions of a
+    // row outside the region are skipped.
     __shared__ float sb[BI * BJ * BK];
-    __shared__ float sd[DI * DJ * DK];
-    __shared__ float sr[RI * RJ * RK];
+    __shared__ float sd[DI * DJ * BK];
+    __shared__ float sr[RI * RJ * BK];
     __shared__ int gi[BI], gj[BJ], gk[BK];
     const MgTileMap<TI, TJ, TK> T(n);
     const int tid = threadIdx.x;
@@ -264,29 +269,31 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
     lds_barrier();
     // Sweeps from zero with weights c1 then c2: d2 = -(c1 + c2) b - c1 c2 A b.
     const float cs = c1 + c2, cp = c1 * c2;
-    tile_for<DI * DJ * DK, NT>([&](int x) {
-        const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
-        const int o = ((i + 1) * BJ + (j + 1)) * BK + (k + 1);
+    tile_for<DI * DJ * BK, NT>([&](int x) {
+        const int q = x % BK, j = (x / BK) % DJ, i = x / (BK * DJ);
+        if (q < 1 || q > DK) return;
+        const int o = ((i + 1) * BJ + (j + 1)) * BK + q;
         const float bc = sb[o];
         const float lap = sb[o - BJ * BK] + sb[o + BJ * BK] + sb[o - BK] + sb[o + BK] + sb[o - 1] +
                           sb[o + 1] - 6.0f * bc;
         sd[x] = -cs * bc - cp * lap;
     });
     lds_barrier();
-    tile_for<RI * RJ * RK, NT>([&](int x) {
-        const int k = x % RK, j = (x / RK) % RJ, i = x / (RK * RJ);
-        const int o = ((i + 1) * DJ + (j + 1)) * DK + (k + 1);
+    tile_for<RI * RJ * BK, NT>([&](int x) {
+        const int q = x % BK, j = (x / BK) % RJ, i = x / (BK * RJ);
+        if (q < 2 || q > RK + 1) return;
+        const int o = ((i + 1) * DJ + (j + 1)) * BK + q;
         const float dc = sd[o];
-        const float lap = sd[o - DJ * DK] + sd[o + DJ * DK] + sd[o - DK] + sd[o + DK] + sd[o - 1] +
+        const float lap = sd[o - DJ * BK] + sd[o + DJ * BK] + sd[o - BK] + sd[o + BK] + sd[o - 1] +
                           sd[o + 1] - 6.0f * dc;
-        sr[x] = sb[((i + 2) * BJ + (j + 2)) * BK + (k + 2)] - lap;
+        sr[x] = sb[((i + 2) * BJ + (j + 2)) * BK + q] - lap;
     });
     // Owned d2 (this loop and the residual pass only read sd).
     tile_for<TI * TJ * TK, NT>([&](int x) {
         const int k = x % TK, j = (x / TK) % TJ, i = x / (TK * TJ);
         if (T.i0 + i < n && T.j0 + j < n && T.k0 + k < n)
             d2out[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] =
-                sd[((i + 2) * DJ + (j + 2)) * DK + (k + 2)];
+                sd[((i + 2) * DJ + (j + 2)) * BK + (k + 3)];
     });
     lds_barrier();
     const int nc = n / 2;
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
         const int K = x % CK, J = (x / CK) % CJ, I = x / (CK * CJ);
         const int gI = T.i0 / 2 + I, gJ = T.j0 / 2 + J, gK = T.k0 / 2 + K;
         if (gI >= nc || gJ >= nc || gK >= nc) return;
-        const int o = ((2 * I + 1) * RJ + (2 * J + 1)) * RK + (2 * K + 1);
+        const int o = ((2 * I + 1) * RJ + (2 * J + 1)) * BK + (2 * K + 3);
         float acc = 0.0f;
 #pragma unroll
         for (int a = -1; a <= 1; a++)
@@ -303,7 +310,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
 #pragma unroll
                 for (int cc = -1; cc <= 1; cc++) {
                     const float w = (float)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
-                    acc += w * sr[o + (a * RJ + bb) * RK + cc];
+                    acc += w * sr[o + (a * RJ + bb) * BK + cc];
                 }
         bcoarse[(gI * nc + gJ) * nc + gK] = acc * (4.0f / 64.0f);
     });

@@ -67,6 +67,7 @@ Sim::Sim() {
     red_side.pmax = dalloc<double>(kMaxPartials);
     red_side.counter = dalloc<unsigned int>(4);
     if (const char* e = std::getenv("MAZE_TF_OVERLAP")) tf_overlap = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MAZE_INNER_MAX")) inner_max = std::atoi(e);
     st = dalloc<CGState>(1);
     MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
     std::memset(st_host, 0, sizeof(CGState));
@@ -697,7 +698,7 @@ int Sim::maze_mixed() {
     phip = phi;
     phi = tmp;
     tmp = old_phip;
-    const int inner_limit = n * n;
+    const int inner_limit = inner_max > 0 ? inner_max : n * n;
     const RefineArgs ra{tol, inner_rtol > 0 ? inner_rtol : (use_mg ? 1e-3 : 1e-4), inner_limit,
                         use_mg ? 1 : 0};
     // The two buffers the refinement passes alternate between (st->ysel).

@@ -51,6 +51,9 @@ struct Sim {
     // preconditioned CG: measured fastest at 4096 ions / 128^3, where two
     // refinement passes of about three cycles each reach the FP64 stop).
     double inner_rtol = 0.0;
+    // Iteration budget of each inner solve of the mixed path (<= 0: n^2, i.e.
+    // the relative stop decides); MAZE_INNER_MAX sets it.
+    int inner_max = 0;
     int pb = 0;
     int nonpolar = 0;
     double w = 0, kbar2 = 0;
