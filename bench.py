@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=15)
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--cpu-child", action="store_true")
+    ap.add_argument("--pb-radii-only", action="store_true",
+                    help="only the radius-based PB step (8000 ions, 160^3); prints its JSON")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one 256 MiB copy kernel first (byte-counter calibration)")
     args = ap.parse_args()
@@ -333,6 +335,9 @@ def main():
         lib.lib.maze_calib_copy.argtypes = [__import__("ctypes").c_long]
         lib.lib.maze_calib_copy(32 << 20)
 
+    if args.pb_radii_only:
+        print(json.dumps(pb_radii_rate(lib, args.steps, args.prof_steps)))
+        return
     head = run_path(lib, PATHS["mixed_mg"], args.steps, args.warmup, args.steps, args.prof_steps)
     line_extra = {}
     if not args.no_extra:
