@@ -13,6 +13,10 @@
  * Part 2 lists this library's extensions (no reference counterpart, or a
  * counterpart that exists only as commented-out code).
  *
+ * Part 3 is the replica engine: many independent MD replicas on one GPU,
+ * each with its own state, advanced together (no reference counterpart; the
+ * reference runs one replica per process).
+ *
  * Units are atomic units throughout (Bohr, Hartree, electron masses, a.u.
  * time).  Grid arrays are n*n*n, C order [i][j][k], k fastest.
  */
@@ -278,6 +282,50 @@ int maze_test_mg_up(const float *b, const float *d2, const float *e, float *d, i
 /* The one-work-group coarse tail of the V-cycle set up for an n^3 grid, on the
  * first level it covers (its size is returned; -1 if the grid has no tail). */
 int maze_test_mg_tail(const float *b, float *d, int n);
+
+/* ====================== Part 3: the replica engine ======================== */
+/* Each request is one MaZe-LCG replica: NVE velocity Verlet, CIC charges,
+ * Tosi-Fumi pair forces (cutoff L/2), vacuum dielectric, the reference's two
+ * start-up steps with cold solves, then `steps` MD steps.  Requests are
+ * admitted in arrival order while a slot is free and their device-memory
+ * estimate fits the budget; admitted replicas advance concurrently, each
+ * step queued as soon as the replica's previous solve state is published.
+ * A replica's results do not depend on what else runs (own buffers,
+ * deterministic kernels).  Not thread-safe; one engine per thread. */
+typedef struct maze_engine maze_engine;
+/* budget <= 0: 90 % of the free device memory; max_slots: replicas at once. */
+maze_engine *maze_engine_create(long long hbm_budget_bytes, int max_slots);
+void maze_engine_destroy(maze_engine *e);
+/* Device bytes one replica needs (grids, multigrid levels, particles, pair
+ * partials, scratch), and the precision it runs: 2 (FP32 multigrid-PCG in
+ * FP64 refinement) needs an even grid whose halvings end at or below 96
+ * points, else 0 (FP64 CG) is used; 1 runs as asked. */
+long long maze_engine_estimate_bytes(int n_grid, int np, int precision);
+int maze_engine_precision_for(int n_grid, int precision);
+/* Queues a request arriving arrival_s seconds after the start of the next
+ * maze_engine_run.  Arrays as solver_initialize_particles (tf_params: n_typ^2
+ * x 5 Tosi-Fumi parameters in atomic units); dt in a.u.; tol the absolute
+ * residual tolerance of the field solve.  Returns the request id, -1 for
+ * invalid arguments, -2 if the request alone exceeds the budget. */
+int maze_engine_submit(maze_engine *e, double arrival_s, int n_grid, double L, double tol,
+                       int precision, int np, int n_typ, const int *types, const double *pos,
+                       const double *vel, const double *mass, const double *charges,
+                       const double *tf_params, double dt, int steps);
+/* Runs until every queued request has finished; returns 0. */
+int maze_engine_run(maze_engine *e);
+/* Final positions, velocities and total forces of a finished request (np x 3
+ * each; any pointer may be NULL).  Returns 0, or -1 if not finished. */
+int maze_engine_result(maze_engine *e, int id, double *pos, double *vel, double *fcs_tot);
+/* Per request (up to max values): arrival, admission and finish time (s from
+ * the start of the run), steps done, mean / p50 / p99 step latency (ms),
+ * estimated and measured device bytes, precision run, unconverged solves,
+ * bytes used of the replica's arena (one device allocation per replica) and
+ * bytes that did not fit it.  Returns the number of values written. */
+int maze_engine_request_stats(maze_engine *e, int id, double *out, int max);
+/* Last run (up to max values): wall time (s), replica-steps/s, atom-steps/s,
+ * p50 / p99 step latency over all steps (ms), peak estimated device bytes in
+ * use, most replicas running at once. */
+int maze_engine_stats(maze_engine *e, double *out, int max);
 
 #ifdef __cplusplus
 }

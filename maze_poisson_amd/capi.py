@@ -107,6 +107,18 @@ EXTENSION_ABI = [
     ("solver_set_mg_omega", None, [_c_double]),
     ("solver_mg_levels", _c_int, []),
     ("solver_set_charge_check", None, [_c_int]),
+    # the replica engine (many independent replicas on one GPU)
+    ("maze_engine_create", ctypes.c_void_p, [ctypes.c_longlong, _c_int]),
+    ("maze_engine_destroy", None, [ctypes.c_void_p]),
+    ("maze_engine_estimate_bytes", ctypes.c_longlong, [_c_int, _c_int, _c_int]),
+    ("maze_engine_precision_for", _c_int, [_c_int, _c_int]),
+    ("maze_engine_submit", _c_int, [ctypes.c_void_p, _c_double, _c_int, _c_double, _c_double,
+                                    _c_int, _c_int, _c_int, _I1, _D2, _D2, _D1, _D1, _D1,
+                                    _c_double, _c_int]),
+    ("maze_engine_run", _c_int, [ctypes.c_void_p]),
+    ("maze_engine_result", _c_int, [ctypes.c_void_p, _c_int, _D2, _D2, _D2]),
+    ("maze_engine_request_stats", _c_int, [ctypes.c_void_p, _c_int, _D1, _c_int]),
+    ("maze_engine_stats", _c_int, [ctypes.c_void_p, _D1, _c_int]),
 ]
 
 

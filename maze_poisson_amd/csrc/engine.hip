@@ -1,0 +1,362 @@
+// engine.hip -- many independent replicas on one GPU, advanced from one host
+// thread: the batched ensemble and the continuous-batching request loop.
+//
+// Each replica is a full Sim (its own device state, streams and reduction
+// buffers), so replicas never share memory and a replica's trajectory is the
+// same alone and in any mix.  The loop keeps every admitted replica's next
+// step queued: a replica whose solver state has been published (Sim::
+// step_ready) is finished (forces, second integrator half) and its next step
+// is issued at once, so the GPU always holds work of several replicas on
+// several streams and small grids fill the chip together.  Requests are
+// admitted in arrival order when a slot is free and their device-memory
+// estimate fits the budget; a finished replica's results are copied out and
+// its memory returned before the next admission.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "sim.hpp"
+
+namespace maze {
+
+namespace {
+
+double now_s() {
+    using C = std::chrono::steady_clock;
+    return std::chrono::duration<double>(C::now().time_since_epoch()).count();
+}
+
+double quantile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double x = q * (double)(v.size() - 1);
+    const size_t i = (size_t)x;
+    if (i + 1 >= v.size()) return v.back();
+    return v[i] + (x - (double)i) * (v[i + 1] - v[i]);
+}
+
+}  // namespace
+
+struct Request {
+    // inputs
+    double arrival = 0;
+    int n = 0, np = 0, n_typ = 0, precision = kMixedMG, steps = 0;
+    double L = 0, tol = 0, dt = 0;
+    std::vector<int> types;
+    std::vector<double> pos, vel, mass, charges, tf;
+    // progress
+    int state = 0;  // 0 waiting, 1 running, 2 done
+    int used_precision = kFP64;
+    int done = 0, failures = 0;
+    long long est_bytes = 0, meas_bytes = -1, arena_used = 0, arena_overflow = 0;
+    std::unique_ptr<DevArena> arena;
+    double t_admit = 0, t_done = 0, t_last = 0;
+    std::vector<double> lat;  // seconds per step, as observed by the loop
+    std::unique_ptr<Sim> sim;
+    std::vector<double> out_pos, out_vel, out_f;
+};
+
+// The precision a replica of an n^3 grid runs: the FP32 multigrid needs an
+// even grid that halves to at most 96 points (Sim::mg_sizes); otherwise the
+// FP64 CG, which is faster than the FP32 Jacobi-CG inside refinement.
+static int precision_for(int n, int want) {
+    if (want == kMixedMG && Sim::mg_sizes(n).empty()) return kFP64;
+    return want;
+}
+
+// Device bytes a replica carves from its arena: the Sim's reduction buffers
+// and solver state, the grids, the multigrid levels, the particle arrays, the
+// Tosi-Fumi partials and the sort scratch (each allocation 256-byte aligned).
+static long long arena_bytes(int n, int np, int precision) {
+    const long long G = (long long)n * n * n;
+    long long b = 4LL * 8 * (1 << 15) + (64 << 10);  // partials of both streams, scalars
+    b += 9 * 8 * G;                                   // q, y, phi, phi_prev, phi*, sigma, r, p0, p1
+    if (precision != kFP64) b += 8 * G + 4 * 4 * G;   // second y, FP32 d, r, p0, p1
+    if (precision == kMixedMG) {
+        const std::vector<int> sz = Sim::mg_sizes(n);
+        for (size_t l = 0; l < sz.size(); l++) {
+            const long long g = (long long)sz[l] * sz[l] * sz[l];
+            b += (l == 0 ? 2 : 3) * 4 * g;  // level 0 borrows the FP32 residual as b
+        }
+        if (!sz.empty()) {
+            const long long c = sz.back();
+            b += 8 * (c * c + c * c * c);  // coarsest eigenbasis and inverse eigenvalues
+            if (c > kMgCoarseLds) b += 2 * 8 * c * c * c;  // its work arrays
+        }
+    }
+    unsigned long cells = (unsigned long)n * n * n;
+    int key_bits = 1;
+    while ((1ul << key_bits) < cells) key_bits++;
+    b += (long long)np * (4 + 5 * 24 + 2 * 8 + 12 + 5 * 4 + 8 * 8 + 8);
+    b += 4LL * ((long long)n * n + 1);
+    b += 8LL * tf_part_doubles(np) + 7LL * 8 * 16 * 16;  // pair table, up to 16 types
+    b += (long long)sort_tmp_bytes(std::max(np, 1), key_bits);
+    return b + 64LL * (long long)kArenaAlign;  // alignment of up to 64 allocations
+}
+
+// Device memory a replica takes: its arena (hipMalloc rounds to 2 MiB) plus
+// what the runtime holds for its two streams and events (measured: 8 MiB).
+constexpr long long kReplicaExtra = 10LL << 20;
+static long long estimate_bytes(int n, int np, int precision) {
+    const long long mib2 = 2LL << 20;
+    return (arena_bytes(n, np, precision) + mib2 - 1) / mib2 * mib2 + kReplicaExtra;
+}
+
+struct Engine {
+    long long budget = 0;
+    int max_slots = 1;
+    std::vector<Request> reqs;
+    // last run
+    long long used = 0, peak = 0;
+    int max_running = 0;
+    double t0 = 0, t_first = 0, t_end = 0;
+
+    void admit(Request& r) {
+        r.used_precision = precision_for(r.n, r.precision);
+        size_t free0 = 0, total = 0;
+        MAZE_HIP(hipMemGetInfo(&free0, &total));
+        r.arena = std::make_unique<DevArena>();
+        r.arena->size = (size_t)arena_bytes(r.n, r.np, r.used_precision);
+        MAZE_HIP(hipMalloc(&r.arena->base, r.arena->size));
+        arena_install(r.arena.get());
+        r.sim = std::make_unique<Sim>();
+        Sim& s = *r.sim;
+        const double h = r.L / r.n;
+        s.init_grid(r.n, r.L, h, r.tol, 1.0, 1.0, kMazeLCG, kPNone);
+        s.precision = r.used_precision;
+        s.init_particles(r.n, r.n_typ, r.L, h, r.np, 0, r.types.data(), r.pos.data(),
+                         r.vel.data(), r.mass.data(), r.charges.data(), r.tf.data());
+        s.init_integrator(r.dt, 0.0, 0.0, kVerlet, 0);
+        // The two start-up steps with cold solves (maze_poisson/solver.py:376-398).
+        for (int k = 0; k < 2; k++) {
+            s.update_charges();
+            s.init_field();
+            s.forces_elec();
+            s.forces_noel(false);
+            s.forces_tot();
+            if (k == 0) s.part1();
+            else s.part2();
+        }
+        r.t_admit = r.t_last = now_s() - t0;
+        r.state = 1;
+        used += r.est_bytes;
+        peak = std::max(peak, used);
+        if (r.steps > 0) s.step_issue();  // allocates the mixed solver's arrays
+        arena_install(nullptr);
+        size_t free1 = 0;
+        MAZE_HIP(hipMemGetInfo(&free1, &total));
+        r.meas_bytes = (long long)free0 - (long long)free1;
+        r.arena_used = (long long)r.arena->used;
+        r.arena_overflow = (long long)r.arena->overflow;
+        if (r.steps == 0) retire(r);
+    }
+
+    void retire(Request& r) {
+        Sim& s = *r.sim;
+        r.out_pos.resize(3L * r.np);
+        r.out_vel.resize(3L * r.np);
+        r.out_f.resize(3L * r.np);
+        if (r.np > 0) {
+            MAZE_HIP(hipMemcpyAsync(r.out_pos.data(), s.p.pos, 24L * r.np, hipMemcpyDeviceToHost,
+                                    s.stream));
+            MAZE_HIP(hipMemcpyAsync(r.out_vel.data(), s.p.vel, 24L * r.np, hipMemcpyDeviceToHost,
+                                    s.stream));
+            MAZE_HIP(hipMemcpyAsync(r.out_f.data(), s.p.ft, 24L * r.np, hipMemcpyDeviceToHost,
+                                    s.stream));
+        }
+        s.sync();
+        r.sim.reset();
+        arena_forget(r.arena.get());
+        MAZE_HIP(hipFree(r.arena->base));
+        r.arena.reset();
+        used -= r.est_bytes;
+        r.state = 2;
+        r.t_done = now_s() - t0;
+    }
+
+    // Advances every waiting request to the end of its steps.
+    int run() {
+        t0 = now_s();
+        std::vector<int> order;
+        for (int i = 0; i < (int)reqs.size(); i++)
+            if (reqs[i].state == 0) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
+        std::vector<int> running;
+        size_t next = 0;
+        int left = (int)order.size();
+        used = peak = 0;
+        max_running = 0;
+        t_first = order.empty() ? 0.0 : reqs[order[0]].arrival;
+        while (left > 0) {
+            const double t = now_s() - t0;
+            while (next < order.size()) {
+                Request& r = reqs[order[next]];
+                if (r.arrival > t || (int)running.size() >= max_slots || used + r.est_bytes > budget)
+                    break;
+                admit(r);
+                next++;
+                if (r.state == 2) left--;
+                else running.push_back(order[next - 1]);
+            }
+            max_running = std::max(max_running, (int)running.size());
+            bool progressed = false;
+            for (size_t a = 0; a < running.size();) {
+                Request& r = reqs[running[a]];
+                if (!r.sim->step_ready()) {
+                    a++;
+                    continue;
+                }
+                progressed = true;
+                if (r.sim->step_finish() < 0) r.failures++;
+                const double tn = now_s() - t0;
+                r.lat.push_back(tn - r.t_last);
+                r.t_last = tn;
+                if (++r.done == r.steps) {
+                    retire(r);
+                    running.erase(running.begin() + (long)a);
+                    left--;
+                    continue;
+                }
+                r.sim->step_issue();
+                a++;
+            }
+            if (!progressed) __builtin_ia32_pause();
+        }
+        t_end = now_s() - t0;
+        return 0;
+    }
+};
+
+}  // namespace maze
+
+using maze::Engine;
+using maze::Request;
+
+extern "C" {
+
+struct maze_engine;
+
+maze_engine* maze_engine_create(long long hbm_budget_bytes, int max_slots) {
+    auto* e = new Engine();
+    if (hbm_budget_bytes <= 0) {
+        size_t free_b = 0, total = 0;
+        MAZE_HIP(hipMemGetInfo(&free_b, &total));
+        hbm_budget_bytes = (long long)(0.9 * (double)free_b);
+    }
+    e->budget = hbm_budget_bytes;
+    e->max_slots = std::max(1, max_slots);
+    return reinterpret_cast<maze_engine*>(e);
+}
+
+void maze_engine_destroy(maze_engine* h) { delete reinterpret_cast<Engine*>(h); }
+
+long long maze_engine_estimate_bytes(int n_grid, int np, int precision) {
+    return maze::estimate_bytes(n_grid, np, maze::precision_for(n_grid, precision));
+}
+
+int maze_engine_precision_for(int n_grid, int precision) {
+    return maze::precision_for(n_grid, precision);
+}
+
+int maze_engine_submit(maze_engine* h, double arrival_s, int n_grid, double L, double tol,
+                       int precision, int np, int n_typ, const int* types, const double* pos,
+                       const double* vel, const double* mass, const double* charges,
+                       const double* tf_params, double dt, int steps) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    if (n_grid < 4 || L <= 0 || np < 0 || n_typ < 1 || n_typ > 16 || steps < 0 || precision < 0 ||
+        precision > 2) {
+        std::fprintf(stderr, "maze_engine_submit: invalid request\n");
+        return -1;
+    }
+    Request r;
+    r.arrival = arrival_s;
+    r.n = n_grid;
+    r.L = L;
+    r.tol = tol;
+    r.precision = precision;
+    r.np = np;
+    r.n_typ = n_typ;
+    r.dt = dt;
+    r.steps = steps;
+    r.types.assign(types, types + np);
+    r.pos.assign(pos, pos + 3L * np);
+    r.vel.assign(vel, vel + 3L * np);
+    r.mass.assign(mass, mass + np);
+    r.charges.assign(charges, charges + np);
+    r.tf.assign(tf_params, tf_params + 5L * n_typ * n_typ);
+    r.est_bytes = maze::estimate_bytes(n_grid, np, maze::precision_for(n_grid, precision));
+    if (r.est_bytes > e.budget) {
+        std::fprintf(stderr,
+                     "maze_engine_submit: request needs about %lld bytes, above the budget of "
+                     "%lld\n",
+                     r.est_bytes, e.budget);
+        return -2;
+    }
+    e.reqs.push_back(std::move(r));
+    return (int)e.reqs.size() - 1;
+}
+
+int maze_engine_run(maze_engine* h) { return reinterpret_cast<Engine*>(h)->run(); }
+
+int maze_engine_result(maze_engine* h, int id, double* pos, double* vel, double* fcs_tot) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    if (id < 0 || id >= (int)e.reqs.size() || e.reqs[id].state != 2) return -1;
+    const Request& r = e.reqs[id];
+    const size_t b = 24L * r.np;
+    if (pos) std::memcpy(pos, r.out_pos.data(), b);
+    if (vel) std::memcpy(vel, r.out_vel.data(), b);
+    if (fcs_tot) std::memcpy(fcs_tot, r.out_f.data(), b);
+    return 0;
+}
+
+// out: arrival, admitted, finished (s from the start of the run), steps done,
+// mean, p50 and p99 step latency (ms), estimated and measured device bytes,
+// precision run, solver failures, arena bytes used, bytes beyond the arena.
+int maze_engine_request_stats(maze_engine* h, int id, double* out, int max) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    if (id < 0 || id >= (int)e.reqs.size()) return -1;
+    const Request& r = e.reqs[id];
+    double mean = 0;
+    for (double x : r.lat) mean += x;
+    if (!r.lat.empty()) mean /= (double)r.lat.size();
+    const double v[13] = {r.arrival, r.t_admit, r.t_done, (double)r.done, 1e3 * mean,
+                          1e3 * maze::quantile(r.lat, 0.5), 1e3 * maze::quantile(r.lat, 0.99),
+                          (double)r.est_bytes, (double)r.meas_bytes, (double)r.used_precision,
+                          (double)r.failures, (double)r.arena_used, (double)r.arena_overflow};
+    const int k = std::min(max, 13);
+    for (int i = 0; i < k; i++) out[i] = v[i];
+    return k;
+}
+
+// out: wall time of the last run (s), replica-steps/s, atom-steps/s, p50 and
+// p99 step latency over all steps (ms), peak estimated device bytes, most
+// replicas running at once.
+int maze_engine_stats(maze_engine* h, double* out, int max) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    std::vector<double> all;
+    double steps = 0, atom_steps = 0;
+    for (const Request& r : e.reqs) {
+        if (r.state != 2) continue;
+        all.insert(all.end(), r.lat.begin(), r.lat.end());
+        steps += r.done;
+        atom_steps += (double)r.done * r.np;
+    }
+    const double wall = e.t_end;
+    const double v[7] = {wall,
+                         wall > 0 ? steps / wall : 0.0,
+                         wall > 0 ? atom_steps / wall : 0.0,
+                         1e3 * maze::quantile(all, 0.5),
+                         1e3 * maze::quantile(all, 0.99),
+                         (double)e.peak,
+                         (double)e.max_running};
+    const int k = std::min(max, 7);
+    for (int i = 0; i < k; i++) out[i] = v[i];
+    return k;
+}
+
+}  // extern "C"

@@ -139,8 +139,15 @@ void launch_mg_post(hipStream_t s, const GridDims& d, const float* b, const floa
                     float* dout, double c, CGState* st, int dot, Reduce red);
 void launch_mg_restrict(hipStream_t s, int n, const float* r, float* bc, CGState* st);
 void launch_mg_prolong_add(hipStream_t s, int nc, const float* ec, float* d, CGState* st);
+// Exact solve on the coarsest level: one work-group with the grid in LDS for
+// c <= kMgCoarseLds, else six axis passes over the work arrays w0, w1 (c^3
+// doubles each) for c <= kMgCoarseMax.
+constexpr int kMgCoarseLds = 12;
+constexpr int kMgCoarseMax = 96;
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st);
+void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,
+                          const double* Qf, const double* ilam, CGState* st);
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                     float* p_new, CGState* st, Reduce red, int first);
 // Fused level kernels of the V-cycle on an n^3 level (n even):

@@ -776,7 +776,7 @@ void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d
 // real orthonormal eigenbasis Qf (row t = eigenvector t), divide by the
 // eigenvalue sum (the constant mode is dropped), transform back.  A and B
 // are c^3 double work arrays in LDS; `nt` threads cooperate.
-constexpr int kMaxCoarse = 12;
+constexpr int kMaxCoarse = kMgCoarseLds;
 
 __device__ void coarse_exact(int c, const double* Q, const double* __restrict__ ilam, double* A,
                              double* B, int tid, int nt) {
@@ -845,6 +845,65 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse(int c, const float* __rest
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st) {
     MAZE_LAUNCH(k_mg_coarse, dim3(1), dim3(kBlock), 0, s, c, b, e, Qf, ilam, st);
+}
+
+// ---- exact solve on a larger coarsest grid (kMgCoarseLds < c <= kMgCoarseMax) --
+// The separable transform of coarse_exact with one axis per kernel over
+// global work arrays.  A pass transforms the last axis of its input and
+// writes it first, out[t][i][j] = sum_m M[t][m] in[i][j][m] with M = Qf
+// (forward) or Qf^T (inverse), so three passes cover the three axes and
+// restore the order.  A work-group stages kCoarseRows consecutive rows of one
+// plane (contiguous in memory) in LDS.  Mode 1 reads the FP32 right-hand
+// side, mode 2 scales by the inverse eigenvalues (after the third forward
+// pass), mode 3 writes the FP32 correction; mode 0 is double to double.
+constexpr int kCoarseRows = 16;
+
+template <int Mode, bool Inv>
+__global__ __launch_bounds__(kBlock) void k_mg_coarse_axis(int c, const void* __restrict__ in_,
+                                                           void* __restrict__ out_,
+                                                           const double* __restrict__ Qf,
+                                                           const double* __restrict__ ilam,
+                                                           const CGState* st) {
+    if (st->done) return;
+    __shared__ double rows[kCoarseRows * kMgCoarseMax];
+    const int nj = (c + kCoarseRows - 1) / kCoarseRows;
+    const int i = (int)blockIdx.x / nj, j0 = ((int)blockIdx.x % nj) * kCoarseRows;
+    const int J = min(kCoarseRows, c - j0);
+    const long base = ((long)i * c + j0) * c;
+    for (int x = threadIdx.x; x < J * c; x += kBlock) {
+        if constexpr (Mode == 1) rows[x] = (double)static_cast<const float*>(in_)[base + x];
+        else rows[x] = static_cast<const double*>(in_)[base + x];
+    }
+    lds_barrier();
+    for (int x = threadIdx.x; x < J * c; x += kBlock) {
+        const int jj = x % J, t = x / J;
+        const double* r = rows + jj * c;
+        double s = 0.0;
+        if constexpr (Inv) {
+            for (int m = 0; m < c; m++) s += Qf[m * c + t] * r[m];
+        } else {
+            for (int m = 0; m < c; m++) s += Qf[t * c + m] * r[m];
+        }
+        const long o = ((long)t * c + i) * c + j0 + jj;
+        if constexpr (Mode == 2) s *= ilam[o];
+        if constexpr (Mode == 3) static_cast<float*>(out_)[o] = (float)s;
+        else static_cast<double*>(out_)[o] = s;
+    }
+}
+
+void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,
+                          const double* Qf, const double* ilam, CGState* st) {
+    if (c > kMgCoarseMax) {
+        std::fprintf(stderr, "maze: coarsest multigrid level %d above %d\n", c, kMgCoarseMax);
+        std::exit(1);
+    }
+    const dim3 grid(c * ((c + kCoarseRows - 1) / kCoarseRows)), block(kBlock);
+    MAZE_LAUNCH((k_mg_coarse_axis<1, false>), grid, block, 0, s, c, b, w0, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis<0, false>), grid, block, 0, s, c, w0, w1, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis<2, false>), grid, block, 0, s, c, w1, w0, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis<0, true>), grid, block, 0, s, c, w0, w1, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis<0, true>), grid, block, 0, s, c, w1, w0, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis<3, true>), grid, block, 0, s, c, w0, e, Qf, ilam, st);
 }
 
 // ---- the coarse tail of the V-cycle in one work-group --------------------------

@@ -25,18 +25,53 @@ static void fatal(const char* msg) {
     std::exit(1);
 }
 
+// Device arenas: while one is installed on the calling thread, replica
+// allocations are carved from it (256-byte aligned) instead of hipMalloc;
+// allocations that do not fit fall back to hipMalloc and are counted.
+namespace {
+thread_local DevArena* t_arena = nullptr;
+std::vector<const DevArena*> g_arenas;  // live arenas, for dfree
+
+bool in_arena(const void* p) {
+    const char* c = static_cast<const char*>(p);
+    for (const DevArena* a : g_arenas)
+        if (c >= a->base && c < a->base + a->size) return true;
+    return false;
+}
+}  // namespace
+
+void arena_install(DevArena* a) {
+    t_arena = a;
+    if (a && std::find(g_arenas.begin(), g_arenas.end(), a) == g_arenas.end()) g_arenas.push_back(a);
+}
+
+void arena_forget(const DevArena* a) {
+    g_arenas.erase(std::remove(g_arenas.begin(), g_arenas.end(), a), g_arenas.end());
+    if (t_arena == a) t_arena = nullptr;
+}
+
 template <class T>
 static T* dalloc(long count) {
     T* p = nullptr;
     if (count <= 0) count = 1;
-    MAZE_HIP(hipMalloc(&p, sizeof(T) * (size_t)count));
-    MAZE_HIP(hipMemset(p, 0, sizeof(T) * (size_t)count));
+    const size_t bytes = sizeof(T) * (size_t)count;
+    if (DevArena* a = t_arena) {
+        const size_t off = (a->used + kArenaAlign - 1) & ~(kArenaAlign - 1);
+        if (off + bytes <= a->size) {
+            p = reinterpret_cast<T*>(a->base + off);
+            a->used = off + bytes;
+        } else {
+            a->overflow += bytes;
+        }
+    }
+    if (!p) MAZE_HIP(hipMalloc(&p, bytes));
+    MAZE_HIP(hipMemset(p, 0, bytes));
     return p;
 }
 
 template <class T>
 static void dfree(T*& p) {
-    if (p) MAZE_HIP(hipFree(p));
+    if (p && !in_arena(p)) MAZE_HIP(hipFree(p));
     p = nullptr;
 }
 
@@ -133,8 +168,7 @@ void Sim::free_particles() {
     dfree(p.ids_sorted);
     dfree(p.row_start);
     dfree(p.counts);
-    if (p.sort_tmp) MAZE_HIP(hipFree(p.sort_tmp));
-    p.sort_tmp = nullptr;
+    dfree(p.sort_tmp);
     p.np = 0;
     free_pb();
 }
@@ -144,7 +178,7 @@ void Sim::free_pb() {
     for (int** a : {&pbd.rmap, &pbd.rfrc, &pbd.ids, &pbd.ids_sorted}) dfree(*a);
     dfree(pbd.keys);
     dfree(pbd.keys_sorted);
-    if (pbd.sort_tmp) MAZE_HIP(hipFree(pbd.sort_tmp));
+    dfree(pbd.sort_tmp);
     pbd = PBDev{};
     pb_flag_pending = 0;
 }
@@ -260,7 +294,7 @@ void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, int ca
     key_bits = 1;
     while ((1ul << key_bits) < cells) key_bits++;
     p.sort_tmp_bytes = sort_tmp_bytes(std::max(np, 1), key_bits);
-    MAZE_HIP(hipMalloc(&p.sort_tmp, std::max<size_t>(p.sort_tmp_bytes, 16)));
+    p.sort_tmp = dalloc<char>((long)std::max<size_t>(p.sort_tmp_bytes, 16));
     sync();
 }
 
@@ -300,7 +334,7 @@ void Sim::init_pb_particles(double gamma_np, double beta_np, const double* radii
     pbd.fdb = dalloc<double>(3L * np); pbd.fib = dalloc<double>(3L * np);
     pbd.fnp = dalloc<double>(3L * np);
     pbd.sort_bytes = pb_sort_bytes(std::max(np, 1), pbd.key_bits);
-    MAZE_HIP(hipMalloc(&pbd.sort_tmp, std::max<size_t>(pbd.sort_bytes, 16)));
+    pbd.sort_tmp = dalloc<char>((long)std::max<size_t>(pbd.sort_bytes, 16));
     pbd.flag = &ps->pb_flag;
     if (np > 0) {
         MAZE_HIP(hipMemcpy(pbd.radii, radii, np * sizeof(double), hipMemcpyHostToDevice));
@@ -363,8 +397,21 @@ void Sim::init_integrator(double dt_, double T_, double gamma, int type, int en)
 // number not arrive within a second, the stream is synchronised and the
 // number checked once more.
 void Sim::read_state() {
-    const unsigned int want = ++pub_seq;
-    launch_publish_state(stream, st, &ps->q_sum, pub_dev, want);
+    publish();
+    wait_published();
+}
+
+void Sim::publish() {
+    want_seq = ++pub_seq;
+    launch_publish_state(stream, st, &ps->q_sum, pub_dev, want_seq);
+}
+
+bool Sim::published() const {
+    return __atomic_load_n(&pub_host->seq, __ATOMIC_ACQUIRE) == want_seq;
+}
+
+void Sim::wait_published() {
+    const unsigned int want = want_seq;
     const auto t0 = std::chrono::steady_clock::now();
     for (long spin = 0;; spin++) {
         if (__atomic_load_n(&pub_host->seq, __ATOMIC_ACQUIRE) == want) break;
@@ -439,27 +486,42 @@ void Sim::prof_flush_passes(const int* iters, int npass) {
 // device-side stop flag is polled between chunks.
 int Sim::cg_solve(const GridDims& d, const PBCoef& pbc_, const double* b, double* x, double tol_,
                   int limit, double* rbuf, double* pa, double* pbuf, int* pred) {
-    launch_cg_reset(stream, st, tol_, 0.0, limit, 0);
-    launch_cg_init<double>(stream, d, pbc_, x, b, rbuf, st, red);
-    int launched = 0;
-    int chunk = std::max(4, *pred + 1);
-    double* pin = pa;
-    double* pout = pbuf;
-    for (;;) {
-        for (int c = 0; c < chunk && launched < limit; c++, launched++) {
-            timed(kTCgDir, launched,
-                  [&] { launch_cg_dir<double>(stream, d, pbc_, pin, rbuf, pout, st, red, launched == 0); });
-            timed(kTCgUpdate, launched,
-                  [&] { launch_cg_update<double>(stream, d, pbc_, pout, x, rbuf, st, red); });
-            std::swap(pin, pout);
-        }
-        read_state();
-        if (st_host->done || launched >= limit) break;
-        chunk = std::max(4, chunk / 4);
+    cg_issue(CgRun{d, pbc_, x, rbuf, pa, pbuf, limit, 0, std::max(4, *pred + 1), pred}, b, tol_);
+    return cg_finish();
+}
+
+// The first chunk of a CG solve, up to the publication of its state.
+void Sim::cg_issue(const CgRun& c, const double* b, double tol_) {
+    cg = c;
+    launch_cg_reset(stream, st, tol_, 0.0, cg.limit, 0);
+    launch_cg_init<double>(stream, cg.d, cg.pbc, cg.x, b, cg.rbuf, st, red);
+    cg_chunk();
+    publish();
+}
+
+void Sim::cg_chunk() {
+    for (int c = 0; c < cg.chunk && cg.launched < cg.limit; c++, cg.launched++) {
+        const int it = cg.launched;
+        timed(kTCgDir, it, [&] {
+            launch_cg_dir<double>(stream, cg.d, cg.pbc, cg.pin, cg.rbuf, cg.pout, st, red, it == 0);
+        });
+        timed(kTCgUpdate, it,
+              [&] { launch_cg_update<double>(stream, cg.d, cg.pbc, cg.pout, cg.x, cg.rbuf, st, red); });
+        std::swap(cg.pin, cg.pout);
     }
-    const int done_iters = st_host->res >= 0 ? st_host->res + 1 : launched;
+}
+
+// Waits for the published state; further chunks until the stop.
+int Sim::cg_finish() {
+    wait_published();
+    while (!st_host->done && cg.launched < cg.limit) {
+        cg.chunk = std::max(4, cg.chunk / 4);
+        cg_chunk();
+        read_state();
+    }
+    const int done_iters = st_host->res >= 0 ? st_host->res + 1 : cg.launched;
     prof_flush(done_iters);
-    *pred = done_iters;
+    *cg.pred = done_iters;
     return st_host->res;
 }
 
@@ -546,17 +608,26 @@ void Sim::free_mg() {
     mg.clear();
     dfree(mg_Qf);
     dfree(mg_ilam);
+    dfree(mg_w0);
+    dfree(mg_w1);
     mg_ok = -1;
 }
 
-// Levels n, n/2, ... while the size is even and above 12; the last level
-// (<= 12 points per side) is solved exactly.  Unavailable when halving stops
-// at an odd size above 12.
+// Levels n, n/2, ... while the size is even and above 12; the last level is
+// solved exactly (in one work-group up to 12 points per side, by axis passes
+// over global work arrays up to 96).  Unavailable when halving stops at an
+// odd size above 96.
+std::vector<int> Sim::mg_sizes(int n) {
+    std::vector<int> sizes{n};
+    while (sizes.back() % 2 == 0 && sizes.back() > kMgCoarseLds) sizes.push_back(sizes.back() / 2);
+    if (sizes.back() > kMgCoarseMax || sizes.size() < 2) sizes.clear();
+    return sizes;
+}
+
 bool Sim::setup_mg() {
     if (mg_ok >= 0) return mg_ok == 1;
-    std::vector<int> sizes{n};
-    while (sizes.back() % 2 == 0 && sizes.back() > 12) sizes.push_back(sizes.back() / 2);
-    if (sizes.back() > 12 || sizes.size() < 2) {
+    const std::vector<int> sizes = mg_sizes(n);
+    if (sizes.empty()) {
         mg_ok = 0;
         return false;
     }
@@ -608,6 +679,10 @@ bool Sim::setup_mg() {
             }
     mg_Qf = dalloc<double>(c * c);
     mg_ilam = dalloc<double>(c * c * c);
+    if (c > kMgCoarseLds) {
+        mg_w0 = dalloc<double>(c * c * c);
+        mg_w1 = dalloc<double>(c * c * c);
+    }
     MAZE_HIP(hipMemcpy(mg_Qf, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(mg_ilam, il.data(), il.size() * sizeof(double), hipMemcpyHostToDevice));
     // The one-work-group tail takes over from the first level below the
@@ -641,7 +716,8 @@ void Sim::vcycle(int l, int iter) {
         return;
     }
     if (l == (int)mg.size() - 1) {
-        launch_mg_coarse(stream, mg_c, L.b, L.d, mg_Qf, mg_ilam, st);
+        if (mg_c <= kMgCoarseLds) launch_mg_coarse(stream, mg_c, L.b, L.d, mg_Qf, mg_ilam, st);
+        else launch_mg_coarse_big(stream, mg_c, L.b, L.d, mg_w0, mg_w1, mg_Qf, mg_ilam, st);
         return;
     }
     const bool top = l == 0;
@@ -658,6 +734,11 @@ void Sim::vcycle(int l, int iter) {
 
 // MaZe step in FP64, library/src/verlet.c:31-71.
 int Sim::maze_fp64() {
+    fp64_issue();
+    return fp64_finish();
+}
+
+void Sim::fp64_issue() {
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
     timed(kTPred, 0,
           [&] { launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma); });
@@ -665,7 +746,12 @@ int Sim::maze_fp64() {
     phip = phi;
     phi = tmp;
     tmp = old_phip;
-    const int res = cg_solve(dims, pbc(), sigma, y, tol, n * n, r, p0, p1, &last_iters);
+    cg_issue(CgRun{dims, pbc(), y, r, p0, p1, n * n, 0, std::max(4, last_iters + 1), &last_iters},
+             sigma, tol);
+}
+
+int Sim::fp64_finish() {
+    const int res = cg_finish();
     launch_axpy(stream, G, -1.0, y, phi);
     return res;
 }
@@ -684,13 +770,21 @@ int Sim::maze_fp64() {
 // converged y satisfies the same stop test; only the start differs).  The
 // three y buffers then rotate: y (y_n-1), y_hist (y_n-2) and y_alt (spare).
 int Sim::maze_mixed() {
+    mixed_issue();
+    return mixed_finish();
+}
+
+// Everything up to the publication of the state after the previous step's
+// schedule of passes.
+void Sim::mixed_issue() {
     if (!d32) {
         d32 = dalloc<float>(G); r32 = dalloc<float>(G);
         q32a = dalloc<float>(G); q32b = dalloc<float>(G); y_alt = dalloc<double>(G);
     }
     if (y_extrap && !y_hist) y_hist = dalloc<double>(G);
-    const bool use_mg = precision == kMixedMG && !pb && setup_mg();
-    if (use_mg) mg[0].b = r32;
+    MixedRun& m = mx;
+    m.use_mg = precision == kMixedMG && !pb && setup_mg();
+    if (m.use_mg) mg[0].b = r32;
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
     timed(kTPred, 0,
           [&] { launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma); });
@@ -698,60 +792,68 @@ int Sim::maze_mixed() {
     phip = phi;
     phi = tmp;
     tmp = old_phip;
-    const int inner_limit = inner_max > 0 ? inner_max : n * n;
-    const RefineArgs ra{tol, inner_rtol > 0 ? inner_rtol : (use_mg ? 1e-3 : 1e-4), inner_limit,
-                        use_mg ? 1 : 0};
+    m.inner_limit = inner_max > 0 ? inner_max : n * n;
+    m.ra = RefineArgs{tol, inner_rtol > 0 ? inner_rtol : (m.use_mg ? 1e-3 : 1e-4), m.inner_limit,
+                      m.use_mg ? 1 : 0};
     // The two buffers the refinement passes alternate between (st->ysel).
-    const bool extrap = y_extrap && hist_ok;
-    double* ya = extrap ? y_alt : y;
-    double* yb = extrap ? y_hist : y_alt;
+    m.extrap = y_extrap && hist_ok;
+    m.ya = m.extrap ? y_alt : y;
+    m.yb = m.extrap ? y_hist : y_alt;
     if (y_extrap && !hist_ok) launch_copy(stream, G, y, y_hist);
-    // Each refinement pass also starts the next pass's inner solve.
-    auto refine = [&](int add) {
+    if (m.extrap)
         timed(kTRefine, 0, [&] {
-            launch_refine(stream, dims, pbc(), ya, yb, d32, sigma, r32, ra, st, red, add);
-        });
-    };
-    if (extrap)
-        timed(kTRefine, 0, [&] {
-            launch_refine_extrap(stream, dims, pbc(), y, y_hist, y_alt, sigma, r32, ra, st, red);
+            launch_refine_extrap(stream, dims, pbc(), y, y_hist, y_alt, sigma, r32, m.ra, st, red);
         });
     else
-        refine(0);
-    auto queue_pass = [&](int k, int m, bool poll) {
-        cur_pass = k;
-        int launched = 0;
-        for (; launched < std::min(m, inner_limit); launched++) inner_iter(use_mg, d32, launched);
-        if (poll) {
-            read_state();
-            while (!st_host->done && launched < inner_limit) {
-                inner_iter(use_mg, d32, launched++);
-                read_state();
-            }
-        }
-        refine(1);
-        cur_pass = -1;
-    };
-    int queued = 0;
-    for (; queued < (int)sched.size(); queued++) {
-        const int m = sched[queued] >= 0 ? std::max(sched[queued], 1) : 1 - sched[queued];
-        queue_pass(queued, m, false);
+        mixed_refine(0);
+    for (m.queued = 0; m.queued < (int)sched.size(); m.queued++) {
+        const int k = sched[m.queued];
+        mixed_pass(m.queued, k >= 0 ? std::max(k, 1) : 1 - k, false);
     }
-    read_state();
+    publish();
+}
+
+// Each refinement pass also starts the next pass's inner solve.
+void Sim::mixed_refine(int add) {
+    timed(kTRefine, 0, [&] {
+        launch_refine(stream, dims, pbc(), mx.ya, mx.yb, d32, sigma, r32, mx.ra, st, red, add);
+    });
+}
+
+// One refinement pass of `m` inner iterations; poll: iterate until the
+// inner stop (reading the state after each iteration).
+void Sim::mixed_pass(int k, int m, bool poll) {
+    cur_pass = k;
+    int launched = 0;
+    for (; launched < std::min(m, mx.inner_limit); launched++) inner_iter(mx.use_mg, d32, launched);
+    if (poll) {
+        read_state();
+        while (!st_host->done && launched < mx.inner_limit) {
+            inner_iter(mx.use_mg, d32, launched++);
+            read_state();
+        }
+    }
+    mixed_refine(1);
+    cur_pass = -1;
+}
+
+int Sim::mixed_finish() {
+    MixedRun& m = mx;
+    wait_published();
     while (!st_host->outer_done && st_host->passes < kMaxOuter) {
-        queue_pass(queued++, 1, true);
+        mixed_pass(m.queued++, 1, true);
         read_state();
     }
     const int passes = std::min(st_host->passes, kMaxPasses);
     sched.assign(st_host->pass_iters, st_host->pass_iters + passes);
     prof_flush_passes(st_host->pass_iters, passes);
-    double* fin = st_host->ysel ? yb : ya;
-    if (!extrap) {
+    double* fin = st_host->ysel ? m.yb : m.ya;
+    if (!m.extrap) {
         if (fin != y) std::swap(y, y_alt);
         if (y_extrap) hist_ok = 1;  // y_hist holds y_n-1 (copied above)
     } else {
         // y_n-1 becomes the history, the result the current y.
-        double* other = fin == ya ? yb : ya;
+        double* other = fin == m.ya ? m.yb : m.ya;
         y_hist = y;
         y = fin;
         y_alt = other;
@@ -940,9 +1042,17 @@ void Sim::part2() {
 // One MD step in the order of maze_poisson/solver.py:483-492; the Tosi-Fumi
 // forces overlap the charge spread and the field solve on the side stream.
 void Sim::md_step() {
+    step_issue();
+    step_finish();
+}
+
+// The first half of an MD step, queued without waiting: the first integrator
+// half, the pair forces (side stream), the charge spread and the field solve
+// up to the publication of the solver state.
+void Sim::step_issue() {
     part1();
-    const bool overlap = tf_overlap && p.np > 0;
-    if (overlap) {
+    step_tf = tf_overlap && p.np > 0;
+    if (step_tf) {
         MAZE_HIP(hipEventRecord(ev_pos, stream));
         MAZE_HIP(hipStreamWaitEvent(side, ev_pos, 0));
         timed(kTTf, 0, [&] { launch_tf(side, p, ps, red_side); });
@@ -950,17 +1060,39 @@ void Sim::md_step() {
     }
     update_charges(true);
     update_eps_k2();
-    const unsigned long reads = pub_count;
-    update_field();
-    if (pub_count == reads) read_state();  // every solve reads its state; kept for safety
+    step_wait = 1;
+    if (grid_type == kMazeLCG) {
+        if (precision == kFP64) fp64_issue();
+        else mixed_issue();
+    } else {
+        // The other solvers wait for their state themselves; one more read
+        // only if none was made (it brings the charge sum).
+        const unsigned long reads = pub_count;
+        step_res = update_field();
+        if (pub_count == reads) publish();
+        else step_wait = 0;
+    }
+}
+
+// Whether the state published by step_issue has arrived (never blocks).
+bool Sim::step_ready() const { return !step_wait || published(); }
+
+// The second half: completes the solve (further passes if the queued ones
+// fell short), then the forces and the second integrator half.  Returns the
+// solver's result.
+int Sim::step_finish() {
+    int res = step_res;
+    if (grid_type == kMazeLCG) res = precision == kFP64 ? fp64_finish() : mixed_finish();
+    else if (step_wait) wait_published();
     check_charge();
     check_pb_flag();
     forces_elec();
-    if (!overlap) forces_noel(false);
+    if (!step_tf) forces_noel(false);
     if (pbd.active) forces_pb();
-    if (overlap) MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
+    if (step_tf) MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
     forces_tot();
     part2();
+    return res;
 }
 
 void Sim::rescale_velocities() { launch_rescale_velocities(stream, p, ps); }

@@ -16,6 +16,19 @@ enum ItgType { kOVRVO = 0, kVerlet = 1 };
 // inside FP64 iterative refinement.
 enum Precision { kFP64 = 0, kMixedCG = 1, kMixedMG = 2 };
 
+// A block of device memory that replica allocations are carved from while it
+// is installed (arena_install) on the allocating thread: one hipMalloc per
+// replica and an exact account of its footprint.  dfree leaves arena memory
+// alone; the owner frees the block after arena_forget.
+constexpr size_t kArenaAlign = 256;
+struct DevArena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    size_t overflow = 0;  // bytes that did not fit and came from hipMalloc
+};
+void arena_install(DevArena* a);  // nullptr: plain hipMalloc again
+void arena_forget(const DevArena* a);
+
 // One level of the FP32 multigrid preconditioner.
 struct MGLevel {
     int n;
@@ -78,6 +91,7 @@ struct Sim {
     std::vector<MGLevel> mg;
     double* mg_Qf = nullptr;
     double* mg_ilam = nullptr;
+    double *mg_w0 = nullptr, *mg_w1 = nullptr;  // work arrays of a coarsest level above 12^3
     int mg_c = 0;
     // Weights of the two Jacobi smoothing sweeps (pre: omega then omega2;
     // post: omega2 then omega).  Default: the degree-2 Chebyshev polynomial
@@ -98,7 +112,28 @@ struct Sim {
     StatePub* pub_host = nullptr;  // host-coherent, written by the GPU (read_state)
     StatePub* pub_dev = nullptr;   // its device address
     unsigned int pub_seq = 0;
-    unsigned long pub_count = 0;   // read_state calls so far
+    unsigned long pub_count = 0;   // states read so far
+    unsigned int want_seq = 0;     // sequence number of the last publication queued
+
+    // A CG solve in progress (cg_issue .. cg_finish).
+    struct CgRun {
+        GridDims d;
+        PBCoef pbc;
+        double* x;
+        double* rbuf;
+        double *pin, *pout;
+        int limit, launched, chunk;
+        int* pred;
+    } cg{};
+    // A mixed-precision solve in progress (mixed_issue .. mixed_finish).
+    struct MixedRun {
+        bool use_mg = false, extrap = false;
+        double *ya = nullptr, *yb = nullptr;
+        RefineArgs ra{};
+        int inner_limit = 0, queued = 0;
+    } mx;
+    // An MD step in progress (step_issue .. step_finish).
+    int step_tf = 0, step_wait = 0, step_res = 0;
     Reduce red{};
     // The Tosi-Fumi forces depend on the positions only, so one MD step
     // (md_step) runs them on a second stream (default priority) concurrently
@@ -196,6 +231,13 @@ struct Sim {
     void part1();
     void part2();
     void md_step();
+    // md_step in two halves for callers that advance several replicas from
+    // one thread: step_issue queues the step up to the solver-state
+    // publication and returns at once, step_ready polls for that state, and
+    // step_finish completes the step (md_step = step_issue + step_finish).
+    void step_issue();
+    bool step_ready() const;
+    int step_finish();
     void rescale_velocities();
     int check_thermostat();
     double kinetic();
@@ -213,17 +255,31 @@ struct Sim {
     void inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated);
     int inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit, int first_chunk);
     bool setup_mg();
+    // Level sizes of the FP32 multigrid for an n^3 grid (empty: unavailable).
+    static std::vector<int> mg_sizes(int n);
     void free_mg();
     void vcycle(int level, int iter);
     int maze_fp64();
+    void fp64_issue();
+    int fp64_finish();
     int maze_mixed();
+    void mixed_issue();
+    int mixed_finish();
+    void mixed_refine(int add);
+    void mixed_pass(int k, int m, bool poll);
+    void cg_issue(const CgRun& c, const double* b, double tol);
+    void cg_chunk();
+    int cg_finish();
     int lcg_update();
     int mg_update(bool maze);
     void mg_vcycle(const double* in, double* out);
     PBCoef pbc() const;
 
     void sync() { MAZE_HIP(hipStreamSynchronize(stream)); }
-    void read_state();
+    void read_state();         // publish + wait_published
+    void publish();            // queue the publication of the state (sequence want_seq)
+    bool published() const;    // the state of want_seq has arrived
+    void wait_published();     // spin until it has, then copy it to st_host
     hipEvent_t next_event();
     void prof_flush(int iters_done);
     void prof_flush_passes(const int* iters, int npass);

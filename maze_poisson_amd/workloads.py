@@ -186,3 +186,53 @@ def pb_eps_field(n: int, L: float, seed: int, lo: float = 2.0, hi: float = 80.0,
         S.append(acc)
     smax = max(s.max() for s in S)
     return tuple(np.ascontiguousarray(lo + (hi - lo) * s / smax) for s in S)
+
+
+# ---- many-replica workloads ------------------------------------------------------
+ENSEMBLE_SIZES = (216, 512, 1000, 2048, 4096)
+
+
+def even_round(x: float) -> int:
+    """The even integer nearest to x."""
+    return 2 * int(round(x / 2.0))
+
+
+def ensemble_systems(count: int = 64, seed0: int = 4000, spacing_ang: float = 0.41,
+                     sizes=ENSEMBLE_SIZES, density_range=(1.30, 1.40)) -> list:
+    """Replicas of uneven size: replica r draws its ion count from `sizes` and its
+    density uniformly from `density_range` with generator seed0 + r, and gets the
+    even grid nearest to L / spacing_ang."""
+    out = []
+    for r in range(count):
+        g = np.random.default_rng(seed0 + r)
+        n_ions = int(g.choice(sizes))
+        rho = float(g.uniform(*density_range))
+        n_grid = even_round(box_length_ang(n_ions, rho) / spacing_ang)
+        out.append(nacl_system(n_ions, n_grid, seed=seed0 + r, density=rho))
+    return out
+
+
+@dataclass
+class Request:
+    arrival: float      # s after the start of the run
+    system: System
+    steps: int
+
+
+def serving_trace(count: int, rate_hz: float, seed: int = 5, n_min: int = 216,
+                  n_max: int = 32768, steps_range=(100, 2000), spacing_ang: float = 0.41,
+                  density: float = 1.379) -> list:
+    """Poisson arrivals at `rate_hz`; ion counts log-uniform in [n_min, n_max] (rounded
+    to even); steps uniform in steps_range (inclusive); even grid nearest to
+    L / spacing_ang."""
+    g = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += float(g.exponential(1.0 / rate_hz))
+        n_ions = even_round(math.exp(g.uniform(math.log(n_min), math.log(n_max))))
+        steps = int(g.integers(steps_range[0], steps_range[1] + 1))
+        n_grid = even_round(box_length_ang(n_ions, density) / spacing_ang)
+        sys_ = nacl_system(n_ions, n_grid, seed=seed * 100003 + i, density=density)
+        out.append(Request(t, sys_, steps))
+    return out

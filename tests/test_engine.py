@@ -1,0 +1,110 @@
+"""The replica engine (many independent replicas on one GPU, C-ABI maze_engine_*):
+every replica's trajectory is bit-identical to the single-replica C-ABI path (the
+driver's sequence through solver_md_loop_iter), alone and in any mix, under any
+memory budget; the admission estimate covers what a replica allocates."""
+import numpy as np
+import pytest
+
+from maze_poisson_amd import driver, workloads as W
+from maze_poisson_amd.engine import ReplicaEngine
+
+STEPS = 6
+
+# (ions, grid, precision asked): 48 halves to 12 (one-work-group coarse solve),
+# 66 and 50 halve to odd 33 and 25 (axis-pass coarse solve); 40 runs FP64 as asked.
+CASES = [(216, 48, 2), (512, 66, 2), (216, 50, 2), (216, 40, 0)]
+
+
+def systems():
+    return [W.nacl_system(n_ions, n, seed=40 + i) for i, (n_ions, n, _) in enumerate(CASES)]
+
+
+def single(lib, s, precision, steps):
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    driver.initialize(lib, s, cfg)
+    lib.lib.solver_set_precision(precision)
+    driver.bootstrap(lib, cfg)
+    for _ in range(steps):
+        lib.solver_md_loop_iter()
+    snap = driver.snapshot(lib, s, fields=False)
+    lib.solver_finalize()
+    return snap
+
+
+def same(a, b):
+    return all(np.array_equal(a[k], b[k]) for k in ("pos", "vel", "fcs_tot"))
+
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def reference_runs(hip):
+    return [single(hip, s, prec, STEPS) for s, (_, _, prec) in zip(systems(), CASES)]
+
+
+def test_engine_matches_single_replica_path(hip, reference_runs):
+    with ReplicaEngine(hip, max_slots=8) as e:
+        ids = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(systems(), CASES)]
+        st = e.run()
+        assert st["max_running"] == len(CASES)
+        for rid, ref in zip(ids, reference_runs):
+            rs = e.request_stats(rid)
+            assert rs["steps"] == STEPS and rs["failures"] == 0
+            assert same(e.result(rid), ref), rid
+        precs = [e.request_stats(r)["precision"] for r in ids]
+        assert precs == [2, 2, 2, 0]
+
+
+def test_engine_alone_and_under_budget(hip, reference_runs):
+    ss = systems()
+    with ReplicaEngine(hip, max_slots=1) as e:  # one at a time
+        ids = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(ss, CASES)]
+        assert e.run()["max_running"] == 1
+        for rid, ref in zip(ids, reference_runs):
+            assert same(e.result(rid), ref)
+    est = [ReplicaEngine.estimate_bytes(hip, s.n_grid, s.n_p, prec)
+           for s, (_, _, prec) in zip(ss, CASES)]
+    budget = int(max(est) * 1.5)  # room for one replica at a time, whatever the order
+    with ReplicaEngine(hip, budget_bytes=budget, max_slots=64) as e:
+        ids = [e.submit(s, STEPS, precision=prec, arrival=0.001 * i)
+               for i, (s, (_, _, prec)) in enumerate(zip(ss, CASES))]
+        st = e.run()
+        assert st["peak_bytes"] <= budget
+        for rid, ref in zip(ids, reference_runs):
+            assert same(e.result(rid), ref)
+        with pytest.raises(MemoryError):
+            e.submit(W.nacl_system(4096, 128, seed=2), 1)
+
+
+def test_engine_mix_of_many(hip, reference_runs):
+    """The four reference replicas inside a mix of twelve of uneven sizes."""
+    ss = systems()
+    extra = W.ensemble_systems(8, sizes=(216, 512), seed0=4100)
+    with ReplicaEngine(hip, max_slots=16) as e:
+        mine = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(ss, CASES)]
+        others = [e.submit(s, STEPS + 3 * i) for i, s in enumerate(extra)]
+        st = e.run()
+        assert st["max_running"] == 12
+        for rid, ref in zip(mine, reference_runs):
+            assert same(e.result(rid), ref)
+        for rid in others:
+            rs = e.request_stats(rid)
+            assert rs["failures"] == 0 and rs["p99_ms"] >= rs["p50_ms"] > 0
+
+
+def test_engine_memory_estimate(hip):
+    """Every allocation of a replica fits its arena (one device allocation sized by
+    the admission estimate), and the estimate covers the drop of free device memory
+    over the set-up and first step of the first replica of a run (later ones can
+    see memory of retired replicas coming back)."""
+    ss = systems()
+    with ReplicaEngine(hip, max_slots=1) as e:
+        ids = [e.submit(s, 1, precision=prec) for s, (_, _, prec) in zip(ss, CASES)]
+        e.run()
+        for k, rid in enumerate(ids):
+            rs = e.request_stats(rid)
+            assert rs["arena_overflow"] == 0, rs
+            assert 0 < rs["arena_used"] <= rs["est_bytes"] <= rs["arena_used"] + (14 << 20), rs
+            if k == 0:
+                assert 0 < rs["meas_bytes"] <= rs["est_bytes"], rs

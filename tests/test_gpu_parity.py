@@ -243,10 +243,11 @@ def test_ovrvo_thermostat_matches_oracle(hip, oracle):
     assert rel(g["pos"], o["pos"]) <= 1e-11
 
 
-@pytest.mark.parametrize("n", [32, 64])
+@pytest.mark.parametrize("n", [32, 64, 52, 78])
 def test_mg_pcg_solves(hip_raw, oracle_raw, n):
     """FP32 multigrid-preconditioned CG: few iterations, FP32-level agreement with the
-    FP64 solution of the same system."""
+    FP64 solution of the same system.  52 and 78 halve to 13 and 39 points: their
+    coarsest level is solved by the axis passes over global work arrays."""
     b = zero_mean(rand_grid(n, 31))
     x = np.zeros_like(b)
     it = hip_raw.maze_test_mg_pcg(b, x, 1e-5, n)
@@ -394,6 +395,24 @@ def test_mixed_precision_forces(hip, precision):
     cfg = driver.RunConfig(tol=W.poisson_tol(s))
     _, _, f64 = run(hip, s, tight, 3, precision=0)
     _, _, fmx = run(hip, s, cfg, 3, precision=precision)
+    d = fmx["fcs_tot"] - f64["fcs_tot"]
+    err = np.sqrt((d ** 2).mean()) / np.sqrt((f64["fcs_tot"] ** 2).mean())
+    assert err <= 1e-5, err
+
+
+@pytest.mark.parametrize("n", [66, 102])
+def test_mixed_mg_forces_odd_coarsest(hip, n):
+    """Grids that halve once to an odd size (33, 51: solved exactly by axis passes)
+    still meet the 1e-5 force RMS bound of the mixed multigrid path."""
+    s = W.nacl_system(512, n, seed=1)
+    tight = driver.RunConfig(tol=W.poisson_tol(s) * 1e-3)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    _, _, f64 = run(hip, s, tight, 3, precision=0)
+    driver.initialize(hip, s, cfg)
+    assert hip.lib.solver_mg_levels() == 2
+    hip.solver_finalize()
+    its, _, fmx = run(hip, s, cfg, 3, precision=2)
+    assert max(its) <= 12, its
     d = fmx["fcs_tot"] - f64["fcs_tot"]
     err = np.sqrt((d ** 2).mean()) / np.sqrt((f64["fcs_tot"] ** 2).mean())
     assert err <= 1e-5, err
