@@ -275,6 +275,59 @@ def cpu_baseline(steps, cores):
                       f"after the 2 bootstrap steps and 1 untimed step"}
 
 
+def ensemble_rate(lib, replicas, steps, slots):
+    """Uneven-size replicas advanced together by the replica engine: the rate of the
+    whole run (set-up and start-up steps included) and the step rate from the
+    difference of two runs of `steps` and 2 x `steps` steps per replica."""
+    from maze_poisson_amd.engine import ReplicaEngine
+    ss = W.ensemble_systems(replicas)
+    runs = {}
+    for k in (steps, 2 * steps):
+        with ReplicaEngine(lib, max_slots=slots) as e:
+            ids = [e.submit(s, k) for s in ss]
+            t = time.perf_counter()
+            st = e.run()
+            runs[k] = (time.perf_counter() - t, st,
+                       [e.request_stats(r) for r in ids])
+    (t1, st1, rs1), (t2, _, _) = runs[steps], runs[2 * steps]
+    atoms = sum(s.n_p for s in ss)
+    return {"replicas": replicas, "slots": slots, "steps_each": steps,
+            "sizes": sorted({(s.n_p, s.n_grid) for s in ss}),
+            "fp32_multigrid_replicas": sum(1 for r in rs1 if r["precision"] == 2),
+            "replica_steps_per_s": round(replicas * steps / (t2 - t1), 1),
+            "atom_steps_per_s": round(atoms * steps / (t2 - t1), 0),
+            "whole_run_replica_steps_per_s": round(replicas * steps / t1, 1),
+            "p50_ms": round(st1["p50_ms"], 3), "p99_ms": round(st1["p99_ms"], 3),
+            "max_running": int(st1["max_running"]),
+            "host_issue_us_per_step": round(st1["host_issue_s"] / (replicas * steps) * 1e6, 1),
+            "unconverged_solves": int(sum(r["failures"] for r in rs1))}
+
+
+def serving_run(lib, count, rate_hz, steps_range, slots, budget_gb):
+    """Poisson-arrival MD-segment requests through the replica engine (continuous
+    batching: admission by free slot and device-memory budget while others run)."""
+    from maze_poisson_amd.engine import ReplicaEngine
+    trace = W.serving_trace(count, rate_hz, steps_range=steps_range)
+    with ReplicaEngine(lib, budget_bytes=int(budget_gb * 2**30), max_slots=slots) as e:
+        ids = [e.submit(r.system, r.steps, arrival=r.arrival) for r in trace]
+        st = e.run()
+        rs = [e.request_stats(i) for i in ids]
+    waits = [r["admit_s"] - r["arrival_s"] for r in rs]
+    return {"requests": count, "rate_hz": rate_hz, "steps_range": list(steps_range),
+            "slots": slots, "budget_gb": budget_gb,
+            "ions_range": [min(r.system.n_p for r in trace), max(r.system.n_p for r in trace)],
+            "grid_range": [min(r.system.n_grid for r in trace),
+                           max(r.system.n_grid for r in trace)],
+            "wall_s": round(st["wall_s"], 3),
+            "atom_steps_per_s": round(st["atom_steps_per_s"], 0),
+            "replica_steps_per_s": round(st["replica_steps_per_s"], 1),
+            "p50_step_ms": round(st["p50_ms"], 3), "p99_step_ms": round(st["p99_ms"], 3),
+            "max_running": int(st["max_running"]), "peak_gb": round(st["peak_bytes"] / 2**30, 2),
+            "p50_admission_wait_s": round(float(np.median(waits)), 4),
+            "max_admission_wait_s": round(float(np.max(waits)), 4),
+            "unconverged_solves": int(sum(r["failures"] for r in rs))}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_4096ions_128grid.json")
 
 
@@ -318,6 +371,16 @@ def main():
     ap.add_argument("--cpu-child", action="store_true")
     ap.add_argument("--pb-radii-only", action="store_true",
                     help="only the radius-based PB step (8000 ions, 160^3); prints its JSON")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="only the 64-replica ensemble (replica engine); prints its JSON")
+    ap.add_argument("--ensemble-steps", type=int, default=50)
+    ap.add_argument("--serving", action="store_true",
+                    help="only the serving trace (replica engine); prints its JSON")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=[100, 2000])
+    ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one 256 MiB copy kernel first (byte-counter calibration)")
     args = ap.parse_args()
@@ -338,6 +401,17 @@ def main():
     if args.pb_radii_only:
         print(json.dumps(pb_radii_rate(lib, args.steps, args.prof_steps)))
         return
+    if args.ensemble:
+        ens = ensemble_rate(lib, 64, args.ensemble_steps, 64)
+        ser = ensemble_rate(lib, 64, max(10, args.ensemble_steps // 2), 1)
+        ens["serial_replica_steps_per_s"] = ser["replica_steps_per_s"]
+        print(json.dumps(ens))
+        return
+    if args.serving:
+        print(json.dumps(serving_run(lib, args.serve_requests, args.serve_rate,
+                                     tuple(args.serve_steps), args.serve_slots,
+                                     args.serve_budget_gb)))
+        return
     head = run_path(lib, PATHS["mixed_mg"], args.steps, args.warmup, args.steps, args.prof_steps)
     line_extra = {}
     if not args.no_extra:
@@ -355,6 +429,7 @@ def main():
         line_extra = {
             "runner_steps_per_s": round(runner_rate(lib, 200), 2),
             "pb_radii": pb_radii_rate(lib, max(10, args.steps // 10), 5),
+            "ensemble_64_replicas": ensemble_rate(lib, 64, 50, 64),
             "python_driver_steps_per_s": round(driver_rate(lib, PATHS["mixed_mg"], 100, False), 2),
             "python_driver_with_getters_steps_per_s": round(
                 driver_rate(lib, PATHS["mixed_mg"], 100, True), 2),

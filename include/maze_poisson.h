@@ -291,14 +291,19 @@ int maze_test_mg_tail(const float *b, float *d, int n);
  * estimate fits the budget; admitted replicas advance concurrently, each
  * step queued as soon as the replica's previous solve state is published.
  * A replica's results do not depend on what else runs (own buffers,
- * deterministic kernels).  Not thread-safe; one engine per thread. */
+ * deterministic kernels).  The engine's functions are not thread-safe: call
+ * them from one thread (maze_engine_run starts its own workers). */
 typedef struct maze_engine maze_engine;
 /* budget <= 0: 90 % of the free device memory; max_slots: replicas at once. */
 maze_engine *maze_engine_create(long long hbm_budget_bytes, int max_slots);
 void maze_engine_destroy(maze_engine *e);
+/* Host threads that drive the replicas (default 1): each admits and steps
+ * its own share of the slots.  Measured on 64 replicas: one thread is
+ * fastest (kernel launches from several threads contend in the runtime). */
+void maze_engine_set_threads(maze_engine *e, int threads);
 /* Device bytes one replica needs (grids, multigrid levels, particles, pair
  * partials, scratch), and the precision it runs: 2 (FP32 multigrid-PCG in
- * FP64 refinement) needs an even grid whose halvings end at or below 96
+ * FP64 refinement) needs an even grid whose halvings end at or below 128
  * points, else 0 (FP64 CG) is used; 1 runs as asked. */
 long long maze_engine_estimate_bytes(int n_grid, int np, int precision);
 int maze_engine_precision_for(int n_grid, int precision);
@@ -324,7 +329,8 @@ int maze_engine_result(maze_engine *e, int id, double *pos, double *vel, double 
 int maze_engine_request_stats(maze_engine *e, int id, double *out, int max);
 /* Last run (up to max values): wall time (s), replica-steps/s, atom-steps/s,
  * p50 / p99 step latency over all steps (ms), peak estimated device bytes in
- * use, most replicas running at once. */
+ * use, most replicas running at once, host seconds spent admitting (set-up
+ * and start-up steps), issuing and finishing steps. */
 int maze_engine_stats(maze_engine *e, double *out, int max);
 
 #ifdef __cplusplus

@@ -12,11 +12,14 @@
 // estimate fits the budget; a finished replica's results are copied out and
 // its memory returned before the next admission.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "sim.hpp"
@@ -61,7 +64,7 @@ struct Request {
 };
 
 // The precision a replica of an n^3 grid runs: the FP32 multigrid needs an
-// even grid that halves to at most 96 points (Sim::mg_sizes); otherwise the
+// even grid that halves to at most 128 points (Sim::mg_sizes); otherwise the
 // FP64 CG, which is faster than the FP32 Jacobi-CG inside refinement.
 static int precision_for(int n, int want) {
     if (want == kMixedMG && Sim::mg_sizes(n).empty()) return kFP64;
@@ -109,12 +112,21 @@ static long long estimate_bytes(int n, int np, int precision) {
 struct Engine {
     long long budget = 0;
     int max_slots = 1;
+    int threads = 1;
     std::vector<Request> reqs;
-    // last run
+    // last run; the admission queue and the memory account are shared by the
+    // worker threads under `mu`
+    std::mutex mu;
+    std::vector<int> order;
+    size_t next = 0;
+    std::atomic<int> left{0};
+    int running = 0, max_running = 0;
     long long used = 0, peak = 0;
-    int max_running = 0;
-    double t0 = 0, t_first = 0, t_end = 0;
+    double t0 = 0, t_end = 0;
+    double host_admit = 0, host_issue = 0, host_finish = 0;  // host seconds per phase
 
+    // Set-up of an admitted request: its arena, the replica state, the two
+    // start-up steps with cold solves and the first step's issue.
     void admit(Request& r) {
         r.used_precision = precision_for(r.n, r.precision);
         size_t free0 = 0, total = 0;
@@ -143,18 +155,16 @@ struct Engine {
         }
         r.t_admit = r.t_last = now_s() - t0;
         r.state = 1;
-        used += r.est_bytes;
-        peak = std::max(peak, used);
         if (r.steps > 0) s.step_issue();  // allocates the mixed solver's arrays
         arena_install(nullptr);
         size_t free1 = 0;
         MAZE_HIP(hipMemGetInfo(&free1, &total));
-        r.meas_bytes = (long long)free0 - (long long)free1;
+        r.meas_bytes = (long long)free0 - (long long)free1;  // exact with one worker only
         r.arena_used = (long long)r.arena->used;
         r.arena_overflow = (long long)r.arena->overflow;
-        if (r.steps == 0) retire(r);
     }
 
+    // Results out, replica and arena freed.
     void retire(Request& r) {
         Sim& s = *r.sim;
         r.out_pos.resize(3L * r.np);
@@ -173,59 +183,98 @@ struct Engine {
         arena_forget(r.arena.get());
         MAZE_HIP(hipFree(r.arena->base));
         r.arena.reset();
-        used -= r.est_bytes;
         r.state = 2;
         r.t_done = now_s() - t0;
+        std::lock_guard<std::mutex> g(mu);
+        used -= r.est_bytes;
+        running--;
+        left--;
     }
 
-    // Advances every waiting request to the end of its steps.
-    int run() {
-        t0 = now_s();
-        std::vector<int> order;
-        for (int i = 0; i < (int)reqs.size(); i++)
-            if (reqs[i].state == 0) order.push_back(i);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
-        std::vector<int> running;
-        size_t next = 0;
-        int left = (int)order.size();
-        used = peak = 0;
-        max_running = 0;
-        t_first = order.empty() ? 0.0 : reqs[order[0]].arrival;
-        while (left > 0) {
-            const double t = now_s() - t0;
-            while (next < order.size()) {
-                Request& r = reqs[order[next]];
-                if (r.arrival > t || (int)running.size() >= max_slots || used + r.est_bytes > budget)
-                    break;
-                admit(r);
-                next++;
-                if (r.state == 2) left--;
-                else running.push_back(order[next - 1]);
+    // Next request of the queue for worker `mine` (holding `held` replicas of
+    // at most `share`), or -1: arrived, a slot free, its estimate in budget.
+    int take(int held, int share) {
+        std::lock_guard<std::mutex> g(mu);
+        if (next >= order.size() || held >= share || running >= max_slots) return -1;
+        Request& r = reqs[order[next]];
+        if (r.arrival > now_s() - t0 || used + r.est_bytes > budget) return -1;
+        used += r.est_bytes;
+        peak = std::max(peak, used);
+        running++;
+        max_running = std::max(max_running, running);
+        return order[next++];
+    }
+
+    // One worker: admits requests while it has room, finishes the steps whose
+    // state has arrived and issues their next steps.
+    void worker(int share) {
+        std::vector<int> mine;
+        double h_admit = 0, h_issue = 0, h_finish = 0;
+        while (left.load() > 0) {
+            for (int id; (id = take((int)mine.size(), share)) >= 0;) {
+                const double ta = now_s();
+                admit(reqs[id]);
+                h_admit += now_s() - ta;
+                if (reqs[id].steps == 0) retire(reqs[id]);
+                else mine.push_back(id);
             }
-            max_running = std::max(max_running, (int)running.size());
             bool progressed = false;
-            for (size_t a = 0; a < running.size();) {
-                Request& r = reqs[running[a]];
+            for (size_t a = 0; a < mine.size();) {
+                Request& r = reqs[mine[a]];
                 if (!r.sim->step_ready()) {
                     a++;
                     continue;
                 }
                 progressed = true;
+                const double tf = now_s();
                 if (r.sim->step_finish() < 0) r.failures++;
                 const double tn = now_s() - t0;
+                h_finish += tn + t0 - tf;
                 r.lat.push_back(tn - r.t_last);
                 r.t_last = tn;
                 if (++r.done == r.steps) {
                     retire(r);
-                    running.erase(running.begin() + (long)a);
-                    left--;
+                    mine.erase(mine.begin() + (long)a);
                     continue;
                 }
+                const double ti = now_s();
                 r.sim->step_issue();
+                h_issue += now_s() - ti;
                 a++;
             }
-            if (!progressed) __builtin_ia32_pause();
+            if (!progressed) {
+                if (mine.empty()) std::this_thread::yield();
+                else __builtin_ia32_pause();
+            }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        host_admit += h_admit;
+        host_issue += h_issue;
+        host_finish += h_finish;
+    }
+
+    // Advances every waiting request to the end of its steps, with `threads`
+    // host threads each driving its own share of the replicas.
+    int run() {
+        t0 = now_s();
+        order.clear();
+        for (int i = 0; i < (int)reqs.size(); i++)
+            if (reqs[i].state == 0) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
+        next = 0;
+        left = (int)order.size();
+        running = max_running = 0;
+        used = peak = 0;
+        host_admit = host_issue = host_finish = 0;
+        const int nt = std::max(1, std::min(threads, max_slots));
+        const int share = (max_slots + nt - 1) / nt;
+        if (nt == 1) {
+            worker(share);
+        } else {
+            std::vector<std::thread> ts;
+            for (int w = 0; w < nt; w++) ts.emplace_back([this, share] { worker(share); });
+            for (auto& t : ts) t.join();
         }
         t_end = now_s() - t0;
         return 0;
@@ -250,7 +299,12 @@ maze_engine* maze_engine_create(long long hbm_budget_bytes, int max_slots) {
     }
     e->budget = hbm_budget_bytes;
     e->max_slots = std::max(1, max_slots);
+    e->threads = 1;
     return reinterpret_cast<maze_engine*>(e);
+}
+
+void maze_engine_set_threads(maze_engine* h, int threads) {
+    reinterpret_cast<Engine*>(h)->threads = std::max(1, std::min(threads, 64));
 }
 
 void maze_engine_destroy(maze_engine* h) { delete reinterpret_cast<Engine*>(h); }
@@ -335,7 +389,8 @@ int maze_engine_request_stats(maze_engine* h, int id, double* out, int max) {
 
 // out: wall time of the last run (s), replica-steps/s, atom-steps/s, p50 and
 // p99 step latency over all steps (ms), peak estimated device bytes, most
-// replicas running at once.
+// replicas running at once, host seconds spent admitting, issuing and
+// finishing steps.
 int maze_engine_stats(maze_engine* h, double* out, int max) {
     Engine& e = *reinterpret_cast<Engine*>(h);
     std::vector<double> all;
@@ -347,14 +402,17 @@ int maze_engine_stats(maze_engine* h, double* out, int max) {
         atom_steps += (double)r.done * r.np;
     }
     const double wall = e.t_end;
-    const double v[7] = {wall,
-                         wall > 0 ? steps / wall : 0.0,
-                         wall > 0 ? atom_steps / wall : 0.0,
-                         1e3 * maze::quantile(all, 0.5),
-                         1e3 * maze::quantile(all, 0.99),
-                         (double)e.peak,
-                         (double)e.max_running};
-    const int k = std::min(max, 7);
+    const double v[10] = {wall,
+                          wall > 0 ? steps / wall : 0.0,
+                          wall > 0 ? atom_steps / wall : 0.0,
+                          1e3 * maze::quantile(all, 0.5),
+                          1e3 * maze::quantile(all, 0.99),
+                          (double)e.peak,
+                          (double)e.max_running,
+                          e.host_admit,
+                          e.host_issue,
+                          e.host_finish};
+    const int k = std::min(max, 10);
     for (int i = 0; i < k; i++) out[i] = v[i];
     return k;
 }

@@ -143,7 +143,7 @@ void launch_mg_prolong_add(hipStream_t s, int nc, const float* ec, float* d, CGS
 // c <= kMgCoarseLds, else six axis passes over the work arrays w0, w1 (c^3
 // doubles each) for c <= kMgCoarseMax.
 constexpr int kMgCoarseLds = 12;
-constexpr int kMgCoarseMax = 96;
+constexpr int kMgCoarseMax = 128;
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st);
 void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,

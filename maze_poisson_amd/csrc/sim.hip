@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "sim.hpp"
 
@@ -30,10 +31,12 @@ static void fatal(const char* msg) {
 // allocations that do not fit fall back to hipMalloc and are counted.
 namespace {
 thread_local DevArena* t_arena = nullptr;
+std::mutex g_arena_mu;
 std::vector<const DevArena*> g_arenas;  // live arenas, for dfree
 
 bool in_arena(const void* p) {
     const char* c = static_cast<const char*>(p);
+    std::lock_guard<std::mutex> g(g_arena_mu);
     for (const DevArena* a : g_arenas)
         if (c >= a->base && c < a->base + a->size) return true;
     return false;
@@ -42,12 +45,14 @@ bool in_arena(const void* p) {
 
 void arena_install(DevArena* a) {
     t_arena = a;
+    std::lock_guard<std::mutex> g(g_arena_mu);
     if (a && std::find(g_arenas.begin(), g_arenas.end(), a) == g_arenas.end()) g_arenas.push_back(a);
 }
 
 void arena_forget(const DevArena* a) {
-    g_arenas.erase(std::remove(g_arenas.begin(), g_arenas.end(), a), g_arenas.end());
     if (t_arena == a) t_arena = nullptr;
+    std::lock_guard<std::mutex> g(g_arena_mu);
+    g_arenas.erase(std::remove(g_arenas.begin(), g_arenas.end(), a), g_arenas.end());
 }
 
 template <class T>
@@ -615,8 +620,8 @@ void Sim::free_mg() {
 
 // Levels n, n/2, ... while the size is even and above 12; the last level is
 // solved exactly (in one work-group up to 12 points per side, by axis passes
-// over global work arrays up to 96).  Unavailable when halving stops at an
-// odd size above 96.
+// over global work arrays up to 128).  Unavailable when halving stops at an
+// odd size above 128.
 std::vector<int> Sim::mg_sizes(int n) {
     std::vector<int> sizes{n};
     while (sizes.back() % 2 == 0 && sizes.back() > kMgCoarseLds) sizes.push_back(sizes.back() / 2);

@@ -24,20 +24,23 @@ REQUEST_FIELDS = ("arrival_s", "admit_s", "done_s", "steps", "mean_ms", "p50_ms"
                   "est_bytes", "meas_bytes", "precision", "failures", "arena_used",
                   "arena_overflow")
 STATS_FIELDS = ("wall_s", "replica_steps_per_s", "atom_steps_per_s", "p50_ms", "p99_ms",
-                "peak_bytes", "max_running")
+                "peak_bytes", "max_running", "host_admit_s", "host_issue_s", "host_finish_s")
 
 
 class ReplicaEngine:
     """Owns one ``maze_engine``; requests are :class:`~maze_poisson_amd.workloads.System`
     replicas with a step count and an arrival time."""
 
-    def __init__(self, lib: MazeLib, budget_bytes: int = 0, max_slots: int = 64):
+    def __init__(self, lib: MazeLib, budget_bytes: int = 0, max_slots: int = 64,
+                 threads: int | None = None):
         if not lib.has("maze_engine_create"):
             raise RuntimeError(f"{lib.path} has no replica engine")
         self.lib = lib
         self.h = lib.maze_engine_create(int(budget_bytes), int(max_slots))
         if not self.h:
             raise RuntimeError("maze_engine_create failed")
+        if threads is not None:
+            lib.maze_engine_set_threads(self.h, int(threads))
         self.systems = {}
 
     def close(self):

@@ -44,7 +44,7 @@ def reference_runs(hip):
 
 
 def test_engine_matches_single_replica_path(hip, reference_runs):
-    with ReplicaEngine(hip, max_slots=8) as e:
+    with ReplicaEngine(hip, max_slots=8, threads=1) as e:
         ids = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(systems(), CASES)]
         st = e.run()
         assert st["max_running"] == len(CASES)
@@ -78,19 +78,26 @@ def test_engine_alone_and_under_budget(hip, reference_runs):
 
 
 def test_engine_mix_of_many(hip, reference_runs):
-    """The four reference replicas inside a mix of twelve of uneven sizes."""
+    """The four reference replicas inside a mix of twelve of uneven sizes, driven by
+    one host thread (all twelve admitted before the first step) and by four."""
     ss = systems()
     extra = W.ensemble_systems(8, sizes=(216, 512), seed0=4100)
-    with ReplicaEngine(hip, max_slots=16) as e:
-        mine = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(ss, CASES)]
-        others = [e.submit(s, STEPS + 3 * i) for i, s in enumerate(extra)]
-        st = e.run()
-        assert st["max_running"] == 12
-        for rid, ref in zip(mine, reference_runs):
-            assert same(e.result(rid), ref)
-        for rid in others:
-            rs = e.request_stats(rid)
-            assert rs["failures"] == 0 and rs["p99_ms"] >= rs["p50_ms"] > 0
+    results = []
+    for threads in (1, 4):
+        with ReplicaEngine(hip, max_slots=16, threads=threads) as e:
+            mine = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(ss, CASES)]
+            others = [e.submit(s, STEPS + 3 * i) for i, s in enumerate(extra)]
+            st = e.run()
+            if threads == 1:
+                assert st["max_running"] == 12
+            for rid, ref in zip(mine, reference_runs):
+                assert same(e.result(rid), ref)
+            for rid in others:
+                rs = e.request_stats(rid)
+                assert rs["failures"] == 0 and rs["p99_ms"] >= rs["p50_ms"] > 0
+            results.append([e.result(r) for r in others])
+    for a, b in zip(*results):
+        assert same(a, b)
 
 
 def test_engine_memory_estimate(hip):

@@ -400,6 +400,19 @@ def test_mixed_precision_forces(hip, precision):
     assert err <= 1e-5, err
 
 
+def test_mg_pcg_largest_coarsest(hip_raw):
+    """254^3 halves once to 127, the largest coarsest level solved exactly (axis
+    passes over 127^3 work arrays): the FP32 MG-PCG solution satisfies the discrete
+    equation to FP32 accuracy in few iterations."""
+    n = 254
+    b = zero_mean(rand_grid(n, 33))
+    x = np.zeros_like(b)
+    it = hip_raw.maze_test_mg_pcg(b, x, 1e-5, n)
+    assert 0 < it <= 20, it
+    r = _lap(x) - b
+    assert np.abs(r).max() <= 1e-3 * np.abs(b).max()
+
+
 @pytest.mark.parametrize("n", [66, 102])
 def test_mixed_mg_forces_odd_coarsest(hip, n):
     """Grids that halve once to an odd size (33, 51: solved exactly by axis passes)
