@@ -301,6 +301,10 @@ void maze_engine_destroy(maze_engine *e);
  * its own share of the slots.  Measured on 64 replicas: one thread is
  * fastest (kernel launches from several threads contend in the runtime). */
 void maze_engine_set_threads(maze_engine *e, int threads);
+/* 1 (default): each replica captures its multigrid-PCG inner iteration (the
+ * V-cycle kernels, direction and update) once as HIP graphs and replays them;
+ * 0: separate launches.  Results are identical. */
+void maze_engine_set_graphs(maze_engine *e, int on);
 /* Device bytes one replica needs (grids, multigrid levels, particles, pair
  * partials, scratch), and the precision it runs: 2 (FP32 multigrid-PCG in
  * FP64 refinement) needs an even grid whose halvings end at or below 128

@@ -223,6 +223,7 @@ void solver_set_y_extrapolation(int on) {
 void solver_set_mg_omega(double omega) {
     S().mg_omega = S().mg_omega2 = omega;
     S().sched.clear();
+    S().drop_graphs();
 }
 
 // Two different sweep weights (a degree-2 polynomial smoother).
@@ -230,6 +231,7 @@ void solver_set_mg_omegas(double omega1, double omega2) {
     S().mg_omega = omega1;
     S().mg_omega2 = omega2;
     S().sched.clear();
+    S().drop_graphs();
 }
 
 // Number of multigrid levels for the current grid (0: unavailable).

@@ -113,6 +113,7 @@ struct Engine {
     long long budget = 0;
     int max_slots = 1;
     int threads = 1;
+    int graphs = 1;  // replicas replay their inner iterations as HIP graphs
     std::vector<Request> reqs;
     // last run; the admission queue and the memory account are shared by the
     // worker threads under `mu`
@@ -143,6 +144,7 @@ struct Engine {
         s.init_particles(r.n, r.n_typ, r.L, h, r.np, 0, r.types.data(), r.pos.data(),
                          r.vel.data(), r.mass.data(), r.charges.data(), r.tf.data());
         s.init_integrator(r.dt, 0.0, 0.0, kVerlet, 0);
+        s.use_graphs = graphs;
         // The two start-up steps with cold solves (maze_poisson/solver.py:376-398).
         for (int k = 0; k < 2; k++) {
             s.update_charges();
@@ -306,6 +308,8 @@ maze_engine* maze_engine_create(long long hbm_budget_bytes, int max_slots) {
 void maze_engine_set_threads(maze_engine* h, int threads) {
     reinterpret_cast<Engine*>(h)->threads = std::max(1, std::min(threads, 64));
 }
+
+void maze_engine_set_graphs(maze_engine* h, int on) { reinterpret_cast<Engine*>(h)->graphs = on != 0; }
 
 void maze_engine_destroy(maze_engine* h) { delete reinterpret_cast<Engine*>(h); }
 

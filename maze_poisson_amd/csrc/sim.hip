@@ -560,6 +560,30 @@ int Sim::cg_count_solve(const GridDims& d, const PBCoef& cpb, const double* b, d
 // One iteration of the FP32-storage inner solve (Jacobi-CG, or CG with the
 // multigrid preconditioner of Sim::vcycle); `it` is its 0-based index.
 void Sim::inner_iter(bool use_mg, float* x, int it) {
+    if (use_graphs && use_mg && !prof && x == d32) {
+        const int v = it == 0 ? 0 : ((it & 1) ? 1 : 2);
+        if (!inner_graph[v]) {
+            hipGraph_t g = nullptr;
+            MAZE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            inner_body(use_mg, x, it);
+            MAZE_HIP(hipStreamEndCapture(stream, &g));
+            MAZE_HIP(hipGraphInstantiate(&inner_graph[v], g, nullptr, nullptr, 0));
+            MAZE_HIP(hipGraphDestroy(g));
+        }
+        MAZE_HIP(hipGraphLaunch(inner_graph[v], stream));
+        return;
+    }
+    inner_body(use_mg, x, it);
+}
+
+void Sim::drop_graphs() {
+    for (hipGraphExec_t& g : inner_graph) {
+        if (g) MAZE_HIP(hipGraphExecDestroy(g));
+        g = nullptr;
+    }
+}
+
+void Sim::inner_body(bool use_mg, float* x, int it) {
     float* pin = (it & 1) ? q32b : q32a;
     float* pout = (it & 1) ? q32a : q32b;
     if (use_mg) {
@@ -605,6 +629,7 @@ int Sim::inner_solve(bool use_mg, const float* b, float* x, double rtol, int lim
 }
 
 void Sim::free_mg() {
+    drop_graphs();
     for (size_t l = 0; l < mg.size(); l++) {
         if (l > 0) dfree(mg[l].b);
         dfree(mg[l].d);

@@ -105,6 +105,13 @@ struct Sim {
         return 1.0 / (k == 0 ? mid + half : mid - half);
     }
     int mg_ok = -1;             // -1 not set up, 0 unavailable for this n, 1 ready
+    // Replay each multigrid-PCG inner iteration as a captured HIP graph (three
+    // variants: the first iteration, odd and even ones) instead of its seven
+    // or more separate launches: the kernels and their arguments are fixed
+    // for a replica (buffers, weights; the stop flags are read on the device).
+    // Dropped with the multigrid levels (free_mg) and by set_mg_omegas.
+    int use_graphs = 0;
+    hipGraphExec_t inner_graph[3] = {nullptr, nullptr, nullptr};
     int mg_tail = 0;            // first level handled by the one-work-group tail kernel
 
     CGState* st = nullptr;      // device
@@ -252,6 +259,8 @@ struct Sim {
                        double rtol, int maxit, double* rbuf, double* pa, double* pbuf, int* pred);
     void restrict_pb_coef();
     void inner_iter(bool use_mg, float* x, int it);
+    void inner_body(bool use_mg, float* x, int it);
+    void drop_graphs();
     void inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated);
     int inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit, int first_chunk);
     bool setup_mg();

@@ -32,7 +32,7 @@ class ReplicaEngine:
     replicas with a step count and an arrival time."""
 
     def __init__(self, lib: MazeLib, budget_bytes: int = 0, max_slots: int = 64,
-                 threads: int | None = None):
+                 threads: int | None = None, graphs: bool | None = None):
         if not lib.has("maze_engine_create"):
             raise RuntimeError(f"{lib.path} has no replica engine")
         self.lib = lib
@@ -41,6 +41,8 @@ class ReplicaEngine:
             raise RuntimeError("maze_engine_create failed")
         if threads is not None:
             lib.maze_engine_set_threads(self.h, int(threads))
+        if graphs is not None:
+            lib.maze_engine_set_graphs(self.h, 1 if graphs else 0)
         self.systems = {}
 
     def close(self):

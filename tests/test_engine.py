@@ -79,12 +79,13 @@ def test_engine_alone_and_under_budget(hip, reference_runs):
 
 def test_engine_mix_of_many(hip, reference_runs):
     """The four reference replicas inside a mix of twelve of uneven sizes, driven by
-    one host thread (all twelve admitted before the first step) and by four."""
+    one host thread replaying inner iterations as graphs (all twelve admitted
+    before the first step) and by four threads with separate launches."""
     ss = systems()
     extra = W.ensemble_systems(8, sizes=(216, 512), seed0=4100)
     results = []
-    for threads in (1, 4):
-        with ReplicaEngine(hip, max_slots=16, threads=threads) as e:
+    for threads, graphs in ((1, True), (4, False)):
+        with ReplicaEngine(hip, max_slots=16, threads=threads, graphs=graphs) as e:
             mine = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(ss, CASES)]
             others = [e.submit(s, STEPS + 3 * i) for i, s in enumerate(extra)]
             st = e.run()
