@@ -60,6 +60,7 @@ struct Request {
     double t_admit = 0, t_done = 0, t_last = 0;
     std::vector<double> lat;  // seconds per step, as observed by the loop
     std::unique_ptr<Sim> sim;
+    std::pair<hipStream_t, hipStream_t> streams{nullptr, nullptr};
     std::vector<double> out_pos, out_vel, out_f;
 };
 
@@ -126,6 +127,37 @@ struct Engine {
     double t0 = 0, t_end = 0;
     double host_admit = 0, host_issue = 0, host_finish = 0;  // host seconds per phase
 
+    // Streams of retired replicas, reused by the next ones (creating a stream
+    // costs milliseconds); destroyed with the engine.
+    std::vector<std::pair<hipStream_t, hipStream_t>> stream_pool;
+
+    std::pair<hipStream_t, hipStream_t> take_streams() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (!stream_pool.empty()) {
+                auto p = stream_pool.back();
+                stream_pool.pop_back();
+                return p;
+            }
+        }
+        std::pair<hipStream_t, hipStream_t> p;
+        MAZE_HIP(hipStreamCreateWithFlags(&p.first, hipStreamNonBlocking));
+        MAZE_HIP(hipStreamCreateWithFlags(&p.second, hipStreamNonBlocking));
+        return p;
+    }
+
+    void give_streams(std::pair<hipStream_t, hipStream_t> p) {
+        std::lock_guard<std::mutex> g(mu);
+        stream_pool.push_back(p);
+    }
+
+    ~Engine() {
+        for (auto& p : stream_pool) {
+            MAZE_HIP(hipStreamDestroy(p.first));
+            MAZE_HIP(hipStreamDestroy(p.second));
+        }
+    }
+
     // Set-up of an admitted request: its arena, the replica state, the two
     // start-up steps with cold solves and the first step's issue.
     void admit(Request& r) {
@@ -136,7 +168,8 @@ struct Engine {
         r.arena->size = (size_t)arena_bytes(r.n, r.np, r.used_precision);
         MAZE_HIP(hipMalloc(&r.arena->base, r.arena->size));
         arena_install(r.arena.get());
-        r.sim = std::make_unique<Sim>();
+        r.streams = take_streams();
+        r.sim = std::make_unique<Sim>(r.streams.first, r.streams.second);
         Sim& s = *r.sim;
         const double h = r.L / r.n;
         s.init_grid(r.n, r.L, h, r.tol, 1.0, 1.0, kMazeLCG, kPNone);
@@ -181,7 +214,8 @@ struct Engine {
                                     s.stream));
         }
         s.sync();
-        r.sim.reset();
+        r.sim.reset();  // waits for both streams
+        give_streams(r.streams);
         arena_forget(r.arena.get());
         MAZE_HIP(hipFree(r.arena->base));
         r.arena.reset();

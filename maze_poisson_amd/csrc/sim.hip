@@ -98,9 +98,19 @@ static int choose_slab(int n) {
     return (int)s;
 }
 
-Sim::Sim() {
-    MAZE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    MAZE_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+Sim::Sim() : Sim(nullptr, nullptr) {}
+
+// main / side: streams the caller keeps (both or neither), for callers that
+// create many replicas (stream creation costs milliseconds).
+Sim::Sim(hipStream_t main, hipStream_t side_) {
+    own_streams = main == nullptr;
+    if (own_streams) {
+        MAZE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        MAZE_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    } else {
+        stream = main;
+        side = side_;
+    }
     MAZE_HIP(hipEventCreateWithFlags(&ev_pos, hipEventDisableTiming));
     MAZE_HIP(hipEventCreateWithFlags(&ev_tf, hipEventDisableTiming));
     red_side.psum = dalloc<double>(kMaxPartials);
@@ -144,8 +154,11 @@ Sim::~Sim() {
     MAZE_HIP(hipEventDestroy(ev_pos));
     MAZE_HIP(hipEventDestroy(ev_tf));
     MAZE_HIP(hipStreamSynchronize(side));
-    MAZE_HIP(hipStreamDestroy(side));
-    MAZE_HIP(hipStreamDestroy(stream));
+    drop_graphs();
+    if (own_streams) {
+        MAZE_HIP(hipStreamDestroy(side));
+        MAZE_HIP(hipStreamDestroy(stream));
+    }
 }
 
 void Sim::free_grid() {

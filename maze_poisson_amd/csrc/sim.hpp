@@ -212,7 +212,9 @@ struct Sim {
     }
 
     Sim();
+    Sim(hipStream_t main, hipStream_t side);
     ~Sim();
+    int own_streams = 1;
 
     void init_grid(int n, double L, double h, double tol, double eps, double eps_int, int type,
                    int precond);
