@@ -482,6 +482,7 @@ int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, i
 int maze_test_mg_tail(const float* b, float* d, int n) {
     Sim* s = scratch_sim(n);
     s->precision = maze::kMixedMG;
+    s->mg_stop = 12;  // the hierarchy whose smallest levels run in the tail
     if (!s->setup_mg() || s->mg_tail >= (int)s->mg.size()) {
         delete s;
         return -1;

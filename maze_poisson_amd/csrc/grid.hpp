@@ -140,9 +140,10 @@ void launch_mg_post(hipStream_t s, const GridDims& d, const float* b, const floa
 void launch_mg_restrict(hipStream_t s, int n, const float* r, float* bc, CGState* st);
 void launch_mg_prolong_add(hipStream_t s, int nc, const float* ec, float* d, CGState* st);
 // Exact solve on the coarsest level: one work-group with the grid in LDS for
-// c <= kMgCoarseLds, else six axis passes over the work arrays w0, w1 (c^3
-// doubles each) for c <= kMgCoarseMax.
-constexpr int kMgCoarseLds = 12;
+// c <= kMgCoarseLds, else six
+// axis passes over the work arrays w0, w1 (c^3 doubles each) for
+// c <= kMgCoarseMax.
+constexpr int kMgCoarseLds = 16;
 constexpr int kMgCoarseMax = 128;
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st);

@@ -776,7 +776,7 @@ void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d
 // real orthonormal eigenbasis Qf (row t = eigenvector t), divide by the
 // eigenvalue sum (the constant mode is dropped), transform back.  A and B
 // are c^3 double work arrays in LDS; `nt` threads cooperate.
-constexpr int kMaxCoarse = kMgCoarseLds;
+constexpr int kMaxCoarse = 12;  // coarsest level of the one-work-group tail
 
 __device__ void coarse_exact(int c, const double* Q, const double* __restrict__ ilam, double* A,
                              double* B, int tid, int nt) {
@@ -825,26 +825,87 @@ __device__ void coarse_exact(int c, const double* Q, const double* __restrict__ 
     lds_barrier();
 }
 
-__global__ __launch_bounds__(kBlock) void k_mg_coarse(int c, const float* __restrict__ b,
-                                                      float* __restrict__ e,
-                                                      const double* __restrict__ Qf,
-                                                      const double* __restrict__ ilam,
-                                                      const CGState* st) {
+// Exact solve of a coarsest level of at most kMgCoarseLds^3 points in one
+// work-group: the separable transform of coarse_exact, one axis per pass
+// between two LDS arrays.  Four threads share one line of the level along
+// the axis being transformed (the same input address: a broadcast) and each
+// forms four of its c outputs.  Lines are padded to c + 1 values, which puts
+// the lines of one wave on different LDS banks in all three orientations.
+// The eigenbasis sits in LDS twice, as QF[m][t] = Q[t][m] (forward) and
+// QI[m][t] = Q[m][t] (inverse), zero-padded to kMgCoarseLds^2.
+// CT > 0 fixes c at compile time (fully unrolled line loops); CT = 0 reads it
+// from c_arg.  The inverse eigenvalues of a thread's points are loaded first
+// and kept in registers.
+template <int CT>
+__global__ __launch_bounds__(4 * kMgCoarseLds * kMgCoarseLds) void k_mg_coarse(
+    int c_arg, const float* __restrict__ b, float* __restrict__ e, const double* __restrict__ Qf,
+    const double* __restrict__ ilam, const CGState* st) {
     if (st->done) return;
-    __shared__ double A[kMaxCoarse * kMaxCoarse * kMaxCoarse];
-    __shared__ double B[kMaxCoarse * kMaxCoarse * kMaxCoarse];
-    __shared__ double Q[kMaxCoarse * kMaxCoarse];
-    const int N = c * c * c;
-    for (int t = threadIdx.x; t < c * c; t += kBlock) Q[t] = Qf[t];
-    for (int t = threadIdx.x; t < N; t += kBlock) A[t] = (double)b[t];
-    __syncthreads();
-    coarse_exact(c, Q, ilam, A, B, threadIdx.x, kBlock);
-    for (int t = threadIdx.x; t < N; t += kBlock) e[t] = (float)A[t];
+    constexpr int CM = kMgCoarseLds, NT = 4 * CM * CM, PER = CM * CM * CM / NT;
+    __shared__ double A[CM * CM * (CM + 1)], B[CM * CM * (CM + 1)];
+    __shared__ double QF[CM * CM], QI[CM * CM];
+    const int c = CT > 0 ? CT : c_arg;
+    const int tid = threadIdx.x, c2 = c * c, N = c2 * c, pitch = c + 1;
+    double il[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) il[k] = tid + k * NT < N ? ilam[tid + k * NT] : 0.0;
+    if (tid < CM * CM) {
+        const int m = tid / CM, t = tid % CM;
+        const bool in = m < c && t < c;
+        QF[tid] = in ? Qf[t * c + m] : 0.0;
+        QI[tid] = in ? Qf[m * c + t] : 0.0;
+    }
+    for (int x = tid; x < N; x += NT) A[(x / c) * pitch + x % c] = (double)b[x];
+    lds_barrier();
+    // axis 0: lines along k at (i, j) = (u, v); 1: along j at (i, k); 2: along i at (j, k).
+    auto pass = [&](const double* src, double* dst, int axis, bool inv) {
+        const int line = tid >> 2, t0 = (tid & 3) * (CM / 4);
+        if (line < c2) {
+            const int u = line / c, v = line % c;
+            const int base = axis == 0 ? (u * c + v) * pitch : (axis == 1 ? u * c * pitch + v : u * pitch + v);
+            const int stride = axis == 0 ? 1 : (axis == 1 ? pitch : c * pitch);
+            const double* Qm = (inv ? QI : QF) + t0;
+            double s[CM / 4];
+#pragma unroll
+            for (int j = 0; j < CM / 4; j++) s[j] = 0.0;
+#pragma unroll
+            for (int m = 0; m < (CT > 0 ? CT : CM); m++) {
+                if (CT == 0 && m >= c) break;
+                const double am = src[base + m * stride];
+#pragma unroll
+                for (int j = 0; j < CM / 4; j++) s[j] = fma(Qm[m * CM + j], am, s[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < CM / 4; j++)
+                if (t0 + j < c) dst[base + (t0 + j) * stride] = s[j];
+        }
+        lds_barrier();
+    };
+    pass(A, B, 0, false);
+    pass(B, A, 1, false);
+    pass(A, B, 2, false);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int x = tid + k * NT;
+        if (x < N) B[(x / c) * pitch + x % c] *= il[k];
+    }
+    lds_barrier();
+    pass(B, A, 2, true);
+    pass(A, B, 1, true);
+    pass(B, A, 0, true);
+    for (int x = tid; x < N; x += NT) e[x] = (float)A[(x / c) * pitch + x % c];
 }
 
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st) {
-    MAZE_LAUNCH(k_mg_coarse, dim3(1), dim3(kBlock), 0, s, c, b, e, Qf, ilam, st);
+    if (c > kMgCoarseLds) {
+        std::fprintf(stderr, "maze: coarsest level %d above %d for the one-work-group solve\n", c,
+                     kMgCoarseLds);
+        std::exit(1);
+    }
+    const dim3 grid(1), block(4 * kMgCoarseLds * kMgCoarseLds);
+    if (c == kMgCoarseLds) MAZE_LAUNCH(k_mg_coarse<kMgCoarseLds>, grid, block, 0, s, c, b, e, Qf, ilam, st);
+    else MAZE_LAUNCH(k_mg_coarse<0>, grid, block, 0, s, c, b, e, Qf, ilam, st);
 }
 
 // ---- exact solve on a larger coarsest grid (kMgCoarseLds < c <= kMgCoarseMax) --

@@ -656,20 +656,32 @@ void Sim::free_mg() {
     mg_ok = -1;
 }
 
-// Levels n, n/2, ... while the size is even and above 12; the last level is
-// solved exactly (in one work-group up to 12 points per side, by axis passes
-// over global work arrays up to 128).  Unavailable when halving stops at an
-// odd size above 128.
-std::vector<int> Sim::mg_sizes(int n) {
+// Levels n, n/2, ...: halved once, then while the size is even and above
+// `stop`; the last level is solved exactly (in one work-group up to 16 points
+// per side, by axis passes over global work arrays up to 128).  Unavailable
+// when halving stops at an odd size above 128.  stop = 16 (default): the
+// one-work-group exact solve replaces the two smallest levels; stop = 12 is
+// the earlier hierarchy, whose levels of at most 16^3 run in the one-
+// work-group tail (MAZE_MG_COARSE=12).
+int Sim::default_mg_stop() {
+    static const int stop = [] {
+        const char* e = std::getenv("MAZE_MG_COARSE");
+        return e && std::atoi(e) == 12 ? 12 : kMgCoarseLds;
+    }();
+    return stop;
+}
+
+std::vector<int> Sim::mg_sizes(int n, int stop) {
     std::vector<int> sizes{n};
-    while (sizes.back() % 2 == 0 && sizes.back() > kMgCoarseLds) sizes.push_back(sizes.back() / 2);
+    while (sizes.back() % 2 == 0 && sizes.back() >= 8 && (sizes.size() == 1 || sizes.back() > stop))
+        sizes.push_back(sizes.back() / 2);
     if (sizes.back() > kMgCoarseMax || sizes.size() < 2) sizes.clear();
     return sizes;
 }
 
 bool Sim::setup_mg() {
     if (mg_ok >= 0) return mg_ok == 1;
-    const std::vector<int> sizes = mg_sizes(n);
+    const std::vector<int> sizes = mg_sizes(n, mg_stop);
     if (sizes.empty()) {
         mg_ok = 0;
         return false;
@@ -732,7 +744,7 @@ bool Sim::setup_mg() {
     // finest whose remaining levels fit its LDS (<= 16^3); without such a
     // level the exact coarsest solve runs as its own kernel.
     mg_tail = (int)mg.size();
-    for (int l = 1; l < (int)mg.size(); l++) {
+    for (int l = 1; l < (int)mg.size() && mg_stop <= 12; l++) {
         int ns[8];
         const int nl = (int)mg.size() - l;
         if (nl > 8) continue;

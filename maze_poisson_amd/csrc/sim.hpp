@@ -267,7 +267,9 @@ struct Sim {
     int inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit, int first_chunk);
     bool setup_mg();
     // Level sizes of the FP32 multigrid for an n^3 grid (empty: unavailable).
-    static std::vector<int> mg_sizes(int n);
+    static std::vector<int> mg_sizes(int n, int stop = default_mg_stop());
+    static int default_mg_stop();
+    int mg_stop = default_mg_stop();  // see mg_sizes
     void free_mg();
     void vcycle(int level, int iter);
     int maze_fp64();
