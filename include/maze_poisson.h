@@ -209,9 +209,9 @@ void solver_set_y_extrapolation(int on);
  * get_energy_elec keeps the reference's 1/2 sum_g phi_g. */
 double get_energy_elec_qphi(void);
 /* Mode 2 = FP32 multigrid-preconditioned CG inside FP64 iterative
- * refinement (V(2,2) with two Jacobi-type sweeps per side, exact FP64
+ * refinement (V(2,2) with two Jacobi-type sweeps per side, exact
  * coarsest solve); falls back to mode 1 when the grid has no multigrid
- * hierarchy (halving reaches an odd size above 12) or the PB operator is
+ * hierarchy (halving reaches an odd size above 128) or the PB operator is
  * enabled.  One damping for both sweeps (plain damped Jacobi): */
 void solver_set_mg_omega(double omega);
 /* Different weights for the two smoothing sweeps of each level (pre-smoothing
