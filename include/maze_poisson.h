@@ -279,6 +279,10 @@ int maze_test_mg_up2(const float *b, const float *d2, const float *e, float *d, 
                      double c2, int tile);
 int maze_test_mg_up(const float *b, const float *d2, const float *e, float *d, int n, double c,
                     int tile);
+/* The exact solve of a coarsest multigrid level of c^3 points (that of a
+ * (2c)^3 grid: one work-group up to 16^3, axis passes above); returns c, or
+ * -1 if (2c)^3 has no two-level hierarchy. */
+int maze_test_mg_coarse(const float *b, float *e, int c);
 /* The one-work-group coarse tail of the V-cycle set up for an n^3 grid, on the
  * first level it covers (its size is returned; -1 if the grid has no tail). */
 int maze_test_mg_tail(const float *b, float *d, int n);

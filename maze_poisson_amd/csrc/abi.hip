@@ -479,6 +479,22 @@ int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, i
 // The one-work-group tail of the V-cycle as set up for an n^3 fine grid: from
 // the first level it takes over (returned, or -1 if there is none), on FP32
 // host buffers of that level's size.
+int maze_test_mg_coarse(const float* b, float* e, int c) {
+    Sim* s = scratch_sim(2 * c);
+    s->precision = maze::kMixedMG;
+    if (!s->setup_mg() || s->mg.size() != 2 || s->mg_c != c) {
+        delete s;
+        return -1;
+    }
+    const long g = (long)c * c * c;
+    MAZE_HIP(hipMemcpy(s->mg[1].b, b, g * sizeof(float), hipMemcpyHostToDevice));
+    s->vcycle(1, 0);  // the coarsest level: its exact solve only
+    s->sync();
+    MAZE_HIP(hipMemcpy(e, s->mg[1].d, g * sizeof(float), hipMemcpyDeviceToHost));
+    delete s;
+    return c;
+}
+
 int maze_test_mg_tail(const float* b, float* d, int n) {
     Sim* s = scratch_sim(n);
     s->precision = maze::kMixedMG;

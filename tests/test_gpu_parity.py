@@ -355,6 +355,19 @@ def test_chebyshev_smoother_damps_upper_spectrum():
     assert np.abs(p[mu >= 1 / 3]).max() <= 1 / t2 + 1e-9
 
 
+@pytest.mark.parametrize("c", [8, 13, 16, 25, 33])
+def test_mg_coarse_exact(hip_raw, c):
+    """The exact solve of the coarsest level (one work-group in FP32 up to 16^3, FP64
+    axis passes above) against the FFT solve of the same periodic system."""
+    b = np.random.default_rng(c).standard_normal((c, c, c))
+    b = (b - b.mean()).astype(np.float32)
+    e = np.zeros_like(b)
+    assert hip_raw.maze_test_mg_coarse(b, e, c) == c
+    x = _exact_periodic(b.astype(np.float64))
+    tol = 2e-5 if c <= 16 else 1e-6
+    assert np.abs(e - x).max() <= tol * np.abs(x).max()
+
+
 @pytest.mark.parametrize("n", [32, 128])
 def test_mg_tail(hip_raw, n):
     """The one-work-group tail (16^3 -> exact 8^3 -> 16^3) vs the same cycle in FP64
