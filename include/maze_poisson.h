@@ -279,6 +279,11 @@ int maze_test_mg_up2(const float *b, const float *d2, const float *e, float *d, 
                      double c2, int tile);
 int maze_test_mg_up(const float *b, const float *d2, const float *e, float *d, int n, double c,
                     int tile);
+/* The FP32 multigrid-preconditioned CG for the linear PB operator with the
+ * given face maps and k2 (n^3 each), from x = 0 to the relative residual
+ * rtol; returns the iterations, or -1 if the grid has no PB hierarchy. */
+int maze_test_pb_mg_pcg(const double *b, double *x, double rtol, int n, const double *eps_x,
+                        const double *eps_y, const double *eps_z, const double *k2);
 /* The exact solve of a coarsest multigrid level of c^3 points (that of a
  * (2c)^3 grid: one work-group up to 16^3, axis passes above); returns c, or
  * -1 if (2c)^3 has no two-level hierarchy. */

@@ -479,6 +479,38 @@ int maze_test_mg_up(const float* b, const float* d2, const float* e, float* d, i
 // The one-work-group tail of the V-cycle as set up for an n^3 fine grid: from
 // the first level it takes over (returned, or -1 if there is none), on FP32
 // host buffers of that level's size.
+int maze_test_pb_mg_pcg(const double* b, double* x, double rtol, int n, const double* ex,
+                        const double* ey, const double* ez, const double* k2) {
+    Sim* s = scratch_sim(n);
+    s->init_pb(0.0, 0.0);
+    s->set_eps_k2(ex, ey, ez, k2);
+    s->precision = maze::kMixedMG;
+    if (!s->setup_pbmg()) {
+        delete s;
+        return -1;
+    }
+    const long G = s->G;
+    std::vector<float> bf(G), xf(G);
+    for (long i = 0; i < G; i++) bf[i] = (float)b[i];
+    float *db = nullptr, *dx = nullptr, *dr = nullptr, *pa = nullptr, *pb = nullptr;
+    MAZE_HIP(hipMalloc(&db, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dx, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&dr, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&pa, G * sizeof(float)));
+    MAZE_HIP(hipMalloc(&pb, G * sizeof(float)));
+    MAZE_HIP(hipMemcpy(db, bf.data(), G * sizeof(float), hipMemcpyHostToDevice));
+    s->r32 = dr;
+    s->q32a = pa;
+    s->q32b = pb;
+    const int it = s->inner_solve(true, db, dx, rtol, n * n, 1);
+    MAZE_HIP(hipMemcpy(xf.data(), dx, G * sizeof(float), hipMemcpyDeviceToHost));
+    for (long i = 0; i < G; i++) x[i] = (double)xf[i];
+    s->r32 = s->q32a = s->q32b = nullptr;
+    for (float* a : {db, dx, dr, pa, pb}) MAZE_HIP(hipFree(a));
+    delete s;
+    return it;
+}
+
 int maze_test_mg_coarse(const float* b, float* e, int c) {
     Sim* s = scratch_sim(2 * c);
     s->precision = maze::kMixedMG;

@@ -150,7 +150,20 @@ void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const doub
 void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,
                           const double* Qf, const double* ilam, CGState* st);
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
-                    float* p_new, CGState* st, Reduce red, int first);
+                    float* p_new, CGState* st, Reduce red, int first, const PBCoef& pb = PBCoef{});
+// FP32 multigrid for the linear PB operator (mg_kernels.hip).
+void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* g,
+                      const float* b, const float* din, float* dout, double c, CGState* st,
+                      int dot, Reduce red);
+void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* b,
+                     const float* din, float* r, CGState* st);
+void launch_pb_first(hipStream_t s, int n, const float* b, const float* g, double c, float* d,
+                     CGState* st);
+void launch_pb_idg(hipStream_t s, int n, const PBCoef& pb, float* g);
+void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef& f, double* cx, double* cy, double* cz,
+                       double* ck2);
+void launch_pb_coarse(hipStream_t s, int c, const PBCoef& pb, const float* g, const float* b,
+                      float* x, int iters, double lo, double hi, CGState* st);
 // Fused level kernels of the V-cycle on an n^3 level (n even):
 // down: d2 = two pre-smoothing sweeps from zero, b_coarse = restricted residual;
 // up:   d = two post-smoothing sweeps of d2 + prolong(e); dot: also <b, d> and

@@ -1227,13 +1227,14 @@ struct BodyPcgDir {
     }
 };
 
-template <bool First>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, const float* p_old, const float* v,
-                                                    float* p_new, Reduce red, CGState* st) {
+template <bool First, bool PB>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, PBCoef pb, const float* p_old,
+                                                    const float* v, float* p_new, Reduce red,
+                                                    CGState* st) {
     if (st->done) return;
     const SrcPcgDir<First> src{p_old, v, First ? 0.0 : st->beta};
     BodyPcgDir bd{p_new};
-    march<false>(d, PBCoef{}, src, bd);
+    march<PB>(d, pb, src, bd);
     __shared__ double sh[kBlock / kWave];
     const double S = block_sum(bd.s, sh);
     if (publish_arrive(red.psum, S, nullptr, 0.0, red.counter, gridDim.x)) {
@@ -1246,13 +1247,244 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, const float* p_o
 }
 
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
-                    float* p_new, CGState* st, Reduce red, int first) {
-    if (first)
-        MAZE_LAUNCH(k_pcg_dir<true>, dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
-                           red, st);
+                    float* p_new, CGState* st, Reduce red, int first, const PBCoef& pb) {
+    DISPATCH_PB(pb, {
+        if (first)
+            MAZE_LAUNCH((k_pcg_dir<true, kPB>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, p_old,
+                        v, p_new, red, st);
+        else
+            MAZE_LAUNCH((k_pcg_dir<false, kPB>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, p_old,
+                        v, p_new, red, st);
+    });
+}
+
+// ---- FP32 multigrid for the linear Poisson-Boltzmann operator ----------------
+// A u = sum_f eps_f (u_nb - u) - k2 u with the face coefficients of each level
+// (FP64; level 0: the maps) and an FP32 inverse diagonal g = 1 / (sum of the
+// six face coefficients + k2).  Damped Jacobi: d <- d - w g (b - A d), with the
+// Chebyshev weights of the constant-coefficient cycle (the Jacobi-scaled
+// spectrum again lies in (0, 2)).  The passes are unfused march passes; the
+// restriction (4 x full weighting) and the trilinear prolongation are the
+// constant-coefficient kernels.  Coarse coefficients (Sim::pbmg_coefs): the
+// two fine faces a coarse face spans combine in series (harmonic mean), the
+// transverse fine faces by full weighting; k2 is fully weighted and scaled by
+// (H/h)^2 = 4 like the residual.  The coarsest level (at most 16^3) runs a
+// fixed number of Chebyshev iterations in one work-group, so the cycle is a
+// fixed linear operator.
+
+struct BodyPbSmooth {
+    static constexpr bool kReduce = false, kMax = false, kSkipWhenDone = true;
+    const float* b;
+    const float* g;
+    float* dout;
+    double c;
+    double s = 0.0, m = 0.0;
+    struct Raw { float b, g; };
+    __device__ Raw load(int i) const { return Raw{b[i], g[i]}; }
+    __device__ void apply(int i, const Raw& r, double v, double lap) {
+        dout[i] = (float)(v - c * (double)r.g * ((double)r.b - lap));
+    }
+    __device__ static void finalize(CGState*, double, double) {}
+};
+
+// Last post-smoothing sweep of the finest level with <r, z> and the PCG beta.
+struct BodyPbSmoothDot {
+    static constexpr bool kReduce = true, kMax = false, kSkipWhenDone = true;
+    const float* b;
+    const float* g;
+    float* dout;
+    double c;
+    double s = 0.0, m = 0.0;
+    struct Raw { float b, g; };
+    __device__ Raw load(int i) const { return Raw{b[i], g[i]}; }
+    __device__ void apply(int i, const Raw& r, double v, double lap) {
+        const float z = (float)(v - c * (double)r.g * ((double)r.b - lap));
+        dout[i] = z;
+        s += (double)r.b * (double)z;
+    }
+    __device__ static void finalize(CGState* st, double S, double) {
+        st->beta = st->first ? 0.0 : S / st->rdv;
+        st->rdv = S;
+    }
+};
+
+void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* g,
+                      const float* b, const float* din, float* dout, double c, CGState* st,
+                      int dot, Reduce red) {
+    if (dot)
+        run_pass<true>(s, d, pb, SrcArr<float>{din}, BodyPbSmoothDot{b, g, dout, c}, red, st);
     else
-        MAZE_LAUNCH(k_pcg_dir<false>, dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v,
-                           p_new, red, st);
+        run_pass<true>(s, d, pb, SrcArr<float>{din}, BodyPbSmooth{b, g, dout, c}, Reduce{}, st);
+}
+
+void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* b,
+                     const float* din, float* r, CGState* st) {
+    run_pass<true>(s, d, pb, SrcArr<float>{din}, BodyMgResid{b, r}, Reduce{}, st);
+}
+
+// First sweep from zero: d = -c g b.
+__global__ void k_pb_first(long N, const float* __restrict__ b, const float* __restrict__ g,
+                           double c, float* __restrict__ d, const CGState* st) {
+    if (st->done) return;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x)
+        d[t] = (float)(-c * (double)g[t] * (double)b[t]);
+}
+
+void launch_pb_first(hipStream_t s, int n, const float* b, const float* g, double c, float* d,
+                     CGState* st) {
+    const long N = (long)n * n * n;
+    MAZE_LAUNCH(k_pb_first, dim3(pw_blocks_mg(N)), dim3(kBlock), 0, s, N, b, g, c, d, st);
+}
+
+// g = 1 / (eps_x + eps_x(i-1) + eps_y + eps_y(j-1) + eps_z + eps_z(k-1) + k2).
+__global__ void k_pb_idg(int n, PBCoef pb, float* __restrict__ g) {
+    const long N = (long)n * n * n, n2 = (long)n * n;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % n), j = (int)((t / n) % n), i = (int)(t / n2);
+        const long im = (i == 0 ? n - 1 : i - 1) * n2 + (long)j * n + k;
+        const long jm = (long)i * n2 + (j == 0 ? n - 1 : j - 1) * (long)n + k;
+        const long km = (long)i * n2 + (long)j * n + (k == 0 ? n - 1 : k - 1);
+        const double dg = pb.ex[t] + pb.ex[im] + pb.ey[t] + pb.ey[jm] + pb.ez[t] + pb.ez[km] +
+                          pb.k2[t];
+        g[t] = (float)(1.0 / dg);
+    }
+}
+
+void launch_pb_idg(hipStream_t s, int n, const PBCoef& pb, float* g) {
+    const long N = (long)n * n * n;
+    MAZE_LAUNCH(k_pb_idg, dim3(pw_blocks_mg(N)), dim3(kBlock), 0, s, n, pb, g);
+}
+
+// Coarse coefficients of an (nf/2)^3 level from the nf^3 one.
+__global__ void k_pb_coarsen(int nf, PBCoef f, double* __restrict__ cx, double* __restrict__ cy,
+                             double* __restrict__ cz, double* __restrict__ ck2) {
+    const int M = nf / 2;
+    const long total = (long)M * M * M, N = nf, N2 = N * N;
+    auto w = [nf](int i) { return i < 0 ? i + nf : (i >= nf ? i - nf : i); };
+    auto harm = [](double a, double b) { return 2.0 * a * b / (a + b); };
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int K = (int)(t % M), J = (int)((t / M) % M), I = (int)(t / ((long)M * M));
+        const int i = 2 * I, j = 2 * J, k = 2 * K;
+        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
+        for (int a = -1; a <= 1; a++) {
+            const double wa = a == 0 ? 2.0 : 1.0;
+            for (int b = -1; b <= 1; b++) {
+                const double wb = b == 0 ? 2.0 : 1.0;
+                // x faces at (i, j + a, k + b) and (i + 1, j + a, k + b)
+                const long xa = (long)i * N2 + w(j + a) * N + w(k + b);
+                sx += wa * wb * harm(f.ex[xa], f.ex[xa + N2 * (i + 1 < nf ? 1 : 1 - nf)]);
+                // y faces at (i + a, j, k + b) and (i + a, j + 1, k + b)
+                const long ya = (long)w(i + a) * N2 + (long)j * N + w(k + b);
+                sy += wa * wb * harm(f.ey[ya], f.ey[ya + N * (j + 1 < nf ? 1 : 1 - nf)]);
+                // z faces at (i + a, j + b, k) and (i + a, j + b, k + 1)
+                const long za = (long)w(i + a) * N2 + w(j + b) * N + k;
+                sz += wa * wb * harm(f.ez[za], f.ez[za + (k + 1 < nf ? 1 : 1 - nf)]);
+                for (int c = -1; c <= 1; c++) {
+                    const double wc = c == 0 ? 2.0 : 1.0;
+                    s2 += wa * wb * wc * f.k2[(long)w(i + a) * N2 + w(j + b) * N + w(k + c)];
+                }
+            }
+        }
+        cx[t] = sx / 16.0;
+        cy[t] = sy / 16.0;
+        cz[t] = sz / 16.0;
+        ck2[t] = 4.0 * s2 / 64.0;
+    }
+}
+
+void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef& f, double* cx, double* cy, double* cz,
+                       double* ck2) {
+    const long M = nf / 2;
+    MAZE_LAUNCH(k_pb_coarsen, dim3(pw_blocks_mg(M * M * M)), dim3(kBlock), 0, s, nf, f, cx, cy,
+                cz, ck2);
+}
+
+// Coarsest level (c^3 <= kMgCoarseLds^3): `iters` Chebyshev iterations for
+// A x = b from x = 0, Jacobi-preconditioned, for the spectrum [lo, hi] of
+// g (-A) (Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1), in
+// one work-group with the level in LDS; the update direction alternates
+// between two arrays so that one barrier per iteration suffices.
+constexpr int kPbCoarseThreads = 1024;
+
+__global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef pb,
+                                                             const float* __restrict__ g,
+                                                             const float* __restrict__ b,
+                                                             float* __restrict__ x_out, int iters,
+                                                             double lo, double hi,
+                                                             const CGState* st) {
+    if (st->done) return;
+    constexpr int CMAX = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds;
+    constexpr int PER = CMAX / kPbCoarseThreads;
+    __shared__ float dv[2][CMAX];
+    __shared__ float ex[CMAX], ey[CMAX], ez[CMAX], k2[CMAX];
+    const int N = c * c * c, c2 = c * c, tid = threadIdx.x;
+    float x[PER], r[PER], gg[PER];
+    int nb[PER][6];
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int t = tid + q * kPbCoarseThreads;
+        if (t < N) {
+            ex[t] = (float)pb.ex[t];
+            ey[t] = (float)pb.ey[t];
+            ez[t] = (float)pb.ez[t];
+            k2[t] = (float)pb.k2[t];
+            const int k = t % c, j = (t / c) % c, i = t / c2;
+            nb[q][0] = (i == c - 1 ? 0 : i + 1) * c2 + j * c + k;
+            nb[q][1] = (i == 0 ? c - 1 : i - 1) * c2 + j * c + k;
+            nb[q][2] = i * c2 + (j == c - 1 ? 0 : j + 1) * c + k;
+            nb[q][3] = i * c2 + (j == 0 ? c - 1 : j - 1) * c + k;
+            nb[q][4] = i * c2 + j * c + (k == c - 1 ? 0 : k + 1);
+            nb[q][5] = i * c2 + j * c + (k == 0 ? c - 1 : k - 1);
+            x[q] = 0.0f;
+            r[q] = -b[t];  // residual of (-A) x = -b at x = 0
+            gg[q] = g[t];
+            dv[0][t] = (float)(gg[q] * r[q] / theta);
+        }
+    }
+    lds_barrier();
+    int cur = 0;
+    for (int it = 0; it < iters; it++) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho);
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = tid + q * kPbCoarseThreads;
+            if (t < N) {
+                const float* d = dv[cur];
+                const float dc = d[t];
+                // (-A) d = sum_f eps_f (d - d_nb) + k2 d
+                const float Ad = ex[t] * (dc - d[nb[q][0]]) + ex[nb[q][1]] * (dc - d[nb[q][1]]) +
+                                 ey[t] * (dc - d[nb[q][2]]) + ey[nb[q][3]] * (dc - d[nb[q][3]]) +
+                                 ez[t] * (dc - d[nb[q][4]]) + ez[nb[q][5]] * (dc - d[nb[q][5]]) +
+                                 k2[t] * dc;
+                x[q] += dc;
+                r[q] -= Ad;
+                dv[cur ^ 1][t] = (float)(rho1 * rho * dc + 2.0 * rho1 / delta * gg[q] * r[q]);
+            }
+        }
+        rho = rho1;
+        cur ^= 1;
+        lds_barrier();
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int t = tid + q * kPbCoarseThreads;
+        if (t < N) x_out[t] = x[q];
+    }
+}
+
+void launch_pb_coarse(hipStream_t s, int c, const PBCoef& pb, const float* g, const float* b,
+                      float* x, int iters, double lo, double hi, CGState* st) {
+    if (c > kMgCoarseLds) {
+        std::fprintf(stderr, "maze: PB multigrid coarsest level %d above %d\n", c, kMgCoarseLds);
+        std::exit(1);
+    }
+    MAZE_LAUNCH(k_pb_coarse, dim3(1), dim3(kPbCoarseThreads), 0, s, c, pb, g, b, x, iters, lo, hi,
+                st);
 }
 
 }  // namespace maze

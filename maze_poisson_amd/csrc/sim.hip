@@ -168,6 +168,7 @@ void Sim::free_grid() {
     for (float** a : {&d32, &r32, &q32a, &q32b}) dfree(*a);
     for (double** a : {&cex, &cey, &cez, &ck2}) dfree(*a);
     free_mg();
+    free_pbmg();
     sched.clear();
     dfree(y_hist);
     hist_ok = 0;
@@ -241,6 +242,7 @@ void Sim::init_grid(int n_, double L_, double h_, double tol_, double eps, doubl
 // library/src/grid_general.c:85-100 (maps default to vacuum until set).
 void Sim::init_pb(double w_, double kbar2_, int nonpolar_) {
     pb = 1; w = w_; kbar2 = kbar2_; nonpolar = nonpolar_;
+    pbmg_dirty = 1;
     ex = dalloc<double>(G); ey = dalloc<double>(G); ez = dalloc<double>(G); k2 = dalloc<double>(G);
     launch_fill(stream, G, 1.0, ex);
     launch_fill(stream, G, 1.0, ey);
@@ -251,6 +253,7 @@ void Sim::init_pb(double w_, double kbar2_, int nonpolar_) {
 }
 
 void Sim::set_eps_k2(const double* ex_, const double* ey_, const double* ez_, const double* k2_) {
+    pbmg_dirty = 1;
     if (!pb) fatal("maze: solver_set_eps_k2 needs solver_initialize_grid_pois_boltz first");
     MAZE_HIP(hipMemcpy(ex, ex_, G * sizeof(double), hipMemcpyHostToDevice));
     MAZE_HIP(hipMemcpy(ey, ey_, G * sizeof(double), hipMemcpyHostToDevice));
@@ -368,6 +371,7 @@ void Sim::init_pb_particles(double gamma_np, double beta_np, const double* radii
 // library/src/grid_general.c:138-301 (the maps of the current positions);
 // without radii the maps set by set_eps_k2 are kept.
 void Sim::update_eps_k2() {
+    if (pbd.active) pbmg_dirty = 1;
     if (!pb || !pbd.active) return;
     launch_pb_bin(stream, pbd, p.pos);
     timed(kTPbMaps, 0, [&] { launch_pb_maps(stream, pbd, p.pos, ex, ey, ez, k2); });
@@ -573,7 +577,7 @@ int Sim::cg_count_solve(const GridDims& d, const PBCoef& cpb, const double* b, d
 // One iteration of the FP32-storage inner solve (Jacobi-CG, or CG with the
 // multigrid preconditioner of Sim::vcycle); `it` is its 0-based index.
 void Sim::inner_iter(bool use_mg, float* x, int it) {
-    if (use_graphs && use_mg && !prof && x == d32) {
+    if (use_graphs && use_mg && !pb && !prof && x == d32) {
         const int v = it == 0 ? 0 : ((it & 1) ? 1 : 2);
         if (!inner_graph[v]) {
             hipGraph_t g = nullptr;
@@ -599,7 +603,15 @@ void Sim::drop_graphs() {
 void Sim::inner_body(bool use_mg, float* x, int it) {
     float* pin = (it & 1) ? q32b : q32a;
     float* pout = (it & 1) ? q32a : q32b;
-    if (use_mg) {
+    if (use_mg && pb) {  // PB multigrid-PCG
+        vcycle_pb(0);
+        timed(kTCgDirF32, it, [&] {
+            launch_pcg_dir(stream, dims, pin, pbmg[0].d, pout, st, red, it == 0, pbc());
+        });
+        timed(kTCgUpdateF32, it, [&] {
+            launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red, it == 0);
+        });
+    } else if (use_mg) {
         vcycle(0, it);
         timed(kTCgDirF32, it,
               [&] { launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red, it == 0); });
@@ -620,8 +632,13 @@ void Sim::inner_body(bool use_mg, float* x, int it) {
 // library/src/plcg.c:104-166.  gated: skipped once the outer loop is done.
 void Sim::inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated) {
     launch_cg_reset(stream, st, 0.0, rtol, limit, 1, use_mg ? 1 : 0, gated);
-    launch_cg_init<float>(stream, dims, use_mg ? PBCoef{} : pbc(), nullptr, b, r32, st, red, x);
-    if (use_mg) mg[0].b = r32;
+    launch_cg_init<float>(stream, dims, use_mg && !pb ? PBCoef{} : pbc(), nullptr, b, r32, st, red,
+                          x);
+    if (use_mg && !pb) mg[0].b = r32;
+    if (use_mg && pb) {
+        pbmg_coefs();
+        pbmg[0].b = r32;
+    }
 }
 
 // Complete inner solve, polling the device after every iteration past the
@@ -758,6 +775,79 @@ bool Sim::setup_mg() {
     return true;
 }
 
+// The PB multigrid needs a hierarchy whose coarsest level fits one work-group.
+bool Sim::setup_pbmg() {
+    if (pbmg_ok >= 0) return pbmg_ok == 1;
+    const std::vector<int> sizes = mg_sizes(n, mg_stop);
+    if (!pb || sizes.empty() || sizes.back() > kMgCoarseLds) {
+        pbmg_ok = 0;
+        return false;
+    }
+    for (size_t l = 0; l < sizes.size(); l++) {
+        PbLevel L;
+        L.n = sizes[l];
+        L.dims = make_dims(L.n, choose_slab(L.n));
+        const long g = (long)L.n * L.n * L.n;
+        if (l > 0) {
+            L.ex = dalloc<double>(g); L.ey = dalloc<double>(g);
+            L.ez = dalloc<double>(g); L.k2 = dalloc<double>(g);
+            L.b = dalloc<float>(g);
+        }
+        L.g = dalloc<float>(g); L.d = dalloc<float>(g); L.t = dalloc<float>(g);
+        pbmg.push_back(L);
+    }
+    pbmg_dirty = 1;
+    pbmg_ok = 1;
+    return true;
+}
+
+void Sim::free_pbmg() {
+    for (PbLevel& L : pbmg) {
+        for (double** a : {&L.ex, &L.ey, &L.ez, &L.k2}) dfree(*a);
+        for (float** a : {&L.g, &L.d, &L.t}) dfree(*a);
+        if (&L != &pbmg[0]) dfree(L.b);
+    }
+    pbmg.clear();
+    pbmg_ok = -1;
+    pbmg_dirty = 1;
+}
+
+// Coefficients of every level from the current maps (after each map change).
+void Sim::pbmg_coefs() {
+    if (!pbmg_dirty) return;
+    for (size_t l = 0; l < pbmg.size(); l++) {
+        PbLevel& L = pbmg[l];
+        if (l == 0) {
+            L.cf = pbc();
+        } else {
+            launch_pb_coarsen(stream, pbmg[l - 1].n, pbmg[l - 1].cf, L.ex, L.ey, L.ez, L.k2);
+            L.cf = PBCoef{L.ex, L.ey, L.ez, L.k2};
+        }
+        launch_pb_idg(stream, L.n, L.cf, L.g);
+    }
+    pbmg_dirty = 0;
+}
+
+// V(2,2) for the PB operator: two damped-Jacobi sweeps from zero, residual,
+// restriction, the coarser cycle, prolongation-correction, two sweeps (the
+// last one on the finest level with <r, z> and the PCG beta).
+void Sim::vcycle_pb(int l) {
+    PbLevel& L = pbmg[l];
+    if (l == (int)pbmg.size() - 1) {
+        launch_pb_coarse(stream, L.n, L.cf, L.g, L.b, L.d, pb_coarse_iters, pb_coarse_lo, 2.0, st);
+        return;
+    }
+    PbLevel& C = pbmg[l + 1];
+    launch_pb_first(stream, L.n, L.b, L.g, mg_omega, L.t, st);
+    launch_pb_smooth(stream, L.dims, L.cf, L.g, L.b, L.t, L.d, mg_omega2, st, 0, red);
+    launch_pb_resid(stream, L.dims, L.cf, L.b, L.d, L.t, st);
+    launch_mg_restrict(stream, L.n, L.t, C.b, st);
+    vcycle_pb(l + 1);
+    launch_mg_prolong_add(stream, C.n, C.d, L.d, st);
+    launch_pb_smooth(stream, L.dims, L.cf, L.g, L.b, L.d, L.t, mg_omega2, st, 0, red);
+    launch_pb_smooth(stream, L.dims, L.cf, L.g, L.b, L.t, L.d, mg_omega, st, l == 0 ? 1 : 0, red);
+}
+
 // V(2,2): pre-smooth, residual, restrict, recurse, prolong-correct, post-smooth,
 // as one fused down and one fused up kernel per level above the tail.
 void Sim::vcycle(int l, int iter) {
@@ -838,8 +928,12 @@ void Sim::mixed_issue() {
     }
     if (y_extrap && !y_hist) y_hist = dalloc<double>(G);
     MixedRun& m = mx;
-    m.use_mg = precision == kMixedMG && !pb && setup_mg();
-    if (m.use_mg) mg[0].b = r32;
+    m.use_mg = precision == kMixedMG && (pb ? setup_pbmg() : setup_mg());
+    if (m.use_mg && !pb) mg[0].b = r32;
+    if (m.use_mg && pb) {
+        pbmg_coefs();
+        pbmg[0].b = r32;
+    }
     const double coef = pb ? (4 * kPi) / h : (4 * kPi) / (h * eps_s);
     timed(kTPred, 0,
           [&] { launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma); });

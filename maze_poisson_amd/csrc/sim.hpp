@@ -105,6 +105,21 @@ struct Sim {
         return 1.0 / (k == 0 ? mid + half : mid - half);
     }
     int mg_ok = -1;             // -1 not set up, 0 unavailable for this n, 1 ready
+    // FP32 multigrid for the linear PB operator (precision kMixedMG with PB):
+    // per level the face coefficients (level 0: the maps; coarser: owned
+    // FP64 arrays), the inverse diagonal and the cycle's vectors.
+    struct PbLevel {
+        int n = 0;
+        GridDims dims{};
+        PBCoef cf{};
+        double *ex = nullptr, *ey = nullptr, *ez = nullptr, *k2 = nullptr;
+        float *g = nullptr, *b = nullptr, *d = nullptr, *t = nullptr;
+    };
+    std::vector<PbLevel> pbmg;
+    int pbmg_ok = -1;           // -1 not set up, 0 unavailable, 1 ready
+    int pbmg_dirty = 1;         // the maps changed since the coefficients were built
+    int pb_coarse_iters = 24;   // Chebyshev iterations on the coarsest level
+    double pb_coarse_lo = 0.02; // their lower spectrum bound (upper: 2)
     // Replay each multigrid-PCG inner iteration as a captured HIP graph (three
     // variants: the first iteration, odd and even ones) instead of its seven
     // or more separate launches: the kernels and their arguments are fixed
@@ -266,6 +281,10 @@ struct Sim {
     void inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated);
     int inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit, int first_chunk);
     bool setup_mg();
+    bool setup_pbmg();
+    void free_pbmg();
+    void pbmg_coefs();
+    void vcycle_pb(int level);
     // Level sizes of the FP32 multigrid for an n^3 grid (empty: unavailable).
     static std::vector<int> mg_sizes(int n, int stop = default_mg_stop());
     static int default_mg_stop();

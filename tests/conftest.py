@@ -119,6 +119,7 @@ def hip_raw(hip):
         "maze_test_mg_up2": (ci, [F, F, F, F, ci, cd, cd, ci]),
         "maze_test_mg_tail": (ci, [F, F, ci]),
         "maze_test_mg_coarse": (ci, [F, F, ci]),
+        "maze_test_pb_mg_pcg": (ci, [D, D, cd, ci, D, D, D, D]),
         "solver_set_mg_omega": (None, [cd]),
         "solver_mg_levels": (ci, []),
         "solver_set_precision": (ci, [ci]),

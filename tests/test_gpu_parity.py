@@ -103,6 +103,34 @@ def test_rbgs_pb_and_coefficient_restriction_bitexact(hip_raw, oracle_raw, n):
     assert np.array_equal(g[3], o)
 
 
+def _pb_apply(u, ex, ey, ez, k2):
+    """The PB operator: sum over the six faces of eps_f (u_nb - u) - k2 u (face maps at
+    the +x, +y, +z faces of each point)."""
+    out = -k2 * u
+    for axis, e in enumerate((ex, ey, ez)):
+        out += e * (np.roll(u, -1, axis) - u) + np.roll(e, 1, axis) * (np.roll(u, 1, axis) - u)
+    return out
+
+
+@pytest.mark.parametrize("n,smooth", [(32, True), (64, True), (32, False)])
+def test_pb_mg_pcg_solves(hip_raw, n, smooth):
+    """FP32 PB multigrid-preconditioned CG on a smooth dielectric in [2, 80] (sum of
+    Gaussians) or on face values drawn independently in [2, 80]: the solution satisfies
+    the PB equation (checked with a NumPy restatement of the operator) in few
+    iterations."""
+    if smooth:
+        eps = W.pb_eps_field(n, 40.0, seed=n)
+    else:
+        eps, _ = pb_coef(n, 320)
+    k2 = np.full((n, n, n), 0.02)
+    b = rand_grid(n, 321)
+    x = np.zeros_like(b)
+    it = hip_raw.maze_test_pb_mg_pcg(b, x, 1e-5, n, *eps, k2)
+    assert 0 < it <= (30 if smooth else 60), it
+    r = _pb_apply(x, *eps, k2) - b
+    assert np.abs(r).max() <= 1e-3 * np.abs(b).max()
+
+
 def test_vcycle_pb_matches_oracle(hip_raw, oracle_raw):
     n = 32
     eps, k2 = pb_coef(n, 310)
@@ -424,6 +452,23 @@ def test_mg_pcg_largest_coarsest(hip_raw):
     assert 0 < it <= 20, it
     r = _lap(x) - b
     assert np.abs(r).max() <= 1e-3 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("radii", [False, True])
+def test_mixed_mg_pb_forces(hip, radii):
+    """Linear PB with the FP32 PB multigrid-PCG inside FP64 refinement (precision 2):
+    force RMS error <= 1e-5 relative to the FP64 solve converged 1000x tighter, for
+    the synthetic dielectric and for the radius-based boundary."""
+    s = W.nacl_system(512, 64, seed=6)
+    kw = dict(eps_s=80.0, pb_radii=True, pb_w=1.0) if radii else dict(pb=True)
+    tight = driver.RunConfig(tol=W.poisson_tol(s) * 1e-3, **kw)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s), **kw)
+    _, _, f64 = run(hip, s, tight, 3, precision=0)
+    its, _, fmx = run(hip, s, cfg, 3, precision=2)
+    assert max(its) <= 60, its
+    d = fmx["fcs_tot"] - f64["fcs_tot"]
+    err = np.sqrt((d ** 2).mean()) / np.sqrt((f64["fcs_tot"] ** 2).mean())
+    assert err <= 1e-5, err
 
 
 @pytest.mark.parametrize("n", [66, 102])
