@@ -136,16 +136,48 @@ def driver_rate(lib, precision, steps, getters):
 PB_IONS, PB_GRID, PB_SEED = 8000, 160, 3
 
 
+def pb_rate(lib, cfg, s, precision, steps):
+    """MD steps/s of a PB configuration (two warm-up steps after the start-up)."""
+    driver.initialize(lib, s, cfg)
+    lib.solver_set_precision(precision)
+    driver.bootstrap(lib, cfg)
+    for _ in range(2):
+        lib.solver_md_loop_iter()
+    lib.solver_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.solver_md_loop_iter()
+    lib.solver_sync()
+    el = time.perf_counter() - t0
+    lib.solver_finalize()
+    return steps / el
+
+
+def pb_synthetic(lib, steps):
+    """OVRVO Langevin (1500 K, 1 / ps) with linearized PB on a synthetic dielectric in
+    [2, 80] and 0.1 M screening, 8000 ions / 160^3: the FP32 PB multigrid-PCG in FP64
+    refinement against FP64 CG."""
+    s = W.nacl_system(PB_IONS, PB_GRID, seed=PB_SEED)
+    cfg = driver.RunConfig(method="MAZE-LCG", tol=W.poisson_tol(s), integrator="OVRVO",
+                           thermostat=True, gamma=2.4189e-5, pb=True)
+    mg = pb_rate(lib, cfg, s, PATHS["mixed_mg"], steps)
+    f64 = pb_rate(lib, cfg, s, PATHS["fp64"], max(3, steps // 10))
+    return {"workload": f"{PB_IONS}-ion NaCl melt, {PB_GRID}^3 grid, OVRVO 1500 K, linear PB "
+                        "with a synthetic dielectric in [2, 80] (8 Gaussians) and 0.1 M screening",
+            "mixed_mg_steps_per_s": round(mg, 3), "fp64_steps_per_s": round(f64, 3),
+            "speedup_vs_fp64": round(mg / f64, 2)}
+
+
 def pb_radii_rate(lib, steps, prof_steps):
     """MD steps/s with the PB maps rebuilt from the radii every step and the boundary
-    forces in the total (field solve: FP32 Jacobi CG inside FP64 refinement), and the
-    average durations of the map and force kernels."""
+    forces in the total (field solve: FP32 PB multigrid-PCG inside FP64 refinement), and
+    the average durations of the map and force kernels."""
     L = lib.lib
     s = W.nacl_system(PB_IONS, PB_GRID, seed=PB_SEED)
     cfg = driver.RunConfig(method="MAZE-LCG", tol=W.poisson_tol(s), integrator="VERLET",
                            eps_s=80.0, pb_radii=True, pb_w=1.0)
     driver.initialize(lib, s, cfg)
-    lib.solver_set_precision(PATHS["mixed_cg"])
+    lib.solver_set_precision(PATHS["mixed_mg"])
     driver.bootstrap(lib, cfg)
     for _ in range(3):
         lib.solver_md_loop_iter()
@@ -164,7 +196,8 @@ def pb_radii_rate(lib, steps, prof_steps):
     prof = {k: prof_get(L, k) for k in ("pb_maps", "pb_forces")}
     lib.solver_finalize()
     out = {"workload": f"{PB_IONS}-ion NaCl melt, {PB_GRID}^3 grid, PB maps from radii every "
-                       "step + DB/IB/NP forces, MAZE-LCG FP32 CG in FP64 refinement",
+                       "step + DB/IB/NP forces, MAZE-LCG FP32 PB multigrid-PCG in FP64 "
+                       "refinement",
            "steps_per_s": round(steps / el, 3)}
     G = PB_GRID ** 3
     for k, (tot, cnt) in prof.items():
@@ -429,6 +462,7 @@ def main():
         line_extra = {
             "runner_steps_per_s": round(runner_rate(lib, 200), 2),
             "pb_radii": pb_radii_rate(lib, max(10, args.steps // 10), 5),
+            "pb_synthetic": pb_synthetic(lib, 30),
             "ensemble_64_replicas": ensemble_rate(lib, 64, 50, 64),
             "python_driver_steps_per_s": round(driver_rate(lib, PATHS["mixed_mg"], 100, False), 2),
             "python_driver_with_getters_steps_per_s": round(

@@ -211,8 +211,11 @@ double get_energy_elec_qphi(void);
 /* Mode 2 = FP32 multigrid-preconditioned CG inside FP64 iterative
  * refinement (V(2,2) with two Jacobi-type sweeps per side, exact
  * coarsest solve); falls back to mode 1 when the grid has no multigrid
- * hierarchy (halving reaches an odd size above 128) or the PB operator is
- * enabled.  One damping for both sweeps (plain damped Jacobi): */
+ * hierarchy (halving reaches an odd size above 128).  With the PB operator
+ * the cycle is the variable-coefficient one (coefficients coarsened from the
+ * maps after every map change, Chebyshev iterations on a coarsest level of at
+ * most 16^3; otherwise mode 1).  One damping for both sweeps (plain damped
+ * Jacobi): */
 void solver_set_mg_omega(double omega);
 /* Different weights for the two smoothing sweeps of each level (pre-smoothing
  * omega1 then omega2, post-smoothing omega2 then omega1).  Default: the
