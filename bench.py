@@ -308,7 +308,7 @@ def cpu_baseline(steps, cores):
                       f"after the 2 bootstrap steps and 1 untimed step"}
 
 
-def ensemble_rate(lib, replicas, steps, slots):
+def ensemble_rate(lib, replicas, steps, slots, graphs=True):
     """Uneven-size replicas advanced together by the replica engine: the rate of the
     whole run (set-up and start-up steps included) and the step rate from the
     difference of two runs of `steps` and 2 x `steps` steps per replica."""
@@ -316,7 +316,7 @@ def ensemble_rate(lib, replicas, steps, slots):
     ss = W.ensemble_systems(replicas)
     runs = {}
     for k in (steps, 2 * steps):
-        with ReplicaEngine(lib, max_slots=slots) as e:
+        with ReplicaEngine(lib, max_slots=slots, graphs=graphs) as e:
             ids = [e.submit(s, k) for s in ss]
             t = time.perf_counter()
             st = e.run()
@@ -404,9 +404,13 @@ def main():
     ap.add_argument("--cpu-child", action="store_true")
     ap.add_argument("--pb-radii-only", action="store_true",
                     help="only the radius-based PB step (8000 ions, 160^3); prints its JSON")
+    ap.add_argument("--pb-synthetic-only", action="store_true",
+                    help="only the synthetic-dielectric PB line (8000 ions, 160^3); prints its JSON")
     ap.add_argument("--ensemble", action="store_true",
                     help="only the 64-replica ensemble (replica engine); prints its JSON")
     ap.add_argument("--ensemble-steps", type=int, default=50)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="ensemble replicas launch every kernel separately (no HIP graphs)")
     ap.add_argument("--serving", action="store_true",
                     help="only the serving trace (replica engine); prints its JSON")
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -434,9 +438,12 @@ def main():
     if args.pb_radii_only:
         print(json.dumps(pb_radii_rate(lib, args.steps, args.prof_steps)))
         return
+    if args.pb_synthetic_only:
+        print(json.dumps(pb_synthetic(lib, args.steps)))
+        return
     if args.ensemble:
-        ens = ensemble_rate(lib, 64, args.ensemble_steps, 64)
-        ser = ensemble_rate(lib, 64, max(10, args.ensemble_steps // 2), 1)
+        ens = ensemble_rate(lib, 64, args.ensemble_steps, 64, not args.no_graphs)
+        ser = ensemble_rate(lib, 64, max(10, args.ensemble_steps // 2), 1, not args.no_graphs)
         ens["serial_replica_steps_per_s"] = ser["replica_steps_per_s"]
         print(json.dumps(ens))
         return
