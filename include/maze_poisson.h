@@ -193,6 +193,10 @@ void get_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
  * 2 = as 1 with the multigrid-preconditioned FP32 CG below.  Returns 0, or
  * -1 for an unknown mode. */
 int solver_set_precision(int mode);
+/* 1: the cold solves of solver_init_field use the mixed path of the current
+ * precision mode (refinement to the same FP64 stop test) instead of FP64 CG;
+ * 0 (default): FP64 CG as in the reference.  MAZE-LCG only. */
+void solver_set_mixed_init(int on);
 int solver_get_precision(void);
 /* The refinement passes of the last mixed-precision solve: returns their
  * number and writes each pass's inner iterations (negative when the pass
@@ -317,6 +321,9 @@ void maze_engine_set_threads(maze_engine *e, int threads);
  * V-cycle kernels, direction and update) once as HIP graphs and replays them;
  * 0: separate launches.  Results are identical. */
 void maze_engine_set_graphs(maze_engine *e, int on);
+/* 1 (default): the replicas' start-up cold solves use the mixed path of their
+ * precision (solver_set_mixed_init); 0: FP64 CG as in the reference. */
+void maze_engine_set_mixed_init(maze_engine *e, int on);
 /* Device bytes one replica needs (grids, multigrid levels, particles, pair
  * partials, scratch), and the precision it runs: 2 (FP32 multigrid-PCG in
  * FP64 refinement) needs an even grid whose halvings end at or below 128

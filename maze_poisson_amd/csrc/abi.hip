@@ -197,6 +197,9 @@ int solver_set_precision(int mode) {
 
 int solver_get_precision() { return S().precision; }
 
+// Cold solves with the mixed path (1) or the reference's FP64 CG (0, default).
+void solver_set_mixed_init(int on) { S().mixed_init = on != 0; }
+
 // Refinement passes of the last mixed-precision solve and each pass's inner
 // iterations (negative: stopped by the iteration budget), at most `max`.
 int solver_get_last_passes(int* iters, int max) {

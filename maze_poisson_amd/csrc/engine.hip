@@ -115,6 +115,7 @@ struct Engine {
     int max_slots = 1;
     int threads = 1;
     int graphs = 1;  // replicas replay their inner iterations as HIP graphs
+    int mixed_init = 1;  // start-up cold solves with the mixed path (Sim::mixed_init)
     std::vector<Request> reqs;
     // last run; the admission queue and the memory account are shared by the
     // worker threads under `mu`
@@ -178,6 +179,7 @@ struct Engine {
                          r.vel.data(), r.mass.data(), r.charges.data(), r.tf.data());
         s.init_integrator(r.dt, 0.0, 0.0, kVerlet, 0);
         s.use_graphs = graphs;
+        s.mixed_init = mixed_init;
         // The two start-up steps with cold solves (maze_poisson/solver.py:376-398).
         for (int k = 0; k < 2; k++) {
             s.update_charges();
@@ -344,6 +346,10 @@ void maze_engine_set_threads(maze_engine* h, int threads) {
 }
 
 void maze_engine_set_graphs(maze_engine* h, int on) { reinterpret_cast<Engine*>(h)->graphs = on != 0; }
+
+void maze_engine_set_mixed_init(maze_engine* h, int on) {
+    reinterpret_cast<Engine*>(h)->mixed_init = on != 0;
+}
 
 void maze_engine_destroy(maze_engine* h) { delete reinterpret_cast<Engine*>(h); }
 

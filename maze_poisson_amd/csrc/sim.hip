@@ -1130,8 +1130,44 @@ void Sim::init_field() {
     launch_copy(stream, G, phi, phip);
     launch_scale_copy(stream, G, c, q, sigma);
     launch_fill(stream, G, 0.0, phi);
+    if (mixed_init && precision != kFP64 && grid_type == kMazeLCG) {
+        init_field_mixed();
+        return;
+    }
     int pred = std::max(32, last_iters);
     cg_solve(dims, pbc(), sigma, phi, tol, n * n, r, p0, p1, &pred);
+}
+
+// The cold solve A phi = sigma from phi = 0 with the mixed path's refinement
+// loop (FP32 inner solves, FP64 residual and the same stop test
+// max|A phi - sigma| <= tol) instead of FP64 CG; phi* (tmp) is free here and
+// serves as the second iterate.  The MD steps' pass schedule is left alone.
+void Sim::init_field_mixed() {
+    if (!d32) {
+        d32 = dalloc<float>(G); r32 = dalloc<float>(G);
+        q32a = dalloc<float>(G); q32b = dalloc<float>(G); y_alt = dalloc<double>(G);
+    }
+    MixedRun& m = mx;
+    m.use_mg = precision == kMixedMG && (pb ? setup_pbmg() : setup_mg());
+    if (m.use_mg && !pb) mg[0].b = r32;
+    if (m.use_mg && pb) {
+        pbmg_coefs();
+        pbmg[0].b = r32;
+    }
+    m.inner_limit = inner_max > 0 ? inner_max : n * n;
+    m.ra = RefineArgs{tol, inner_rtol > 0 ? inner_rtol : (m.use_mg ? 1e-3 : 1e-4), m.inner_limit,
+                      m.use_mg ? 1 : 0};
+    m.extrap = false;
+    m.ya = phi;
+    m.yb = tmp;
+    mixed_refine(0);
+    read_state();
+    for (m.queued = 0; !st_host->outer_done && st_host->passes < kMaxOuter;) {
+        mixed_pass(m.queued++, 1, true);
+        read_state();
+    }
+    if (!st_host->outer_done) fatal("maze: the mixed-precision cold solve did not converge");
+    if (st_host->ysel) launch_copy(stream, G, m.yb, phi);
 }
 
 int Sim::update_field() {

@@ -248,6 +248,10 @@ struct Sim {
     int update_charges(bool defer = false);
     int check_charge();
     void init_field();
+    void init_field_mixed();
+    // Cold solves (init_field) with the mixed path when precision != kFP64
+    // (replica engine; solver_set_mixed_init), instead of the reference's FP64 CG.
+    int mixed_init = 0;
     int update_field();
     void forces_elec();
     double forces_noel(bool read_back = true);

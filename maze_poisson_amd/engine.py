@@ -32,7 +32,8 @@ class ReplicaEngine:
     replicas with a step count and an arrival time."""
 
     def __init__(self, lib: MazeLib, budget_bytes: int = 0, max_slots: int = 64,
-                 threads: int | None = None, graphs: bool | None = None):
+                 threads: int | None = None, graphs: bool | None = None,
+                 mixed_init: bool | None = None):
         if not lib.has("maze_engine_create"):
             raise RuntimeError(f"{lib.path} has no replica engine")
         self.lib = lib
@@ -43,6 +44,8 @@ class ReplicaEngine:
             lib.maze_engine_set_threads(self.h, int(threads))
         if graphs is not None:
             lib.maze_engine_set_graphs(self.h, 1 if graphs else 0)
+        if mixed_init is not None:
+            lib.maze_engine_set_mixed_init(self.h, 1 if mixed_init else 0)
         self.systems = {}
 
     def close(self):

@@ -20,9 +20,11 @@ def systems():
 
 
 def single(lib, s, precision, steps):
+    """The single-replica C-ABI path with the engine's defaults (mixed cold solves)."""
     cfg = driver.RunConfig(tol=W.poisson_tol(s))
     driver.initialize(lib, s, cfg)
     lib.lib.solver_set_precision(precision)
+    lib.lib.solver_set_mixed_init(1)
     driver.bootstrap(lib, cfg)
     for _ in range(steps):
         lib.solver_md_loop_iter()
@@ -99,6 +101,24 @@ def test_engine_mix_of_many(hip, reference_runs):
             results.append([e.result(r) for r in others])
     for a, b in zip(*results):
         assert same(a, b)
+
+
+def test_engine_reference_cold_solves(hip):
+    """With mixed cold solves off, a replica equals the reference call sequence of the
+    single-replica path bit for bit (FP64 CG start-up)."""
+    s = systems()[0]
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    driver.initialize(hip, s, cfg)
+    hip.lib.solver_set_precision(2)
+    driver.bootstrap(hip, cfg)
+    for _ in range(STEPS):
+        hip.solver_md_loop_iter()
+    ref = driver.snapshot(hip, s, fields=False)
+    hip.solver_finalize()
+    with ReplicaEngine(hip, mixed_init=False) as e:
+        rid = e.submit(s, STEPS)
+        e.run()
+        assert same(e.result(rid), ref)
 
 
 def test_engine_memory_estimate(hip):

@@ -489,6 +489,28 @@ def test_mixed_mg_forces_odd_coarsest(hip, n):
     assert err <= 1e-5, err
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_cold_solve(hip, precision):
+    """solver_set_mixed_init(1): the start-up cold solves run the mixed path to the same
+    FP64 stop test; forces after the start-up and three steps stay within 1e-5 RMS of
+    the reference FP64-CG start-up."""
+    s = W.nacl_system(512, 64, seed=8)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    out = []
+    for mixed in (0, 1):
+        driver.initialize(hip, s, cfg)
+        hip.lib.solver_set_precision(precision)
+        hip.lib.solver_set_mixed_init(mixed)
+        driver.bootstrap(hip, cfg)
+        for _ in range(3):
+            driver.md_step(hip, cfg)
+        out.append(driver.snapshot(hip, s, fields=False))
+        hip.solver_finalize()
+    d = out[1]["fcs_tot"] - out[0]["fcs_tot"]
+    err = np.sqrt((d ** 2).mean()) / np.sqrt((out[0]["fcs_tot"] ** 2).mean())
+    assert err <= 1e-5, err
+
+
 def test_determinism(hip):
     s = W.nacl_system(512, 32, seed=5)
     cfg = driver.RunConfig(tol=W.poisson_tol(s))
