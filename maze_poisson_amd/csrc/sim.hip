@@ -598,6 +598,10 @@ void Sim::drop_graphs() {
         if (g) MAZE_HIP(hipGraphExecDestroy(g));
         g = nullptr;
     }
+    for (hipGraphExec_t& g : pass_graph) {
+        if (g) MAZE_HIP(hipGraphExecDestroy(g));
+        g = nullptr;
+    }
 }
 
 void Sim::inner_body(bool use_mg, float* x, int it) {
@@ -972,6 +976,23 @@ void Sim::mixed_refine(int add) {
 // One refinement pass of `m` inner iterations; poll: iterate until the
 // inner stop (reading the state after each iteration).
 void Sim::mixed_pass(int k, int m, bool poll) {
+    // A queued pass of up to kPassGraphs iterations replays one captured graph
+    // (its iterations and the refinement), one per length and y buffer order.
+    if (!poll && use_graphs && mx.use_mg && !pb && !prof && !mx.extrap && m >= 1 &&
+        m <= kPassGraphs && m <= mx.inner_limit) {
+        const int v = 2 * (m - 1) + (mx.ya < mx.yb ? 0 : 1);  // y and y_alt swap roles
+        if (!pass_graph[v]) {
+            hipGraph_t g = nullptr;
+            MAZE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            for (int it = 0; it < m; it++) inner_body(mx.use_mg, d32, it);
+            mixed_refine(1);
+            MAZE_HIP(hipStreamEndCapture(stream, &g));
+            MAZE_HIP(hipGraphInstantiate(&pass_graph[v], g, nullptr, nullptr, 0));
+            MAZE_HIP(hipGraphDestroy(g));
+        }
+        MAZE_HIP(hipGraphLaunch(pass_graph[v], stream));
+        return;
+    }
     cur_pass = k;
     int launched = 0;
     for (; launched < std::min(m, mx.inner_limit); launched++) inner_iter(mx.use_mg, d32, launched);

@@ -127,6 +127,10 @@ struct Sim {
     // Dropped with the multigrid levels (free_mg) and by set_mg_omegas.
     int use_graphs = 0;
     hipGraphExec_t inner_graph[3] = {nullptr, nullptr, nullptr};
+    // Whole queued refinement passes (m inner iterations and the refinement)
+    // as graphs, per length m <= kPassGraphs and order of the two y buffers.
+    static constexpr int kPassGraphs = 6;
+    hipGraphExec_t pass_graph[2 * kPassGraphs] = {};
     int mg_tail = 0;            // first level handled by the one-work-group tail kernel
 
     CGState* st = nullptr;      // device
