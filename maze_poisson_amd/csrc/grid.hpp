@@ -51,13 +51,17 @@ void launch_publish_state(hipStream_t s, const CGState* st, const double* q_sum,
                           unsigned int seq);
 
 // Face-centred dielectric maps and screening term of the PB operator; the
-// Poisson operator passes nullptrs.
-struct PBCoef {
-    const double* ex;
-    const double* ey;
-    const double* ez;
-    const double* k2;
+// Poisson operator passes nullptrs.  The FP32 form serves the FP32 inner
+// solves of the PB multigrid path.
+template <class T>
+struct PBCoefT {
+    const T* ex;
+    const T* ey;
+    const T* ez;
+    const T* k2;
 };
+using PBCoef = PBCoefT<double>;
+using PBCoef32 = PBCoefT<float>;
 
 struct Reduce {
     double* psum;
@@ -151,18 +155,25 @@ void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double
                           const double* Qf, const double* ilam, CGState* st);
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                     float* p_new, CGState* st, Reduce red, int first, const PBCoef& pb = PBCoef{});
-// FP32 multigrid for the linear PB operator (mg_kernels.hip).
-void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* g,
+// FP32 multigrid for the linear PB operator (mg_kernels.hip), with FP32
+// coefficients on every level.
+void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
+                    float* p_new, CGState* st, Reduce red, int first, const PBCoef32& pb);
+void launch_cg_update32(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* p,
+                        float* x, float* r, CGState* st, Reduce red, int zero_x);
+void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* g,
                       const float* b, const float* din, float* dout, double c, CGState* st,
                       int dot, Reduce red);
-void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* b,
+void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* b,
                      const float* din, float* r, CGState* st);
 void launch_pb_first(hipStream_t s, int n, const float* b, const float* g, double c, float* d,
                      CGState* st);
-void launch_pb_idg(hipStream_t s, int n, const PBCoef& pb, float* g);
-void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef& f, double* cx, double* cy, double* cz,
-                       double* ck2);
-void launch_pb_coarse(hipStream_t s, int c, const PBCoef& pb, const float* g, const float* b,
+void launch_pb_to32(hipStream_t s, long count, const PBCoef& in, float* ex, float* ey, float* ez,
+                    float* k2);
+void launch_pb_idg(hipStream_t s, int n, const PBCoef32& pb, float* g);
+void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef32& f, float* cx, float* cy, float* cz,
+                       float* ck2);
+void launch_pb_coarse(hipStream_t s, int c, const PBCoef32& pb, const float* g, const float* b,
                       float* x, int iters, double lo, double hi, CGState* st);
 // Fused level kernels of the V-cycle on an n^3 level (n even):
 // down: d2 = two pre-smoothing sweeps from zero, b_coarse = restricted residual;

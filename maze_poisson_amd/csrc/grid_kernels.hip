@@ -278,9 +278,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(GridDims d, PBCoef pb, const 
     reduce_and_finalize<false>(bd, red, st);
 }
 
-template <bool PB, class Real, bool ZeroX>
-__global__ __launch_bounds__(kBlock) void k_cg_update(GridDims d, PBCoef pb, const Real* p,
-                                                      Real* x, Real* r, Reduce red, CGState* st) {
+template <bool PB, class Real, bool ZeroX, class C = PBCoef>
+__global__ __launch_bounds__(kBlock) void k_cg_update(GridDims d, C pb, const Real* p, Real* x,
+                                                      Real* r, Reduce red, CGState* st) {
     if (st->done) return;
     BodyUpdate<Real, ZeroX> bd{x, r, st->alpha};
     march<PB>(d, pb, SrcArr<Real>{p}, bd);
@@ -311,6 +311,16 @@ void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const 
         DISPATCH_PB(pb, MAZE_LAUNCH((k_cg_update<kPB, Real, false>), dim3(d.blocks),
                                            dim3(kBlock), 0, s, d, pb, p, x, r, red, st));
     }
+}
+
+void launch_cg_update32(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* p,
+                        float* x, float* r, CGState* st, Reduce red, int zero_x) {
+    if (zero_x)
+        MAZE_LAUNCH((k_cg_update<true, float, true, PBCoef32>), dim3(d.blocks), dim3(kBlock), 0, s,
+                    d, pb, p, x, r, red, st);
+    else
+        MAZE_LAUNCH((k_cg_update<true, float, false, PBCoef32>), dim3(d.blocks), dim3(kBlock), 0,
+                    s, d, pb, p, x, r, red, st);
 }
 
 template void launch_cg_init<double>(hipStream_t, const GridDims&, const PBCoef&, const double*,

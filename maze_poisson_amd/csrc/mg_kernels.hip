@@ -1227,8 +1227,8 @@ struct BodyPcgDir {
     }
 };
 
-template <bool First, bool PB>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, PBCoef pb, const float* p_old,
+template <bool First, bool PB, class C = PBCoef>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, C pb, const float* p_old,
                                                     const float* v, float* p_new, Reduce red,
                                                     CGState* st) {
     if (st->done) return;
@@ -1258,9 +1258,19 @@ void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const 
     });
 }
 
+void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
+                    float* p_new, CGState* st, Reduce red, int first, const PBCoef32& pb) {
+    if (first)
+        MAZE_LAUNCH((k_pcg_dir<true, true, PBCoef32>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb,
+                    p_old, v, p_new, red, st);
+    else
+        MAZE_LAUNCH((k_pcg_dir<false, true, PBCoef32>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb,
+                    p_old, v, p_new, red, st);
+}
+
 // ---- FP32 multigrid for the linear Poisson-Boltzmann operator ----------------
-// A u = sum_f eps_f (u_nb - u) - k2 u with the face coefficients of each level
-// (FP64; level 0: the maps) and an FP32 inverse diagonal g = 1 / (sum of the
+// A u = sum_f eps_f (u_nb - u) - k2 u with FP32 face coefficients on every level
+// (level 0: a copy of the maps) and an FP32 inverse diagonal g = 1 / (sum of the
 // six face coefficients + k2).  Damped Jacobi: d <- d - w g (b - A d), with the
 // Chebyshev weights of the constant-coefficient cycle (the Jacobi-scaled
 // spectrum again lies in (0, 2)).  The passes are unfused march passes; the
@@ -1270,7 +1280,8 @@ void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const 
 // transverse fine faces by full weighting; k2 is fully weighted and scaled by
 // (H/h)^2 = 4 like the residual.  The coarsest level (at most 16^3) runs a
 // fixed number of Chebyshev iterations in one work-group, so the cycle is a
-// fixed linear operator.
+// fixed linear operator.  The PCG direction and update of the inner solve use
+// the same FP32 coefficients (the FP64 refinement keeps the FP64 maps).
 
 struct BodyPbSmooth {
     static constexpr bool kReduce = false, kMax = false, kSkipWhenDone = true;
@@ -1308,7 +1319,7 @@ struct BodyPbSmoothDot {
     }
 };
 
-void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* g,
+void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* g,
                       const float* b, const float* din, float* dout, double c, CGState* st,
                       int dot, Reduce red) {
     if (dot)
@@ -1317,7 +1328,7 @@ void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef& pb, const 
         run_pass<true>(s, d, pb, SrcArr<float>{din}, BodyPbSmooth{b, g, dout, c}, Reduce{}, st);
 }
 
-void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef& pb, const float* b,
+void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* b,
                      const float* din, float* r, CGState* st) {
     run_pass<true>(s, d, pb, SrcArr<float>{din}, BodyMgResid{b, r}, Reduce{}, st);
 }
@@ -1337,8 +1348,25 @@ void launch_pb_first(hipStream_t s, int n, const float* b, const float* g, doubl
     MAZE_LAUNCH(k_pb_first, dim3(pw_blocks_mg(N)), dim3(kBlock), 0, s, N, b, g, c, d, st);
 }
 
+// FP32 copies of the FP64 maps.
+__global__ void k_pb_to32(long N, PBCoef in, float* __restrict__ ex, float* __restrict__ ey,
+                          float* __restrict__ ez, float* __restrict__ k2) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        ex[t] = (float)in.ex[t];
+        ey[t] = (float)in.ey[t];
+        ez[t] = (float)in.ez[t];
+        k2[t] = (float)in.k2[t];
+    }
+}
+
+void launch_pb_to32(hipStream_t s, long count, const PBCoef& in, float* ex, float* ey, float* ez,
+                    float* k2) {
+    MAZE_LAUNCH(k_pb_to32, dim3(pw_blocks_mg(count)), dim3(kBlock), 0, s, count, in, ex, ey, ez, k2);
+}
+
 // g = 1 / (eps_x + eps_x(i-1) + eps_y + eps_y(j-1) + eps_z + eps_z(k-1) + k2).
-__global__ void k_pb_idg(int n, PBCoef pb, float* __restrict__ g) {
+__global__ void k_pb_idg(int n, PBCoef32 pb, float* __restrict__ g) {
     const long N = (long)n * n * n, n2 = (long)n * n;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
          t += (long)gridDim.x * blockDim.x) {
@@ -1346,20 +1374,20 @@ __global__ void k_pb_idg(int n, PBCoef pb, float* __restrict__ g) {
         const long im = (i == 0 ? n - 1 : i - 1) * n2 + (long)j * n + k;
         const long jm = (long)i * n2 + (j == 0 ? n - 1 : j - 1) * (long)n + k;
         const long km = (long)i * n2 + (long)j * n + (k == 0 ? n - 1 : k - 1);
-        const double dg = pb.ex[t] + pb.ex[im] + pb.ey[t] + pb.ey[jm] + pb.ez[t] + pb.ez[km] +
-                          pb.k2[t];
+        const double dg = (double)pb.ex[t] + pb.ex[im] + pb.ey[t] + pb.ey[jm] + pb.ez[t] +
+                          pb.ez[km] + pb.k2[t];
         g[t] = (float)(1.0 / dg);
     }
 }
 
-void launch_pb_idg(hipStream_t s, int n, const PBCoef& pb, float* g) {
+void launch_pb_idg(hipStream_t s, int n, const PBCoef32& pb, float* g) {
     const long N = (long)n * n * n;
     MAZE_LAUNCH(k_pb_idg, dim3(pw_blocks_mg(N)), dim3(kBlock), 0, s, n, pb, g);
 }
 
 // Coarse coefficients of an (nf/2)^3 level from the nf^3 one.
-__global__ void k_pb_coarsen(int nf, PBCoef f, double* __restrict__ cx, double* __restrict__ cy,
-                             double* __restrict__ cz, double* __restrict__ ck2) {
+__global__ void k_pb_coarsen(int nf, PBCoef32 f, float* __restrict__ cx, float* __restrict__ cy,
+                             float* __restrict__ cz, float* __restrict__ ck2) {
     const int M = nf / 2;
     const long total = (long)M * M * M, N = nf, N2 = N * N;
     auto w = [nf](int i) { return i < 0 ? i + nf : (i >= nf ? i - nf : i); };
@@ -1388,15 +1416,15 @@ __global__ void k_pb_coarsen(int nf, PBCoef f, double* __restrict__ cx, double* 
                 }
             }
         }
-        cx[t] = sx / 16.0;
-        cy[t] = sy / 16.0;
-        cz[t] = sz / 16.0;
-        ck2[t] = 4.0 * s2 / 64.0;
+        cx[t] = (float)(sx / 16.0);
+        cy[t] = (float)(sy / 16.0);
+        cz[t] = (float)(sz / 16.0);
+        ck2[t] = (float)(4.0 * s2 / 64.0);
     }
 }
 
-void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef& f, double* cx, double* cy, double* cz,
-                       double* ck2) {
+void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef32& f, float* cx, float* cy, float* cz,
+                       float* ck2) {
     const long M = nf / 2;
     MAZE_LAUNCH(k_pb_coarsen, dim3(pw_blocks_mg(M * M * M)), dim3(kBlock), 0, s, nf, f, cx, cy,
                 cz, ck2);
@@ -1409,7 +1437,7 @@ void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef& f, double* cx, doubl
 // between two arrays so that one barrier per iteration suffices.
 constexpr int kPbCoarseThreads = 1024;
 
-__global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef pb,
+__global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef32 pb,
                                                              const float* __restrict__ g,
                                                              const float* __restrict__ b,
                                                              float* __restrict__ x_out, int iters,
@@ -1429,10 +1457,10 @@ __global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef pb
     for (int q = 0; q < PER; q++) {
         const int t = tid + q * kPbCoarseThreads;
         if (t < N) {
-            ex[t] = (float)pb.ex[t];
-            ey[t] = (float)pb.ey[t];
-            ez[t] = (float)pb.ez[t];
-            k2[t] = (float)pb.k2[t];
+            ex[t] = pb.ex[t];
+            ey[t] = pb.ey[t];
+            ez[t] = pb.ez[t];
+            k2[t] = pb.k2[t];
             const int k = t % c, j = (t / c) % c, i = t / c2;
             nb[q][0] = (i == c - 1 ? 0 : i + 1) * c2 + j * c + k;
             nb[q][1] = (i == 0 ? c - 1 : i - 1) * c2 + j * c + k;
@@ -1477,7 +1505,7 @@ __global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef pb
     }
 }
 
-void launch_pb_coarse(hipStream_t s, int c, const PBCoef& pb, const float* g, const float* b,
+void launch_pb_coarse(hipStream_t s, int c, const PBCoef32& pb, const float* g, const float* b,
                       float* x, int iters, double lo, double hi, CGState* st) {
     if (c > kMgCoarseLds) {
         std::fprintf(stderr, "maze: PB multigrid coarsest level %d above %d\n", c, kMgCoarseLds);

@@ -610,10 +610,10 @@ void Sim::inner_body(bool use_mg, float* x, int it) {
     if (use_mg && pb) {  // PB multigrid-PCG
         vcycle_pb(0);
         timed(kTCgDirF32, it, [&] {
-            launch_pcg_dir(stream, dims, pin, pbmg[0].d, pout, st, red, it == 0, pbc());
+            launch_pcg_dir(stream, dims, pin, pbmg[0].d, pout, st, red, it == 0, pbmg[0].cf);
         });
         timed(kTCgUpdateF32, it, [&] {
-            launch_cg_update<float>(stream, dims, pbc(), pout, x, r32, st, red, it == 0);
+            launch_cg_update32(stream, dims, pbmg[0].cf, pout, x, r32, st, red, it == 0);
         });
     } else if (use_mg) {
         vcycle(0, it);
@@ -792,11 +792,10 @@ bool Sim::setup_pbmg() {
         L.n = sizes[l];
         L.dims = make_dims(L.n, choose_slab(L.n));
         const long g = (long)L.n * L.n * L.n;
-        if (l > 0) {
-            L.ex = dalloc<double>(g); L.ey = dalloc<double>(g);
-            L.ez = dalloc<double>(g); L.k2 = dalloc<double>(g);
-            L.b = dalloc<float>(g);
-        }
+        L.ex = dalloc<float>(g); L.ey = dalloc<float>(g);
+        L.ez = dalloc<float>(g); L.k2 = dalloc<float>(g);
+        L.cf = PBCoef32{L.ex, L.ey, L.ez, L.k2};
+        if (l > 0) L.b = dalloc<float>(g);
         L.g = dalloc<float>(g); L.d = dalloc<float>(g); L.t = dalloc<float>(g);
         pbmg.push_back(L);
     }
@@ -807,8 +806,7 @@ bool Sim::setup_pbmg() {
 
 void Sim::free_pbmg() {
     for (PbLevel& L : pbmg) {
-        for (double** a : {&L.ex, &L.ey, &L.ez, &L.k2}) dfree(*a);
-        for (float** a : {&L.g, &L.d, &L.t}) dfree(*a);
+        for (float** a : {&L.ex, &L.ey, &L.ez, &L.k2, &L.g, &L.d, &L.t}) dfree(*a);
         if (&L != &pbmg[0]) dfree(L.b);
     }
     pbmg.clear();
@@ -816,17 +814,13 @@ void Sim::free_pbmg() {
     pbmg_dirty = 1;
 }
 
-// Coefficients of every level from the current maps (after each map change).
+// FP32 coefficients of every level from the current maps (after each map change).
 void Sim::pbmg_coefs() {
     if (!pbmg_dirty) return;
     for (size_t l = 0; l < pbmg.size(); l++) {
         PbLevel& L = pbmg[l];
-        if (l == 0) {
-            L.cf = pbc();
-        } else {
-            launch_pb_coarsen(stream, pbmg[l - 1].n, pbmg[l - 1].cf, L.ex, L.ey, L.ez, L.k2);
-            L.cf = PBCoef{L.ex, L.ey, L.ez, L.k2};
-        }
+        if (l == 0) launch_pb_to32(stream, G, pbc(), L.ex, L.ey, L.ez, L.k2);
+        else launch_pb_coarsen(stream, pbmg[l - 1].n, pbmg[l - 1].cf, L.ex, L.ey, L.ez, L.k2);
         launch_pb_idg(stream, L.n, L.cf, L.g);
     }
     pbmg_dirty = 0;

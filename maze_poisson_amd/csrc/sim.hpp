@@ -106,13 +106,13 @@ struct Sim {
     }
     int mg_ok = -1;             // -1 not set up, 0 unavailable for this n, 1 ready
     // FP32 multigrid for the linear PB operator (precision kMixedMG with PB):
-    // per level the face coefficients (level 0: the maps; coarser: owned
-    // FP64 arrays), the inverse diagonal and the cycle's vectors.
+    // per level FP32 face coefficients (level 0: a copy of the maps), the
+    // inverse diagonal and the cycle's vectors.
     struct PbLevel {
         int n = 0;
         GridDims dims{};
-        PBCoef cf{};
-        double *ex = nullptr, *ey = nullptr, *ez = nullptr, *k2 = nullptr;
+        PBCoef32 cf{};
+        float *ex = nullptr, *ey = nullptr, *ez = nullptr, *k2 = nullptr;
         float *g = nullptr, *b = nullptr, *d = nullptr, *t = nullptr;
     };
     std::vector<PbLevel> pbmg;

@@ -94,9 +94,8 @@ struct PBPlane {
     double exc, eyc, ezc, eym, ezm, k2c;
 };
 
-template <bool PB>
-__device__ __forceinline__ PBPlane pb_load(const PBCoef& pb, int base, int pl,
-                                                 const PlaneNb& nb) {
+template <bool PB, class C>
+__device__ __forceinline__ PBPlane pb_load(const C& pb, int base, int pl, const PlaneNb& nb) {
     PBPlane r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (PB) {
         const int c = base + pl;
@@ -110,9 +109,8 @@ __device__ __forceinline__ PBPlane pb_load(const PBCoef& pb, int base, int pl,
     return r;
 }
 
-template <bool PB, class Src, class Body>
-__device__ __forceinline__ void march(const GridDims& d, const PBCoef& pb, const Src& src,
-                                      Body& body) {
+template <bool PB, class Src, class Body, class C>
+__device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src& src, Body& body) {
     using SR = typename Src::Raw;
     using BR = typename Body::Raw;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -205,20 +203,20 @@ __device__ __forceinline__ void reduce_and_finalize(Body& bd, Reduce red, CGStat
 
 // One sweep with an optional single-pass reduction of (sum, max) accumulated
 // by the body; the last work-group calls Body::finalize with the totals.
-template <bool PB, class Src, class Body>
-__global__ __launch_bounds__(kBlock) void k_pass(GridDims d, PBCoef pb, Src src, Body body,
-                                                 Reduce red, CGState* st) {
+template <bool PB, class Src, class Body, class C>
+__global__ __launch_bounds__(kBlock) void k_pass(GridDims d, C pb, Src src, Body body, Reduce red,
+                                                 CGState* st) {
     if (Body::kSkipWhenDone && st->done) return;
     Body bd = body;
     march<PB>(d, pb, src, bd);
     if constexpr (Body::kReduce) reduce_and_finalize<Body::kMax>(bd, red, st);
 }
 
-template <bool PB, class Src, class Body>
-static void run_pass(hipStream_t s, const GridDims& d, const PBCoef& pb, const Src& src,
+template <bool PB, class Src, class Body, class C>
+static void run_pass(hipStream_t s, const GridDims& d, const C& pb, const Src& src,
                      const Body& body, Reduce red, CGState* st) {
-    MAZE_LAUNCH((k_pass<PB, Src, Body>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, src,
-                       body, red, st);
+    MAZE_LAUNCH((k_pass<PB, Src, Body, C>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb, src, body,
+                red, st);
 }
 
 #define DISPATCH_PB(pb, ...)                \
