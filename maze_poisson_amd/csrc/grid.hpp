@@ -164,10 +164,10 @@ void launch_cg_update32(hipStream_t s, const GridDims& d, const PBCoef32& pb, co
 void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* g,
                       const float* b, const float* din, float* dout, double c, CGState* st,
                       int dot, Reduce red);
+void launch_pb_pre(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* g,
+                   const float* b, float* dout, double c1, double c2, CGState* st);
 void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* b,
                      const float* din, float* r, CGState* st);
-void launch_pb_first(hipStream_t s, int n, const float* b, const float* g, double c, float* d,
-                     CGState* st);
 void launch_pb_to32(hipStream_t s, long count, const PBCoef& in, float* ex, float* ey, float* ez,
                     float* k2);
 void launch_pb_idg(hipStream_t s, int n, const PBCoef32& pb, float* g);

@@ -1328,24 +1328,27 @@ void launch_pb_smooth(hipStream_t s, const GridDims& d, const PBCoef32& pb, cons
         run_pass<true>(s, d, pb, SrcArr<float>{din}, BodyPbSmooth{b, g, dout, c}, Reduce{}, st);
 }
 
+// First sweep from zero evaluated on the fly: d1 = -c1 g b (rounded to FP32),
+// so the two pre-smoothing sweeps are one pass: d2 = d1 - c2 g (b - A d1).
+struct SrcPbFirst {
+    const float* b;
+    const float* g;
+    double c;
+    struct Raw { float b, g; };
+    __device__ __forceinline__ Raw load(int i) const { return Raw{b[i], g[i]}; }
+    __device__ __forceinline__ double val(const Raw& r) const {
+        return (double)(float)(-c * (double)r.g * (double)r.b);
+    }
+};
+
+void launch_pb_pre(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* g,
+                   const float* b, float* dout, double c1, double c2, CGState* st) {
+    run_pass<true>(s, d, pb, SrcPbFirst{b, g, c1}, BodyPbSmooth{b, g, dout, c2}, Reduce{}, st);
+}
+
 void launch_pb_resid(hipStream_t s, const GridDims& d, const PBCoef32& pb, const float* b,
                      const float* din, float* r, CGState* st) {
     run_pass<true>(s, d, pb, SrcArr<float>{din}, BodyMgResid{b, r}, Reduce{}, st);
-}
-
-// First sweep from zero: d = -c g b.
-__global__ void k_pb_first(long N, const float* __restrict__ b, const float* __restrict__ g,
-                           double c, float* __restrict__ d, const CGState* st) {
-    if (st->done) return;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
-         t += (long)gridDim.x * blockDim.x)
-        d[t] = (float)(-c * (double)g[t] * (double)b[t]);
-}
-
-void launch_pb_first(hipStream_t s, int n, const float* b, const float* g, double c, float* d,
-                     CGState* st) {
-    const long N = (long)n * n * n;
-    MAZE_LAUNCH(k_pb_first, dim3(pw_blocks_mg(N)), dim3(kBlock), 0, s, N, b, g, c, d, st);
 }
 
 // FP32 copies of the FP64 maps.

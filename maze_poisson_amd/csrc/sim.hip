@@ -836,8 +836,7 @@ void Sim::vcycle_pb(int l) {
         return;
     }
     PbLevel& C = pbmg[l + 1];
-    launch_pb_first(stream, L.n, L.b, L.g, mg_omega, L.t, st);
-    launch_pb_smooth(stream, L.dims, L.cf, L.g, L.b, L.t, L.d, mg_omega2, st, 0, red);
+    launch_pb_pre(stream, L.dims, L.cf, L.g, L.b, L.d, mg_omega, mg_omega2, st);
     launch_pb_resid(stream, L.dims, L.cf, L.b, L.d, L.t, st);
     launch_mg_restrict(stream, L.n, L.t, C.b, st);
     vcycle_pb(l + 1);
