@@ -1180,8 +1180,12 @@ void Sim::init_field_mixed() {
         mixed_pass(m.queued++, 1, true);
         read_state();
     }
-    if (!st_host->outer_done) fatal("maze: the mixed-precision cold solve did not converge");
+    const bool converged = st_host->outer_done;
     if (st_host->ysel) launch_copy(stream, G, m.yb, phi);
+    if (!converged) {  // finish with FP64 CG from the refined iterate
+        int pred = std::max(32, last_iters);
+        cg_solve(dims, pbc(), sigma, phi, tol, n * n, r, p0, p1, &pred);
+    }
 }
 
 int Sim::update_field() {
