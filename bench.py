@@ -336,18 +336,19 @@ def ensemble_rate(lib, replicas, steps, slots, graphs=True):
             "unconverged_solves": int(sum(r["failures"] for r in rs1))}
 
 
-def serving_run(lib, count, rate_hz, steps_range, slots, budget_gb):
+def serving_run(lib, count, rate_hz, steps_range, slots, budget_gb, threads=1):
     """Poisson-arrival MD-segment requests through the replica engine (continuous
     batching: admission by free slot and device-memory budget while others run)."""
     from maze_poisson_amd.engine import ReplicaEngine
     trace = W.serving_trace(count, rate_hz, steps_range=steps_range)
-    with ReplicaEngine(lib, budget_bytes=int(budget_gb * 2**30), max_slots=slots) as e:
+    with ReplicaEngine(lib, budget_bytes=int(budget_gb * 2**30), max_slots=slots,
+                       threads=threads) as e:
         ids = [e.submit(r.system, r.steps, arrival=r.arrival) for r in trace]
         st = e.run()
         rs = [e.request_stats(i) for i in ids]
     waits = [r["admit_s"] - r["arrival_s"] for r in rs]
     return {"requests": count, "rate_hz": rate_hz, "steps_range": list(steps_range),
-            "slots": slots, "budget_gb": budget_gb,
+            "slots": slots, "budget_gb": budget_gb, "host_threads": threads,
             "ions_range": [min(r.system.n_p for r in trace), max(r.system.n_p for r in trace)],
             "grid_range": [min(r.system.n_grid for r in trace),
                            max(r.system.n_grid for r in trace)],
@@ -418,6 +419,7 @@ def main():
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[100, 2000])
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-threads", type=int, default=1)
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one 256 MiB copy kernel first (byte-counter calibration)")
     args = ap.parse_args()
@@ -450,7 +452,7 @@ def main():
     if args.serving:
         print(json.dumps(serving_run(lib, args.serve_requests, args.serve_rate,
                                      tuple(args.serve_steps), args.serve_slots,
-                                     args.serve_budget_gb)))
+                                     args.serve_budget_gb, args.serve_threads)))
         return
     head = run_path(lib, PATHS["mixed_mg"], args.steps, args.warmup, args.steps, args.prof_steps)
     line_extra = {}

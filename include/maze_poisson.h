@@ -318,8 +318,10 @@ void maze_engine_destroy(maze_engine *e);
  * fastest (kernel launches from several threads contend in the runtime). */
 void maze_engine_set_threads(maze_engine *e, int threads);
 /* 1 (default): each replica captures its multigrid-PCG inner iteration (the
- * V-cycle kernels, direction and update) once as HIP graphs and replays them;
- * 0: separate launches.  Results are identical. */
+ * V-cycle kernels, direction and update) and its queued refinement passes
+ * once as HIP graphs and replays them; 0: separate launches.  Results are
+ * identical.  Runs with more than one host thread always launch separately
+ * (concurrent HIP calls from other threads invalidate a stream capture). */
 void maze_engine_set_graphs(maze_engine *e, int on);
 /* 1 (default): the replicas' start-up cold solves use the mixed path of their
  * precision (solver_set_mixed_init); 0: FP64 CG as in the reference. */

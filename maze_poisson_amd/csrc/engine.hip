@@ -127,6 +127,7 @@ struct Engine {
     long long used = 0, peak = 0;
     double t0 = 0, t_end = 0;
     double host_admit = 0, host_issue = 0, host_finish = 0;  // host seconds per phase
+    int run_threads = 1;  // worker threads of the current run
 
     // Streams of retired replicas, reused by the next ones (creating a stream
     // costs milliseconds); destroyed with the engine.
@@ -178,7 +179,10 @@ struct Engine {
         s.init_particles(r.n, r.n_typ, r.L, h, r.np, 0, r.types.data(), r.pos.data(),
                          r.vel.data(), r.mass.data(), r.charges.data(), r.tf.data());
         s.init_integrator(r.dt, 0.0, 0.0, kVerlet, 0);
-        s.use_graphs = graphs;
+        // Stream capture is invalidated by HIP calls that other threads make
+        // meanwhile (hipStreamEndCapture reports it), so only a one-thread run
+        // replays graphs.
+        s.use_graphs = graphs && run_threads == 1;
         s.mixed_init = mixed_init;
         // The two start-up steps with cold solves (maze_poisson/solver.py:376-398).
         for (int k = 0; k < 2; k++) {
@@ -306,6 +310,7 @@ struct Engine {
         used = peak = 0;
         host_admit = host_issue = host_finish = 0;
         const int nt = std::max(1, std::min(threads, max_slots));
+        run_threads = nt;
         const int share = (max_slots + nt - 1) / nt;
         if (nt == 1) {
             worker(share);

@@ -81,12 +81,12 @@ def test_engine_alone_and_under_budget(hip, reference_runs):
 
 def test_engine_mix_of_many(hip, reference_runs):
     """The four reference replicas inside a mix of twelve of uneven sizes, driven by
-    one host thread replaying inner iterations as graphs (all twelve admitted
-    before the first step) and by four threads with separate launches."""
+    one host thread replaying graphs (all twelve admitted before the first step)
+    and by four threads, which launch separately even with graphs asked for."""
     ss = systems()
     extra = W.ensemble_systems(8, sizes=(216, 512), seed0=4100)
     results = []
-    for threads, graphs in ((1, True), (4, False)):
+    for threads, graphs in ((1, True), (4, True)):  # four threads launch separately
         with ReplicaEngine(hip, max_slots=16, threads=threads, graphs=graphs) as e:
             mine = [e.submit(s, STEPS, precision=prec) for s, (_, _, prec) in zip(ss, CASES)]
             others = [e.submit(s, STEPS + 3 * i) for i, s in enumerate(extra)]
