@@ -176,6 +176,7 @@ struct Engine {
         const double h = r.L / r.n;
         s.init_grid(r.n, r.L, h, r.tol, 1.0, 1.0, kMazeLCG, kPNone);
         s.precision = r.used_precision;
+        s.mg_autotune = 1;
         s.init_particles(r.n, r.n_typ, r.L, h, r.np, 0, r.types.data(), r.pos.data(),
                          r.vel.data(), r.mass.data(), r.charges.data(), r.tf.data());
         s.init_integrator(r.dt, 0.0, 0.0, kVerlet, 0);

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 #include "sim.hpp"
@@ -722,6 +723,7 @@ bool Sim::setup_mg() {
             }
             if (q && *q >= '0' && *q < '0' + kMgTiles) L.tile = *q - '0';
         }
+        L.tile_up = L.tile;
         L.d = dalloc<float>(g);
         L.t = dalloc<float>(g);
         mg.push_back(L);
@@ -776,7 +778,66 @@ bool Sim::setup_mg() {
         }
     }
     mg_ok = 1;
+    if (mg_autotune) tune_mg_tiles();
     return true;
+}
+
+namespace {
+std::mutex g_tune_mu;
+std::map<int, std::pair<int, int>> g_tuned;  // level size -> (down tile, up tile)
+}  // namespace
+
+// Times every tile shape that can run a level (five launches each, on the
+// level's own buffers with a scratch solver state) and keeps the fastest.
+void Sim::tune_mg_tiles() {
+    CGState* tst = dalloc<CGState>(1);  // zeroed: not done
+    hipEvent_t a, b;
+    MAZE_HIP(hipEventCreate(&a));
+    MAZE_HIP(hipEventCreate(&b));
+    const double c1 = mg_omega / 6.0, c2 = mg_omega2 / 6.0;
+    for (int l = 0; l + 1 < (int)mg.size() && l < mg_tail; l++) {
+        MGLevel& L = mg[l];
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tuned.find(L.n);
+            if (it != g_tuned.end()) {
+                L.tile = it->second.first;
+                if (l > 0) L.tile_up = it->second.second;
+                continue;
+            }
+        }
+        auto time_it = [&](auto&& launch) {
+            launch();  // warm-up
+            MAZE_HIP(hipEventRecord(a, stream));
+            for (int r = 0; r < 5; r++) launch();
+            MAZE_HIP(hipEventRecord(b, stream));
+            MAZE_HIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            MAZE_HIP(hipEventElapsedTime(&ms, a, b));
+            return ms;
+        };
+        float best_d = 1e30f, best_u = 1e30f;
+        int td = L.tile, tu = L.tile_up;
+        for (int t = 0; t < kMgTiles; t++) {
+            if (!mg_tile_ok(t, L.n)) continue;
+            const float md = time_it(
+                [&] { launch_mg_down(stream, L.n, t, L.t, L.d, mg[l + 1].b, c1, c2, tst); });
+            if (md < best_d) best_d = md, td = t;
+            if (l == 0) continue;  // the finest up kernel reduces <r, z>: default shape
+            const float mu = time_it([&] {
+                launch_mg_up(stream, L.n, t, L.t, L.d, mg[l + 1].d, L.b, c1, c2, tst, 0, red);
+            });
+            if (mu < best_u) best_u = mu, tu = t;
+        }
+        L.tile = td;
+        if (l > 0) L.tile_up = tu;
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tuned[L.n] = {td, tu};
+    }
+    MAZE_HIP(hipEventDestroy(a));
+    MAZE_HIP(hipEventDestroy(b));
+    sync();
+    dfree(tst);
 }
 
 // The PB multigrid needs a hierarchy whose coarsest level fits one work-group.
@@ -870,7 +931,7 @@ void Sim::vcycle(int l, int iter) {
     t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st); });
     vcycle(l + 1, iter);
     t(kTMgUp, [&] {
-        launch_mg_up(stream, L.n, L.tile, L.b, L.t, mg[l + 1].d, L.d, c1, c2, st, top, red);
+        launch_mg_up(stream, L.n, L.tile_up, L.b, L.t, mg[l + 1].d, L.d, c1, c2, st, top, red);
     });
 }
 

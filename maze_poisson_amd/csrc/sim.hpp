@@ -36,7 +36,8 @@ struct MGLevel {
     float* b;  // right-hand side (level 0: the CG residual)
     float* d;  // correction / cycle output
     float* t;  // residual and Jacobi ping-pong buffer
-    int tile;  // MgTile of the fused level kernels
+    int tile;     // MgTile of the fused down kernel
+    int tile_up;  // MgTile of the fused up kernel
 };
 
 // Per-kernel timing collected with HIP events on the solver stream.
@@ -289,6 +290,11 @@ struct Sim {
     void inner_start(bool use_mg, const float* b, float* x, double rtol, int limit, int gated);
     int inner_solve(bool use_mg, const float* b, float* x, double rtol, int limit, int first_chunk);
     bool setup_mg();
+    void tune_mg_tiles();
+    // Pick each level's tile shapes by timing (replica engine): the down
+    // kernels and the up kernels below the finest level, which reduce nothing,
+    // so the results do not depend on the choice.  Cached per level size.
+    int mg_autotune = 0;
     bool setup_pbmg();
     void free_pbmg();
     void pbmg_coefs();
