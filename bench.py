@@ -375,6 +375,19 @@ def measured_traffic(name):
         return None
 
 
+COPY_GBS = {}
+
+
+def copy_rates(lib):
+    """The copy kernel's measured rate on this box: over 26 MB (about the dominant
+    kernel's working set, resident in the 256 MB Infinity Cache) and over 2 GB
+    (HBM)."""
+    if not COPY_GBS and lib.has("maze_copy_bandwidth"):
+        COPY_GBS["cache_26mb"] = round(lib.maze_copy_bandwidth(26_214_400 // 16, 50), 1)
+        COPY_GBS["hbm_2gb"] = round(lib.maze_copy_bandwidth(1 << 27, 10), 1)
+    return COPY_GBS
+
+
 def roofline(prof):
     """Dominant grid kernel (largest total event time) of the headline path."""
     G = N_GRID ** 3
@@ -386,9 +399,14 @@ def roofline(prof):
     avg_s = tot_ms / cnt * 1e-3
     bpl = BYTES_PER_POINT[name] * G
     ach = bpl / avg_s / 1e9
-    return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
-            "avg_us": round(avg_s * 1e6, 2), "launches": cnt, "bytes_per_launch": bpl}
+    out = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
+           "avg_us": round(avg_s * 1e6, 2), "launches": cnt, "bytes_per_launch": bpl}
+    if COPY_GBS:
+        out["copy_gbs_measured"] = dict(COPY_GBS)
+        out["frac_of_copy_cache"] = round(ach / COPY_GBS["cache_26mb"], 4)
+        out["frac_of_copy_hbm"] = round(ach / COPY_GBS["hbm_2gb"], 4)
+    return out
 
 
 def main():
@@ -454,6 +472,7 @@ def main():
                                      tuple(args.serve_steps), args.serve_slots,
                                      args.serve_budget_gb, args.serve_threads)))
         return
+    copy_rates(lib)
     head = run_path(lib, PATHS["mixed_mg"], args.steps, args.warmup, args.steps, args.prof_steps)
     line_extra = {}
     if not args.no_extra:

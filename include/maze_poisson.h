@@ -193,6 +193,10 @@ void get_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
  * 2 = as 1 with the multigrid-preconditioned FP32 CG below.  Returns 0, or
  * -1 for an unknown mode. */
 int solver_set_precision(int mode);
+/* Bandwidth in GB/s (bytes read + written) of the library's copy kernel over
+ * n_doubles doubles, averaged over reps launches: the on-box copy rate that
+ * memory-bound kernels are compared with (no reference counterpart). */
+double maze_copy_bandwidth(long n_doubles, int reps);
 /* 1: the cold solves of solver_init_field use the mixed path of the current
  * precision mode (refinement to the same FP64 stop test) instead of FP64 CG;
  * 0 (default): FP64 CG as in the reference.  MAZE-LCG only. */

@@ -114,6 +114,7 @@ EXTENSION_ABI = [
     ("maze_engine_set_graphs", None, [ctypes.c_void_p, _c_int]),
     ("maze_engine_set_mixed_init", None, [ctypes.c_void_p, _c_int]),
     ("solver_set_mixed_init", None, [_c_int]),
+    ("maze_copy_bandwidth", _c_double, [ctypes.c_long, _c_int]),
     ("maze_engine_estimate_bytes", ctypes.c_longlong, [_c_int, _c_int, _c_int]),
     ("maze_engine_precision_for", _c_int, [_c_int, _c_int]),
     ("maze_engine_submit", _c_int, [ctypes.c_void_p, _c_double, _c_int, _c_double, _c_double,

@@ -301,6 +301,35 @@ void maze_calib_copy(long n_doubles) {
     MAZE_HIP(hipFree(b));
 }
 
+// Bandwidth (GB/s, bytes read + written) of the library's copy kernel over
+// n_doubles doubles, averaged over `reps` launches after one warm-up: the
+// on-box reference a memory-bound kernel is compared with.
+double maze_copy_bandwidth(long n_doubles, int reps) {
+    if (n_doubles <= 0 || reps <= 0) return 0.0;
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    MAZE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    MAZE_HIP(hipEventCreate(&e0));
+    MAZE_HIP(hipEventCreate(&e1));
+    MAZE_HIP(hipMalloc(&a, n_doubles * sizeof(double)));
+    MAZE_HIP(hipMalloc(&b, n_doubles * sizeof(double)));
+    maze::launch_fill(s, n_doubles, 1.0, a);
+    maze::launch_copy(s, n_doubles, a, b);
+    MAZE_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) maze::launch_copy(s, n_doubles, a, b);
+    MAZE_HIP(hipEventRecord(e1, s));
+    MAZE_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    MAZE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    MAZE_HIP(hipFree(a));
+    MAZE_HIP(hipFree(b));
+    MAZE_HIP(hipEventDestroy(e0));
+    MAZE_HIP(hipEventDestroy(e1));
+    MAZE_HIP(hipStreamDestroy(s));
+    return 16.0 * (double)n_doubles * reps / (ms * 1e-3) / 1e9;
+}
+
 // ---- kernel-level entry points on host buffers (for parity tests) ------------------
 // Each runs on a private replica and copies in and out; none touches the
 // process-global state.
