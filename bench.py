@@ -492,6 +492,7 @@ def main():
             "pb_radii": pb_radii_rate(lib, max(10, args.steps // 10), 5),
             "pb_synthetic": pb_synthetic(lib, 30),
             "ensemble_64_replicas": ensemble_rate(lib, 64, 50, 64),
+            "serving": serving_run(lib, 16, 4.0, (100, 600), 32, 64.0),
             "python_driver_steps_per_s": round(driver_rate(lib, PATHS["mixed_mg"], 100, False), 2),
             "python_driver_with_getters_steps_per_s": round(
                 driver_rate(lib, PATHS["mixed_mg"], 100, True), 2),
