@@ -55,7 +55,8 @@ struct Request {
     int state = 0;  // 0 waiting, 1 running, 2 done
     int used_precision = kFP64;
     int done = 0, failures = 0;
-    long long est_bytes = 0, meas_bytes = -1, arena_used = 0, arena_overflow = 0;
+    long long est_bytes = 0, meas_bytes = -1, arena_used = 0, arena_overflow = 0, free0 = 0;
+    int boot = 0;  // start-up stage: 1 / 2 while the first / second cold solve runs
     std::unique_ptr<DevArena> arena;
     double t_admit = 0, t_done = 0, t_last = 0;
     std::vector<double> lat;  // seconds per step, as observed by the loop
@@ -185,25 +186,46 @@ struct Engine {
         // replays graphs.
         s.use_graphs = graphs && run_threads == 1;
         s.mixed_init = mixed_init;
-        // The two start-up steps with cold solves (maze_poisson/solver.py:376-398).
-        for (int k = 0; k < 2; k++) {
-            s.update_charges();
-            s.init_field();
-            s.forces_elec();
-            s.forces_noel(false);
-            s.forces_tot();
-            if (k == 0) s.part1();
-            else s.part2();
-        }
-        r.t_admit = r.t_last = now_s() - t0;
+        // The two start-up steps with cold solves (maze_poisson/solver.py:376-398)
+        // run through the loop like MD steps: the first cold solve is queued
+        // here, and boot() continues each time its state has arrived.
+        s.update_charges(true);  // checked when the solve's state arrives
+        s.init_field_issue();
+        arena_install(nullptr);
+        r.free0 = (long long)free0;
         r.state = 1;
+        r.boot = 1;
+    }
+
+    // Next start-up stage of a replica whose cold solve has published its state;
+    // returns true once the replica is stepping (its first MD step queued).
+    bool boot(Request& r) {
+        Sim& s = *r.sim;
+        arena_install(r.arena.get());
+        s.init_field_finish();
+        s.check_charge();
+        s.forces_elec();
+        s.forces_noel(false);
+        s.forces_tot();
+        if (r.boot == 1) {
+            s.part1();
+            s.update_charges(true);
+            s.init_field_issue();
+            r.boot = 2;
+            arena_install(nullptr);
+            return false;
+        }
+        s.part2();
+        r.boot = 0;
+        r.t_admit = r.t_last = now_s() - t0;
         if (r.steps > 0) s.step_issue();  // allocates the mixed solver's arrays
         arena_install(nullptr);
-        size_t free1 = 0;
+        size_t free1 = 0, total = 0;
         MAZE_HIP(hipMemGetInfo(&free1, &total));
-        r.meas_bytes = (long long)free0 - (long long)free1;  // exact with one worker only
+        r.meas_bytes = r.free0 - (long long)free1;  // exact with one replica set up at a time
         r.arena_used = (long long)r.arena->used;
         r.arena_overflow = (long long)r.arena->overflow;
+        return true;
     }
 
     // Results out, replica and arena freed.
@@ -258,12 +280,28 @@ struct Engine {
                 const double ta = now_s();
                 admit(reqs[id]);
                 h_admit += now_s() - ta;
-                if (reqs[id].steps == 0) retire(reqs[id]);
-                else mine.push_back(id);
+                mine.push_back(id);
             }
             bool progressed = false;
             for (size_t a = 0; a < mine.size();) {
                 Request& r = reqs[mine[a]];
+                if (r.boot) {
+                    if (!r.sim->init_field_ready()) {
+                        a++;
+                        continue;
+                    }
+                    progressed = true;
+                    const double ta = now_s();
+                    const bool stepping = boot(r);
+                    h_admit += now_s() - ta;
+                    if (stepping && r.steps == 0) {
+                        retire(r);
+                        mine.erase(mine.begin() + (long)a);
+                        continue;
+                    }
+                    a++;
+                    continue;
+                }
                 if (!r.sim->step_ready()) {
                     a++;
                     continue;

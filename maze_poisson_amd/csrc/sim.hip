@@ -1032,7 +1032,8 @@ void Sim::mixed_refine(int add) {
 void Sim::mixed_pass(int k, int m, bool poll) {
     // A queued pass of up to kPassGraphs iterations replays one captured graph
     // (its iterations and the refinement), one per length and y buffer order.
-    if (!poll && use_graphs && mx.use_mg && !pb && !prof && !mx.extrap && m >= 1 &&
+    const bool md_buffers = (mx.ya == y || mx.ya == y_alt) && (mx.yb == y || mx.yb == y_alt);
+    if (!poll && use_graphs && mx.use_mg && !pb && !prof && !mx.extrap && md_buffers && m >= 1 &&
         m <= kPassGraphs && m <= mx.inner_limit) {
         const int v = 2 * (m - 1) + (mx.ya < mx.yb ? 0 : 1);  // y and y_alt swap roles
         if (!pass_graph[v]) {
@@ -1198,6 +1199,14 @@ int Sim::check_charge() {
 // library/src/grid_maze_lcg.c:100-124 and the matching init_field of the
 // other grid types: cold solve A phi = -4 pi q / (h eps) from zero.
 void Sim::init_field() {
+    init_field_issue();
+    init_field_finish();
+}
+
+// The cold solve in two halves, like an MD step: init_field_issue queues it up
+// to the publication of its state (init_field_ready polls for it) and
+// init_field_finish completes it.
+void Sim::init_field_issue() {
     hist_ok = 0;
     double c = -4 * kPi / h;
     if (!pb) c /= eps_s;
@@ -1205,19 +1214,30 @@ void Sim::init_field() {
     launch_copy(stream, G, phi, phip);
     launch_scale_copy(stream, G, c, q, sigma);
     launch_fill(stream, G, 0.0, phi);
-    if (mixed_init && precision != kFP64 && grid_type == kMazeLCG) {
-        init_field_mixed();
+    cold_mixed = mixed_init && precision != kFP64 && grid_type == kMazeLCG;
+    if (cold_mixed) {
+        cold_issue();
         return;
     }
-    int pred = std::max(32, last_iters);
-    cg_solve(dims, pbc(), sigma, phi, tol, n * n, r, p0, p1, &pred);
+    cold_pred = std::max(32, last_iters);
+    cg_issue(CgRun{dims, pbc(), phi, r, p0, p1, n * n, 0, std::max(4, cold_pred + 1), &cold_pred},
+             sigma, tol);
 }
 
-// The cold solve A phi = sigma from phi = 0 with the mixed path's refinement
-// loop (FP32 inner solves, FP64 residual and the same stop test
-// max|A phi - sigma| <= tol) instead of FP64 CG; phi* (tmp) is free here and
-// serves as the second iterate.  The MD steps' pass schedule is left alone.
-void Sim::init_field_mixed() {
+bool Sim::init_field_ready() const { return published(); }
+
+void Sim::init_field_finish() {
+    if (cold_mixed) cold_finish();
+    else cg_finish();
+}
+
+// The mixed cold solve: the refinement loop from phi = 0 to the FP64 stop test
+// max|A phi - sigma| <= tol, with FP32 inner solves; phi* (tmp) is free here
+// and serves as the second iterate.  kColdPasses passes of up to kColdIters
+// inner iterations are queued at once (iterations after an inner solve has
+// met its tolerance, and passes after the loop has, are no-ops); further
+// passes are polled.  The MD steps' pass schedule is left alone.
+void Sim::cold_issue() {
     if (!d32) {
         d32 = dalloc<float>(G); r32 = dalloc<float>(G);
         q32a = dalloc<float>(G); q32b = dalloc<float>(G); y_alt = dalloc<double>(G);
@@ -1236,8 +1256,15 @@ void Sim::init_field_mixed() {
     m.ya = phi;
     m.yb = tmp;
     mixed_refine(0);
-    read_state();
-    for (m.queued = 0; !st_host->outer_done && st_host->passes < kMaxOuter;) {
+    for (m.queued = 0; m.queued < kColdPasses; m.queued++)
+        mixed_pass(m.queued, std::min(kColdIters, m.inner_limit), false);
+    publish();
+}
+
+void Sim::cold_finish() {
+    MixedRun& m = mx;
+    wait_published();
+    while (!st_host->outer_done && st_host->passes < kMaxOuter) {
         mixed_pass(m.queued++, 1, true);
         read_state();
     }

@@ -253,10 +253,17 @@ struct Sim {
     int update_charges(bool defer = false);
     int check_charge();
     void init_field();
-    void init_field_mixed();
+    void init_field_issue();
+    bool init_field_ready() const;
+    void init_field_finish();
+    void cold_issue();
+    void cold_finish();
     // Cold solves (init_field) with the mixed path when precision != kFP64
     // (replica engine; solver_set_mixed_init), instead of the reference's FP64 CG.
     int mixed_init = 0;
+    int cold_mixed = 0;  // the cold solve in progress is the mixed one
+    int cold_pred = 32;  // FP64 cold solve: iterations of the first chunk
+    static constexpr int kColdPasses = 4, kColdIters = 6;
     int update_field();
     void forces_elec();
     double forces_noel(bool read_back = true);
