@@ -353,8 +353,11 @@ int maze_engine_result(maze_engine *e, int id, double *pos, double *vel, double 
 /* Per request (up to max values): arrival, admission and finish time (s from
  * the start of the run), steps done, mean / p50 / p99 step latency (ms),
  * estimated and measured device bytes, precision run, unconverged solves,
- * bytes used of the replica's arena (one device allocation per replica) and
- * bytes that did not fit it.  Returns the number of values written. */
+ * bytes used of the replica's arena (one device allocation per replica),
+ * bytes that did not fit it, and the drop of free device memory when the
+ * request's streams were created (0 when it reused pooled streams; streams
+ * are made four pairs at a time, and the first ones carry the runtime's
+ * per-process hardware-queue set-up).  Returns the number of values written. */
 int maze_engine_request_stats(maze_engine *e, int id, double *out, int max);
 /* Last run (up to max values): wall time (s), replica-steps/s, atom-steps/s,
  * p50 / p99 step latency over all steps (ms), peak estimated device bytes in

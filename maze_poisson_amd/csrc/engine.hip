@@ -56,6 +56,7 @@ struct Request {
     int used_precision = kFP64;
     int done = 0, failures = 0;
     long long est_bytes = 0, meas_bytes = -1, arena_used = 0, arena_overflow = 0, free0 = 0;
+    long long stream_bytes = 0;  // drop of free memory when its streams were created (0: reused)
     int boot = 0;  // start-up stage: 1 / 2 while the first / second cold solve runs
     std::unique_ptr<DevArena> arena;
     double t_admit = 0, t_done = 0, t_last = 0;
@@ -104,7 +105,10 @@ static long long arena_bytes(int n, int np, int precision) {
 }
 
 // Device memory a replica takes: its arena (hipMalloc rounds to 2 MiB) plus
-// what the runtime holds for its two streams and events (measured: 8 MiB).
+// what the runtime allocates beyond it for the replica's events and launches
+// (measured: at most 2 MiB). Per-process costs stay outside: the first
+// kernels load their code objects (17 MiB) and the first streams set up the
+// hardware queues (184 MiB).
 constexpr long long kReplicaExtra = 10LL << 20;
 static long long estimate_bytes(int n, int np, int precision) {
     const long long mib2 = 2LL << 20;
@@ -134,18 +138,30 @@ struct Engine {
     // costs milliseconds); destroyed with the engine.
     std::vector<std::pair<hipStream_t, hipStream_t>> stream_pool;
 
-    std::pair<hipStream_t, hipStream_t> take_streams() {
-        {
-            std::lock_guard<std::mutex> g(mu);
-            if (!stream_pool.empty()) {
-                auto p = stream_pool.back();
-                stream_pool.pop_back();
-                return p;
-            }
+    // New streams come four main streams, then their four side streams: the
+    // runtime hands streams to its hardware queues (four by default) in turn,
+    // so the replicas' solve streams spread over all queues instead of
+    // alternating with the short pair-force streams. The pool so holds at most
+    // three idle pairs beyond those of the replicas that ever ran at once.
+    // `created` gets the drop of free device memory over a new batch: the
+    // runtime sets up its hardware queues (a per-process cost, at most four
+    // queues) as the first streams are made, so it is not a per-replica cost.
+    std::pair<hipStream_t, hipStream_t> take_streams(long long& created) {
+        created = 0;
+        std::lock_guard<std::mutex> g(mu);
+        if (stream_pool.empty()) {
+            constexpr int kBatch = 4;
+            size_t f0 = 0, f1 = 0, total = 0;
+            MAZE_HIP(hipMemGetInfo(&f0, &total));
+            hipStream_t mains[kBatch], sides[kBatch];
+            for (auto& m : mains) MAZE_HIP(hipStreamCreateWithFlags(&m, hipStreamNonBlocking));
+            for (auto& sd : sides) MAZE_HIP(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+            for (int k = kBatch - 1; k >= 0; k--) stream_pool.push_back({mains[k], sides[k]});
+            MAZE_HIP(hipMemGetInfo(&f1, &total));
+            created = (long long)f0 - (long long)f1;
         }
-        std::pair<hipStream_t, hipStream_t> p;
-        MAZE_HIP(hipStreamCreateWithFlags(&p.first, hipStreamNonBlocking));
-        MAZE_HIP(hipStreamCreateWithFlags(&p.second, hipStreamNonBlocking));
+        auto p = stream_pool.back();
+        stream_pool.pop_back();
         return p;
     }
 
@@ -165,13 +181,15 @@ struct Engine {
     // start-up steps with cold solves and the first step's issue.
     void admit(Request& r) {
         r.used_precision = precision_for(r.n, r.precision);
+        // Streams come from the pool, created four pairs at a time and counted
+        // apart from the replica's own memory.
+        r.streams = take_streams(r.stream_bytes);
         size_t free0 = 0, total = 0;
         MAZE_HIP(hipMemGetInfo(&free0, &total));
         r.arena = std::make_unique<DevArena>();
         r.arena->size = (size_t)arena_bytes(r.n, r.np, r.used_precision);
         MAZE_HIP(hipMalloc(&r.arena->base, r.arena->size));
         arena_install(r.arena.get());
-        r.streams = take_streams();
         r.sim = std::make_unique<Sim>(r.streams.first, r.streams.second);
         Sim& s = *r.sim;
         const double h = r.L / r.n;
@@ -466,11 +484,12 @@ int maze_engine_request_stats(maze_engine* h, int id, double* out, int max) {
     double mean = 0;
     for (double x : r.lat) mean += x;
     if (!r.lat.empty()) mean /= (double)r.lat.size();
-    const double v[13] = {r.arrival, r.t_admit, r.t_done, (double)r.done, 1e3 * mean,
+    const double v[14] = {r.arrival, r.t_admit, r.t_done, (double)r.done, 1e3 * mean,
                           1e3 * maze::quantile(r.lat, 0.5), 1e3 * maze::quantile(r.lat, 0.99),
                           (double)r.est_bytes, (double)r.meas_bytes, (double)r.used_precision,
-                          (double)r.failures, (double)r.arena_used, (double)r.arena_overflow};
-    const int k = std::min(max, 13);
+                          (double)r.failures, (double)r.arena_used, (double)r.arena_overflow,
+                          (double)r.stream_bytes};
+    const int k = std::min(max, 14);
     for (int i = 0; i < k; i++) out[i] = v[i];
     return k;
 }

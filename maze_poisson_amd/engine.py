@@ -22,7 +22,7 @@ PRECISION = {"FP64": 0, "MIXED-CG": 1, "MIXED-MG": 2}
 
 REQUEST_FIELDS = ("arrival_s", "admit_s", "done_s", "steps", "mean_ms", "p50_ms", "p99_ms",
                   "est_bytes", "meas_bytes", "precision", "failures", "arena_used",
-                  "arena_overflow")
+                  "arena_overflow", "stream_bytes")
 STATS_FIELDS = ("wall_s", "replica_steps_per_s", "atom_steps_per_s", "p50_ms", "p99_ms",
                 "peak_bytes", "max_running", "host_admit_s", "host_issue_s", "host_finish_s")
 
