@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -115,6 +116,14 @@ static long long estimate_bytes(int n, int np, int precision) {
     return (arena_bytes(n, np, precision) + mib2 - 1) / mib2 * mib2 + kReplicaExtra;
 }
 
+// Hardware queues the runtime gives one process (its GPU_MAX_HW_QUEUES, 4 by
+// default).
+static int hw_queues() {
+    const char* v = std::getenv("GPU_MAX_HW_QUEUES");
+    const int q = v ? std::atoi(v) : 4;
+    return std::min(std::max(q, 1), 32);
+}
+
 struct Engine {
     long long budget = 0;
     int max_slots = 1;
@@ -138,11 +147,12 @@ struct Engine {
     // costs milliseconds); destroyed with the engine.
     std::vector<std::pair<hipStream_t, hipStream_t>> stream_pool;
 
-    // New streams come four main streams, then their four side streams: the
-    // runtime hands streams to its hardware queues (four by default) in turn,
-    // so the replicas' solve streams spread over all queues instead of
-    // alternating with the short pair-force streams. The pool so holds at most
-    // three idle pairs beyond those of the replicas that ever ran at once.
+    // New streams come Q main streams, then their Q side streams, Q being the
+    // runtime's hardware queues per process (GPU_MAX_HW_QUEUES, four by
+    // default): the runtime hands streams to its queues in turn, so the
+    // replicas' solve streams spread over all queues instead of alternating
+    // with the short pair-force streams. The pool so holds at most Q - 1 idle
+    // pairs beyond those of the replicas that ever ran at once.
     // `created` gets the drop of free device memory over a new batch: the
     // runtime sets up its hardware queues (a per-process cost, at most four
     // queues) as the first streams are made, so it is not a per-replica cost.
@@ -150,13 +160,13 @@ struct Engine {
         created = 0;
         std::lock_guard<std::mutex> g(mu);
         if (stream_pool.empty()) {
-            constexpr int kBatch = 4;
+            const int q = hw_queues();
             size_t f0 = 0, f1 = 0, total = 0;
             MAZE_HIP(hipMemGetInfo(&f0, &total));
-            hipStream_t mains[kBatch], sides[kBatch];
+            std::vector<hipStream_t> mains(q), sides(q);
             for (auto& m : mains) MAZE_HIP(hipStreamCreateWithFlags(&m, hipStreamNonBlocking));
             for (auto& sd : sides) MAZE_HIP(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
-            for (int k = kBatch - 1; k >= 0; k--) stream_pool.push_back({mains[k], sides[k]});
+            for (int k = q - 1; k >= 0; k--) stream_pool.push_back({mains[k], sides[k]});
             MAZE_HIP(hipMemGetInfo(&f1, &total));
             created = (long long)f0 - (long long)f1;
         }
