@@ -118,6 +118,7 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     red_side.pmax = dalloc<double>(kMaxPartials);
     red_side.counter = dalloc<unsigned int>(4);
     if (const char* e = std::getenv("MAZE_TF_OVERLAP")) tf_overlap = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MAZE_TF_LATE")) tf_late = std::atoi(e) != 0;
     if (const char* e = std::getenv("MAZE_INNER_MAX")) inner_max = std::atoi(e);
     st = dalloc<CGState>(1);
     MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
@@ -1331,7 +1332,7 @@ void Sim::part2() {
 }
 
 // One MD step in the order of maze_poisson/solver.py:483-492; the Tosi-Fumi
-// forces overlap the charge spread and the field solve on the side stream.
+// forces overlap the field solve on the side stream.
 void Sim::md_step() {
     step_issue();
     step_finish();
@@ -1343,13 +1344,15 @@ void Sim::md_step() {
 void Sim::step_issue() {
     part1();
     step_tf = tf_overlap && p.np > 0;
-    if (step_tf) {
+    auto issue_tf = [&] {
         MAZE_HIP(hipEventRecord(ev_pos, stream));
         MAZE_HIP(hipStreamWaitEvent(side, ev_pos, 0));
         timed(kTTf, 0, [&] { launch_tf(side, p, ps, red_side); });
         MAZE_HIP(hipEventRecord(ev_tf, side));
-    }
+    };
+    if (step_tf && !tf_late) issue_tf();
     update_charges(true);
+    if (step_tf && tf_late) issue_tf();
     update_eps_k2();
     step_wait = 1;
     if (grid_type == kMazeLCG) {
