@@ -56,9 +56,12 @@ char *get_integrator_type_str(int n);
 /* library/src/solver_md.c:19-38: prints a banner, selects the GPU. */
 void solver_initialize(void);
 
-/* library/src/solver_md.c:40-44 -> grid_general.c:24-83.  Exits(1) with a
- * message for FFT (not provided), multigrid on an odd n or n < 32, and
- * MAZE-LCG preconditioners other than NONE/JACOBI. */
+/* library/src/solver_md.c:40-44 -> grid_general.c:24-83.  MAZE-LCG takes the
+ * preconditioners NONE, JACOBI and MG (library/src/grid_maze_lcg.c:130-145;
+ * MG: one reference two-grid V-cycle per application, FP64 path).  Exits(1)
+ * with a message for FFT (not provided), multigrid or the MG preconditioner
+ * on an odd n or n < 32, and the SSOR and BLOCKJACOBI preconditioners (not
+ * provided). */
 void solver_initialize_grid(int n, double L, double h, double tol, double eps_s, double eps_int,
                             int grid_type, int precond_type);
 

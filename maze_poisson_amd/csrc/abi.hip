@@ -212,6 +212,7 @@ int solver_get_last_passes(int* iters, int max) {
 void solver_set_inner_rtol(double rtol) {
     S().inner_rtol = rtol;
     S().sched.clear();
+    S().drop_graphs();
 }
 
 // Start each mixed-precision MaZe solve from 2 y_n-1 - y_n-2 (1) or from
@@ -220,6 +221,7 @@ void solver_set_y_extrapolation(int on) {
     S().y_extrap = on ? 1 : 0;
     S().hist_ok = 0;
     S().sched.clear();
+    S().drop_graphs();
 }
 
 // One damping for both sweeps of the multigrid smoother (plain damped Jacobi).

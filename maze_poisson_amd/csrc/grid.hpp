@@ -85,6 +85,11 @@ void launch_cg_dir(hipStream_t s, const GridDims& d, const PBCoef& pb, const Rea
 template <class Real>
 void launch_cg_update(hipStream_t s, const GridDims& d, const PBCoef& pb, const Real* p, Real* x,
                       Real* r, CGState* st, Reduce red, int zero_x = 0);
+// <r, v> of the explicit-preconditioner recurrence (library/src/plcg.c:127,
+// :146-148): first: r_dot_v = <r, v>; else beta = <r, v> / r_dot_v and
+// r_dot_v = <r, v>.
+void launch_dot_rv(hipStream_t s, const GridDims& d, const double* r, const double* v, CGState* st,
+                   Reduce red, int first);
 // gated: the solve is started only while the outer refinement loop runs.
 void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int limit, int count_mode,
                      int pcg = 0, int gated = 0, int plain = 0);

@@ -239,6 +239,22 @@ struct BodyTrueRes {
     __device__ static void finalize(CGState* st, double, double M) { st->resid = M; }
 };
 
+// <r, v> for the explicit-preconditioner CG (library/src/plcg.c:127, :146-148).
+struct BodyDotRV {
+    static constexpr bool kReduce = true, kMax = false, kSkipWhenDone = false;
+    const double* r;
+    const double* v;
+    int first;
+    double s = 0.0, m = 0.0;
+    struct Raw { double r, v; };
+    __device__ Raw load(int c) const { return Raw{r[c], v[c]}; }
+    __device__ void apply(int, const Raw& o, double, double) { s += o.r * o.v; }
+    __device__ void finalize(CGState* st, double S, double) const {
+        if (!first) st->beta = S / st->rdv;
+        st->rdv = S;
+    }
+};
+
 // ---- launchers --------------------------------------------------------------
 void launch_laplace(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* u,
                     double* out) {
@@ -335,6 +351,12 @@ template void launch_cg_update<double>(hipStream_t, const GridDims&, const PBCoe
                                        double*, double*, CGState*, Reduce, int);
 template void launch_cg_update<float>(hipStream_t, const GridDims&, const PBCoef&, const float*,
                                       float*, float*, CGState*, Reduce, int);
+
+void launch_dot_rv(hipStream_t s, const GridDims& d, const double* r, const double* v, CGState* st,
+                   Reduce red, int first) {
+    BodyDotRV body{r, v, first};
+    run_pass<false>(s, d, PBCoef{}, SrcZero{}, body, red, st);
+}
 
 __global__ void k_cg_reset(CGState* st, double tol, double rtol, int limit, int count_mode,
                            int pcg, int gated, int plain) {

@@ -99,6 +99,8 @@ static int choose_slab(int n) {
     return (int)s;
 }
 
+GridDims grid_dims_for(int n) { return make_dims(n, choose_slab(n)); }
+
 Sim::Sim() : Sim(nullptr, nullptr) {}
 
 // main / side: streams the caller keeps (both or neither), for callers that
@@ -121,6 +123,7 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     if (const char* e = std::getenv("MAZE_TF_LATE")) tf_late = std::atoi(e) != 0;
     if (const char* e = std::getenv("MAZE_INNER_MAX")) inner_max = std::atoi(e);
     st = dalloc<CGState>(1);
+    st_c = dalloc<CGState>(1);
     MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
     std::memset(st_host, 0, sizeof(CGState));
     red.psum = dalloc<double>(kMaxPartials);
@@ -136,10 +139,15 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
 }
 
 Sim::~Sim() {
+    // Nothing may still run on either stream, or sit in a graph, when the
+    // buffers it uses are freed.
     sync();
+    MAZE_HIP(hipStreamSynchronize(side));
+    drop_graphs();
     free_grid();
     free_particles();
     dfree(st);
+    dfree(st_c);
     dfree(red.psum);
     dfree(red.pmax);
     dfree(red.counter);
@@ -155,8 +163,6 @@ Sim::~Sim() {
     for (auto e : ev_pool) MAZE_HIP(hipEventDestroy(e));
     MAZE_HIP(hipEventDestroy(ev_pos));
     MAZE_HIP(hipEventDestroy(ev_tf));
-    MAZE_HIP(hipStreamSynchronize(side));
-    drop_graphs();
     if (own_streams) {
         MAZE_HIP(hipStreamDestroy(side));
         MAZE_HIP(hipStreamDestroy(stream));
@@ -165,7 +171,7 @@ Sim::~Sim() {
 
 void Sim::free_grid() {
     for (double** a : {&q, &y, &phi, &phip, &tmp, &sigma, &r, &p0, &p1, &y_alt, &ex, &ey, &ez, &k2,
-                       &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1})
+                       &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1, &pv})
         dfree(*a);
     for (float** a : {&d32, &r32, &q32a, &q32b}) dfree(*a);
     for (double** a : {&cex, &cey, &cez, &ck2}) dfree(*a);
@@ -217,23 +223,28 @@ void Sim::init_grid(int n_, double L_, double h_, double tol_, double eps, doubl
     if (type < 0 || type > 4) fatal("maze: invalid grid type");
     if ((type == kMG || type == kMazeMG) && (n_ % 2 != 0 || n_ < 32))
         fatal("Multigrid: requires an even grid of at least 32 points per side");
-    if (type == kMazeLCG && precond_ > kPJacobi)
-        fatal("maze: MAZE-LCG provides the NONE and JACOBI preconditioners");
+    if (type == kMazeLCG && precond_ > kPMG)
+        fatal("maze: MAZE-LCG provides the NONE, JACOBI and MG preconditioners (SSOR and "
+              "BLOCKJACOBI are not provided)");
+    // library/src/multigrid.c:114-121 exits when the first halving falls below 16.
+    if (type == kMazeLCG && precond_ == kPMG && (n_ % 2 != 0 || n_ < 32))
+        fatal("Multigrid preconditioner: requires an even grid of at least 32 points per side");
     free_grid();
     n = n_; L = L_; h = h_; tol = tol_; eps_s = eps; eps_int = eps_int_;
     grid_type = type; precond = precond_;
     pb = 0;
     G = (long)n * n * n;
-    dims = make_dims(n, choose_slab(n));
+    dims = grid_dims_for(n);
     if (dims.blocks > kMaxPartials) fatal("maze: grid too large for the reduction buffers");
     q = dalloc<double>(G); y = dalloc<double>(G); phi = dalloc<double>(G);
     phip = dalloc<double>(G); tmp = dalloc<double>(G); sigma = dalloc<double>(G);
     r = dalloc<double>(G); p0 = dalloc<double>(G); p1 = dalloc<double>(G);
-    if (type == kMG || type == kMazeMG) {
+    if (type == kMG || type == kMazeMG || (type == kMazeLCG && precond_ == kPMG)) {
         nc = n / 2;
         cdims = make_dims(nc, choose_slab(nc));
         const long Gc = (long)nc * nc * nc;
         mg_r = dalloc<double>(G);
+        if (type == kMazeLCG) pv = dalloc<double>(G);
         c_rhs = dalloc<double>(Gc); c_eps = dalloc<double>(Gc); c_r = dalloc<double>(Gc);
         c_p0 = dalloc<double>(Gc); c_p1 = dalloc<double>(Gc);
     }
@@ -420,14 +431,14 @@ void Sim::init_integrator(double dt_, double T_, double gamma, int type, int en)
 // wake-up of a blocking wait on the critical path of every solve.  Should the
 // number not arrive within a second, the stream is synchronised and the
 // number checked once more.
-void Sim::read_state() {
-    publish();
+void Sim::read_state(const CGState* which) {
+    publish(which);
     wait_published();
 }
 
-void Sim::publish() {
+void Sim::publish(const CGState* which) {
     want_seq = ++pub_seq;
-    launch_publish_state(stream, st, &ps->q_sum, pub_dev, want_seq);
+    launch_publish_state(stream, which ? which : st, &ps->q_sum, pub_dev, want_seq);
 }
 
 bool Sim::published() const {
@@ -554,7 +565,8 @@ int Sim::cg_finish() {
 // multigrid.c:13-96); PB (coarse coefficients cpb): plain CG (library/src/
 // multigrid_pb.c:16-98).
 int Sim::cg_count_solve(const GridDims& d, const PBCoef& cpb, const double* b, double* x,
-                        double rtol, int maxit, double* rbuf, double* pa, double* pbuf, int* pred) {
+                        double rtol, int maxit, double* rbuf, double* pa, double* pbuf, int* pred,
+                        CGState* st) {
     const int plain = cpb.ex ? 1 : 0;
     launch_cg_reset(stream, st, 0.0, rtol, maxit, 1, 0, 0, plain);
     launch_cg_init<double>(stream, d, cpb, x, b, rbuf, st, red, nullptr, plain);
@@ -568,7 +580,7 @@ int Sim::cg_count_solve(const GridDims& d, const PBCoef& cpb, const double* b, d
             launch_cg_update<double>(stream, d, cpb, pout, x, rbuf, st, red);
             std::swap(pin, pout);
         }
-        read_state();
+        read_state(st);
         if (st_host->done || launched >= maxit) break;
         chunk = std::max(4, chunk / 4);
     }
@@ -1037,7 +1049,14 @@ void Sim::mixed_pass(int k, int m, bool poll) {
     if (!poll && use_graphs && mx.use_mg && !pb && !prof && !mx.extrap && md_buffers && m >= 1 &&
         m <= kPassGraphs && m <= mx.inner_limit) {
         const int v = 2 * (m - 1) + (mx.ya < mx.yb ? 0 : 1);  // y and y_alt swap roles
+        const RefineArgs& ga = pass_graph_ra[v];
+        if (pass_graph[v] && (ga.otol != mx.ra.otol || ga.rtol != mx.ra.rtol ||
+                              ga.limit != mx.ra.limit || ga.pcg != mx.ra.pcg)) {
+            MAZE_HIP(hipGraphExecDestroy(pass_graph[v]));
+            pass_graph[v] = nullptr;
+        }
         if (!pass_graph[v]) {
+            pass_graph_ra[v] = mx.ra;
             hipGraph_t g = nullptr;
             MAZE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
             for (int it = 0; it < m; it++) inner_body(mx.use_mg, d32, it);
@@ -1099,24 +1118,72 @@ int Sim::lcg_update() {
     return cg_solve(dims, pbc(), y, phi, tol, n * n, r, p0, p1, &last_iters);
 }
 
-// Reference two-grid V-cycle, library/src/multigrid.c:99-168 (3 RBGS sweeps),
-// or its Poisson-Boltzmann form, library/src/multigrid_pb.c:275-355 (4 RBGS-PB
-// sweeps, coarse coefficients restricted once per coefficient update).
-void Sim::mg_vcycle(const double* in, double* out) {
+// Reference two-grid V-cycle, library/src/multigrid.c:99-168 (3 RBGS sweeps
+// for the multigrid solvers, 5 as the MAZE-LCG preconditioner,
+// library/src/precond_mg.c:3-5), or its Poisson-Boltzmann form,
+// library/src/multigrid_pb.c:275-355 (4 RBGS-PB sweeps, coarse coefficients
+// restricted once per coefficient update).  sm <= 0: the solver's default.
+void Sim::mg_vcycle(const double* in, double* out, int sm) {
+    if (sm <= 0) sm = pb ? 4 : 3;
     if (pb)
-        launch_rbgs_pb(stream, dims, pbc(), in, out, 4);
+        launch_rbgs_pb(stream, dims, pbc(), in, out, sm);
     else
-        launch_rbgs(stream, dims, in, out, 3);
+        launch_rbgs(stream, dims, in, out, sm);
     launch_mg_residual(stream, dims, pbc(), in, out, mg_r);
     launch_restrict27(stream, n, mg_r, c_rhs);
     launch_fill(stream, (long)nc * nc * nc, 0.0, c_eps);
     const PBCoef cpb = pb ? PBCoef{cex, cey, cez, ck2} : PBCoef{};
-    cg_count_solve(cdims, cpb, c_rhs, c_eps, 1e-5, 50, c_r, c_p0, c_p1, &last_coarse);
+    cg_count_solve(cdims, cpb, c_rhs, c_eps, 1e-5, 50, c_r, c_p0, c_p1, &last_coarse, st_c);
     launch_prolong_add(stream, nc, c_eps, out);
     if (pb)
-        launch_rbgs_pb(stream, dims, pbc(), in, out, 4);
+        launch_rbgs_pb(stream, dims, pbc(), in, out, sm);
     else
-        launch_rbgs(stream, dims, in, out, 3);
+        launch_rbgs(stream, dims, in, out, sm);
+}
+
+// MaZe step with the reference's multigrid preconditioner,
+// library/src/verlet.c:31-71 with conj_grad_precond (grid_maze_lcg.c:130-145).
+int Sim::maze_pcg_mg() {
+    const double coef = (4 * kPi) / (h * eps_s);
+    launch_pred_sigma(stream, dims, PBCoef{}, phi, phip, q, coef, tmp, sigma);
+    double* old_phip = phip;
+    phip = phi;
+    phi = tmp;
+    tmp = old_phip;
+    const int res = pcg_mg_solve(sigma, y, tol, n * n);
+    launch_axpy(stream, G, -1.0, y, phi);
+    return res;
+}
+
+// Explicit-preconditioner CG, library/src/plcg.c:104-166, with P^-1 = one
+// reference V-cycle of 5 RBGS sweeps per side started from the previous v
+// (library/src/precond_mg.c:3-5; multigrid.c:170-173 does not clear `out`).
+// The first V-cycle of a solve starts from v = 0 (the reference's v is fresh
+// malloc'd memory there, library/src/plcg.c:120).  Every iteration waits for
+// the state: the V-cycle's coarse CG is itself polled from the host.
+// Returns the 0-based iteration index at convergence, or -1 after `limit`.
+int Sim::pcg_mg_solve(const double* b, double* x, double tol_, int limit) {
+    launch_cg_reset(stream, st, tol_, 0.0, limit, 0, /*pcg=*/1, 0, /*plain=*/1);
+    launch_cg_init<double>(stream, dims, PBCoef{}, x, b, r, st, red, nullptr, 1);  // r = A x - b
+    launch_fill(stream, G, 0.0, pv);
+    read_state();
+    if (st_host->done) return st_host->res;  // zero residual: x unchanged
+    mg_vcycle(r, pv, 5);
+    launch_dot_rv(stream, dims, r, pv, st, red, 1);
+    double* pin = p0;
+    double* pout = p1;
+    for (int it = 0; it < limit; it++) {
+        // p = -v (first) or beta p - v, then <p, A p> and alpha; the plain
+        // recurrence's coefficient -1 multiplies v in place of r.
+        launch_cg_dir<double>(stream, dims, PBCoef{}, pin, pv, pout, st, red, it == 0);
+        launch_cg_update<double>(stream, dims, PBCoef{}, pout, x, r, st, red);
+        std::swap(pin, pout);
+        read_state();
+        if (st_host->done) break;
+        mg_vcycle(r, pv, 5);
+        launch_dot_rv(stream, dims, r, pv, st, red, 0);
+    }
+    return st_host->res;
 }
 
 // Coarse PB coefficients of the reference two-grid cycle.
@@ -1282,6 +1349,7 @@ int Sim::update_field() {
         case kLCG:
             return lcg_update();
         case kMazeLCG:
+            if (precision == kFP64 && precond == kPMG && !pb) return maze_pcg_mg();
             return precision == kFP64 ? maze_fp64() : maze_mixed();  // kMixedCG / kMixedMG
         case kMG:
             return mg_update(false);
@@ -1355,7 +1423,8 @@ void Sim::step_issue() {
     if (step_tf && tf_late) issue_tf();
     update_eps_k2();
     step_wait = 1;
-    if (grid_type == kMazeLCG) {
+    const bool pcg_mg = grid_type == kMazeLCG && precision == kFP64 && precond == kPMG && !pb;
+    if (grid_type == kMazeLCG && !pcg_mg) {
         if (precision == kFP64) fp64_issue();
         else mixed_issue();
     } else {
@@ -1376,7 +1445,8 @@ bool Sim::step_ready() const { return !step_wait || published(); }
 // solver's result.
 int Sim::step_finish() {
     int res = step_res;
-    if (grid_type == kMazeLCG) res = precision == kFP64 ? fp64_finish() : mixed_finish();
+    const bool pcg_mg = grid_type == kMazeLCG && precision == kFP64 && precond == kPMG && !pb;
+    if (grid_type == kMazeLCG && !pcg_mg) res = precision == kFP64 ? fp64_finish() : mixed_finish();
     else if (step_wait) wait_published();
     check_charge();
     check_pb_flag();

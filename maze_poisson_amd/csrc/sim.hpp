@@ -52,6 +52,10 @@ struct KernelTimer {
     double bytes_per_launch = 0.0;
 };
 
+// The launch shape of the grid passes of an n^3 grid (planes per work-group
+// sized for about 512 work-groups per sweep); every replica uses it.
+GridDims grid_dims_for(int n);
+
 struct Sim {
     hipStream_t stream = nullptr;
 
@@ -132,9 +136,17 @@ struct Sim {
     // as graphs, per length m <= kPassGraphs and order of the two y buffers.
     static constexpr int kPassGraphs = 6;
     hipGraphExec_t pass_graph[2 * kPassGraphs] = {};
+    // The refinement arguments each pass graph was captured with (they are
+    // kernel arguments, fixed in the graph): a pass whose arguments differ
+    // (new tolerance, inner stop or budget) is captured again.
+    RefineArgs pass_graph_ra[2 * kPassGraphs] = {};
     int mg_tail = 0;            // first level handled by the one-work-group tail kernel
 
     CGState* st = nullptr;      // device
+    // State of the coarse CG inside the reference V-cycle, apart from st so
+    // that a V-cycle can precondition an outer CG (MAZE-LCG with precond MG).
+    CGState* st_c = nullptr;    // device
+    double* pv = nullptr;       // preconditioned residual v of the explicit PCG
     CGState* st_host = nullptr; // pinned
     StatePub* pub_host = nullptr;  // host-coherent, written by the GPU (read_state)
     StatePub* pub_dev = nullptr;   // its device address
@@ -294,7 +306,11 @@ struct Sim {
     int cg_solve(const GridDims& d, const PBCoef& pbc, const double* b, double* x, double tol,
                  int limit, double* rbuf, double* pa, double* pbuf, int* pred);
     int cg_count_solve(const GridDims& d, const PBCoef& cpb, const double* b, double* x,
-                       double rtol, int maxit, double* rbuf, double* pa, double* pbuf, int* pred);
+                       double rtol, int maxit, double* rbuf, double* pa, double* pbuf, int* pred,
+                       CGState* cst);
+    // MAZE-LCG with the reference's multigrid preconditioner (FP64 only).
+    int maze_pcg_mg();
+    int pcg_mg_solve(const double* b, double* x, double tol, int limit);
     void restrict_pb_coef();
     void inner_iter(bool use_mg, float* x, int it);
     void inner_body(bool use_mg, float* x, int it);
@@ -330,12 +346,13 @@ struct Sim {
     int cg_finish();
     int lcg_update();
     int mg_update(bool maze);
-    void mg_vcycle(const double* in, double* out);
+    void mg_vcycle(const double* in, double* out, int sm = 0);
     PBCoef pbc() const;
 
     void sync() { MAZE_HIP(hipStreamSynchronize(stream)); }
-    void read_state();         // publish + wait_published
-    void publish();            // queue the publication of the state (sequence want_seq)
+    void read_state(const CGState* which = nullptr);  // publish + wait_published
+    // Queue the publication of the state (sequence want_seq): st, or `which`.
+    void publish(const CGState* which = nullptr);
     bool published() const;    // the state of want_seq has arrived
     void wait_published();     // spin until it has, then copy it to st_host
     hipEvent_t next_event();

@@ -165,12 +165,55 @@ int or_conj_grad_jacobi(const double *b, double *x, double tol, int n) {
     return res;
 }
 
+/* CG with the reference's multigrid preconditioner: v = one two-grid V-cycle
+ * with 5 RBGS sweeps per side started from the previous v
+ * (library/src/precond_mg.c:3-5, multigrid.c:99-168 via multigrid_apply_recursive
+ * :170-173, whose zeroing of `out` is commented out), inside the explicit
+ * recurrence of library/src/plcg.c:104-166.  v holds the V-cycle's start and
+ * keeps the previous v from one iteration to the next.  The reference takes
+ * its first start from freshly malloc'd memory (mpi_grid_allocate,
+ * library/src/mpi_base.c:223-233); here it starts from zero, which is what
+ * the reference gets whenever malloc hands out fresh pages (the golden
+ * vectors are made that way). */
+int or_conj_grad_mg(const double *b, double *x, double tol, int n) {
+    const long G = (long)n * n * n, limit = (long)n * n;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *Ap = (double *)malloc(G * sizeof(double));
+    double *v = (double *)calloc(G, sizeof(double));
+    double *p = (double *)malloc(G * sizeof(double));
+    int res = -1;
+    or_laplace(x, r, n);
+    axpy(b, r, -1.0, G);
+    or_mg_vcycle(r, v, n, 5);
+    memcpy(p, v, G * sizeof(double));
+    scal(p, -1.0, G);
+    double r_dot_v = dot(r, v, G);
+    for (long it = 0; it < limit; it++) {
+        or_laplace(p, Ap, n);
+        double alpha = r_dot_v / dot(p, Ap, G);
+        axpy(p, x, alpha, G);
+        axpy(Ap, r, alpha, G);
+        (void)dot(r, r, G);
+        if (amax(r, G) <= tol) { res = (int)it; break; }
+        or_mg_vcycle(r, v, n, 5);
+        double rn_dot_vn = dot(r, v, G);
+        double beta = rn_dot_vn / r_dot_v;
+        r_dot_v = rn_dot_vn;
+        scal(p, beta, G);
+        axpy(v, p, -1.0, G);
+    }
+    free(r); free(Ap); free(v); free(p);
+    return res;
+}
+
 /* MaZe step: phi* = 2 phi - phi_prev, phi_prev <- phi; sigma = A phi* +
  * (4 pi / h) q; solve A y = sigma warm-started from y; phi = phi* - y.
  * library/src/verlet.c:31-71 (h already multiplied by eps_s by the caller,
- * library/src/grid_maze_lcg.c:158-164). */
+ * library/src/grid_maze_lcg.c:158-164).  precond: 0 the implicit-Jacobi CG
+ * (conj_grad), 1 the explicit Jacobi preconditioner, 2 the multigrid one
+ * (conj_grad_precond, library/src/grid_maze_lcg.c:130-145). */
 int or_verlet_poisson(double tol, double h, double *phi, double *phi_prev, const double *q,
-                      double *y, int n, int jacobi_explicit) {
+                      double *y, int n, int precond) {
     const long G = (long)n * n * n;
     double *sigma = (double *)malloc(G * sizeof(double));
     for (long i = 0; i < G; i++) {
@@ -180,7 +223,9 @@ int or_verlet_poisson(double tol, double h, double *phi, double *phi_prev, const
     }
     or_laplace(phi, sigma, n);
     axpy(q, sigma, (4 * OR_PI) / h, G);
-    int it = jacobi_explicit ? or_conj_grad_jacobi(sigma, y, tol, n) : or_conj_grad(sigma, y, tol, n);
+    int it = precond == 2   ? or_conj_grad_mg(sigma, y, tol, n)
+             : precond == 1 ? or_conj_grad_jacobi(sigma, y, tol, n)
+                            : or_conj_grad(sigma, y, tol, n);
     axpy(y, phi, -1.0, G);
     free(sigma);
     return it;

@@ -36,13 +36,14 @@ void or_laplace_pb(const double *u, double *out, int n, const double *ex,
 /* ---- Krylov solvers (return the 0-based iteration index, -1 if capped) -- */
 int or_conj_grad(const double *b, double *x, double tol, int n);
 int or_conj_grad_jacobi(const double *b, double *x, double tol, int n);
+int or_conj_grad_mg(const double *b, double *x, double tol, int n);
 int or_conj_grad_pb(const double *b, double *x, double tol, int n,
                     const double *ex, const double *ey, const double *ez,
                     const double *k2);
 
 /* ---- MaZe constraint step ---------------------------------------------- */
 int or_verlet_poisson(double tol, double h, double *phi, double *phi_prev,
-                      const double *q, double *y, int n, int jacobi_explicit);
+                      const double *q, double *y, int n, int precond);
 int or_verlet_poisson_pb(double tol, double h, double *phi, double *phi_prev,
                          const double *q, double *y, int n, const double *ex,
                          const double *ey, const double *ez, const double *k2);

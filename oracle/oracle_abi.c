@@ -96,8 +96,10 @@ void solver_initialize_grid(int n, double L, double h, double tol, double eps, d
     if (grid_type < 0 || grid_type > 4) die("oracle: invalid grid type");
     if ((grid_type == G_MG || grid_type == G_MAZE_MG) && (n % 2 || n < 32))
         die("oracle: multigrid needs an even n >= 32");
-    if (grid_type == G_MAZE_LCG && precond_type > P_JACOBI)
-        die("oracle: MAZE-LCG provides the NONE and JACOBI preconditioners only");
+    if (grid_type == G_MAZE_LCG && precond_type > P_MG)
+        die("oracle: MAZE-LCG provides the NONE, JACOBI and MG preconditioners only");
+    if (grid_type == G_MAZE_LCG && precond_type == P_MG && (n % 2 || n < 32))
+        die("oracle: the multigrid preconditioner needs an even n >= 32");
     memset(&g, 0, sizeof(g));
     g.n = n; g.L = L; g.h = h; g.tol = tol; g.eps_s = eps; g.eps_int = eps_int;
     g.type = grid_type; g.precond = precond_type;
@@ -290,7 +292,7 @@ int solver_update_field(void) {
             return or_verlet_poisson_pb(g.tol, g.h, g.phi_n, g.phi_p, g.q, g.y, g.n, g.ex, g.ey,
                                         g.ez, g.k2);
         return or_verlet_poisson(g.tol, g.h * g.eps_s, g.phi_n, g.phi_p, g.q, g.y, g.n,
-                                 g.precond == P_JACOBI);
+                                 g.precond);
     case G_MG:
         return mg_update();
     case G_MAZE_MG: {

@@ -7,7 +7,7 @@ drives it through its C-ABI and its exported kernel functions on seeded inputs a
 stores inputs and outputs as .npz fixtures.  Run with OMP_NUM_THREADS=1: the reference's
 charge scatter races under OpenMP (library/src/charges.c:123-140).
 
-    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+    OMP_NUM_THREADS=1 MALLOC_MMAP_THRESHOLD_=65536 python tests/golden/make_golden.py
 """
 import ctypes
 import json
@@ -238,6 +238,12 @@ def main():
         "maze_lcg_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=5),
         "maze_lcg_jacobi_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=3,
                                        precond="JACOBI"),
+        # The multigrid preconditioner starts its first V-cycle of every solve from a
+        # freshly malloc'd v (library/src/plcg.c:118-127, precond_mg.c:3-5); with
+        # MALLOC_MMAP_THRESHOLD_ set, glibc serves every grid-sized block from fresh
+        # (zeroed) pages, which makes that start zero and the vectors reproducible.
+        "maze_lcg_mg_512_32": dict(n_ions=512, n_grid=32, seed=1, method="MAZE-LCG", steps=3,
+                                   precond="MG"),
         "lcg_512_32": dict(n_ions=512, n_grid=32, seed=4, method="LCG", steps=3),
         "mg_512_32": dict(n_ions=512, n_grid=32, seed=2, method="MULTIGRID", steps=2),
         "maze_mg_512_32": dict(n_ions=512, n_grid=32, seed=2, method="MAZE-MULTIGRID", steps=2),
@@ -270,6 +276,8 @@ def main():
         cases = {k: v for k, v in cases.items() if k in only}
     for name, kw in cases.items():
         kw = dict(kw)
+        if kw.get("precond") == "MG" and not os.environ.get("MALLOC_MMAP_THRESHOLD_"):
+            sys.exit(f"{name}: run with MALLOC_MMAP_THRESHOLD_=65536 (see the case's comment)")
         args = {k: kw.pop(k) for k in ("n_ions", "n_grid", "seed", "method", "steps")}
         vec = md_vectors(lib, **args, **kw)
         np.savez_compressed(os.path.join(HERE, f"ref_md_{name}.npz"), **vec)

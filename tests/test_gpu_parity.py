@@ -177,6 +177,19 @@ def rel(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
 
 
+def assert_observables(g, o, s):
+    """Kinetic energy, temperature, momentum and the reference's energy_elec
+    (1/2 sum_g phi_g, library/src/grid_general.c:304-316) of two end states: each to
+    the accuracy of what it sums (velocities 1e-11 of their largest, fields 1e-7)."""
+    for k in ("kinetic", "temperature"):
+        assert abs(g[k] - o[k]) <= 1e-10 * abs(o[k]), (k, g[k], o[k])
+    mv = (s.mass[:, None] * np.abs(o["vel"])).sum()  # momentum is a near-cancelling sum
+    assert np.abs(g["momentum"] - o["momentum"]).max() <= 1e-9 * mv, (g["momentum"], o["momentum"])
+    scale = 0.5 * np.abs(o["field"]).sum()
+    assert abs(g["energy_elec"] - o["energy_elec"]) <= 1e-7 * scale, (g["energy_elec"],
+                                                                        o["energy_elec"])
+
+
 @pytest.mark.parametrize("method", ["MAZE-LCG", "LCG"])
 def test_md_sequence_cg(hip, oracle, method):
     s = W.nacl_system(512, 32, seed=1)
@@ -192,6 +205,65 @@ def test_md_sequence_cg(hip, oracle, method):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
     assert rel(g["q"], o["q"]) <= 1e-12
     assert rel(v_g, v_o) <= 1e-12
+    assert_observables(g, o, s)
+
+
+@pytest.mark.parametrize("precond", ["JACOBI", "MG"])
+def test_md_sequence_precond(hip, oracle, precond):
+    """MAZE-LCG with the reference's explicit preconditioners (library/src/plcg.c:104-166):
+    JACOBI (run here in the algebraically identical implicit form) and MG (one reference
+    two-grid V-cycle with 5 RBGS sweeps per side, started from the previous v).  The MG
+    form stagnates far above the tolerance in the reference too, so both sides stop at
+    the n^2 iteration cap (result -1); the 1024 iterations still agree to rounding."""
+    s = W.nacl_system(512, 32, seed=1)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s), precond=precond)
+    its_g, v_g, g = run(hip, s, cfg, 3)
+    its_o, v_o, o = run(oracle, s, cfg, 3)
+    if precond == "MG":
+        assert its_g == its_o == [-1, -1, -1], (its_g, its_o)
+    else:
+        assert all(b >= 0 and abs(a - b) <= 1 for a, b in zip(its_g, its_o)), (its_g, its_o)
+    for k in ("pos", "vel"):
+        assert rel(g[k], o[k]) <= 1e-11, k
+    for k in ("fcs_elec", "fcs_tot", "field", "field_prev"):
+        assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+    assert_observables(g, o, s)
+
+
+def test_verlet_thermostat_rescale_and_check(hip, oracle):
+    """Velocity Verlet with the rescaling thermostat on (library/src/integrator_verlet.c:
+    26-35), then solver_rescale_velocities (particles.c:841-857: each type's velocity sum
+    times 2 / n_p subtracted) and solver_check_thermostat (solver_md.c:210-222, which
+    switches the thermostat off within 100 K of the target), then one more step."""
+    s = W.nacl_system(216, 32, seed=3)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s), thermostat=True, T=1300.0)
+    out = []
+    for lib in (hip, oracle):
+        driver.initialize(lib, s, cfg)
+        driver.bootstrap(lib, cfg)
+        its = [driver.md_step(lib, cfg)[0] for _ in range(3)]
+        before = driver.snapshot(lib, s)
+        lib.solver_rescale_velocities()
+        after = driver.snapshot(lib, s, fields=False)
+        chk = lib.solver_check_thermostat()
+        its.append(driver.md_step(lib, cfg)[0])
+        last = driver.snapshot(lib, s)
+        lib.solver_finalize()
+        out.append((its, before, after, chk, last))
+    (ig, bg, ag, cg, lg), (io, bo, ao, co, lo) = out
+    assert all(abs(a - b) <= 1 for a, b in zip(ig, io)), (ig, io)
+    assert cg == co
+    assert rel(bg["vel"], bo["vel"]) <= 1e-11 and rel(bg["pos"], bo["pos"]) <= 1e-11
+    # The rescale's closed form from the state before it.
+    want = bg["vel"].copy()
+    for t in range(s.n_typ):
+        want[s.types == t] -= 2 * bg["vel"][s.types == t].sum(axis=0) / s.n_p
+    assert np.abs(ag["vel"] - want).max() <= 1e-13 * np.abs(want).max()
+    assert rel(ag["vel"], ao["vel"]) <= 1e-11
+    for k in ("pos", "vel"):
+        assert rel(lg[k], lo[k]) <= 1e-11, k
+    assert rel(lg["fcs_tot"], lo["fcs_tot"]) <= 1e-7
+    assert_observables(lg, lo, s)
 
 
 @pytest.mark.parametrize("method", ["MULTIGRID", "MAZE-MULTIGRID"])
@@ -203,6 +275,7 @@ def test_md_sequence_mg(hip, oracle, method):
     assert all(abs(a - b) <= 1 for a, b in zip(its_g, its_o)), (its_g, its_o)
     for k in ("fcs_elec", "fcs_tot", "field"):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
+    assert_observables(g, o, s)
 
 
 @pytest.mark.parametrize("method", ["MAZE-LCG", "LCG", "MULTIGRID", "MAZE-MULTIGRID"])
@@ -223,6 +296,7 @@ def test_md_sequence_pb(hip, oracle, method):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
     assert rel(g["pos"], o["pos"]) <= 1e-11
     assert rel(v_g, v_o) <= 1e-12
+    assert_observables(g, o, s)
 
 
 @pytest.mark.parametrize("cas", ["SPL_QUADR", "SPL_CUBIC"])
@@ -240,6 +314,7 @@ def test_md_sequence_spline(hip, oracle, cas):
     for k in ("fcs_elec", "fcs_tot", "field"):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
     assert rel(g["pos"], o["pos"]) <= 1e-11
+    assert_observables(g, o, s)
 
 
 def test_cic_bitexact_at_64(hip, oracle):
@@ -648,6 +723,7 @@ def test_md_sequence_pb_radii(hip, oracle, method):
     for k in ("fcs_elec", "fcs_db", "fcs_ib", "fcs_np", "fcs_tot", "field"):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
     assert rel(g["pos"], o["pos"]) <= 1e-11
+    assert_observables(g, o, s)
 
 
 @pytest.mark.parametrize("n", [32, 64, 128])
