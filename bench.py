@@ -47,8 +47,12 @@ BYTES_PER_POINT = {
 }
 
 
-def make_system():
-    s = W.nacl_system(N_IONS, N_GRID, seed=SEED)
+# The plain reference-method case (512 ions, 64^3 grid, seed 1): its FP64 GPU and CPU rows.
+C1_IONS, C1_GRID, C1_SEED = 512, 64, 1
+
+
+def make_system(n_ions=N_IONS, n_grid=N_GRID, seed=SEED):
+    s = W.nacl_system(n_ions, n_grid, seed=seed)
     return s, W.poisson_tol(s)
 
 
@@ -63,8 +67,8 @@ def prof_get(L, name):
     return t.value, c.value
 
 
-def start(lib, precision, tol_factor=1.0):
-    s, tol = make_system()
+def start(lib, precision, tol_factor=1.0, system=None):
+    s, tol = make_system(*(system or ()))
     cfg = driver.RunConfig(method="MAZE-LCG", tol=tol * tol_factor, integrator="VERLET")
     driver.initialize(lib, s, cfg)
     lib.solver_set_precision(precision)
@@ -72,14 +76,14 @@ def start(lib, precision, tol_factor=1.0):
     return s
 
 
-def run_path(lib, precision, steps, warmup, lat_steps, prof_steps):
+def run_path(lib, precision, steps, warmup, lat_steps, prof_steps, system=None):
     """Bootstrap and warm up, then three phases on the same trajectory:
     1. `steps` MD steps back to back between two device synchronisations (throughput);
     2. `lat_steps` steps each followed by a synchronisation (step latency p50/p99);
     3. `prof_steps` steps with HIP events around the timed kernels (per-kernel
        average durations; the events add gaps, so this phase is not timed)."""
     L = lib.lib
-    start(lib, precision)
+    start(lib, precision, system=system)
     for _ in range(warmup):
         lib.solver_md_loop_iter()
     lib.solver_sync()
@@ -272,40 +276,95 @@ def rms_rel(a, b):
     return float(np.sqrt(((a - b) ** 2).mean()) / np.sqrt((b ** 2).mean()))
 
 
-def cpu_baseline_child(steps):
+# The reference's phase clocks (maze_poisson/solver.py:411-481): integrator (both
+# halves), charges, field, forces (field + non-electrostatic + total).
+CPU_PHASES = ("integrator", "charges", "field", "forces_field", "forces_notelec", "forces")
+
+
+def cpu_baseline_child(steps, n_ions, n_grid, seed):
     """The reference C (compiled from its own sources with its own flags, OpenMP)
-    on the host: bootstrap, one untimed step, then `steps` timed MD steps."""
+    on the host: bootstrap, one untimed step, then `steps` timed MD steps through the
+    reference driver's call sequence, each phase timed on its own."""
     from maze_poisson_amd.capi import MazeLib
     lib = MazeLib(os.path.join(ROOT, "oracle", "_ref", "libmaze_ref_fast.so"), extensions=False)
-    s, tol = make_system()
+    s, tol = make_system(n_ions, n_grid, seed)
     driver.initialize(lib, s, driver.RunConfig(method="MAZE-LCG", tol=tol, integrator="VERLET"))
     driver.bootstrap(lib)
     driver.md_step(lib)
-    t0 = time.perf_counter()
+    pc = time.perf_counter
+    ph = {k: 0.0 for k in CPU_PHASES}
+    lat = []
+    t_start = pc()
     for _ in range(steps):
-        driver.md_step(lib)
-    el = time.perf_counter() - t0
-    print(json.dumps({"steps_per_s": steps / el, "elapsed": el}))
+        t0 = pc()
+        lib.integrator_part_1()
+        t1 = pc()
+        lib.solver_update_charges()
+        t2 = pc()
+        lib.solver_update_field()
+        t3 = pc()
+        lib.solver_compute_forces_elec()
+        t4 = pc()
+        lib.solver_compute_forces_noel()
+        t5 = pc()
+        lib.solver_compute_forces_tot()
+        t6 = pc()
+        lib.integrator_part_2()
+        t7 = pc()
+        ph["integrator"] += (t1 - t0) + (t7 - t6)
+        ph["charges"] += t2 - t1
+        ph["field"] += t3 - t2
+        ph["forces_field"] += t4 - t3
+        ph["forces_notelec"] += t5 - t4
+        ph["forces"] += t6 - t3
+        lat.append(t7 - t0)
+    el = pc() - t_start
+    lat = np.array(lat) * 1e3
+    print(json.dumps({"steps_per_s": steps / el, "elapsed": el,
+                      "p50_ms": float(np.percentile(lat, 50)),
+                      "p99_ms": float(np.percentile(lat, 99)),
+                      "phases_ms_per_step": {k: v / steps * 1e3 for k, v in ph.items()}}))
 
 
-def cpu_baseline(steps, cores):
+def host_cores():
+    """Cores this process may run on (its CPU affinity)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_run(steps, cores, system=(N_IONS, N_GRID, SEED)):
+    """One reference CPU measurement in a child process (OMP_NUM_THREADS = cores)."""
     path = os.path.join(ROOT, "oracle", "_ref", "libmaze_ref_fast.so")
     if not os.path.exists(path):
         return None
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    n_ions, n_grid, seed = system
     try:
         out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child",
-                              "--cpu-steps", str(steps)], env=env, capture_output=True,
-                             timeout=300, text=True)
+                              "--cpu-steps", str(steps), "--cpu-system", str(n_ions), str(n_grid),
+                              str(seed)], env=env, capture_output=True, timeout=300, text=True)
         r = json.loads([ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1])
     except Exception as e:  # a reported baseline, never required
         sys.stderr.write(f"cpu baseline failed: {e}\n")
         return None
-    return {"value": round(r["steps_per_s"], 4), "unit": "MD steps/s", "cores": cores,
+    return {"steps_per_s": round(r["steps_per_s"], 4), "cores": cores, "steps": steps,
+            "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3),
+            "phases_ms_per_step": {k: round(v, 3) for k, v in r["phases_ms_per_step"].items()}}
+
+
+def cpu_baseline(steps, cores):
+    r = cpu_run(steps, cores)
+    if r is None:
+        return None, None
+    line = {"value": r["steps_per_s"], "unit": "MD steps/s", "cores": cores,
             "kind": "reference",
             "sample": f"reference C compiled from its sources (-O3 -ffast-math -mavx2, OpenMP "
-                      f"x{cores}) on the same 4096-ion / 128^3 workload: {steps} MD steps "
-                      f"after the 2 bootstrap steps and 1 untimed step"}
+                      f"x{cores}, all cores of this process's affinity) on the same 4096-ion / "
+                      f"128^3 workload: {steps} MD steps after the 2 bootstrap steps and 1 "
+                      f"untimed step"}
+    return line, r
 
 
 def ensemble_rate(lib, replicas, steps, slots, graphs=True):
@@ -419,8 +478,11 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=30,
                     help="steps run with per-kernel HIP events after the timed steps")
     ap.add_argument("--cpu-steps", type=int, default=15)
-    ap.add_argument("--cpu-cores", type=int, default=16)
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="threads of the CPU baseline (default: the cores of this process)")
     ap.add_argument("--cpu-child", action="store_true")
+    ap.add_argument("--cpu-system", type=int, nargs=3, default=[N_IONS, N_GRID, SEED],
+                    metavar=("IONS", "GRID", "SEED"), help=argparse.SUPPRESS)
     ap.add_argument("--pb-radii-only", action="store_true",
                     help="only the radius-based PB step (8000 ions, 160^3); prints its JSON")
     ap.add_argument("--pb-synthetic-only", action="store_true",
@@ -442,7 +504,7 @@ def main():
                     help="launch one 256 MiB copy kernel first (byte-counter calibration)")
     args = ap.parse_args()
     if args.cpu_child:
-        cpu_baseline_child(args.cpu_steps)
+        cpu_baseline_child(args.cpu_steps, *args.cpu_system)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; a multi-GPU run would shard independent "
@@ -504,7 +566,21 @@ def main():
             "solver_iters_per_step": iters,
             "fp64_roofline": roofline(fp64["prof"]),
         }
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps, args.cpu_cores)
+        c1 = run_path(lib, PATHS["fp64"], 100, args.warmup, 100, 0,
+                      system=(C1_IONS, C1_GRID, C1_SEED))
+        line_extra["fp64_512_ions_64_grid"] = {
+            "steps_per_s": round(c1["steps_per_s"], 2), "p50_ms": round(c1["p50_ms"], 4),
+            "p99_ms": round(c1["p99_ms"], 4)}
+    cores = args.cpu_cores or host_cores()
+    cpu, cpu_detail = (None, None) if args.no_cpu else cpu_baseline(args.cpu_steps, cores)
+    if cpu_detail is not None:
+        # The plain reference-method case (512 ions / 64^3) on the host with all cores and
+        # with one thread, beside its FP64 GPU row.
+        c1 = (C1_IONS, C1_GRID, C1_SEED)
+        cpu_detail = {"4096_ions_128_grid": cpu_detail,
+                      "512_ions_64_grid": cpu_run(20, cores, c1),
+                      "512_ions_64_grid_one_thread": cpu_run(10, 1, c1)}
+        line_extra["cpu_baseline_detail"] = cpu_detail
     line = {
         "metric": "MD steps/s at 4096 ions/128^3 grid, FP32+refine vs FP64; p50/p99 step latency",
         "value": round(head["steps_per_s"], 3),

@@ -77,7 +77,8 @@ static int precision_for(int n, int want) {
 
 // Device bytes a replica carves from its arena: the Sim's reduction buffers
 // and solver state, the grids, the multigrid levels, the particle arrays, the
-// Tosi-Fumi partials and the sort scratch (each allocation 256-byte aligned).
+// Tosi-Fumi partials and the charge-spread lists (each allocation 256-byte
+// aligned).
 static long long arena_bytes(int n, int np, int precision) {
     const long long G = (long long)n * n * n;
     long long b = 4LL * 8 * (1 << 15) + (64 << 10);  // partials of both streams, scalars
@@ -95,13 +96,11 @@ static long long arena_bytes(int n, int np, int precision) {
             if (c > kMgCoarseLds) b += 2 * 8 * c * c * c;  // its work arrays
         }
     }
-    unsigned long cells = (unsigned long)n * n * n;
-    int key_bits = 1;
-    while ((1ul << key_bits) < cells) key_bits++;
-    b += (long long)np * (4 + 5 * 24 + 2 * 8 + 12 + 5 * 4 + 8 * 8 + 8);
-    b += 4LL * ((long long)n * n + 1);
+    // types, pos/vel/forces, mass/charges, base cell, pair counts, the CIC
+    // contributions with their node and link entries, the pair energies
+    b += (long long)np * (4 + 5 * 24 + 2 * 8 + 12 + 4 + 8 * (8 + 4 + 4) + 8);
+    b += 8 * G;                                           // spread list heads
     b += 8LL * tf_part_doubles(np) + 7LL * 8 * 16 * 16;  // pair table, up to 16 types
-    b += (long long)sort_tmp_bytes(std::max(np, 1), key_bits);
     return b + 64LL * (long long)kArenaAlign;  // alignment of up to 64 allocations
 }
 

@@ -903,12 +903,101 @@ __global__ __launch_bounds__(4 * kMgCoarseLds * kMgCoarseLds) void k_mg_coarse(
     for (int x = tid; x < N; x += NT) e[x] = (float)A[(x / c) * pitch + x % c];
 }
 
+// The same solve with each axis pass as a matrix product on the matrix cores:
+// a pass maps the c^2 lines of the level (rows) through the c x c eigenbasis,
+// out[line][t] = sum_m in[line][m] M[m][t], as 16-line tiles of
+// v_mfma_f32_16x16x4_f32 (four K-steps of 4 cover m < 16).  Wave w owns
+// lines 16 w .. 16 w + 15 in every pass; each lane keeps the eigenbasis
+// operands of both directions in registers for all six passes.  Operand
+// layout of the 16x16x4 f32 instruction (wave64): A[i][k] from lane
+// i + 16 k, B[k][j] from lane j + 16 k, and result register r of lane
+// j + 16 q is D[4 q + r][j].  Values past c (padding) enter as exact zeros.
+// FP32 products with FP32 accumulation, as the VALU form.
+typedef float mg_v4f __attribute__((ext_vector_type(4)));
+constexpr int kMgCoarseMfmaThreads = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds / 4;
+
+__global__ __launch_bounds__(kMgCoarseMfmaThreads) void k_mg_coarse_mfma(
+    int c, const float* __restrict__ b, float* __restrict__ e, const double* __restrict__ Qf,
+    const double* __restrict__ ilam, const CGState* st) {
+    if (st->done) return;
+    constexpr int CM = kMgCoarseLds, NT = kMgCoarseMfmaThreads, PER = CM * CM * CM / NT;
+    static_assert(CM == 16, "one 16-line tile per wave with 16-wide operands");
+    __shared__ float A[CM * CM * (CM + 1)], B[CM * CM * (CM + 1)];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c2 = c * c, N = c2 * c, pitch = c + 1;
+    float il[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) il[k] = tid + k * NT < N ? (float)ilam[tid + k * NT] : 0.0f;
+    // B operands: forward M[m][t] = Q[t][m], inverse M[m][t] = Q[m][t], with
+    // m = 4 kk + lane / 16 and t = lane % 16.
+    float qf[4], qi[4];
+    const int col = lane & 15, krow = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+        const int m = 4 * kk + krow;
+        const bool in = m < c && col < c;
+        qf[kk] = in ? (float)Qf[col * c + m] : 0.0f;
+        qi[kk] = in ? (float)Qf[m * c + col] : 0.0f;
+    }
+    for (int x = tid; x < N; x += NT) A[(x / c) * pitch + x % c] = b[x];
+    lds_barrier();
+    // axis 0: lines along k at (i, j) = (u, v); 1: along j at (i, k); 2: along i at (j, k).
+    auto line_base = [&](int L, int axis) {
+        const int u = L / c, v = L - (L / c) * c;
+        return axis == 0 ? (u * c + v) * pitch : (axis == 1 ? u * c * pitch + v : u * pitch + v);
+    };
+    auto pass = [&](const float* src, float* dst, int axis, bool inv) {
+        const int stride = axis == 0 ? 1 : (axis == 1 ? pitch : c * pitch);
+        const int L0 = 16 * wave;
+        if (L0 < c2) {
+            const int La = L0 + col;  // A row of this lane
+            const int ba = La < c2 ? line_base(La, axis) : 0;
+            mg_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int m = 4 * kk + krow;
+                const float a = (La < c2 && m < c) ? src[ba + m * stride] : 0.0f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, inv ? qi[kk] : qf[kk], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int Ld = L0 + 4 * krow + r;  // D row of register r
+                if (Ld < c2 && col < c) dst[line_base(Ld, axis) + col * stride] = acc[r];
+            }
+        }
+        lds_barrier();
+    };
+    pass(A, B, 0, false);
+    pass(B, A, 1, false);
+    pass(A, B, 2, false);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int x = tid + k * NT;
+        if (x < N) B[(x / c) * pitch + x % c] *= il[k];
+    }
+    lds_barrier();
+    pass(B, A, 2, true);
+    pass(A, B, 1, true);
+    pass(B, A, 0, true);
+    for (int x = tid; x < N; x += NT) e[x] = A[(x / c) * pitch + x % c];
+}
+
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st) {
     if (c > kMgCoarseLds) {
         std::fprintf(stderr, "maze: coarsest level %d above %d for the one-work-group solve\n", c,
                      kMgCoarseLds);
         std::exit(1);
+    }
+    // MAZE_MG_COARSE_VALU=1 selects the vector-ALU form (the same transform).
+    static const bool valu = [] {
+        const char* v = std::getenv("MAZE_MG_COARSE_VALU");
+        return v && std::atoi(v) != 0;
+    }();
+    if (!valu) {
+        MAZE_LAUNCH(k_mg_coarse_mfma, dim3(1), dim3(kMgCoarseMfmaThreads), 0, s, c, b, e, Qf, ilam,
+                    st);
+        return;
     }
     const dim3 grid(1), block(4 * kMgCoarseLds * kMgCoarseLds);
     if (c == kMgCoarseLds)

@@ -5,7 +5,6 @@
 // :132-183; library/src/integrator_verlet.c:37-70;
 // library/src/integrator_ovrvo.c:45-123) so that per-ion results match the
 // CPU oracle bit for bit wherever no transcendental function is involved.
-#include <hipcub/hipcub.hpp>
 
 #include "particles.hpp"
 
@@ -26,61 +25,35 @@ static int blocks_for(long n) {
     return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
 }
 
-// Base cell floor(pos/h), wrapped into [0, n) (library/src/particles.c:315-321),
-// and its linear key for the cell sort.
+// One thread per item, no grid-stride loop: every item gets its own thread.
+static int blocks_exact(long n) {
+    const long b = (n + kBlock - 1) / kBlock;
+    if (b > 0x7fffffffL) {
+        std::fprintf(stderr, "maze: %ld items exceed one launch\n", n);
+        std::exit(1);
+    }
+    return (int)(b < 1 ? 1 : b);
+}
+
+// Base cell floor(pos/h), wrapped into [0, n) (library/src/particles.c:315-321).
 __global__ void k_cic_base(int np, int n, double h, const double* __restrict__ pos,
-                           int* __restrict__ base, unsigned int* __restrict__ keys,
-                           int* __restrict__ ids) {
+                           int* __restrict__ base) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    int b[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         int c = (int)floor(pos[3 * i + d] / h);
         c %= n;
         if (c < 0) c += n;
-        b[d] = c;
         base[3 * i + d] = c;
     }
-    keys[i] = ((unsigned int)b[0] * n + b[1]) * n + b[2];
-    ids[i] = i;
 }
 
 void launch_cic_base(hipStream_t s, const PDev& p) {
     if (p.np == 0) return;
-    MAZE_LAUNCH(k_cic_base, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h,
-                       p.pos, p.base, p.keys, p.ids);
+    MAZE_LAUNCH(k_cic_base, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
+                p.base);
 }
-
-size_t sort_tmp_bytes(int np, int key_bits) {
-    size_t bytes = 0;
-    MAZE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (unsigned int*)nullptr,
-                                                (unsigned int*)nullptr, (int*)nullptr,
-                                                (int*)nullptr, np, 0, key_bits));
-    return bytes;
-}
-
-// row_start[r] = first sorted ion whose cell lies in row r = i n + j (keys of
-// row r are [r n, (r + 1) n)); row_start[n^2] = np.
-__global__ void k_row_start(int np, int n, const unsigned int* __restrict__ keys_sorted,
-                            int* __restrict__ row_start) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r > n * n) return;
-    row_start[r] = lower_bound_u32(keys_sorted, np, (unsigned int)r * (unsigned int)n);
-}
-
-// Stable radix sort of (cell key, ion id): ions of one cell stay in id order;
-// then the start of every cell row in the sorted order.
-void launch_cic_sort(hipStream_t s, const PDev& p, int key_bits) {
-    if (p.np == 0) return;
-    size_t bytes = p.sort_tmp_bytes;
-    MAZE_HIP(hipcub::DeviceRadixSort::SortPairs(p.sort_tmp, bytes, p.keys, p.keys_sorted, p.ids,
-                                                p.ids_sorted, p.np, 0, key_bits, s));
-    MAZE_LAUNCH(k_row_start, dim3(blocks_for((long)p.n * p.n + 1)), dim3(kBlock), 0, s, p.np,
-                p.n, p.keys_sorted, p.row_start);
-}
-
-
 
 // Quadratic and cubic B-spline weights (library/src/charges.c:16-38).
 __device__ __forceinline__ double spline_quadr_w(double x, double L, double h) {
@@ -138,131 +111,173 @@ __device__ __forceinline__ double assign_contrib(const double* pos, const double
     return charges[j] * app;
 }
 
-// Owner-computes gather: the thread of (ion i, node c) finds every ion whose
-// stencil covers node g = base_i + offset_c: ions whose base cell lies in the
-// kSpan^3 block g - offsets, looked up row by row ((i, j) fixed, k along the
-// row) in the sorted cell keys.  Only the lowest ion id writes q[g], summing
-// the contributions in increasing ion id -- the order of the reference's
-// serial scatter -- so no float atomics are needed and q matches the
-// reference's serial result.
+// The charge spread without float atomics, in two passes over the (ion i,
+// node c) contributions t = i nn + c -- the order of the reference's serial
+// scatter, ion by ion and node by node (library/src/charges.c:123-141).
+//
+// link: thread t computes its contribution, clears the node q held from the
+// previous spread (zero_old), and pushes itself onto its node's list with
+// one 64-bit exchange (the tag is the spread's epoch, so lists of earlier
+// spreads read as empty without clearing the head array).
+// own:  the thread at the head of a node's list -- exactly one per touched
+// node -- walks the list, sums the contributions in increasing t starting
+// from 0.0 as the reference's += does, and writes q once.  The exchange
+// order only decides which thread sums; the sum itself is in t order, so q
+// equals the reference's serial result bit for bit.  Lists are short (a CIC
+// node is reached from the 8 cells around it, about 0.02 ions at melt
+// density); the selection below costs (length)^2 loads.
+__device__ __forceinline__ unsigned long long spread_tag(const PScalars* ps) {
+    return (unsigned long long)(ps->epoch + 1u) << 32;
+}
+
 template <int CAS>
-__global__ void k_assign_gather(int np, int n, double h, const double* __restrict__ pos,
-                                const double* __restrict__ charges, const int* __restrict__ base,
-                                const unsigned int* __restrict__ keys_sorted,
-                                const int* __restrict__ ids_sorted,
-                                const int* __restrict__ row_start, double* __restrict__ q,
-                                double* __restrict__ contrib) {
+__global__ void k_assign_link(int np, int n, double h, const double* __restrict__ pos,
+                              const double* __restrict__ charges, const int* __restrict__ base,
+                              const PScalars* __restrict__ ps, double* __restrict__ contrib,
+                              int* __restrict__ node, int* __restrict__ link,
+                              unsigned long long* __restrict__ head, double* __restrict__ q,
+                              int zero_old) {
     using A = Assign<CAS>;
-    constexpr int NN = A::kNodes, S = A::kSpan;
+    constexpr int NN = A::kNodes;
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= (long)NN * np) return;
     const int i = (int)(t / NN), c = (int)(t % NN);
     const double L = n * h;
-    // Indices stay within one period of [0, n): wrap by one conditional step.
     auto wrap = [n](int v) { return v < 0 ? v + n : (v >= n ? v - n : v); };
     const int gi = wrap(base[3 * i] + A::off(c, 0));
     const int gj = wrap(base[3 * i + 1] + A::off(c, 1));
     const int gk = wrap(base[3 * i + 2] + A::off(c, 2));
-    const double mine = assign_contrib<A>(pos, charges, i, gi, gj, gk, L, h);
-    contrib[t] = mine;
+    contrib[t] = assign_contrib<A>(pos, charges, i, gi, gj, gk, L, h);
+    const int g = (gi * n + gj) * n + gk;
+    if (zero_old) {
+        const int old = node[t];
+        q[old] = 0.0;
+    }
+    node[t] = g;
+    const unsigned long long tag = spread_tag(ps);
+    const unsigned long long prev = atomicExch(&head[g], tag | (unsigned int)t);
+    link[t] = (prev & 0xffffffff00000000ull) == tag ? (int)(unsigned int)prev : -1;
+}
 
-    // Rows (ci, cj) of candidate base cells and their ranges in the sorted keys.
-    int lo[S * S], hi[S * S];
-    unsigned int rows[S * S];
-    // Whether a base cell at k index ck covers node gk along k (node - base in
-    // [kLo, kLo + S), periodic).
-    auto covers_k = [&](int ck) { return wrap(wrap(gk - ck) - A::kLo) < S; };
-    int owner = i;
-    for (int a = 0; a < S; a++)
-        for (int b = 0; b < S; b++) {
-            const int ci = wrap(gi - (A::kLo + a));
-            const int cj = wrap(gj - (A::kLo + b));
-            const int ri = ci * n + cj;
-            const unsigned int row = (unsigned int)ri * n;
-            const int r0 = row_start[ri];
-            const int r1 = row_start[ri + 1];
-            lo[a * S + b] = r0;
-            hi[a * S + b] = r1;
-            rows[a * S + b] = row;
-            for (int s = r0; s < r1; s++)
-                if (covers_k((int)(keys_sorted[s] - row))) owner = min(owner, ids_sorted[s]);
-        }
-    if (owner != i) return;
-    // Sum contributions in increasing ion id (selection over the candidates).
+__global__ void k_assign_own(long total, const PScalars* __restrict__ ps,
+                             const double* __restrict__ contrib, const int* __restrict__ node,
+                             const int* __restrict__ link,
+                             const unsigned long long* __restrict__ head, double* __restrict__ q) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int g = node[t];
+    if (head[g] != (spread_tag(ps) | (unsigned int)t)) return;
     double acc = 0.0;
     int last = -1;
     for (;;) {
         int next = 0x7fffffff;
-        for (int r = 0; r < S * S; r++)
-            for (int s = lo[r]; s < hi[r]; s++) {
-                if (!covers_k((int)(keys_sorted[s] - rows[r]))) continue;
-                const int j = ids_sorted[s];
-                if (j > last && j < next) next = j;
-            }
+        for (int u = (int)t; u >= 0; u = link[u])
+            if (u > last && u < next) next = u;
         if (next == 0x7fffffff) break;
-        acc += next == i ? mine : assign_contrib<A>(pos, charges, next, gi, gj, gk, L, h);
+        acc += contrib[next];
         last = next;
     }
-    q[((long)gi * n + gj) * n + gk] = acc;
+    q[g] = acc;
 }
 
-void launch_cic_gather(hipStream_t s, const PDev& p, double* q) {
+void launch_assign(hipStream_t s, const PDev& p, const PScalars* ps, double* q, int zero_old) {
     if (p.np == 0) return;
-    const int nb = blocks_for((long)p.nn * p.np);
+    const long total = (long)p.nn * p.np;
+    const int nb = blocks_exact(total);
     if (p.cas == 0)
-        MAZE_LAUNCH(k_assign_gather<0>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
-                    p.charges, p.base, p.keys_sorted, p.ids_sorted, p.row_start, q,
-                    p.contrib);
+        MAZE_LAUNCH(k_assign_link<0>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
+                    p.charges, p.base, ps, p.contrib, p.node, p.link, p.head, q, zero_old);
     else if (p.cas == 1)
-        MAZE_LAUNCH(k_assign_gather<1>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
-                    p.charges, p.base, p.keys_sorted, p.ids_sorted, p.row_start, q,
-                    p.contrib);
+        MAZE_LAUNCH(k_assign_link<1>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
+                    p.charges, p.base, ps, p.contrib, p.node, p.link, p.head, q, zero_old);
     else
-        MAZE_LAUNCH(k_assign_gather<2>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
-                    p.charges, p.base, p.keys_sorted, p.ids_sorted, p.row_start, q,
-                    p.contrib);
+        MAZE_LAUNCH(k_assign_link<2>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, p.pos,
+                    p.charges, p.base, ps, p.contrib, p.node, p.link, p.head, q, zero_old);
+    MAZE_LAUNCH(k_assign_own, dim3(nb), dim3(kBlock), 0, s, total, ps, p.contrib, p.node, p.link,
+                p.head, q);
+}
+
+// Sum of the contributions (the reference's charge-conservation total) and
+// the end of the spread's epoch.
+__global__ void k_contrib_sum(long total, const double* __restrict__ v, Reduce red,
+                              PScalars* ps) {
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x)
+        s += v[i];
+    __shared__ double sh[kBlock / kWave];
+    const double S = block_sum(s, sh);
+    if (publish_arrive(red.psum, S, nullptr, 0.0, red.counter, gridDim.x)) {
+        const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            ps->q_sum = T;
+            ps->epoch += 1u;
+            reset_counter(red.counter);
+        }
+    }
 }
 
 void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) {
-    launch_sum_scaled(s, (long)p.nn * p.np, 1.0, p.contrib, &ps->q_sum, red);
+    const long total = (long)p.nn * p.np;
+    long b = (total + 4L * kBlock - 1) / (4L * kBlock);
+    b = b < 1 ? 1 : (b > 2048 ? 2048 : b);
+    MAZE_LAUNCH(k_contrib_sum, dim3((int)b), dim3(kBlock), 0, s, total, p.contrib, red, ps);
 }
 
 // Field force -q w grad(phi) by central differences at the assignment nodes
-// (library/src/forces.c:52-94).
+// (library/src/forces.c:52-94).  One thread per (ion, node): each forms its
+// node's three terms q w E, and the node-0 thread of the ion adds them in node
+// order starting from 0.0 -- the reference's loop order, so the force matches
+// its serial sum bit for bit.  A 256-thread work-group holds 256 / nn whole
+// ions (nn = 8 or 64 divides 256).
 template <int CAS>
-__global__ void k_force_fd(int np, int n, double h, const double* __restrict__ phi,
-                           const int* __restrict__ base, const double* __restrict__ charges,
-                           const double* __restrict__ pos, double* __restrict__ forces) {
+__global__ __launch_bounds__(kBlock) void k_force_fd(int np, int n, double h,
+                                                     const double* __restrict__ phi,
+                                                     const int* __restrict__ base,
+                                                     const double* __restrict__ charges,
+                                                     const double* __restrict__ pos,
+                                                     double* __restrict__ forces) {
     using A = Assign<CAS>;
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= np) return;
-    const long N = n, N2 = N * N;
-    const double h2 = 2.0 * h, L = N * h;
-    const double px = pos[3 * ip], py = pos[3 * ip + 1], pz = pos[3 * ip + 2], chg = charges[ip];
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int c = 0; c < A::kNodes; c++) {
+    constexpr int NN = A::kNodes;
+    static_assert(kBlock % NN == 0, "whole ions per work-group");
+    __shared__ double term[3][kBlock];
+    const long t = blockIdx.x * (long)kBlock + threadIdx.x;
+    const int ip = (int)(t / NN), c = (int)(t % NN);
+    if (ip < np) {
+        const long N = n, N2 = N * N;
+        const double h2 = 2.0 * h, L = N * h;
         const long i = (base[3 * ip] + A::off(c, 0) + n) % n;
         const long j = (base[3 * ip + 1] + A::off(c, 1) + n) % n;
         const long k = (base[3 * ip + 2] + A::off(c, 2) + n) % n;
-        const double qc = chg * A::w(px - i * h, L, h) * A::w(py - j * h, L, h) * A::w(pz - k * h, L, h);
         const long im = (i == 0 ? N - 1 : i - 1), ii = (i == N - 1 ? 0 : i + 1);
         const long jm = (j == 0 ? N - 1 : j - 1), jj = (j == N - 1 ? 0 : j + 1);
         const long km = (k == 0 ? N - 1 : k - 1), kk = (k == N - 1 ? 0 : k + 1);
-        double E = (phi[im * N2 + j * N + k] - phi[ii * N2 + j * N + k]) / h2;
-        f0 += qc * E;
-        E = (phi[i * N2 + jm * N + k] - phi[i * N2 + jj * N + k]) / h2;
-        f1 += qc * E;
-        E = (phi[i * N2 + j * N + km] - phi[i * N2 + j * N + kk]) / h2;
-        f2 += qc * E;
+        const double pxm = phi[im * N2 + j * N + k], pxp = phi[ii * N2 + j * N + k];
+        const double pym = phi[i * N2 + jm * N + k], pyp = phi[i * N2 + jj * N + k];
+        const double pzm = phi[i * N2 + j * N + km], pzp = phi[i * N2 + j * N + kk];
+        const double qc = charges[ip] * A::w(pos[3 * ip] - i * h, L, h) *
+                          A::w(pos[3 * ip + 1] - j * h, L, h) * A::w(pos[3 * ip + 2] - k * h, L, h);
+        term[0][threadIdx.x] = qc * ((pxm - pxp) / h2);
+        term[1][threadIdx.x] = qc * ((pym - pyp) / h2);
+        term[2][threadIdx.x] = qc * ((pzm - pzp) / h2);
     }
-    forces[3 * ip] = f0;
-    forces[3 * ip + 1] = f1;
-    forces[3 * ip + 2] = f2;
+    __syncthreads();
+    if (ip < np && c == 0) {
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+        for (int u = 0; u < NN; u++) {
+            f0 += term[0][threadIdx.x + u];
+            f1 += term[1][threadIdx.x + u];
+            f2 += term[2][threadIdx.x + u];
+        }
+        forces[3 * ip] = f0;
+        forces[3 * ip + 1] = f1;
+        forces[3 * ip + 2] = f2;
+    }
 }
 
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi) {
     if (p.np == 0) return;
-    const int nb = blocks_for(p.np);
+    const int nb = blocks_exact((long)p.nn * p.np);
     if (p.cas == 0)
         MAZE_LAUNCH(k_force_fd<0>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi, p.base,
                     p.charges, p.pos, p.fe);

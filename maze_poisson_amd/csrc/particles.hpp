@@ -19,18 +19,18 @@ struct PDev {
     double* fn;       // Tosi-Fumi force (np, 3)
     double* ft;       // total force (np, 3)
     int* base;        // CIC base cell (np, 3), wrapped into [0, n)
-    unsigned int* keys;
-    unsigned int* keys_sorted;
-    int* ids;
-    int* ids_sorted;
-    int* row_start;   // (n^2 + 1) first sorted ion of each cell row (i, j)
+    // Charge spread: per grid node the head of a list of the (ion, node)
+    // contributions it receives, tagged with the spread's epoch (high 32 bits;
+    // entries of an older epoch count as empty, so the array is never cleared);
+    // per contribution its grid node and the next entry of its node's list.
+    unsigned long long* head;  // (n^3)
+    int* node;        // (np, nn)
+    int* link;        // (np, nn)
     double* table;    // Tosi-Fumi constants per type pair, 7 each
     double* tpart;    // Tosi-Fumi partials per 64-ion block pair (tf_part_doubles)
     double* eion;     // Tosi-Fumi energy per ion (each pair's energy on both ions)
     int* counts;      // pairs within the cutoff per ion
     double* contrib;  // per-(ion, node) assignment contribution, (np, nn)
-    void* sort_tmp;
-    size_t sort_tmp_bytes;
 };
 
 // Device-resident scalars of the particle side.
@@ -42,12 +42,14 @@ struct PScalars {
     double mom[3];
     double typesum[3 * 16];
     int pb_flag;       // set by the PB map kernel when a tile overflows
+    unsigned int epoch;  // charge spreads completed (tags the node lists)
 };
 
-size_t sort_tmp_bytes(int np, int key_bits);
 void launch_cic_base(hipStream_t s, const PDev& p);
-void launch_cic_sort(hipStream_t s, const PDev& p, int key_bits);
-void launch_cic_gather(hipStream_t s, const PDev& p, double* q);
+// The charge spread onto q (q must hold zeros except at the nodes of the
+// previous spread, which zero_old clears; without zero_old q must be all zero).
+void launch_assign(hipStream_t s, const PDev& p, const PScalars* ps, double* q, int zero_old);
+// Sum of all contributions into ps->q_sum; advances ps->epoch.
 void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi);
 long tf_part_doubles(int np);

@@ -189,13 +189,10 @@ void Sim::free_particles() {
                        &p.tpart, &p.eion, &p.contrib})
         dfree(*a);
     dfree(p.base);
-    dfree(p.keys);
-    dfree(p.keys_sorted);
-    dfree(p.ids);
-    dfree(p.ids_sorted);
-    dfree(p.row_start);
+    dfree(p.head);
+    dfree(p.node);
+    dfree(p.link);
     dfree(p.counts);
-    dfree(p.sort_tmp);
     p.np = 0;
     free_pb();
 }
@@ -237,6 +234,7 @@ void Sim::init_grid(int n_, double L_, double h_, double tol_, double eps, doubl
     dims = grid_dims_for(n);
     if (dims.blocks > kMaxPartials) fatal("maze: grid too large for the reduction buffers");
     q = dalloc<double>(G); y = dalloc<double>(G); phi = dalloc<double>(G);
+    spread_zero_old = 0;
     phip = dalloc<double>(G); tmp = dalloc<double>(G); sigma = dalloc<double>(G);
     r = dalloc<double>(G); p0 = dalloc<double>(G); p1 = dalloc<double>(G);
     if (type == kMG || type == kMazeMG || (type == kMazeLCG && precond_ == kPMG)) {
@@ -294,10 +292,11 @@ void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, int ca
     p.mass = dalloc<double>(np); p.charges = dalloc<double>(np);
     p.fe = dalloc<double>(3L * np); p.fn = dalloc<double>(3L * np); p.ft = dalloc<double>(3L * np);
     p.base = dalloc<int>(3L * np);
-    p.keys = dalloc<unsigned int>(np); p.keys_sorted = dalloc<unsigned int>(np);
-    p.ids = dalloc<int>(np); p.ids_sorted = dalloc<int>(np);
-    p.row_start = dalloc<int>((long)n_ * n_ + 1);
+    p.head = dalloc<unsigned long long>((long)n_ * n_ * n_);
+    p.node = dalloc<int>((long)p.nn * np);
+    p.link = dalloc<int>((long)p.nn * np);
     p.contrib = dalloc<double>((long)p.nn * np);
+    spread_zero_old = 0;
     p.tpart = dalloc<double>(tf_part_doubles(np));
     p.eion = dalloc<double>(np);
     p.counts = dalloc<int>(np);
@@ -324,11 +323,6 @@ void Sim::init_particles(int n_, int n_typ, double L_, double h_, int np, int ca
         MAZE_HIP(hipMemcpy(p.mass, mass, np * sizeof(double), hipMemcpyHostToDevice));
         MAZE_HIP(hipMemcpy(p.charges, charges, np * sizeof(double), hipMemcpyHostToDevice));
     }
-    const unsigned long cells = (unsigned long)n_ * n_ * n_;
-    key_bits = 1;
-    while ((1ul << key_bits) < cells) key_bits++;
-    p.sort_tmp_bytes = sort_tmp_bytes(std::max(np, 1), key_bits);
-    p.sort_tmp = dalloc<char>((long)std::max<size_t>(p.sort_tmp_bytes, 16));
     sync();
 }
 
@@ -1241,9 +1235,12 @@ int Sim::mg_update(bool maze) {
 // is made by check_charge() after the next read_state() instead of here.
 int Sim::update_charges(bool defer) {
     launch_cic_base(stream, p);
-    launch_cic_sort(stream, p, key_bits);
-    launch_fill(stream, G, 0.0, q);
-    launch_cic_gather(stream, p, q);
+    // q holds the previous spread, which the spread clears node by node; after
+    // a new grid or particle set, clear it whole once.
+    if (p.np > 0 && p.n != n) fatal("maze: the particles' grid size differs from the field grid");
+    if (!spread_zero_old) launch_fill(stream, G, 0.0, q);
+    launch_assign(stream, p, ps, q, spread_zero_old);
+    spread_zero_old = p.np > 0 ? 1 : 0;
     launch_contrib_sum(stream, p, ps, red);
     if (defer) return 0;  // read_state() brings q_sum with the solver state
     MAZE_HIP(hipMemcpyAsync(&ps_host->q_sum, &ps->q_sum, sizeof(double), hipMemcpyDeviceToHost,

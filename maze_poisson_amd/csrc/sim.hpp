@@ -204,7 +204,9 @@ struct Sim {
     PDev p{};
     PScalars* ps = nullptr;      // device
     PScalars* ps_host = nullptr; // pinned
-    int key_bits = 1;
+    // The last charge spread left q zero except at its own nodes (the next one
+    // clears those instead of the whole grid).
+    int spread_zero_old = 0;
     double q_tot = 0.0;          // library/src/solver_md.c:16
     int strict_charge_check = 1;
     // Poisson-Boltzmann boundary from ion radii (maps and boundary forces).
