@@ -306,6 +306,21 @@ int maze_test_mg_coarse(const float *b, float *e, int c);
  * first level it covers (its size is returned; -1 if the grid has no tail). */
 int maze_test_mg_tail(const float *b, float *d, int n);
 
+/* Allocation-free FP64 CG solve of A x = b (the implicit-Jacobi CG of
+ * maze_test_conj_grad, library/src/plcg.c:28-90) on caller-owned DEVICE
+ * buffers b and x (n^3 doubles each; x holds the start and receives the
+ * solution) on `stream` (a hipStream_t, or NULL for the default stream);
+ * blocks until the solve has finished.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code around graph captures,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its work space is one
+ * statically allocated device array per GPU (6.4 MB, held until the process
+ * ends): grids of at most 64^3 points.  Calls that overlap on one GPU share
+ * that array and race; serialise them.  Returns the 0-based iteration index
+ * at convergence, -1 if not converged in n^2 iterations, -2 for n outside
+ * 2..64 or a null pointer (nothing is touched), -3 if the work space cannot be
+ * found on the current device. */
+int maze_conj_grad_noalloc(const double *b, double *x, double tol, int n, void *stream);
+
 /* ====================== Part 3: the replica engine ======================== */
 /* Each request is one MaZe-LCG replica: NVE velocity Verlet, CIC charges,
  * Tosi-Fumi pair forces (cutoff L/2), vacuum dielectric, the reference's two

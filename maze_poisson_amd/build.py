@@ -16,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libmaze_poisson.so")
 SOURCES = ["grid_kernels.hip", "mg_kernels.hip", "particle_kernels.hip", "pb_kernels.hip", "sim.hip",
-           "abi.hip", "engine.hip"]
+           "abi.hip", "engine.hip", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
