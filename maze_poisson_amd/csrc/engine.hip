@@ -81,7 +81,8 @@ static int precision_for(int n, int want) {
 // aligned).
 static long long arena_bytes(int n, int np, int precision) {
     const long long G = (long long)n * n * n;
-    long long b = 4LL * 8 * (1 << 15) + (64 << 10);  // partials of both streams, scalars
+    // reduction partials of both streams and of the deferred inner iteration, scalars
+    long long b = 8LL * 8 * (1 << 15) + (64 << 10);
     b += 9 * 8 * G;                                   // q, y, phi, phi_prev, phi*, sigma, r, p0, p1
     if (precision != kFP64) b += 8 * G + 4 * 4 * G;   // second y, FP32 d, r, p0, p1
     if (precision == kMixedMG) {

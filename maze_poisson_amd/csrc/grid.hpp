@@ -36,6 +36,21 @@ struct CGState {
     int inner_total; // inner iterations summed over the applied passes
     int pass_iters[kMaxPasses];  // per pass: iterations if converged, else -(iterations)
     double otol;     // absolute stop of the outer loop
+    // <r, z> of the multigrid-PCG inner iterations by iteration parity: the
+    // direction pass of iteration k writes slot k & 1 and reads the other.
+    double rdv2[2];
+};
+
+// Partials of the multigrid-PCG inner iteration's reductions, folded by the
+// kernel that consumes them instead of by a last-arriving work-group
+// (deferred finalisation: no arrival counter, no tail on the producer):
+// <r, z> of the finest up kernel (pz), <p, A p> of the direction pass (pp),
+// <r, r> and max|r| of the update (pus, pum).
+struct DeferRed {
+    double* pz;
+    double* pp;
+    double* pus;
+    double* pum;
 };
 
 // What the host reads after a solve: the solver state and the charge sum of
@@ -105,9 +120,11 @@ struct RefineArgs {
     double otol, rtol;
     int limit, pcg;
 };
+// defer: the inner solve before this pass ended with a deferred update whose
+// stop test the refinement pass applies first (nullptr: none pending).
 void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, double* y0, double* y1,
                    const float* dcorr, const double* sigma, float* r32, RefineArgs a, CGState* st,
-                   Reduce red, int add_d);
+                   Reduce red, int add_d, const DeferRed* defer = nullptr);
 // First pass from the extrapolated iterate y_out = 2 y - y_prev (then y_out is
 // the loop's y0).
 void launch_refine_extrap(hipStream_t s, const GridDims& d, const PBCoef& pb, const double* y,
@@ -158,6 +175,19 @@ void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const doub
                       const double* ilam, CGState* st);
 void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,
                           const double* Qf, const double* ilam, CGState* st);
+// The multigrid-PCG direction and update with deferred finalisation (Poisson
+// operator).  pcg_dir: beta from the finest up kernel's partials (zcnt of
+// them; 0 for the first direction), writes <r, z> to rdv2[par] and its
+// <p, A p> partials.  update: alpha = rdv2[par] / <p, A p> from those
+// partials, x += alpha p, r += alpha A p, partials of <r, r> and max|r|; the
+// next consumer applies the stop test.
+void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
+                       float* p_new, CGState* st, const DeferRed& dr, int zcnt, int first, int par);
+void launch_cg_update_df(hipStream_t s, const GridDims& d, const float* p, float* x, float* r,
+                         CGState* st, const DeferRed& dr, int zero_x, int par);
+// Applies the deferred stop test of the last update (one work-group): for a
+// host read of the state in the middle of an inner solve.
+void launch_df_finalize(hipStream_t s, CGState* st, const DeferRed& dr, int cnt);
 void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                     float* p_new, CGState* st, Reduce red, int first, const PBCoef& pb = PBCoef{});
 // FP32 multigrid for the linear PB operator (mg_kernels.hip), with FP32
@@ -202,10 +232,16 @@ bool mg_tile_ok(int tile, int n);
 const char* mg_tile_name(int tile);
 // Smoother weights c1, c2 (damping / 6): pre-smoothing sweeps c1 then c2,
 // post-smoothing c2 then c1 (polynomials in A, so the cycle stays symmetric).
+// stop: the finest level's down kernel of an inner iteration after the first
+// also applies the previous update's deferred stop test (nullptr: none).
 void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
-                    double c1, double c2, CGState* st);
+                    double c1, double c2, CGState* st, const DeferRed* stop = nullptr,
+                    int stop_cnt = 0);
+// dot: the finest level also writes its <b, d> partials to red.psum (one per
+// work-group, folded by the PCG direction pass; mg_up_blocks counts them).
 void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d2, const float* e,
                   float* d, double c1, double c2, CGState* st, int dot, Reduce red);
+int mg_up_blocks(int n, int tile);
 // Rest of the V-cycle from a small level (sizes n[0..nlev), finest first) down
 // to the exact solve and back, in one work-group with the data in LDS:
 // reads that level's right-hand side b, writes its correction d.

@@ -111,56 +111,6 @@ struct BodyDir {
     }
 };
 
-// x += alpha p; r += alpha A p; <r, r>, max|r|; stop test and beta
-// (library/src/plcg.c:64-80; relative stop of library/src/multigrid.c:74-78).
-// ZeroX: x is taken as 0 (first update of a solve started from zero), so it
-// is written without being read.
-template <class Real, bool ZeroX = false>
-struct BodyUpdate {
-    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = true;
-    Real* x;
-    Real* r;
-    double alpha;
-    double s = 0.0, m = 0.0;
-    struct Raw { Real x, r; };
-    __device__ Raw load(int c) const { return Raw{ZeroX ? (Real)0 : x[c], r[c]}; }
-    __device__ void apply(int c, const Raw& o, double v, double lap) {
-        x[c] = (Real)((double)o.x + alpha * v);
-        const Real rn = (Real)((double)o.r + alpha * lap);
-        r[c] = rn;
-        const double rd = (double)rn;
-        s += rd * rd;
-        m = fmax(m, fabs(rd));
-    }
-    __device__ static void finalize(CGState* st, double S, double M) {
-        st->rr = S;
-        st->rmax = M;
-        if (st->count_mode) {
-            if (M <= st->rtol * st->r0) {
-                st->done = 1;
-                st->res = st->iter + 1;
-                return;
-            }
-        } else if (M <= st->tol) {
-            st->done = 1;
-            st->res = st->iter;
-            return;
-        }
-        if (!st->pcg) {
-            // <r_new, v_new> with v_new = -r_new/6, or <r_new, r_new> for plain CG
-            // (library/src/plcg.c:74-78, library/src/multigrid_pb.c:83-86).
-            const double rn_dot_vn = st->plain ? S : -S / 6.0;
-            st->beta = rn_dot_vn / st->rdv;
-            st->rdv = rn_dot_vn;
-        }
-        st->iter += 1;
-        if (st->iter >= st->limit) {
-            st->done = 1;
-            st->res = st->count_mode ? st->limit : -1;
-        }
-    }
-};
-
 // y_out = the new iterate (y + d, or the extrapolated start 2 y - y_prev);
 // the FP64 residual b = sigma - A y_out; max|b|.  Advances the outer
 // refinement loop and starts the next inner solve A x = b from x = 0:
@@ -174,6 +124,8 @@ struct BodyRefine {
     float* r32;
     double otol, rtol;
     int limit, pcg;
+    // Outcome of the inner solve of this pass (Add): stopped, and its result.
+    int in_done = 0, in_res = 0;
     double s = 0.0, m = 0.0;
     using Raw = double;
     __device__ Raw load(int c) const { return sigma[c]; }
@@ -192,7 +144,7 @@ struct BodyRefine {
         if (Add) {
             // The inner solve of this pass: its iteration count if it met its
             // relative stop, else minus the iterations it was given.
-            const int it = st->done ? max(st->res, 0) : -st->iter;
+            const int it = in_done ? max(in_res, 0) : -st->iter;
             const int k = st->passes;
             if (k < kMaxPasses) st->pass_iters[k] = it;
             st->inner_total += it >= 0 ? it : -it;
@@ -388,19 +340,37 @@ void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int li
 // no host round trip; a pass after the outer loop converged returns at once.
 // An inner solve that applied no update (stopped before its first one) left
 // d untouched, so d then counts as zero.
+// dps, dpm, dcnt: the partials of the inner solve's last update when its stop
+// test is deferred (multigrid-PCG); every work-group applies it: a stop found
+// there, or one recorded by an earlier kernel (st->done), ends the solve.
 template <bool PB, bool Add>
 __global__ __launch_bounds__(kBlock) void k_refine(GridDims d, PBCoef pb, double* y0, double* y1,
                                                    const float* dc, const double* sigma,
                                                    float* r32, RefineArgs a, Reduce red,
-                                                   CGState* st) {
-    int sel = 0;
+                                                   CGState* st, const double* dps,
+                                                   const double* dpm, int dcnt) {
+    int sel = 0, in_done = 0, in_res = 0;
     double dscale = 0.0;
     if (Add) {
         if (st->outer_done) return;
         sel = st->ysel;
-        dscale = (st->done ? st->res : st->iter) > 0 ? 1.0 : 0.0;
+        int r = INT_MIN;
+        if (dps) {
+            __shared__ double shs[2 * kBlock / kWave];
+            r = df_stop_test<kBlock>(st, dps, dpm, dcnt, shs);
+        }
+        if (r != INT_MIN) {
+            in_done = 1;
+            in_res = r;
+        } else if (st->done) {
+            in_done = 1;
+            in_res = st->res;
+        }
+        dscale = (in_done ? in_res : st->iter) > 0 ? 1.0 : 0.0;
     }
     BodyRefine<Add, Add> bd{sel ? y0 : y1, sigma, r32, a.otol, a.rtol, a.limit, a.pcg};
+    bd.in_done = in_done;
+    bd.in_res = in_res;
     march<PB>(d, pb, SrcSum<Add>{sel ? y1 : y0, dc, dscale}, bd);
     reduce_and_finalize<true>(bd, red, st);
 }
@@ -425,13 +395,17 @@ void launch_refine_extrap(hipStream_t s, const GridDims& d, const PBCoef& pb, co
 
 void launch_refine(hipStream_t s, const GridDims& d, const PBCoef& pb, double* y0, double* y1,
                    const float* dcorr, const double* sigma, float* r32, RefineArgs a, CGState* st,
-                   Reduce red, int add_d) {
+                   Reduce red, int add_d, const DeferRed* defer) {
+    const double* dps = add_d && defer ? defer->pus : nullptr;
+    const double* dpm = add_d && defer ? defer->pum : nullptr;
     if (add_d) {
-        DISPATCH_PB(pb, MAZE_LAUNCH((k_refine<kPB, true>), dim3(d.blocks), dim3(kBlock), 0,
-                                           s, d, pb, y0, y1, dcorr, sigma, r32, a, red, st));
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_refine<kPB, true>), dim3(d.blocks), dim3(kBlock), 0, s,
+                                    d, pb, y0, y1, dcorr, sigma, r32, a, red, st, dps, dpm,
+                                    d.blocks));
     } else {
-        DISPATCH_PB(pb, MAZE_LAUNCH((k_refine<kPB, false>), dim3(d.blocks), dim3(kBlock), 0,
-                                           s, d, pb, y0, y1, dcorr, sigma, r32, a, red, st));
+        DISPATCH_PB(pb, MAZE_LAUNCH((k_refine<kPB, false>), dim3(d.blocks), dim3(kBlock), 0, s,
+                                    d, pb, y0, y1, dcorr, sigma, r32, a, red, st, dps, dpm,
+                                    d.blocks));
     }
 }
 

@@ -240,12 +240,28 @@ int mg_tiles(int n) {
     return ((n + TI - 1) / TI) * ((n + TJ - 1) / TJ) * ((n + TK - 1) / TK);
 }
 
+// Stop test of the previous inner iteration's update (deferred finalisation),
+// run by every work-group of the finest down kernel before its work.
+struct DownStop {
+    const double* pus;
+    const double* pum;
+    int cnt;
+};
+
+template <int NT>
+__device__ __forceinline__ bool down_stopped(CGState* st, const DownStop& ds) {
+    if (!ds.pus) return st->done != 0;
+    __shared__ double sh[2 * NT / kWave];
+    const bool stop = df_stop_test<NT>(st, ds.pus, ds.pum, ds.cnt, sh) != INT_MIN;
+    return stop || st->done;
+}
+
 template <int TI, int TJ, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__ b,
                                                     float* __restrict__ d2out,
                                                     float* __restrict__ bcoarse, float c1,
-                                                    float c2, const CGState* st) {
-    if (st->done) return;
+                                                    float c2, CGState* st, DownStop ds) {
+    if (down_stopped<NT>(st, ds)) return;
     constexpr int BI = TI + 5, BJ = TJ + 5, BK = TK + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
     constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
@@ -402,15 +418,11 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
             if (Dot) s += (double)bc * (double)z;
         }
     });
-    if constexpr (Dot) {
-        struct {
-            double s, m;
-            __device__ void finalize(CGState* q, double S, double) const {
-                q->beta = q->first ? 0.0 : S / q->rdv;
-                q->rdv = S;
-            }
-        } bd{s, 0.0};
-        reduce_and_finalize<false, decltype(bd), NT>(bd, red, st);
+    if constexpr (Dot) {  // <b, d> partial for the direction pass (deferred finalisation)
+        __shared__ double sh[2 * NT / kWave];
+        double m = 0.0;
+        block_sum_max_lds<NT>(s, m, sh);
+        if (tid == 0) red.psum[blockIdx.x] = s;
     }
 }
 
@@ -418,9 +430,9 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
 // Sim takes the default per level (mg_default_tile) unless MAZE_MG_TILES names one.
 template <int TI, int TJ, int TK, int NT>
 static void down_t(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
-                   double c2, CGState* st) {
+                   double c2, CGState* st, DownStop ds = DownStop{nullptr, nullptr, 0}) {
     MAZE_LAUNCH((k_mg_down<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s, n, b,
-                d2, bc, (float)c1, (float)c2, st);
+                d2, bc, (float)c1, (float)c2, st, ds);
 }
 
 template <int TI, int TJ, int TK, int NT>
@@ -451,8 +463,8 @@ template <int S>
 __global__ __launch_bounds__(kBlock) void k_mg_down_march(int n, const float* __restrict__ b,
                                                           float* __restrict__ d2out,
                                                           float* __restrict__ bcoarse, float c1,
-                                                          float c2, const CGState* st) {
-    if (st->done) return;
+                                                          float c2, CGState* st, DownStop ds) {
+    if (down_stopped<kBlock>(st, ds)) return;
     constexpr int WB = kMarchC + 6, WD = kMarchC + 4, WR = kMarchC + 2;
     constexpr int PB = WB * WB, PD = WD * WD, PR = WR * WR;
     static_assert(PB <= 2 * kBlock, "two B-plane points per thread");
@@ -695,15 +707,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_up_march(int n, const float* __re
         step(u + 1, rb);
         step(u + 2, rc);
     }
-    if constexpr (Dot) {
-        struct {
-            double s, m;
-            __device__ void finalize(CGState* q, double S_, double) const {
-                q->beta = q->first ? 0.0 : S_ / q->rdv;
-                q->rdv = S_;
-            }
-        } bd{s, 0.0};
-        reduce_and_finalize<false, decltype(bd), kBlock>(bd, red, st);
+    if constexpr (Dot) {  // <b, d> partial for the direction pass (deferred finalisation)
+        __shared__ double shd[2 * kBlock / kWave];
+        double m = 0.0;
+        block_sum_max_lds<kBlock>(s, m, shd);
+        if (threadIdx.x == 0) red.psum[blockIdx.x] = s;
     }
 }
 
@@ -716,10 +724,10 @@ bool mg_tile_ok(int tile, int n) {
 
 template <int S>
 static void down_m(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
-                   double c2, CGState* st) {
+                   double c2, CGState* st, DownStop ds) {
     const int nblk = n / kMarchC;
     MAZE_LAUNCH((k_mg_down_march<S>), dim3(nblk * nblk * (n / S)), dim3(kBlock), 0, s, n, b, d2,
-                bc, (float)c1, (float)c2, st);
+                bc, (float)c1, (float)c2, st, ds);
 }
 
 template <int S>
@@ -748,16 +756,30 @@ const char* mg_tile_name(int tile) {
 int mg_default_tile(int n) { return n >= 64 ? kMgT8x16x16_1024 : kMgT4x8x8_512; }
 
 void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
-                    double c1, double c2, CGState* st) {
+                    double c1, double c2, CGState* st, const DeferRed* stop, int stop_cnt) {
+    if (!mg_tile_ok(tile, n)) tile = mg_default_tile(n);
+    const DownStop ds = stop ? DownStop{stop->pus, stop->pum, stop_cnt} : DownStop{nullptr, nullptr, 0};
+    switch (tile) {
+        case kMgMarch16: down_m<16>(s, n, b, d2, bc, c1, c2, st, ds); break;
+        case kMgMarch8: down_m<8>(s, n, b, d2, bc, c1, c2, st, ds); break;
+        case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st, ds); break;
+        case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st, ds); break;
+        case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st, ds); break;
+        case kMgT4x8x8_512: down_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st, ds); break;
+        default: down_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st, ds); break;
+    }
+}
+
+// Work-groups of the up kernel at a level (the partials its dot writes).
+int mg_up_blocks(int n, int tile) {
     if (!mg_tile_ok(tile, n)) tile = mg_default_tile(n);
     switch (tile) {
-        case kMgMarch16: down_m<16>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgMarch8: down_m<8>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT4x8x8_512: down_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st); break;
-        default: down_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgMarch16: return (n / kMarchC) * (n / kMarchC) * (n / 16);
+        case kMgMarch8: return (n / kMarchC) * (n / kMarchC) * (n / 8);
+        case kMgT8x16x16_1024:
+        case kMgT8x16x16_512: return mg_tiles<8, 16, 16>(n);
+        case kMgT8x8x16_512: return mg_tiles<8, 8, 16>(n);
+        default: return mg_tiles<4, 8, 8>(n);
     }
 }
 
@@ -916,15 +938,19 @@ __global__ __launch_bounds__(4 * kMgCoarseLds * kMgCoarseLds) void k_mg_coarse(
 typedef float mg_v4f __attribute__((ext_vector_type(4)));
 constexpr int kMgCoarseMfmaThreads = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds / 4;
 
+// C (the level's side) is a compile-time constant, so every index division by
+// it is a multiply-shift (a run-time divisor cost more than the transforms).
+template <int C>
 __global__ __launch_bounds__(kMgCoarseMfmaThreads) void k_mg_coarse_mfma(
-    int c, const float* __restrict__ b, float* __restrict__ e, const double* __restrict__ Qf,
+    const float* __restrict__ b, float* __restrict__ e, const double* __restrict__ Qf,
     const double* __restrict__ ilam, const CGState* st) {
     if (st->done) return;
     constexpr int CM = kMgCoarseLds, NT = kMgCoarseMfmaThreads, PER = CM * CM * CM / NT;
     static_assert(CM == 16, "one 16-line tile per wave with 16-wide operands");
+    static_assert(C >= 2 && C <= CM, "coarsest level of 2..16 points per side");
     __shared__ float A[CM * CM * (CM + 1)], B[CM * CM * (CM + 1)];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int c2 = c * c, N = c2 * c, pitch = c + 1;
+    constexpr int c = C, c2 = c * c, N = c2 * c, pitch = c + 1;
     float il[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) il[k] = tid + k * NT < N ? (float)ilam[tid + k * NT] : 0.0f;
@@ -995,9 +1021,20 @@ void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const doub
         return v && std::atoi(v) != 0;
     }();
     if (!valu) {
-        MAZE_LAUNCH(k_mg_coarse_mfma, dim3(1), dim3(kMgCoarseMfmaThreads), 0, s, c, b, e, Qf, ilam,
-                    st);
+        switch (c) {
+#define MAZE_COARSE_CASE(C)                                                                 \
+    case C:                                                                                 \
+        MAZE_LAUNCH(k_mg_coarse_mfma<C>, dim3(1), dim3(kMgCoarseMfmaThreads), 0, s, b, e, Qf, \
+                    ilam, st);                                                              \
         return;
+            MAZE_COARSE_CASE(2) MAZE_COARSE_CASE(3) MAZE_COARSE_CASE(4) MAZE_COARSE_CASE(5)
+            MAZE_COARSE_CASE(6) MAZE_COARSE_CASE(7) MAZE_COARSE_CASE(8) MAZE_COARSE_CASE(9)
+            MAZE_COARSE_CASE(10) MAZE_COARSE_CASE(11) MAZE_COARSE_CASE(12) MAZE_COARSE_CASE(13)
+            MAZE_COARSE_CASE(14) MAZE_COARSE_CASE(15) MAZE_COARSE_CASE(16)
+#undef MAZE_COARSE_CASE
+            default:
+                break;
+        }
     }
     const dim3 grid(1), block(4 * kMgCoarseLds * kMgCoarseLds);
     if (c == kMgCoarseLds)
@@ -1372,6 +1409,103 @@ void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const 
     else
         MAZE_LAUNCH((k_pcg_dir<false, true, PBCoef32>), dim3(d.blocks), dim3(kBlock), 0, s, d, pb,
                     p_old, v, p_new, red, st);
+}
+
+// Direction pass with deferred finalisation: beta = <r, z>_k / <r, z>_(k-1),
+// both from partials (this iteration's from the finest up kernel, folded
+// here; the previous one kept in rdv2 by parity), then p = -z (first) or
+// beta p - z and the plain per-work-group partials of <p, A p>.
+template <bool First>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const float* p_old,
+                                                       const float* v, float* p_new,
+                                                       const double* __restrict__ pz, int zcnt,
+                                                       double* __restrict__ pp, int par,
+                                                       CGState* st) {
+    if (st->done) return;
+    __shared__ double sh[2 * kBlock / kWave];
+    const double S = fold_sum<kBlock>(pz, zcnt, sh);
+    const double beta = First ? 0.0 : S / st->rdv2[par ^ 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rdv2[par] = S;
+        st->rdv = S;
+        st->beta = beta;
+        st->first = 0;
+    }
+    const SrcPcgDir<First> src{p_old, v, beta};
+    BodyPcgDir bd{p_new};
+    march<false>(d, PBCoef{}, src, bd);
+    double s = bd.s, m = 0.0;
+    block_sum_max_lds<kBlock>(s, m, sh);
+    if (threadIdx.x == 0) pp[blockIdx.x] = s;
+}
+
+// Update with deferred finalisation: alpha = <r, z>_k / <p, A p> (folded from
+// the direction pass's partials; a vanishing <p, A p> stops the solve as the
+// breakdown guard of BodyDir does), x += alpha p, r += alpha A p, and plain
+// partials of <r, r> and max|r| for the consumer's stop test.  Work-group 0
+// advances st->iter (no other work-group of this kernel reads it).
+template <bool ZeroX>
+__global__ __launch_bounds__(kBlock) void k_cg_update_df(GridDims d, const float* p, float* x,
+                                                         float* r, const double* __restrict__ pp,
+                                                         double* __restrict__ pus,
+                                                         double* __restrict__ pum, int par,
+                                                         CGState* st) {
+    if (st->done) return;
+    __shared__ double sh[2 * kBlock / kWave];
+    const double pAp = fold_sum<kBlock>(pp, d.blocks, sh);
+    const bool breakdown = fabs(pAp) < 1e-300;
+    const double alpha = breakdown ? 0.0 : st->rdv2[par] / pAp;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pAp = pAp;
+        st->alpha = alpha;
+        if (breakdown) {
+            st->res = st->iter;
+            st->done = 1;
+        } else {
+            st->iter = st->iter + 1;
+        }
+    }
+    if (breakdown) return;
+    BodyUpdate<float, ZeroX> bd{x, r, alpha};
+    march<false>(d, PBCoef{}, SrcArr<float>{p}, bd);
+    double s = bd.s, m = bd.m;
+    block_sum_max_lds<kBlock>(s, m, sh);
+    if (threadIdx.x == 0) {
+        pus[blockIdx.x] = s;
+        pum[blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_df_finalize(CGState* st, const double* pus,
+                                                        const double* pum, int cnt) {
+    if (st->done) return;
+    __shared__ double sh[2 * kBlock / kWave];
+    (void)df_stop_test<kBlock>(st, pus, pum, cnt, sh);
+}
+
+void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
+                       float* p_new, CGState* st, const DeferRed& dr, int zcnt, int first,
+                       int par) {
+    if (first)
+        MAZE_LAUNCH((k_pcg_dir_df<true>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
+                    dr.pz, zcnt, dr.pp, par, st);
+    else
+        MAZE_LAUNCH((k_pcg_dir_df<false>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
+                    dr.pz, zcnt, dr.pp, par, st);
+}
+
+void launch_cg_update_df(hipStream_t s, const GridDims& d, const float* p, float* x, float* r,
+                         CGState* st, const DeferRed& dr, int zero_x, int par) {
+    if (zero_x)
+        MAZE_LAUNCH((k_cg_update_df<true>), dim3(d.blocks), dim3(kBlock), 0, s, d, p, x, r, dr.pp,
+                    dr.pus, dr.pum, par, st);
+    else
+        MAZE_LAUNCH((k_cg_update_df<false>), dim3(d.blocks), dim3(kBlock), 0, s, d, p, x, r, dr.pp,
+                    dr.pus, dr.pum, par, st);
+}
+
+void launch_df_finalize(hipStream_t s, CGState* st, const DeferRed& dr, int cnt) {
+    MAZE_LAUNCH(k_df_finalize, dim3(1), dim3(kBlock), 0, s, st, dr.pus, dr.pum, cnt);
 }
 
 // ---- FP32 multigrid for the linear Poisson-Boltzmann operator ----------------

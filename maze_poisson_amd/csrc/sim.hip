@@ -129,6 +129,10 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     red.psum = dalloc<double>(kMaxPartials);
     red.pmax = dalloc<double>(kMaxPartials);
     red.counter = dalloc<unsigned int>(4);
+    dred.pz = dalloc<double>(kMaxPartials);
+    dred.pp = dalloc<double>(kMaxPartials);
+    dred.pus = dalloc<double>(kMaxPartials);
+    dred.pum = dalloc<double>(kMaxPartials);
     dscalar = dalloc<double>(8);
     MAZE_HIP(hipHostMalloc(&hscalar, 8 * sizeof(double)));
     ps = dalloc<PScalars>(1);
@@ -151,6 +155,10 @@ Sim::~Sim() {
     dfree(red.psum);
     dfree(red.pmax);
     dfree(red.counter);
+    dfree(dred.pz);
+    dfree(dred.pp);
+    dfree(dred.pus);
+    dfree(dred.pum);
     dfree(red_side.psum);
     dfree(red_side.pmax);
     dfree(red_side.counter);
@@ -624,11 +632,16 @@ void Sim::inner_body(bool use_mg, float* x, int it) {
             launch_cg_update32(stream, dims, pbmg[0].cf, pout, x, r32, st, red, it == 0);
         });
     } else if (use_mg) {
+        // Deferred finalisation: the finest up kernel, the direction pass and
+        // the update write per-work-group partials that the next kernel folds.
+        const int zcnt = mg_up_blocks(mg[0].n, mg[0].tile_up);
+        if (zcnt > kMaxPartials) fatal("maze: finest multigrid tiles exceed the reduction buffers");
         vcycle(0, it);
-        timed(kTCgDirF32, it,
-              [&] { launch_pcg_dir(stream, dims, pin, mg[0].d, pout, st, red, it == 0); });
+        timed(kTCgDirF32, it, [&] {
+            launch_pcg_dir_df(stream, dims, pin, mg[0].d, pout, st, dred, zcnt, it == 0, it & 1);
+        });
         timed(kTCgUpdateF32, it, [&] {
-            launch_cg_update<float>(stream, dims, PBCoef{}, pout, x, r32, st, red, it == 0);
+            launch_cg_update_df(stream, dims, pout, x, r32, st, dred, it == 0, it & 1);
         });
     } else {
         timed(kTCgDirF32, it,
@@ -660,14 +673,19 @@ int Sim::inner_solve(bool use_mg, const float* b, float* x, double rtol, int lim
     inner_start(use_mg, b, x, rtol, limit, 0);
     int launched = 0;
     for (; launched < std::min(first_chunk, limit); launched++) inner_iter(use_mg, x, launched);
-    read_state();
+    inner_read_state(use_mg);
     while (!st_host->done && launched < limit) {
         inner_iter(use_mg, x, launched++);
-        read_state();
+        inner_read_state(use_mg);
     }
     const int it = st_host->res;
     prof_flush(it);
     return it;
+}
+
+void Sim::inner_read_state(bool use_mg) {
+    if (inner_deferred(use_mg)) launch_df_finalize(stream, st, dred, dims.blocks);
+    read_state();
 }
 
 void Sim::free_mg() {
@@ -935,10 +953,18 @@ void Sim::vcycle(int l, int iter) {
         if (top) timed(timer, iter, f);
         else f();
     };
-    t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st); });
+    // The finest down kernel of an iteration after the first applies the
+    // previous update's deferred stop test; the finest up kernel writes the
+    // <r, z> partials the direction pass folds.
+    const bool stop = top && iter > 0;
+    t(kTMgDown, [&] {
+        launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st,
+                       stop ? &dred : nullptr, dims.blocks);
+    });
     vcycle(l + 1, iter);
     t(kTMgUp, [&] {
-        launch_mg_up(stream, L.n, L.tile_up, L.b, L.t, mg[l + 1].d, L.d, c1, c2, st, top, red);
+        launch_mg_up(stream, L.n, L.tile_up, L.b, L.t, mg[l + 1].d, L.d, c1, c2, st, top,
+                     Reduce{dred.pz, nullptr, nullptr});
     });
 }
 
@@ -1030,7 +1056,8 @@ void Sim::mixed_issue() {
 // Each refinement pass also starts the next pass's inner solve.
 void Sim::mixed_refine(int add) {
     timed(kTRefine, 0, [&] {
-        launch_refine(stream, dims, pbc(), mx.ya, mx.yb, d32, sigma, r32, mx.ra, st, red, add);
+        launch_refine(stream, dims, pbc(), mx.ya, mx.yb, d32, sigma, r32, mx.ra, st, red, add,
+                      inner_deferred(mx.use_mg) ? &dred : nullptr);
     });
 }
 
@@ -1066,10 +1093,10 @@ void Sim::mixed_pass(int k, int m, bool poll) {
     int launched = 0;
     for (; launched < std::min(m, mx.inner_limit); launched++) inner_iter(mx.use_mg, d32, launched);
     if (poll) {
-        read_state();
+        inner_read_state(mx.use_mg);
         while (!st_host->done && launched < mx.inner_limit) {
             inner_iter(mx.use_mg, d32, launched++);
-            read_state();
+            inner_read_state(mx.use_mg);
         }
     }
     mixed_refine(1);

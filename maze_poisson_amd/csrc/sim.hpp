@@ -174,6 +174,8 @@ struct Sim {
     // An MD step in progress (step_issue .. step_finish).
     int step_tf = 0, step_wait = 0, step_res = 0;
     Reduce red{};
+    // Partials of the multigrid-PCG inner iteration (deferred finalisation).
+    DeferRed dred{};
     // The Tosi-Fumi forces depend on the positions only, so one MD step
     // (md_step) runs them on a second stream (default priority) concurrently
     // with the charge spread and the field solve, with their own reduction
@@ -335,6 +337,11 @@ struct Sim {
     int mg_stop = default_mg_stop();  // see mg_sizes
     void free_mg();
     void vcycle(int level, int iter);
+    // The inner iteration's stop test is deferred to the next consumer
+    // (multigrid-PCG on the Poisson operator); a host read of the state in the
+    // middle of an inner solve applies it first.
+    bool inner_deferred(bool use_mg) const { return use_mg && !pb; }
+    void inner_read_state(bool use_mg);
     int maze_fp64();
     void fp64_issue();
     int fp64_finish();

@@ -201,6 +201,141 @@ __device__ __forceinline__ void reduce_and_finalize(Body& bd, Reduce red, CGStat
     }
 }
 
+// x += alpha p; r += alpha A p; <r, r>, max|r|; stop test and beta
+// (library/src/plcg.c:64-80; relative stop of library/src/multigrid.c:74-78).
+// ZeroX: x is taken as 0 (first update of a solve started from zero), so it
+// is written without being read.
+template <class Real, bool ZeroX = false>
+struct BodyUpdate {
+    static constexpr bool kReduce = true, kMax = true, kSkipWhenDone = true;
+    Real* x;
+    Real* r;
+    double alpha;
+    double s = 0.0, m = 0.0;
+    struct Raw { Real x, r; };
+    __device__ Raw load(int c) const { return Raw{ZeroX ? (Real)0 : x[c], r[c]}; }
+    __device__ void apply(int c, const Raw& o, double v, double lap) {
+        x[c] = (Real)((double)o.x + alpha * v);
+        const Real rn = (Real)((double)o.r + alpha * lap);
+        r[c] = rn;
+        const double rd = (double)rn;
+        s += rd * rd;
+        m = fmax(m, fabs(rd));
+    }
+    __device__ static void finalize(CGState* st, double S, double M) {
+        st->rr = S;
+        st->rmax = M;
+        if (st->count_mode) {
+            if (M <= st->rtol * st->r0) {
+                st->done = 1;
+                st->res = st->iter + 1;
+                return;
+            }
+        } else if (M <= st->tol) {
+            st->done = 1;
+            st->res = st->iter;
+            return;
+        }
+        if (!st->pcg) {
+            // <r_new, v_new> with v_new = -r_new/6, or <r_new, r_new> for plain CG
+            // (library/src/plcg.c:74-78, library/src/multigrid_pb.c:83-86).
+            const double rn_dot_vn = st->plain ? S : -S / 6.0;
+            st->beta = rn_dot_vn / st->rdv;
+            st->rdv = rn_dot_vn;
+        }
+        st->iter += 1;
+        if (st->iter >= st->limit) {
+            st->done = 1;
+            st->res = st->count_mode ? st->limit : -1;
+        }
+    }
+};
+
+// ---- deferred finalisation ----------------------------------------------------
+// The producer writes one partial per work-group with a plain store and ends;
+// every work-group of the consumer folds all partials itself, in the same
+// fixed order, so all of them derive the same scalars.  The kernel boundary
+// between producer and consumer makes the partials visible; no work-group of
+// the producer waits for a store or an arrival counter.
+
+// Work-group sum and max whose barrier orders LDS only (global stores issued
+// before it stay in flight).  sh holds 2 NT / 64 doubles; results in all threads.
+template <int NT>
+__device__ __forceinline__ void block_sum_max_lds(double& s, double& m, double* sh) {
+    s = wave_sum(s);
+    m = wave_max(m);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sh[w] = s;
+        sh[NT / kWave + w] = m;
+    }
+    lds_barrier();
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < NT / kWave; i++) {
+        a += sh[i];
+        b = fmax(b, sh[NT / kWave + i]);
+    }
+    s = a;
+    m = b;
+}
+
+// Sum (and max) of partials[0..cnt) in a fixed order; valid in all threads.
+// Called once per kernel, before any other use of sh.
+template <int NT>
+__device__ __forceinline__ double fold_sum(const double* __restrict__ p, int cnt, double* sh) {
+    double a = 0.0, m = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += NT) a += p[i];
+    block_sum_max_lds<NT>(a, m, sh);
+    lds_barrier();  // sh may be reused after the fold
+    return a;
+}
+
+template <int NT>
+__device__ __forceinline__ void fold_sum_max(const double* __restrict__ ps,
+                                             const double* __restrict__ pm, int cnt, double* sh,
+                                             double& S, double& M) {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+        a += ps[i];
+        b = fmax(b, pm[i]);
+    }
+    block_sum_max_lds<NT>(a, b, sh);
+    lds_barrier();
+    S = a;
+    M = b;
+}
+
+// The stop test of a multigrid-PCG update, from its partials (pus, pum): the
+// same decision as BodyUpdate::finalize, with st->iter already advanced by the
+// update.  Returns the result code if the test stops the solve (converged or
+// out of budget), else INT_MIN; work-group 0 records a stop in st.  The test
+// reads only fields that no consumer writes, so every work-group of a kernel
+// decides alike whenever work-group 0 writes; a stop recorded by an earlier
+// kernel (st->done, e.g. a breakdown) is the caller's to check.
+// sh: 2 NT / 64 doubles.
+template <int NT>
+__device__ __forceinline__ int df_stop_test(CGState* st, const double* pus, const double* pum,
+                                            int cnt, double* sh) {
+    double S, M;
+    fold_sum_max<NT>(pus, pum, cnt, sh, S, M);
+    const int k = st->iter;
+    int res = INT_MIN;
+    if (st->count_mode) {
+        if (M <= st->rtol * st->r0) res = k;
+        else if (k >= st->limit) res = st->limit;
+    } else {
+        if (M <= st->tol) res = k - 1;
+        else if (k >= st->limit) res = -1;
+    }
+    if (res != INT_MIN && blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rr = S;
+        st->rmax = M;
+        st->res = res;
+        st->done = 1;
+    }
+    return res;
+}
+
 // One sweep with an optional single-pass reduction of (sum, max) accumulated
 // by the body; the last work-group calls Body::finalize with the totals.
 template <bool PB, class Src, class Body, class C>
