@@ -342,7 +342,9 @@ void launch_cg_reset(hipStream_t s, CGState* st, double tol, double rtol, int li
 // d untouched, so d then counts as zero.
 // dps, dpm, dcnt: the partials of the inner solve's last update when its stop
 // test is deferred (multigrid-PCG); every work-group applies it: a stop found
-// there, or one recorded by an earlier kernel (st->done), ends the solve.
+// there, or one recorded by an earlier kernel (st->done), ends the solve.  The
+// outcome goes into the pass record of the last work-group's finalize, the
+// only writer of the solver state in this kernel.
 template <bool PB, bool Add>
 __global__ __launch_bounds__(kBlock) void k_refine(GridDims d, PBCoef pb, double* y0, double* y1,
                                                    const float* dc, const double* sigma,
@@ -357,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_refine(GridDims d, PBCoef pb, double
         int r = INT_MIN;
         if (dps) {
             __shared__ double shs[2 * kBlock / kWave];
-            r = df_stop_test<kBlock>(st, dps, dpm, dcnt, shs);
+            r = df_stop_test<kBlock>(st, dps, dpm, dcnt, shs, /*record=*/false);
         }
         if (r != INT_MIN) {
             in_done = 1;

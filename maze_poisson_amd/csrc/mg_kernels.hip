@@ -256,12 +256,45 @@ __device__ __forceinline__ bool down_stopped(CGState* st, const DownStop& ds) {
     return stop || st->done;
 }
 
+// The same test in two halves, so that the partials' loads overlap the tile's
+// own: down_prefetch loads this thread's first entries (issued before the
+// tile loads, consumed after them), down_decide folds the rest and decides.
+// Work-group 0 records a stop while later work-groups may be starting, so a
+// kernel using these reads the flag of earlier kernels with wg_done (a
+// decision of the whole work-group) and down_decide reads no flag.
+struct DownPre {
+    double s, m;
+};
+
+template <int NT>
+__device__ __forceinline__ DownPre down_prefetch(const DownStop& ds) {
+    DownPre p{0.0, 0.0};
+    if (ds.pus && (int)threadIdx.x < ds.cnt) {
+        p.s = ds.pus[threadIdx.x];
+        p.m = ds.pum[threadIdx.x];
+    }
+    return p;
+}
+
+template <int NT>
+__device__ __forceinline__ bool down_decide(CGState* st, const DownStop& ds, DownPre p) {
+    __shared__ double sh[2 * NT / kWave];
+    for (int i = threadIdx.x + NT; i < ds.cnt; i += NT) {
+        p.s += ds.pus[i];
+        p.m = fmax(p.m, ds.pum[i]);
+    }
+    block_sum_max_lds<NT>(p.s, p.m, sh);
+    return df_decide(st, p.s, p.m) != INT_MIN;
+}
+
 template <int TI, int TJ, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__ b,
                                                     float* __restrict__ d2out,
                                                     float* __restrict__ bcoarse, float c1,
                                                     float c2, CGState* st, DownStop ds) {
-    if (down_stopped<NT>(st, ds)) return;
+    __shared__ int ended;
+    if (threadIdx.x == 0) ended = st->done;
+    const DownPre pre = down_prefetch<NT>(ds);
     constexpr int BI = TI + 5, BJ = TJ + 5, BK = TK + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
     constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
@@ -282,10 +315,12 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
     if (tid >= 64 && tid < 64 + BJ) gj[tid - 64] = wrap_mod(T.j0 - 3 + tid - 64, n) * n;
     if (tid >= 128 && tid < 128 + BK) gk[tid - 128] = wrap_mod(T.k0 - 3 + tid - 128, n);
     lds_barrier();
+    if (ended) return;  // the whole work-group (the index tables need every wave)
     tile_for<BI * BJ * BK, NT>([&](int x) {
         const int k = x % BK, j = (x / BK) % BJ, i = x / (BK * BJ);
         sb[x] = b[gi[i] + gj[j] + gk[k]];
     });
+    if (ds.pus && down_decide<NT>(st, ds, pre)) return;
     lds_barrier();
     // Sweeps from zero with weights c1 then c2: d2 = -(c1 + c2) b - c1 c2 A b.
     const float cs = c1 + c2, cp = c1 * c2;
@@ -426,24 +461,217 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
     }
 }
 
+// ---- the same fused level kernels with 16-byte loads ---------------------------
+// For n % 8 == 0 and n >= 32.  Every row of a tile's region is read as the
+// 4-float groups of the aligned window [k0 - 4, k0 + TK + 4) (fine levels) or
+// [k0/2 - 4, k0/2 + TK/2 + 4) (the coarse correction): group starts are
+// multiples of 4 and n (nc) is a multiple of 4, so a wrapped group stays
+// contiguous.  LDS rows are RW = TK + 8 floats (a few unused columns).  The
+// arithmetic is the scalar kernels', value for value and in the same order,
+// so the results are bit-identical to them (tested); about 10 % faster at
+// 128^3 (one 16-byte request instead of four 4-byte ones, and no index
+// tables).
+__device__ __forceinline__ int wrap1(int v, int m) { return v < 0 ? v + m : (v >= m ? v - m : v); }
+
+template <int TI, int TJ, int TK, int NT>
+__global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restrict__ b,
+                                                    float* __restrict__ d2out,
+                                                    float* __restrict__ bcoarse, float c1,
+                                                    float c2, CGState* st, DownStop ds) {
+    if (wg_done(st)) return;
+    const DownPre pre = down_prefetch<NT>(ds);
+    constexpr int BI = TI + 5, BJ = TJ + 5;
+    constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
+    constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
+    constexpr int CI = TI / 2, CJ = TJ / 2, CK = TK / 2;
+    constexpr int RW = TK + 8, RV = RW / 4;
+    __shared__ __attribute__((aligned(16))) float sb[BI * BJ * RW];
+    __shared__ float sd[DI * DJ * RW];
+    __shared__ float sr[RI * RJ * RW];
+    const MgTileMap<TI, TJ, TK> T(n);
+    const int n2 = n * n;
+    tile_for<BI * BJ * RV, NT>([&](int x) {
+        const int g = x % RV, r = x / RV, j = r % BJ, i = r / BJ;
+        const int gi = wrap1(T.i0 - 3 + i, n), gj = wrap1(T.j0 - 3 + j, n);
+        const int gk = wrap1(T.k0 - 4 + 4 * g, n);
+        *reinterpret_cast<float4*>(&sb[r * RW + 4 * g]) =
+            *reinterpret_cast<const float4*>(&b[(long)gi * n2 + gj * n + gk]);
+    });
+    if (ds.pus && down_decide<NT>(st, ds, pre)) return;
+    lds_barrier();
+    // d2 on the D region (local k = fine k + 2, LDS column k + 2).
+    const float cs = c1 + c2, cp = c1 * c2;
+    tile_for<DI * DJ * DK, NT>([&](int x) {
+        const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
+        const int o = ((i + 1) * BJ + (j + 1)) * RW + k + 2;
+        const float bc = sb[o];
+        const float lap = sb[o - BJ * RW] + sb[o + BJ * RW] + sb[o - RW] + sb[o + RW] + sb[o - 1] +
+                          sb[o + 1] - 6.0f * bc;
+        sd[(i * DJ + j) * RW + k + 2] = -cs * bc - cp * lap;
+    });
+    lds_barrier();
+    // Residual on the R region (local k = fine k + 1, LDS column k + 3).
+    tile_for<RI * RJ * RK, NT>([&](int x) {
+        const int k = x % RK, j = (x / RK) % RJ, i = x / (RK * RJ);
+        const int o = ((i + 1) * DJ + (j + 1)) * RW + k + 3;
+        const float dc = sd[o];
+        const float lap = sd[o - DJ * RW] + sd[o + DJ * RW] + sd[o - RW] + sd[o + RW] + sd[o - 1] +
+                          sd[o + 1] - 6.0f * dc;
+        sr[(i * RJ + j) * RW + k + 3] = sb[((i + 2) * BJ + (j + 2)) * RW + k + 3] - lap;
+    });
+    tile_for<TI * TJ * TK, NT>([&](int x) {
+        const int k = x % TK, j = (x / TK) % TJ, i = x / (TK * TJ);
+        if (T.i0 + i < n && T.j0 + j < n && T.k0 + k < n)
+            d2out[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] =
+                sd[((i + 2) * DJ + (j + 2)) * RW + k + 4];
+    });
+    lds_barrier();
+    const int nc = n / 2;
+    tile_for<CI * CJ * CK, NT>([&](int x) {
+        const int K = x % CK, J = (x / CK) % CJ, I = x / (CK * CJ);
+        const int gI = T.i0 / 2 + I, gJ = T.j0 / 2 + J, gK = T.k0 / 2 + K;
+        if (gI >= nc || gJ >= nc || gK >= nc) return;
+        const int o = ((2 * I + 1) * RJ + (2 * J + 1)) * RW + (2 * K + 4);
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = -1; a <= 1; a++)
+#pragma unroll
+            for (int bb = -1; bb <= 1; bb++)
+#pragma unroll
+                for (int cc = -1; cc <= 1; cc++) {
+                    const float w = (float)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
+                    acc += w * sr[o + (a * RJ + bb) * RW + cc];
+                }
+        bcoarse[(gI * nc + gJ) * nc + gK] = acc * (4.0f / 64.0f);
+    });
+}
+
+template <int TI, int TJ, int TK, int NT, bool Dot>
+__global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict__ b,
+                                                  const float* __restrict__ d2,
+                                                  const float* __restrict__ e,
+                                                  float* __restrict__ dout, float c1, float c2,
+                                                  Reduce red, CGState* st) {
+    if (st->done) return;
+    constexpr int DI = TI + 4, DJ = TJ + 4, DK = TK + 4;
+    constexpr int PI = TI + 2, PJ = TJ + 2, PK = TK + 2;
+    constexpr int EI = TI / 2 + 3, EJ = TJ / 2 + 3;
+    constexpr int RW = TK + 8, RV = RW / 4;
+    constexpr int RWE = TK / 2 + 8, RVE = RWE / 4;
+    __shared__ __attribute__((aligned(16))) float sd[DI * DJ * RW];
+    __shared__ __attribute__((aligned(16))) float sb[PI * PJ * RW];
+    __shared__ __attribute__((aligned(16))) float se[EI * EJ * RWE];
+    __shared__ float sp[PI * PJ * PK];
+    const MgTileMap<TI, TJ, TK> T(n);
+    const int tid = threadIdx.x;
+    const int n2 = n * n, nc = n / 2, nc2 = nc * nc;
+    tile_for<DI * DJ * RV, NT>([&](int x) {
+        const int g = x % RV, r = x / RV, j = r % DJ, i = r / DJ;
+        const int gi = wrap1(T.i0 - 2 + i, n), gj = wrap1(T.j0 - 2 + j, n);
+        const int gk = wrap1(T.k0 - 4 + 4 * g, n);
+        *reinterpret_cast<float4*>(&sd[r * RW + 4 * g]) =
+            *reinterpret_cast<const float4*>(&d2[(long)gi * n2 + gj * n + gk]);
+    });
+    tile_for<PI * PJ * RV, NT>([&](int x) {
+        const int g = x % RV, r = x / RV, j = r % PJ, i = r / PJ;
+        const int gi = wrap1(T.i0 - 1 + i, n), gj = wrap1(T.j0 - 1 + j, n);
+        const int gk = wrap1(T.k0 - 4 + 4 * g, n);
+        *reinterpret_cast<float4*>(&sb[r * RW + 4 * g]) =
+            *reinterpret_cast<const float4*>(&b[(long)gi * n2 + gj * n + gk]);
+    });
+    tile_for<EI * EJ * RVE, NT>([&](int x) {
+        const int g = x % RVE, r = x / RVE, j = r % EJ, i = r / EJ;
+        const int gi = wrap1(T.i0 / 2 - 1 + i, nc), gj = wrap1(T.j0 / 2 - 1 + j, nc);
+        const int gk = wrap1(T.k0 / 2 - 4 + 4 * g, nc);
+        *reinterpret_cast<float4*>(&se[r * RWE + 4 * g]) =
+            *reinterpret_cast<const float4*>(&e[(long)gi * nc2 + gj * nc + gk]);
+    });
+    lds_barrier();
+    // d = d2 + trilinear(e) on the D region: local k = fine k + 2 (LDS column
+    // k + 2); the coarse column of local k is (k >> 1) + 3.
+    tile_for<DI * DJ * DK, NT>([&](int x) {
+        const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
+        const int a = i >> 1, bb = j >> 1, cc = (k >> 1) + 3;
+        const float wi = (i & 1) ? 0.5f : 0.0f, wj = (j & 1) ? 0.5f : 0.0f,
+                    wk = (k & 1) ? 0.5f : 0.0f;
+        const int o = (a * EJ + bb) * RWE + cc;
+        const float e000 = se[o], e001 = se[o + 1], e010 = se[o + RWE], e011 = se[o + RWE + 1];
+        const float e100 = se[o + EJ * RWE], e101 = se[o + EJ * RWE + 1];
+        const float e110 = se[o + EJ * RWE + RWE], e111 = se[o + EJ * RWE + RWE + 1];
+        const float v00 = e000 + wk * (e001 - e000);
+        const float v01 = e010 + wk * (e011 - e010);
+        const float v10 = e100 + wk * (e101 - e100);
+        const float v11 = e110 + wk * (e111 - e110);
+        const float v0 = v00 + wj * (v01 - v00);
+        const float v1 = v10 + wj * (v11 - v10);
+        const int q = (i * DJ + j) * RW + k + 2;
+        sd[q] = sd[q] + (v0 + wi * (v1 - v0));
+    });
+    lds_barrier();
+    tile_for<PI * PJ * PK, NT>([&](int x) {
+        const int k = x % PK, j = (x / PK) % PJ, i = x / (PK * PJ);
+        const int o = ((i + 1) * DJ + (j + 1)) * RW + (k + 3);
+        const float dc = sd[o];
+        const float lap = sd[o - DJ * RW] + sd[o + DJ * RW] + sd[o - RW] + sd[o + RW] + sd[o - 1] +
+                          sd[o + 1] - 6.0f * dc;
+        sp[x] = dc - c2 * (sb[(i * PJ + j) * RW + (k + 3)] - lap);
+    });
+    lds_barrier();
+    double s = 0.0;
+    tile_for<TI * TJ * TK, NT>([&](int x) {
+        const int k = x % TK, j = (x / TK) % TJ, i = x / (TK * TJ);
+        const int o = ((i + 1) * PJ + (j + 1)) * PK + (k + 1);
+        const float pc = sp[o];
+        const float lap = sp[o - PJ * PK] + sp[o + PJ * PK] + sp[o - PK] + sp[o + PK] + sp[o - 1] +
+                          sp[o + 1] - 6.0f * pc;
+        const float bc = sb[((i + 1) * PJ + (j + 1)) * RW + (k + 4)];
+        const float z = pc - c1 * (bc - lap);
+        if (T.i0 + i < n && T.j0 + j < n && T.k0 + k < n) {
+            dout[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] = z;
+            if (Dot) s += (double)bc * (double)z;
+        }
+    });
+    if constexpr (Dot) {  // <b, d> partial for the direction pass (deferred finalisation)
+        __shared__ double sh[2 * NT / kWave];
+        double m = 0.0;
+        block_sum_max_lds<NT>(s, m, sh);
+        if (tid == 0) red.psum[blockIdx.x] = s;
+    }
+}
+
+inline bool mg_vec_ok(int n) { return n >= 32 && n % 8 == 0; }
+
 // Tile shapes (fine points per work-group, threads) of the fused level kernels;
 // Sim takes the default per level (mg_default_tile) unless MAZE_MG_TILES names one.
 template <int TI, int TJ, int TK, int NT>
 static void down_t(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
                    double c2, CGState* st, DownStop ds = DownStop{nullptr, nullptr, 0}) {
-    MAZE_LAUNCH((k_mg_down<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s, n, b,
-                d2, bc, (float)c1, (float)c2, st, ds);
+    if (mg_vec_ok(n))
+        MAZE_LAUNCH((k_mg_down_vec<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s,
+                    n, b, d2, bc, (float)c1, (float)c2, st, ds);
+    else
+        MAZE_LAUNCH((k_mg_down<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s, n,
+                    b, d2, bc, (float)c1, (float)c2, st, ds);
 }
 
 template <int TI, int TJ, int TK, int NT>
 static void up_t(hipStream_t s, int n, const float* b, const float* d2, const float* e, float* d,
                  double c1, double c2, CGState* st, int dot, Reduce red) {
-    if (dot)
-        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, true>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s,
-                    n, b, d2, e, d, (float)c1, (float)c2, red, st);
-    else
-        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, false>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0,
-                    s, n, b, d2, e, d, (float)c1, (float)c2, red, st);
+    const dim3 g(mg_tiles<TI, TJ, TK>(n));
+    if (mg_vec_ok(n)) {
+        if (dot)
+            MAZE_LAUNCH((k_mg_up_vec<TI, TJ, TK, NT, true>), g, dim3(NT), 0, s, n, b, d2, e, d,
+                        (float)c1, (float)c2, red, st);
+        else
+            MAZE_LAUNCH((k_mg_up_vec<TI, TJ, TK, NT, false>), g, dim3(NT), 0, s, n, b, d2, e, d,
+                        (float)c1, (float)c2, red, st);
+    } else if (dot) {
+        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, true>), g, dim3(NT), 0, s, n, b, d2, e, d, (float)c1,
+                    (float)c2, red, st);
+    } else {
+        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, false>), g, dim3(NT), 0, s, n, b, d2, e, d,
+                    (float)c1, (float)c2, red, st);
+    }
 }
 
 // ---- plane-marching level kernels ---------------------------------------------
@@ -1423,51 +1651,56 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const float* 
                                                        CGState* st) {
     if (st->done) return;
     __shared__ double sh[2 * kBlock / kWave];
-    const double S = fold_sum<kBlock>(pz, zcnt, sh);
-    const double beta = First ? 0.0 : S / st->rdv2[par ^ 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->rdv2[par] = S;
-        st->rdv = S;
-        st->beta = beta;
-        st->first = 0;
-    }
-    const SrcPcgDir<First> src{p_old, v, beta};
+    SrcPcgDir<First> src{p_old, v, 0.0};
     BodyPcgDir bd{p_new};
-    march<false>(d, PBCoef{}, src, bd);
+    // beta is folded while the first plane's loads are in flight.
+    march<false>(d, PBCoef{}, src, bd, [&] {
+        const double S = fold_sum<kBlock>(pz, zcnt, sh);
+        const double beta = First ? 0.0 : S / st->rdv2[par ^ 1];
+        src.beta = beta;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->rdv2[par] = S;
+            st->rdv = S;
+            st->beta = beta;
+            st->first = 0;
+        }
+    });
     double s = bd.s, m = 0.0;
     block_sum_max_lds<kBlock>(s, m, sh);
     if (threadIdx.x == 0) pp[blockIdx.x] = s;
 }
 
 // Update with deferred finalisation: alpha = <r, z>_k / <p, A p> (folded from
-// the direction pass's partials; a vanishing <p, A p> stops the solve as the
-// breakdown guard of BodyDir does), x += alpha p, r += alpha A p, and plain
-// partials of <r, r> and max|r| for the consumer's stop test.  Work-group 0
-// advances st->iter (no other work-group of this kernel reads it).
+// the direction pass's partials while the first plane's loads are in flight),
+// x += alpha p, r += alpha A p, and plain partials of <r, r> and max|r| for
+// the consumer's stop test.  A vanishing <p, A p> stops the solve, as the
+// breakdown guard of BodyDir does (alpha 0: x and r keep their values).
+// Work-group 0 advances st->iter (no other work-group of this kernel reads it).
 template <bool ZeroX>
 __global__ __launch_bounds__(kBlock) void k_cg_update_df(GridDims d, const float* p, float* x,
                                                          float* r, const double* __restrict__ pp,
                                                          double* __restrict__ pus,
                                                          double* __restrict__ pum, int par,
                                                          CGState* st) {
-    if (st->done) return;
+    if (wg_done(st)) return;  // work-group 0 may record a breakdown meanwhile
     __shared__ double sh[2 * kBlock / kWave];
-    const double pAp = fold_sum<kBlock>(pp, d.blocks, sh);
-    const bool breakdown = fabs(pAp) < 1e-300;
-    const double alpha = breakdown ? 0.0 : st->rdv2[par] / pAp;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->pAp = pAp;
-        st->alpha = alpha;
-        if (breakdown) {
-            st->res = st->iter;
-            st->done = 1;
-        } else {
-            st->iter = st->iter + 1;
+    BodyUpdate<float, ZeroX> bd{x, r, 0.0};
+    march<false>(d, PBCoef{}, SrcArr<float>{p}, bd, [&] {
+        const double pAp = fold_sum<kBlock>(pp, d.blocks, sh);
+        const bool breakdown = fabs(pAp) < 1e-300;
+        const double alpha = breakdown ? 0.0 : st->rdv2[par] / pAp;
+        bd.alpha = alpha;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->pAp = pAp;
+            st->alpha = alpha;
+            if (breakdown) {
+                st->res = st->iter;
+                st->done = 1;
+            } else {
+                st->iter = st->iter + 1;
+            }
         }
-    }
-    if (breakdown) return;
-    BodyUpdate<float, ZeroX> bd{x, r, alpha};
-    march<false>(d, PBCoef{}, SrcArr<float>{p}, bd);
+    });
     double s = bd.s, m = bd.m;
     block_sum_max_lds<kBlock>(s, m, sh);
     if (threadIdx.x == 0) {

@@ -109,15 +109,27 @@ __device__ __forceinline__ PBPlane pb_load(const C& pb, int base, int pl, const 
     return r;
 }
 
-template <bool PB, class Src, class Body, class C>
-__device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src& src, Body& body) {
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+// hook: called once by every thread of the work-group (also those past the
+// end of the plane), at one point of uniform control flow, once the first
+// plane's loads are in flight and before any value is computed: a kernel that
+// derives a coefficient of its source or body from a reduction (deferred
+// finalisation) sets it there, so the fold's latency overlaps the loads.  (A
+// hook holds work-group barriers, so it must not sit in divergent code.)
+template <bool PB, class Src, class Body, class C, class Hook = NoHook>
+__device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src& src, Body& body,
+                                      Hook&& hook = Hook{}) {
     using SR = typename Src::Raw;
     using BR = typename Body::Raw;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int chunk = b % d.chunks;
     const int s = b / d.chunks;
-    const int pl = chunk * kBlock + threadIdx.x;
-    if (pl >= d.n2) return;
+    const int pl0 = chunk * kBlock + threadIdx.x;
+    const bool active = pl0 < d.n2;
+    const int pl = active ? pl0 : 0;  // threads past the end load nothing
     const int n = d.n, n2 = d.n2;
     const PlaneNb nb = plane_nb(pl, n, n2);
     const int i0 = s * d.slab;
@@ -143,10 +155,19 @@ __device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src&
         o.cf = pb_load<PB>(pb, base, pl, nb);
         return o;
     };
-    double vm = src.val(src.load(im0 * n2 + pl));
-    double vc = src.val(src.load(i0 * n2 + pl));
+    SR rm{}, rc{};
     double exm = 0.0;
-    if (PB) exm = pb.ex[im0 * n2 + pl];
+    Ops A{};
+    if (active) {
+        rm = src.load(im0 * n2 + pl);
+        rc = src.load(i0 * n2 + pl);
+        if (PB) exm = pb.ex[im0 * n2 + pl];
+        A = fetch(i0);
+    }
+    hook();
+    if (!active) return;
+    double vm = src.val(rm);
+    double vc = src.val(rc);
     auto compute = [&](int i, const Ops& o) {
         const double vp = src.val(o.p), vjm = src.val(o.jm), vjp = src.val(o.jp);
         const double vkm = src.val(o.km), vkp = src.val(o.kp);
@@ -164,7 +185,6 @@ __device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src&
         vc = vp;
     };
     // Two operand sets alternate (no register copies of pending loads).
-    Ops A = fetch(i0);
     for (int i = i0; i < i1; i += 2) {
         const Ops B = fetch(i + 1);
         compute(i, A);
@@ -313,11 +333,22 @@ __device__ __forceinline__ void fold_sum_max(const double* __restrict__ ps,
 // decides alike whenever work-group 0 writes; a stop recorded by an earlier
 // kernel (st->done, e.g. a breakdown) is the caller's to check.
 // sh: 2 NT / 64 doubles.
-template <int NT>
-__device__ __forceinline__ int df_stop_test(CGState* st, const double* pus, const double* pum,
-                                            int cnt, double* sh) {
-    double S, M;
-    fold_sum_max<NT>(pus, pum, cnt, sh, S, M);
+// Whether the solve has ended, read once per work-group (one barrier).  A
+// kernel in which work-group 0 may record a stop while other work-groups are
+// starting needs this form: were every wave to read st->done itself, the
+// waves of one work-group could see different values, and those that went on
+// would fold or use LDS contents that the waves that left were to write.
+__device__ __forceinline__ bool wg_done(const CGState* st) {
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = st->done;
+    lds_barrier();
+    return flag != 0;
+}
+
+// record: work-group 0 writes a stop into st.  A kernel whose own last
+// work-group rewrites the solver state (the refinement pass) must not record:
+// two work-groups writing the same fields in one kernel leave either value.
+__device__ __forceinline__ int df_decide(CGState* st, double S, double M, bool record = true) {
     const int k = st->iter;
     int res = INT_MIN;
     if (st->count_mode) {
@@ -327,13 +358,21 @@ __device__ __forceinline__ int df_stop_test(CGState* st, const double* pus, cons
         if (M <= st->tol) res = k - 1;
         else if (k >= st->limit) res = -1;
     }
-    if (res != INT_MIN && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (record && res != INT_MIN && blockIdx.x == 0 && threadIdx.x == 0) {
         st->rr = S;
         st->rmax = M;
         st->res = res;
         st->done = 1;
     }
     return res;
+}
+
+template <int NT>
+__device__ __forceinline__ int df_stop_test(CGState* st, const double* pus, const double* pum,
+                                            int cnt, double* sh, bool record = true) {
+    double S, M;
+    fold_sum_max<NT>(pus, pum, cnt, sh, S, M);
+    return df_decide(st, S, M, record);
 }
 
 // One sweep with an optional single-pass reduction of (sum, max) accumulated
