@@ -178,9 +178,11 @@ void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double
 // The multigrid-PCG direction and update with deferred finalisation (Poisson
 // operator).  pcg_dir: beta from the finest up kernel's partials (zcnt of
 // them; 0 for the first direction), writes <r, z> to rdv2[par] and its
-// <p, A p> partials.  update: alpha = rdv2[par] / <p, A p> from those
-// partials, x += alpha p, r += alpha A p, partials of <r, r> and max|r|; the
-// next consumer applies the stop test.
+// <p, A p> partials; after the first iteration it first applies the previous
+// update's stop test.  update: alpha = rdv2[par] / <p, A p> from those
+// partials, x += alpha p, r += alpha A p, partials of <r, r> and max|r| for
+// the next consumer's stop test (the next direction pass, the refinement
+// pass or launch_df_finalize).
 void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                        float* p_new, CGState* st, const DeferRed& dr, int zcnt, int first, int par);
 void launch_cg_update_df(hipStream_t s, const GridDims& d, const float* p, float* x, float* r,
@@ -232,11 +234,8 @@ bool mg_tile_ok(int tile, int n);
 const char* mg_tile_name(int tile);
 // Smoother weights c1, c2 (damping / 6): pre-smoothing sweeps c1 then c2,
 // post-smoothing c2 then c1 (polynomials in A, so the cycle stays symmetric).
-// stop: the finest level's down kernel of an inner iteration after the first
-// also applies the previous update's deferred stop test (nullptr: none).
 void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
-                    double c1, double c2, CGState* st, const DeferRed* stop = nullptr,
-                    int stop_cnt = 0);
+                    double c1, double c2, CGState* st);
 // dot: the finest level also writes its <b, d> partials to red.psum (one per
 // work-group, folded by the PCG direction pass; mg_up_blocks counts them).
 void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d2, const float* e,

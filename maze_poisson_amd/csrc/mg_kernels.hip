@@ -240,61 +240,12 @@ int mg_tiles(int n) {
     return ((n + TI - 1) / TI) * ((n + TJ - 1) / TJ) * ((n + TK - 1) / TK);
 }
 
-// Stop test of the previous inner iteration's update (deferred finalisation),
-// run by every work-group of the finest down kernel before its work.
-struct DownStop {
-    const double* pus;
-    const double* pum;
-    int cnt;
-};
-
-template <int NT>
-__device__ __forceinline__ bool down_stopped(CGState* st, const DownStop& ds) {
-    if (!ds.pus) return st->done != 0;
-    __shared__ double sh[2 * NT / kWave];
-    const bool stop = df_stop_test<NT>(st, ds.pus, ds.pum, ds.cnt, sh) != INT_MIN;
-    return stop || st->done;
-}
-
-// The same test in two halves, so that the partials' loads overlap the tile's
-// own: down_prefetch loads this thread's first entries (issued before the
-// tile loads, consumed after them), down_decide folds the rest and decides.
-// Work-group 0 records a stop while later work-groups may be starting, so a
-// kernel using these reads the flag of earlier kernels with wg_done (a
-// decision of the whole work-group) and down_decide reads no flag.
-struct DownPre {
-    double s, m;
-};
-
-template <int NT>
-__device__ __forceinline__ DownPre down_prefetch(const DownStop& ds) {
-    DownPre p{0.0, 0.0};
-    if (ds.pus && (int)threadIdx.x < ds.cnt) {
-        p.s = ds.pus[threadIdx.x];
-        p.m = ds.pum[threadIdx.x];
-    }
-    return p;
-}
-
-template <int NT>
-__device__ __forceinline__ bool down_decide(CGState* st, const DownStop& ds, DownPre p) {
-    __shared__ double sh[2 * NT / kWave];
-    for (int i = threadIdx.x + NT; i < ds.cnt; i += NT) {
-        p.s += ds.pus[i];
-        p.m = fmax(p.m, ds.pum[i]);
-    }
-    block_sum_max_lds<NT>(p.s, p.m, sh);
-    return df_decide(st, p.s, p.m) != INT_MIN;
-}
-
 template <int TI, int TJ, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__ b,
                                                     float* __restrict__ d2out,
                                                     float* __restrict__ bcoarse, float c1,
-                                                    float c2, CGState* st, DownStop ds) {
-    __shared__ int ended;
-    if (threadIdx.x == 0) ended = st->done;
-    const DownPre pre = down_prefetch<NT>(ds);
+                                                    float c2, const CGState* st) {
+    if (st->done) return;
     constexpr int BI = TI + 5, BJ = TJ + 5, BK = TK + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
     constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
@@ -315,12 +266,10 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
     if (tid >= 64 && tid < 64 + BJ) gj[tid - 64] = wrap_mod(T.j0 - 3 + tid - 64, n) * n;
     if (tid >= 128 && tid < 128 + BK) gk[tid - 128] = wrap_mod(T.k0 - 3 + tid - 128, n);
     lds_barrier();
-    if (ended) return;  // the whole work-group (the index tables need every wave)
     tile_for<BI * BJ * BK, NT>([&](int x) {
         const int k = x % BK, j = (x / BK) % BJ, i = x / (BK * BJ);
         sb[x] = b[gi[i] + gj[j] + gk[k]];
     });
-    if (ds.pus && down_decide<NT>(st, ds, pre)) return;
     lds_barrier();
     // Sweeps from zero with weights c1 then c2: d2 = -(c1 + c2) b - c1 c2 A b.
     const float cs = c1 + c2, cp = c1 * c2;
@@ -477,9 +426,8 @@ template <int TI, int TJ, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restrict__ b,
                                                     float* __restrict__ d2out,
                                                     float* __restrict__ bcoarse, float c1,
-                                                    float c2, CGState* st, DownStop ds) {
-    if (wg_done(st)) return;
-    const DownPre pre = down_prefetch<NT>(ds);
+                                                    float c2, const CGState* st) {
+    if (st->done) return;
     constexpr int BI = TI + 5, BJ = TJ + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
     constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
@@ -497,7 +445,6 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
         *reinterpret_cast<float4*>(&sb[r * RW + 4 * g]) =
             *reinterpret_cast<const float4*>(&b[(long)gi * n2 + gj * n + gk]);
     });
-    if (ds.pus && down_decide<NT>(st, ds, pre)) return;
     lds_barrier();
     // d2 on the D region (local k = fine k + 2, LDS column k + 2).
     const float cs = c1 + c2, cp = c1 * c2;
@@ -645,13 +592,13 @@ inline bool mg_vec_ok(int n) { return n >= 32 && n % 8 == 0; }
 // Sim takes the default per level (mg_default_tile) unless MAZE_MG_TILES names one.
 template <int TI, int TJ, int TK, int NT>
 static void down_t(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
-                   double c2, CGState* st, DownStop ds = DownStop{nullptr, nullptr, 0}) {
+                   double c2, CGState* st) {
     if (mg_vec_ok(n))
         MAZE_LAUNCH((k_mg_down_vec<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s,
-                    n, b, d2, bc, (float)c1, (float)c2, st, ds);
+                    n, b, d2, bc, (float)c1, (float)c2, st);
     else
         MAZE_LAUNCH((k_mg_down<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s, n,
-                    b, d2, bc, (float)c1, (float)c2, st, ds);
+                    b, d2, bc, (float)c1, (float)c2, st);
 }
 
 template <int TI, int TJ, int TK, int NT>
@@ -691,8 +638,8 @@ template <int S>
 __global__ __launch_bounds__(kBlock) void k_mg_down_march(int n, const float* __restrict__ b,
                                                           float* __restrict__ d2out,
                                                           float* __restrict__ bcoarse, float c1,
-                                                          float c2, CGState* st, DownStop ds) {
-    if (down_stopped<kBlock>(st, ds)) return;
+                                                          float c2, const CGState* st) {
+    if (st->done) return;
     constexpr int WB = kMarchC + 6, WD = kMarchC + 4, WR = kMarchC + 2;
     constexpr int PB = WB * WB, PD = WD * WD, PR = WR * WR;
     static_assert(PB <= 2 * kBlock, "two B-plane points per thread");
@@ -952,10 +899,10 @@ bool mg_tile_ok(int tile, int n) {
 
 template <int S>
 static void down_m(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
-                   double c2, CGState* st, DownStop ds) {
+                   double c2, CGState* st) {
     const int nblk = n / kMarchC;
     MAZE_LAUNCH((k_mg_down_march<S>), dim3(nblk * nblk * (n / S)), dim3(kBlock), 0, s, n, b, d2,
-                bc, (float)c1, (float)c2, st, ds);
+                bc, (float)c1, (float)c2, st);
 }
 
 template <int S>
@@ -978,23 +925,24 @@ const char* mg_tile_name(int tile) {
     return tile >= 0 && tile < kMgTiles ? names[tile] : "?";
 }
 
-// Large levels: 8x16x16 tiles with 1024 threads (a tile's steps are short
-// dependent chains, so a work-group needs many waves); small levels: 4x8x8
-// tiles with 512 threads (more work-groups for few points).
-int mg_default_tile(int n) { return n >= 64 ? kMgT8x16x16_1024 : kMgT4x8x8_512; }
+// Levels of 128^3 and more: 8x16x16 tiles with 512 threads (1024 or more
+// work-groups); smaller levels: 4x8x8 tiles with 512 threads, so that a 64^3
+// level still has 1024 work-groups (8x16x16 would leave half the CUs idle).
+// Measured at 4096 ions / 128^3 against 8x16x16 with 1024 threads on both
+// fine levels: 1718 against 1672 steps/s.
+int mg_default_tile(int n) { return n >= 128 ? kMgT8x16x16_512 : kMgT4x8x8_512; }
 
 void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
-                    double c1, double c2, CGState* st, const DeferRed* stop, int stop_cnt) {
+                    double c1, double c2, CGState* st) {
     if (!mg_tile_ok(tile, n)) tile = mg_default_tile(n);
-    const DownStop ds = stop ? DownStop{stop->pus, stop->pum, stop_cnt} : DownStop{nullptr, nullptr, 0};
     switch (tile) {
-        case kMgMarch16: down_m<16>(s, n, b, d2, bc, c1, c2, st, ds); break;
-        case kMgMarch8: down_m<8>(s, n, b, d2, bc, c1, c2, st, ds); break;
-        case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st, ds); break;
-        case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st, ds); break;
-        case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st, ds); break;
-        case kMgT4x8x8_512: down_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st, ds); break;
-        default: down_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st, ds); break;
+        case kMgMarch16: down_m<16>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgMarch8: down_m<8>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT4x8x8_512: down_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st); break;
+        default: down_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st); break;
     }
 }
 
@@ -1642,19 +1590,30 @@ void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const 
 // Direction pass with deferred finalisation: beta = <r, z>_k / <r, z>_(k-1),
 // both from partials (this iteration's from the finest up kernel, folded
 // here; the previous one kept in rdv2 by parity), then p = -z (first) or
-// beta p - z and the plain per-work-group partials of <p, A p>.
+// beta p - z and the plain per-work-group partials of <p, A p>.  After the
+// first iteration it also applies the previous update's stop test (its
+// partials pus, pum, ucnt of them): a stop ends the solve here, before p
+// changes (the V-cycle before it wrote only work arrays).  Work-group 0
+// records the stop while later work-groups may be starting, so the flag of
+// earlier kernels is read once per work-group.
 template <bool First>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const float* p_old,
                                                        const float* v, float* p_new,
                                                        const double* __restrict__ pz, int zcnt,
+                                                       const double* __restrict__ pus,
+                                                       const double* __restrict__ pum, int ucnt,
                                                        double* __restrict__ pp, int par,
                                                        CGState* st) {
-    if (st->done) return;
+    if (First ? st->done != 0 : wg_done(st)) return;
     __shared__ double sh[2 * kBlock / kWave];
     SrcPcgDir<First> src{p_old, v, 0.0};
     BodyPcgDir bd{p_new};
-    // beta is folded while the first plane's loads are in flight.
-    march<false>(d, PBCoef{}, src, bd, [&] {
+    // The folds run while the first plane's loads are in flight.
+    const bool go = march<false>(d, PBCoef{}, src, bd, [&] {
+        if (!First) {
+            __shared__ double shu[2 * kBlock / kWave];
+            if (df_stop_test<kBlock>(st, pus, pum, ucnt, shu) != INT_MIN) return false;
+        }
         const double S = fold_sum<kBlock>(pz, zcnt, sh);
         const double beta = First ? 0.0 : S / st->rdv2[par ^ 1];
         src.beta = beta;
@@ -1664,7 +1623,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const float* 
             st->beta = beta;
             st->first = 0;
         }
+        return true;
     });
+    if (!go) return;
     double s = bd.s, m = 0.0;
     block_sum_max_lds<kBlock>(s, m, sh);
     if (threadIdx.x == 0) pp[blockIdx.x] = s;
@@ -1700,6 +1661,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_df(GridDims d, const float
                 st->iter = st->iter + 1;
             }
         }
+        return true;
     });
     double s = bd.s, m = bd.m;
     block_sum_max_lds<kBlock>(s, m, sh);
@@ -1721,10 +1683,10 @@ void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const float* p_old, con
                        int par) {
     if (first)
         MAZE_LAUNCH((k_pcg_dir_df<true>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
-                    dr.pz, zcnt, dr.pp, par, st);
+                    dr.pz, zcnt, dr.pus, dr.pum, d.blocks, dr.pp, par, st);
     else
         MAZE_LAUNCH((k_pcg_dir_df<false>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
-                    dr.pz, zcnt, dr.pp, par, st);
+                    dr.pz, zcnt, dr.pus, dr.pum, d.blocks, dr.pp, par, st);
 }
 
 void launch_cg_update_df(hipStream_t s, const GridDims& d, const float* p, float* x, float* r,

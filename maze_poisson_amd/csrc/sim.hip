@@ -953,14 +953,8 @@ void Sim::vcycle(int l, int iter) {
         if (top) timed(timer, iter, f);
         else f();
     };
-    // The finest down kernel of an iteration after the first applies the
-    // previous update's deferred stop test; the finest up kernel writes the
-    // <r, z> partials the direction pass folds.
-    const bool stop = top && iter > 0;
-    t(kTMgDown, [&] {
-        launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st,
-                       stop ? &dred : nullptr, dims.blocks);
-    });
+    // The finest up kernel writes the <r, z> partials the direction pass folds.
+    t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st); });
     vcycle(l + 1, iter);
     t(kTMgUp, [&] {
         launch_mg_up(stream, L.n, L.tile_up, L.b, L.t, mg[l + 1].d, L.d, c1, c2, st, top,

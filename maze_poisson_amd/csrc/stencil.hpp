@@ -82,7 +82,8 @@ struct SrcZero {
 // ---- the i-marching stencil sweep -------------------------------------------
 // Calls body.apply(c, body.load(c), value_at_c, (A value)_c) for every point of
 // this work-group's tile.  Threads past the end of the plane do nothing but
-// stay for the work-group reductions.
+// stay for the work-group reductions.  Returns the hook's verdict (true
+// without a hook).
 //
 // Software-pipelined by one plane: the operands of plane i+1 are requested
 // before plane i is computed, so each thread has two planes of loads in
@@ -110,7 +111,7 @@ __device__ __forceinline__ PBPlane pb_load(const C& pb, int base, int pl, const 
 }
 
 struct NoHook {
-    __device__ void operator()() const {}
+    __device__ bool operator()() const { return true; }
 };
 
 // hook: called once by every thread of the work-group (also those past the
@@ -118,9 +119,11 @@ struct NoHook {
 // plane's loads are in flight and before any value is computed: a kernel that
 // derives a coefficient of its source or body from a reduction (deferred
 // finalisation) sets it there, so the fold's latency overlaps the loads.  (A
-// hook holds work-group barriers, so it must not sit in divergent code.)
+// hook holds work-group barriers, so it must not sit in divergent code.)  It
+// returns whether the sweep goes on; false (the same for the whole
+// work-group) ends it before any value is computed or stored.
 template <bool PB, class Src, class Body, class C, class Hook = NoHook>
-__device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src& src, Body& body,
+__device__ __forceinline__ bool march(const GridDims& d, const C& pb, const Src& src, Body& body,
                                       Hook&& hook = Hook{}) {
     using SR = typename Src::Raw;
     using BR = typename Body::Raw;
@@ -164,8 +167,8 @@ __device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src&
         if (PB) exm = pb.ex[im0 * n2 + pl];
         A = fetch(i0);
     }
-    hook();
-    if (!active) return;
+    const bool go = hook();
+    if (!active || !go) return go;
     double vm = src.val(rm);
     double vc = src.val(rc);
     auto compute = [&](int i, const Ops& o) {
@@ -192,6 +195,7 @@ __device__ __forceinline__ void march(const GridDims& d, const C& pb, const Src&
         A = fetch(i + 2);
         compute(i + 1, B);
     }
+    return true;
 }
 
 // Single-pass reduction of the (sum, max) a body accumulated: the last
