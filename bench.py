@@ -12,7 +12,10 @@ charge spreading, MaZe field solve, electrostatic + Tosi-Fumi forces,
 integrator part 2), all state resident in HBM.  Three field-solve paths:
   fp64      the reference algorithm (Jacobi CG) in FP64,
   mixed_mg  FP32 multigrid-preconditioned CG inside FP64 refinement (headline),
-  mixed_cg  FP32 Jacobi CG inside FP64 refinement.
+  mixed_cg  FP32 Jacobi CG inside FP64 refinement,
+  fp64_mg   FP64 CG preconditioned by the headline's V-cycle carried out in FP64
+            (the headline's algorithm without its FP32 storage: mixed_mg against
+            fp64_mg is the precision's gain, fp64_mg against fp64 the algorithm's).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -33,7 +36,7 @@ from maze_poisson_amd import driver, workloads as W  # noqa: E402
 
 N_IONS, N_GRID, SEED = 4096, 128, 2
 HBM_PEAK_GBS = 8000.0
-PATHS = {"fp64": 0, "mixed_cg": 1, "mixed_mg": 2}
+PATHS = {"fp64": 0, "mixed_cg": 1, "mixed_mg": 2, "fp64_mg": 3}
 
 # Algorithmic HBM bytes per grid point of each timed kernel (arrays read + written
 # once each; FP64 = 8 B, FP32 = 4 B).
@@ -541,6 +544,7 @@ def main():
         k = max(10, args.steps // 2)
         fp64 = run_path(lib, PATHS["fp64"], k, args.warmup, k, args.prof_steps)
         mcg = run_path(lib, PATHS["mixed_cg"], max(5, args.steps // 10), 2, 0, 0)
+        fmg = run_path(lib, PATHS["fp64_mg"], k, args.warmup, k, 0)
         # Force error of each path after 5 steps, against the FP64 solve converged
         # 1000x tighter from the same start.
         f_ref, _ = forces_after(lib, PATHS["fp64"], 1e-3, 5)
@@ -561,6 +565,9 @@ def main():
             "fp64_steps_per_s": round(fp64["steps_per_s"], 3),
             "fp64_p50_ms": round(fp64["p50_ms"], 4), "fp64_p99_ms": round(fp64["p99_ms"], 4),
             "mixed_cg_steps_per_s": round(mcg["steps_per_s"], 3),
+            "fp64_mg_steps_per_s": round(fmg["steps_per_s"], 3),
+            "fp64_mg_p50_ms": round(fmg["p50_ms"], 4),
+            "speedup_vs_fp64_mg": round(head["steps_per_s"] / fmg["steps_per_s"], 3),
             "speedup_vs_fp64": round(head["steps_per_s"] / fp64["steps_per_s"], 3),
             "force_rms_rel_err_vs_fp64_tight": errs,
             "solver_iters_per_step": iters,

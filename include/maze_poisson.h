@@ -193,8 +193,11 @@ void get_eps_k2(double *eps_x, double *eps_y, double *eps_z, double *k2);
 /* Field-solve arithmetic: 0 = FP64 CG as in the reference (default);
  * 1 = FP32-storage CG inside FP64 iterative refinement, stopping on the
  * reference's criterion max|A y - sigma| <= tol evaluated in FP64;
- * 2 = as 1 with the multigrid-preconditioned FP32 CG below.  Returns 0, or
- * -1 for an unknown mode. */
+ * 2 = as 1 with the multigrid-preconditioned FP32 CG below;
+ * 3 = FP64 CG preconditioned by mode 2's V-cycle carried out in FP64, to the
+ * same stop test (MAZE-LCG on the Poisson operator where mode 2 runs the
+ * one-work-group coarsest solve; otherwise the MaZe solve runs mode 0).
+ * Returns 0, or -1 for an unknown mode. */
 int solver_set_precision(int mode);
 /* Bandwidth in GB/s (bytes read + written) of the library's copy kernel over
  * n_doubles doubles, averaged over reps launches: the on-box copy rate that
@@ -210,7 +213,7 @@ int solver_get_precision(void);
  * stopped on its iteration budget) to iters[0 .. min(count, max)). */
 int solver_get_last_passes(int *iters, int max);
 /* Relative residual reduction of each FP32 inner solve; <= 0 restores the
- * default of the mode (1e-4 for mode 1, 1e-3 for mode 2). */
+ * default of the mode (1e-4 for mode 1, 1e-6 for mode 2). */
 void solver_set_inner_rtol(double rtol);
 /* Mixed-precision MaZe solve: start from the linear extrapolation
  * 2 y_n-1 - y_n-2 of the correction field (1) instead of y_n-1 (0, the

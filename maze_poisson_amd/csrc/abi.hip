@@ -189,7 +189,7 @@ void get_eps_k2(double* ex, double* ey, double* ez, double* k2) {
 }
 
 int solver_set_precision(int mode) {
-    if (mode < 0 || mode > 2) return -1;
+    if (mode < 0 || mode > 3) return -1;
     S().precision = mode;
     S().sched.clear();
     return 0;

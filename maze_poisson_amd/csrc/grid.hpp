@@ -173,6 +173,8 @@ constexpr int kMgCoarseLds = 16;
 constexpr int kMgCoarseMax = 128;
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,
                       const double* ilam, CGState* st);
+void launch_mg_coarse(hipStream_t s, int c, const double* b, double* e, const double* Qf,
+                      const double* ilam, CGState* st);
 void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,
                           const double* Qf, const double* ilam, CGState* st);
 // The multigrid-PCG direction and update with deferred finalisation (Poisson
@@ -186,6 +188,11 @@ void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double
 void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                        float* p_new, CGState* st, const DeferRed& dr, int zcnt, int first, int par);
 void launch_cg_update_df(hipStream_t s, const GridDims& d, const float* p, float* x, float* r,
+                         CGState* st, const DeferRed& dr, int zero_x, int par);
+// The same in FP64 (the FP64 multigrid-PCG).
+void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const double* p_old, const double* v,
+                       double* p_new, CGState* st, const DeferRed& dr, int zcnt, int first, int par);
+void launch_cg_update_df(hipStream_t s, const GridDims& d, const double* p, double* x, double* r,
                          CGState* st, const DeferRed& dr, int zero_x, int par);
 // Applies the deferred stop test of the last update (one work-group): for a
 // host read of the state in the middle of an inner solve.
@@ -241,6 +248,13 @@ void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, f
 void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d2, const float* e,
                   float* d, double c1, double c2, CGState* st, int dot, Reduce red);
 int mg_up_blocks(int n, int tile);
+// The level kernels in FP64 (a fixed tile shape per level size); the finest up
+// kernel's partials number mg_up_blocks64(n).
+void launch_mg_down(hipStream_t s, int n, const double* b, double* d2, double* bc, double c1,
+                    double c2, CGState* st);
+void launch_mg_up(hipStream_t s, int n, const double* b, const double* d2, const double* e,
+                  double* d, double c1, double c2, CGState* st, int dot, Reduce red);
+int mg_up_blocks64(int n);
 // Rest of the V-cycle from a small level (sizes n[0..nlev), finest first) down
 // to the exact solve and back, in one work-group with the data in LDS:
 // reads that level's right-hand side b, writes its correction d.

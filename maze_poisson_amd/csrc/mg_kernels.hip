@@ -240,11 +240,11 @@ int mg_tiles(int n) {
     return ((n + TI - 1) / TI) * ((n + TJ - 1) / TJ) * ((n + TK - 1) / TK);
 }
 
-template <int TI, int TJ, int TK, int NT>
-__global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__ b,
-                                                    float* __restrict__ d2out,
-                                                    float* __restrict__ bcoarse, float c1,
-                                                    float c2, const CGState* st) {
+template <int TI, int TJ, int TK, int NT, class R = float>
+__global__ __launch_bounds__(NT) void k_mg_down(int n, const R* __restrict__ b,
+                                                    R* __restrict__ d2out,
+                                                    R* __restrict__ bcoarse, R c1,
+                                                    R c2, const CGState* st) {
     if (st->done) return;
     constexpr int BI = TI + 5, BJ = TJ + 5, BK = TK + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
@@ -255,9 +255,9 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
     // consecutive LDS words in every pass: d2 at (i, j, q) = D point (i, j,
     // q - 1), the residual at (i, j, q) = R point (i, j, q - 2); positions of a
     // row outside the region are skipped.
-    __shared__ float sb[BI * BJ * BK];
-    __shared__ float sd[DI * DJ * BK];
-    __shared__ float sr[RI * RJ * BK];
+    __shared__ R sb[BI * BJ * BK];
+    __shared__ R sd[DI * DJ * BK];
+    __shared__ R sr[RI * RJ * BK];
     __shared__ int gi[BI], gj[BJ], gk[BK];
     const MgTileMap<TI, TJ, TK> T(n);
     const int tid = threadIdx.x;
@@ -272,14 +272,14 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
     });
     lds_barrier();
     // Sweeps from zero with weights c1 then c2: d2 = -(c1 + c2) b - c1 c2 A b.
-    const float cs = c1 + c2, cp = c1 * c2;
+    const R cs = c1 + c2, cp = c1 * c2;
     tile_for<DI * DJ * BK, NT>([&](int x) {
         const int q = x % BK, j = (x / BK) % DJ, i = x / (BK * DJ);
         if (q < 1 || q > DK) return;
         const int o = ((i + 1) * BJ + (j + 1)) * BK + q;
-        const float bc = sb[o];
-        const float lap = sb[o - BJ * BK] + sb[o + BJ * BK] + sb[o - BK] + sb[o + BK] + sb[o - 1] +
-                          sb[o + 1] - 6.0f * bc;
+        const R bc = sb[o];
+        const R lap = sb[o - BJ * BK] + sb[o + BJ * BK] + sb[o - BK] + sb[o + BK] + sb[o - 1] +
+                          sb[o + 1] - R(6) * bc;
         sd[x] = -cs * bc - cp * lap;
     });
     lds_barrier();
@@ -287,9 +287,9 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
         const int q = x % BK, j = (x / BK) % RJ, i = x / (BK * RJ);
         if (q < 2 || q > RK + 1) return;
         const int o = ((i + 1) * DJ + (j + 1)) * BK + q;
-        const float dc = sd[o];
-        const float lap = sd[o - DJ * BK] + sd[o + DJ * BK] + sd[o - BK] + sd[o + BK] + sd[o - 1] +
-                          sd[o + 1] - 6.0f * dc;
+        const R dc = sd[o];
+        const R lap = sd[o - DJ * BK] + sd[o + DJ * BK] + sd[o - BK] + sd[o + BK] + sd[o - 1] +
+                          sd[o + 1] - R(6) * dc;
         sr[x] = sb[((i + 2) * BJ + (j + 2)) * BK + q] - lap;
     });
     // Owned d2 (this loop and the residual pass only read sd).
@@ -306,35 +306,35 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const float* __restrict__
         const int gI = T.i0 / 2 + I, gJ = T.j0 / 2 + J, gK = T.k0 / 2 + K;
         if (gI >= nc || gJ >= nc || gK >= nc) return;
         const int o = ((2 * I + 1) * RJ + (2 * J + 1)) * BK + (2 * K + 3);
-        float acc = 0.0f;
+        R acc = R(0);
 #pragma unroll
         for (int a = -1; a <= 1; a++)
 #pragma unroll
             for (int bb = -1; bb <= 1; bb++)
 #pragma unroll
                 for (int cc = -1; cc <= 1; cc++) {
-                    const float w = (float)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
+                    const R w = (R)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
                     acc += w * sr[o + (a * RJ + bb) * BK + cc];
                 }
-        bcoarse[(gI * nc + gJ) * nc + gK] = acc * (4.0f / 64.0f);
+        bcoarse[(gI * nc + gJ) * nc + gK] = acc * (R(4) / R(64));
     });
 }
 
-template <int TI, int TJ, int TK, int NT, bool Dot>
-__global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b,
-                                                  const float* __restrict__ d2,
-                                                  const float* __restrict__ e,
-                                                  float* __restrict__ dout, float c1, float c2,
+template <int TI, int TJ, int TK, int NT, bool Dot, class R = float>
+__global__ __launch_bounds__(NT) void k_mg_up(int n, const R* __restrict__ b,
+                                                  const R* __restrict__ d2,
+                                                  const R* __restrict__ e,
+                                                  R* __restrict__ dout, R c1, R c2,
                                                   Reduce red,
                                                   CGState* st) {
     if (st->done) return;
     constexpr int DI = TI + 4, DJ = TJ + 4, DK = TK + 4;
     constexpr int PI = TI + 2, PJ = TJ + 2, PK = TK + 2;
     constexpr int EI = TI / 2 + 3, EJ = TJ / 2 + 3, EK = TK / 2 + 3;
-    __shared__ float sd[DI * DJ * DK];
-    __shared__ float sp[PI * PJ * PK];
-    __shared__ float sb[PI * PJ * PK];
-    __shared__ float se[EI * EJ * EK];
+    __shared__ R sd[DI * DJ * DK];
+    __shared__ R sp[PI * PJ * PK];
+    __shared__ R sb[PI * PJ * PK];
+    __shared__ R se[EI * EJ * EK];
     __shared__ int gi[DI], gj[DJ], gk[DK], ci[EI], cj[EJ], ck[EK];
     const MgTileMap<TI, TJ, TK> T(n);
     const int tid = threadIdx.x;
@@ -364,27 +364,27 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
     tile_for<DI * DJ * DK, NT>([&](int x) {
         const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
         const int a = i >> 1, bb = j >> 1, cc = k >> 1;
-        const float wi = (i & 1) ? 0.5f : 0.0f, wj = (j & 1) ? 0.5f : 0.0f,
-                    wk = (k & 1) ? 0.5f : 0.0f;
+        const R wi = (i & 1) ? R(0.5) : R(0), wj = (j & 1) ? R(0.5) : R(0),
+                    wk = (k & 1) ? R(0.5) : R(0);
         const int o = (a * EJ + bb) * EK + cc;
-        const float e000 = se[o], e001 = se[o + 1], e010 = se[o + EK], e011 = se[o + EK + 1];
-        const float e100 = se[o + EJ * EK], e101 = se[o + EJ * EK + 1];
-        const float e110 = se[o + EJ * EK + EK], e111 = se[o + EJ * EK + EK + 1];
-        const float v00 = e000 + wk * (e001 - e000);
-        const float v01 = e010 + wk * (e011 - e010);
-        const float v10 = e100 + wk * (e101 - e100);
-        const float v11 = e110 + wk * (e111 - e110);
-        const float v0 = v00 + wj * (v01 - v00);
-        const float v1 = v10 + wj * (v11 - v10);
+        const R e000 = se[o], e001 = se[o + 1], e010 = se[o + EK], e011 = se[o + EK + 1];
+        const R e100 = se[o + EJ * EK], e101 = se[o + EJ * EK + 1];
+        const R e110 = se[o + EJ * EK + EK], e111 = se[o + EJ * EK + EK + 1];
+        const R v00 = e000 + wk * (e001 - e000);
+        const R v01 = e010 + wk * (e011 - e010);
+        const R v10 = e100 + wk * (e101 - e100);
+        const R v11 = e110 + wk * (e111 - e110);
+        const R v0 = v00 + wj * (v01 - v00);
+        const R v1 = v10 + wj * (v11 - v10);
         sd[x] = sd[x] + (v0 + wi * (v1 - v0));
     });
     lds_barrier();
     tile_for<PI * PJ * PK, NT>([&](int x) {
         const int k = x % PK, j = (x / PK) % PJ, i = x / (PK * PJ);
         const int o = ((i + 1) * DJ + (j + 1)) * DK + (k + 1);
-        const float dc = sd[o];
-        const float lap = sd[o - DJ * DK] + sd[o + DJ * DK] + sd[o - DK] + sd[o + DK] + sd[o - 1] +
-                          sd[o + 1] - 6.0f * dc;
+        const R dc = sd[o];
+        const R lap = sd[o - DJ * DK] + sd[o + DJ * DK] + sd[o - DK] + sd[o + DK] + sd[o - 1] +
+                          sd[o + 1] - R(6) * dc;
         sp[x] = dc - c2 * (sb[x] - lap);
     });
     lds_barrier();
@@ -392,11 +392,11 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const float* __restrict__ b
     tile_for<TI * TJ * TK, NT>([&](int x) {
         const int k = x % TK, j = (x / TK) % TJ, i = x / (TK * TJ);
         const int o = ((i + 1) * PJ + (j + 1)) * PK + (k + 1);
-        const float pc = sp[o];
-        const float lap = sp[o - PJ * PK] + sp[o + PJ * PK] + sp[o - PK] + sp[o + PK] + sp[o - 1] +
-                          sp[o + 1] - 6.0f * pc;
-        const float bc = sb[o];
-        const float z = pc - c1 * (bc - lap);
+        const R pc = sp[o];
+        const R lap = sp[o - PJ * PK] + sp[o + PJ * PK] + sp[o - PK] + sp[o + PK] + sp[o - 1] +
+                          sp[o + 1] - R(6) * pc;
+        const R bc = sb[o];
+        const R z = pc - c1 * (bc - lap);
         if (T.i0 + i < n && T.j0 + j < n && T.k0 + k < n) {
             dout[(T.i0 + i) * n2 + (T.j0 + j) * n + T.k0 + k] = z;
             if (Dot) s += (double)bc * (double)z;
@@ -973,6 +973,43 @@ void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d
     }
 }
 
+// The level kernels in FP64 (the FP64 multigrid-PCG): the scalar kernels with
+// double operands in two tile shapes, 8x8x16 with 512 threads from 128^3 (62
+// and 56 KB of LDS for down and up) and 4x8x8 with 512 threads below.
+int mg_up_blocks64(int n) { return n >= 128 ? mg_tiles<8, 8, 16>(n) : mg_tiles<4, 8, 8>(n); }
+
+void launch_mg_down(hipStream_t s, int n, const double* b, double* d2, double* bc, double c1,
+                    double c2, CGState* st) {
+    if (n >= 128)
+        MAZE_LAUNCH((k_mg_down<8, 8, 16, 512, double>), dim3(mg_tiles<8, 8, 16>(n)), dim3(512), 0,
+                    s, n, b, d2, bc, c1, c2, st);
+    else
+        MAZE_LAUNCH((k_mg_down<4, 8, 8, 512, double>), dim3(mg_tiles<4, 8, 8>(n)), dim3(512), 0, s,
+                    n, b, d2, bc, c1, c2, st);
+}
+
+void launch_mg_up(hipStream_t s, int n, const double* b, const double* d2, const double* e,
+                  double* d, double c1, double c2, CGState* st, int dot, Reduce red) {
+    const dim3 blk(512);
+    if (n >= 128) {
+        const dim3 g(mg_tiles<8, 8, 16>(n));
+        if (dot)
+            MAZE_LAUNCH((k_mg_up<8, 8, 16, 512, true, double>), g, blk, 0, s, n, b, d2, e, d, c1,
+                        c2, red, st);
+        else
+            MAZE_LAUNCH((k_mg_up<8, 8, 16, 512, false, double>), g, blk, 0, s, n, b, d2, e, d, c1,
+                        c2, red, st);
+    } else {
+        const dim3 g(mg_tiles<4, 8, 8>(n));
+        if (dot)
+            MAZE_LAUNCH((k_mg_up<4, 8, 8, 512, true, double>), g, blk, 0, s, n, b, d2, e, d, c1,
+                        c2, red, st);
+        else
+            MAZE_LAUNCH((k_mg_up<4, 8, 8, 512, false, double>), g, blk, 0, s, n, b, d2, e, d, c1,
+                        c2, red, st);
+    }
+}
+
 // ---- exact FP64 solve on the coarsest grid -----------------------------------
 // A e = b on a c^3 grid (c <= 12): forward transform along k, j, i with the
 // real orthonormal eigenbasis Qf (row t = eigenvector t), divide by the
@@ -1041,9 +1078,9 @@ __device__ void coarse_exact(int c, const double* Q, const double* __restrict__ 
 // CT > 0 fixes c at compile time (fully unrolled line loops); CT = 0 reads it
 // from c_arg.  The inverse eigenvalues of a thread's points are loaded first
 // and kept in registers.
-template <int CT, class T>
+template <int CT, class T, class IO = float>
 __global__ __launch_bounds__(4 * kMgCoarseLds * kMgCoarseLds) void k_mg_coarse(
-    int c_arg, const float* __restrict__ b, float* __restrict__ e, const double* __restrict__ Qf,
+    int c_arg, const IO* __restrict__ b, IO* __restrict__ e, const double* __restrict__ Qf,
     const double* __restrict__ ilam, const CGState* st) {
     if (st->done) return;
     constexpr int CM = kMgCoarseLds, NT = 4 * CM * CM, PER = CM * CM * CM / NT;
@@ -1098,7 +1135,7 @@ __global__ __launch_bounds__(4 * kMgCoarseLds * kMgCoarseLds) void k_mg_coarse(
     pass(B, A, 2, true);
     pass(A, B, 1, true);
     pass(B, A, 0, true);
-    for (int x = tid; x < N; x += NT) e[x] = (float)A[(x / c) * pitch + x % c];
+    for (int x = tid; x < N; x += NT) e[x] = (IO)A[(x / c) * pitch + x % c];
 }
 
 // The same solve with each axis pass as a matrix product on the matrix cores:
@@ -1217,6 +1254,22 @@ void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const doub
         MAZE_LAUNCH((k_mg_coarse<kMgCoarseLds, float>), grid, block, 0, s, c, b, e, Qf, ilam, st);
     else
         MAZE_LAUNCH((k_mg_coarse<0, float>), grid, block, 0, s, c, b, e, Qf, ilam, st);
+}
+
+// FP64 form (the FP64 multigrid-PCG): the vector-ALU transform in FP64.
+void launch_mg_coarse(hipStream_t s, int c, const double* b, double* e, const double* Qf,
+                      const double* ilam, CGState* st) {
+    if (c > kMgCoarseLds) {
+        std::fprintf(stderr, "maze: coarsest level %d above %d for the one-work-group solve\n", c,
+                     kMgCoarseLds);
+        std::exit(1);
+    }
+    const dim3 grid(1), block(4 * kMgCoarseLds * kMgCoarseLds);
+    if (c == kMgCoarseLds)
+        MAZE_LAUNCH((k_mg_coarse<kMgCoarseLds, double, double>), grid, block, 0, s, c, b, e, Qf,
+                    ilam, st);
+    else
+        MAZE_LAUNCH((k_mg_coarse<0, double, double>), grid, block, 0, s, c, b, e, Qf, ilam, st);
 }
 
 // ---- exact solve on a larger coarsest grid (kMgCoarseLds < c <= kMgCoarseMax) --
@@ -1508,29 +1561,32 @@ void launch_mg_tail(hipStream_t s, int nlev, const int* n, const float* b, float
 
 // PCG direction p = -v (first) or beta p - v, with <p, A p>
 // (library/src/plcg.c:130-155 form, v = M^-1 r).
-template <bool First>
+// R: the storage type of p and v (float, or double for the FP64 multigrid-PCG);
+// the value is rounded to it so that every thread sees the stored value.
+template <class R, bool First>
 struct SrcPcgDir {
-    const float* p;
-    const float* v;
+    const R* p;
+    const R* v;
     double beta;
-    struct Raw { float p, v; };
+    struct Raw { R p, v; };
     __device__ __forceinline__ Raw load(int i) const {
-        if (First) return Raw{0.0f, v[i]};
+        if (First) return Raw{R(0), v[i]};
         return Raw{p[i], v[i]};
     }
     __device__ __forceinline__ double val(const Raw& x) const {
         const double w = First ? -(double)x.v : (double)x.p * beta - (double)x.v;
-        return (double)(float)w;
+        return (double)(R)w;
     }
 };
 
+template <class R>
 struct BodyPcgDir {
-    float* p_new;
+    R* p_new;
     double s = 0.0, m = 0.0;
     using Raw = NoRaw;
     __device__ Raw load(int) const { return Raw{}; }
     __device__ void apply(int c, const Raw&, double v, double lap) {
-        p_new[c] = (float)v;
+        p_new[c] = (R)v;
         s += v * lap;
     }
     __device__ static void finalize(CGState* st, double S) {
@@ -1551,15 +1607,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, C pb, const floa
                                                     const float* v, float* p_new, Reduce red,
                                                     CGState* st) {
     if (st->done) return;
-    const SrcPcgDir<First> src{p_old, v, First ? 0.0 : st->beta};
-    BodyPcgDir bd{p_new};
+    const SrcPcgDir<float, First> src{p_old, v, First ? 0.0 : st->beta};
+    BodyPcgDir<float> bd{p_new};
     march<PB>(d, pb, src, bd);
     __shared__ double sh[kBlock / kWave];
     const double S = block_sum(bd.s, sh);
     if (publish_arrive(red.psum, S, nullptr, 0.0, red.counter, gridDim.x)) {
         const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
         if (threadIdx.x == 0) {
-            BodyPcgDir::finalize(st, T);
+            BodyPcgDir<float>::finalize(st, T);
             reset_counter(red.counter);
         }
     }
@@ -1596,9 +1652,9 @@ void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const 
 // changes (the V-cycle before it wrote only work arrays).  Work-group 0
 // records the stop while later work-groups may be starting, so the flag of
 // earlier kernels is read once per work-group.
-template <bool First>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const float* p_old,
-                                                       const float* v, float* p_new,
+template <bool First, class R = float>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const R* p_old,
+                                                       const R* v, R* p_new,
                                                        const double* __restrict__ pz, int zcnt,
                                                        const double* __restrict__ pus,
                                                        const double* __restrict__ pum, int ucnt,
@@ -1606,8 +1662,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const float* 
                                                        CGState* st) {
     if (First ? st->done != 0 : wg_done(st)) return;
     __shared__ double sh[2 * kBlock / kWave];
-    SrcPcgDir<First> src{p_old, v, 0.0};
-    BodyPcgDir bd{p_new};
+    SrcPcgDir<R, First> src{p_old, v, 0.0};
+    BodyPcgDir<R> bd{p_new};
     // The folds run while the first plane's loads are in flight.
     const bool go = march<false>(d, PBCoef{}, src, bd, [&] {
         if (!First) {
@@ -1637,16 +1693,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const float* 
 // the consumer's stop test.  A vanishing <p, A p> stops the solve, as the
 // breakdown guard of BodyDir does (alpha 0: x and r keep their values).
 // Work-group 0 advances st->iter (no other work-group of this kernel reads it).
-template <bool ZeroX>
-__global__ __launch_bounds__(kBlock) void k_cg_update_df(GridDims d, const float* p, float* x,
-                                                         float* r, const double* __restrict__ pp,
+template <bool ZeroX, class R = float>
+__global__ __launch_bounds__(kBlock) void k_cg_update_df(GridDims d, const R* p, R* x,
+                                                         R* r, const double* __restrict__ pp,
                                                          double* __restrict__ pus,
                                                          double* __restrict__ pum, int par,
                                                          CGState* st) {
     if (wg_done(st)) return;  // work-group 0 may record a breakdown meanwhile
     __shared__ double sh[2 * kBlock / kWave];
-    BodyUpdate<float, ZeroX> bd{x, r, 0.0};
-    march<false>(d, PBCoef{}, SrcArr<float>{p}, bd, [&] {
+    BodyUpdate<R, ZeroX> bd{x, r, 0.0};
+    march<false>(d, PBCoef{}, SrcArr<R>{p}, bd, [&] {
         const double pAp = fold_sum<kBlock>(pp, d.blocks, sh);
         const bool breakdown = fabs(pAp) < 1e-300;
         const double alpha = breakdown ? 0.0 : st->rdv2[par] / pAp;
@@ -1678,25 +1734,48 @@ __global__ __launch_bounds__(kBlock) void k_df_finalize(CGState* st, const doubl
     (void)df_stop_test<kBlock>(st, pus, pum, cnt, sh);
 }
 
+template <class R>
+static void pcg_dir_df_t(hipStream_t s, const GridDims& d, const R* p_old, const R* v, R* p_new,
+                         CGState* st, const DeferRed& dr, int zcnt, int first, int par) {
+    if (first)
+        MAZE_LAUNCH((k_pcg_dir_df<true, R>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v,
+                    p_new, dr.pz, zcnt, dr.pus, dr.pum, d.blocks, dr.pp, par, st);
+    else
+        MAZE_LAUNCH((k_pcg_dir_df<false, R>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v,
+                    p_new, dr.pz, zcnt, dr.pus, dr.pum, d.blocks, dr.pp, par, st);
+}
+
+template <class R>
+static void cg_update_df_t(hipStream_t s, const GridDims& d, const R* p, R* x, R* r, CGState* st,
+                           const DeferRed& dr, int zero_x, int par) {
+    if (zero_x)
+        MAZE_LAUNCH((k_cg_update_df<true, R>), dim3(d.blocks), dim3(kBlock), 0, s, d, p, x, r,
+                    dr.pp, dr.pus, dr.pum, par, st);
+    else
+        MAZE_LAUNCH((k_cg_update_df<false, R>), dim3(d.blocks), dim3(kBlock), 0, s, d, p, x, r,
+                    dr.pp, dr.pus, dr.pum, par, st);
+}
+
 void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const float* p_old, const float* v,
                        float* p_new, CGState* st, const DeferRed& dr, int zcnt, int first,
                        int par) {
-    if (first)
-        MAZE_LAUNCH((k_pcg_dir_df<true>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
-                    dr.pz, zcnt, dr.pus, dr.pum, d.blocks, dr.pp, par, st);
-    else
-        MAZE_LAUNCH((k_pcg_dir_df<false>), dim3(d.blocks), dim3(kBlock), 0, s, d, p_old, v, p_new,
-                    dr.pz, zcnt, dr.pus, dr.pum, d.blocks, dr.pp, par, st);
+    pcg_dir_df_t(s, d, p_old, v, p_new, st, dr, zcnt, first, par);
+}
+
+void launch_pcg_dir_df(hipStream_t s, const GridDims& d, const double* p_old, const double* v,
+                       double* p_new, CGState* st, const DeferRed& dr, int zcnt, int first,
+                       int par) {
+    pcg_dir_df_t(s, d, p_old, v, p_new, st, dr, zcnt, first, par);
 }
 
 void launch_cg_update_df(hipStream_t s, const GridDims& d, const float* p, float* x, float* r,
                          CGState* st, const DeferRed& dr, int zero_x, int par) {
-    if (zero_x)
-        MAZE_LAUNCH((k_cg_update_df<true>), dim3(d.blocks), dim3(kBlock), 0, s, d, p, x, r, dr.pp,
-                    dr.pus, dr.pum, par, st);
-    else
-        MAZE_LAUNCH((k_cg_update_df<false>), dim3(d.blocks), dim3(kBlock), 0, s, d, p, x, r, dr.pp,
-                    dr.pus, dr.pum, par, st);
+    cg_update_df_t(s, d, p, x, r, st, dr, zero_x, par);
+}
+
+void launch_cg_update_df(hipStream_t s, const GridDims& d, const double* p, double* x, double* r,
+                         CGState* st, const DeferRed& dr, int zero_x, int par) {
+    cg_update_df_t(s, d, p, x, r, st, dr, zero_x, par);
 }
 
 void launch_df_finalize(hipStream_t s, CGState* st, const DeferRed& dr, int cnt) {

@@ -122,6 +122,7 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     if (const char* e = std::getenv("MAZE_TF_OVERLAP")) tf_overlap = std::atoi(e) != 0;
     if (const char* e = std::getenv("MAZE_TF_LATE")) tf_late = std::atoi(e) != 0;
     if (const char* e = std::getenv("MAZE_INNER_MAX")) inner_max = std::atoi(e);
+    if (const char* e = std::getenv("MAZE_INNER_RTOL")) inner_rtol = std::atof(e);  // tuning
     st = dalloc<CGState>(1);
     st_c = dalloc<CGState>(1);
     MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
@@ -696,6 +697,13 @@ void Sim::free_mg() {
         dfree(mg[l].t);
     }
     mg.clear();
+    for (size_t l = 0; l < mg64.size(); l++) {
+        if (l > 0) dfree(mg64[l].b);
+        dfree(mg64[l].d);
+        dfree(mg64[l].t);
+    }
+    mg64.clear();
+    mg64_ok = -1;
     dfree(mg_Qf);
     dfree(mg_ilam);
     dfree(mg_w0);
@@ -986,6 +994,97 @@ int Sim::fp64_finish() {
     return res;
 }
 
+// ---- FP64 multigrid-PCG (precision kFP64MG) ----------------------------------
+// The FP32 hierarchy's level sizes with FP64 arrays.  Unavailable for PB, for
+// grids without the FP32 multigrid, and for the hierarchies whose smallest
+// levels run in the one-work-group tail or whose coarsest level is above 16^3
+// (those solves stay FP32); the MaZe step then runs the reference's FP64 CG.
+bool Sim::setup_mg64() {
+    if (mg64_ok >= 0) return mg64_ok == 1;
+    mg64_ok = 0;
+    if (pb || !setup_mg() || mg_tail < (int)mg.size() || mg_c > kMgCoarseLds) return false;
+    for (size_t l = 0; l < mg.size(); l++) {
+        MGLevel64 L;
+        L.n = mg[l].n;
+        const long g = (long)L.n * L.n * L.n;
+        L.b = l == 0 ? nullptr : dalloc<double>(g);
+        L.d = dalloc<double>(g);
+        L.t = dalloc<double>(g);
+        mg64.push_back(L);
+    }
+    mg64_ok = 1;
+    return true;
+}
+
+bool Sim::use_fp64mg() {
+    return precision == kFP64MG && grid_type == kMazeLCG && setup_mg64();
+}
+
+// vcycle() with FP64 operands: V(2,2) with the same sweep weights, the
+// 27-point restriction, trilinear prolongation and the exact coarsest solve;
+// the finest up kernel writes the <r, z> partials the direction pass folds.
+void Sim::vcycle64(int l) {
+    MGLevel64& L = mg64[l];
+    const double c1 = mg_omega / 6.0, c2 = mg_omega2 / 6.0;
+    if (l == (int)mg64.size() - 1) {
+        launch_mg_coarse(stream, mg_c, L.b, L.d, mg_Qf, mg_ilam, st);
+        return;
+    }
+    launch_mg_down(stream, L.n, L.b, L.t, mg64[l + 1].b, c1, c2, st);
+    vcycle64(l + 1);
+    launch_mg_up(stream, L.n, L.b, L.t, mg64[l + 1].d, L.d, c1, c2, st, l == 0 ? 1 : 0,
+                 Reduce{dred.pz, nullptr, nullptr});
+}
+
+// MaZe step with the FP64 multigrid-PCG: the prediction of library/src/
+// verlet.c:31-71, then the explicit-preconditioner CG of library/src/
+// plcg.c:104-166 (r = A y - sigma, p = -z, x += alpha p, r += alpha A p) with
+// P^-1 = vcycle64, from the previous solution y to max|r| <= tol, with the
+// deferred finalisation of the FP32 inner iteration.  The previous step's
+// iteration count is queued at once; further iterations are polled.
+void Sim::fp64mg_issue() {
+    const double coef = (4 * kPi) / (h * eps_s);
+    timed(kTPred, 0,
+          [&] { launch_pred_sigma(stream, dims, pbc(), phi, phip, q, coef, tmp, sigma); });
+    double* old_phip = phip;
+    phip = phi;
+    phi = tmp;
+    tmp = old_phip;
+    mg64[0].b = r;
+    launch_cg_reset(stream, st, tol, 0.0, n * n, 0, 1);
+    launch_cg_init<double>(stream, dims, pbc(), y, sigma, r, st, red);
+    mg64_launched = 0;
+    fp64mg_chunk(mg64_pred);
+    launch_df_finalize(stream, st, dred, dims.blocks);
+    publish();
+}
+
+void Sim::fp64mg_chunk(int count) {
+    const int zcnt = mg_up_blocks64(n);
+    if (zcnt > kMaxPartials) fatal("maze: finest multigrid tiles exceed the reduction buffers");
+    for (int c = 0; c < count && mg64_launched < n * n; c++, mg64_launched++) {
+        const int it = mg64_launched;
+        double* pin = (it & 1) ? p1 : p0;
+        double* pout = (it & 1) ? p0 : p1;
+        vcycle64(0);
+        launch_pcg_dir_df(stream, dims, pin, mg64[0].d, pout, st, dred, zcnt, it == 0, it & 1);
+        launch_cg_update_df(stream, dims, pout, y, r, st, dred, 0, it & 1);
+    }
+}
+
+int Sim::fp64mg_finish() {
+    wait_published();
+    while (!st_host->done && mg64_launched < n * n) {
+        fp64mg_chunk(1);
+        launch_df_finalize(stream, st, dred, dims.blocks);
+        read_state();
+    }
+    const int res = st_host->res;
+    mg64_pred = std::max(1, res >= 0 ? res + 1 : mg64_pred);
+    launch_axpy(stream, G, -1.0, y, phi);
+    return res;
+}
+
 // MaZe step with FP32-storage inner solves inside FP64 iterative refinement:
 // each refinement pass recomputes the true residual sigma - A y in FP64, and
 // the loop stops on the reference's criterion max|A y - sigma| <= tol; each
@@ -1026,9 +1125,8 @@ void Sim::mixed_issue() {
     phip = phi;
     phi = tmp;
     tmp = old_phip;
-    m.inner_limit = inner_max > 0 ? inner_max : n * n;
-    m.ra = RefineArgs{tol, inner_rtol > 0 ? inner_rtol : (m.use_mg ? 1e-3 : 1e-4), m.inner_limit,
-                      m.use_mg ? 1 : 0};
+    m.inner_limit = inner_limit_for(m.use_mg);
+    m.ra = RefineArgs{tol, inner_rtol_for(m.use_mg), m.inner_limit, m.use_mg ? 1 : 0};
     // The two buffers the refinement passes alternate between (st->ysel).
     m.extrap = y_extrap && hist_ok;
     m.ya = m.extrap ? y_alt : y;
@@ -1300,7 +1398,8 @@ void Sim::init_field_issue() {
     launch_copy(stream, G, phi, phip);
     launch_scale_copy(stream, G, c, q, sigma);
     launch_fill(stream, G, 0.0, phi);
-    cold_mixed = mixed_init && precision != kFP64 && grid_type == kMazeLCG;
+    cold_mixed = mixed_init && (precision == kMixedCG || precision == kMixedMG) &&
+                 grid_type == kMazeLCG;
     if (cold_mixed) {
         cold_issue();
         return;
@@ -1335,9 +1434,8 @@ void Sim::cold_issue() {
         pbmg_coefs();
         pbmg[0].b = r32;
     }
-    m.inner_limit = inner_max > 0 ? inner_max : n * n;
-    m.ra = RefineArgs{tol, inner_rtol > 0 ? inner_rtol : (m.use_mg ? 1e-3 : 1e-4), m.inner_limit,
-                      m.use_mg ? 1 : 0};
+    m.inner_limit = inner_limit_for(m.use_mg);
+    m.ra = RefineArgs{tol, inner_rtol_for(m.use_mg), m.inner_limit, m.use_mg ? 1 : 0};
     m.extrap = false;
     m.ya = phi;
     m.yb = tmp;
@@ -1368,7 +1466,12 @@ int Sim::update_field() {
             return lcg_update();
         case kMazeLCG:
             if (precision == kFP64 && precond == kPMG && !pb) return maze_pcg_mg();
-            return precision == kFP64 ? maze_fp64() : maze_mixed();  // kMixedCG / kMixedMG
+            if (use_fp64mg()) {
+                fp64mg_issue();
+                return fp64mg_finish();
+            }
+            if (precision == kFP64 || precision == kFP64MG) return maze_fp64();
+            return maze_mixed();  // kMixedCG / kMixedMG
         case kMG:
             return mg_update(false);
         case kMazeMG: {
@@ -1443,7 +1546,9 @@ void Sim::step_issue() {
     step_wait = 1;
     const bool pcg_mg = grid_type == kMazeLCG && precision == kFP64 && precond == kPMG && !pb;
     if (grid_type == kMazeLCG && !pcg_mg) {
-        if (precision == kFP64) fp64_issue();
+        step_mg64 = use_fp64mg() ? 1 : 0;
+        if (step_mg64) fp64mg_issue();
+        else if (precision == kFP64 || precision == kFP64MG) fp64_issue();
         else mixed_issue();
     } else {
         // The other solvers wait for their state themselves; one more read
@@ -1464,7 +1569,10 @@ bool Sim::step_ready() const { return !step_wait || published(); }
 int Sim::step_finish() {
     int res = step_res;
     const bool pcg_mg = grid_type == kMazeLCG && precision == kFP64 && precond == kPMG && !pb;
-    if (grid_type == kMazeLCG && !pcg_mg) res = precision == kFP64 ? fp64_finish() : mixed_finish();
+    if (grid_type == kMazeLCG && !pcg_mg) {
+        if (step_mg64) res = fp64mg_finish();
+        else res = precision == kFP64 || precision == kFP64MG ? fp64_finish() : mixed_finish();
+    }
     else if (step_wait) wait_published();
     check_charge();
     check_pb_flag();

@@ -14,7 +14,9 @@ enum PrecondType { kPNone = 0, kPJacobi = 1, kPMG = 2, kPSSOR = 3, kPBlockJacobi
 enum ItgType { kOVRVO = 0, kVerlet = 1 };
 // Arithmetic of the field solve: the reference FP64 CG, or FP32-storage CG
 // inside FP64 iterative refinement.
-enum Precision { kFP64 = 0, kMixedCG = 1, kMixedMG = 2 };
+// kFP64MG: FP64 CG preconditioned by kMixedMG's V-cycle carried out in FP64
+// (the same solver without the FP32 storage; Poisson operator only).
+enum Precision { kFP64 = 0, kMixedCG = 1, kMixedMG = 2, kFP64MG = 3 };
 
 // A block of device memory that replica allocations are carved from while it
 // is installed (arena_install) on the allocating thread: one hipMalloc per
@@ -65,13 +67,27 @@ struct Sim {
     int grid_type = kMazeLCG, precond = kPNone;
     int precision = kFP64;
     // Relative residual reduction of each inner solve; <= 0 selects the
-    // default of the mode (1e-4 for Jacobi-CG, 1e-3 for the multigrid-
-    // preconditioned CG: measured fastest at 4096 ions / 128^3, where two
-    // refinement passes of about three cycles each reach the FP64 stop).
+    // default of the mode (kCgInnerRtol for Jacobi-CG, kMgInnerRtol for the
+    // multigrid-preconditioned CG).
     double inner_rtol = 0.0;
-    // Iteration budget of each inner solve of the mixed path (<= 0: n^2, i.e.
-    // the relative stop decides); MAZE_INNER_MAX sets it.
+    // Iteration budget of each inner solve of the mixed path (<= 0: the
+    // default of the mode, n^2 for Jacobi-CG and kMgInnerMax for multigrid-
+    // PCG); MAZE_INNER_MAX sets it.
     int inner_max = 0;
+    // Multigrid-PCG inner solves reduce their residual by 1e-6: one
+    // refinement pass of 4 V-cycles then meets the FP64 stop at 4096 ions /
+    // 128^3, where 1e-3 took two passes of 2 + 3 cycles (2124 against 1729
+    // steps/s; 3e-6 .. 5e-7 measured alike, 1e-5 needs 3 + 5 cycles); the
+    // PB paths and the ensemble gain alike.  An FP32 solve does not stall in
+    // its recursive residual, but the budget bounds a pass anyway.
+    static constexpr double kMgInnerRtol = 1e-6, kCgInnerRtol = 1e-4;
+    static constexpr int kMgInnerMax = 40;
+    double inner_rtol_for(bool use_mg) const {
+        return inner_rtol > 0 ? inner_rtol : (use_mg ? kMgInnerRtol : kCgInnerRtol);
+    }
+    int inner_limit_for(bool use_mg) const {
+        return inner_max > 0 ? inner_max : (use_mg ? kMgInnerMax : n * n);
+    }
     int pb = 0;
     int nonpolar = 0;
     double w = 0, kbar2 = 0;
@@ -337,6 +353,24 @@ struct Sim {
     int mg_stop = default_mg_stop();  // see mg_sizes
     void free_mg();
     void vcycle(int level, int iter);
+    // FP64 multigrid-PCG (precision kFP64MG): the levels of mg with FP64
+    // arrays; the V-cycle of vcycle() with FP64 operands preconditions FP64
+    // CG from the previous solution to the reference's stop test.
+    struct MGLevel64 {
+        int n;
+        double *b, *d, *t;
+    };
+    std::vector<MGLevel64> mg64;
+    int mg64_ok = -1;
+    int mg64_launched = 0;
+    int mg64_pred = 4;  // iterations queued before the first poll (last step's count)
+    bool setup_mg64();
+    void vcycle64(int level);
+    void fp64mg_issue();
+    void fp64mg_chunk(int count);
+    int fp64mg_finish();
+    bool use_fp64mg();
+    int step_mg64 = 0;
     // The inner iteration's stop test is deferred to the next consumer
     // (multigrid-PCG on the Poisson operator); a host read of the state in the
     // middle of an inner solve applies it first.

@@ -552,6 +552,37 @@ def test_mixed_precision_forces(hip, precision):
     assert err <= 1e-5, err
 
 
+def test_fp64_multigrid_pcg(hip):
+    """Precision 3: FP64 CG preconditioned by the V-cycle of precision 2 carried out
+    in FP64, to the reference's stop test.  Force RMS error <= 1e-5 relative to FP64
+    CG converged 1000x tighter, through the per-call driver sequence and through
+    solver_md_loop_iter, in a fraction of the Jacobi-CG iterations; a grid whose
+    hierarchy has no one-work-group coarsest solve (66 -> 33) runs precision 0."""
+    s = W.nacl_system(512, 64, seed=1)
+    tight = driver.RunConfig(tol=W.poisson_tol(s) * 1e-3)
+    cfg = driver.RunConfig(tol=W.poisson_tol(s))
+    _, _, f64 = run(hip, s, tight, 3, precision=0)
+    its0, _, _ = run(hip, s, cfg, 3, precision=0)
+    its3, _, f3 = run(hip, s, cfg, 3, precision=3)
+    assert min(its3) >= 0 and 4 * max(its3) < min(its0), (its3, its0)
+    norm = np.sqrt((f64["fcs_tot"] ** 2).mean())
+    assert np.sqrt(((f3["fcs_tot"] - f64["fcs_tot"]) ** 2).mean()) / norm <= 1e-5
+    driver.initialize(hip, s, cfg)
+    hip.lib.solver_set_precision(3)
+    driver.bootstrap(hip, cfg)
+    for _ in range(3):
+        hip.solver_md_loop_iter()
+    f = np.zeros((s.n_p, 3))
+    hip.get_fcs_tot(f)
+    hip.solver_finalize()
+    assert np.sqrt(((f - f64["fcs_tot"]) ** 2).mean()) / norm <= 1e-5
+    s66 = W.nacl_system(216, 66, seed=3)
+    c66 = driver.RunConfig(tol=W.poisson_tol(s66))
+    a = run(hip, s66, c66, 2, precision=0)
+    b = run(hip, s66, c66, 2, precision=3)
+    assert a[0] == b[0] and np.array_equal(a[2]["fcs_tot"], b[2]["fcs_tot"])
+
+
 def test_mg_pcg_largest_coarsest(hip_raw):
     """254^3 halves once to 127, the largest coarsest level solved exactly (axis
     passes over 127^3 work arrays): the FP32 MG-PCG solution satisfies the discrete
