@@ -486,7 +486,7 @@ int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c, int
 int maze_test_mg_up2(const float* b, const float* d2, const float* e, float* d, int n, double c1,
                      double c2, int tile) {
     if (n < 2 || n % 2 != 0 || tile >= maze::kMgTiles) return -1;
-    if (tile < 0) tile = maze::mg_default_tile(n);
+    if (tile < 0) tile = maze::mg_default_tile_up(n);
     Sim* s = scratch_sim(n);
     const long G = s->G, Gc = G / 8;
     float *db = nullptr, *dd2 = nullptr, *de = nullptr, *dd = nullptr;

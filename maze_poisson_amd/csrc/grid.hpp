@@ -234,7 +234,8 @@ enum MgTile {
     kMgMarch8,   // plane march, 16x16 columns, 8 planes per work-group (n % 16 == 0)
     kMgTiles
 };
-int mg_default_tile(int n);
+int mg_default_tile(int n);     // down kernel
+int mg_default_tile_up(int n);  // up kernel
 // Whether `tile` can run a level of n^3 points (the launchers fall back to the
 // default tile otherwise).
 bool mg_tile_ok(int tile, int n);

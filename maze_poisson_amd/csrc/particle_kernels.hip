@@ -230,13 +230,25 @@ void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red) 
 // order starting from 0.0 -- the reference's loop order, so the force matches
 // its serial sum bit for bit.  A 256-thread work-group holds 256 / nn whole
 // ions (nn = 8 or 64 divides 256).
-template <int CAS>
+// Kick: the MD step's path also forms the total force as k_ftot<false> does
+// and the velocity half-kick v += cdt F_tot / m as k_kick does, value for
+// value (one launch instead of three).
+struct ForceKick {
+    const double* fn;  // non-electrostatic force
+    double* ft;        // total force
+    double* vel;
+    const double* mass;
+    double cdt;
+};
+
+template <int CAS, bool Kick = false>
 __global__ __launch_bounds__(kBlock) void k_force_fd(int np, int n, double h,
                                                      const double* __restrict__ phi,
                                                      const int* __restrict__ base,
                                                      const double* __restrict__ charges,
                                                      const double* __restrict__ pos,
-                                                     double* __restrict__ forces) {
+                                                     double* __restrict__ forces,
+                                                     ForceKick fk = ForceKick{}) {
     using A = Assign<CAS>;
     constexpr int NN = A::kNodes;
     static_assert(kBlock % NN == 0, "whole ions per work-group");
@@ -272,6 +284,17 @@ __global__ __launch_bounds__(kBlock) void k_force_fd(int np, int n, double h,
         forces[3 * ip] = f0;
         forces[3 * ip + 1] = f1;
         forces[3 * ip + 2] = f2;
+        if (Kick) {
+            const double fe[3] = {f0, f1, f2};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double v = 0.0;
+                v += 1.0 * fe[d];
+                v += 1.0 * fk.fn[3 * ip + d];
+                fk.ft[3 * ip + d] = v;
+                fk.vel[3 * ip + d] += fk.cdt * v / fk.mass[ip];
+            }
+        }
     }
 }
 
@@ -280,13 +303,28 @@ void launch_force_fd(hipStream_t s, const PDev& p, const double* phi) {
     const int nb = blocks_exact((long)p.nn * p.np);
     if (p.cas == 0)
         MAZE_LAUNCH(k_force_fd<0>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi, p.base,
-                    p.charges, p.pos, p.fe);
+                    p.charges, p.pos, p.fe, ForceKick{});
     else if (p.cas == 1)
         MAZE_LAUNCH(k_force_fd<1>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi, p.base,
-                    p.charges, p.pos, p.fe);
+                    p.charges, p.pos, p.fe, ForceKick{});
     else
         MAZE_LAUNCH(k_force_fd<2>, dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi, p.base,
-                    p.charges, p.pos, p.fe);
+                    p.charges, p.pos, p.fe, ForceKick{});
+}
+
+void launch_force_fd_kick(hipStream_t s, const PDev& p, const double* phi, double cdt) {
+    if (p.np == 0) return;
+    const int nb = blocks_exact((long)p.nn * p.np);
+    const ForceKick fk{p.fn, p.ft, p.vel, p.mass, cdt};
+    if (p.cas == 0)
+        MAZE_LAUNCH((k_force_fd<0, true>), dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
+                    p.base, p.charges, p.pos, p.fe, fk);
+    else if (p.cas == 1)
+        MAZE_LAUNCH((k_force_fd<1, true>), dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
+                    p.base, p.charges, p.pos, p.fe, fk);
+    else
+        MAZE_LAUNCH((k_force_fd<2, true>), dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
+                    p.base, p.charges, p.pos, p.fe, fk);
 }
 
 // Shifted-force Tosi-Fumi over all pairs within r_cut (library/src/forces.c:
@@ -549,9 +587,12 @@ __device__ __forceinline__ double wrap_pos(double a, double L) {
     return a;
 }
 
-// Velocity Verlet part 1 (library/src/integrator_verlet.c:12-53).
+// Velocity Verlet part 1 (library/src/integrator_verlet.c:12-53); with base
+// (the MD step's path) also the charge assignment's base cell of the new
+// position, as k_cic_base forms it.
 __global__ void k_verlet1(int np, double dt, double L, double* pos, double* vel, const double* ft,
-                          const double* mass, const PScalars* ps, int rescale) {
+                          const double* mass, const PScalars* ps, int rescale, int n, double h,
+                          int* base) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const double scale = rescale ? ps->scale : 1.0;
@@ -561,14 +602,22 @@ __global__ void k_verlet1(int np, double dt, double L, double* pos, double* vel,
         if (rescale) v *= scale;
         v += 0.5 * dt * ft[k] / mass[i];
         vel[k] = v;
-        pos[k] = wrap_pos(pos[k] + dt * v, L);
+        const double x = wrap_pos(pos[k] + dt * v, L);
+        pos[k] = x;
+        if (base) {
+            int c = (int)floor(x / h);
+            c %= n;
+            if (c < 0) c += n;
+            base[k] = c;
+        }
     }
 }
 
-void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale) {
+void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale,
+                         bool with_base) {
     if (p.np == 0) return;
     MAZE_LAUNCH(k_verlet1, dim3(blocks_for(p.np)), dim3(kBlock), 0, s, p.np, dt, p.L, p.pos,
-                       p.vel, p.ft, p.mass, ps, rescale);
+                       p.vel, p.ft, p.mass, ps, rescale, p.n, p.h, with_base ? p.base : nullptr);
 }
 
 // v += cdt F / m with cdt = 0.5 dt (Verlet) or 0.5 dt c2 (OVRVO V block).

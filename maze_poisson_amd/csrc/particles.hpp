@@ -52,6 +52,9 @@ void launch_assign(hipStream_t s, const PDev& p, const PScalars* ps, double* q, 
 // Sum of all contributions into ps->q_sum; advances ps->epoch.
 void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi);
+// The MD step's path: the field force, F_tot = F_elec + F_noel and the
+// velocity half-kick v += cdt F_tot / m in one launch (no PB boundary forces).
+void launch_force_fd_kick(hipStream_t s, const PDev& p, const double* phi, double cdt);
 long tf_part_doubles(int np);
 void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 // The same in pieces: the 64-ion block pairs (I, J >= I) of rows I in [i_lo, i_hi),
@@ -64,7 +67,9 @@ void launch_force_total(hipStream_t s, const PDev& p, const double* fdb = nullpt
                         const double* fib = nullptr, const double* fnp = nullptr);
 void launch_kinetic(hipStream_t s, const PDev& p, PScalars* ps, Reduce red, double T_target,
                     int want_scale);
-void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale);
+// with_base: also the base cells of the new positions (launch_cic_base's work).
+void launch_verlet_part1(hipStream_t s, const PDev& p, double dt, const PScalars* ps, int rescale,
+                         bool with_base = false);
 void launch_kick(hipStream_t s, const PDev& p, double cdt);
 void launch_drift(hipStream_t s, const PDev& p, double c2dt);
 void launch_o_block(hipStream_t s, const PDev& p, uint64_t seed, uint64_t call, double c1s,

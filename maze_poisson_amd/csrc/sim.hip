@@ -748,15 +748,15 @@ bool Sim::setup_mg() {
         const long g = (long)L.n * L.n * L.n;
         L.b = l == 0 ? nullptr : dalloc<float>(g);
         L.tile = mg_default_tile(L.n);
+        L.tile_up = mg_default_tile_up(L.n);
         if (const char* e = std::getenv("MAZE_MG_TILES")) {  // "t0,t1,...": tuning override
             const char* q = e;
             for (size_t k = 0; k < l && q; k++) {
                 q = std::strchr(q, ',');
                 if (q) q++;
             }
-            if (q && *q >= '0' && *q < '0' + kMgTiles) L.tile = *q - '0';
+            if (q && *q >= '0' && *q < '0' + kMgTiles) L.tile = L.tile_up = *q - '0';
         }
-        L.tile_up = L.tile;
         L.d = dalloc<float>(g);
         L.t = dalloc<float>(g);
         mg.push_back(L);
@@ -1352,8 +1352,8 @@ int Sim::mg_update(bool maze) {
 
 // library/src/solver_md.c:91-107.  defer: the charge-conservation check
 // is made by check_charge() after the next read_state() instead of here.
-int Sim::update_charges(bool defer) {
-    launch_cic_base(stream, p);
+int Sim::update_charges(bool defer, bool base_done) {
+    if (!base_done) launch_cic_base(stream, p);
     // q holds the previous spread, which the spread clears node by node; after
     // a new grid or particle set, clear it whole once.
     if (p.np > 0 && p.n != n) fatal("maze: the particles' grid size differs from the field grid");
@@ -1500,15 +1500,16 @@ void Sim::forces_tot() {
 }
 
 // library/src/integrator_verlet.c:12-71, integrator_ovrvo.c:45-138.
-void Sim::part1() {
+bool Sim::part1(bool with_base) {
     if (itype == kVerlet) {
         if (enabled) launch_kinetic(stream, p, ps, red, T, 1);
-        launch_verlet_part1(stream, p, dt, ps, enabled);
-    } else {
-        if (enabled) launch_o_block(stream, p, seed, calls++, std::sqrt(c1), (1 - c1) * kBoltz * T);
-        launch_kick(stream, p, 0.5 * dt * c2);
-        launch_drift(stream, p, c2 * dt);
+        launch_verlet_part1(stream, p, dt, ps, enabled, with_base);
+        return with_base;
     }
+    if (enabled) launch_o_block(stream, p, seed, calls++, std::sqrt(c1), (1 - c1) * kBoltz * T);
+    launch_kick(stream, p, 0.5 * dt * c2);
+    launch_drift(stream, p, c2 * dt);
+    return false;
 }
 
 void Sim::part2() {
@@ -1531,7 +1532,7 @@ void Sim::md_step() {
 // half, the pair forces (side stream), the charge spread and the field solve
 // up to the publication of the solver state.
 void Sim::step_issue() {
-    part1();
+    const bool base_done = part1(true);
     step_tf = tf_overlap && p.np > 0;
     auto issue_tf = [&] {
         MAZE_HIP(hipEventRecord(ev_pos, stream));
@@ -1540,7 +1541,7 @@ void Sim::step_issue() {
         MAZE_HIP(hipEventRecord(ev_tf, side));
     };
     if (step_tf && !tf_late) issue_tf();
-    update_charges(true);
+    update_charges(true, base_done);
     if (step_tf && tf_late) issue_tf();
     update_eps_k2();
     step_wait = 1;
@@ -1576,9 +1577,19 @@ int Sim::step_finish() {
     else if (step_wait) wait_published();
     check_charge();
     check_pb_flag();
+    if (!pbd.active) {
+        // The field force, the total force and the velocity half-kick in one
+        // launch (the same values as forces_elec, forces_tot and part2's kick).
+        if (!step_tf) forces_noel(false);
+        if (step_tf) MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
+        launch_force_fd_kick(stream, p, phi, itype == kVerlet ? 0.5 * dt : 0.5 * dt * c2);
+        if (itype != kVerlet && enabled)
+            launch_o_block(stream, p, seed, calls++, std::sqrt(c1), (1 - c1) * kBoltz * T);
+        return res;
+    }
     forces_elec();
     if (!step_tf) forces_noel(false);
-    if (pbd.active) forces_pb();
+    forces_pb();
     if (step_tf) MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
     forces_tot();
     part2();

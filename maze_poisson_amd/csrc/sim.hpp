@@ -287,7 +287,8 @@ struct Sim {
     void check_pb_flag();
     void free_pb();
 
-    int update_charges(bool defer = false);
+    // base_done: the base cells were formed by part1(true).
+    int update_charges(bool defer = false, bool base_done = false);
     int check_charge();
     void init_field();
     void init_field_issue();
@@ -305,7 +306,9 @@ struct Sim {
     void forces_elec();
     double forces_noel(bool read_back = true);
     void forces_tot();
-    void part1();
+    // with_base (the MD step's path): Verlet's first half also forms the base
+    // cells of the charge assignment; returns whether it did.
+    bool part1(bool with_base = false);
     void part2();
     void md_step();
     // md_step in two halves for callers that advance several replicas from
