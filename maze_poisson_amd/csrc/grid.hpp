@@ -219,6 +219,15 @@ void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef32& f, float* cx, floa
                        float* ck2);
 void launch_pb_coarse(hipStream_t s, int c, const PBCoef32& pb, const float* g, const float* b,
                       float* x, int iters, double lo, double hi, CGState* st);
+// The level above the coarsest (n <= 20, even) and the coarsest (n / 2) of the
+// PB V-cycle in one work-group: reads that level's b, writes its d; the same
+// values as the separate passes (pre-smoothing with c1 then c2, residual,
+// restriction, `iters` coarsest iterations on [lo, hi], prolongation, post-
+// smoothing with c2 then c1).
+constexpr int kPbTailMaxN = 20;
+void launch_pb_tail(hipStream_t s, int n, const PBCoef32& pf, const float* gf, const float* b,
+                    float* d, const PBCoef32& pc, const float* gc, double c1, double c2, int iters,
+                    double lo, double hi, CGState* st);
 // Fused level kernels of the V-cycle on an n^3 level (n even):
 // down: d2 = two pre-smoothing sweeps from zero, b_coarse = restricted residual;
 // up:   d = two post-smoothing sweeps of d2 + prolong(e); dot: also <b, d> and

@@ -1008,7 +1008,9 @@ const char* mg_tile_name(int tile) {
 // levels: 4x8x8 tiles with 512 threads, so that a 64^3 level still has 1024
 // work-groups (8x16x16 would leave half the CUs idle).
 int mg_default_tile(int n) { return n >= 128 ? kMgT8x16x16_1024 : kMgT4x8x8_512; }
-int mg_default_tile_up(int n) { return n >= 128 ? kMgT8x16x16_512 : kMgT4x8x8_512; }
+int mg_default_tile_up(int n) {  // 64^3: 5.55 us with 256 threads against 6.22 with 512
+    return n >= 128 ? kMgT8x16x16_512 : (n >= 64 ? kMgT4x8x8_256 : kMgT4x8x8_512);
+}
 
 void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
                     double c1, double c2, CGState* st) {
@@ -2032,17 +2034,15 @@ void launch_pb_coarsen(hipStream_t s, int nf, const PBCoef32& f, float* cx, floa
 // between two arrays so that one barrier per iteration suffices.
 constexpr int kPbCoarseThreads = 1024;
 
-__global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef32 pb,
-                                                             const float* __restrict__ g,
-                                                             const float* __restrict__ b,
-                                                             float* __restrict__ x_out, int iters,
-                                                             double lo, double hi,
-                                                             const CGState* st) {
-    if (st->done) return;
-    constexpr int CMAX = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds;
-    constexpr int PER = CMAX / kPbCoarseThreads;
-    __shared__ float dv[2][CMAX];
-    __shared__ float ex[CMAX], ey[CMAX], ez[CMAX], k2[CMAX];
+// The iterations on a c^3 level (c^3 <= CMAX) by NT threads: coefficients pb
+// and g, right-hand side b (global or LDS), result x_out (global or LDS); dv
+// (2 CMAX) and ex, ey, ez, k2 (CMAX each) are the caller's LDS.
+template <int CMAX, int NT>
+__device__ __forceinline__ void pb_cheb(int c, const PBCoef32& pb, const float* g, const float* b,
+                                        float* x_out, int iters, double lo, double hi,
+                                        float (*dv)[CMAX], float* ex, float* ey, float* ez,
+                                        float* k2) {
+    constexpr int PER = (CMAX + NT - 1) / NT;
     const int N = c * c * c, c2 = c * c, tid = threadIdx.x;
     float x[PER], r[PER], gg[PER];
     int nb[PER][6];
@@ -2050,7 +2050,7 @@ __global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef32 
     double rho = 1.0 / sigma;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int t = tid + q * kPbCoarseThreads;
+        const int t = tid + q * NT;
         if (t < N) {
             ex[t] = pb.ex[t];
             ey[t] = pb.ey[t];
@@ -2075,7 +2075,7 @@ __global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef32 
         const double rho1 = 1.0 / (2.0 * sigma - rho);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int t = tid + q * kPbCoarseThreads;
+            const int t = tid + q * NT;
             if (t < N) {
                 const float* d = dv[cur];
                 const float dc = d[t];
@@ -2095,9 +2095,158 @@ __global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef32 
     }
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int t = tid + q * kPbCoarseThreads;
+        const int t = tid + q * NT;
         if (t < N) x_out[t] = x[q];
     }
+}
+
+__global__ __launch_bounds__(kPbCoarseThreads) void k_pb_coarse(int c, PBCoef32 pb,
+                                                             const float* __restrict__ g,
+                                                             const float* __restrict__ b,
+                                                             float* __restrict__ x_out, int iters,
+                                                             double lo, double hi,
+                                                             const CGState* st) {
+    if (st->done) return;
+    constexpr int CMAX = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds;
+    __shared__ float dv[2][CMAX];
+    __shared__ float ex[CMAX], ey[CMAX], ez[CMAX], k2[CMAX];
+    pb_cheb<CMAX, kPbCoarseThreads>(c, pb, g, b, x_out, iters, lo, hi, dv, ex, ey, ez, k2);
+}
+
+// ---- the two smallest PB levels in one work-group --------------------------
+// The level above the coarsest (n <= kPbTailMaxN) and the coarsest (n / 2)
+// with the level in LDS: the passes of Sim::vcycle_pb for that level --
+// pre-smoothing (both sweeps, the first from zero), residual, restriction,
+// the coarsest level's iterations, prolongation-correction and the two
+// post-smoothing sweeps -- each one the same arithmetic, in the same order,
+// as the separate kernel it replaces (k_pass with SrcPbFirst + BodyPbSmooth,
+// BodyMgResid, k_mg_restrict, k_pb_coarse, k_mg_prolong_add, BodyPbSmooth),
+// so the result is bit-identical to them, in one launch instead of seven.
+constexpr int kPbTailF = kPbTailMaxN * kPbTailMaxN * kPbTailMaxN;
+constexpr int kPbTailC = kPbTailF / 8;
+
+// The march's PB operator at point (i, j, k) of an n^3 level for the values u
+// (LDS): sum_f eps_f (u_nb - u) - k2 u, in its order and in FP64.
+__device__ __forceinline__ double pb_lap_tail(const float* u, const PBCoef32& pb, int n, int i,
+                                              int j, int k) {
+    const int n2 = n * n, t = i * n2 + j * n + k;
+    const int ip = i == n - 1 ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
+    const int jp = j == n - 1 ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    const int kp = k == n - 1 ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const double vp = u[ip * n2 + j * n + k], vm = u[im * n2 + j * n + k];
+    const double vjp = u[i * n2 + jp * n + k], vjm = u[i * n2 + jm * n + k];
+    const double vkp = u[i * n2 + j * n + kp], vkm = u[i * n2 + j * n + km];
+    const double vc = u[t];
+    const double exc = pb.ex[t], exm = pb.ex[im * n2 + j * n + k];
+    const double eyc = pb.ey[t], eym = pb.ey[i * n2 + jm * n + k];
+    const double ezc = pb.ez[t], ezm = pb.ez[i * n2 + j * n + km];
+    const double k2c = pb.k2[t];
+    return vp * exc + vm * exm + vjp * eyc + vjm * eym + vkp * ezc + vkm * ezm -
+           vc * (exc + exm + eyc + eym + ezc + ezm + k2c);
+}
+
+__global__ __launch_bounds__(kPbCoarseThreads) void k_pb_tail(int n, PBCoef32 pf,
+                                                           const float* __restrict__ gf,
+                                                           const float* __restrict__ bin,
+                                                           float* __restrict__ dout, PBCoef32 pc,
+                                                           const float* __restrict__ gc, double c1,
+                                                           double c2, int iters, double lo,
+                                                           double hi, const CGState* st) {
+    if (st->done) return;
+    constexpr int NT = kPbCoarseThreads;
+    __shared__ float B[kPbTailF], D[kPbTailF], T[kPbTailF];
+    __shared__ float cb[kPbTailC], ce[kPbTailC];
+    __shared__ float dv[2][kPbTailC];
+    __shared__ float cex[kPbTailC], cey[kPbTailC], cez[kPbTailC], ck2[kPbTailC];
+    const int tid = threadIdx.x, n2 = n * n, N = n2 * n, c = n / 2, C = c * c * c;
+    // A thread's points t = tid + q NT, q < PER, unrolled so that the loads of
+    // all of them are in flight together.
+    constexpr int PER = (kPbTailF + NT - 1) / NT;
+    auto each = [&](auto&& f) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = tid + q * NT;
+            if (t < N) f(t, t / n2, (t / n) % n, t % n);
+        }
+    };
+    // First pre-smoothing sweep from zero, as SrcPbFirst evaluates it.
+    each([&](int t, int, int, int) {
+        const float bv = bin[t];
+        B[t] = bv;
+        T[t] = (float)(-c1 * (double)gf[t] * (double)bv);
+    });
+    lds_barrier();
+    each([&](int t, int i, int j, int k) {  // second sweep (BodyPbSmooth, weight c2)
+        const double lap = pb_lap_tail(T, pf, n, i, j, k);
+        D[t] = (float)((double)T[t] - c2 * (double)gf[t] * ((double)B[t] - lap));
+    });
+    lds_barrier();
+    each([&](int t, int i, int j, int k) {  // residual (BodyMgResid)
+        T[t] = (float)((double)B[t] - pb_lap_tail(D, pf, n, i, j, k));
+    });
+    lds_barrier();
+    for (int u = tid; u < C; u += NT) {  // restriction (k_mg_restrict)
+        const int K = u % c, J = (u / c) % c, I = u / (c * c);
+        const int i = 2 * I, j = 2 * J, k = 2 * K;
+        const int ii[3] = {(i == 0 ? n - 1 : i - 1), i, i + 1};
+        const int jj[3] = {(j == 0 ? n - 1 : j - 1), j, j + 1};
+        const int kk[3] = {(k == 0 ? n - 1 : k - 1), k, k + 1};
+        double acc = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) {
+                    const double w =
+                        (double)((a == 1 ? 2 : 1) * (bb == 1 ? 2 : 1) * (cc == 1 ? 2 : 1));
+                    acc += w * (double)T[ii[a] * n2 + jj[bb] * n + kk[cc]];
+                }
+        cb[u] = (float)(acc * (4.0 / 64.0));
+    }
+    lds_barrier();
+    pb_cheb<kPbTailC, NT>(c, pc, gc, cb, ce, iters, lo, hi, dv, cex, cey, cez, ck2);
+    lds_barrier();
+    each([&](int t, int i, int j, int k) {  // prolongation-correction (k_mg_prolong_add)
+        const int a0 = i >> 1, a1 = (a0 + 1 == c) ? 0 : a0 + 1;
+        const int b0 = j >> 1, b1 = (b0 + 1 == c) ? 0 : b0 + 1;
+        const int c0 = k >> 1, cc1 = (c0 + 1 == c) ? 0 : c0 + 1;
+        const double wi = (i & 1) ? 0.5 : 0.0, wj = (j & 1) ? 0.5 : 0.0, wk = (k & 1) ? 0.5 : 0.0;
+        const int M = c, M2 = c * c;
+        const float e000 = ce[a0 * M2 + b0 * M + c0], e001 = ce[a0 * M2 + b0 * M + cc1];
+        const float e010 = ce[a0 * M2 + b1 * M + c0], e011 = ce[a0 * M2 + b1 * M + cc1];
+        const float e100 = ce[a1 * M2 + b0 * M + c0], e101 = ce[a1 * M2 + b0 * M + cc1];
+        const float e110 = ce[a1 * M2 + b1 * M + c0], e111 = ce[a1 * M2 + b1 * M + cc1];
+        const float dt = D[t];
+        const double v00 = e000 + wk * ((double)e001 - e000);
+        const double v01 = e010 + wk * ((double)e011 - e010);
+        const double v10 = e100 + wk * ((double)e101 - e100);
+        const double v11 = e110 + wk * ((double)e111 - e110);
+        const double v0 = v00 + wj * (v01 - v00);
+        const double v1 = v10 + wj * (v11 - v10);
+        D[t] = (float)((double)dt + v0 + wi * (v1 - v0));
+    });
+    lds_barrier();
+    each([&](int t, int i, int j, int k) {  // post-smoothing, weight c2
+        const double lap = pb_lap_tail(D, pf, n, i, j, k);
+        T[t] = (float)((double)D[t] - c2 * (double)gf[t] * ((double)B[t] - lap));
+    });
+    lds_barrier();
+    each([&](int t, int i, int j, int k) {  // post-smoothing, weight c1
+        const double lap = pb_lap_tail(T, pf, n, i, j, k);
+        dout[t] = (float)((double)T[t] - c1 * (double)gf[t] * ((double)B[t] - lap));
+    });
+}
+
+void launch_pb_tail(hipStream_t s, int n, const PBCoef32& pf, const float* gf, const float* b,
+                    float* d, const PBCoef32& pc, const float* gc, double c1, double c2, int iters,
+                    double lo, double hi, CGState* st) {
+    if (n > kPbTailMaxN || n % 2 != 0) {
+        std::fprintf(stderr, "maze: PB tail level %d above %d or odd\n", n, kPbTailMaxN);
+        std::exit(1);
+    }
+    MAZE_LAUNCH(k_pb_tail, dim3(1), dim3(kPbCoarseThreads), 0, s, n, pf, gf, b, d, pc, gc, c1, c2,
+                iters, lo, hi, st);
 }
 
 void launch_pb_coarse(hipStream_t s, int c, const PBCoef32& pb, const float* g, const float* b,

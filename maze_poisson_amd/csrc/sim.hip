@@ -123,6 +123,7 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     if (const char* e = std::getenv("MAZE_TF_LATE")) tf_late = std::atoi(e) != 0;
     if (const char* e = std::getenv("MAZE_INNER_MAX")) inner_max = std::atoi(e);
     if (const char* e = std::getenv("MAZE_INNER_RTOL")) inner_rtol = std::atof(e);  // tuning
+    if (const char* e = std::getenv("MAZE_PB_TAIL")) pb_tail = std::atoi(e) != 0;
     st = dalloc<CGState>(1);
     st_c = dalloc<CGState>(1);
     MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
@@ -930,6 +931,11 @@ void Sim::vcycle_pb(int l) {
         return;
     }
     PbLevel& C = pbmg[l + 1];
+    if (pb_tail && l > 0 && l + 2 == (int)pbmg.size() && L.n <= kPbTailMaxN) {
+        launch_pb_tail(stream, L.n, L.cf, L.g, L.b, L.d, C.cf, C.g, mg_omega, mg_omega2,
+                       pb_coarse_iters, pb_coarse_lo, 2.0, st);
+        return;
+    }
     launch_pb_pre(stream, L.dims, L.cf, L.g, L.b, L.d, mg_omega, mg_omega2, st);
     launch_pb_resid(stream, L.dims, L.cf, L.b, L.d, L.t, st);
     launch_mg_restrict(stream, L.n, L.t, C.b, st);

@@ -140,6 +140,13 @@ struct Sim {
     int pbmg_ok = -1;           // -1 not set up, 0 unavailable, 1 ready
     int pbmg_dirty = 1;         // the maps changed since the coefficients were built
     int pb_coarse_iters = 24;   // Chebyshev iterations on the coarsest level
+    // The two smallest PB levels in one work-group (launch_pb_tail) when the
+    // level above the coarsest has at most 20 points per side and is not the
+    // finest (MAZE_PB_TAIL=1; the same values as the separate passes).  Off by
+    // default: at 8000 ions / 160^3 the work-group took 62 us per cycle
+    // against 44.5 us for the seven launches it replaces (one CU streams the
+    // level's coefficient arrays from L2), 644 against 688 steps/s.
+    int pb_tail = 0;
     double pb_coarse_lo = 0.02; // their lower spectrum bound (upper: 2)
     // Replay each multigrid-PCG inner iteration as a captured HIP graph (three
     // variants: the first iteration, odd and even ones) instead of its seven

@@ -131,6 +131,24 @@ def test_pb_mg_pcg_solves(hip_raw, n, smooth):
     assert np.abs(r).max() <= 1e-3 * np.abs(b).max()
 
 
+@pytest.mark.parametrize("n", [40, 80])
+def test_pb_tail_matches_separate_passes(hip_raw, n, monkeypatch):
+    """The two smallest PB levels in one work-group (MAZE_PB_TAIL=1; 20^3 and its
+    10^3 coarsest level here) give the FP32 PB multigrid-PCG solve bit for bit as
+    their passes run one kernel each (the default)."""
+    eps = W.pb_eps_field(n, 40.0, seed=n)
+    k2 = np.full((n, n, n), 0.02)
+    b = rand_grid(n, 330)
+    out = []
+    for tail in ("1", "0"):
+        monkeypatch.setenv("MAZE_PB_TAIL", tail)
+        x = np.zeros_like(b)
+        it = hip_raw.maze_test_pb_mg_pcg(b, x, 1e-5, n, *eps, k2)
+        out.append((it, x))
+    assert out[0][0] == out[1][0] > 0, (out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_vcycle_pb_matches_oracle(hip_raw, oracle_raw):
     n = 32
     eps, k2 = pb_coef(n, 310)
