@@ -1229,6 +1229,7 @@ __global__ __launch_bounds__(4 * kMgCoarseLds * kMgCoarseLds) void k_mg_coarse(
 // j + 16 q is D[4 q + r][j].  Values past c (padding) enter as exact zeros.
 // FP32 products with FP32 accumulation, as the VALU form.
 typedef float mg_v4f __attribute__((ext_vector_type(4)));
+typedef double mg_v4d __attribute__((ext_vector_type(4)));
 constexpr int kMgCoarseMfmaThreads = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds / 4;
 
 // C (the level's side) is a compile-time constant, so every index division by
@@ -1395,6 +1396,68 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse_axis(int c, const void* __
     }
 }
 
+// The axis passes on the matrix cores (v_mfma_f64_16x16x4f64): a pass is the
+// product OUT[line][t] = sum_m IN[line][m] M[m][t] of the c^2 lines of the
+// level (rows of c values, contiguous) with the c x c eigenbasis M = Qf^T
+// (forward) or Qf (inverse).  A work-group takes 16 consecutive lines of one
+// plane (i, j0 .. j0 + 15), stages them in LDS zero-padded to a multiple of 4
+// values, and each of its four waves forms 16 x 16 output tiles (t0 .. t0 +
+// 15) in K-steps of 4.  Operand layout of the 16x16x4 f64 instruction
+// (wave64, checked on the GPU): A[i][k] from lane i + 16 k, B[k][j] from lane
+// j + 16 k, result register r of lane j + 16 q is D[q + 4 r][j] (the f32 form
+// has D[4 q + r][j]); values past c enter as exact zeros.  The products are summed four at a time into the accumulator, so the
+// sums differ from the vector-ALU form in the last bits.
+template <int Mode, bool Inv>
+__global__ __launch_bounds__(kBlock) void k_mg_coarse_axis_mfma(int c, const void* __restrict__ in_,
+                                                                void* __restrict__ out_,
+                                                                const double* __restrict__ Qf,
+                                                                const double* __restrict__ ilam,
+                                                                const CGState* st) {
+    if (st->done) return;
+    constexpr int PMAX = kMgCoarseMax + 5;  // padded row pitch (odd, against bank conflicts)
+    __shared__ double rows[16 * PMAX];
+    const int ng = (c + 15) / 16;
+    const int i = (int)blockIdx.x / ng, j0 = ((int)blockIdx.x % ng) * 16;
+    const int J = min(16, c - j0);
+    const int K4 = (c + 3) & ~3, P = K4 + 1;
+    const long base = ((long)i * c + j0) * c;
+    for (int x = threadIdx.x; x < 16 * K4; x += kBlock) {
+        const int line = x / K4, m = x % K4;
+        double v = 0.0;
+        if (line < J && m < c) {
+            if constexpr (Mode == 1) v = (double)static_cast<const float*>(in_)[base + (long)line * c + m];
+            else v = static_cast<const double*>(in_)[base + (long)line * c + m];
+        }
+        rows[line * P + m] = v;
+    }
+    lds_barrier();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int tt = wave; tt < (c + 15) / 16; tt += kBlock / 64) {
+        const int t0 = tt * 16, tb = t0 + lr;
+        mg_v4d acc = {0.0, 0.0, 0.0, 0.0};
+        for (int m0 = 0; m0 < K4; m0 += 4) {
+            const int m = m0 + lq;
+            const double a = rows[lr * P + m];
+            const double b = (m < c && tb < c) ? (Inv ? Qf[m * c + tb] : Qf[tb * c + m]) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        if (tb < c) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int line = lq + 4 * r;
+                if (line < J) {
+                    const long o = ((long)tb * c + i) * c + j0 + line;
+                    double s = acc[r];
+                    if constexpr (Mode == 2) s *= ilam[o];
+                    if constexpr (Mode == 3) static_cast<float*>(out_)[o] = (float)s;
+                    else static_cast<double*>(out_)[o] = s;
+                }
+            }
+        }
+    }
+}
+
 template <int R>
 static void coarse_axis_passes(hipStream_t s, int c, const float* b, float* e, double* w0,
                                double* w1, const double* Qf, const double* ilam, CGState* st) {
@@ -1407,21 +1470,35 @@ static void coarse_axis_passes(hipStream_t s, int c, const float* b, float* e, d
     MAZE_LAUNCH((k_mg_coarse_axis<3, true, R>), grid, block, 0, s, c, w0, e, Qf, ilam, st);
 }
 
+static void coarse_axis_passes_mfma(hipStream_t s, int c, const float* b, float* e, double* w0,
+                                    double* w1, const double* Qf, const double* ilam,
+                                    CGState* st) {
+    const dim3 grid(c * ((c + 15) / 16)), block(kBlock);
+    MAZE_LAUNCH((k_mg_coarse_axis_mfma<1, false>), grid, block, 0, s, c, b, w0, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis_mfma<0, false>), grid, block, 0, s, c, w0, w1, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis_mfma<2, false>), grid, block, 0, s, c, w1, w0, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis_mfma<0, true>), grid, block, 0, s, c, w0, w1, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis_mfma<0, true>), grid, block, 0, s, c, w1, w0, Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_axis_mfma<3, true>), grid, block, 0, s, c, w0, e, Qf, ilam, st);
+}
+
 void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,
                           const double* Qf, const double* ilam, CGState* st) {
     if (c > kMgCoarseMax) {
         std::fprintf(stderr, "maze: coarsest multigrid level %d above %d\n", c, kMgCoarseMax);
         std::exit(1);
     }
-    // Rows per work-group: 4 by default (about one output per thread; passes on
-    // coarsest grids of 65^3, 51^3 and 41^3 points measured 7.3 us on average
-    // against 9.6 us with 16 rows), MAZE_COARSE_ROWS=8 or 16 for the others.
+    // The matrix-core passes by default; MAZE_COARSE_ROWS=4, 8 or 16 selects the
+    // vector-ALU passes with that many rows per work-group (4 was their best:
+    // passes on coarsest grids of 65^3, 51^3 and 41^3 points measured 7.3 us on
+    // average against 9.6 us with 16 rows).
     static const int rows = [] {
         const char* v = std::getenv("MAZE_COARSE_ROWS");
-        const int r = v ? std::atoi(v) : 4;
-        return r == 8 || r == 16 ? r : 4;
+        const int r = v ? std::atoi(v) : 0;
+        return r == 4 || r == 8 || r == 16 ? r : 0;
     }();
-    if (rows == 4) coarse_axis_passes<4>(s, c, b, e, w0, w1, Qf, ilam, st);
+    if (rows == 0) coarse_axis_passes_mfma(s, c, b, e, w0, w1, Qf, ilam, st);
+    else if (rows == 4) coarse_axis_passes<4>(s, c, b, e, w0, w1, Qf, ilam, st);
     else if (rows == 8) coarse_axis_passes<8>(s, c, b, e, w0, w1, Qf, ilam, st);
     else coarse_axis_passes<16>(s, c, b, e, w0, w1, Qf, ilam, st);
 }
