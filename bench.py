@@ -168,7 +168,7 @@ def pb_synthetic(lib, steps):
     cfg = driver.RunConfig(method="MAZE-LCG", tol=W.poisson_tol(s), integrator="OVRVO",
                            thermostat=True, gamma=2.4189e-5, pb=True)
     mg = pb_rate(lib, cfg, s, PATHS["mixed_mg"], steps)
-    f64 = pb_rate(lib, cfg, s, PATHS["fp64"], max(3, steps // 10))
+    f64 = pb_rate(lib, cfg, s, PATHS["fp64"], max(3, min(10, steps // 10)))
     return {"workload": f"{PB_IONS}-ion NaCl melt, {PB_GRID}^3 grid, OVRVO 1500 K, linear PB "
                         "with a synthetic dielectric in [2, 80] (8 Gaussians) and 0.1 M screening",
             "mixed_mg_steps_per_s": round(mg, 3), "fp64_steps_per_s": round(f64, 3),
@@ -556,9 +556,11 @@ def main():
         line_extra = {
             "runner_steps_per_s": round(runner_rate(lib, 200), 2),
             "pb_radii": pb_radii_rate(lib, max(10, args.steps // 10), 5),
-            "pb_synthetic": pb_synthetic(lib, 30),
-            "ensemble_64_replicas": ensemble_rate(lib, 64, 50, 64),
-            "serving": serving_run(lib, 16, 4.0, (100, 600), 32, 64.0),
+            # The workloads' own step counts: 500 PB steps, 200 steps per replica,
+            # 100-2000 steps per serving request.
+            "pb_synthetic": pb_synthetic(lib, 500),
+            "ensemble_64_replicas": ensemble_rate(lib, 64, 200, 64),
+            "serving": serving_run(lib, 16, 4.0, (100, 2000), 32, 64.0),
             "python_driver_steps_per_s": round(driver_rate(lib, PATHS["mixed_mg"], 100, False), 2),
             "python_driver_with_getters_steps_per_s": round(
                 driver_rate(lib, PATHS["mixed_mg"], 100, True), 2),

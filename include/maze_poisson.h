@@ -285,7 +285,8 @@ int maze_test_mg_vcycle_pb(const double *in, double *out, int n, const double *e
                            const double *eps_y, const double *eps_z, const double *k2);
 int maze_test_mg_pcg(const double *b, double *x, double rtol, int n);
 /* The fused FP32 multigrid level kernels (n even, c = damping / 6, tile = the
- * work-group tile shape 0..4, or -1 for the level's default):
+ * work-group tile shape 0..4, 5..6 for the plane-marching kernels, 64 + 0..4
+ * for shape 0..4 with the scalar-load kernels, or -1 for the level's default):
  * down: d2 = two damped-Jacobi sweeps from zero on A d = b, bc = 4 x
  *       full-weighting restriction of b - A d2 ((n/2)^3 values);
  * up:   d = two damped-Jacobi sweeps on A d = b from d2 + trilinear(e). */

@@ -461,7 +461,7 @@ int maze_test_prolong_add(const double* in, double* out, int nc) {
 // c1 then c2 (the *2 forms) or c for both.
 int maze_test_mg_down2(const float* b, float* d2, float* bc, int n, double c1, double c2,
                        int tile) {
-    if (n < 2 || n % 2 != 0 || tile >= maze::kMgTiles) return -1;
+    if (n < 2 || n % 2 != 0 || (tile >= maze::kMgTiles && !maze::mg_tile_ok(tile, n))) return -1;
     if (tile < 0) tile = maze::mg_default_tile(n);
     Sim* s = scratch_sim(n);
     const long G = s->G, Gc = G / 8;
@@ -485,7 +485,7 @@ int maze_test_mg_down(const float* b, float* d2, float* bc, int n, double c, int
 
 int maze_test_mg_up2(const float* b, const float* d2, const float* e, float* d, int n, double c1,
                      double c2, int tile) {
-    if (n < 2 || n % 2 != 0 || tile >= maze::kMgTiles) return -1;
+    if (n < 2 || n % 2 != 0 || (tile >= maze::kMgTiles && !maze::mg_tile_ok(tile, n))) return -1;
     if (tile < 0) tile = maze::mg_default_tile_up(n);
     Sim* s = scratch_sim(n);
     const long G = s->G, Gc = G / 8;

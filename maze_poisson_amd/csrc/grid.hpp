@@ -243,6 +243,10 @@ enum MgTile {
     kMgMarch8,   // plane march, 16x16 columns, 8 planes per work-group (n % 16 == 0)
     kMgTiles
 };
+// Or-ed into a tile id below kMgMarch16: the same tile shape with the scalar-load
+// kernels that serve odd and small levels, in place of the column-mapped
+// vector-load kernels (the two compute every value alike).
+constexpr int kMgScalarLoads = 64;
 int mg_default_tile(int n);     // down kernel
 int mg_default_tile_up(int n);  // up kernel
 // Whether `tile` can run a level of n^3 points (the launchers fall back to the

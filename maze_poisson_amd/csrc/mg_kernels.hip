@@ -410,24 +410,20 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const R* __restrict__ b,
     }
 }
 
-// ---- the same fused level kernels with 16-byte loads, column-mapped -------------
-// For n % 8 == 0 and n >= 32.  Every row of a tile's region is read as the
-// 4-float groups of the aligned window [k0 - 4, k0 + TK + 4) (fine levels) or
-// [k0/2 - 4, k0/2 + TK/2 + 4) (the coarse correction): group starts are
-// multiples of 4 and n (nc) is a multiple of 4, so a wrapped group stays
-// contiguous.  LDS rows are RW = TK + 8 floats (a few unused columns).
+// ---- the same fused level kernels with vector loads, column-mapped -------------
+// For every even n >= 32.  Every row of a tile's region is read as the V-float
+// groups of the aligned window [k0 - 4, k0 + TK + 4) (fine levels) or the
+// VE-float groups of [k0/2 - 4, k0/2 + TK/2 + 4) (the coarse correction).  V
+// and VE are the largest of 4, 2 and 1 that divide n and n / 2: group starts
+// are multiples of the width and so is the row length, so a wrapped group
+// stays contiguous and aligned (n % 8 == 0: 16-byte loads throughout; n = 82,
+// 102, 130: 8-byte fine loads and scalar coarse loads).  LDS rows are
+// RW = TK + 8 floats (a few unused columns).
 // A thread owns one (j, k) column of a region and walks its planes i, so the
 // region indices are formed once per thread instead of by division per
-// element, the i-1 / i / i+1 values of a sweep stay in registers (5 LDS reads
-// per point instead of 7), and the prolongation forms each coarse plane's
-// bilinear value once per column.  The arithmetic per point is the scalar
-// kernels', value for value and in the same order, so the level outputs are
-// bit-identical to them (only the order in which the dot partial's terms are
-// added differs).  At 128^3, 8x16x16 tiles: up 20.1 -> 15.0 us (512
-// threads), down 14.2 -> 11.8 us (1024 threads).
 __device__ __forceinline__ int wrap1(int v, int m) { return v < 0 ? v + m : (v >= m ? v - m : v); }
 
-// Loads of a region of PI_ planes x PJ_ rows x RV_ 16-byte groups: thread t
+// Loads of a region of PI_ planes x PJ_ rows x RV_ load groups: thread t
 // takes group e = t % (PJ_ RV_) of planes h, h + G, ... with h = t / (PJ_ RV_).
 template <int PI_, int PJ_, int RV_, int NT, class F>
 __device__ __forceinline__ void plane_groups(F&& f) {
@@ -450,7 +446,18 @@ __device__ __forceinline__ void plane_groups(F&& f) {
     }
 }
 
-template <int TI, int TJ, int TK, int NT, bool Dot>
+// V consecutive floats (V = 4, 2 or 1) from global memory to LDS.
+template <int V>
+__device__ __forceinline__ void copy_group(float* dst, const float* src) {
+    if constexpr (V == 4)
+        *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(src);
+    else if constexpr (V == 2)
+        *reinterpret_cast<float2*>(dst) = *reinterpret_cast<const float2*>(src);
+    else
+        *dst = *src;
+}
+
+template <int TI, int TJ, int TK, int NT, bool Dot, int V, int VE>
 __global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict__ b,
                                                   const float* __restrict__ d2,
                                                   const float* __restrict__ e,
@@ -460,8 +467,8 @@ __global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict
     constexpr int DI = TI + 4, DJ = TJ + 4, DK = TK + 4;
     constexpr int PI = TI + 2, PJ = TJ + 2, PK = TK + 2;
     constexpr int EI = TI / 2 + 3, EJ = TJ / 2 + 3;
-    constexpr int RW = TK + 8, RV = RW / 4;
-    constexpr int RWE = TK / 2 + 8, RVE = RWE / 4;
+    constexpr int RW = TK + 8, RV = RW / V;
+    constexpr int RWE = TK / 2 + 8, RVE = RWE / VE;
     static_assert(DJ * DK <= NT && PJ * PK <= NT, "one thread per column");
     __shared__ __attribute__((aligned(16))) float sd[DI * DJ * RW];
     __shared__ __attribute__((aligned(16))) float sb[PI * PJ * RW];
@@ -472,21 +479,18 @@ __global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict
     const int n2 = n * n, nc = n / 2, nc2 = nc * nc;
     plane_groups<DI, DJ, RV, NT>([&](int i, int j, int g) {
         const int gi = wrap1(T.i0 - 2 + i, n), gj = wrap1(T.j0 - 2 + j, n);
-        const int gk = wrap1(T.k0 - 4 + 4 * g, n);
-        *reinterpret_cast<float4*>(&sd[(i * DJ + j) * RW + 4 * g]) =
-            *reinterpret_cast<const float4*>(&d2[(long)gi * n2 + gj * n + gk]);
+        const int gk = wrap1(T.k0 - 4 + V * g, n);
+        copy_group<V>(&sd[(i * DJ + j) * RW + V * g], &d2[(long)gi * n2 + gj * n + gk]);
     });
     plane_groups<PI, PJ, RV, NT>([&](int i, int j, int g) {
         const int gi = wrap1(T.i0 - 1 + i, n), gj = wrap1(T.j0 - 1 + j, n);
-        const int gk = wrap1(T.k0 - 4 + 4 * g, n);
-        *reinterpret_cast<float4*>(&sb[(i * PJ + j) * RW + 4 * g]) =
-            *reinterpret_cast<const float4*>(&b[(long)gi * n2 + gj * n + gk]);
+        const int gk = wrap1(T.k0 - 4 + V * g, n);
+        copy_group<V>(&sb[(i * PJ + j) * RW + V * g], &b[(long)gi * n2 + gj * n + gk]);
     });
     plane_groups<EI, EJ, RVE, NT>([&](int i, int j, int g) {
         const int gi = wrap1(T.i0 / 2 - 1 + i, nc), gj = wrap1(T.j0 / 2 - 1 + j, nc);
-        const int gk = wrap1(T.k0 / 2 - 4 + 4 * g, nc);
-        *reinterpret_cast<float4*>(&se[(i * EJ + j) * RWE + 4 * g]) =
-            *reinterpret_cast<const float4*>(&e[(long)gi * nc2 + gj * nc + gk]);
+        const int gk = wrap1(T.k0 / 2 - 4 + VE * g, nc);
+        copy_group<VE>(&se[(i * EJ + j) * RWE + VE * g], &e[(long)gi * nc2 + gj * nc + gk]);
     });
     lds_barrier();
     // d = d2 + trilinear(e) on the D region: column (j, k), coarse row bb =
@@ -566,7 +570,7 @@ __global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict
     }
 }
 
-template <int TI, int TJ, int TK, int NT>
+template <int TI, int TJ, int TK, int NT, int V>
 __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restrict__ b,
                                                     float* __restrict__ d2out,
                                                     float* __restrict__ bcoarse, float c1,
@@ -576,7 +580,7 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
     constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
     constexpr int CI = TI / 2, CJ = TJ / 2, CK = TK / 2;
-    constexpr int RW = TK + 8, RV = RW / 4;
+    constexpr int RW = TK + 8, RV = RW / V;
     static_assert(DJ * DK <= NT && RJ * RK <= NT, "one thread per column");
     __shared__ __attribute__((aligned(16))) float sb[BI * BJ * RW];
     __shared__ float sd[DI * DJ * RW];
@@ -586,9 +590,8 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
     const int n2 = n * n;
     plane_groups<BI, BJ, RV, NT>([&](int i, int j, int g) {
         const int gi = wrap1(T.i0 - 3 + i, n), gj = wrap1(T.j0 - 3 + j, n);
-        const int gk = wrap1(T.k0 - 4 + 4 * g, n);
-        *reinterpret_cast<float4*>(&sb[(i * BJ + j) * RW + 4 * g]) =
-            *reinterpret_cast<const float4*>(&b[(long)gi * n2 + gj * n + gk]);
+        const int gk = wrap1(T.k0 - 4 + V * g, n);
+        copy_group<V>(&sb[(i * BJ + j) * RW + V * g], &b[(long)gi * n2 + gj * n + gk]);
     });
     lds_barrier();
     const float cs = c1 + c2, cp = c1 * c2;
@@ -662,40 +665,63 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
     });
 }
 
-inline bool mg_vec_ok(int n) { return n >= 32 && n % 8 == 0; }
+// Load widths of the column-mapped kernels: 0 where they do not apply (n odd
+// or below 32: the halo windows then reach past one period), else the fine
+// width V (4 or 2) times 8 plus the coarse width VE (4, 2 or 1).  scalar: the
+// scalar-load tile kernels instead (tile ids or-ed with kMgScalarLoads).
+inline int mg_vec_widths(int n, bool scalar) {
+    static const bool all_scalar = [] {  // MAZE_MG_SCALAR=1: tuning override
+        const char* e = std::getenv("MAZE_MG_SCALAR");
+        return e && std::atoi(e) != 0;
+    }();
+    if (scalar || all_scalar || n < 32 || n % 2 != 0) return 0;
+    const int nc = n / 2;
+    const int v = n % 4 == 0 ? 4 : 2, ve = nc % 4 == 0 ? 4 : (nc % 2 == 0 ? 2 : 1);
+    return 8 * v + ve;
+}
 
 // Tile shapes (fine points per work-group, threads) of the fused level kernels;
 // Sim takes the defaults per level (mg_default_tile, mg_default_tile_up) unless
 // MAZE_MG_TILES names one.
 template <int TI, int TJ, int TK, int NT>
 static void down_t(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
-                   double c2, CGState* st) {
-    if (mg_vec_ok(n))
-        MAZE_LAUNCH((k_mg_down_vec<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s,
-                    n, b, d2, bc, (float)c1, (float)c2, st);
-    else
-        MAZE_LAUNCH((k_mg_down<TI, TJ, TK, NT>), dim3(mg_tiles<TI, TJ, TK>(n)), dim3(NT), 0, s, n,
-                    b, d2, bc, (float)c1, (float)c2, st);
+                   double c2, CGState* st, bool scalar) {
+    const dim3 g(mg_tiles<TI, TJ, TK>(n));
+    const float f1 = (float)c1, f2 = (float)c2;
+    switch (mg_vec_widths(n, scalar) / 8) {
+        case 4: MAZE_LAUNCH((k_mg_down_vec<TI, TJ, TK, NT, 4>), g, dim3(NT), 0, s, n, b, d2, bc, f1, f2, st); break;
+        case 2: MAZE_LAUNCH((k_mg_down_vec<TI, TJ, TK, NT, 2>), g, dim3(NT), 0, s, n, b, d2, bc, f1, f2, st); break;
+        default: MAZE_LAUNCH((k_mg_down<TI, TJ, TK, NT>), g, dim3(NT), 0, s, n, b, d2, bc, f1, f2, st);
+    }
+}
+
+template <int TI, int TJ, int TK, int NT, bool Dot>
+static void up_dt(hipStream_t s, int n, const float* b, const float* d2, const float* e, float* d,
+                  double c1, double c2, CGState* st, Reduce red, bool scalar) {
+    const dim3 g(mg_tiles<TI, TJ, TK>(n));
+    const float f1 = (float)c1, f2 = (float)c2;
+    switch (mg_vec_widths(n, scalar)) {
+        case 8 * 4 + 4:
+            MAZE_LAUNCH((k_mg_up_vec<TI, TJ, TK, NT, Dot, 4, 4>), g, dim3(NT), 0, s, n, b, d2, e, d, f1, f2, red, st);
+            break;
+        case 8 * 4 + 2:
+            MAZE_LAUNCH((k_mg_up_vec<TI, TJ, TK, NT, Dot, 4, 2>), g, dim3(NT), 0, s, n, b, d2, e, d, f1, f2, red, st);
+            break;
+        case 8 * 2 + 1:
+            MAZE_LAUNCH((k_mg_up_vec<TI, TJ, TK, NT, Dot, 2, 1>), g, dim3(NT), 0, s, n, b, d2, e, d, f1, f2, red, st);
+            break;
+        default:
+            MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, Dot>), g, dim3(NT), 0, s, n, b, d2, e, d, f1, f2, red, st);
+    }
 }
 
 template <int TI, int TJ, int TK, int NT>
 static void up_t(hipStream_t s, int n, const float* b, const float* d2, const float* e, float* d,
-                 double c1, double c2, CGState* st, int dot, Reduce red) {
-    const dim3 g(mg_tiles<TI, TJ, TK>(n));
-    if (mg_vec_ok(n)) {
-        if (dot)
-            MAZE_LAUNCH((k_mg_up_vec<TI, TJ, TK, NT, true>), g, dim3(NT), 0, s, n, b, d2, e, d,
-                        (float)c1, (float)c2, red, st);
-        else
-            MAZE_LAUNCH((k_mg_up_vec<TI, TJ, TK, NT, false>), g, dim3(NT), 0, s, n, b, d2, e, d,
-                        (float)c1, (float)c2, red, st);
-    } else if (dot) {
-        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, true>), g, dim3(NT), 0, s, n, b, d2, e, d, (float)c1,
-                    (float)c2, red, st);
-    } else {
-        MAZE_LAUNCH((k_mg_up<TI, TJ, TK, NT, false>), g, dim3(NT), 0, s, n, b, d2, e, d,
-                    (float)c1, (float)c2, red, st);
-    }
+                 double c1, double c2, CGState* st, int dot, Reduce red, bool scalar) {
+    if (dot)
+        up_dt<TI, TJ, TK, NT, true>(s, n, b, d2, e, d, c1, c2, st, red, scalar);
+    else
+        up_dt<TI, TJ, TK, NT, false>(s, n, b, d2, e, d, c1, c2, st, red, scalar);
 }
 
 // ---- plane-marching level kernels ---------------------------------------------
@@ -968,6 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up_march(int n, const float* __re
 }
 
 bool mg_tile_ok(int tile, int n) {
+    if (tile >= kMgScalarLoads && tile < kMgScalarLoads + kMgMarch16) tile -= kMgScalarLoads;
     if (tile < 0 || tile >= kMgTiles) return false;
     if (tile == kMgMarch16) return n >= 32 && n % 16 == 0;
     if (tile == kMgMarch8) return n >= 32 && n % 16 == 0;
@@ -999,6 +1026,8 @@ const char* mg_tile_name(int tile) {
     static const char* names[kMgTiles] = {"8x16x16/1024", "8x16x16/512", "8x8x16/512",
                                           "4x8x8/512",    "4x8x8/256",   "march16x16/S16",
                                           "march16x16/S8"};
+    if (tile >= kMgScalarLoads && tile < kMgScalarLoads + kMgMarch16)
+        tile -= kMgScalarLoads;  // the same shape with scalar loads
     return tile >= 0 && tile < kMgTiles ? names[tile] : "?";
 }
 
@@ -1015,21 +1044,22 @@ int mg_default_tile_up(int n) {  // 64^3: 5.55 us with 256 threads against 6.22 
 void launch_mg_down(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
                     double c1, double c2, CGState* st) {
     if (!mg_tile_ok(tile, n)) tile = mg_default_tile(n);
-    switch (tile) {
+    const bool sc = tile >= kMgScalarLoads;
+    switch (sc ? tile - kMgScalarLoads : tile) {
         case kMgMarch16: down_m<16>(s, n, b, d2, bc, c1, c2, st); break;
         case kMgMarch8: down_m<8>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st); break;
-        case kMgT4x8x8_512: down_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st); break;
-        default: down_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st); break;
+        case kMgT8x16x16_1024: down_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st, sc); break;
+        case kMgT8x16x16_512: down_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st, sc); break;
+        case kMgT8x8x16_512: down_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st, sc); break;
+        case kMgT4x8x8_512: down_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st, sc); break;
+        default: down_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st, sc); break;
     }
 }
 
 // Work-groups of the up kernel at a level (the partials its dot writes).
 int mg_up_blocks(int n, int tile) {
     if (!mg_tile_ok(tile, n)) tile = mg_default_tile_up(n);
-    switch (tile) {
+    switch (tile >= kMgScalarLoads ? tile - kMgScalarLoads : tile) {
         case kMgMarch16: return (n / kMarchC) * (n / kMarchC) * (n / 16);
         case kMgMarch8: return (n / kMarchC) * (n / kMarchC) * (n / 8);
         case kMgT8x16x16_1024:
@@ -1042,14 +1072,15 @@ int mg_up_blocks(int n, int tile) {
 void launch_mg_up(hipStream_t s, int n, int tile, const float* b, const float* d2, const float* e,
                   float* d, double c1, double c2, CGState* st, int dot, Reduce red) {
     if (!mg_tile_ok(tile, n)) tile = mg_default_tile_up(n);
-    switch (tile) {
+    const bool sc = tile >= kMgScalarLoads;
+    switch (sc ? tile - kMgScalarLoads : tile) {
         case kMgMarch16: up_m<16>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
         case kMgMarch8: up_m<8>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
-        case kMgT8x16x16_1024: up_t<8, 16, 16, 1024>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
-        case kMgT8x16x16_512: up_t<8, 16, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
-        case kMgT8x8x16_512: up_t<8, 8, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
-        case kMgT4x8x8_512: up_t<4, 8, 8, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
-        default: up_t<4, 8, 8, 256>(s, n, b, d2, e, d, c1, c2, st, dot, red); break;
+        case kMgT8x16x16_1024: up_t<8, 16, 16, 1024>(s, n, b, d2, e, d, c1, c2, st, dot, red, sc); break;
+        case kMgT8x16x16_512: up_t<8, 16, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red, sc); break;
+        case kMgT8x8x16_512: up_t<8, 8, 16, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red, sc); break;
+        case kMgT4x8x8_512: up_t<4, 8, 8, 512>(s, n, b, d2, e, d, c1, c2, st, dot, red, sc); break;
+        default: up_t<4, 8, 8, 256>(s, n, b, d2, e, d, c1, c2, st, dot, red, sc); break;
     }
 }
 

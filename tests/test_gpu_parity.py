@@ -444,8 +444,11 @@ def _prolong_trilinear(e):
     return e
 
 
-MG_TILE_CASES = ([(n, -1) for n in (8, 16, 20, 64, 72)] + [(n, t) for n in (20, 64) for t in range(5)]
-                 + [(n, t) for n in (32, 64) for t in (5, 6)])  # 5, 6: plane-marching kernels
+# 34, 50, 130: n = 2 mod 4 (8-byte fine loads, scalar coarse loads); 36: n = 4 mod 8 (16-byte fine
+# loads, 8-byte coarse loads); 5, 6: plane-marching kernels.
+MG_TILE_CASES = ([(n, -1) for n in (8, 16, 20, 34, 36, 50, 64, 72, 130)]
+                 + [(n, t) for n in (20, 34, 36, 64) for t in range(5)]
+                 + [(n, t) for n in (32, 64) for t in (5, 6)])
 
 
 @pytest.mark.parametrize("n,tile", MG_TILE_CASES)
@@ -809,6 +812,32 @@ def test_md_sequence_pb_radii(hip, oracle, method):
         assert rel(g[k], o[k]) <= 1e-7, (k, rel(g[k], o[k]))
     assert rel(g["pos"], o["pos"]) <= 1e-11
     assert_observables(g, o, s)
+
+
+@pytest.mark.parametrize("n", [32, 34, 36, 82, 130])
+@pytest.mark.parametrize("tile", [0, 1, 3, 4])
+def test_mg_vector_loads_bitwise_equal_scalar_loads(hip_raw, n, tile):
+    """The column-mapped level kernels (16-, 8- or 4-byte loads by n mod 8) compute every
+    value as the scalar-load kernels of the same tile shape (tile id + 64): d2, the coarse
+    right-hand side and the up kernel's output agree bit for bit, partial tiles included."""
+    c1, c2 = (w / 6.0 for w in chebyshev_weights())
+    g = np.random.default_rng(700 + n)
+    b = g.standard_normal((n, n, n)).astype(np.float32)
+    d2 = g.standard_normal((n, n, n)).astype(np.float32)
+    e = g.standard_normal((n // 2,) * 3).astype(np.float32)
+    down, up = {}, {}
+    for t in (tile, 64 + tile):
+        d2o = np.zeros_like(b)
+        bc = np.zeros((n // 2,) * 3, np.float32)
+        assert hip_raw.maze_test_mg_down2(b, d2o, bc, n, c1, c2, t) == 0
+        down[t] = (d2o, bc)
+        d = np.zeros_like(b)
+        assert hip_raw.maze_test_mg_up2(b, d2, e, d, n, c1, c2, t) == 0
+        up[t] = d
+    assert np.array_equal(down[tile][0], down[64 + tile][0])
+    assert np.array_equal(down[tile][1], down[64 + tile][1])
+    assert np.array_equal(up[tile], up[64 + tile])
+    assert np.abs(up[tile]).max() > 0
 
 
 @pytest.mark.parametrize("n", [32, 64, 128])
