@@ -367,8 +367,49 @@ int maze_engine_submit(maze_engine *e, double arrival_s, int n_grid, double L, d
                        int precision, int np, int n_typ, const int *types, const double *pos,
                        const double *vel, const double *mass, const double *charges,
                        const double *tf_params, double dt, int steps);
+/* Physics of a queued request, replacing submit's defaults (MAZE-LCG without
+ * preconditioner, CIC, Tosi-Fumi, velocity Verlet without thermostat); the
+ * single-replica path's functions with the same arguments are named.
+ * grid_type, precond: solver_initialize_grid's numbering (LCG, MULTIGRID,
+ * MAZE-LCG, MAZE-MULTIGRID; NONE, JACOBI, MG for the two CG methods); the
+ * mixed precisions apply to MAZE-LCG without preconditioner, the rest runs in
+ * FP64.  ca_scheme: solver_initialize_particles' CIC / SPL_QUADR / SPL_CUBIC.
+ * eps_s, eps_int: solver_initialize_grid.  itg_type, T, gamma, thermostat:
+ * solver_initialize_integrator.  seed (if seed_set): solver_set_rng_seed, the
+ * OVRVO noise stream.  rescale: solver_rescale_velocities after the two
+ * start-up steps (maze_poisson/solver.py:408-409).  Returns 0, -1 for an
+ * unknown or already admitted request or an invalid value, -2 if the request
+ * then exceeds the budget (the physics is left unchanged). */
+int maze_engine_set_physics(maze_engine *e, int id, int grid_type, int precond, int ca_scheme,
+                            double eps_s, double eps_int, int itg_type, double T, double gamma,
+                            int thermostat, int seed_set, unsigned long long seed, int rescale);
+/* Linear Poisson-Boltzmann on given maps: solver_initialize_grid_pois_boltz(0,
+ * kbar2, 0) then solver_set_eps_k2 (n^3 face-centred values each).  Returns
+ * as maze_engine_set_physics. */
+int maze_engine_set_pb_maps(maze_engine *e, int id, double kbar2, const double *eps_x,
+                            const double *eps_y, const double *eps_z, const double *k2);
+/* Linear Poisson-Boltzmann with the maps rebuilt from the ion radii every step
+ * and the boundary forces in the total: solver_initialize_grid_pois_boltz(w,
+ * kbar2, nonpolar) then solver_initialize_particles_pois_boltz(gamma_np,
+ * beta_np, radii: np values).  Returns as maze_engine_set_physics. */
+int maze_engine_set_pb_radii(maze_engine *e, int id, double w, double kbar2, int nonpolar,
+                             double gamma_np, double beta_np, const double *radii);
+/* Withdraws a queued request before maze_engine_run.  Returns 0 or -1. */
+int maze_engine_cancel(maze_engine *e, int id);
+/* Keep the final field phi of a queued request for maze_engine_field (n^3
+ * doubles of host memory until the engine is destroyed).  Returns 0 or -1. */
+int maze_engine_keep_field(maze_engine *e, int id, int on);
 /* Runs until every queued request has finished; returns 0. */
 int maze_engine_run(maze_engine *e);
+/* Observables of a finished request after its last step, as the
+ * single-replica getters give them: get_kinetic_energy, get_temperature,
+ * get_energy_elec, the Tosi-Fumi potential energy (solver_compute_forces_noel)
+ * and get_momentum (3 values).  Writes up to max of these 7 values and returns
+ * their number, or -1 if the request has not finished. */
+int maze_engine_observables(maze_engine *e, int id, double *out, int max);
+/* The final field phi (n^3) of a finished request kept with
+ * maze_engine_keep_field.  Returns 0, or -1. */
+int maze_engine_field(maze_engine *e, int id, double *phi);
 /* Final positions, velocities and total forces of a finished request (np x 3
  * each; any pointer may be NULL).  Returns 0, or -1 if not finished. */
 int maze_engine_result(maze_engine *e, int id, double *pos, double *vel, double *fcs_tot);

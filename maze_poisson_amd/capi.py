@@ -120,7 +120,18 @@ EXTENSION_ABI = [
     ("maze_engine_submit", _c_int, [ctypes.c_void_p, _c_double, _c_int, _c_double, _c_double,
                                     _c_int, _c_int, _c_int, _I1, _D2, _D2, _D1, _D1, _D1,
                                     _c_double, _c_int]),
+    ("maze_engine_set_physics", _c_int, [ctypes.c_void_p, _c_int, _c_int, _c_int, _c_int,
+                                         _c_double, _c_double, _c_int, _c_double, _c_double,
+                                         _c_int, _c_int, ctypes.c_ulonglong, _c_int]),
+    ("maze_engine_set_pb_maps", _c_int, [ctypes.c_void_p, _c_int, _c_double, _D3, _D3, _D3,
+                                         _D3]),
+    ("maze_engine_set_pb_radii", _c_int, [ctypes.c_void_p, _c_int, _c_double, _c_double, _c_int,
+                                          _c_double, _c_double, _D1]),
+    ("maze_engine_keep_field", _c_int, [ctypes.c_void_p, _c_int, _c_int]),
+    ("maze_engine_cancel", _c_int, [ctypes.c_void_p, _c_int]),
     ("maze_engine_run", _c_int, [ctypes.c_void_p]),
+    ("maze_engine_observables", _c_int, [ctypes.c_void_p, _c_int, _D1, _c_int]),
+    ("maze_engine_field", _c_int, [ctypes.c_void_p, _c_int, _D3]),
     ("maze_engine_result", _c_int, [ctypes.c_void_p, _c_int, _D2, _D2, _D2]),
     ("maze_engine_request_stats", _c_int, [ctypes.c_void_p, _c_int, _D1, _c_int]),
     ("maze_engine_stats", _c_int, [ctypes.c_void_p, _D1, _c_int]),

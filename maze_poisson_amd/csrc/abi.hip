@@ -155,7 +155,7 @@ double get_kinetic_energy() { return S().kinetic(); }
 double get_energy_elec() { return S().energy_elec(); }
 double get_energy_elec_qphi() { return S().energy_qphi(); }
 void get_momentum(double* o) { S().momentum(o); }
-double get_temperature() { return 2 * S().kinetic() / (3 * S().p.np * 3.1668e-6); }
+double get_temperature() { return S().temperature(); }
 
 // ---- extensions -------------------------------------------------------------------
 void solver_md_loop_iter() { S().md_step(); }

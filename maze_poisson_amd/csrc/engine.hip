@@ -45,6 +45,21 @@ double quantile(std::vector<double> v, double q) {
 
 }  // namespace
 
+// Physics of a request beyond maze_engine_submit's defaults (MAZE-LCG without
+// a preconditioner, CIC, Tosi-Fumi, velocity Verlet without thermostat).
+struct Phys {
+    int grid_type = kMazeLCG, precond = kPNone, cas = 0;
+    double eps_s = 1.0, eps_int = 1.0;
+    int itype = kVerlet, thermostat = 0, rescale = 0;
+    double T = 0.0, gamma = 0.0;
+    int seed_set = 0;
+    unsigned long long seed = 0;
+    int pb = 0;  // 0 Poisson, 1 PB on given maps, 2 PB maps from the ion radii every step
+    double w = 0.0, kbar2 = 0.0, gamma_np = 0.0, beta_np = 0.0;
+    int nonpolar = 0;
+    std::vector<double> ex, ey, ez, k2, radii;
+};
+
 struct Request {
     // inputs
     double arrival = 0;
@@ -52,8 +67,10 @@ struct Request {
     double L = 0, tol = 0, dt = 0;
     std::vector<int> types;
     std::vector<double> pos, vel, mass, charges, tf;
+    Phys ph;
+    int keep_field = 0;
     // progress
-    int state = 0;  // 0 waiting, 1 running, 2 done
+    int state = 0;  // 0 waiting, 1 running, 2 done, 3 withdrawn
     int used_precision = kFP64;
     int done = 0, failures = 0;
     long long est_bytes = 0, meas_bytes = -1, arena_used = 0, arena_overflow = 0, free0 = 0;
@@ -64,14 +81,22 @@ struct Request {
     std::vector<double> lat;  // seconds per step, as observed by the loop
     std::unique_ptr<Sim> sim;
     std::pair<hipStream_t, hipStream_t> streams{nullptr, nullptr};
-    std::vector<double> out_pos, out_vel, out_f;
+    std::vector<double> out_pos, out_vel, out_f, out_phi;
+    double obs[7] = {0, 0, 0, 0, 0, 0, 0};  // kinetic, temperature, E_elec, V_TF, momentum
 };
 
 // The precision a replica of an n^3 grid runs: the FP32 multigrid needs an
-// even grid that halves to at most 128 points (Sim::mg_sizes); otherwise the
-// FP64 CG, which is faster than the FP32 Jacobi-CG inside refinement.
-static int precision_for(int n, int want) {
-    if (want == kMixedMG && Sim::mg_sizes(n).empty()) return kFP64;
+// even grid that halves to at most 128 points (Sim::mg_sizes), and with PB a
+// coarsest level of at most 16 points; otherwise the FP64 CG, which is faster
+// than the FP32 Jacobi-CG inside refinement.  The mixed paths replace the
+// MAZE-LCG solve without preconditioner; the reference's other methods and
+// preconditioners run in FP64 as asked.
+static int precision_for(int n, int want, const Phys& ph) {
+    if (ph.grid_type != kMazeLCG || ph.precond != kPNone) return kFP64;
+    if (want == kMixedMG) {
+        const std::vector<int> sz = Sim::mg_sizes(n);
+        if (sz.empty() || (ph.pb && sz.back() > kMgCoarseLds)) return kFP64;
+    }
     return want;
 }
 
@@ -79,30 +104,56 @@ static int precision_for(int n, int want) {
 // and solver state, the grids, the multigrid levels, the particle arrays, the
 // Tosi-Fumi partials and the charge-spread lists (each allocation 256-byte
 // aligned).
-static long long arena_bytes(int n, int np, int precision) {
+static long long arena_bytes(int n, int np, int precision, const Phys& ph) {
     const long long G = (long long)n * n * n;
     // reduction partials of both streams and of the deferred inner iteration, scalars
     long long b = 8LL * 8 * (1 << 15) + (64 << 10);
+    int allocs = 64;
     b += 9 * 8 * G;                                   // q, y, phi, phi_prev, phi*, sigma, r, p0, p1
+    if (ph.grid_type == kMG || ph.grid_type == kMazeMG ||
+        (ph.grid_type == kMazeLCG && ph.precond == kPMG)) {
+        const long long Gc = G / 8;                   // the reference's two-grid cycle
+        b += 8 * G + 5 * 8 * Gc;                      // fine residual, coarse rhs / eps / r / p
+        if (ph.grid_type == kMazeLCG) b += 8 * G;     // preconditioned residual
+        if (ph.pb && ph.grid_type != kMazeLCG) b += 4 * 8 * Gc;  // coarse PB coefficients
+    }
+    if (ph.pb) b += 4 * 8 * G;                        // dielectric maps and screening
     if (precision != kFP64) b += 8 * G + 4 * 4 * G;   // second y, FP32 d, r, p0, p1
     if (precision == kMixedMG) {
         const std::vector<int> sz = Sim::mg_sizes(n);
         for (size_t l = 0; l < sz.size(); l++) {
             const long long g = (long long)sz[l] * sz[l] * sz[l];
-            b += (l == 0 ? 2 : 3) * 4 * g;  // level 0 borrows the FP32 residual as b
+            if (ph.pb)  // FP32 coefficients, inverse diagonal, d, t (+ b below the finest)
+                b += (l == 0 ? 7 : 8) * 4 * g;
+            else
+                b += (l == 0 ? 2 : 3) * 4 * g;  // level 0 borrows the FP32 residual as b
         }
-        if (!sz.empty()) {
+        allocs += 8 * (int)sz.size();
+        if (!sz.empty() && !ph.pb) {
             const long long c = sz.back();
             b += 8 * (c * c + c * c * c);  // coarsest eigenbasis and inverse eigenvalues
             if (c > kMgCoarseLds) b += 2 * 8 * c * c * c;  // its work arrays
         }
     }
-    // types, pos/vel/forces, mass/charges, base cell, pair counts, the CIC
-    // contributions with their node and link entries, the pair energies
-    b += (long long)np * (4 + 5 * 24 + 2 * 8 + 12 + 4 + 8 * (8 + 4 + 4) + 8);
+    // types, pos/vel/forces, mass/charges, base cell, pair counts, the charge
+    // spread's contributions with their node and link entries (8 nodes per ion
+    // for CIC, 64 for the splines), the pair energies
+    const long long nn = ph.cas == 0 ? 8 : 64;
+    b += (long long)np * (4 + 5 * 24 + 2 * 8 + 12 + 4 + nn * (8 + 4 + 4) + 8);
     b += 8 * G;                                           // spread list heads
     b += 8LL * tf_part_doubles(np) + 7LL * 8 * 16 * 16;  // pair table, up to 16 types
-    return b + 64LL * (long long)kArenaAlign;  // alignment of up to 64 allocations
+    if (ph.pb == 2) {
+        // radii, squared reach, map / force reach, bin keys and ids (sorted and
+        // not), the three boundary forces, the key sort's work space
+        const int nb = (n + kPbTile - 1) / kPbTile;
+        const unsigned long bins = (unsigned long)nb * nb * nb;
+        int key_bits = 1;
+        while ((1ul << key_bits) < bins) key_bits++;
+        b += (long long)np * (3 * 8 + 2 * 4 + 4 * 4 + 3 * 24);
+        b += (long long)std::max<size_t>(pb_sort_bytes(std::max(np, 1), key_bits), 16);
+        allocs += 16;
+    }
+    return b + (long long)allocs * (long long)kArenaAlign;  // alignment of each allocation
 }
 
 // Device memory a replica takes: its arena (hipMalloc rounds to 2 MiB) plus
@@ -111,9 +162,9 @@ static long long arena_bytes(int n, int np, int precision) {
 // kernels load their code objects (17 MiB) and the first streams set up the
 // hardware queues (184 MiB).
 constexpr long long kReplicaExtra = 10LL << 20;
-static long long estimate_bytes(int n, int np, int precision) {
+static long long estimate_bytes(int n, int np, int precision, const Phys& ph) {
     const long long mib2 = 2LL << 20;
-    return (arena_bytes(n, np, precision) + mib2 - 1) / mib2 * mib2 + kReplicaExtra;
+    return (arena_bytes(n, np, precision, ph) + mib2 - 1) / mib2 * mib2 + kReplicaExtra;
 }
 
 // Hardware queues the runtime gives one process (its GPU_MAX_HW_QUEUES, 4 by
@@ -190,25 +241,37 @@ struct Engine {
     // Set-up of an admitted request: its arena, the replica state, the two
     // start-up steps with cold solves and the first step's issue.
     void admit(Request& r) {
-        r.used_precision = precision_for(r.n, r.precision);
+        r.used_precision = precision_for(r.n, r.precision, r.ph);
         // Streams come from the pool, created four pairs at a time and counted
         // apart from the replica's own memory.
         r.streams = take_streams(r.stream_bytes);
         size_t free0 = 0, total = 0;
         MAZE_HIP(hipMemGetInfo(&free0, &total));
         r.arena = std::make_unique<DevArena>();
-        r.arena->size = (size_t)arena_bytes(r.n, r.np, r.used_precision);
+        r.arena->size = (size_t)arena_bytes(r.n, r.np, r.used_precision, r.ph);
         MAZE_HIP(hipMalloc(&r.arena->base, r.arena->size));
         arena_install(r.arena.get());
         r.sim = std::make_unique<Sim>(r.streams.first, r.streams.second);
         Sim& s = *r.sim;
         const double h = r.L / r.n;
-        s.init_grid(r.n, r.L, h, r.tol, 1.0, 1.0, kMazeLCG, kPNone);
+        const Phys& ph = r.ph;
+        // The single-replica set-up order (maze_poisson/solver.py:97-106):
+        // grid, PB grid and maps, particles, PB radii, integrator.
+        s.init_grid(r.n, r.L, h, r.tol, ph.eps_s, ph.eps_int, ph.grid_type, ph.precond);
+        if (ph.pb) {
+            s.init_pb(ph.w, ph.kbar2, ph.nonpolar);
+            if (ph.pb == 1) s.set_eps_k2(ph.ex.data(), ph.ey.data(), ph.ez.data(), ph.k2.data());
+        }
         s.precision = r.used_precision;
         s.mg_autotune = 1;
-        s.init_particles(r.n, r.n_typ, r.L, h, r.np, 0, r.types.data(), r.pos.data(),
+        s.init_particles(r.n, r.n_typ, r.L, h, r.np, ph.cas, r.types.data(), r.pos.data(),
                          r.vel.data(), r.mass.data(), r.charges.data(), r.tf.data());
-        s.init_integrator(r.dt, 0.0, 0.0, kVerlet, 0);
+        if (ph.pb == 2) s.init_pb_particles(ph.gamma_np, ph.beta_np, ph.radii.data());
+        s.init_integrator(r.dt, ph.T, ph.gamma, ph.itype, ph.thermostat);
+        if (ph.seed_set) {
+            s.seed = ph.seed;
+            s.calls = 0;
+        }
         // Stream capture is invalidated by HIP calls that other threads make
         // meanwhile (hipStreamEndCapture reports it), so only a one-thread run
         // replays graphs.
@@ -218,6 +281,7 @@ struct Engine {
         // run through the loop like MD steps: the first cold solve is queued
         // here, and boot() continues each time its state has arrived.
         s.update_charges(true);  // checked when the solve's state arrives
+        s.update_eps_k2();       // maps from the radii (a no-op otherwise)
         s.init_field_issue();
         arena_install(nullptr);
         r.free0 = (long long)free0;
@@ -234,16 +298,19 @@ struct Engine {
         s.check_charge();
         s.forces_elec();
         s.forces_noel(false);
+        if (r.ph.pb == 2) s.forces_pb();
         s.forces_tot();
         if (r.boot == 1) {
             s.part1();
             s.update_charges(true);
+            s.update_eps_k2();
             s.init_field_issue();
             r.boot = 2;
             arena_install(nullptr);
             return false;
         }
         s.part2();
+        if (r.ph.rescale) s.rescale_velocities();  // maze_poisson/solver.py:408-409
         r.boot = 0;
         r.t_admit = r.t_last = now_s() - t0;
         if (r.steps > 0) s.step_issue();  // allocates the mixed solver's arrays
@@ -271,6 +338,18 @@ struct Engine {
                                     s.stream));
         }
         s.sync();
+        // Observables at the last step: kinetic energy, temperature, electrostatic
+        // energy, Tosi-Fumi potential energy, momentum (the single-replica getters).
+        r.obs[0] = s.kinetic();
+        r.obs[1] = r.np > 0 ? s.temperature() : 0.0;
+        r.obs[2] = s.energy_elec();
+        r.obs[3] = s.pe_tf();
+        s.momentum(r.obs + 4);
+        if (r.keep_field) {
+            r.out_phi.resize((size_t)s.G);
+            MAZE_HIP(hipMemcpy(r.out_phi.data(), s.phi, s.G * sizeof(double),
+                               hipMemcpyDeviceToHost));
+        }
         r.sim.reset();  // waits for both streams
         give_streams(r.streams);
         arena_forget(r.arena.get());
@@ -426,11 +505,12 @@ void maze_engine_set_mixed_init(maze_engine* h, int on) {
 void maze_engine_destroy(maze_engine* h) { delete reinterpret_cast<Engine*>(h); }
 
 long long maze_engine_estimate_bytes(int n_grid, int np, int precision) {
-    return maze::estimate_bytes(n_grid, np, maze::precision_for(n_grid, precision));
+    const maze::Phys ph;
+    return maze::estimate_bytes(n_grid, np, maze::precision_for(n_grid, precision, ph), ph);
 }
 
 int maze_engine_precision_for(int n_grid, int precision) {
-    return maze::precision_for(n_grid, precision);
+    return maze::precision_for(n_grid, precision, maze::Phys{});
 }
 
 int maze_engine_submit(maze_engine* h, double arrival_s, int n_grid, double L, double tol,
@@ -459,7 +539,7 @@ int maze_engine_submit(maze_engine* h, double arrival_s, int n_grid, double L, d
     r.mass.assign(mass, mass + np);
     r.charges.assign(charges, charges + np);
     r.tf.assign(tf_params, tf_params + 5L * n_typ * n_typ);
-    r.est_bytes = maze::estimate_bytes(n_grid, np, maze::precision_for(n_grid, precision));
+    r.est_bytes = maze::estimate_bytes(n_grid, np, maze::precision_for(n_grid, precision, r.ph), r.ph);
     if (r.est_bytes > e.budget) {
         std::fprintf(stderr,
                      "maze_engine_submit: request needs about %lld bytes, above the budget of "
@@ -469,6 +549,126 @@ int maze_engine_submit(maze_engine* h, double arrival_s, int n_grid, double L, d
     }
     e.reqs.push_back(std::move(r));
     return (int)e.reqs.size() - 1;
+}
+
+// A queued request whose physics may still change, or nullptr.
+static Request* queued(maze_engine* h, int id) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    if (id < 0 || id >= (int)e.reqs.size() || e.reqs[id].state != 0) return nullptr;
+    return &e.reqs[id];
+}
+
+// The estimate of a request after a change of its physics: -2 (and the change
+// undone) if it no longer fits the budget.
+static int reestimate(maze_engine* h, Request& r, const maze::Phys& old) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    const long long est = maze::estimate_bytes(r.n, r.np, maze::precision_for(r.n, r.precision, r.ph), r.ph);
+    if (est > e.budget) {
+        std::fprintf(stderr, "maze_engine: request needs about %lld bytes, above the budget of %lld\n",
+                     est, e.budget);
+        r.ph = old;
+        return -2;
+    }
+    r.est_bytes = est;
+    return 0;
+}
+
+int maze_engine_set_physics(maze_engine* h, int id, int grid_type, int precond, int ca_scheme,
+                            double eps_s, double eps_int, int itg_type, double T, double gamma,
+                            int thermostat, int seed_set, unsigned long long seed, int rescale) {
+    Request* r = queued(h, id);
+    const bool mg_type = grid_type == maze::kMG || grid_type == maze::kMazeMG ||
+                         (grid_type == maze::kMazeLCG && precond == maze::kPMG);
+    const bool ok = r && (grid_type == maze::kLCG || grid_type == maze::kMG ||
+                          grid_type == maze::kMazeLCG || grid_type == maze::kMazeMG) &&
+                    precond >= maze::kPNone && precond <= maze::kPMG &&
+                    (precond == maze::kPNone || grid_type == maze::kLCG || grid_type == maze::kMazeLCG) &&
+                    (!mg_type || (r->n % 2 == 0 && r->n >= 32)) && ca_scheme >= 0 && ca_scheme <= 2 &&
+                    (itg_type == maze::kOVRVO || itg_type == maze::kVerlet) && eps_s > 0 && eps_int > 0;
+    if (!ok) {
+        std::fprintf(stderr, "maze_engine_set_physics: invalid request or value\n");
+        return -1;
+    }
+    const maze::Phys old = r->ph;
+    maze::Phys& ph = r->ph;
+    ph.grid_type = grid_type;
+    ph.precond = precond;
+    ph.cas = ca_scheme;
+    ph.eps_s = eps_s;
+    ph.eps_int = eps_int;
+    ph.itype = itg_type;
+    ph.T = T;
+    ph.gamma = gamma;
+    ph.thermostat = thermostat != 0;
+    ph.seed_set = seed_set != 0;
+    ph.seed = seed;
+    ph.rescale = rescale != 0;
+    return reestimate(h, *r, old);
+}
+
+int maze_engine_set_pb_maps(maze_engine* h, int id, double kbar2, const double* eps_x,
+                            const double* eps_y, const double* eps_z, const double* k2) {
+    Request* r = queued(h, id);
+    if (!r || !eps_x || !eps_y || !eps_z || !k2) {
+        std::fprintf(stderr, "maze_engine_set_pb_maps: invalid request or map\n");
+        return -1;
+    }
+    const maze::Phys old = r->ph;
+    maze::Phys& ph = r->ph;
+    const size_t G = (size_t)r->n * r->n * r->n;
+    ph.pb = 1;
+    ph.w = 0.0;
+    ph.kbar2 = kbar2;
+    ph.nonpolar = 0;
+    ph.ex.assign(eps_x, eps_x + G);
+    ph.ey.assign(eps_y, eps_y + G);
+    ph.ez.assign(eps_z, eps_z + G);
+    ph.k2.assign(k2, k2 + G);
+    ph.radii.clear();
+    return reestimate(h, *r, old);
+}
+
+int maze_engine_set_pb_radii(maze_engine* h, int id, double w, double kbar2, int nonpolar,
+                             double gamma_np, double beta_np, const double* radii) {
+    Request* r = queued(h, id);
+    bool ok = r && radii && w >= 0;
+    // The radius plus the transition width must span less than the grid (Sim::init_pb_particles).
+    for (int i = 0; ok && i < r->np; i++) {
+        const double hh = r->L / r->n;
+        ok = radii[i] >= 0 && 2 * ((int)std::floor((radii[i] + w) / hh) + 2) + 1 <= r->n;
+    }
+    if (!ok) {
+        std::fprintf(stderr, "maze_engine_set_pb_radii: invalid request, width or radius\n");
+        return -1;
+    }
+    const maze::Phys old = r->ph;
+    maze::Phys& ph = r->ph;
+    ph.pb = 2;
+    ph.w = w;
+    ph.kbar2 = kbar2;
+    ph.nonpolar = nonpolar != 0;
+    ph.gamma_np = gamma_np;
+    ph.beta_np = beta_np;
+    ph.radii.assign(radii, radii + r->np);
+    ph.ex.clear();
+    ph.ey.clear();
+    ph.ez.clear();
+    ph.k2.clear();
+    return reestimate(h, *r, old);
+}
+
+int maze_engine_cancel(maze_engine* h, int id) {
+    Request* r = queued(h, id);
+    if (!r) return -1;
+    r->state = 3;  // withdrawn: never admitted
+    return 0;
+}
+
+int maze_engine_keep_field(maze_engine* h, int id, int on) {
+    Request* r = queued(h, id);
+    if (!r) return -1;
+    r->keep_field = on != 0;
+    return 0;
 }
 
 int maze_engine_run(maze_engine* h) { return reinterpret_cast<Engine*>(h)->run(); }
@@ -481,6 +681,24 @@ int maze_engine_result(maze_engine* h, int id, double* pos, double* vel, double*
     if (pos) std::memcpy(pos, r.out_pos.data(), b);
     if (vel) std::memcpy(vel, r.out_vel.data(), b);
     if (fcs_tot) std::memcpy(fcs_tot, r.out_f.data(), b);
+    return 0;
+}
+
+int maze_engine_observables(maze_engine* h, int id, double* out, int max) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    if (id < 0 || id >= (int)e.reqs.size() || e.reqs[id].state != 2 || !out) return -1;
+    const int k = std::max(0, std::min(max, 7));
+    std::memcpy(out, e.reqs[id].obs, k * sizeof(double));
+    return k;
+}
+
+int maze_engine_field(maze_engine* h, int id, double* phi) {
+    Engine& e = *reinterpret_cast<Engine*>(h);
+    if (id < 0 || id >= (int)e.reqs.size() || e.reqs[id].state != 2 || !phi ||
+        e.reqs[id].out_phi.empty())
+        return -1;
+    const Request& r = e.reqs[id];
+    std::memcpy(phi, r.out_phi.data(), r.out_phi.size() * sizeof(double));
     return 0;
 }
 

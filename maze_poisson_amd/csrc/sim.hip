@@ -1612,6 +1612,15 @@ double Sim::kinetic() {
     return ps_host->kinetic;
 }
 
+double Sim::temperature() { return 2 * kinetic() / (3 * p.np * kBoltz); }
+
+double Sim::pe_tf() {
+    MAZE_HIP(hipMemcpyAsync(&ps_host->pe_tf, &ps->pe_tf, sizeof(double), hipMemcpyDeviceToHost,
+                            stream));
+    sync();
+    return ps_host->pe_tf;
+}
+
 int Sim::check_thermostat() {
     if (enabled) {
         const double temp = 2 * kinetic() / (3 * p.np * kBoltz);

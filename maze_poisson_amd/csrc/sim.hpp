@@ -328,6 +328,8 @@ struct Sim {
     void rescale_velocities();
     int check_thermostat();
     double kinetic();
+    double temperature();  // 2 K / (3 N k_B)
+    double pe_tf();        // Tosi-Fumi potential energy of the last pair-force pass
     double energy_elec();
     double energy_qphi();
     void momentum(double* out);

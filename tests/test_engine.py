@@ -144,3 +144,103 @@ def test_engine_memory_estimate(hip):
                 assert rs["stream_bytes"] >= 0, rs
             else:
                 assert rs["stream_bytes"] == 0, rs
+
+
+# Physics beyond the defaults, each through the engine and through the single-replica
+# path with the same configuration: (ions, grid, system seed, density, precision,
+# RunConfig fields, OVRVO seed, rescale after start-up).
+GAMMA_1PS = 2.4189e-5  # 1 / ps in a.u.
+PHYS_CASES = {
+    # OVRVO at 1500 K with the thermostat, linear PB on a synthetic dielectric, FP32 PB
+    # multigrid-PCG in FP64 refinement
+    "ovrvo_pb_maps": (216, 32, 3, 1.379, 2, dict(integrator="OVRVO", thermostat=True,
+                                                 gamma=GAMMA_1PS, pb=True, eps_seed=4), 11, False),
+    # cubic splines, PB maps rebuilt from the radii with the non-polar term
+    "spline_pb_radii": (216, 36, 32, 1.379, 2, dict(cas="SPL_CUBIC", eps_s=80.0, pb_radii=True,
+                                                    pb_w=1.0, nonpolar=True, gamma_np=0.05),
+                        None, False),
+    # the reference LCG method with the Jacobi preconditioner, quadratic splines, the
+    # velocity rescale after start-up
+    "lcg_jacobi_rescale": (216, 32, 5, 1.379, 0, dict(method="LCG", precond="JACOBI",
+                                                      cas="SPL_QUADR"), None, True),
+    # MAZE-LCG with the reference's multigrid preconditioner, Verlet thermostat
+    "maze_lcg_mg_thermostat": (216, 32, 6, 1.379, 0, dict(precond="MG", thermostat=True,
+                                                          T=1300.0), None, False),
+    # the reference's MaZe multigrid
+    "maze_multigrid": (216, 32, 7, 1.379, 0, dict(method="MAZE-MULTIGRID"), None, False),
+}
+
+
+def phys_system(name):
+    n_ions, n, seed, dens, prec, fields, rng, rescale = PHYS_CASES[name]
+    s = W.nacl_system(n_ions, n, seed=seed, density=dens)
+    return s, driver.RunConfig(tol=W.poisson_tol(s), **fields), prec, rng, rescale
+
+
+def phys_single(lib, name, steps):
+    s, cfg, prec, rng, rescale = phys_system(name)
+    driver.initialize(lib, s, cfg)
+    if rng is not None:
+        lib.solver_set_rng_seed(rng)
+    lib.lib.solver_set_precision(prec)
+    lib.lib.solver_set_mixed_init(1)
+    driver.bootstrap(lib, cfg)
+    if rescale:
+        lib.solver_rescale_velocities()
+    for _ in range(steps):
+        lib.solver_md_loop_iter()
+    snap = driver.snapshot(lib, s)
+    snap["potential_tf"] = lib.solver_compute_forces_noel()  # same positions, same value
+    lib.solver_finalize()
+    return snap
+
+
+@pytest.fixture(scope="module")
+def phys_runs(hip):
+    return {name: phys_single(hip, name, STEPS) for name in PHYS_CASES}
+
+
+def test_engine_physics_matches_single_replica_path(hip, phys_runs):
+    """Every physics option of the engine (integrator and thermostat, charge assignment,
+    method and preconditioner, PB on given maps or from the radii, the start-up rescale),
+    all replicas in one mix with a default one: trajectories, observables and the final
+    field equal the single-replica path's bit for bit."""
+    with ReplicaEngine(hip, max_slots=8) as e:
+        ids = {}
+        for name in PHYS_CASES:
+            s, cfg, prec, rng, rescale = phys_system(name)
+            ids[name] = e.submit(s, STEPS, precision=prec, cfg=cfg, seed=rng, rescale=rescale,
+                                 keep_field=True)
+        plain = e.submit(systems()[0], STEPS)
+        st = e.run()
+        assert st["max_running"] == len(PHYS_CASES) + 1
+        for name, rid in ids.items():
+            ref = phys_runs[name]
+            rs = e.request_stats(rid)
+            assert rs["steps"] == STEPS and rs["failures"] == 0, name
+            assert rs["arena_overflow"] == 0, name
+            assert same(e.result(rid), ref), name
+            ob = e.observables(rid)
+            for k in ("kinetic", "temperature", "energy_elec", "potential_tf"):
+                assert ob[k] == ref[k], (name, k, ob[k], ref[k])
+            assert np.array_equal(ob["momentum"], ref["momentum"]), name
+            assert np.array_equal(e.field(rid), ref["field"]), name
+        assert e.request_stats(plain)["failures"] == 0
+        with pytest.raises(RuntimeError):
+            e.field(plain)  # not kept
+
+
+def test_engine_physics_rejects_invalid(hip):
+    s = systems()[0]
+    with ReplicaEngine(hip, max_slots=1) as e:
+        rid = e.submit(s, 1)
+        with pytest.raises(ValueError):  # the multigrid methods need an even n >= 32
+            e.submit(W.nacl_system(64, 16, seed=1), 1, cfg=driver.RunConfig(method="MAZE-MULTIGRID"))
+        with pytest.raises(ValueError):  # SSOR is not provided
+            e.submit(s, 1, cfg=driver.RunConfig(precond="SSOR"))
+        st = e.run()  # the refused requests were withdrawn
+        assert st["replica_steps_per_s"] > 0 and e.request_stats(rid)["steps"] == 1
+        for other in (rid + 1, rid + 2):
+            assert e.request_stats(other)["steps"] == 0
+            with pytest.raises(RuntimeError):
+                e.result(other)
