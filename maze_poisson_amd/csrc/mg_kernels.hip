@@ -463,7 +463,10 @@ __global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict
                                                   const float* __restrict__ e,
                                                   float* __restrict__ dout, float c1, float c2,
                                                   Reduce red, CGState* st) {
-    if (st->done) return;
+    // The stop flag is tested once the operand loads are in flight, so its
+    // latency does not precede them; nothing writes it during this kernel, so
+    // every wave of a work-group takes the same branch.
+    const int done = st->done;
     constexpr int DI = TI + 4, DJ = TJ + 4, DK = TK + 4;
     constexpr int PI = TI + 2, PJ = TJ + 2, PK = TK + 2;
     constexpr int EI = TI / 2 + 3, EJ = TJ / 2 + 3;
@@ -492,6 +495,7 @@ __global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict
         const int gk = wrap1(T.k0 / 2 - 4 + VE * g, nc);
         copy_group<VE>(&se[(i * EJ + j) * RWE + VE * g], &e[(long)gi * nc2 + gj * nc + gk]);
     });
+    if (done) return;
     lds_barrier();
     // d = d2 + trilinear(e) on the D region: column (j, k), coarse row bb =
     // j >> 1, coarse column (k >> 1) + 3; the bilinear value of every coarse
@@ -575,7 +579,7 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
                                                     float* __restrict__ d2out,
                                                     float* __restrict__ bcoarse, float c1,
                                                     float c2, const CGState* st) {
-    if (st->done) return;
+    const int done = st->done;  // tested once the loads are in flight (as k_mg_up_vec)
     constexpr int BI = TI + 5, BJ = TJ + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
     constexpr int RI = TI + 1, RJ = TJ + 1, RK = TK + 1;
@@ -593,6 +597,7 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
         const int gk = wrap1(T.k0 - 4 + V * g, n);
         copy_group<V>(&sb[(i * BJ + j) * RW + V * g], &b[(long)gi * n2 + gj * n + gk]);
     });
+    if (done) return;
     lds_barrier();
     const float cs = c1 + c2, cp = c1 * c2;
     // d2 on the D region (LDS column k + 2), walking i.
@@ -1269,7 +1274,7 @@ template <int C>
 __global__ __launch_bounds__(kMgCoarseMfmaThreads) void k_mg_coarse_mfma(
     const float* __restrict__ b, float* __restrict__ e, const double* __restrict__ Qf,
     const double* __restrict__ ilam, const CGState* st) {
-    if (st->done) return;
+    const int done = st->done;  // tested once the loads are in flight (as k_mg_up_vec)
     constexpr int CM = kMgCoarseLds, NT = kMgCoarseMfmaThreads, PER = CM * CM * CM / NT;
     static_assert(CM == 16, "one 16-line tile per wave with 16-wide operands");
     static_assert(C >= 2 && C <= CM, "coarsest level of 2..16 points per side");
@@ -1291,6 +1296,7 @@ __global__ __launch_bounds__(kMgCoarseMfmaThreads) void k_mg_coarse_mfma(
         qi[kk] = in ? (float)Qf[m * c + col] : 0.0f;
     }
     for (int x = tid; x < N; x += NT) A[(x / c) * pitch + x % c] = b[x];
+    if (done) return;
     lds_barrier();
     // axis 0: lines along k at (i, j) = (u, v); 1: along j at (i, k); 2: along i at (j, k).
     auto line_base = [&](int L, int axis) {
