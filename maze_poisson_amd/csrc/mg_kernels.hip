@@ -1495,6 +1495,148 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse_axis_mfma(int c, const voi
     }
 }
 
+// ---- the same solve in three kernels for c <= kMgPlaneMax -------------------------
+// The six axis passes regrouped by plane: the forward transform of the two
+// in-plane axes of every plane i (Z_i = Qf X_i Qf^T), then per line along i the
+// forward transform, the scaling by the inverse eigenvalues and the inverse
+// transform (the scaling is point-wise, so the i axis goes forward and back in
+// one kernel), then the inverse transform of the two in-plane axes (X_i =
+// Qf^T Z_i Qf).  Every product runs on the FP64 matrix cores with its operands
+// in LDS (the same operand layout as k_mg_coarse_axis_mfma); arrays keep the
+// natural [i][j][k] order, so each kernel reads and writes contiguous rows.
+// One plane of up to 80 x 80 values, its intermediate and the eigenbasis take
+// 152 KB of LDS.
+constexpr int kMgPlaneMax = 80;
+constexpr int kMgPlaneThreads = 1024;
+constexpr int kMgPlanePitch = kMgPlaneMax + 1;
+
+// One 16 x 16 tile of the product A B (K-steps of 4 over kq values) on the
+// matrix cores; a(row, m) and b(m, col) give the operands, tile-relative.
+template <class FA, class FB>
+__device__ __forceinline__ mg_v4d mfma_tile_f64(int kq, FA&& a, FB&& b) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    mg_v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int m0 = 0; m0 < kq; m0 += 4)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a(lr, m0 + lk), b(m0 + lk, lr), acc, 0, 0, 0);
+    return acc;
+}
+
+// Both in-plane axes of plane blockIdx.x: forward Z = Qf X Qf^T, inverse
+// Z = Qf^T X Qf.  In: the FP32 right-hand side (forward) or doubles; Out:
+// doubles, or the FP32 correction (inverse).
+template <bool Fwd, class In, class Out>
+__global__ __launch_bounds__(kMgPlaneThreads) void k_mg_coarse_plane(int c,
+                                                                    const In* __restrict__ in,
+                                                                    Out* __restrict__ out,
+                                                                    const double* __restrict__ Qf,
+                                                                    const CGState* st) {
+    const int done = st->done;  // tested once the loads are in flight (as k_mg_up_vec)
+    constexpr int P = kMgPlanePitch;
+    __shared__ double X[kMgPlaneMax * P], Y[kMgPlaneMax * P], Q[kMgPlaneMax * P];
+    const int CP = (c + 15) & ~15, K4 = (c + 3) & ~3;
+    const long plane = (long)blockIdx.x * c * c;
+    for (int x = threadIdx.x; x < CP * CP; x += kMgPlaneThreads) {
+        const int r = x / CP, q = x % CP;
+        const bool in_c = r < c && q < c;
+        Q[r * P + q] = in_c ? Qf[r * c + q] : 0.0;
+        X[r * P + q] = in_c ? (double)in[plane + (long)r * c + q] : 0.0;
+    }
+    if (done) return;
+    lds_barrier();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int nt = CP / 16, rows1 = (K4 + 15) / 16;  // Y rows past K4 are not needed
+    // Y = X B1 along the columns: B1[m][t] = Qf[t][m] (forward) or Qf[m][t].
+    for (int tt = wave; tt < rows1 * nt; tt += kMgPlaneThreads / 64) {
+        const int r0 = (tt / nt) * 16, t0 = (tt % nt) * 16;
+        const mg_v4d acc = mfma_tile_f64(
+            K4, [&](int r, int m) { return X[(r0 + r) * P + m]; },
+            [&](int m, int t) { return Fwd ? Q[(t0 + t) * P + m] : Q[m * P + t0 + t]; });
+#pragma unroll
+        for (int r = 0; r < 4; r++) Y[(r0 + lq + 4 * r) * P + t0 + lr] = acc[r];
+    }
+    lds_barrier();
+    // Z = A2 Y along the rows: A2[s][j] = Qf[s][j] (forward) or Qf[j][s].
+    for (int tt = wave; tt < nt * nt; tt += kMgPlaneThreads / 64) {
+        const int s0 = (tt / nt) * 16, t0 = (tt % nt) * 16;
+        const mg_v4d acc = mfma_tile_f64(
+            K4, [&](int s, int j) { return Fwd ? Q[(s0 + s) * P + j] : Q[j * P + s0 + s]; },
+            [&](int j, int t) { return Y[j * P + t0 + t]; });
+        const int t = t0 + lr;
+        if (t < c) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int sr = s0 + lq + 4 * r;
+                if (sr < c) out[plane + (long)sr * c + t] = (Out)acc[r];
+            }
+        }
+    }
+}
+
+// The i axis of 16 lines (j = blockIdx.x / ng, k = k0 .. k0 + 15): forward,
+// scaled by the inverse eigenvalues, inverse; in place on w.
+constexpr int kMgLineThreads = 256;
+__global__ __launch_bounds__(kMgLineThreads) void k_mg_coarse_line(int c, double* __restrict__ w,
+                                                                   const double* __restrict__ Qf,
+                                                                   const double* __restrict__ ilam,
+                                                                   const CGState* st) {
+    const int done = st->done;  // tested once the loads are in flight (as k_mg_up_vec)
+    constexpr int P = kMgPlanePitch;
+    __shared__ double L[16 * P], O[16 * P], Q[kMgPlaneMax * P];
+    const int CP = (c + 15) & ~15, K4 = (c + 3) & ~3, c2 = c * c;
+    const int ng = (c + 15) / 16;
+    const int j = (int)blockIdx.x / ng, k0 = ((int)blockIdx.x % ng) * 16;
+    for (int x = threadIdx.x; x < CP * CP; x += kMgLineThreads) {
+        const int r = x / CP, q = x % CP;
+        Q[r * P + q] = r < c && q < c ? Qf[r * c + q] : 0.0;
+    }
+    for (int x = threadIdx.x; x < 16 * CP; x += kMgLineThreads) {
+        const int l = x & 15, i = x >> 4;  // consecutive threads: consecutive k
+        L[l * P + i] = (i < c && k0 + l < c) ? w[(long)i * c2 + j * c + k0 + l] : 0.0;
+    }
+    if (done) return;
+    lds_barrier();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int nt = CP / 16;
+    // O[l][t] = ilam[t][j][k0 + l] sum_i L[l][i] Qf[t][i]
+    for (int tt = wave; tt < nt; tt += kMgLineThreads / 64) {
+        const int t0 = tt * 16;
+        const mg_v4d acc = mfma_tile_f64(
+            K4, [&](int l, int m) { return L[l * P + m]; },
+            [&](int m, int t) { return Q[(t0 + t) * P + m]; });
+        const int t = t0 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int l = lq + 4 * r;
+            O[l * P + t] = (t < c && k0 + l < c) ? acc[r] * ilam[(long)t * c2 + j * c + k0 + l] : 0.0;
+        }
+    }
+    lds_barrier();
+    // L[l][i] = sum_t O[l][t] Qf[t][i]
+    for (int tt = wave; tt < nt; tt += kMgLineThreads / 64) {
+        const int i0 = tt * 16;
+        const mg_v4d acc = mfma_tile_f64(
+            K4, [&](int l, int m) { return O[l * P + m]; },
+            [&](int m, int i) { return Q[m * P + i0 + i]; });
+#pragma unroll
+        for (int r = 0; r < 4; r++) L[(lq + 4 * r) * P + i0 + lr] = acc[r];
+    }
+    lds_barrier();
+    for (int x = threadIdx.x; x < 16 * c; x += kMgLineThreads) {
+        const int l = x & 15, i = x >> 4;
+        if (k0 + l < c) w[(long)i * c2 + j * c + k0 + l] = L[l * P + i];
+    }
+}
+
+static void coarse_plane_solve(hipStream_t s, int c, const float* b, float* e, double* w0,
+                               const double* Qf, const double* ilam, CGState* st) {
+    MAZE_LAUNCH((k_mg_coarse_plane<true, float, double>), dim3(c), dim3(kMgPlaneThreads), 0, s, c,
+                b, w0, Qf, st);
+    MAZE_LAUNCH(k_mg_coarse_line, dim3(c * ((c + 15) / 16)), dim3(kMgLineThreads), 0, s, c, w0,
+                Qf, ilam, st);
+    MAZE_LAUNCH((k_mg_coarse_plane<false, double, float>), dim3(c), dim3(kMgPlaneThreads), 0, s,
+                c, w0, e, Qf, st);
+}
+
 template <int R>
 static void coarse_axis_passes(hipStream_t s, int c, const float* b, float* e, double* w0,
                                double* w1, const double* Qf, const double* ilam, CGState* st) {
@@ -1529,12 +1671,15 @@ void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double
     // vector-ALU passes with that many rows per work-group (4 was their best:
     // passes on coarsest grids of 65^3, 51^3 and 41^3 points measured 7.3 us on
     // average against 9.6 us with 16 rows).
+    // Up to kMgPlaneMax the three plane / line kernels (MAZE_COARSE_ROWS=1
+    // selects the six matrix-core axis passes there too).
     static const int rows = [] {
         const char* v = std::getenv("MAZE_COARSE_ROWS");
         const int r = v ? std::atoi(v) : 0;
-        return r == 4 || r == 8 || r == 16 ? r : 0;
+        return r == 1 || r == 4 || r == 8 || r == 16 ? r : 0;
     }();
-    if (rows == 0) coarse_axis_passes_mfma(s, c, b, e, w0, w1, Qf, ilam, st);
+    if (rows == 0 && c <= kMgPlaneMax) coarse_plane_solve(s, c, b, e, w0, Qf, ilam, st);
+    else if (rows <= 1) coarse_axis_passes_mfma(s, c, b, e, w0, w1, Qf, ilam, st);
     else if (rows == 4) coarse_axis_passes<4>(s, c, b, e, w0, w1, Qf, ilam, st);
     else if (rows == 8) coarse_axis_passes<8>(s, c, b, e, w0, w1, Qf, ilam, st);
     else coarse_axis_passes<16>(s, c, b, e, w0, w1, Qf, ilam, st);

@@ -515,10 +515,11 @@ def test_chebyshev_smoother_damps_upper_spectrum():
     assert np.abs(p[mu >= 1 / 3]).max() <= 1 / t2 + 1e-9
 
 
-@pytest.mark.parametrize("c", [8, 13, 16, 25, 33])
+@pytest.mark.parametrize("c", [8, 13, 16, 25, 33, 41, 51, 65, 79, 81, 99])
 def test_mg_coarse_exact(hip_raw, c):
-    """The exact solve of the coarsest level (one work-group in FP32 up to 16^3, FP64
-    axis passes above) against the FFT solve of the same periodic system."""
+    """The exact solve of the coarsest level (one work-group in FP32 up to 16^3; FP64
+    plane and line kernels up to 80 points, six FP64 axis passes above) against the FFT
+    solve of the same periodic system."""
     b = np.random.default_rng(c).standard_normal((c, c, c))
     b = (b - b.mean()).astype(np.float32)
     e = np.zeros_like(b)
