@@ -175,6 +175,19 @@ void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const doub
                       const double* ilam, CGState* st);
 void launch_mg_coarse(hipStream_t s, int c, const double* b, double* e, const double* Qf,
                       const double* ilam, CGState* st);
+// The down kernel of the level above a 16^3 coarsest grid with the coarse
+// solve in its last work-group (count: a zeroed device counter the kernel
+// leaves zeroed; e: the coarse correction).  Returns false, launching nothing,
+// where that form is not built (the caller then launches the two kernels).
+struct CoarseTail {
+    unsigned int* count;
+    const double* Qf;
+    const double* ilam;
+    float* e;
+};
+bool mg_down_coarse_ok(int n, int tile, int c);
+bool launch_mg_down_coarse(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
+                           double c1, double c2, CGState* st, int c, const CoarseTail& tail);
 void launch_mg_coarse_big(hipStream_t s, int c, const float* b, float* e, double* w0, double* w1,
                           const double* Qf, const double* ilam, CGState* st);
 // The multigrid-PCG direction and update with deferred finalisation (Poisson

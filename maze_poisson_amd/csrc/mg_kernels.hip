@@ -574,11 +574,14 @@ __global__ __launch_bounds__(NT) void k_mg_up_vec(int n, const float* __restrict
     }
 }
 
-template <int TI, int TJ, int TK, int NT, int V>
-__global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restrict__ b,
-                                                    float* __restrict__ d2out,
-                                                    float* __restrict__ bcoarse, float c1,
-                                                    float c2, const CGState* st) {
+// The down kernel's work; false if the solve has stopped (nothing written).
+// Shared: the coarse right-hand side is stored at device scope, for a last
+// work-group that reads it in the same kernel (k_mg_down_coarse).
+template <int TI, int TJ, int TK, int NT, int V, bool Shared>
+__device__ __forceinline__ bool mg_down_vec_body(int n, const float* __restrict__ b,
+                                                 float* __restrict__ d2out,
+                                                 float* __restrict__ bcoarse, float c1, float c2,
+                                                 const CGState* st) {
     const int done = st->done;  // tested once the loads are in flight (as k_mg_up_vec)
     constexpr int BI = TI + 5, BJ = TJ + 5;
     constexpr int DI = TI + 3, DJ = TJ + 3, DK = TK + 3;
@@ -597,7 +600,7 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
         const int gk = wrap1(T.k0 - 4 + V * g, n);
         copy_group<V>(&sb[(i * BJ + j) * RW + V * g], &b[(long)gi * n2 + gj * n + gk]);
     });
-    if (done) return;
+    if (done) return false;
     lds_barrier();
     const float cs = c1 + c2, cp = c1 * c2;
     // d2 on the D region (LDS column k + 2), walking i.
@@ -666,8 +669,22 @@ __global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restri
                     const float w = (float)((a == 0 ? 2 : 1) * (bb == 0 ? 2 : 1) * (cc == 0 ? 2 : 1));
                     acc += w * sr[o + (a * RJ + bb) * RW + cc];
                 }
-        bcoarse[(gI * nc + gJ) * nc + gK] = acc * (4.0f / 64.0f);
+        const float v = acc * (4.0f / 64.0f);
+        if constexpr (Shared)
+            __hip_atomic_store(bcoarse + (gI * nc + gJ) * nc + gK, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else
+            bcoarse[(gI * nc + gJ) * nc + gK] = v;
     });
+    return true;
+}
+
+template <int TI, int TJ, int TK, int NT, int V>
+__global__ __launch_bounds__(NT) void k_mg_down_vec(int n, const float* __restrict__ b,
+                                                    float* __restrict__ d2out,
+                                                    float* __restrict__ bcoarse, float c1,
+                                                    float c2, const CGState* st) {
+    mg_down_vec_body<TI, TJ, TK, NT, V, false>(n, b, d2out, bcoarse, c1, c2, st);
 }
 
 // Load widths of the column-mapped kernels: 0 where they do not apply (n odd
@@ -1268,75 +1285,163 @@ typedef float mg_v4f __attribute__((ext_vector_type(4)));
 typedef double mg_v4d __attribute__((ext_vector_type(4)));
 constexpr int kMgCoarseMfmaThreads = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds / 4;
 
-// C (the level's side) is a compile-time constant, so every index division by
-// it is a multiply-shift (a run-time divisor cost more than the transforms).
+// The solve on NT threads (any multiple of 64 up to 1024): wave w takes the
+// 16-line tiles w, w + NT / 64, ... of every pass.  load() issues the loads of
+// the eigenbasis operands and the inverse eigenvalues into registers; solve()
+// runs the six passes and the scaling on the right-hand side in A (rows of
+// c + 1 floats) and writes the correction e.  C (the level's side) is a
+// compile-time constant, so every index division by it is a multiply-shift (a
+// run-time divisor cost more than the transforms).
+template <int C, int NT>
+struct CoarseMfma {
+    static constexpr int CM = kMgCoarseLds, PER = (CM * CM * CM + NT - 1) / NT;
+    static_assert(CM == 16, "16-line tiles with 16-wide operands");
+    static_assert(C >= 2 && C <= CM, "coarsest level of 2..16 points per side");
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
+    static constexpr int c = C, c2 = c * c, N = c2 * c, pitch = c + 1;
+    static constexpr int kLds = CM * CM * (CM + 1);  // floats of A and of B
+    float il[PER];
+    // B operands: forward M[m][t] = Q[t][m], inverse M[m][t] = Q[m][t], with
+    // m = 4 kk + lane / 16 and t = lane % 16.
+    float qf[4], qi[4];
+
+    __device__ __forceinline__ void load(const double* Qf, const double* ilam) {
+        const int tid = threadIdx.x, lane = tid & 63;
+#pragma unroll
+        for (int k = 0; k < PER; k++) il[k] = tid + k * NT < N ? (float)ilam[tid + k * NT] : 0.0f;
+        const int col = lane & 15, krow = lane >> 4;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+            const int m = 4 * kk + krow;
+            const bool in = m < c && col < c;
+            qf[kk] = in ? (float)Qf[col * c + m] : 0.0f;
+            qi[kk] = in ? (float)Qf[m * c + col] : 0.0f;
+        }
+    }
+
+    static __device__ __forceinline__ int rhs_index(int x) { return (x / c) * pitch + x % c; }
+
+    // The right-hand side is in A; the barrier that makes it visible comes first.
+    __device__ __forceinline__ void solve(float* A, float* B, float* __restrict__ e) {
+        const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+        const int col = lane & 15, krow = lane >> 4;
+        lds_barrier();
+        // axis 0: lines along k at (i, j) = (u, v); 1: along j at (i, k); 2: along i at (j, k).
+        auto line_base = [&](int L, int axis) {
+            const int u = L / c, v = L - (L / c) * c;
+            return axis == 0 ? (u * c + v) * pitch : (axis == 1 ? u * c * pitch + v : u * pitch + v);
+        };
+        auto pass = [&](const float* src, float* dst, int axis, bool inv) {
+            const int stride = axis == 0 ? 1 : (axis == 1 ? pitch : c * pitch);
+            for (int L0 = 16 * wave; L0 < c2; L0 += 16 * (NT / 64)) {
+                const int La = L0 + col;  // A row of this lane
+                const int ba = La < c2 ? line_base(La, axis) : 0;
+                mg_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int m = 4 * kk + krow;
+                    const float a = (La < c2 && m < c) ? src[ba + m * stride] : 0.0f;
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, inv ? qi[kk] : qf[kk], acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int Ld = L0 + 4 * krow + r;  // D row of register r
+                    if (Ld < c2 && col < c) dst[line_base(Ld, axis) + col * stride] = acc[r];
+                }
+            }
+            lds_barrier();
+        };
+        pass(A, B, 0, false);
+        pass(B, A, 1, false);
+        pass(A, B, 2, false);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int x = tid + k * NT;
+            if (x < N) B[rhs_index(x)] *= il[k];
+        }
+        lds_barrier();
+        pass(B, A, 2, true);
+        pass(A, B, 1, true);
+        pass(B, A, 0, true);
+        for (int x = tid; x < N; x += NT) e[x] = A[rhs_index(x)];
+    }
+};
+
 template <int C>
 __global__ __launch_bounds__(kMgCoarseMfmaThreads) void k_mg_coarse_mfma(
     const float* __restrict__ b, float* __restrict__ e, const double* __restrict__ Qf,
     const double* __restrict__ ilam, const CGState* st) {
+    using CS = CoarseMfma<C, kMgCoarseMfmaThreads>;
     const int done = st->done;  // tested once the loads are in flight (as k_mg_up_vec)
-    constexpr int CM = kMgCoarseLds, NT = kMgCoarseMfmaThreads, PER = CM * CM * CM / NT;
-    static_assert(CM == 16, "one 16-line tile per wave with 16-wide operands");
-    static_assert(C >= 2 && C <= CM, "coarsest level of 2..16 points per side");
-    __shared__ float A[CM * CM * (CM + 1)], B[CM * CM * (CM + 1)];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int c = C, c2 = c * c, N = c2 * c, pitch = c + 1;
-    float il[PER];
-#pragma unroll
-    for (int k = 0; k < PER; k++) il[k] = tid + k * NT < N ? (float)ilam[tid + k * NT] : 0.0f;
-    // B operands: forward M[m][t] = Q[t][m], inverse M[m][t] = Q[m][t], with
-    // m = 4 kk + lane / 16 and t = lane % 16.
-    float qf[4], qi[4];
-    const int col = lane & 15, krow = lane >> 4;
-#pragma unroll
-    for (int kk = 0; kk < 4; kk++) {
-        const int m = 4 * kk + krow;
-        const bool in = m < c && col < c;
-        qf[kk] = in ? (float)Qf[col * c + m] : 0.0f;
-        qi[kk] = in ? (float)Qf[m * c + col] : 0.0f;
-    }
-    for (int x = tid; x < N; x += NT) A[(x / c) * pitch + x % c] = b[x];
+    __shared__ float A[CS::kLds], B[CS::kLds];
+    CS cs;
+    cs.load(Qf, ilam);
+    for (int x = threadIdx.x; x < CS::N; x += kMgCoarseMfmaThreads) A[CS::rhs_index(x)] = b[x];
     if (done) return;
-    lds_barrier();
-    // axis 0: lines along k at (i, j) = (u, v); 1: along j at (i, k); 2: along i at (j, k).
-    auto line_base = [&](int L, int axis) {
-        const int u = L / c, v = L - (L / c) * c;
-        return axis == 0 ? (u * c + v) * pitch : (axis == 1 ? u * c * pitch + v : u * pitch + v);
-    };
-    auto pass = [&](const float* src, float* dst, int axis, bool inv) {
-        const int stride = axis == 0 ? 1 : (axis == 1 ? pitch : c * pitch);
-        const int L0 = 16 * wave;
-        if (L0 < c2) {
-            const int La = L0 + col;  // A row of this lane
-            const int ba = La < c2 ? line_base(La, axis) : 0;
-            mg_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kk = 0; kk < 4; kk++) {
-                const int m = 4 * kk + krow;
-                const float a = (La < c2 && m < c) ? src[ba + m * stride] : 0.0f;
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, inv ? qi[kk] : qf[kk], acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int Ld = L0 + 4 * krow + r;  // D row of register r
-                if (Ld < c2 && col < c) dst[line_base(Ld, axis) + col * stride] = acc[r];
-            }
-        }
-        lds_barrier();
-    };
-    pass(A, B, 0, false);
-    pass(B, A, 1, false);
-    pass(A, B, 2, false);
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int x = tid + k * NT;
-        if (x < N) B[(x / c) * pitch + x % c] *= il[k];
+    cs.solve(A, B, e);
+}
+
+// The level above a coarsest grid of C points, with the coarse solve in the
+// same kernel: every work-group stores its part of the coarse right-hand side
+// at device scope (the XCDs' L2 caches are not coherent with each other),
+// waits until those stores have completed and counts its arrival; the last to
+// arrive reads the whole right-hand side back at device scope, solves the
+// coarse system (CoarseMfma on the kernel's threads) and writes its
+// correction, then clears the counter for the next cycle.  The same hand-off
+// as the single-pass reductions; it saves the coarse kernel's launch and its
+// wait for a free CU.
+template <int TI, int TJ, int TK, int NT, int V, int C>
+__global__ __launch_bounds__(NT) void k_mg_down_coarse(int n, const float* __restrict__ b,
+                                                       float* __restrict__ d2out,
+                                                       float* __restrict__ bcoarse, float c1,
+                                                       float c2, const CGState* st,
+                                                       CoarseTail tail) {
+    if (!mg_down_vec_body<TI, TJ, TK, NT, V, true>(n, b, d2out, bcoarse, c1, c2, st)) return;
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0)
+        last = __hip_atomic_fetch_add(tail.count, 1u, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    using CS = CoarseMfma<C, NT>;
+    __shared__ float A[CS::kLds], B[CS::kLds];
+    CS cs;
+    cs.load(tail.Qf, tail.ilam);
+    for (int x = threadIdx.x; x < CS::N; x += NT)
+        A[CS::rhs_index(x)] =
+            __hip_atomic_load(bcoarse + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) reset_counter(tail.count);
+    cs.solve(A, B, tail.e);
+}
+
+template <int TI, int TJ, int TK, int NT>
+static void down_coarse_t(hipStream_t s, int n, const float* b, float* d2, float* bc, double c1,
+                          double c2, CGState* st, const CoarseTail& tail) {
+    MAZE_LAUNCH((k_mg_down_coarse<TI, TJ, TK, NT, 4, kMgCoarseLds>), dim3(mg_tiles<TI, TJ, TK>(n)),
+                dim3(NT), 0, s, n, b, d2, bc, (float)c1, (float)c2, st, tail);
+}
+
+// Built for the 32^3 level above the 16^3 coarsest (16-byte loads) in the five
+// tile shapes; the plane-marching and scalar-load shapes keep the separate
+// coarse kernel.
+bool mg_down_coarse_ok(int n, int tile, int c) {
+    return c == kMgCoarseLds && n == 2 * c && mg_vec_widths(n, false) == 8 * 4 + 4 && tile >= 0 &&
+           tile < kMgMarch16;
+}
+
+bool launch_mg_down_coarse(hipStream_t s, int n, int tile, const float* b, float* d2, float* bc,
+                           double c1, double c2, CGState* st, int c, const CoarseTail& tail) {
+    if (!mg_down_coarse_ok(n, tile, c)) return false;
+    switch (tile) {
+        case kMgT8x16x16_1024: down_coarse_t<8, 16, 16, 1024>(s, n, b, d2, bc, c1, c2, st, tail); return true;
+        case kMgT8x16x16_512: down_coarse_t<8, 16, 16, 512>(s, n, b, d2, bc, c1, c2, st, tail); return true;
+        case kMgT8x8x16_512: down_coarse_t<8, 8, 16, 512>(s, n, b, d2, bc, c1, c2, st, tail); return true;
+        case kMgT4x8x8_512: down_coarse_t<4, 8, 8, 512>(s, n, b, d2, bc, c1, c2, st, tail); return true;
+        case kMgT4x8x8_256: down_coarse_t<4, 8, 8, 256>(s, n, b, d2, bc, c1, c2, st, tail); return true;
+        default: return false;  // plane-marching and scalar-load shapes
     }
-    lds_barrier();
-    pass(B, A, 2, true);
-    pass(A, B, 1, true);
-    pass(B, A, 0, true);
-    for (int x = tid; x < N; x += NT) e[x] = A[(x / c) * pitch + x % c];
 }
 
 void launch_mg_coarse(hipStream_t s, int c, const float* b, float* e, const double* Qf,

@@ -124,6 +124,7 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     if (const char* e = std::getenv("MAZE_INNER_MAX")) inner_max = std::atoi(e);
     if (const char* e = std::getenv("MAZE_INNER_RTOL")) inner_rtol = std::atof(e);  // tuning
     if (const char* e = std::getenv("MAZE_PB_TAIL")) pb_tail = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MAZE_MG_FUSE_COARSE")) mg_fuse_coarse = std::atoi(e) != 0;
     st = dalloc<CGState>(1);
     st_c = dalloc<CGState>(1);
     MAZE_HIP(hipHostMalloc(&st_host, sizeof(CGState)));
@@ -709,6 +710,7 @@ void Sim::free_mg() {
     dfree(mg_ilam);
     dfree(mg_w0);
     dfree(mg_w1);
+    dfree(mg_coarse_count);
     mg_ok = -1;
 }
 
@@ -791,6 +793,7 @@ bool Sim::setup_mg() {
             }
     mg_Qf = dalloc<double>(c * c);
     mg_ilam = dalloc<double>(c * c * c);
+    mg_coarse_count = dalloc<unsigned int>(1);  // zeroed
     if (c > kMgCoarseLds) {
         mg_w0 = dalloc<double>(c * c * c);
         mg_w1 = dalloc<double>(c * c * c);
@@ -967,9 +970,19 @@ void Sim::vcycle(int l, int iter) {
         if (top) timed(timer, iter, f);
         else f();
     };
-    // The finest up kernel writes the <r, z> partials the direction pass folds.
-    t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st); });
-    vcycle(l + 1, iter);
+    // Above a 16^3 coarsest level the down kernel's last work-group solves it.
+    const bool fused = mg_fuse_coarse && l + 2 == (int)mg.size() && l + 1 < mg_tail &&
+                       mg_down_coarse_ok(L.n, L.tile, mg_c);
+    if (fused)
+        t(kTMgDown, [&] {
+            launch_mg_down_coarse(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st, mg_c,
+                                  CoarseTail{mg_coarse_count, mg_Qf, mg_ilam, mg[l + 1].d});
+        });
+    else {
+        // The finest up kernel writes the <r, z> partials the direction pass folds.
+        t(kTMgDown, [&] { launch_mg_down(stream, L.n, L.tile, L.b, L.t, mg[l + 1].b, c1, c2, st); });
+        vcycle(l + 1, iter);
+    }
     t(kTMgUp, [&] {
         launch_mg_up(stream, L.n, L.tile_up, L.b, L.t, mg[l + 1].d, L.d, c1, c2, st, top,
                      Reduce{dred.pz, nullptr, nullptr});

@@ -114,6 +114,11 @@ struct Sim {
     double* mg_ilam = nullptr;
     double *mg_w0 = nullptr, *mg_w1 = nullptr;  // work arrays of a coarsest level above 12^3
     int mg_c = 0;
+    // Arrival counter of the down kernel that also solves a 16^3 coarsest level
+    // (launch_mg_down_coarse, opt-in with MAZE_MG_FUSE_COARSE=1: bit-identical to
+    // the two kernels, measured 0.9 % slower on the 4096-ion / 128^3 step).
+    unsigned int* mg_coarse_count = nullptr;
+    int mg_fuse_coarse = 0;
     // Weights of the two Jacobi smoothing sweeps (pre: omega then omega2;
     // post: omega2 then omega).  Default: the degree-2 Chebyshev polynomial
     // for the Jacobi-scaled spectrum [1/3, 2] (the weights are the reciprocal

@@ -415,6 +415,27 @@ def test_mg_pcg_solves(hip_raw, oracle_raw, n):
     assert np.abs(d).max() <= 1e-4 * np.abs(xo).max()
 
 
+@pytest.mark.parametrize("n,tile", [(32, None), (64, None), (128, None)] +
+                         [(64, t) for t in range(5)])
+def test_mg_fused_coarse_matches_separate(hip_raw, n, tile, monkeypatch):
+    """The down kernel above the 16^3 coarsest level with the coarse solve in its last
+    work-group (MAZE_MG_FUSE_COARSE=1) gives the multigrid-PCG solve bit for bit as the
+    down kernel and the coarse kernel launched one after the other (the default), in
+    every tile shape of the 32^3 level."""
+    if tile is not None:
+        monkeypatch.setenv("MAZE_MG_TILES", f"3,{tile}")
+    b = zero_mean(rand_grid(n, 77))
+    out = []
+    for fuse in ("1", "0", "1"):  # every V-cycle of a solve reuses the arrival counter
+        monkeypatch.setenv("MAZE_MG_FUSE_COARSE", fuse)
+        x = np.zeros_like(b)
+        it = hip_raw.maze_test_mg_pcg(b, x, 1e-5, n)
+        out.append((it, x))
+    assert out[0][0] == out[1][0] == out[2][0] > 0, [o[0] for o in out]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[2][1], out[1][1])
+
+
 def _lap(u):
     return sum(np.roll(u, s, axis=a) for a in range(3) for s in (1, -1)) - 6 * u
 
