@@ -1294,22 +1294,30 @@ constexpr int kMgCoarseMfmaThreads = kMgCoarseLds * kMgCoarseLds * kMgCoarseLds 
 // run-time divisor cost more than the transforms).
 template <int C, int NT>
 struct CoarseMfma {
-    static constexpr int CM = kMgCoarseLds, PER = (CM * CM * CM + NT - 1) / NT;
+    static constexpr int CM = kMgCoarseLds;
     static_assert(CM == 16, "16-line tiles with 16-wide operands");
     static_assert(C >= 2 && C <= CM, "coarsest level of 2..16 points per side");
     static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
     static constexpr int c = C, c2 = c * c, N = c2 * c, pitch = c + 1;
+    static constexpr int NW = NT / 64, TPW = ((c2 + 15) / 16 + NW - 1) / NW;  // tiles per wave
     static constexpr int kLds = CM * CM * (CM + 1);  // floats of A and of B
-    float il[PER];
+    // Inverse eigenvalues of this lane's results in the pass along i: tile t,
+    // register r is line L0 + 4 (lane / 16) + r (the point (j, k)) at i = lane % 16.
+    float il[TPW][4];
     // B operands: forward M[m][t] = Q[t][m], inverse M[m][t] = Q[m][t], with
     // m = 4 kk + lane / 16 and t = lane % 16.
     float qf[4], qi[4];
 
     __device__ __forceinline__ void load(const double* Qf, const double* ilam) {
-        const int tid = threadIdx.x, lane = tid & 63;
-#pragma unroll
-        for (int k = 0; k < PER; k++) il[k] = tid + k * NT < N ? (float)ilam[tid + k * NT] : 0.0f;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const int col = lane & 15, krow = lane >> 4;
+#pragma unroll
+        for (int t = 0; t < TPW; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int Ld = 16 * (wave + t * NW) + 4 * krow + r;
+                il[t][r] = Ld < c2 && col < c ? (float)ilam[(col * c + Ld / c) * c + Ld % c] : 0.0f;
+            }
 #pragma unroll
         for (int kk = 0; kk < 4; kk++) {
             const int m = 4 * kk + krow;
@@ -1331,36 +1339,49 @@ struct CoarseMfma {
             const int u = L / c, v = L - (L / c) * c;
             return axis == 0 ? (u * c + v) * pitch : (axis == 1 ? u * c * pitch + v : u * pitch + v);
         };
-        auto pass = [&](const float* src, float* dst, int axis, bool inv) {
+        // One 16-line tile of a pass: the transform of lines L0 .. L0 + 15 from src.
+        auto tile = [&](const float* src, int L0, int axis, bool inv) {
             const int stride = axis == 0 ? 1 : (axis == 1 ? pitch : c * pitch);
-            for (int L0 = 16 * wave; L0 < c2; L0 += 16 * (NT / 64)) {
-                const int La = L0 + col;  // A row of this lane
-                const int ba = La < c2 ? line_base(La, axis) : 0;
-                mg_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int La = L0 + col;  // A row of this lane
+            const int ba = La < c2 ? line_base(La, axis) : 0;
+            mg_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    const int m = 4 * kk + krow;
-                    const float a = (La < c2 && m < c) ? src[ba + m * stride] : 0.0f;
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, inv ? qi[kk] : qf[kk], acc, 0, 0, 0);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int Ld = L0 + 4 * krow + r;  // D row of register r
-                    if (Ld < c2 && col < c) dst[line_base(Ld, axis) + col * stride] = acc[r];
-                }
+            for (int kk = 0; kk < 4; kk++) {
+                const int m = 4 * kk + krow;
+                const float a = (La < c2 && m < c) ? src[ba + m * stride] : 0.0f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, inv ? qi[kk] : qf[kk], acc, 0, 0, 0);
             }
+            return acc;
+        };
+        auto put = [&](float* dst, int L0, int axis, const mg_v4f& acc) {
+            const int stride = axis == 0 ? 1 : (axis == 1 ? pitch : c * pitch);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int Ld = L0 + 4 * krow + r;  // D row of register r
+                if (Ld < c2 && col < c) dst[line_base(Ld, axis) + col * stride] = acc[r];
+            }
+        };
+        auto pass = [&](const float* src, float* dst, int axis, bool inv) {
+            for (int L0 = 16 * wave; L0 < c2; L0 += 16 * NW) put(dst, L0, axis, tile(src, L0, axis, inv));
             lds_barrier();
         };
         pass(A, B, 0, false);
         pass(B, A, 1, false);
-        pass(A, B, 2, false);
+        // Along i: forward, scaled by the inverse eigenvalues, and back.  A wave
+        // reads and writes only the positions of its own lines here, and its
+        // LDS operations complete in order, so no barrier is needed inside.
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int x = tid + k * NT;
-            if (x < N) B[rhs_index(x)] *= il[k];
+        for (int t = 0; t < TPW; t++) {
+            const int L0 = 16 * (wave + t * NW);
+            if (L0 < c2) {
+                mg_v4f acc = tile(A, L0, 2, false);
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[r] *= il[t][r];
+                put(B, L0, 2, acc);
+                put(A, L0, 2, tile(B, L0, 2, true));
+            }
         }
         lds_barrier();
-        pass(B, A, 2, true);
         pass(A, B, 1, true);
         pass(B, A, 0, true);
         for (int x = tid; x < N; x += NT) e[x] = A[rhs_index(x)];
