@@ -1689,11 +1689,30 @@ __global__ __launch_bounds__(kMgPlaneThreads) void k_mg_coarse_plane(int c,
     __shared__ double X[kMgPlaneMax * P], Y[kMgPlaneMax * P], Q[kMgPlaneMax * P];
     const int CP = (c + 15) & ~15, K4 = (c + 3) & ~3;
     const long plane = (long)blockIdx.x * c * c;
-    for (int x = threadIdx.x; x < CP * CP; x += kMgPlaneThreads) {
+    // All of a thread's loads are issued before the first store (every load
+    // goes to a valid address; values past c are zeroed afterwards), so the
+    // staging takes one memory round trip.
+    constexpr int U = (kMgPlaneMax * kMgPlaneMax + kMgPlaneThreads - 1) / kMgPlaneThreads;
+    double vq[U], vx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int x = min((int)threadIdx.x + u * kMgPlaneThreads, CP * CP - 1);
         const int r = x / CP, q = x % CP;
         const bool in_c = r < c && q < c;
-        Q[r * P + q] = in_c ? Qf[r * c + q] : 0.0;
-        X[r * P + q] = in_c ? (double)in[plane + (long)r * c + q] : 0.0;
+        const double a = Qf[in_c ? r * c + q : 0];
+        const double v = (double)in[plane + (in_c ? (long)r * c + q : 0)];
+        vq[u] = in_c ? a : 0.0;
+        vx[u] = in_c ? v : 0.0;
+    }
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int x = (int)threadIdx.x + u * kMgPlaneThreads;
+        if (x < CP * CP) {
+            const int r = x / CP, q = x % CP;
+            Q[r * P + q] = vq[u];
+            X[r * P + q] = vx[u];
+        }
     }
     if (done) return;
     lds_barrier();
@@ -1739,30 +1758,68 @@ __global__ __launch_bounds__(kMgLineThreads) void k_mg_coarse_line(int c, double
     const int CP = (c + 15) & ~15, K4 = (c + 3) & ~3, c2 = c * c;
     const int ng = (c + 15) / 16;
     const int j = (int)blockIdx.x / ng, k0 = ((int)blockIdx.x % ng) * 16;
-    for (int x = threadIdx.x; x < CP * CP; x += kMgLineThreads) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int nt = CP / 16;
+    // All of a thread's loads (the eigenbasis, the 16 lines and the inverse
+    // eigenvalues its result tiles take) are issued before the first store
+    // (every load goes to a valid address; values past c are zeroed after), so
+    // the staging takes one memory round trip.
+    constexpr int UQ = (kMgPlaneMax * kMgPlaneMax + kMgLineThreads - 1) / kMgLineThreads;
+    constexpr int UL = (16 * kMgPlaneMax + kMgLineThreads - 1) / kMgLineThreads;
+    constexpr int TW = (kMgPlaneMax / 16 + kMgLineThreads / 64 - 1) / (kMgLineThreads / 64);
+    double vq[UQ], vl[UL], il[TW][4];
+#pragma unroll
+    for (int u = 0; u < UQ; u++) {
+        const int x = min((int)threadIdx.x + u * kMgLineThreads, CP * CP - 1);
         const int r = x / CP, q = x % CP;
-        Q[r * P + q] = r < c && q < c ? Qf[r * c + q] : 0.0;
+        const bool in_c = r < c && q < c;
+        const double a = Qf[in_c ? r * c + q : 0];
+        vq[u] = in_c ? a : 0.0;
     }
-    for (int x = threadIdx.x; x < 16 * CP; x += kMgLineThreads) {
+#pragma unroll
+    for (int u = 0; u < UL; u++) {
+        const int x = min((int)threadIdx.x + u * kMgLineThreads, 16 * CP - 1);
         const int l = x & 15, i = x >> 4;  // consecutive threads: consecutive k
-        L[l * P + i] = (i < c && k0 + l < c) ? w[(long)i * c2 + j * c + k0 + l] : 0.0;
+        const bool in_c = i < c && k0 + l < c;
+        const double a = w[in_c ? (long)i * c2 + j * c + k0 + l : 0];
+        vl[u] = in_c ? a : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < TW; v++) {
+        const int t = (wave + v * (kMgLineThreads / 64)) * 16 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int l = lq + 4 * r;
+            const bool in_c = t < c && k0 + l < c;
+            const double a = ilam[in_c ? (long)t * c2 + j * c + k0 + l : 0];
+            il[v][r] = in_c ? a : 0.0;
+        }
+    }
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int u = 0; u < UQ; u++) {
+        const int x = (int)threadIdx.x + u * kMgLineThreads;
+        if (x < CP * CP) Q[(x / CP) * P + x % CP] = vq[u];
+    }
+#pragma unroll
+    for (int u = 0; u < UL; u++) {
+        const int x = (int)threadIdx.x + u * kMgLineThreads;
+        if (x < 16 * CP) L[(x & 15) * P + (x >> 4)] = vl[u];
     }
     if (done) return;
     lds_barrier();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
-    const int nt = CP / 16;
     // O[l][t] = ilam[t][j][k0 + l] sum_i L[l][i] Qf[t][i]
-    for (int tt = wave; tt < nt; tt += kMgLineThreads / 64) {
+#pragma unroll
+    for (int v = 0; v < TW; v++) {
+        const int tt = wave + v * (kMgLineThreads / 64);
+        if (tt >= nt) break;
         const int t0 = tt * 16;
         const mg_v4d acc = mfma_tile_f64(
             K4, [&](int l, int m) { return L[l * P + m]; },
             [&](int m, int t) { return Q[(t0 + t) * P + m]; });
         const int t = t0 + lr;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int l = lq + 4 * r;
-            O[l * P + t] = (t < c && k0 + l < c) ? acc[r] * ilam[(long)t * c2 + j * c + k0 + l] : 0.0;
-        }
+        for (int r = 0; r < 4; r++) O[(lq + 4 * r) * P + t] = acc[r] * il[v][r];
     }
     lds_barrier();
     // L[l][i] = sum_t O[l][t] Qf[t][i]
