@@ -222,6 +222,29 @@ __device__ __forceinline__ void tile_for(F&& f) {
     }
 }
 
+// A staging copy of N elements in tile_for's mapping, in two halves: load()
+// into registers and store() to LDS, so that a kernel issues the loads of all
+// its regions before the first store (a load-and-store loop waits for each
+// load in turn).  f(x) must read a valid address for every x < N; the threads
+// past N load element N - 1 again and skip the store.
+template <int N, int NT, class T>
+struct TileStage {
+    static constexpr int U = (N + NT - 1) / NT;
+    T v[U];
+    template <class F>
+    __device__ __forceinline__ void load(F&& f) {
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = f(min((int)threadIdx.x + u * NT, N - 1));
+    }
+    __device__ __forceinline__ void store(T* lds) const {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int x = (int)threadIdx.x + u * NT;
+            if (x < N) lds[x] = v[u];
+        }
+    }
+};
+
 template <int TI, int TJ, int TK>
 struct MgTileMap {
     int i0, j0, k0;
@@ -266,10 +289,13 @@ __global__ __launch_bounds__(NT) void k_mg_down(int n, const R* __restrict__ b,
     if (tid >= 64 && tid < 64 + BJ) gj[tid - 64] = wrap_mod(T.j0 - 3 + tid - 64, n) * n;
     if (tid >= 128 && tid < 128 + BK) gk[tid - 128] = wrap_mod(T.k0 - 3 + tid - 128, n);
     lds_barrier();
-    tile_for<BI * BJ * BK, NT>([&](int x) {
+    TileStage<BI * BJ * BK, NT, R> lb;
+    lb.load([&](int x) {
         const int k = x % BK, j = (x / BK) % BJ, i = x / (BK * BJ);
-        sb[x] = b[gi[i] + gj[j] + gk[k]];
+        return b[gi[i] + gj[j] + gk[k]];
     });
+    __asm__ volatile("" ::: "memory");  // every load is issued before the first store
+    lb.store(sb);
     lds_barrier();
     // Sweeps from zero with weights c1 then c2: d2 = -(c1 + c2) b - c1 c2 A b.
     const R cs = c1 + c2, cp = c1 * c2;
@@ -346,18 +372,25 @@ __global__ __launch_bounds__(NT) void k_mg_up(int n, const R* __restrict__ b,
     if (tid >= 160 && tid < 160 + EJ) cj[tid - 160] = wrap_mod(T.j0 / 2 - 1 + tid - 160, nc) * nc;
     if (tid >= 192 && tid < 192 + EK) ck[tid - 192] = wrap_mod(T.k0 / 2 - 1 + tid - 192, nc);
     lds_barrier();
-    tile_for<DI * DJ * DK, NT>([&](int x) {
+    TileStage<DI * DJ * DK, NT, R> ld;
+    TileStage<PI * PJ * PK, NT, R> lb;
+    TileStage<EI * EJ * EK, NT, R> le;
+    ld.load([&](int x) {
         const int k = x % DK, j = (x / DK) % DJ, i = x / (DK * DJ);
-        sd[x] = d2[gi[i] + gj[j] + gk[k]];
+        return d2[gi[i] + gj[j] + gk[k]];
     });
-    tile_for<PI * PJ * PK, NT>([&](int x) {
+    lb.load([&](int x) {
         const int k = x % PK, j = (x / PK) % PJ, i = x / (PK * PJ);
-        sb[x] = b[gi[i + 1] + gj[j + 1] + gk[k + 1]];
+        return b[gi[i + 1] + gj[j + 1] + gk[k + 1]];
     });
-    tile_for<EI * EJ * EK, NT>([&](int x) {
+    le.load([&](int x) {
         const int k = x % EK, j = (x / EK) % EJ, i = x / (EK * EJ);
-        se[x] = e[ci[i] + cj[j] + ck[k]];
+        return e[ci[i] + cj[j] + ck[k]];
     });
+    __asm__ volatile("" ::: "memory");  // every load is issued before the first store
+    ld.store(sd);
+    lb.store(sb);
+    le.store(se);
     lds_barrier();
     // d = d2 + trilinear(e): fine local index l has coarse neighbours l>>1 and
     // (l>>1)+1 and weight 1/2 on odd l (tile origins are even).
