@@ -91,6 +91,15 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// A load of solver state on the vector memory path (an agent-scope relaxed
+// atomic load): the compiler turns a plain load of a uniform address into a
+// scalar load, and a kernel that needs its scalar kernel arguments before it
+// can issue more loads would then wait for the state's round trip as well.
+template <class T>
+__device__ __forceinline__ T vload_state(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // x^3 rounded once (the reference's pow(x, 3)): x*x split exactly into
 // hi + lo by an FMA, then hi*x + lo*x with one final rounding.
 __device__ __forceinline__ double cube(double x) {

@@ -351,29 +351,42 @@ __global__ __launch_bounds__(kBlock) void k_refine(GridDims d, PBCoef pb, double
                                                    float* r32, RefineArgs a, Reduce red,
                                                    CGState* st, const double* dps,
                                                    const double* dpm, int dcnt) {
-    int sel = 0, in_done = 0, in_res = 0;
-    double dscale = 0.0;
+    int sel = 0;
     if (Add) {
         if (st->outer_done) return;
         sel = st->ysel;
-        int r = INT_MIN;
-        if (dps) {
-            __shared__ double shs[2 * kBlock / kWave];
-            r = df_stop_test<kBlock>(st, dps, dpm, dcnt, shs, /*record=*/false);
-        }
-        if (r != INT_MIN) {
-            in_done = 1;
-            in_res = r;
-        } else if (st->done) {
-            in_done = 1;
-            in_res = st->res;
-        }
-        dscale = (in_done ? in_res : st->iter) > 0 ? 1.0 : 0.0;
     }
+    // Read before the sweep: only the last work-group's finalize writes the
+    // solver state in this kernel, after every work-group's sweep.
+    const int done0 = Add ? st->done : 0, res0 = Add ? st->res : 0, iter0 = Add ? st->iter : 0;
+    __shared__ double shf[3 * kBlock / kWave + 1];
     BodyRefine<Add, Add> bd{sel ? y0 : y1, sigma, r32, a.otol, a.rtol, a.limit, a.pcg};
-    bd.in_done = in_done;
-    bd.in_res = in_res;
-    march<PB>(d, pb, SrcSum<Add>{sel ? y1 : y0, dc, dscale}, bd);
+    SrcSum<Add> src{sel ? y1 : y0, dc, 0.0};
+    // The deferred stop test folds while the first plane's loads are in flight.
+    march<PB>(d, pb, src, bd, [&] {
+        if (Add) {
+            int r = INT_MIN;
+            if (dps) {
+                double Z, S, M;
+                int flag = 0;
+                fold_partials<kBlock, false, true>(nullptr, 0, dps, dpm, dcnt, shf, Z, S, M,
+                                                   flag);
+                r = df_decide(st, S, M, /*record=*/false);
+            }
+            int in_done = 0, in_res = 0;
+            if (r != INT_MIN) {
+                in_done = 1;
+                in_res = r;
+            } else if (done0) {
+                in_done = 1;
+                in_res = res0;
+            }
+            bd.in_done = in_done;
+            bd.in_res = in_res;
+            src.dscale = (in_done ? in_res : iter0) > 0 ? 1.0 : 0.0;
+        }
+        return true;
+    });
     reduce_and_finalize<true>(bd, red, st);
 }
 

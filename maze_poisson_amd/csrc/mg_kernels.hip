@@ -2234,7 +2234,9 @@ void launch_pcg_dir(hipStream_t s, const GridDims& d, const float* p_old, const 
 // partials pus, pum, ucnt of them): a stop ends the solve here, before p
 // changes (the V-cycle before it wrote only work arrays).  Work-group 0
 // records the stop while later work-groups may be starting, so the flag of
-// earlier kernels is read once per work-group.
+// earlier kernels is thread 0's read at the start, handed to every wave by
+// the fold's barrier; it is tested once the first plane's loads and the
+// fold's loads are in flight, so no memory round trip precedes them.
 template <bool First, class R = float>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const R* p_old,
                                                        const R* v, R* p_new,
@@ -2243,18 +2245,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const R* p_ol
                                                        const double* __restrict__ pum, int ucnt,
                                                        double* __restrict__ pp, int par,
                                                        CGState* st) {
-    if (First ? st->done != 0 : wg_done(st)) return;
+    int done = vload_state(&st->done);  // thread 0's value counts
+    const double rz_prev = First ? 1.0 : vload_state(&st->rdv2[par ^ 1]);
     __shared__ double sh[2 * kBlock / kWave];
+    __shared__ double shf[3 * kBlock / kWave + 1];
     SrcPcgDir<R, First> src{p_old, v, 0.0};
     BodyPcgDir<R> bd{p_new};
     // The folds run while the first plane's loads are in flight.
     const bool go = march<false>(d, PBCoef{}, src, bd, [&] {
-        if (!First) {
-            __shared__ double shu[2 * kBlock / kWave];
-            if (df_stop_test<kBlock>(st, pus, pum, ucnt, shu) != INT_MIN) return false;
-        }
-        const double S = fold_sum<kBlock>(pz, zcnt, sh);
-        const double beta = First ? 0.0 : S / st->rdv2[par ^ 1];
+        double S, Su, Mu;
+        fold_partials<kBlock, true, !First>(pz, zcnt, pus, pum, ucnt, shf, S, Su, Mu, done);
+        if (done) return false;
+        if (!First && df_decide(st, Su, Mu) != INT_MIN) return false;
+        const double beta = First ? 0.0 : S / rz_prev;
         src.beta = beta;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->rdv2[par] = S;
@@ -2275,20 +2278,27 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir_df(GridDims d, const R* p_ol
 // x += alpha p, r += alpha A p, and plain partials of <r, r> and max|r| for
 // the consumer's stop test.  A vanishing <p, A p> stops the solve, as the
 // breakdown guard of BodyDir does (alpha 0: x and r keep their values).
-// Work-group 0 advances st->iter (no other work-group of this kernel reads it).
+// Work-group 0 advances st->iter (no other work-group of this kernel reads it)
+// and may record a breakdown while later work-groups start, so the stop flag
+// is thread 0's read at the start, handed to every wave by the fold's barrier.
 template <bool ZeroX, class R = float>
 __global__ __launch_bounds__(kBlock) void k_cg_update_df(GridDims d, const R* p, R* x,
                                                          R* r, const double* __restrict__ pp,
                                                          double* __restrict__ pus,
                                                          double* __restrict__ pum, int par,
                                                          CGState* st) {
-    if (wg_done(st)) return;  // work-group 0 may record a breakdown meanwhile
+    int done = vload_state(&st->done);  // thread 0's value counts
+    const double rz = vload_state(&st->rdv2[par]);
     __shared__ double sh[2 * kBlock / kWave];
+    __shared__ double shf[3 * kBlock / kWave + 1];
     BodyUpdate<R, ZeroX> bd{x, r, 0.0};
-    march<false>(d, PBCoef{}, SrcArr<R>{p}, bd, [&] {
-        const double pAp = fold_sum<kBlock>(pp, d.blocks, sh);
+    const bool go = march<false>(d, PBCoef{}, SrcArr<R>{p}, bd, [&] {
+        double pAp, s0, m0;
+        fold_partials<kBlock, true, false>(pp, d.blocks, nullptr, nullptr, 0, shf, pAp, s0, m0,
+                                           done);
+        if (done) return false;
         const bool breakdown = fabs(pAp) < 1e-300;
-        const double alpha = breakdown ? 0.0 : st->rdv2[par] / pAp;
+        const double alpha = breakdown ? 0.0 : rz / pAp;
         bd.alpha = alpha;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->pAp = pAp;
@@ -2302,6 +2312,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_df(GridDims d, const R* p,
         }
         return true;
     });
+    if (!go) return;
     double s = bd.s, m = bd.m;
     block_sum_max_lds<kBlock>(s, m, sh);
     if (threadIdx.x == 0) {

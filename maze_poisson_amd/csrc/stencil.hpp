@@ -132,7 +132,7 @@ __device__ __forceinline__ bool march(const GridDims& d, const C& pb, const Src&
     const int s = b / d.chunks;
     const int pl0 = chunk * kBlock + threadIdx.x;
     const bool active = pl0 < d.n2;
-    const int pl = active ? pl0 : 0;  // threads past the end load nothing
+    const int pl = active ? pl0 : 0;  // threads past the end compute nothing
     const int n = d.n, n2 = d.n2;
     const PlaneNb nb = plane_nb(pl, n, n2);
     const int i0 = s * d.slab;
@@ -158,15 +158,13 @@ __device__ __forceinline__ bool march(const GridDims& d, const C& pb, const Src&
         o.cf = pb_load<PB>(pb, base, pl, nb);
         return o;
     };
-    SR rm{}, rc{};
-    double exm = 0.0;
-    Ops A{};
-    if (active) {
-        rm = src.load(im0 * n2 + pl);
-        rc = src.load(i0 * n2 + pl);
-        if (PB) exm = pb.ex[im0 * n2 + pl];
-        A = fetch(i0);
-    }
+    // Threads past the end of the plane load point 0 of their planes (in
+    // bounds, unused): unconditional loads leave the compiler no reason to
+    // convert their values before the hook, which would wait for them there.
+    const SR rm = src.load(im0 * n2 + pl);
+    const SR rc = src.load(i0 * n2 + pl);
+    double exm = PB ? pb.ex[im0 * n2 + pl] : 0.0;
+    Ops A = fetch(i0);
     const bool go = hook();
     if (!active || !go) return go;
     double vm = src.val(rm);
@@ -303,17 +301,8 @@ __device__ __forceinline__ void block_sum_max_lds(double& s, double& m, double* 
     m = b;
 }
 
-// Sum (and max) of partials[0..cnt) in a fixed order; valid in all threads.
+// Sum and max of partials[0..cnt) in a fixed order; valid in all threads.
 // Called once per kernel, before any other use of sh.
-template <int NT>
-__device__ __forceinline__ double fold_sum(const double* __restrict__ p, int cnt, double* sh) {
-    double a = 0.0, m = 0.0;
-    for (int i = threadIdx.x; i < cnt; i += NT) a += p[i];
-    block_sum_max_lds<NT>(a, m, sh);
-    lds_barrier();  // sh may be reused after the fold
-    return a;
-}
-
 template <int NT>
 __device__ __forceinline__ void fold_sum_max(const double* __restrict__ ps,
                                              const double* __restrict__ pm, int cnt, double* sh,
@@ -329,14 +318,74 @@ __device__ __forceinline__ void fold_sum_max(const double* __restrict__ ps,
     M = b;
 }
 
-// The stop test of a multigrid-PCG update, from its partials (pus, pum): the
-// same decision as BodyUpdate::finalize, with st->iter already advanced by the
-// update.  Returns the result code if the test stops the solve (converged or
-// out of budget), else INT_MIN; work-group 0 records a stop in st.  The test
-// reads only fields that no consumer writes, so every work-group of a kernel
-// decides alike whenever work-group 0 writes; a stop recorded by an earlier
-// kernel (st->done, e.g. a breakdown) is the caller's to check.
-// sh: 2 NT / 64 doubles.
+// The folds a deferred-finalisation consumer starts from, together: the sum
+// over z[0..zcnt), and the sum over s and the maximum over m (both [0..ucnt);
+// an array left out, HasZ or HasU false, leaves its result 0; a folded count
+// is at least 1), bit for bit as fold_sum_max forms them.
+// A thread reads its slots (t, t + NT, ...) in batches of kFoldBatch slots of
+// every array and issues a batch's loads before it adds the first, so one
+// memory round trip covers the batch (a loop that adds each slot as it
+// arrives costs one round trip per slot and array).  flag: thread 0's value,
+// handed to the whole work-group by the same barrier.  sh: its own
+// 3 NT / 64 + 1 doubles (not reused, so no closing barrier); results in all
+// threads.
+constexpr int kFoldBatch = 4;
+
+template <int NT, bool HasZ, bool HasU>
+__device__ __forceinline__ void fold_partials(const double* __restrict__ pz, int zcnt,
+                                              const double* __restrict__ ps,
+                                              const double* __restrict__ pm, int ucnt,
+                                              double* sh, double& Z, double& S, double& M,
+                                              int& flag) {
+    constexpr int NW = NT / kWave;
+    double az = 0.0, as = 0.0, am = 0.0;
+    if (!HasZ) zcnt = 0;
+    if (!HasU) ucnt = 0;
+    const int cnt = max(zcnt, ucnt);
+    for (int i0 = threadIdx.x; i0 < cnt; i0 += kFoldBatch * NT) {
+        double vz[kFoldBatch], vs[kFoldBatch], vm[kFoldBatch];
+#pragma unroll
+        for (int u = 0; u < kFoldBatch; u++) {
+            const int i = i0 + u * NT;
+            if (HasZ) vz[u] = pz[min(i, zcnt - 1)];
+            if (HasU) {
+                vs[u] = ps[min(i, ucnt - 1)];
+                vm[u] = pm[min(i, ucnt - 1)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFoldBatch; u++) {
+            const int i = i0 + u * NT;
+            if (HasZ && i < zcnt) az += vz[u];
+            if (HasU && i < ucnt) {
+                as += vs[u];
+                am = fmax(am, vm[u]);
+            }
+        }
+    }
+    az = wave_sum(az);
+    as = wave_sum(as);
+    am = wave_max(am);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sh[w] = az;
+        sh[NW + w] = as;
+        sh[2 * NW + w] = am;
+    }
+    if (threadIdx.x == 0) sh[3 * NW] = (double)flag;
+    lds_barrier();
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = 0; i < NW; i++) {
+        a += sh[i];
+        b += sh[NW + i];
+        c = fmax(c, sh[2 * NW + i]);
+    }
+    flag = (int)sh[3 * NW];
+    Z = a;
+    S = b;
+    M = c;
+}
+
 // Whether the solve has ended, read once per work-group (one barrier).  A
 // kernel in which work-group 0 may record a stop while other work-groups are
 // starting needs this form: were every wave to read st->done itself, the
@@ -349,6 +398,14 @@ __device__ __forceinline__ bool wg_done(const CGState* st) {
     return flag != 0;
 }
 
+// The stop test of a multigrid-PCG update, from its partials (pus, pum): the
+// same decision as BodyUpdate::finalize, with st->iter already advanced by the
+// update.  Returns the result code if the test stops the solve (converged or
+// out of budget), else INT_MIN; work-group 0 records a stop in st.  The test
+// reads only fields that no consumer writes, so every work-group of a kernel
+// decides alike whenever work-group 0 writes; a stop recorded by an earlier
+// kernel (st->done, e.g. a breakdown) is the caller's to check.
+// sh: 2 NT / 64 doubles.
 // record: work-group 0 writes a stop into st.  A kernel whose own last
 // work-group rewrites the solver state (the refinement pass) must not record:
 // two work-groups writing the same fields in one kernel leave either value.
