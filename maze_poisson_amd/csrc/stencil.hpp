@@ -88,8 +88,8 @@ struct SrcZero {
 // Software-pipelined by one plane: the operands of plane i+1 are requested
 // before plane i is computed, so each thread has two planes of loads in
 // flight and the march is not one memory latency per plane.  (The extra
-// plane fetched after the last one is in bounds -- indices wrap -- and
-// unused.)  Every pass writes arrays other than the ones its source reads,
+// plane fetched after the last one of an even count is in bounds -- indices
+// wrap -- and unused.)  Every pass writes arrays other than the ones its source reads,
 // so requesting the next plane before this plane's stores is safe.
 struct PBPlane {
     double exc, eyc, ezc, eym, ezm, k2c;
@@ -185,14 +185,23 @@ __device__ __forceinline__ bool march(const GridDims& d, const C& pb, const Src&
         vm = vc;
         vc = vp;
     };
-    // Two operand sets alternate (no register copies of pending loads).
-    for (int i = i0; i < i1; i += 2) {
+    // Two operand sets alternate (no register copies of pending loads).  The
+    // loop has no exit between its halves: with one there, the compiler's
+    // structured control flow adds an edge on which the second set's loads
+    // are still pending at the loop head, and the head then waits for the
+    // newest loads before it issues the next plane's (one plane in flight,
+    // not two).  The scheduling barriers keep each plane's loads ahead of
+    // the other plane's arithmetic.
+    int i = i0;
+    for (; i + 1 < i1; i += 2) {
         const Ops B = fetch(i + 1);
+        __builtin_amdgcn_sched_barrier(0);
         compute(i, A);
-        if (i + 1 >= i1) break;
         A = fetch(i + 2);
+        __builtin_amdgcn_sched_barrier(0);
         compute(i + 1, B);
     }
+    if (i < i1) compute(i, A);
     return true;
 }
 
