@@ -205,10 +205,25 @@ __device__ __forceinline__ double load_partial(const double* p, int i) {
 
 // Single-pass reduction hand-off: thread 0 publishes the work-group's
 // partial(s), waits until those device-scope stores have completed, then
-// bumps the arrival counter; the work-group that sees nblocks-1 reduces all
-// partials in index order (so the result does not depend on arrival order).
-// Only the partials need to be visible to the last work-group, so no
-// device-wide release fence (an L2 write-back on this chip) is issued.
+// arrives; the work-group that arrives last reduces all partials in index
+// order (so the result does not depend on arrival order).  Only the partials
+// need to be visible to the last work-group, so no device-wide release fence
+// (an L2 write-back on this chip) is issued.
+//
+// Arrival is two-level: work-group b bumps the counter of its group b % 8
+// (consecutive ids go round-robin to the 8 XCDs, so a group is one XCD's
+// share), and the last of each group bumps the top counter.  All work-groups
+// of a sweep finish within a short time, and device-scope atomics on one
+// address are performed one after another, so a single counter serialises
+// one atomic per work-group at the end of every reducing kernel; eight group
+// counters on their own cache lines take them side by side.  The chain of
+// returned counts orders every group's stores before the last arrival.
+// counter: kArrivalWords words (the top counter, then one per group, each on
+// its own 128-byte line), all zero between kernels.
+constexpr int kArrivalStride = 32;
+constexpr int kArrivalGroups = 8;
+constexpr int kArrivalWords = (kArrivalGroups + 1) * kArrivalStride;
+
 __device__ __forceinline__ bool publish_arrive(double* psum, double s, double* pmax, double m,
                                                unsigned int* counter, unsigned int nblocks) {
     __shared__ bool last;
@@ -216,12 +231,29 @@ __device__ __forceinline__ bool publish_arrive(double* psum, double s, double* p
         store_partial(psum, blockIdx.x, s);
         if (pmax) store_partial(pmax, blockIdx.x, m);
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const unsigned int prev =
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == nblocks - 1);
+        const unsigned int g = blockIdx.x % kArrivalGroups;
+        const unsigned int groups = nblocks < kArrivalGroups ? nblocks : kArrivalGroups;
+        const unsigned int members = (nblocks - g + kArrivalGroups - 1) / kArrivalGroups;
+        const unsigned int prev = __hip_atomic_fetch_add(counter + kArrivalStride * (1 + g), 1u,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool l = false;
+        if (prev == members - 1) {
+            const unsigned int top =
+                __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            l = (top == groups - 1);
+        }
+        last = l;
     }
     __syncthreads();
     return last;
+}
+
+// Zeroes the counters of publish_arrive (called by the last work-group, when
+// every arrival of the kernel has been counted).
+__device__ __forceinline__ void reset_arrivals(unsigned int* counter) {
+    for (int g = 0; g <= kArrivalGroups; g++)
+        __hip_atomic_store(counter + kArrivalStride * g, 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void reset_counter(unsigned int* counter) {

@@ -507,7 +507,7 @@ __global__ void k_sum_scaled(long n, double scale, const double* __restrict__ v,
         const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
         if (threadIdx.x == 0) {
             *out = T;
-            reset_counter(red.counter);
+            reset_arrivals(red.counter);
         }
     }
 }
@@ -536,7 +536,7 @@ __global__ void k_dot_scaled(long n, double scale, const double* __restrict__ a,
         const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
         if (threadIdx.x == 0) {
             *out = scale * T;
-            reset_counter(red.counter);
+            reset_arrivals(red.counter);
         }
     }
 }

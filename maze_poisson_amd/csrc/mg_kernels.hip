@@ -2199,7 +2199,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(GridDims d, C pb, const floa
         const double T = reduce_partials_sum(red.psum, gridDim.x, sh);
         if (threadIdx.x == 0) {
             BodyPcgDir<float>::finalize(st, T);
-            reset_counter(red.counter);
+            reset_arrivals(red.counter);
         }
     }
 }

@@ -24,7 +24,7 @@ constexpr long kNoAllocMaxPartials = 4096;
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 constexpr size_t kOffState = 0;
 constexpr size_t kOffCounter = align256(kOffState + sizeof(CGState));
-constexpr size_t kOffPsum = align256(kOffCounter + 4 * sizeof(unsigned int));
+constexpr size_t kOffPsum = align256(kOffCounter + kArrivalWords * sizeof(unsigned int));
 constexpr size_t kOffPmax = align256(kOffPsum + kNoAllocMaxPartials * sizeof(double));
 constexpr size_t kOffR = align256(kOffPmax + kNoAllocMaxPartials * sizeof(double));
 constexpr size_t kOffP0 = align256(kOffR + kNoAllocMaxG * sizeof(double));
@@ -58,7 +58,7 @@ unsigned char* scratch_on_current_device() {
 }
 
 __global__ void k_clear_counters(unsigned int* c) {
-    if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < kArrivalWords; i += blockDim.x) c[i] = 0u;
 }
 
 }  // namespace

@@ -212,7 +212,7 @@ __global__ void k_contrib_sum(long total, const double* __restrict__ v, Reduce r
         if (threadIdx.x == 0) {
             ps->q_sum = T;
             ps->epoch += 1u;
-            reset_counter(red.counter);
+            reset_arrivals(red.counter);
         }
     }
 }

@@ -118,7 +118,7 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     MAZE_HIP(hipEventCreateWithFlags(&ev_tf, hipEventDisableTiming));
     red_side.psum = dalloc<double>(kMaxPartials);
     red_side.pmax = dalloc<double>(kMaxPartials);
-    red_side.counter = dalloc<unsigned int>(4);
+    red_side.counter = dalloc<unsigned int>(kArrivalWords);
     if (const char* e = std::getenv("MAZE_TF_OVERLAP")) tf_overlap = std::atoi(e) != 0;
     if (const char* e = std::getenv("MAZE_TF_LATE")) tf_late = std::atoi(e) != 0;
     if (const char* e = std::getenv("MAZE_INNER_MAX")) inner_max = std::atoi(e);
@@ -131,7 +131,7 @@ Sim::Sim(hipStream_t main, hipStream_t side_) {
     std::memset(st_host, 0, sizeof(CGState));
     red.psum = dalloc<double>(kMaxPartials);
     red.pmax = dalloc<double>(kMaxPartials);
-    red.counter = dalloc<unsigned int>(4);
+    red.counter = dalloc<unsigned int>(kArrivalWords);
     dred.pz = dalloc<double>(kMaxPartials);
     dred.pp = dalloc<double>(kMaxPartials);
     dred.pus = dalloc<double>(kMaxPartials);

@@ -227,7 +227,7 @@ __device__ __forceinline__ void reduce_and_finalize(Body& bd, Reduce red, CGStat
             S = reduce_partials_sum<NT>(red.psum, gridDim.x, sh);
         if (threadIdx.x == 0) {
             bd.finalize(st, S, M);
-            reset_counter(red.counter);
+            reset_arrivals(red.counter);
         }
     }
 }
