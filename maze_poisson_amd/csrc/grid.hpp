@@ -132,6 +132,10 @@ void launch_refine_extrap(hipStream_t s, const GridDims& d, const PBCoef& pb, co
                           RefineArgs a, CGState* st, Reduce red);
 
 void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u);
+// phi -= (st->ysel ? yb : ya) if st->outer_done, else nothing (the same
+// values as launch_axpy with alpha -1 on the selected buffer).
+void launch_phi_update_gated(hipStream_t s, long n, const CGState* st, const double* ya,
+                             const double* yb, double* phi);
 void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out);
 void launch_fill(hipStream_t s, long n, double v, double* out);
 void launch_fill_f(hipStream_t s, long n, float v, float* out);

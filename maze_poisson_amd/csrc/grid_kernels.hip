@@ -435,6 +435,16 @@ __global__ void k_axpy(long n, double alpha, const double* __restrict__ v, doubl
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         u[i] += alpha * v[i];
 }
+// phi -= y_final once the mixed-precision outer loop has converged, y_final
+// being the y buffer the device selected (ysel); nothing otherwise.
+__global__ void k_phi_update_gated(long n, const CGState* __restrict__ st,
+                                   const double* __restrict__ ya, const double* __restrict__ yb,
+                                   double* __restrict__ phi) {
+    if (!st->outer_done) return;
+    const double* __restrict__ v = st->ysel ? yb : ya;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        phi[i] += -1.0 * v[i];
+}
 __global__ void k_scale_copy(long n, double c, const double* __restrict__ in, double* __restrict__ out) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         out[i] = in[i] * c;
@@ -480,6 +490,11 @@ static int pw_blocks(long n) {
 
 void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u) {
     MAZE_LAUNCH(k_axpy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, alpha, v, u);
+}
+
+void launch_phi_update_gated(hipStream_t s, long n, const CGState* st, const double* ya,
+                             const double* yb, double* phi) {
+    MAZE_LAUNCH(k_phi_update_gated, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, st, ya, yb, phi);
 }
 void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out) {
     MAZE_LAUNCH(k_scale_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, c, in, out);

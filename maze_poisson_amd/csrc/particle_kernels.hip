@@ -239,6 +239,7 @@ struct ForceKick {
     double* vel;
     const double* mass;
     double cdt;
+    const int* gate = nullptr;  // if set: the launch does nothing while *gate is 0
 };
 
 template <int CAS, bool Kick = false>
@@ -252,6 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fd(int np, int n, double h,
     using A = Assign<CAS>;
     constexpr int NN = A::kNodes;
     static_assert(kBlock % NN == 0, "whole ions per work-group");
+    if (Kick && fk.gate && *fk.gate == 0) return;
     __shared__ double term[3][kBlock];
     const long t = blockIdx.x * (long)kBlock + threadIdx.x;
     const int ip = (int)(t / NN), c = (int)(t % NN);
@@ -312,10 +314,11 @@ void launch_force_fd(hipStream_t s, const PDev& p, const double* phi) {
                     p.charges, p.pos, p.fe, ForceKick{});
 }
 
-void launch_force_fd_kick(hipStream_t s, const PDev& p, const double* phi, double cdt) {
+void launch_force_fd_kick(hipStream_t s, const PDev& p, const double* phi, double cdt,
+                          const int* gate) {
     if (p.np == 0) return;
     const int nb = blocks_exact((long)p.nn * p.np);
-    const ForceKick fk{p.fn, p.ft, p.vel, p.mass, cdt};
+    const ForceKick fk{p.fn, p.ft, p.vel, p.mass, cdt, gate};
     if (p.cas == 0)
         MAZE_LAUNCH((k_force_fd<0, true>), dim3(nb), dim3(kBlock), 0, s, p.np, p.n, p.h, phi,
                     p.base, p.charges, p.pos, p.fe, fk);

@@ -54,7 +54,10 @@ void launch_contrib_sum(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 void launch_force_fd(hipStream_t s, const PDev& p, const double* phi);
 // The MD step's path: the field force, F_tot = F_elec + F_noel and the
 // velocity half-kick v += cdt F_tot / m in one launch (no PB boundary forces).
-void launch_force_fd_kick(hipStream_t s, const PDev& p, const double* phi, double cdt);
+// gate: if set, the launch does nothing unless *gate is nonzero on the device
+// when it runs (work queued before the host knows whether it is due).
+void launch_force_fd_kick(hipStream_t s, const PDev& p, const double* phi, double cdt,
+                          const int* gate = nullptr);
 long tf_part_doubles(int np);
 void launch_tf(hipStream_t s, const PDev& p, PScalars* ps, Reduce red);
 // The same in pieces: the 64-ion block pairs (I, J >= I) of rows I in [i_lo, i_hi),

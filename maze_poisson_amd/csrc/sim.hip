@@ -1118,6 +1118,7 @@ int Sim::fp64mg_finish() {
 // converged y satisfies the same stop test; only the start differs).  The
 // three y buffers then rotate: y (y_n-1), y_hist (y_n-2) and y_alt (spare).
 int Sim::maze_mixed() {
+    spec_post = 0;  // nothing was queued behind this solve
     mixed_issue();
     return mixed_finish();
 }
@@ -1217,6 +1218,7 @@ void Sim::mixed_pass(int k, int m, bool poll) {
 int Sim::mixed_finish() {
     MixedRun& m = mx;
     wait_published();
+    spec_hit = spec_post && st_host->outer_done;
     while (!st_host->outer_done && st_host->passes < kMaxOuter) {
         mixed_pass(m.queued++, 1, true);
         read_state();
@@ -1235,7 +1237,7 @@ int Sim::mixed_finish() {
         y = fin;
         y_alt = other;
     }
-    launch_axpy(stream, G, -1.0, y, phi);
+    if (!spec_hit) launch_axpy(stream, G, -1.0, y, phi);
     const int res = st_host->outer_done ? st_host->inner_total : -1;
     last_res = res;
     return res;
@@ -1564,12 +1566,25 @@ void Sim::step_issue() {
     if (step_tf && tf_late) issue_tf();
     update_eps_k2();
     step_wait = 1;
+    spec_post = spec_hit = 0;
     const bool pcg_mg = grid_type == kMazeLCG && precision == kFP64 && precond == kPMG && !pb;
     if (grid_type == kMazeLCG && !pcg_mg) {
         step_mg64 = use_fp64mg() ? 1 : 0;
         if (step_mg64) fp64mg_issue();
         else if (precision == kFP64 || precision == kFP64MG) fp64_issue();
         else mixed_issue();
+        // The queued passes usually converge: the field update and the
+        // force / half-kick go into the queue now, each doing nothing unless
+        // the device's outer stop flag is set when it runs, so the GPU does
+        // not idle while the host reads the published state.  step_finish
+        // launches them itself when they did nothing.
+        if (!step_mg64 && precision != kFP64 && precision != kFP64MG && step_tf && !pb &&
+            !pbd.active && itype == kVerlet) {
+            MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
+            launch_phi_update_gated(stream, G, st, mx.ya, mx.yb, phi);
+            launch_force_fd_kick(stream, p, phi, 0.5 * dt, &st->outer_done);
+            spec_post = 1;
+        }
     } else {
         // The other solvers wait for their state themselves; one more read
         // only if none was made (it brings the charge sum).
@@ -1600,8 +1615,10 @@ int Sim::step_finish() {
         // The field force, the total force and the velocity half-kick in one
         // launch (the same values as forces_elec, forces_tot and part2's kick).
         if (!step_tf) forces_noel(false);
-        if (step_tf) MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
-        launch_force_fd_kick(stream, p, phi, itype == kVerlet ? 0.5 * dt : 0.5 * dt * c2);
+        if (!spec_hit) {
+            if (step_tf) MAZE_HIP(hipStreamWaitEvent(stream, ev_tf, 0));
+            launch_force_fd_kick(stream, p, phi, itype == kVerlet ? 0.5 * dt : 0.5 * dt * c2);
+        }
         if (itype != kVerlet && enabled)
             launch_o_block(stream, p, seed, calls++, std::sqrt(c1), (1 - c1) * kBoltz * T);
         return res;

@@ -199,8 +199,11 @@ struct Sim {
         RefineArgs ra{};
         int inner_limit = 0, queued = 0;
     } mx;
-    // An MD step in progress (step_issue .. step_finish).
-    int step_tf = 0, step_wait = 0, step_res = 0;
+    // An MD step in progress (step_issue .. step_finish).  spec_post: the
+    // field update and the force / half-kick were queued behind the solve,
+    // gated on the device by the outer loop's stop flag; spec_hit: they ran
+    // (the published state says the loop had converged).
+    int step_tf = 0, step_wait = 0, step_res = 0, spec_post = 0, spec_hit = 0;
     Reduce red{};
     // Partials of the multigrid-PCG inner iteration (deferred finalisation).
     DeferRed dred{};
