@@ -273,6 +273,9 @@ void maze_test_rbgs(const double *in, double *out, int n, int iters);
 int maze_test_restrict27(const double *in, double *out, int n);
 int maze_test_prolong_add(const double *in, double *out, int nc);
 int maze_test_mg_vcycle(const double *in, double *out, int n);
+/* The block-Jacobi preconditioner v = A_B^-1 r on every 10^3 block (n a
+ * multiple of 10; library/src/precond_blockjacobi.c:170-206). */
+int maze_test_blockjacobi(const double *in, double *out, int n);
 /* Reference Poisson-Boltzmann multigrid pieces (library/src/multigrid_pb.c):
  * RBGS-PB sweeps in place, the coarse coefficient restrictions
  * ((n/2)^3 outputs), one two-grid PB V-cycle (n even, >= 32). */

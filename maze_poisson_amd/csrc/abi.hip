@@ -631,6 +631,23 @@ int maze_test_mg_vcycle_pb(const double* in, double* out, int n, const double* e
     return 0;
 }
 
+int maze_test_blockjacobi(const double* in, double* out, int n) {
+    if (n < 10 || n % 10 != 0) return -1;
+    Sim* s = scratch_sim(n);
+    double* tab = nullptr;
+    MAZE_HIP(hipMalloc(&tab, maze::kBlockJacobiTab * sizeof(double)));
+    std::vector<double> t(maze::kBlockJacobiTab);
+    maze::blockjacobi_tables(t.data());
+    MAZE_HIP(hipMemcpy(tab, t.data(), maze::kBlockJacobiTab * sizeof(double), hipMemcpyHostToDevice));
+    MAZE_HIP(hipMemcpy(s->sigma, in, s->G * sizeof(double), hipMemcpyHostToDevice));
+    maze::launch_blockjacobi(s->stream, n, s->sigma, s->y, tab);
+    s->sync();
+    MAZE_HIP(hipMemcpy(out, s->y, s->G * sizeof(double), hipMemcpyDeviceToHost));
+    MAZE_HIP(hipFree(tab));
+    delete s;
+    return 0;
+}
+
 int maze_test_mg_vcycle(const double* in, double* out, int n) {
     if (n < 32 || n % 2 != 0) return -1;
     Sim* s = scratch_sim(n);

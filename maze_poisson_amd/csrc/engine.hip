@@ -117,6 +117,8 @@ static long long arena_bytes(int n, int np, int precision, const Phys& ph) {
         if (ph.grid_type == kMazeLCG) b += 8 * G;     // preconditioned residual
         if (ph.pb && ph.grid_type != kMazeLCG) b += 4 * 8 * Gc;  // coarse PB coefficients
     }
+    if (ph.grid_type == kMazeLCG && ph.precond == kPBlockJacobi)
+        b += 8 * G + 8 * kBlockJacobiTab;             // preconditioned residual, eigenbasis tables
     if (ph.pb) b += 4 * 8 * G;                        // dielectric maps and screening
     if (precision != kFP64) b += 8 * G + 4 * 4 * G;   // second y, FP32 d, r, p0, p1
     if (precision == kMixedMG) {
@@ -581,8 +583,10 @@ int maze_engine_set_physics(maze_engine* h, int id, int grid_type, int precond, 
                          (grid_type == maze::kMazeLCG && precond == maze::kPMG);
     const bool ok = r && (grid_type == maze::kLCG || grid_type == maze::kMG ||
                           grid_type == maze::kMazeLCG || grid_type == maze::kMazeMG) &&
-                    precond >= maze::kPNone && precond <= maze::kPMG &&
+                    precond >= maze::kPNone && precond <= maze::kPBlockJacobi &&
+                    precond != maze::kPSSOR &&
                     (precond == maze::kPNone || grid_type == maze::kLCG || grid_type == maze::kMazeLCG) &&
+                    (precond != maze::kPBlockJacobi || grid_type != maze::kMazeLCG || r->n % 10 == 0) &&
                     (!mg_type || (r->n % 2 == 0 && r->n >= 32)) && ca_scheme >= 0 && ca_scheme <= 2 &&
                     (itg_type == maze::kOVRVO || itg_type == maze::kVerlet) && eps_s > 0 && eps_int > 0;
     if (!ok) {

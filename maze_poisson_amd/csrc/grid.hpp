@@ -137,6 +137,11 @@ void launch_axpy(hipStream_t s, long n, double alpha, const double* v, double* u
 void launch_phi_update_gated(hipStream_t s, long n, const CGState* st, const double* ya,
                              const double* yb, double* phi);
 void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out);
+// Block-Jacobi preconditioner v = A_B^-1 r on every 10^3 block (n % 10 == 0);
+// tab: the blockjacobi_tables() values (1100 doubles) in device memory.
+constexpr int kBlockJacobiTab = 1100;
+void blockjacobi_tables(double* tab);
+void launch_blockjacobi(hipStream_t s, int n, const double* in, double* out, const double* tab);
 void launch_fill(hipStream_t s, long n, double v, double* out);
 void launch_fill_f(hipStream_t s, long n, float v, float* out);
 void launch_copy(hipStream_t s, long n, const double* in, double* out);

@@ -9,6 +9,8 @@
 // (library/src/laplace.c:301-309, :366-379; library/src/plcg.c:41-83;
 // library/src/verlet.c:221-240), so every point-wise result matches the CPU
 // oracle bit for bit; only the order of the global sums differs.
+#include <cmath>
+
 #include "stencil.hpp"
 
 namespace maze {
@@ -499,6 +501,87 @@ void launch_phi_update_gated(hipStream_t s, long n, const CGState* st, const dou
 void launch_scale_copy(hipStream_t s, long n, double c, const double* in, double* out) {
     MAZE_LAUNCH(k_scale_copy, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, c, in, out);
 }
+// ---- block-Jacobi preconditioner ------------------------------------------------
+// v = A_B^-1 r on every 10^3 block of the grid (library/src/precond_blockjacobi.c:
+// 14-69, 170-206), A_B the 7-point Laplacian of one block with zero values outside
+// it.  The reference inverts A_B (1000 x 1000) and applies it as a dense product;
+// here the block operator's eigenbasis applies the same inverse:
+// A_B = T (x) I (x) I + I (x) T (x) I + I (x) I (x) T with T = tridiag(1, -2, 1) of
+// order 10, whose orthonormal eigenvectors S[m][q] = sqrt(2/11) sin((m+1)(q+1) pi/11)
+// form a symmetric orthogonal S, eigenvalues l_m = -4 sin^2((m+1) pi / 22).  So
+// A_B^-1 = (S (x) S (x) S) diag(1 / (l_a + l_b + l_c)) (S (x) S (x) S): six passes of
+// 10-term sums per point instead of a 1000-term one.  One work-group per block, one
+// thread per point; tab holds S (100 values) then the 1000 inverse eigenvalue sums.
+constexpr int kBJ = 10, kBJ3 = kBJ * kBJ * kBJ;
+
+__global__ __launch_bounds__(1024) void k_blockjacobi(int n, const double* __restrict__ in,
+                                                      double* __restrict__ out,
+                                                      const double* __restrict__ tab) {
+    __shared__ double u[kBJ3], w[kBJ3], S[kBJ * kBJ];
+    const int t = threadIdx.x, nb = n / kBJ, b = blockIdx.x;
+    const int bi = b / (nb * nb), bj = (b / nb) % nb, bk = b % nb;
+    const bool act = t < kBJ3;
+    const int a = act ? t / (kBJ * kBJ) : 0, c = act ? (t / kBJ) % kBJ : 0, e = act ? t % kBJ : 0;
+    const long g = (long)(bi * kBJ + a) * n * n + (long)(bj * kBJ + c) * n + bk * kBJ + e;
+    if (t < kBJ * kBJ) S[t] = tab[t];
+    double il = 0.0;
+    if (act) {
+        u[t] = in[g];
+        il = tab[kBJ * kBJ + t];
+    }
+    __syncthreads();
+    // dst[a][c][e] = sum_q S[m][q] src[...q...], m the index of the pass's axis.
+    auto pass = [&](const double* src, double* dst, int axis) {
+        if (act) {
+            double sum = 0.0;
+            if (axis == 2) {
+                const double* r = src + (a * kBJ + c) * kBJ;
+#pragma unroll
+                for (int q = 0; q < kBJ; q++) sum += S[e * kBJ + q] * r[q];
+            } else if (axis == 1) {
+                const double* r = src + a * kBJ * kBJ + e;
+#pragma unroll
+                for (int q = 0; q < kBJ; q++) sum += S[c * kBJ + q] * r[q * kBJ];
+            } else {
+                const double* r = src + c * kBJ + e;
+#pragma unroll
+                for (int q = 0; q < kBJ; q++) sum += S[a * kBJ + q] * r[q * kBJ * kBJ];
+            }
+            dst[t] = sum;
+        }
+        __syncthreads();
+    };
+    pass(u, w, 2);
+    pass(w, u, 1);
+    pass(u, w, 0);
+    if (act) w[t] *= il;
+    __syncthreads();
+    pass(w, u, 2);
+    pass(u, w, 1);
+    pass(w, u, 0);
+    if (act) out[g] = u[t];
+}
+
+void blockjacobi_tables(double* tab) {
+    const double pi = 3.14159265358979323846;
+    double lam[kBJ];
+    for (int m = 0; m < kBJ; m++) {
+        for (int q = 0; q < kBJ; q++)
+            tab[m * kBJ + q] = std::sqrt(2.0 / (kBJ + 1)) * std::sin((m + 1) * (q + 1) * pi / (kBJ + 1));
+        const double sn = std::sin((m + 1) * pi / (2.0 * (kBJ + 1)));
+        lam[m] = -4.0 * sn * sn;
+    }
+    for (int a = 0; a < kBJ; a++)
+        for (int c = 0; c < kBJ; c++)
+            for (int e = 0; e < kBJ; e++)
+                tab[kBJ * kBJ + (a * kBJ + c) * kBJ + e] = 1.0 / (lam[a] + lam[c] + lam[e]);
+}
+
+void launch_blockjacobi(hipStream_t s, int n, const double* in, double* out, const double* tab) {
+    const int nb = n / kBJ;
+    MAZE_LAUNCH(k_blockjacobi, dim3(nb * nb * nb), dim3(1024), 0, s, n, in, out, tab);
+}
+
 void launch_fill(hipStream_t s, long n, double v, double* out) {
     MAZE_LAUNCH(k_fill, dim3(pw_blocks(n)), dim3(kBlock), 0, s, n, v, out);
 }

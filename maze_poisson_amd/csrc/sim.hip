@@ -182,7 +182,7 @@ Sim::~Sim() {
 
 void Sim::free_grid() {
     for (double** a : {&q, &y, &phi, &phip, &tmp, &sigma, &r, &p0, &p1, &y_alt, &ex, &ey, &ez, &k2,
-                       &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1, &pv})
+                       &mg_r, &c_rhs, &c_eps, &c_r, &c_p0, &c_p1, &pv, &bj_tab})
         dfree(*a);
     for (float** a : {&d32, &r32, &q32a, &q32b}) dfree(*a);
     for (double** a : {&cex, &cey, &cez, &ck2}) dfree(*a);
@@ -231,9 +231,13 @@ void Sim::init_grid(int n_, double L_, double h_, double tol_, double eps, doubl
     if (type < 0 || type > 4) fatal("maze: invalid grid type");
     if ((type == kMG || type == kMazeMG) && (n_ % 2 != 0 || n_ < 32))
         fatal("Multigrid: requires an even grid of at least 32 points per side");
-    if (type == kMazeLCG && precond_ > kPMG)
-        fatal("maze: MAZE-LCG provides the NONE, JACOBI and MG preconditioners (SSOR and "
-              "BLOCKJACOBI are not provided)");
+    if (type == kMazeLCG && precond_ == kPSSOR)
+        fatal("maze: MAZE-LCG provides the NONE, JACOBI, MG and BLOCKJACOBI preconditioners "
+              "(SSOR is not provided)");
+    if (type == kMazeLCG && precond_ > kPBlockJacobi) fatal("maze: invalid preconditioner type");
+    // library/src/precond_blockjacobi.c:91-95 exits unless the grid is whole blocks.
+    if (type == kMazeLCG && precond_ == kPBlockJacobi && n_ % 10 != 0)
+        fatal("Error: The size of the input vector is not a multiple of the block size (10)");
     // library/src/multigrid.c:114-121 exits when the first halving falls below 16.
     if (type == kMazeLCG && precond_ == kPMG && (n_ % 2 != 0 || n_ < 32))
         fatal("Multigrid preconditioner: requires an even grid of at least 32 points per side");
@@ -256,6 +260,14 @@ void Sim::init_grid(int n_, double L_, double h_, double tol_, double eps, doubl
         if (type == kMazeLCG) pv = dalloc<double>(G);
         c_rhs = dalloc<double>(Gc); c_eps = dalloc<double>(Gc); c_r = dalloc<double>(Gc);
         c_p0 = dalloc<double>(Gc); c_p1 = dalloc<double>(Gc);
+    }
+    if (type == kMazeLCG && precond_ == kPBlockJacobi) {
+        pv = dalloc<double>(G);
+        bj_tab = dalloc<double>(kBlockJacobiTab);
+        std::vector<double> t(kBlockJacobiTab);
+        blockjacobi_tables(t.data());
+        MAZE_HIP(hipMemcpy(bj_tab, t.data(), kBlockJacobiTab * sizeof(double),
+                           hipMemcpyHostToDevice));
     }
     last_iters = 32;
     sync();
@@ -1289,6 +1301,17 @@ int Sim::maze_pcg_mg() {
     return res;
 }
 
+// v = P^-1 r of the explicit-preconditioner CG: one reference V-cycle of 5
+// RBGS sweeps per side from the previous v (library/src/precond_mg.c:3-5), or
+// the block-Jacobi inverse on every 10^3 block
+// (library/src/precond_blockjacobi.c:170-206), which overwrites v.
+void Sim::precond_apply(const double* rr, double* v) {
+    if (precond == kPBlockJacobi)
+        launch_blockjacobi(stream, n, rr, v, bj_tab);
+    else
+        mg_vcycle(rr, v, 5);
+}
+
 // Explicit-preconditioner CG, library/src/plcg.c:104-166, with P^-1 = one
 // reference V-cycle of 5 RBGS sweeps per side started from the previous v
 // (library/src/precond_mg.c:3-5; multigrid.c:170-173 does not clear `out`).
@@ -1302,7 +1325,7 @@ int Sim::pcg_mg_solve(const double* b, double* x, double tol_, int limit) {
     launch_fill(stream, G, 0.0, pv);
     read_state();
     if (st_host->done) return st_host->res;  // zero residual: x unchanged
-    mg_vcycle(r, pv, 5);
+    precond_apply(r, pv);
     launch_dot_rv(stream, dims, r, pv, st, red, 1);
     double* pin = p0;
     double* pout = p1;
@@ -1314,7 +1337,7 @@ int Sim::pcg_mg_solve(const double* b, double* x, double tol_, int limit) {
         std::swap(pin, pout);
         read_state();
         if (st_host->done) break;
-        mg_vcycle(r, pv, 5);
+        precond_apply(r, pv);
         launch_dot_rv(stream, dims, r, pv, st, red, 0);
     }
     return st_host->res;
@@ -1486,7 +1509,7 @@ int Sim::update_field() {
         case kLCG:
             return lcg_update();
         case kMazeLCG:
-            if (precision == kFP64 && precond == kPMG && !pb) return maze_pcg_mg();
+            if (explicit_pcg()) return maze_pcg_mg();
             if (use_fp64mg()) {
                 fp64mg_issue();
                 return fp64mg_finish();
@@ -1567,7 +1590,7 @@ void Sim::step_issue() {
     update_eps_k2();
     step_wait = 1;
     spec_post = spec_hit = 0;
-    const bool pcg_mg = grid_type == kMazeLCG && precision == kFP64 && precond == kPMG && !pb;
+    const bool pcg_mg = explicit_pcg();
     if (grid_type == kMazeLCG && !pcg_mg) {
         step_mg64 = use_fp64mg() ? 1 : 0;
         if (step_mg64) fp64mg_issue();
@@ -1603,7 +1626,7 @@ bool Sim::step_ready() const { return !step_wait || published(); }
 // solver's result.
 int Sim::step_finish() {
     int res = step_res;
-    const bool pcg_mg = grid_type == kMazeLCG && precision == kFP64 && precond == kPMG && !pb;
+    const bool pcg_mg = explicit_pcg();
     if (grid_type == kMazeLCG && !pcg_mg) {
         if (step_mg64) res = fp64mg_finish();
         else res = precision == kFP64 || precision == kFP64MG ? fp64_finish() : mixed_finish();

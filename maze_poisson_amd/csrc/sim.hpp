@@ -175,6 +175,7 @@ struct Sim {
     // that a V-cycle can precondition an outer CG (MAZE-LCG with precond MG).
     CGState* st_c = nullptr;    // device
     double* pv = nullptr;       // preconditioned residual v of the explicit PCG
+    double* bj_tab = nullptr;   // block-Jacobi eigenbasis tables (blockjacobi_tables)
     CGState* st_host = nullptr; // pinned
     StatePub* pub_host = nullptr;  // host-coherent, written by the GPU (read_state)
     StatePub* pub_dev = nullptr;   // its device address
@@ -351,6 +352,13 @@ struct Sim {
     // MAZE-LCG with the reference's multigrid preconditioner (FP64 only).
     int maze_pcg_mg();
     int pcg_mg_solve(const double* b, double* x, double tol, int limit);
+    void precond_apply(const double* r, double* v);
+    // MAZE-LCG in FP64 with an explicit preconditioner (MG or BLOCKJACOBI):
+    // the reference's conj_grad_precond (library/src/plcg.c:104-166).
+    bool explicit_pcg() const {
+        return grid_type == kMazeLCG && precision == kFP64 &&
+               (precond == kPMG || precond == kPBlockJacobi) && !pb;
+    }
     void restrict_pb_coef();
     void inner_iter(bool use_mg, float* x, int it);
     void inner_body(bool use_mg, float* x, int it);

@@ -165,6 +165,117 @@ int or_conj_grad_jacobi(const double *b, double *x, double tol, int n) {
     return res;
 }
 
+/* Block-Jacobi preconditioner (library/src/precond_blockjacobi.c:14-69 and
+ * :170-206): the grid is cut into 10^3 blocks (n % 10 == 0) and v = A_B^-1 r
+ * on every block, A_B the 7-point Laplacian of one block with zero values
+ * outside it (diagonal -6, +1 for each in-block neighbour).  The reference
+ * inverts A_B with LAPACK's dgetrf / dgetri and applies it as one row-major
+ * dgemm over the reordered blocks; here the inverse comes from Gauss-Jordan
+ * elimination with partial pivoting, and each block's product sums over the
+ * block's points in increasing order, as the loop form of the reference's
+ * dgemm does (library/src/linalg.c:170-180).  Rows and columns follow the
+ * reference's in-block order ii * 100 + jj * 10 + kk. */
+#define OR_BJ 10
+#define OR_BJ3 (OR_BJ * OR_BJ * OR_BJ)
+static double *bj_inv = NULL;
+
+static void bj_build(void) {
+    const int B = OR_BJ, M = OR_BJ3;
+    double *a = (double *)calloc((size_t)M * 2 * M, sizeof(double)); /* [A | I] */
+    for (int i = 0; i < B; i++)
+        for (int j = 0; j < B; j++)
+            for (int k = 0; k < B; k++) {
+                const int r = (i * B + j) * B + k;
+                double *row = a + (size_t)r * 2 * M;
+                row[r] = -6.0;
+                if (i > 0) row[r - B * B] = 1.0;
+                if (i < B - 1) row[r + B * B] = 1.0;
+                if (j > 0) row[r - B] = 1.0;
+                if (j < B - 1) row[r + B] = 1.0;
+                if (k > 0) row[r - 1] = 1.0;
+                if (k < B - 1) row[r + 1] = 1.0;
+                row[M + r] = 1.0;
+            }
+    double *tmp = (double *)malloc((size_t)2 * M * sizeof(double));
+    for (int c = 0; c < M; c++) {
+        int piv = c;
+        for (int r = c + 1; r < M; r++)
+            if (fabs(a[(size_t)r * 2 * M + c]) > fabs(a[(size_t)piv * 2 * M + c])) piv = r;
+        if (piv != c) {
+            memcpy(tmp, a + (size_t)c * 2 * M, (size_t)2 * M * sizeof(double));
+            memcpy(a + (size_t)c * 2 * M, a + (size_t)piv * 2 * M, (size_t)2 * M * sizeof(double));
+            memcpy(a + (size_t)piv * 2 * M, tmp, (size_t)2 * M * sizeof(double));
+        }
+        double *pr = a + (size_t)c * 2 * M;
+        const double d = 1.0 / pr[c];
+        for (int q = 0; q < 2 * M; q++) pr[q] *= d;
+        for (int r = 0; r < M; r++) {
+            if (r == c) continue;
+            double *rr = a + (size_t)r * 2 * M;
+            const double f = rr[c];
+            if (f == 0.0) continue;
+            for (int q = c; q < 2 * M; q++) rr[q] -= f * pr[q];
+        }
+    }
+    bj_inv = (double *)malloc((size_t)M * M * sizeof(double));
+    for (int r = 0; r < M; r++) memcpy(bj_inv + (size_t)r * M, a + (size_t)r * 2 * M + M, (size_t)M * sizeof(double));
+    free(tmp);
+    free(a);
+}
+
+void or_blockjacobi_apply(const double *in, double *out, int n) {
+    const int B = OR_BJ, M = OR_BJ3, nb = n / B;
+    const long N = n, N2 = N * N;
+    if (!bj_inv) bj_build();
+    double blk[OR_BJ3];
+    for (int bi = 0; bi < nb; bi++)
+        for (int bj = 0; bj < nb; bj++)
+            for (int bk = 0; bk < nb; bk++) {
+                for (int l = 0; l < M; l++) {
+                    const int ii = l / (B * B), jj = (l / B) % B, kk = l % B;
+                    blk[l] = in[(bi * B + ii) * N2 + (bj * B + jj) * N + bk * B + kk];
+                }
+                for (int c = 0; c < M; c++) {
+                    double s = 0.0;
+                    for (int l = 0; l < M; l++) s += blk[l] * bj_inv[(size_t)l * M + c];
+                    const int ii = c / (B * B), jj = (c / B) % B, kk = c % B;
+                    out[(bi * B + ii) * N2 + (bj * B + jj) * N + bk * B + kk] = s;
+                }
+            }
+}
+
+/* CG with the block-Jacobi preconditioner, library/src/plcg.c:104-166. */
+int or_conj_grad_bj(const double *b, double *x, double tol, int n) {
+    const long G = (long)n * n * n, limit = (long)n * n;
+    double *r = (double *)malloc(G * sizeof(double));
+    double *Ap = (double *)malloc(G * sizeof(double));
+    double *v = (double *)malloc(G * sizeof(double));
+    double *p = (double *)malloc(G * sizeof(double));
+    int res = -1;
+    or_laplace(x, r, n);
+    axpy(b, r, -1.0, G);
+    or_blockjacobi_apply(r, v, n);
+    memcpy(p, v, G * sizeof(double));
+    scal(p, -1.0, G);
+    double r_dot_v = dot(r, v, G);
+    for (long it = 0; it < limit; it++) {
+        or_laplace(p, Ap, n);
+        double alpha = r_dot_v / dot(p, Ap, G);
+        axpy(p, x, alpha, G);
+        axpy(Ap, r, alpha, G);
+        (void)dot(r, r, G);
+        if (amax(r, G) <= tol) { res = (int)it; break; }
+        or_blockjacobi_apply(r, v, n);
+        double rn_dot_vn = dot(r, v, G);
+        double beta = rn_dot_vn / r_dot_v;
+        r_dot_v = rn_dot_vn;
+        scal(p, beta, G);
+        axpy(v, p, -1.0, G);
+    }
+    free(r); free(Ap); free(v); free(p);
+    return res;
+}
+
 /* CG with the reference's multigrid preconditioner: v = one two-grid V-cycle
  * with 5 RBGS sweeps per side started from the previous v
  * (library/src/precond_mg.c:3-5, multigrid.c:99-168 via multigrid_apply_recursive
@@ -210,8 +321,8 @@ int or_conj_grad_mg(const double *b, double *x, double tol, int n) {
  * (4 pi / h) q; solve A y = sigma warm-started from y; phi = phi* - y.
  * library/src/verlet.c:31-71 (h already multiplied by eps_s by the caller,
  * library/src/grid_maze_lcg.c:158-164).  precond: 0 the implicit-Jacobi CG
- * (conj_grad), 1 the explicit Jacobi preconditioner, 2 the multigrid one
- * (conj_grad_precond, library/src/grid_maze_lcg.c:130-145). */
+ * (conj_grad), 1 the explicit Jacobi preconditioner, 2 the multigrid one,
+ * 4 the block-Jacobi one (conj_grad_precond, library/src/grid_maze_lcg.c:130-145). */
 int or_verlet_poisson(double tol, double h, double *phi, double *phi_prev, const double *q,
                       double *y, int n, int precond) {
     const long G = (long)n * n * n;
@@ -223,7 +334,8 @@ int or_verlet_poisson(double tol, double h, double *phi, double *phi_prev, const
     }
     or_laplace(phi, sigma, n);
     axpy(q, sigma, (4 * OR_PI) / h, G);
-    int it = precond == 2   ? or_conj_grad_mg(sigma, y, tol, n)
+    int it = precond == 4   ? or_conj_grad_bj(sigma, y, tol, n)
+             : precond == 2 ? or_conj_grad_mg(sigma, y, tol, n)
              : precond == 1 ? or_conj_grad_jacobi(sigma, y, tol, n)
                             : or_conj_grad(sigma, y, tol, n);
     axpy(y, phi, -1.0, G);

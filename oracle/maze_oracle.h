@@ -37,6 +37,9 @@ void or_laplace_pb(const double *u, double *out, int n, const double *ex,
 int or_conj_grad(const double *b, double *x, double tol, int n);
 int or_conj_grad_jacobi(const double *b, double *x, double tol, int n);
 int or_conj_grad_mg(const double *b, double *x, double tol, int n);
+int or_conj_grad_bj(const double *b, double *x, double tol, int n);
+/* v = A_B^-1 r on every 10^3 block (n % 10 == 0). */
+void or_blockjacobi_apply(const double *in, double *out, int n);
 int or_conj_grad_pb(const double *b, double *x, double tol, int n,
                     const double *ex, const double *ey, const double *ez,
                     const double *k2);

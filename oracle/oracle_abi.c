@@ -96,8 +96,10 @@ void solver_initialize_grid(int n, double L, double h, double tol, double eps, d
     if (grid_type < 0 || grid_type > 4) die("oracle: invalid grid type");
     if ((grid_type == G_MG || grid_type == G_MAZE_MG) && (n % 2 || n < 32))
         die("oracle: multigrid needs an even n >= 32");
-    if (grid_type == G_MAZE_LCG && precond_type > P_MG)
-        die("oracle: MAZE-LCG provides the NONE, JACOBI and MG preconditioners only");
+    if (grid_type == G_MAZE_LCG && precond_type == P_SSOR)
+        die("oracle: MAZE-LCG provides the NONE, JACOBI, MG and BLOCKJACOBI preconditioners");
+    if (grid_type == G_MAZE_LCG && precond_type == P_BJ && n % 10)
+        die("oracle: the block-Jacobi preconditioner needs n to be a multiple of 10");
     if (grid_type == G_MAZE_LCG && precond_type == P_MG && (n % 2 || n < 32))
         die("oracle: the multigrid preconditioner needs an even n >= 32");
     memset(&g, 0, sizeof(g));

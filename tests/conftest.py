@@ -50,6 +50,8 @@ def oracle_raw(oracle):
         "or_mg_prolong_trilinear": (None, [D, D, ci]),
         "or_mg_cg_coarse": (ci, [D, D, ci, ci, cd]),
         "or_mg_vcycle": (None, [D, D, ci, ci]),
+        "or_blockjacobi_apply": (None, [D, D, ci]),
+        "or_conj_grad_bj": (ci, [D, D, cd, ci]),
         "or_mg_smooth_pb_rbgs": (None, [D, D, ci, ci, D, D, D, D]),
         "or_mg_restrict_eps": (None, [D, D, ci, ci]),
         "or_mg_restrict_k2": (None, [D, D, ci]),
@@ -109,6 +111,7 @@ def hip_raw(hip):
         "maze_test_restrict27": (ci, [D, D, ci]),
         "maze_test_prolong_add": (ci, [D, D, ci]),
         "maze_test_mg_vcycle": (ci, [D, D, ci]),
+        "maze_test_blockjacobi": (ci, [D, D, ci]),
         "maze_test_rbgs_pb": (None, [D, D, ci, ci, D, D, D, D]),
         "maze_test_restrict_pb_coef": (ci, [D, D, D, D, D, D, D, D, ci]),
         "maze_test_mg_vcycle_pb": (ci, [D, D, ci, D, D, D, D]),
