@@ -85,11 +85,11 @@ struct SrcZero {
 // stay for the work-group reductions.  Returns the hook's verdict (true
 // without a hook).
 //
-// Software-pipelined by one plane: the operands of plane i+1 are requested
-// before plane i is computed, so each thread has two planes of loads in
-// flight and the march is not one memory latency per plane.  (The extra
-// plane fetched after the last one of an even count is in bounds -- indices
-// wrap -- and unused.)  Every pass writes arrays other than the ones its source reads,
+// Software-pipelined by two planes: the operands of planes i+1 and i+2 are
+// requested before plane i is computed, so each thread has up to three planes
+// of loads in flight and the march is not one memory latency per plane.  (The
+// planes fetched after the last one are in bounds -- indices wrap -- and
+// unused.)  Every pass writes arrays other than the ones its source reads,
 // so requesting the next plane before this plane's stores is safe.
 struct PBPlane {
     double exc, eyc, ezc, eym, ezm, k2c;
@@ -165,6 +165,9 @@ __device__ __forceinline__ bool march(const GridDims& d, const C& pb, const Src&
     const SR rc = src.load(i0 * n2 + pl);
     double exm = PB ? pb.ex[im0 * n2 + pl] : 0.0;
     Ops A = fetch(i0);
+    // A one-plane slab (small grids) reads its own plane again (cache hits)
+    // instead of the next one; the select keeps the loads branch-free.
+    Ops B = fetch(i0 + 1 < i1 ? i0 + 1 : i0);
     const bool go = hook();
     if (!active || !go) return go;
     double vm = src.val(rm);
@@ -185,23 +188,27 @@ __device__ __forceinline__ bool march(const GridDims& d, const C& pb, const Src&
         vm = vc;
         vc = vp;
     };
-    // Two operand sets alternate (no register copies of pending loads).  The
-    // loop has no exit between its halves: with one there, the compiler's
-    // structured control flow adds an edge on which the second set's loads
-    // are still pending at the loop head, and the head then waits for the
-    // newest loads before it issues the next plane's (one plane in flight,
-    // not two).  The scheduling barriers keep each plane's loads ahead of
-    // the other plane's arithmetic.
+    // Three operand sets rotate (no register copies of pending loads), so two
+    // planes of loads are in flight while a plane is computed.  The loop has
+    // no exit between its thirds: with one there, the compiler's structured
+    // control flow adds an edge on which later sets' loads are still pending
+    // at the loop head, and the head then waits for the newest loads before
+    // it issues the next plane's.  The scheduling barriers keep each plane's
+    // loads ahead of the other planes' arithmetic.
     int i = i0;
-    for (; i + 1 < i1; i += 2) {
-        const Ops B = fetch(i + 1);
+    for (; i + 2 < i1; i += 3) {
+        const Ops P = fetch(i + 2);
         __builtin_amdgcn_sched_barrier(0);
         compute(i, A);
-        A = fetch(i + 2);
+        A = fetch(i + 3);
         __builtin_amdgcn_sched_barrier(0);
         compute(i + 1, B);
+        B = fetch(i + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(i + 2, P);
     }
     if (i < i1) compute(i, A);
+    if (i + 1 < i1) compute(i + 1, B);
     return true;
 }
 
